@@ -1,0 +1,214 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes access to the CPU oracle (oracle/teasar_oracle.cpp).
+
+The oracle is the CPU restatement of the reference's TEASAR path. Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg may import this module; the product
+package (brook_amd) never does.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_I = ctypes.c_int
+_D = ctypes.c_double
+
+DEFAULT_PARAMS = dict(scale=10.0, const=10.0, pdrf_scale=5000.0, pdrf_exponent=16.0,
+                      soma_detection_threshold=1100.0, soma_acceptance_threshold=4000.0,
+                      soma_invalidation_scale=0.5, soma_invalidation_const=0.0)
+_PARAM_ORDER = ["scale", "const", "pdrf_scale", "pdrf_exponent", "soma_detection_threshold",
+                "soma_acceptance_threshold", "soma_invalidation_scale", "soma_invalidation_const"]
+
+
+def build() -> str:
+    """Compile the oracle library (make in oracle/); returns its path."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.oracle_last_error.restype = ctypes.c_char_p
+        L.oracle_ccl.restype = _I64
+        L.oracle_ccl.argtypes = [_P, _I64, _I64, _I64, _P, _P, _P]
+        L.oracle_edt.restype = _I
+        L.oracle_edt.argtypes = [_P, _I64, _I64, _I64, _P, _I, _I, _P]
+        L.oracle_edt_f64.restype = _I
+        L.oracle_edt_f64.argtypes = [_P, _I64, _I64, _I64, _P, _I, _I, _P]
+        L.oracle_pointset_order.restype = _I
+        L.oracle_pointset_order.argtypes = [_P, _I64, _P]
+        L.oracle_border_targets.restype = _I64
+        L.oracle_border_targets.argtypes = [_P, _I64, _I64, _I64, _I64, _P, _P, _I64, _P]
+        L.oracle_geometric_flood.restype = _I
+        L.oracle_geometric_flood.argtypes = [_P, _I64, _I64, _I64, _P, _P, _I64, _P]
+        L.oracle_railroad.restype = _I64
+        L.oracle_railroad.argtypes = [_P, _P, _I64, _I64, _I64, _I64, _P, _I64]
+        L.oracle_prepare.restype = _I64
+        L.oracle_prepare.argtypes = [_P, _I64, _I64, _I64, _P, _P, _D, _I, _I, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P]
+        L.oracle_skeletonize.restype = _P
+        L.oracle_skeletonize.argtypes = [_P, _I64, _I64, _I64, _P, _I64, _P, _D, _I, _I, _P, _I64, _I]
+        L.oracle_result_sizes.argtypes = [_P, _P]
+        L.oracle_result_copy.argtypes = [_P, _P, _P, _P, _P, _P, _P]
+        L.oracle_result_free.argtypes = [_P]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_P)
+
+
+def _fail(what):
+    raise RuntimeError(f"oracle {what}: {lib().oracle_last_error().decode()}")
+
+
+def _fortran_int64(labels):
+    a = np.asarray(labels)
+    while a.ndim < 3:
+        a = a[..., None]
+    if a.dtype == np.uint64:
+        a = a.view(np.int64)
+    return np.asfortranarray(a.astype(np.int64, copy=False)), a.shape
+
+
+def params_vector(teasar_params=None):
+    p = dict(DEFAULT_PARAMS)
+    p.update({k: v for k, v in (teasar_params or {}).items() if k != "max_paths"})
+    return np.array([float(p[k]) for k in _PARAM_ORDER], np.float64)
+
+
+def ccl(labels):
+    """(cc uint32 Fortran array, mapping int64[N+1], roots int64[N])."""
+    a, shape = _fortran_int64(labels)
+    n = a.size
+    cc = np.zeros(shape, np.uint32, order="F")
+    mapping = np.zeros(n + 1, np.int64)
+    roots = np.zeros(max(n, 1), np.int64)
+    count = lib().oracle_ccl(_ptr(a), *shape, _ptr(cc), _ptr(mapping), _ptr(roots))
+    if count < 0:
+        _fail("ccl")
+    return cc, mapping[:count + 1].copy(), roots[:count].copy()
+
+
+def edt(cc, anisotropy=(1, 1, 1), black_border=False, dims=3):
+    cc = np.asfortranarray(np.asarray(cc).astype(np.uint32, copy=False))
+    while cc.ndim < 3:
+        cc = cc[..., None]
+    out = np.zeros(cc.shape, np.float32, order="F")
+    an = np.asarray(anisotropy, np.float32)
+    if lib().oracle_edt(_ptr(cc), *cc.shape, _ptr(an), int(black_border), dims, _ptr(out)):
+        _fail("edt")
+    return out
+
+
+def edt_f64(cc, anisotropy=(1, 1, 1), black_border=False, dims=3):
+    cc = np.asfortranarray(np.asarray(cc).astype(np.uint32, copy=False))
+    while cc.ndim < 3:
+        cc = cc[..., None]
+    out = np.zeros(cc.shape, np.float64, order="F")
+    an = np.asarray([float(np.float32(a)) for a in anisotropy], np.float64)
+    if lib().oracle_edt_f64(_ptr(cc), *cc.shape, _ptr(an), int(black_border), dims, _ptr(out)):
+        _fail("edt_f64")
+    return out
+
+
+def pointset_order(points):
+    pts = np.ascontiguousarray(np.asarray(points, np.int64).reshape(-1, 3))
+    order = np.full(len(pts), -1, np.int64)
+    if lib().oracle_pointset_order(_ptr(pts), len(pts), _ptr(order)):
+        _fail("pointset")
+    return order[order >= 0]
+
+
+def border_targets(cc, count, anisotropy):
+    cc = np.asfortranarray(np.asarray(cc).astype(np.uint32, copy=False))
+    cap = 4 * (cc.shape[0] * cc.shape[1] + cc.shape[1] * cc.shape[2] + cc.shape[0] * cc.shape[2]) + 16
+    pts = np.zeros(3 * cap, np.int64)
+    offs = np.zeros(count + 2, np.int64)
+    an = np.asarray(anisotropy, np.float32)
+    total = lib().oracle_border_targets(_ptr(cc), *cc.shape, count, _ptr(an), _ptr(pts), cap, _ptr(offs))
+    if total < 0:
+        _fail("border_targets")
+    pts = pts[:3 * total].reshape(-1, 3)
+    return [[tuple(int(c) for c in p) for p in pts[offs[i]:offs[i + 1]]] for i in range(count + 1)]
+
+
+def geometric_flood(cc, anisotropy, sources):
+    cc = np.asfortranarray(np.asarray(cc).astype(np.uint32, copy=False))
+    out = np.zeros(cc.shape, np.float32, order="F")
+    src = np.ascontiguousarray(np.asarray(sources, np.int64))
+    an = np.asarray(anisotropy, np.float32)
+    if lib().oracle_geometric_flood(_ptr(cc), *cc.shape, _ptr(an), _ptr(src), len(src), _ptr(out)):
+        _fail("geometric_flood")
+    return out
+
+
+def railroad(field, target, mask=None):
+    field = np.asfortranarray(np.asarray(field, np.float32))
+    cc = np.asfortranarray((np.ones(field.shape, np.uint32) if mask is None else np.asarray(mask) != 0).astype(np.uint32))
+    path = np.zeros(field.size + 1, np.int64)
+    n = lib().oracle_railroad(_ptr(field), _ptr(cc), *field.shape, int(target), _ptr(path), len(path))
+    if n < 0:
+        _fail("railroad")
+    return path[:n].tolist()
+
+
+def prepare(labels, teasar_params=None, anisotropy=(1, 1, 1), dust_threshold=1000, fix_borders=True, threads=1):
+    a, shape = _fortran_int64(labels)
+    n = a.size
+    cc = np.zeros(shape, np.uint32, order="F")
+    dbf, daf, pdrf = (np.zeros(shape, np.float32, order="F") for _ in range(3))
+    cap = n + 2
+    active = np.zeros(cap, np.uint8)
+    dbf_max, max_daf = np.zeros(cap, np.float32), np.zeros(cap, np.float32)
+    root, target = np.zeros(cap, np.int64), np.zeros(cap, np.int64)
+    pv = params_vector(teasar_params)
+    an = np.asarray(anisotropy, np.float32)
+    count = lib().oracle_prepare(_ptr(a), *shape, _ptr(pv), _ptr(an), float(dust_threshold), int(fix_borders), threads,
+                                 _ptr(cc), _ptr(dbf), _ptr(daf), _ptr(pdrf), cap, _ptr(active), _ptr(dbf_max),
+                                 _ptr(max_daf), _ptr(root), _ptr(target))
+    if count < 0:
+        _fail("prepare")
+    k = count + 1
+    return dict(cc=cc, dbf=dbf, daf=daf, pdrf=pdrf, count=count, active=active[:k].copy(), dbf_max=dbf_max[:k].copy(),
+                max_daf=max_daf[:k].copy(), root=root[:k].copy(), target=target[:k].copy())
+
+
+def skeletonize(labels, teasar_params=None, anisotropy=(1, 1, 1), dust_threshold=1000, fix_branching=True,
+                fix_borders=True, extra_targets_before=(), threads=1):
+    """Packed result dict: labels, v_off, e_off, vertices (V,3) f32, edges (E,2) u32, radii f32, stats."""
+    a, shape = _fortran_int64(labels)
+    pv = params_vector(teasar_params)
+    mp = (teasar_params or {}).get("max_paths")
+    an = np.asarray(anisotropy, np.float32)
+    tb = np.ascontiguousarray(np.asarray(list(extra_targets_before), np.int64).reshape(-1, 3))
+    L = lib()
+    h = L.oracle_skeletonize(_ptr(a), *shape, _ptr(pv), -1 if mp is None else int(mp), _ptr(an), float(dust_threshold),
+                             int(fix_branching), int(fix_borders), _ptr(tb), len(tb), threads)
+    if not h:
+        _fail("skeletonize")
+    try:
+        sizes = np.zeros(6, np.int64)
+        L.oracle_result_sizes(h, _ptr(sizes))
+        ns, nv, ne = (int(x) for x in sizes[:3])
+        out = dict(labels=np.zeros(ns, np.int64), v_off=np.zeros(ns + 1, np.int64), e_off=np.zeros(ns + 1, np.int64),
+                   vertices=np.zeros((nv, 3), np.float32), edges=np.zeros((ne, 2), np.uint32),
+                   radii=np.zeros(nv, np.float32))
+        L.oracle_result_copy(h, _ptr(out["labels"]), _ptr(out["v_off"]), _ptr(out["e_off"]), _ptr(out["vertices"]),
+                             _ptr(out["edges"]), _ptr(out["radii"]))
+        out["stats"] = dict(paths=int(sizes[3]), components=int(sizes[4]), invalidated=int(sizes[5]))
+        return out
+    finally:
+        L.oracle_result_free(h)
