@@ -1,0 +1,74 @@
+"""`skeletonize_batch`: independent samples in one call (the reference's batch.py:193-229);
+result[i] equals skeletonize(samples[i], ...)."""
+from __future__ import annotations
+
+import ctypes
+
+from . import _native as native
+from ._intake import DEFAULT_TEASAR_PARAMS, _packed_from_handle, options_struct, volume_struct
+from ._packed import BatchedSkeletons, PackedSkeletons
+
+
+def _split(samples):
+    try:
+        import torch
+        if isinstance(samples, torch.Tensor):
+            return [samples[i] for i in range(samples.shape[0])]
+    except ImportError:
+        pass
+    if hasattr(samples, "ndim") and getattr(samples, "ndim", 0) >= 2 and not isinstance(samples, (list, tuple)):
+        return [samples[i] for i in range(samples.shape[0])]
+    return list(samples)
+
+
+def _per_sample(entries, count, name):
+    if entries is None:
+        return [[] for _ in range(count)]
+    entries = list(entries)
+    if len(entries) != count:
+        raise ValueError(f"brook: {name} has {len(entries)} entries for {count} samples")
+    return [[] if e is None else [tuple(int(c) for c in p) for p in e] for e in entries]
+
+
+def skeletonize_batch(samples, teasar_params=DEFAULT_TEASAR_PARAMS, anisotropy=(1, 1, 1), dust_threshold=1000,
+                      fix_branching=True, fix_borders=True, fill_holes=False, fix_avocados=False,
+                      extra_targets_before=None, extra_targets_after=None, progress=False, output="packed_device",
+                      stream=None, borrow=True, voxel_graph=None):
+    """Skeletonize independent label volumes (shapes may differ) in one call. Returns
+    BatchedSkeletons (device tensors for "packed_device", host arrays for "packed") or, for
+    output="skeletons", a list of `{label: skeleton}` dicts."""
+    if output not in ("packed_device", "packed", "skeletons"):
+        raise ValueError("brook: output must be 'packed_device', 'packed' or 'skeletons'")
+    if voxel_graph is not None:
+        raise NotImplementedError("voxel_graph is not supported by this build")
+    items = _split(samples)
+    before = _per_sample(extra_targets_before, len(items), "extra_targets_before")
+    after = _per_sample(extra_targets_after, len(items), "extra_targets_after")
+    anisotropy = tuple(float(a) for a in anisotropy)
+    keep = []
+    ctx = native.context()
+    parts = []
+    uniform = all(not b for b in before) and all(not a for a in after)
+    if uniform and items:
+        vols = [volume_struct(s, keep)[0] for s in items]
+        opts = options_struct(teasar_params, anisotropy, dust_threshold, fix_branching, fix_borders, fill_holes,
+                              fix_avocados, None, (), (), keep)
+        arr = (ctypes.POINTER(native.Volume) * len(vols))(*[ctypes.pointer(v) for v in vols])
+        handles = (ctypes.c_void_p * len(vols))()
+        native.check(native.lib().brook_skeletonize_batch(ctx.handle, arr, len(vols), ctypes.byref(opts), handles))
+        parts = [_packed_from_handle(ctypes.c_void_p(h)) for h in handles]
+    else:
+        for s, b, a in zip(items, before, after):
+            vol, _ = volume_struct(s, keep)
+            opts = options_struct(teasar_params, anisotropy, dust_threshold, fix_branching, fix_borders, fill_holes,
+                                  fix_avocados, None, b, a, keep)
+            h = ctypes.c_void_p()
+            native.check(native.lib().brook_skeletonize(ctx.handle, ctypes.byref(vol), ctypes.byref(opts), ctypes.byref(h)))
+            parts.append(_packed_from_handle(h))
+    parts = [p if len(p) else PackedSkeletons.from_skeletons({}, anisotropy) for p in parts]
+    if output == "skeletons":
+        return [p.to_skeletons() for p in parts]
+    result = BatchedSkeletons.from_parts(parts)
+    if output == "packed_device":
+        return BatchedSkeletons.from_parts([p.to_device() if len(p) else p for p in parts])
+    return result

@@ -1,0 +1,187 @@
+// Border targets (fix_borders): for every volume face, 2-D components of the face's component
+// ids and their 2-D EDT are computed on the GPU with the volume kernels (a face is a (fy, fx, 1)
+// volume); the farthest-pixel pick with its tie keys and the insertion-ordered point sets are
+// host work, as in the reference's serial plan (borders.cu:53-116, point_set.hpp:11-55).
+#include <algorithm>
+#include <array>
+#include <cstring>
+#include <limits>
+
+#include "borders.hpp"
+
+namespace brook {
+namespace {
+
+// Insertion-only set of integer 3-tuples with CPython's tuple hash and set probing (the order in
+// which Kimimaro's Python set yields its border targets): xxHash-style tuple hash, 9 linear
+// probes then perturbed jumps, growth to 4x (2x above 50000) of the used count at 3/5 fill.
+class TupleSet {
+ public:
+  void add(const Point &p) {
+    if (!place(p, hash(p))) return;
+    ++used_;
+    if (used_ * 5 < (slots_.size() - 1) * 3) return;
+    size_t want = used_ * (used_ > 50000 ? 2 : 4), size = 8;
+    while (size <= want) size <<= 1;
+    std::vector<Slot> old(size);
+    old.swap(slots_);
+    for (const Slot &s : old)
+      if (s.used) place(s.p, s.h);
+  }
+  std::vector<Point> items() const {
+    std::vector<Point> out;
+    for (const Slot &s : slots_)
+      if (s.used) out.push_back(s.p);
+    return out;
+  }
+
+ private:
+  struct Slot {
+    Point p{};
+    uint64_t h = 0;
+    bool used = false;
+  };
+  std::vector<Slot> slots_ = std::vector<Slot>(8);
+  size_t used_ = 0;
+
+  static uint64_t int_hash(int64_t v) {
+    const uint64_t modulus = (uint64_t(1) << 61) - 1;
+    uint64_t magnitude = v < 0 ? uint64_t(-(v + 1)) + 1 : uint64_t(v);
+    int64_t h = int64_t(magnitude % modulus);
+    if (v < 0) h = -h;
+    return uint64_t(h == -1 ? -2 : h);
+  }
+  static uint64_t hash(const Point &p) {
+    const uint64_t prime1 = 11400714785074694791ull, prime2 = 14029467366897019727ull, prime5 = 2870177450012600261ull;
+    uint64_t acc = prime5;
+    for (int64_t v : p) {
+      acc += int_hash(v) * prime2;
+      acc = (acc << 31) | (acc >> 33);
+      acc *= prime1;
+    }
+    acc += uint64_t(p.size()) ^ (prime5 ^ 3527539ull);
+    return acc == ~uint64_t(0) ? 1546275796ull : acc;
+  }
+  bool place(const Point &p, uint64_t h) {
+    const size_t mask = slots_.size() - 1;
+    size_t i = h & mask;
+    uint64_t perturb = h;
+    for (;;) {
+      const size_t run = i + 9 <= mask ? 9 : 0;
+      for (size_t j = 0; j <= run; ++j) {
+        Slot &s = slots_[i + j];
+        if (!s.used) { s.p = p; s.h = h; s.used = true; return true; }
+        if (s.h == h && s.p == p) return false;
+      }
+      perturb >>= 5;
+      i = (i * 5 + 1 + perturb) & mask;
+    }
+  }
+};
+
+float sq(float x, float y, float a, float b, float wx, float wy) {
+  float dx = wx * (x - a), dy = wy * (y - b);
+  return dx * dx + dy * dy;
+}
+
+// Tie keys of a farthest pixel: distance to the component's centre pixel, to the face centre,
+// to the nearest corner, then the edge distance (borders.cu:42-51).
+std::array<float, 4> tie_keys(int64_t ix, int64_t iy, float cx, float cy, int64_t nx, int64_t ny, float wx, float wy) {
+  float x = float(ix), y = float(iy), fx = float(nx), fy = float(ny);
+  float mid_x = float(double(wx * fx) / 2.0), mid_y = float(double(wy * fy) / 2.0);
+  float far_x = float(double(fx) - 0.5), far_y = float(double(fy) - 0.5);
+  float corner = std::min({sq(x, y, -0.5f, -0.5f, wx, wy), sq(x, y, far_x, -0.5f, wx, wy), sq(x, y, far_x, far_y, wx, wy),
+                           sq(x, y, -0.5f, far_x, wx, wy)});
+  float edge = float(std::min({double(wx) * (double(x) - 0.5), double(wx) * (double(fx) - 0.5 - double(x)),
+                               double(wy) * (double(y) - 0.5), double(wy) * (double(fy) - 0.5 - double(y))}));
+  return {sq(x, y, cx, cy, wx, wy), sq(x, y, mid_x, mid_y, wx, wy), corner, edge};
+}
+
+template <class T> std::vector<int64_t> widen(const std::vector<uint8_t> &raw) {
+  size_t n = raw.size() / sizeof(T);
+  std::vector<int64_t> out(n);
+  const T *p = reinterpret_cast<const T *>(raw.data());
+  for (size_t i = 0; i < n; ++i) out[i] = int64_t(p[i]);
+  return out;
+}
+
+}  // namespace
+
+std::vector<std::vector<Point>> border_targets(Context &ctx, const void *cc, int cc_bytes, Dims d, int64_t count,
+                                               const float aniso[3]) {
+  std::vector<TupleSet> sets(size_t(count) + 1);
+  const int axes[3][3] = {{0, 1, 2}, {0, 2, 1}, {1, 2, 0}};
+  const int dims[3] = {d.sx, d.sy, d.sz};
+  for (int face = 0; face < 6; ++face) {
+    const int a = axes[face / 2][0], b = axes[face / 2][1], c = axes[face / 2][2];
+    const int fixed = face % 2 ? dims[c] - 1 : 0, fx = dims[a], fy = dims[b];
+    const int64_t np = int64_t(fx) * fy;
+    Buffer<uint8_t> plane(ctx, size_t(np) * cc_bytes);
+    extract_face(ctx, cc, cc_bytes, d, a, b, c, fixed, plane.data());
+    LabelVolume pv;
+    pv.data = plane.data();
+    pv.bytes = cc_bytes;
+    pv.dims = Dims{fy, fx, 1};
+    Components c2 = connected_components(ctx, pv);
+    if (c2.count == 0) continue;
+    Buffer<float> dist(ctx, size_t(np));
+    const float pa[3] = {aniso[b], aniso[a], 1.0f};
+    edt(ctx, c2.labels.data(), c2.cc_bytes, pv.dims, pa, true, 2, dist.data());
+    std::vector<uint8_t> raw = c2.labels.download(ctx, size_t(np) * c2.cc_bytes);
+    std::vector<int64_t> lab = c2.cc_bytes == 1 ? widen<uint8_t>(raw) : c2.cc_bytes == 2 ? widen<uint16_t>(raw) : widen<uint32_t>(raw);
+    std::vector<float> dv = dist.download(ctx, size_t(np));
+    const size_t k = size_t(c2.count) + 1;
+    std::vector<float> maxdist(k, 0.0f), sumx(k, 0.0f), sumy(k, 0.0f);
+    std::vector<uint32_t> size(k, 0);
+    std::vector<int64_t> first(k, std::numeric_limits<int64_t>::max()), pick(k, -1);
+    for (int64_t x = 0; x < fx; ++x)  // float32 centroid sums in Kimimaro's order (x outer)
+      for (int64_t y = 0; y < fy; ++y) {
+        size_t i = size_t(y + int64_t(fy) * x);
+        int64_t l = lab[i];
+        if (!l) continue;
+        sumx[l] += float(x);
+        sumy[l] += float(y);
+        ++size[l];
+        if (dv[i] > 0) {
+          maxdist[l] = std::max(maxdist[l], dv[i]);
+          first[l] = std::min(first[l], x + int64_t(fx) * y);
+        }
+      }
+    const float wx = aniso[a], wy = aniso[b];
+    std::vector<float> cxs(k, 0.0f), cys(k, 0.0f);
+    for (size_t l = 1; l < k; ++l) {
+      if (!size[l]) continue;
+      float px = (wx * sumx[l]) / float(size[l]), py = (wy * sumy[l]) / float(size[l]);
+      if (px - (wx * float(fx)) / 2.0f < 0) px = px + wx;
+      if (py - (wy * float(fy)) / 2.0f < 0) py = py + wy;
+      cxs[l] = float(int(px / wx));
+      cys[l] = float(int(py / wy));
+    }
+    std::vector<std::array<float, 4>> best(k);
+    for (int64_t y = 0; y < fy; ++y)
+      for (int64_t x = 0; x < fx; ++x) {
+        size_t i = size_t(y + int64_t(fy) * x);
+        int64_t l = lab[i];
+        if (!l || dv[i] == 0 || dv[i] != maxdist[l]) continue;
+        auto key = tie_keys(x, y, cxs[l], cys[l], fx, fy, wx, wy);
+        if (pick[l] < 0 || key < best[l]) { best[l] = key; pick[l] = x + int64_t(fx) * y; }
+      }
+    std::vector<size_t> order;
+    for (size_t l = 1; l < k; ++l)
+      if (pick[l] >= 0) order.push_back(l);
+    std::sort(order.begin(), order.end(), [&](size_t u, size_t v) { return first[u] < first[v]; });
+    for (size_t l : order) {
+      Point p{};
+      p[a] = pick[l] % fx;
+      p[b] = pick[l] / fx;
+      p[c] = fixed;
+      sets.at(size_t(c2.mapping[l])).add(p);
+    }
+  }
+  std::vector<std::vector<Point>> out;
+  out.reserve(sets.size());
+  for (auto &s : sets) out.push_back(s.items());
+  return out;
+}
+
+}  // namespace brook

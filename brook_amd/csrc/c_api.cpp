@@ -1,0 +1,457 @@
+// The C ABI (include/brook.h): handles, error protection and input normalization around the
+// engine (the conventions of the reference's c_api.cpp:17-41, 218-277).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../include/brook.h"
+#include "engine.hpp"
+#include "layout.hpp"
+
+namespace {
+
+// Device memory owned by a result handle, independent of the context that made it.
+struct DeviceMemory {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  int device = 0;
+  DeviceMemory(int dev, size_t n) : bytes(n), device(dev) {
+    if (n) BROOK_HIP(hipMalloc(&ptr, n));
+  }
+  ~DeviceMemory() {
+    if (ptr) { (void)hipSetDevice(device); (void)hipFree(ptr); }
+  }
+};
+
+thread_local std::string g_error;
+
+template <class F> brook_status protect(F &&fn) noexcept {
+  try {
+    g_error.clear();
+    fn();
+    return BROOK_SUCCESS;
+  } catch (const std::invalid_argument &e) {
+    g_error = e.what();
+    return BROOK_INVALID_ARGUMENT;
+  } catch (const std::bad_alloc &e) {
+    g_error = e.what();
+    return BROOK_OUT_OF_MEMORY;
+  } catch (const brook::DeviceError &e) {
+    g_error = e.what();
+    return e.code == hipErrorOutOfMemory ? BROOK_OUT_OF_MEMORY : BROOK_DEVICE_ERROR;
+  } catch (const std::exception &e) {
+    g_error = e.what();
+    return BROOK_INTERNAL_ERROR;
+  } catch (...) {
+    g_error = "unknown exception";
+    return BROOK_INTERNAL_ERROR;
+  }
+}
+
+void require(bool ok, const char *what = "null or invalid brook argument") {
+  if (!ok) throw std::invalid_argument(what);
+}
+
+int dtype_bytes(brook_dtype t) {
+  switch (t) {
+    case BROOK_U8: case BROOK_I8: return 1;
+    case BROOK_U16: case BROOK_I16: return 2;
+    case BROOK_U32: case BROOK_I32: case BROOK_F32: return 4;
+    case BROOK_U64: case BROOK_I64: case BROOK_F64: return 8;
+  }
+  throw std::invalid_argument("invalid dtype");
+}
+bool integer_dtype(brook_dtype t) { return t >= BROOK_U8 && t <= BROOK_I64; }
+
+void check_volume(const brook_volume *v) {
+  require(v != nullptr);
+  require(v->struct_size >= sizeof(brook_volume) && v->abi_version == BROOK_ABI_VERSION, "incompatible brook_volume");
+  for (int a = 0; a < 3; ++a) require(v->shape[a] >= 0, "negative volume shape");
+  int64_t n = v->shape[0] * v->shape[1] * v->shape[2];
+  require(n == 0 || v->data != nullptr, "null volume data");
+  require(n < (int64_t(1) << 31) - 4096, "volume exceeds the int32 voxel index space");
+  require(v->memory == BROOK_HOST || v->memory == BROOK_DEVICE, "invalid memory kind");
+}
+
+// The volume as Fortran-ordered device memory: borrowed when it already is, else copied.
+struct Input {
+  brook::LabelVolume view;
+  std::unique_ptr<brook::Buffer<uint8_t>> holder;
+};
+
+Input normalize(brook::Context &ctx, const brook_volume *v) {
+  check_volume(v);
+  Input in;
+  const int b = dtype_bytes(v->dtype);
+  const int64_t sx = v->shape[0], sy = v->shape[1], sz = v->shape[2], n = sx * sy * sz;
+  in.view.bytes = b;
+  in.view.is_signed = v->dtype >= BROOK_I8 && v->dtype <= BROOK_I64;
+  in.view.dims = brook::Dims{int(sx), int(sy), int(sz)};
+  const bool fortran = (sx <= 1 || v->strides[0] == b) && (sy <= 1 || v->strides[1] == b * sx) &&
+                       (sz <= 1 || v->strides[2] == b * sx * sy);
+  if (n == 0) return in;
+  if (v->memory == BROOK_DEVICE && fortran) {
+    in.view.data = v->data;
+    return in;
+  }
+  in.holder = std::make_unique<brook::Buffer<uint8_t>>(ctx, size_t(n) * b);
+  if (v->memory == BROOK_DEVICE) {
+    const long long strides[3] = {v->strides[0], v->strides[1], v->strides[2]};
+    brook::gather_fortran(ctx, v->data, strides, b, in.view.dims, in.holder->data());
+  } else if (fortran) {
+    BROOK_HIP(hipMemcpyAsync(in.holder->data(), v->data, size_t(n) * b, hipMemcpyHostToDevice, ctx.stream()));
+  } else {
+    std::vector<uint8_t> packed(size_t(n) * b);
+    const char *base = static_cast<const char *>(v->data);
+    size_t o = 0;
+    for (int64_t z = 0; z < sz; ++z)
+      for (int64_t y = 0; y < sy; ++y)
+        for (int64_t x = 0; x < sx; ++x, o += b)
+          std::memcpy(&packed[o], base + x * v->strides[0] + y * v->strides[1] + z * v->strides[2], b);
+    BROOK_HIP(hipMemcpyAsync(in.holder->data(), packed.data(), packed.size(), hipMemcpyHostToDevice, ctx.stream()));
+    ctx.synchronize();
+  }
+  in.view.data = in.holder->data();
+  return in;
+}
+
+brook_dtype unsigned_dtype(int bytes) { return bytes == 1 ? BROOK_U8 : bytes == 2 ? BROOK_U16 : bytes == 4 ? BROOK_U32 : BROOK_U64; }
+
+}  // namespace
+
+struct brook_context {
+  brook::Context value;
+  explicit brook_context(int device) : value(device) {}
+};
+struct brook_array {
+  std::shared_ptr<DeviceMemory> memory;
+  brook_dtype dtype = BROOK_U8;
+  int64_t shape[3] = {0, 0, 0};
+};
+struct brook_components {
+  brook_array labels;
+  std::vector<int64_t> mapping, roots;
+};
+struct brook_skeletons {
+  brook::Packed packed;
+};
+
+namespace {
+
+brook_array *owned_array(brook::Context &ctx, const void *src, size_t bytes, brook_dtype dtype, const int64_t shape[3]) {
+  auto a = std::make_unique<brook_array>();
+  a->memory = std::make_shared<DeviceMemory>(ctx.device(), bytes);
+  if (bytes) BROOK_HIP(hipMemcpyAsync(a->memory->ptr, src, bytes, hipMemcpyDeviceToDevice, ctx.stream()));
+  ctx.synchronize();
+  a->dtype = dtype;
+  std::copy(shape, shape + 3, a->shape);
+  return a.release();
+}
+
+brook::SkeletonizeOptions convert(const brook_skeletonize_options *p, std::vector<uint64_t> *ids) {
+  require(p != nullptr);
+  require(p->struct_size >= sizeof(brook_skeletonize_options) && p->abi_version == BROOK_ABI_VERSION,
+          "incompatible skeletonization options");
+  require(!(p->flags & ~uint32_t(BROOK_FIX_BRANCHING | BROOK_FIX_BORDERS | BROOK_FILTER_OBJECTS | BROOK_DEVICE_OUTPUT |
+                                 BROOK_FILL_HOLES | BROOK_FIX_AVOCADOS)),
+          "unknown skeletonization flags");
+  brook::SkeletonizeOptions o;
+  auto &t = o.teasar;
+  t.scale = p->teasar.scale;
+  t.constant = p->teasar.constant;
+  t.pdrf_scale = p->teasar.pdrf_scale;
+  t.pdrf_exponent = p->teasar.pdrf_exponent;
+  t.soma_detection = p->teasar.soma_detection_threshold;
+  t.soma_acceptance = p->teasar.soma_acceptance_threshold;
+  t.soma_scale = p->teasar.soma_invalidation_scale;
+  t.soma_constant = p->teasar.soma_invalidation_const;
+  t.max_paths = p->teasar.has_max_paths ? p->teasar.max_paths : -1;
+  std::copy(p->anisotropy, p->anisotropy + 3, o.aniso);
+  o.dust = p->dust_threshold;
+  o.fix_branching = p->flags & BROOK_FIX_BRANCHING;
+  o.fix_borders = p->flags & BROOK_FIX_BORDERS;
+  o.fill_holes = p->flags & BROOK_FILL_HOLES;
+  o.fix_avocados = p->flags & BROOK_FIX_AVOCADOS;
+  o.voxel_graph = p->voxel_graph != nullptr;
+  auto points = [](const int64_t *data, size_t count) {
+    require(data || !count);
+    std::vector<brook::Point> out;
+    for (size_t i = 0; i < count; ++i) out.push_back({data[3 * i], data[3 * i + 1], data[3 * i + 2]});
+    return out;
+  };
+  o.before = points(p->targets_before, p->targets_before_count);
+  o.after = points(p->targets_after, p->targets_after_count);
+  if (ids) {
+    ids->clear();
+    if (p->flags & BROOK_FILTER_OBJECTS) {
+      require(p->object_ids || !p->object_ids_count);
+      for (size_t i = 0; i < p->object_ids_count; ++i) ids->push_back(uint64_t(p->object_ids[i]));
+    }
+  }
+  if (p->flags & BROOK_DEVICE_OUTPUT) throw brook::Unsupported("BROOK_DEVICE_OUTPUT is not supported by this build");
+  return o;
+}
+
+brook::Packed run_one(brook::Context &ctx, const brook_volume *v, const brook_skeletonize_options *p,
+                      brook::FieldsOut *fields = nullptr) {
+  std::vector<uint64_t> ids;
+  brook::SkeletonizeOptions o = convert(p, &ids);
+  require(integer_dtype(v ? v->dtype : BROOK_F32), "skeletonize requires integer labels");
+  Input in = normalize(ctx, v);
+  brook::Buffer<uint8_t> masked;
+  if (p->flags & BROOK_FILTER_OBJECTS) {
+    masked = brook::Buffer<uint8_t>(ctx, size_t(in.view.size()) * in.view.bytes);
+    brook::mask_labels(ctx, in.view.data, in.view.bytes, in.view.size(), ids, masked.data());
+    in.view.data = masked.data();
+  }
+  return brook::skeletonize(ctx, in.view, o, fields);
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t brook_abi_version(void) { return BROOK_ABI_VERSION; }
+const char *brook_version(void) { return "0.1.0+mi355x"; }
+const char *brook_last_error(void) { return g_error.c_str(); }
+
+brook_status brook_context_create(int device, brook_context **out) {
+  return protect([&] {
+    require(out != nullptr);
+    *out = nullptr;
+    require(device >= 0, "negative device index");
+    *out = new brook_context(device);
+  });
+}
+void brook_context_destroy(brook_context *context) { delete context; }
+brook_status brook_context_synchronize(brook_context *context) {
+  return protect([&] {
+    require(context != nullptr);
+    brook::ContextCall call(context->value);
+    context->value.synchronize();
+  });
+}
+
+brook_status brook_array_upload(brook_context *ctx, const brook_volume *input, brook_array **out) {
+  return protect([&] {
+    require(ctx && input && out);
+    *out = nullptr;
+    brook::ContextCall call(ctx->value);
+    Input in = normalize(ctx->value, input);
+    *out = owned_array(ctx->value, in.view.data, size_t(in.view.size()) * in.view.bytes, input->dtype, input->shape);
+  });
+}
+void brook_array_destroy(brook_array *array) { delete array; }
+brook_status brook_array_view(const brook_array *a, brook_volume *out) {
+  return protect([&] {
+    require(a && out);
+    *out = brook_volume{};
+    out->struct_size = sizeof(brook_volume);
+    out->abi_version = BROOK_ABI_VERSION;
+    out->data = a->memory->ptr;
+    out->dtype = a->dtype;
+    out->memory = BROOK_DEVICE;
+    int b = dtype_bytes(a->dtype);
+    std::copy(a->shape, a->shape + 3, out->shape);
+    out->strides[0] = b;
+    out->strides[1] = b * a->shape[0];
+    out->strides[2] = b * a->shape[0] * a->shape[1];
+  });
+}
+brook_status brook_array_copy_to_host(const brook_array *a, void *out, size_t bytes) {
+  return protect([&] {
+    require(a && (out || !bytes));
+    require(bytes == a->memory->bytes, "host buffer size does not match the array");
+    (void)hipSetDevice(a->memory->device);
+    if (bytes) BROOK_HIP(hipMemcpy(out, a->memory->ptr, bytes, hipMemcpyDeviceToHost));
+  });
+}
+
+brook_status brook_connected_components(brook_context *ctx, const brook_volume *labels, brook_components **out) {
+  return protect([&] {
+    require(ctx && labels && out);
+    *out = nullptr;
+    require(integer_dtype(labels->dtype), "connected components requires integer labels");
+    brook::ContextCall call(ctx->value);
+    Input in = normalize(ctx->value, labels);
+    brook::Components cc = brook::connected_components(ctx->value, in.view);
+    auto r = std::make_unique<brook_components>();
+    const size_t n = size_t(in.view.size());
+    brook_dtype dt = unsigned_dtype(cc.cc_bytes);
+    if (n) {
+      auto *a = owned_array(ctx->value, cc.labels.data(), n * cc.cc_bytes, dt, labels->shape);
+      r->labels = std::move(*a);
+      delete a;
+    } else {
+      r->labels.memory = std::make_shared<DeviceMemory>(ctx->value.device(), 0);
+      r->labels.dtype = BROOK_U32;
+      std::copy(labels->shape, labels->shape + 3, r->labels.shape);
+    }
+    r->mapping = cc.mapping;
+    r->roots = cc.roots;
+    *out = r.release();
+  });
+}
+void brook_components_destroy(brook_components *c) { delete c; }
+brook_status brook_components_view(const brook_components *c, brook_volume *labels, const int64_t **mapping,
+                                   const int64_t **roots, size_t *count) {
+  return protect([&] {
+    require(c != nullptr);
+    if (labels) {
+      brook_status s = brook_array_view(&c->labels, labels);
+      if (s != BROOK_SUCCESS) throw std::runtime_error(g_error);
+    }
+    if (mapping) *mapping = c->mapping.data();
+    if (roots) *roots = c->roots.data();
+    if (count) *count = c->roots.size();
+  });
+}
+brook_status brook_components_copy_to_host(const brook_components *c, void *out, size_t bytes) {
+  require(c != nullptr);
+  return brook_array_copy_to_host(&c->labels, out, bytes);
+}
+brook_status brook_components_labels(const brook_components *c, brook_array **out) {
+  return protect([&] {
+    require(c && out);
+    *out = new brook_array(c->labels);
+  });
+}
+
+brook_status brook_edt(brook_context *ctx, const brook_volume *labels, const float anisotropy[3], int black_border,
+                       brook_array **out) {
+  return protect([&] {
+    require(ctx && labels && anisotropy && out);
+    *out = nullptr;
+    require(integer_dtype(labels->dtype), "EDT requires integer labels");
+    brook::ContextCall call(ctx->value);
+    Input in = normalize(ctx->value, labels);
+    const size_t n = size_t(in.view.size());
+    auto a = std::make_unique<brook_array>();
+    a->memory = std::make_shared<DeviceMemory>(ctx->value.device(), n * 4);
+    a->dtype = BROOK_F32;
+    std::copy(labels->shape, labels->shape + 3, a->shape);
+    if (n) {
+      // Runs of equal value along an axis lie in one component, so the EDT of the labels equals
+      // the EDT of their component ids; 64-bit labels go through the ids.
+      if (in.view.bytes == 8) {
+        brook::Components cc = brook::connected_components(ctx->value, in.view);
+        brook::edt(ctx->value, cc.labels.data(), cc.cc_bytes, in.view.dims, anisotropy, black_border != 0, 3,
+                   static_cast<float *>(a->memory->ptr));
+        ctx->value.synchronize();
+      } else {
+        brook::edt(ctx->value, in.view.data, in.view.bytes, in.view.dims, anisotropy, black_border != 0, 3,
+                   static_cast<float *>(a->memory->ptr));
+      }
+      ctx->value.synchronize();
+    }
+    *out = a.release();
+  });
+}
+
+void brook_skeletonize_options_init(brook_skeletonize_options *p) {
+  if (!p) return;
+  *p = brook_skeletonize_options{};
+  p->struct_size = sizeof(*p);
+  p->abi_version = BROOK_ABI_VERSION;
+  p->teasar.scale = 1.5;
+  p->teasar.constant = 300;
+  p->teasar.pdrf_scale = 100000;
+  p->teasar.pdrf_exponent = 4;
+  p->teasar.soma_detection_threshold = 750;
+  p->teasar.soma_acceptance_threshold = 3500;
+  p->teasar.soma_invalidation_scale = 2;
+  p->teasar.soma_invalidation_const = 300;
+  p->anisotropy[0] = p->anisotropy[1] = p->anisotropy[2] = 1;
+  p->dust_threshold = 1000;
+  p->flags = BROOK_FIX_BRANCHING | BROOK_FIX_BORDERS;
+}
+
+brook_status brook_skeletonize(brook_context *ctx, const brook_volume *labels, const brook_skeletonize_options *options,
+                               brook_skeletons **out) {
+  return protect([&] {
+    require(ctx && labels && options && out);
+    *out = nullptr;
+    brook::ContextCall call(ctx->value);
+    auto r = std::make_unique<brook_skeletons>();
+    r->packed = run_one(ctx->value, labels, options);
+    *out = r.release();
+  });
+}
+
+brook_status brook_skeletonize_batch(brook_context *ctx, const brook_volume *const *volumes, size_t count,
+                                     const brook_skeletonize_options *options, brook_skeletons **out) {
+  return protect([&] {
+    require(ctx && (volumes || !count) && options && (out || !count));
+    for (size_t i = 0; i < count; ++i) out[i] = nullptr;
+    brook::ContextCall call(ctx->value);
+    std::vector<std::unique_ptr<brook_skeletons>> results;
+    for (size_t i = 0; i < count; ++i) {
+      auto r = std::make_unique<brook_skeletons>();
+      r->packed = run_one(ctx->value, volumes[i], options);
+      results.push_back(std::move(r));
+    }
+    for (size_t i = 0; i < count; ++i) out[i] = results[i].release();
+  });
+}
+
+brook_status brook_skeletons_get_view(const brook_skeletons *r, brook_skeletons_view *out) {
+  return protect([&] {
+    require(r && out);
+    *out = brook_skeletons_view{};
+    const brook::Packed &p = r->packed;
+    out->skeleton_count = p.labels.size();
+    out->vertex_count = p.radii.size();
+    out->edge_count = p.edges.size() / 2;
+    out->labels = p.labels.data();
+    out->vertex_offsets = reinterpret_cast<const int64_t *>(p.v_off.data());
+    out->edge_offsets = reinterpret_cast<const int64_t *>(p.e_off.data());
+    out->vertices = p.vertices.data();
+    out->edges = p.edges.data();
+    out->radii = p.radii.data();
+    std::copy(p.anisotropy.begin(), p.anisotropy.end(), out->anisotropy);
+  });
+}
+void brook_skeletons_destroy(brook_skeletons *r) { delete r; }
+
+brook_status brook_prepare_fields(brook_context *ctx, const brook_volume *labels, const brook_skeletonize_options *options,
+                                  float *dbf, float *daf, float *pdrf, size_t cap, uint8_t *traced, int64_t *root,
+                                  int64_t *target, float *dbf_max, float *max_daf, int64_t *count) {
+  return protect([&] {
+    require(ctx && labels && options);
+    brook::ContextCall call(ctx->value);
+    brook::FieldsOut f;
+    f.dbf = dbf;
+    f.daf = daf;
+    f.pdrf = pdrf;
+    f.stop = true;
+    run_one(ctx->value, labels, options, &f);
+    require(f.active.size() <= cap || (!traced && !root && !target && !dbf_max && !max_daf), "table capacity too small");
+    for (size_t i = 0; i < f.active.size(); ++i) {
+      if (traced) traced[i] = f.active[i];
+      if (root) root[i] = f.root[i];
+      if (target) target[i] = f.target[i];
+      if (dbf_max) dbf_max[i] = f.dbf_max[i];
+      if (max_daf) max_daf[i] = f.max_daf[i];
+    }
+    if (count) *count = f.count;
+  });
+}
+
+brook_status brook_context_stat(const brook_context *ctx, const char *name, double *value) {
+  return protect([&] {
+    require(ctx && name && value);
+    auto it = ctx->value.stats.find(name);
+    *value = it == ctx->value.stats.end() ? 0.0 : it->second;
+  });
+}
+
+brook_status brook_context_set_kernel_timing(brook_context *ctx, int enable) {
+  return protect([&] {
+    require(ctx != nullptr);
+    ctx->value.kernel_timing = enable != 0;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,341 @@
+// Label-aware 26-connected components on gfx950 (ids in ascending order of each component's
+// minimum flat index, as ccl.cu:24-82 assigns them), plus the per-id bbox/count analysis
+// (preamble.cu:10-29) and face extraction for the border planes.
+//
+// Union-find over int32 parents: every x-run is hooked to its first voxel by one wave ballot
+// (no atomics), the 12 backward neighbours in the y-1 / z-1 rows are united with CAS hooking
+// (larger root -> smaller), so each root is its component's minimum flat index. Ids come from a
+// tiled exclusive scan of the root flags in flat order -- no host sort.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "stages.hpp"
+
+namespace brook {
+namespace {
+
+constexpr int kBG = INT32_MIN;  // parent of a background voxel
+constexpr int kTile = 4096;     // voxels per root-count tile (256 threads x 16)
+
+__device__ __forceinline__ unsigned long long lanes_upto(int lane) {
+  return lane == 63 ? ~0ULL : ((2ULL << lane) - 1ULL);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) ccl_init(const T *lab, int *parent, Dims d, int n) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  int lane = threadIdx.x & 63;
+  bool valid = i < n;
+  T l = valid ? lab[i] : T(0);
+  int x = valid ? i % d.sx : 0;
+  T prev = shfl_up_any(l, 1);
+  if (lane == 0 && valid && x > 0) prev = lab[i - 1];
+  bool start = valid && (x == 0 || prev != l);
+  unsigned long long mask = __ballot(start);
+  if (!valid) return;
+  if (l == T(0)) { parent[i] = kBG; return; }
+  unsigned long long below = mask & lanes_upto(lane);
+  int base = i - lane;
+  parent[i] = below ? base + (63 - __clzll((long long)below)) : base - 1;
+}
+
+__device__ __forceinline__ int find_root(int *parent, int v) {
+  int cur = ld_i(parent + v);
+  if (cur != v) {
+    int prev = v, next;
+    while (cur > (next = ld_i(parent + cur))) {
+      st_i(parent + prev, next);  // racy path halving: only ever points closer to the root
+      prev = cur;
+      cur = next;
+    }
+  }
+  return cur;
+}
+
+__device__ __forceinline__ void unite(int *parent, int a, int b) {
+  int va = find_root(parent, a), vb = find_root(parent, b);
+  while (va != vb) {
+    if (va < vb) {
+      int ret = atomicCAS(parent + vb, vb, va);
+      if (ret == vb) return;
+      vb = find_root(parent, ret);
+    } else {
+      int ret = atomicCAS(parent + va, va, vb);
+      if (ret == va) return;
+      va = find_root(parent, ret);
+    }
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) ccl_union(const T *lab, int *parent, Dims d, int n) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  T l = lab[i];
+  if (l == T(0)) return;
+  int x, y, z;
+  unravel(i, d, x, y, z);
+  // rows whose voxels precede i: (y-1, z) and (y-1..y+1, z-1); x-neighbours in i's own row are
+  // one run already. A row's centre voxel of equal label joins its whole run; else both sides.
+  const int rows[4][2] = {{-1, 0}, {-1, -1}, {0, -1}, {1, -1}};
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    int yy = y + rows[r][0], zz = z + rows[r][1];
+    if (yy < 0 || yy >= d.sy || zz < 0) continue;
+    int row = d.sx * (yy + d.sy * zz);
+    if (lab[row + x] == l) { unite(parent, i, row + x); continue; }
+    if (x > 0 && lab[row + x - 1] == l) unite(parent, i, row + x - 1);
+    if (x + 1 < d.sx && lab[row + x + 1] == l) unite(parent, i, row + x + 1);
+  }
+}
+
+__global__ void __launch_bounds__(256) ccl_count(const int *parent, int n, int *tile_counts) {
+  __shared__ int total;
+  if (threadIdx.x == 0) total = 0;
+  __syncthreads();
+  int base = blockIdx.x * kTile + threadIdx.x * 16, c = 0;
+#pragma unroll 4
+  for (int j = 0; j < 16; ++j) {
+    int i = base + j;
+    if (i < n && parent[i] == i) ++c;
+  }
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
+  if ((threadIdx.x & 63) == 0) atomicAdd(&total, c);
+  __syncthreads();
+  if (threadIdx.x == 0) tile_counts[blockIdx.x] = total;
+}
+
+// Exclusive scan of the tile counts in place (one workgroup); total -> *count.
+__global__ void __launch_bounds__(1024) scan_tiles(int *tile_counts, int tiles, long long *count) {
+  __shared__ int sums[1024];
+  int per = (tiles + 1023) / 1024, b = threadIdx.x * per, s = 0;
+  for (int j = 0; j < per; ++j) if (b + j < tiles) s += tile_counts[b + j];
+  sums[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int v = threadIdx.x >= off ? sums[threadIdx.x - off] : 0;
+    __syncthreads();
+    sums[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int run = sums[threadIdx.x] - s;
+  for (int j = 0; j < per; ++j)
+    if (b + j < tiles) { int c = tile_counts[b + j]; tile_counts[b + j] = run; run += c; }
+  if (threadIdx.x == 1023) *count = sums[1023];
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) ccl_mark(const T *lab, int *parent, int n, const int *tile_offsets,
+                                                long long *roots, long long *mapping) {
+  __shared__ int wave_sums[4];
+  int base = blockIdx.x * kTile + threadIdx.x * 16, c = 0;
+  unsigned flags = 0;
+  for (int j = 0; j < 16; ++j) {
+    int i = base + j;
+    if (i < n && parent[i] == i) { flags |= 1u << j; ++c; }
+  }
+  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int incl = c;
+  for (int off = 1; off < 64; off <<= 1) {
+    int v = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += v;
+  }
+  if (lane == 63) wave_sums[wave] = incl;
+  __syncthreads();
+  int before = 0;
+  for (int w = 0; w < wave; ++w) before += wave_sums[w];
+  int id = tile_offsets[blockIdx.x] + before + incl - c;  // ids are 1-based: id + 1 below
+  for (int j = 0; j < 16; ++j) {
+    if (!(flags & (1u << j))) continue;
+    int i = base + j;
+    ++id;
+    parent[i] = -id;
+    roots[id - 1] = i;
+    mapping[id] = (long long)lab[i];
+  }
+}
+
+template <class T, class CC>
+__global__ void __launch_bounds__(256) ccl_relabel(const int *parent, int n, CC *cc) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  int p = parent[i];
+  if (p == kBG) { cc[i] = CC(0); return; }
+  while (p >= 0) p = parent[p];  // roots hold -id now
+  cc[i] = CC(-p);
+}
+
+// ---- analysis: per-id bbox and count, one atomic group per run segment of a wave ----------
+template <class CC>
+__global__ void __launch_bounds__(256) analyze_runs(const CC *cc, Dims d, int n, int *lo, int *hi,
+                                                    unsigned long long *counts) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  int lane = threadIdx.x & 63;
+  bool valid = i < n;
+  CC l = valid ? cc[i] : CC(0);
+  int x = valid ? i % d.sx : 0;
+  CC prev = shfl_up_any(l, 1);
+  bool start = !valid || lane == 0 || x == 0 || prev != l;
+  unsigned long long starts = __ballot(start);
+  if (!valid || l == CC(0) || !start) return;
+  unsigned long long after = starts & ~lanes_upto(lane);
+  int end_lane = after ? __ffsll((long long)after) - 2 : 63;
+  int len = end_lane - lane + 1;
+  int y = (i / d.sx) % d.sy, z = i / (d.sx * d.sy);
+  atomicAdd(counts + l, (unsigned long long)len);
+  int *L = lo + 3 * int(l), *H = hi + 3 * int(l);
+  if (ld_i(L) > x) atomicMin(L, x);
+  if (ld_i(H) < x + len - 1) atomicMax(H, x + len - 1);
+  if (ld_i(L + 1) > y) atomicMin(L + 1, y);
+  if (ld_i(H + 1) < y) atomicMax(H + 1, y);
+  if (ld_i(L + 2) > z) atomicMin(L + 2, z);
+  if (ld_i(H + 2) < z) atomicMax(H + 2, z);
+}
+
+template <class CC>
+__global__ void __launch_bounds__(256) face_copy(const CC *cc, Dims d, int a, int b, int c, int fixed, int fx, int fy,
+                                                 CC *plane) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= fx * fy) return;
+  int p[3];
+  p[a] = i / fy;
+  p[b] = i % fy;
+  p[c] = fixed;
+  plane[i] = cc[flat(p[0], p[1], p[2], d)];
+}
+
+unsigned blocks_for(int64_t n, int per) { return unsigned((n + per - 1) / per); }
+
+template <class T> Components run_ccl(Context &ctx, const LabelVolume &in) {
+  Components out;
+  const int n = int(in.size());
+  const Dims d = in.dims;
+  const T *lab = static_cast<const T *>(in.data);
+  Buffer<int> parent(ctx, n);
+  auto s = ctx.stream();
+  {
+    KernelTimer kt(ctx, "ccl_init");
+    ccl_init<T><<<blocks_for(n, 256), 256, 0, s>>>(lab, parent.data(), d, n);
+    BROOK_LAUNCHED();
+  }
+  {
+    KernelTimer kt(ctx, "ccl_union");
+    ccl_union<T><<<blocks_for(n, 256), 256, 0, s>>>(lab, parent.data(), d, n);
+    BROOK_LAUNCHED();
+  }
+  const int tiles = int(blocks_for(n, kTile));
+  Buffer<int> tile(ctx, tiles);
+  Buffer<long long> total(ctx, 1);
+  {
+    KernelTimer kt(ctx, "ccl_count");
+    ccl_count<<<tiles, 256, 0, s>>>(parent.data(), n, tile.data());
+    BROOK_LAUNCHED();
+  }
+  scan_tiles<<<1, 1024, 0, s>>>(tile.data(), tiles, total.data());
+  BROOK_LAUNCHED();
+  const int64_t count = total.download(ctx, 1)[0];
+  out.count = count;
+  Buffer<long long> roots(ctx, std::max<int64_t>(count, 1)), mapping(ctx, count + 1);
+  mapping.clear(ctx);
+  {
+    KernelTimer kt(ctx, "ccl_mark");
+    ccl_mark<T><<<tiles, 256, 0, s>>>(lab, parent.data(), n, tile.data(), roots.data(), mapping.data());
+    BROOK_LAUNCHED();
+  }
+  out.cc_bytes = id_bytes(count);
+  out.labels = Buffer<uint8_t>(ctx, size_t(n) * out.cc_bytes);
+  {
+    KernelTimer kt(ctx, "ccl_relabel");
+    if (out.cc_bytes == 1)
+      ccl_relabel<T, uint8_t><<<blocks_for(n, 256), 256, 0, s>>>(parent.data(), n, out.labels.data());
+    else if (out.cc_bytes == 2)
+      ccl_relabel<T, uint16_t><<<blocks_for(n, 256), 256, 0, s>>>(parent.data(), n,
+                                                                   reinterpret_cast<uint16_t *>(out.labels.data()));
+    else
+      ccl_relabel<T, uint32_t><<<blocks_for(n, 256), 256, 0, s>>>(parent.data(), n,
+                                                                   reinterpret_cast<uint32_t *>(out.labels.data()));
+    BROOK_LAUNCHED();
+  }
+  auto r = roots.download(ctx, count);
+  auto m = mapping.download(ctx, count + 1);
+  out.roots.assign(r.begin(), r.end());
+  out.mapping.assign(m.begin(), m.end());
+  if (in.is_signed && sizeof(T) < 8)
+    for (auto &v : out.mapping) {
+      const int shift = 64 - 8 * int(sizeof(T));
+      v = int64_t(uint64_t(v) << shift) >> shift;  // sign-extend the label's width
+    }
+  return out;
+}
+
+template <class CC> void run_analyze(Context &ctx, const Components &cc, Dims d, std::vector<Box> &boxes) {
+  const int n = d.sx * d.sy * d.sz;
+  size_t k = size_t(cc.count) + 1;
+  Buffer<int> lo(ctx, 3 * k), hi(ctx, 3 * k);
+  Buffer<unsigned long long> counts(ctx, k);
+  std::vector<int> l0(3 * k, INT32_MAX), h0(3 * k, -1);
+  lo.upload(ctx, l0.data(), l0.size());
+  hi.upload(ctx, h0.data(), h0.size());
+  counts.clear(ctx);
+  analyze_runs<CC><<<blocks_for(n, 256), 256, 0, ctx.stream()>>>(reinterpret_cast<const CC *>(cc.labels.data()), d, n,
+                                                                  lo.data(), hi.data(), counts.data());
+  BROOK_LAUNCHED();
+  auto L = lo.download(ctx), H = hi.download(ctx);
+  auto C = counts.download(ctx);
+  boxes.assign(k, Box{});
+  int64_t fg = 0;
+  for (size_t i = 1; i < k; ++i) {
+    for (int a = 0; a < 3; ++a) { boxes[i].lo[a] = L[3 * i + a]; boxes[i].hi[a] = H[3 * i + a]; }
+    boxes[i].count = int64_t(C[i]);
+    fg += boxes[i].count;
+  }
+  boxes[0].count = int64_t(n) - fg;
+}
+
+}  // namespace
+
+Components connected_components(Context &ctx, const LabelVolume &input) {
+  if (input.size() >= (int64_t(1) << 31) - kTile) throw Unsupported("volume exceeds the int32 voxel index space");
+  if (input.size() == 0) {
+    Components c;
+    c.mapping = {0};
+    return c;
+  }
+  switch (input.bytes) {
+    case 1: return run_ccl<uint8_t>(ctx, input);
+    case 2: return run_ccl<uint16_t>(ctx, input);
+    case 4: return run_ccl<uint32_t>(ctx, input);
+    case 8: return run_ccl<unsigned long long>(ctx, input);
+    default: throw std::invalid_argument("connected components requires integer labels");
+  }
+}
+
+std::vector<Box> analyze(Context &ctx, const Components &cc, Dims d) {
+  std::vector<Box> boxes;
+  if (int64_t(d.sx) * d.sy * d.sz == 0) { boxes.assign(1, Box{}); return boxes; }
+  switch (cc.cc_bytes) {
+    case 1: run_analyze<uint8_t>(ctx, cc, d, boxes); break;
+    case 2: run_analyze<uint16_t>(ctx, cc, d, boxes); break;
+    default: run_analyze<uint32_t>(ctx, cc, d, boxes); break;
+  }
+  return boxes;
+}
+
+void extract_face(Context &ctx, const void *cc, int cc_bytes, Dims d, int a, int b, int c, int fixed, void *plane) {
+  int dims[3] = {d.sx, d.sy, d.sz};
+  int fx = dims[a], fy = dims[b];
+  unsigned g = blocks_for(int64_t(fx) * fy, 256);
+  if (cc_bytes == 1)
+    face_copy<uint8_t><<<g, 256, 0, ctx.stream()>>>(static_cast<const uint8_t *>(cc), d, a, b, c, fixed, fx, fy,
+                                                    static_cast<uint8_t *>(plane));
+  else if (cc_bytes == 2)
+    face_copy<uint16_t><<<g, 256, 0, ctx.stream()>>>(static_cast<const uint16_t *>(cc), d, a, b, c, fixed, fx, fy,
+                                                     static_cast<uint16_t *>(plane));
+  else
+    face_copy<uint32_t><<<g, 256, 0, ctx.stream()>>>(static_cast<const uint32_t *>(cc), d, a, b, c, fixed, fx, fy,
+                                                     static_cast<uint32_t *>(plane));
+  BROOK_LAUNCHED();
+}
+
+}  // namespace brook

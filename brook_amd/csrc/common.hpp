@@ -1,0 +1,81 @@
+// Shared device helpers for the gfx950 kernels: neighbour tables, coherent loads, float tricks.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace brook {
+
+// 26-neighbour offsets in the reference's fixed order (faces, xy/yz/xz edges, corners;
+// common.cuh:33-47). Every flood, walk and invalidation visits neighbours in this order.
+__constant__ static const int kNX[26] = {-1, 1, 0, 0, 0, 0, -1, -1, 1, 1, 0, 0, 0, 0, -1, -1, 1, 1, -1, -1, -1, -1, 1, 1, 1, 1};
+__constant__ static const int kNY[26] = {0, 0, -1, 1, 0, 0, -1, 1, -1, 1, -1, -1, 1, 1, 0, 0, 0, 0, -1, -1, 1, 1, -1, -1, 1, 1};
+__constant__ static const int kNZ[26] = {0, 0, 0, 0, -1, 1, 0, 0, 0, 0, -1, 1, -1, 1, -1, 1, -1, 1, -1, 1, -1, 1, -1, 1, -1, 1};
+// Rank of neighbour k in dijkstra3d's order and its inverse (common.cuh:74-81); opposite offsets.
+__constant__ static const int kD3Rank[26] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 21, 20, 24, 19, 23, 22, 25};
+__constant__ static const int kD3Inv[26] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 22, 20, 19, 24, 23, 21, 25};
+__constant__ static const int kOpp[26] = {1, 0, 3, 2, 5, 4, 9, 8, 7, 6, 13, 12, 11, 10, 17, 16, 15, 14, 25, 24, 23, 22, 21, 20, 19, 18};
+
+constexpr float kInf = __builtin_huge_valf();
+constexpr unsigned long long kMaxKey = ~0ULL;
+
+struct Dims {
+  int sx, sy, sz;
+};
+
+__device__ __forceinline__ void unravel(int v, const Dims &d, int &x, int &y, int &z) {
+  int sxy = d.sx * d.sy;
+  z = v / sxy;
+  int r = v - z * sxy;
+  y = r / d.sx;
+  x = r - y * d.sx;
+}
+__device__ __forceinline__ bool inside(int x, int y, int z, const Dims &d) {
+  return (unsigned)x < (unsigned)d.sx && (unsigned)y < (unsigned)d.sy && (unsigned)z < (unsigned)d.sz;
+}
+__device__ __forceinline__ int flat(int x, int y, int z, const Dims &d) { return x + d.sx * (y + d.sy * z); }
+
+// Loads of memory that other threads change inside the same kernel: relaxed, agent scope, so the
+// access is served by L2 (the CU's vector L1 may hold a stale line otherwise).
+__device__ __forceinline__ float ld_f(const float *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ int ld_i(const int *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ unsigned ld_u(const unsigned *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ unsigned long long ld_u64(const unsigned long long *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned char ld_u8(const unsigned char *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_f(float *p, float v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ void st_i(int *p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ void st_u8(unsigned char *p, unsigned char v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_u64(unsigned long long *p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One vertex-weighted step fl(d + f) that never returns d itself for a positive cost: one ulp up
+// instead (the rule of the reference's brook_vw_step, common.cuh:123-127), which keeps distances
+// strictly increasing along every step so predecessor walks terminate.
+__device__ __forceinline__ float vw_step(float d, float f) {
+  float c = d + f;
+  if (c == d && f > 0.0f && d < kInf) c = __int_as_float(__float_as_int(d) + 1);
+  return c;
+}
+
+// atomicMin of a non-negative float through its int bits (monotone for floats >= 0, +inf included);
+// returns the previous value. A relaxed pre-check skips the atomic when it cannot improve.
+__device__ __forceinline__ float atomic_min_pos(float *addr, float val) {
+  float cur = ld_f(addr);
+  if (cur <= val) return cur;
+  return __int_as_float(atomicMin(reinterpret_cast<int *>(addr), __float_as_int(val)));
+}
+
+// Shuffle of any integer width up by `delta` lanes (wave64).
+template <class T> __device__ __forceinline__ T shfl_up_any(T v, unsigned delta) {
+  if constexpr (sizeof(T) == 8) return T(__shfl_up((unsigned long long)v, delta, 64));
+  else return T(__shfl_up((unsigned)v, delta, 64));
+}
+
+}  // namespace brook

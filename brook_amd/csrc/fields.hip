@@ -1,0 +1,263 @@
+// Field preparation per component label (batched.cu:177-255 restated for gfx950):
+//   * label voxel lists (segments laid out by id, from one wave-aggregated atomic per run),
+//   * one workgroup per label from a device work queue: root-search flood + argmax, DAF flood +
+//     argmax (label-gated geometric SSSP, the unique float32 min-plus fixpoint), no grid barriers,
+//   * PDRF and the alive mask in one streaming pass, the root rails, the target-order keys.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "fields.hpp"
+
+namespace brook {
+namespace {
+
+__device__ __forceinline__ unsigned long long upto(int lane) { return lane == 63 ? ~0ULL : ((2ULL << lane) - 1ULL); }
+
+// Voxel lists: vox[seg_start[l] + slot] = every voxel of id l (slot order is free; consumers
+// that need an order sort the list or reduce over it).
+template <class CC>
+__global__ void __launch_bounds__(256) fill_segments(const CC *cc, int n, Dims d, const int *seg_start, int *fill,
+                                                     int *vox) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  int lane = threadIdx.x & 63;
+  bool valid = i < n;
+  CC l = valid ? cc[i] : CC(0);
+  int x = valid ? i % d.sx : 0;
+  CC prev = shfl_up_any(l, 1);
+  bool start = !valid || lane == 0 || x == 0 || prev != l;
+  unsigned long long starts = __ballot(start);
+  unsigned long long mine = starts & upto(lane);
+  int my_start = 63 - __clzll((long long)mine);  // start lane of this lane's segment
+  int base = 0;
+  if (start && valid && l != CC(0)) {
+    unsigned long long after = starts & ~upto(lane);
+    int end_lane = after ? __ffsll((long long)after) - 2 : 63;
+    base = atomicAdd(fill + l, end_lane - lane + 1);
+  }
+  base = __shfl(base, my_start, 64);
+  if (valid && l != CC(0)) vox[seg_start[l] + base + (lane - my_start)] = i;
+}
+
+__global__ void fill_float(float *p, long long n, float v) {
+  long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Geometric flood of label L from one source inside one workgroup. dist must be +inf on the
+// label's voxels and inq 0 on entry; on exit inq is 0 again.
+template <class CC>
+__device__ void flood_label(const CC *cc, Dims d, CC L, int src, const float *W, float *dist, int *inq, int *cur,
+                            int *nxt, int *counts /*LDS [2]*/) {
+  if (threadIdx.x == 0) {
+    st_f(dist + src, 0.0f);
+    cur[0] = src;
+    counts[0] = 1;
+    counts[1] = 0;
+  }
+  __syncthreads();
+  for (;;) {
+    int nc = counts[0];
+    if (nc == 0) break;
+    for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+      int u = ld_i(cur + i);
+      atomicExch(inq + u, 0);  // clear, THEN read the distance
+      float du = ld_f(dist + u);
+      int x, y, z;
+      unravel(u, d, x, y, z);
+      for (int k = 0; k < 26; ++k) {
+        int nx = x + kNX[k], ny = y + kNY[k], nz = z + kNZ[k];
+        if (!inside(nx, ny, nz, d)) continue;
+        int v = flat(nx, ny, nz, d);
+        if (cc[v] != L) continue;
+        float cand = du + W[k];
+        float old = atomic_min_pos(dist + v, cand);
+        if (old > cand && atomicExch(inq + v, 1) == 0) nxt[atomicAdd(&counts[1], 1)] = v;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) { counts[0] = counts[1]; counts[1] = 0; }
+    int *t = cur; cur = nxt; nxt = t;
+    __syncthreads();
+  }
+}
+
+// Argmax of dist over the label's voxel list: largest finite value, ties -> lowest flat index
+// (the packed key of batched_regular.inc:9-22). Returns the key (0: none).
+__device__ unsigned long long label_argmax(const float *dist, const int *list, int count, unsigned long long *lds) {
+  if (threadIdx.x == 0) *lds = 0;
+  __syncthreads();
+  unsigned long long best = 0;
+  for (int j = threadIdx.x; j < count; j += blockDim.x) {
+    int v = list[j];
+    float f = ld_f(dist + v);
+    if (!(f < kInf)) continue;
+    unsigned long long key = ((unsigned long long)__float_as_uint(f) << 32) | (unsigned long long)(0xFFFFFFFFu - (unsigned)v);
+    best = key > best ? key : best;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    unsigned long long o = __shfl_down(best, off, 64);
+    best = o > best ? o : best;
+  }
+  if ((threadIdx.x & 63) == 0 && best) atomicMax(lds, best);
+  __syncthreads();
+  unsigned long long r = *lds;
+  __syncthreads();
+  return r;
+}
+
+template <class CC>
+__global__ void __launch_bounds__(512) prepare_labels(const CC *cc, Dims d, LabelTables t, const int *queue, int nqueue,
+                                                      int *next, float *dist, int *inq, int *lists_a, int *lists_b) {
+  __shared__ float W[26];
+  __shared__ int counts[2];
+  __shared__ unsigned long long best;
+  __shared__ int item;
+  if (threadIdx.x < 26) W[threadIdx.x] = t.weights[threadIdx.x];
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) item = atomicAdd(next, 1);
+    __syncthreads();
+    if (item >= nqueue) return;
+    const int L = queue[item];
+    const int s0 = t.seg_start[L], cnt = t.seg_count[L];
+    const int *list = t.vox + s0;
+    int *a = lists_a + s0, *b = lists_b + s0;
+    long long root = t.root[L];
+    if (root < 0) {  // no border target: flood from the component's first voxel, root = argmax
+      flood_label<CC>(cc, d, CC(L), int(t.first[L]), W, dist, inq, a, b, counts);
+      unsigned long long key = label_argmax(dist, list, cnt, &best);
+      root = key ? (long long)(0xFFFFFFFFu - (unsigned)key) : -1;
+      for (int j = threadIdx.x; j < cnt; j += blockDim.x) st_f(dist + list[j], kInf);
+      __syncthreads();
+      if (threadIdx.x == 0) t.root[L] = root;
+    }
+    flood_label<CC>(cc, d, CC(L), int(root), W, dist, inq, a, b, counts);
+    unsigned long long key = label_argmax(dist, list, cnt, &best);
+    if (threadIdx.x == 0) {
+      t.target[L] = key ? (long long)(0xFFFFFFFFu - (unsigned)key) : -1;
+      t.max_daf[L] = key ? __uint_as_float((unsigned)(key >> 32)) : 0.0f;
+    }
+  }
+}
+
+// PDRF (batched_regular.inc:93-108) and the alive mask in one pass. use[l] 0: not traced.
+template <class CC>
+__global__ void __launch_bounds__(256) pdrf_alive(const CC *cc, const float *dbf, const float *daf, const float *M,
+                                                  const float *inv, const unsigned char *use, int squarings, float scale,
+                                                  long long n, float *pdrf, unsigned char *alive) {
+  long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  CC l = cc[i];
+  unsigned char u = use[l];
+  alive[i] = (l != CC(0) && u) ? 1 : 0;
+  if (l == CC(0) || u == 0) { pdrf[i] = kInf; return; }
+  float dv = dbf[i];
+  if (dv == 0.0f) dv = kInf;
+  float p = 1.0f - dv * M[l];
+  for (int k = 0; k < squarings; ++k) p = p * p;
+  p = p * scale;
+  if (u == 2) {
+    float a = daf[i];
+    if (isinf(a)) a = 0.0f;
+    p = p + a * inv[l];
+  }
+  pdrf[i] = p;
+}
+
+__global__ void set_rails(float *pdrf, const long long *root, const int *labels, int count) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < count) pdrf[root[labels[i]]] = 0.0f;
+}
+
+// Target-order key of each listed voxel: ascending keys = DAF descending (inf and 0 as 0),
+// flat index descending (lockstep_order.cu:21-43).
+__global__ void order_keys(const int *vox, long long total, const float *daf, unsigned long long *keys) {
+  long long j = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (j >= total) return;
+  int v = vox[j];
+  float f = daf[v];
+  if (isinf(f) || f == 0.0f) f = 0.0f;
+  keys[j] = ((unsigned long long)(0xFFFFFFFFu - __float_as_uint(f)) << 32) | (unsigned long long)(0xFFFFFFFFu - (unsigned)v);
+}
+
+unsigned grid_for(long long n) { return unsigned((n + 255) / 256); }
+
+}  // namespace
+
+void build_segments(Context &ctx, const void *cc, int cc_bytes, Dims d, const std::vector<int> &seg_start,
+                    Buffer<int> &seg_start_dev, Buffer<int> &vox) {
+  const int n = d.sx * d.sy * d.sz;
+  seg_start_dev = Buffer<int>(ctx, seg_start.size());
+  seg_start_dev.upload(ctx, seg_start.data(), seg_start.size());
+  Buffer<int> fill(ctx, seg_start.size());
+  fill.clear(ctx);
+  vox = Buffer<int>(ctx, std::max(1, seg_start.back()));
+  if (cc_bytes == 1)
+    fill_segments<uint8_t><<<grid_for(n), 256, 0, ctx.stream()>>>(static_cast<const uint8_t *>(cc), n, d,
+                                                                  seg_start_dev.data(), fill.data(), vox.data());
+  else if (cc_bytes == 2)
+    fill_segments<uint16_t><<<grid_for(n), 256, 0, ctx.stream()>>>(static_cast<const uint16_t *>(cc), n, d,
+                                                                   seg_start_dev.data(), fill.data(), vox.data());
+  else
+    fill_segments<uint32_t><<<grid_for(n), 256, 0, ctx.stream()>>>(static_cast<const uint32_t *>(cc), n, d,
+                                                                   seg_start_dev.data(), fill.data(), vox.data());
+  BROOK_LAUNCHED();
+}
+
+void fill_value(Context &ctx, float *p, long long n, float v) {
+  if (n <= 0) return;
+  fill_float<<<grid_for(n), 256, 0, ctx.stream()>>>(p, n, v);
+  BROOK_LAUNCHED();
+}
+
+void prepare_fields(Context &ctx, const void *cc, int cc_bytes, Dims d, const LabelTables &t, const int *queue, int nqueue,
+                    float *dist, int *inq, int *lists_a, int *lists_b) {
+  if (nqueue == 0) return;
+  Buffer<int> next(ctx, 1);
+  next.clear(ctx);
+  int grid = std::min(nqueue, ctx.compute_units() * 2);
+  KernelTimer kt(ctx, "prepare_labels");
+  if (cc_bytes == 1)
+    prepare_labels<uint8_t><<<grid, 512, 0, ctx.stream()>>>(static_cast<const uint8_t *>(cc), d, t, queue, nqueue,
+                                                            next.data(), dist, inq, lists_a, lists_b);
+  else if (cc_bytes == 2)
+    prepare_labels<uint16_t><<<grid, 512, 0, ctx.stream()>>>(static_cast<const uint16_t *>(cc), d, t, queue, nqueue,
+                                                             next.data(), dist, inq, lists_a, lists_b);
+  else
+    prepare_labels<uint32_t><<<grid, 512, 0, ctx.stream()>>>(static_cast<const uint32_t *>(cc), d, t, queue, nqueue,
+                                                             next.data(), dist, inq, lists_a, lists_b);
+  BROOK_LAUNCHED();
+  ctx.synchronize();
+}
+
+void pdrf_and_alive(Context &ctx, const void *cc, int cc_bytes, long long n, const float *dbf, const float *daf,
+                    const float *M, const float *inv, const unsigned char *use, int squarings, float scale, float *pdrf,
+                    unsigned char *alive) {
+  KernelTimer kt(ctx, "pdrf_alive");
+  if (cc_bytes == 1)
+    pdrf_alive<uint8_t><<<grid_for(n), 256, 0, ctx.stream()>>>(static_cast<const uint8_t *>(cc), dbf, daf, M, inv, use,
+                                                               squarings, scale, n, pdrf, alive);
+  else if (cc_bytes == 2)
+    pdrf_alive<uint16_t><<<grid_for(n), 256, 0, ctx.stream()>>>(static_cast<const uint16_t *>(cc), dbf, daf, M, inv, use,
+                                                                squarings, scale, n, pdrf, alive);
+  else
+    pdrf_alive<uint32_t><<<grid_for(n), 256, 0, ctx.stream()>>>(static_cast<const uint32_t *>(cc), dbf, daf, M, inv, use,
+                                                                squarings, scale, n, pdrf, alive);
+  BROOK_LAUNCHED();
+}
+
+void root_rails(Context &ctx, float *pdrf, const long long *root, const int *labels, int count) {
+  if (!count) return;
+  set_rails<<<grid_for(count), 256, 0, ctx.stream()>>>(pdrf, root, labels, count);
+  BROOK_LAUNCHED();
+}
+
+void target_order_keys(Context &ctx, const int *vox, long long total, const float *daf, unsigned long long *keys) {
+  if (!total) return;
+  order_keys<<<grid_for(total), 256, 0, ctx.stream()>>>(vox, total, daf, keys);
+  BROOK_LAUNCHED();
+}
+
+}  // namespace brook

@@ -1,0 +1,39 @@
+// Per-label field preparation and the tracer's device tables.
+#pragma once
+#include <vector>
+
+#include "stages.hpp"
+
+namespace brook {
+
+// Device tables indexed by component id (count + 1 entries).
+struct LabelTables {
+  const int *seg_start = nullptr;  // first entry of the id's voxel list
+  const int *seg_count = nullptr;  // voxels of the id
+  const int *vox = nullptr;        // voxel lists (unordered within an id)
+  const long long *first = nullptr;  // the id's minimum flat index (root search source)
+  long long *root = nullptr;       // in: border root or -1; out: root
+  long long *target = nullptr;     // out: DAF argmax (lowest flat index on ties)
+  float *max_daf = nullptr;        // out: DAF maximum
+  const float *weights = nullptr;  // 26 geometric step weights
+};
+
+void build_segments(Context &ctx, const void *cc, int cc_bytes, Dims d, const std::vector<int> &seg_start,
+                    Buffer<int> &seg_start_dev, Buffer<int> &vox);
+void fill_value(Context &ctx, float *p, long long n, float v);
+// Root search and DAF floods for the ids in `queue` (one workgroup per id, largest first).
+// dist: +inf volume in, DAF out; inq: zero volume in and out; lists_*: scratch in segment layout.
+void prepare_fields(Context &ctx, const void *cc, int cc_bytes, Dims d, const LabelTables &t, const int *queue, int nqueue,
+                    float *dist, int *inq, int *lists_a, int *lists_b);
+void pdrf_and_alive(Context &ctx, const void *cc, int cc_bytes, long long n, const float *dbf, const float *daf,
+                    const float *M, const float *inv, const unsigned char *use, int squarings, float scale, float *pdrf,
+                    unsigned char *alive);
+void root_rails(Context &ctx, float *pdrf, const long long *root, const int *labels, int count);
+void target_order_keys(Context &ctx, const int *vox, long long total, const float *daf, unsigned long long *keys);
+// Segmented ascending sort of 64-bit keys (segments = id voxel lists).
+void sort_segments(Context &ctx, const unsigned long long *in, unsigned long long *out, long long total,
+                   const int *seg_begin, const int *seg_end, int segments);
+// Ascending sort of 64-bit keys over [0, end_bit).
+void sort_keys(Context &ctx, unsigned long long *keys, unsigned long long *scratch, long long count, int end_bit);
+
+}  // namespace brook

@@ -1,0 +1,528 @@
+// The TEASAR trace loop on gfx950: one workgroup per component label, labels pulled from a
+// device work queue (largest first), every label's whole path sequence traced inside its
+// workgroup with workgroup barriers only -- no grid-wide barrier, no host round trip per path.
+//
+// Per label, exactly the semantics of the reference's lockstep tracer (labels never interact, so
+// one label at a time is exact; lockstep.cu:91-622, lockstep_coop.inc, lockstep_regular.inc):
+//   target     manual targets first (in pop order), then the first alive voxel of the label's
+//              descending-DAF order at or after its cursor          (lockstep_coop.inc:343-416)
+//   railroad   vertex-weighted SSSP from the target with the one-ulp step, rails absorb, near-far
+//              bands bounded by the best rail; the stop rail is min (distance, flat index)
+//                                                                   (lockstep_coop.inc:71-305)
+//   backtrace  one wave: lane k probes neighbour k, two wave-min reductions give the lowest k
+//              among the smallest distances; the join takes u's first rail in dijkstra3d order
+//                                                                   (lockstep_regular.inc:61-154)
+//   rails      path voxels get PDRF 0                               (lockstep_regular.inc:246-251)
+//   invalidate rolling ball, bands of the bottleneck M on a fixed grid, synchronous rounds from
+//              a snapshot inside a band, exact pulls at band boundaries
+//                                                         (invalidation_kernels.cuh:105-299)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "trace.hpp"
+
+namespace brook {
+namespace {
+
+constexpr int kBlock = 512;
+constexpr int kDQ = 1, kDF = 2, kIQ = 4, kIW = 8;
+constexpr unsigned long long kMinfarNone = 0x7f8000007f800000ULL;
+constexpr unsigned long long kClaimed = 0xFFFFFFFFFFFFFFFEULL;
+constexpr int kLaneFront = 16;  // fronts up to this size relax with one lane per neighbour
+
+__device__ __forceinline__ int wave_min(int v) {
+  for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+template <class CC> struct Label {
+  const TraceArgs &a;
+  const CC *cc;
+  CC L;
+  int seg, count;
+};
+
+struct Shared {
+  int n_cur, n_nxt, n_far, n_farnxt, n_touched, n_keep;
+  int done, target, status, path_len, found, band;
+  long long cursor, seg_end;
+  int pcount;
+  unsigned lowest;
+  float T;
+  unsigned long long rail_best, minfar;
+  int store_pos;
+  float lane_du[kLaneFront];
+  int lane_u[kLaneFront];
+};
+
+// ---- railroad -------------------------------------------------------------------------------
+template <class CC>
+__device__ __forceinline__ void relax(const Label<CC> &lb, Shared &sh, int u, int x, int y, int z, int k, float du,
+                                      float bound, int *nxt, int *far, int *touched) {
+  const TraceArgs &a = lb.a;
+  int nx = x + kNX[k], ny = y + kNY[k], nz = z + kNZ[k];
+  if (!inside(nx, ny, nz, a.d)) return;
+  int v = flat(nx, ny, nz, a.d);
+  if (lb.cc[v] != lb.L) return;
+  float fv = ld_f(a.pdrf + v);
+  float cand = vw_step(du, fv);
+  float old = atomic_min_pos(a.dist + v, cand);
+  if (!(old > cand)) return;
+  if (old >= kInf) touched[atomicAdd(&sh.n_touched, 1)] = v;
+  if (fv == 0.0f) {  // a rail: fixes the cost and the last road voxel; never expanded
+    atomicMin(&sh.rail_best, ((unsigned long long)__float_as_uint(cand) << 32) | (unsigned)v);
+    return;
+  }
+  if (cand <= bound) {
+    if (!(atomicOr(a.state + v, kDQ) & kDQ)) nxt[atomicAdd(&sh.n_nxt, 1)] = v;
+  } else {
+    atomicMin(&sh.minfar, ((unsigned long long)__float_as_uint(cand) << 32) | __float_as_uint(fv));
+    if (!(atomicOr(a.state + v, kDF) & kDF)) far[atomicAdd(&sh.n_far, 1)] = v;
+  }
+}
+
+__device__ __forceinline__ float rail_bound(const Shared &sh) {
+  float b = sh.T;
+  unsigned long long rb = sh.rail_best;
+  if (rb != kMaxKey) b = fminf(b, __uint_as_float((unsigned)(rb >> 32)));
+  return b;
+}
+
+// Returns 1 when the nearest rail is settled (sh.rail_best), 2 when no rail is reachable.
+template <class CC>
+__device__ int railroad(const Label<CC> &lb, Shared &sh, int t, int *near_a, int *near_b, int *far_a, int *far_b,
+                        int *touched) {
+  const TraceArgs &a = lb.a;
+  if (threadIdx.x == 0) {
+    st_f(a.dist + t, 0.0f);
+    touched[0] = t;
+    sh.n_touched = 1;
+    sh.n_cur = sh.n_nxt = sh.n_far = sh.n_farnxt = 0;
+    sh.minfar = kMinfarNone;
+    float ft = ld_f(a.pdrf + t);
+    if (ft == 0.0f) {
+      sh.rail_best = (unsigned)t;
+      sh.done = 1;
+    } else {
+      sh.rail_best = kMaxKey;
+      sh.done = 0;
+      near_a[0] = t;
+      sh.n_cur = 1;
+      atomicOr(a.state + t, kDQ);
+      sh.T = fmaxf(2.0f * ft, 1e-30f);
+    }
+  }
+  __syncthreads();
+  int *cur = near_a, *nxt = near_b, *fcur = far_a, *fnxt = far_b;
+  while (sh.done == 0) {
+    for (;;) {  // settle the near band
+      const int nc = sh.n_cur;
+      if (nc == 0) break;
+      const float bound = rail_bound(sh);
+      if (nc <= kLaneFront) {
+        if ((int)threadIdx.x < nc) {
+          int u = ld_i(cur + threadIdx.x);
+          atomicAnd(a.state + u, ~kDQ);  // clear, THEN read the distance
+          float du = ld_f(a.dist + u);
+          bool skip = ld_f(a.pdrf + u) == 0.0f || du > bound;
+          sh.lane_du[threadIdx.x] = skip ? -1.0f : du;
+          sh.lane_u[threadIdx.x] = u;
+        }
+        __syncthreads();
+        for (int j = threadIdx.x; j < nc * 32; j += blockDim.x) {
+          int i = j >> 5, k = j & 31;
+          float du = sh.lane_du[i];
+          if (k >= 26 || du < 0.0f) continue;
+          int u = sh.lane_u[i], x, y, z;
+          unravel(u, a.d, x, y, z);
+          relax(lb, sh, u, x, y, z, k, du, bound, nxt, fcur, touched);
+        }
+      } else {
+        for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+          int u = ld_i(cur + i);
+          atomicAnd(a.state + u, ~kDQ);
+          float du = ld_f(a.dist + u);
+          if (ld_f(a.pdrf + u) == 0.0f || du > bound) continue;
+          int x, y, z;
+          unravel(u, a.d, x, y, z);
+          for (int k = 0; k < 26; ++k) relax(lb, sh, u, x, y, z, k, du, bound, nxt, fcur, touched);
+        }
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) { sh.n_cur = sh.n_nxt; sh.n_nxt = 0; }
+      int *tmp = cur; cur = nxt; nxt = tmp;
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {  // phase end: nearest rail final? else advance the band
+      unsigned long long rb = sh.rail_best, mf = sh.minfar;
+      if (rb != kMaxKey && (__uint_as_float((unsigned)(rb >> 32)) <= sh.T || mf == kMinfarNone)) sh.done = 1;
+      else if (mf == kMinfarNone) sh.done = 2;
+      else {
+        float dmin = __uint_as_float((unsigned)(mf >> 32)), f = __uint_as_float((unsigned)mf);
+        float w = 2.0f * f, lo = 5e-4f * dmin;
+        if (!(w > lo)) w = lo;
+        sh.T = dmin + w;
+        sh.minfar = kMinfarNone;
+      }
+    }
+    __syncthreads();
+    if (sh.done) break;
+    const int nf = sh.n_far;
+    const float bound = rail_bound(sh);
+    for (int i = threadIdx.x; i < nf; i += blockDim.x) {  // promote far voxels inside the new band
+      int w = ld_i(fcur + i);
+      if (!(ld_i(a.state + w) & kDF)) continue;
+      float dw = ld_f(a.dist + w);
+      if (dw <= bound) {
+        atomicAnd(a.state + w, ~kDF);
+        if (!(atomicOr(a.state + w, kDQ) & kDQ)) cur[atomicAdd(&sh.n_cur, 1)] = w;
+      } else {
+        atomicMin(&sh.minfar, ((unsigned long long)__float_as_uint(dw) << 32) | __float_as_uint(ld_f(a.pdrf + w)));
+        fnxt[atomicAdd(&sh.n_farnxt, 1)] = w;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) { sh.n_far = sh.n_farnxt; sh.n_farnxt = 0; }
+    int *tmp = fcur; fcur = fnxt; fnxt = tmp;
+    __syncthreads();
+  }
+  return sh.done;
+}
+
+// ---- backtrace (wave 0) ---------------------------------------------------------------------
+template <class CC>
+__device__ void backtrace(const Label<CC> &lb, Shared &sh, int t, int *walk) {
+  const TraceArgs &a = lb.a;
+  if (threadIdx.x >= 64) return;
+  const int lane = threadIdx.x;
+  int cur = (int)(unsigned)(sh.rail_best & 0xFFFFFFFFULL);
+  int x, y, z;
+  unravel(cur, a.d, x, y, z);
+  int cnt = 0, kept = 0;
+  for (;;) {
+    int kb = -1;
+    if (cur != t) {
+      float du = kInf;
+      if (lane < 26) {
+        int nx = x + kNX[lane], ny = y + kNY[lane], nz = z + kNZ[lane];
+        if (inside(nx, ny, nz, a.d)) {
+          int u = flat(nx, ny, nz, a.d);
+          if (lb.cc[u] == lb.L && ld_f(a.pdrf + u) != 0.0f) du = ld_f(a.dist + u);
+        }
+      }
+      int db = __float_as_int(du);
+      int mb = wave_min(db);
+      if (mb < 0x7f800000) kb = wave_min(db == mb ? lane : 64);
+    }
+    if (cnt == 0 && kb >= 0) {  // the join at u = cur + N[kb]: u's first rail in dijkstra3d order
+      int ux = x + kNX[kb], uy = y + kNY[kb], uz = z + kNZ[kb];
+      int key = 64;
+      if (lane < 26) {
+        int nx = ux + kNX[lane], ny = uy + kNY[lane], nz = uz + kNZ[lane];
+        if (inside(nx, ny, nz, a.d)) {
+          int v = flat(nx, ny, nz, a.d);
+          if (lb.cc[v] == lb.L && ld_f(a.pdrf + v) == 0.0f) key = kD3Rank[lane];
+        }
+      }
+      int r = wave_min(key);
+      if (r < 26) {
+        int ks = kD3Inv[r];
+        x = ux + kNX[ks]; y = uy + kNY[ks]; z = uz + kNZ[ks];
+        cur = flat(x, y, z, a.d);
+        kb = kOpp[ks];
+      }
+    }
+    if (kept >= lb.count) { kept = -1; break; }  // longer than the label: cannot happen on a valid field
+    if (lane == 0) walk[kept] = cur;
+    ++kept;
+    ++cnt;
+    if (kb < 0) break;
+    x += kNX[kb]; y += kNY[kb]; z += kNZ[kb];
+    cur = flat(x, y, z, a.d);
+  }
+  if (lane == 0) sh.path_len = kept;
+}
+
+// ---- invalidation ---------------------------------------------------------------------------
+__device__ __forceinline__ float offer_distance(const TraceArgs &a, int x, int y, int z, int src) {
+  int qx, qy, qz;
+  unravel(src, a.d, qx, qy, qz);
+  float dx = a.wx * (float)(x - qx), dy = a.wy * (float)(y - qy), dz = a.wz * (float)(z - qz);
+  return sqrtf(dx * dx + dy * dy + dz * dz);
+}
+
+template <class CC>
+__device__ __forceinline__ void offer(const Label<CC> &lb, Shared &sh, int x, int y, int z, int k, unsigned su,
+                                      unsigned mu, float maxd, unsigned hi, int *nxt, int *far, int *touched) {
+  const TraceArgs &a = lb.a;
+  int nx = x + kNX[k], ny = y + kNY[k], nz = z + kNZ[k];
+  if (!inside(nx, ny, nz, a.d)) return;
+  int v = flat(nx, ny, nz, a.d);
+  if (lb.cc[v] != lb.L || !ld_u8(a.alive + v)) return;
+  float d = offer_distance(a, nx, ny, nz, int(su));
+  if (d >= maxd) return;
+  unsigned mb = max(__float_as_uint(d), mu);  // M = max(M[u], d)
+  if (mb < hi) {  // inside the band: claim now
+    unsigned long long cand = ((unsigned long long)mb << 32) | su;
+    unsigned long long cur = ld_u64(a.key + v);
+    if (cur <= cand) return;
+    unsigned long long old = atomicMin(a.key + v, cand);
+    if (old > cand) {
+      if (old == kMaxKey) touched[atomicAdd(&sh.n_touched, 1)] = v;
+      int was = atomicOr(a.state + v, kIQ);
+      if (was & kIW) atomicAnd(a.state + v, ~kIW);
+      if (!(was & kIQ)) nxt[atomicAdd(&sh.n_nxt, 1)] = v;
+    }
+  } else if (ld_u64(a.key + v) == kMaxKey) {  // beyond it: wait for its band
+    int seen = ld_i(a.state + v);
+    while (!(seen & (kIQ | kIW))) {
+      int got = atomicCAS(a.state + v, seen, seen | kIW);
+      if (got == seen) { far[atomicAdd(&sh.n_far, 1)] = v; break; }
+      seen = got;
+    }
+  }
+}
+
+__device__ __forceinline__ int band_of(unsigned mbits, float delta, int kmin) {
+  float m = __uint_as_float(mbits);
+  int k = (int)(m / delta);
+  while ((float)(k + 1) * delta <= m) k++;
+  while (k > 0 && (float)k * delta > m) k--;
+  return k < kmin ? kmin : k;
+}
+
+template <class CC>
+__device__ void invalidate(const Label<CC> &lb, Shared &sh, const int *path, int len, int *la, int *lb_, int *lfar,
+                           int *lkeep, int *touched, unsigned long long *snap, unsigned long long *pull) {
+  const TraceArgs &a = lb.a;
+  if (threadIdx.x == 0) { sh.n_cur = sh.n_nxt = sh.n_far = sh.n_keep = sh.n_touched = 0; sh.band = 0; }
+  __syncthreads();
+  int *cur = la, *nxt = lb_, *far = lfar, *keep = lkeep;
+  for (int j = threadIdx.x; j < len; j += blockDim.x) {  // sources: path voxels still alive
+    int p = path[j];
+    if (!ld_u8(a.alive + p)) continue;
+    if (atomicCAS(a.key + p, kMaxKey, (unsigned long long)(unsigned)p) == kMaxKey) {
+      atomicOr(a.state + p, kIQ);
+      cur[atomicAdd(&sh.n_cur, 1)] = p;
+      touched[atomicAdd(&sh.n_touched, 1)] = p;
+    }
+  }
+  __syncthreads();
+  for (;;) {
+    unsigned hi = __float_as_uint((float)(sh.band + 1) * a.delta);
+    for (;;) {  // synchronous rounds inside the band
+      const int nc = sh.n_cur;
+      if (nc == 0) break;
+      for (int i = threadIdx.x; i < nc; i += blockDim.x) {  // snapshot, re-arm
+        int u = ld_i(cur + i);
+        snap[i] = ld_u64(a.key + u);
+        atomicAnd(a.state + u, ~kIQ);
+      }
+      __syncthreads();
+      const bool lanes = nc <= kLaneFront * 2;
+      const int items = lanes ? nc * 32 : nc;
+      for (int j = threadIdx.x; j < items; j += blockDim.x) {
+        int i = lanes ? (j >> 5) : j;
+        int k0 = lanes ? (j & 31) : 0, k1 = lanes ? k0 + 1 : 26;
+        if (k0 >= 26) continue;
+        int u = ld_i(cur + i);
+        unsigned long long ku = snap[i];
+        unsigned su = (unsigned)ku, mu = (unsigned)(ku >> 32);
+        float maxd = a.scale * a.dbf[su] + a.constant;
+        int x, y, z;
+        unravel(u, a.d, x, y, z);
+        for (int k = k0; k < k1; ++k) offer(lb, sh, x, y, z, k, su, mu, maxd, hi, nxt, far, touched);
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) { sh.n_cur = sh.n_nxt; sh.n_nxt = 0; }
+      int *tmp = cur; cur = nxt; nxt = tmp;
+      __syncthreads();
+    }
+    const int nf = sh.n_far;
+    if (nf == 0) break;
+    if (threadIdx.x == 0) sh.lowest = 0x7f800000u;
+    __syncthreads();
+    for (int i = threadIdx.x; i < nf; i += blockDim.x) {  // band boundary: exact pulls
+      int c = ld_i(far + i);
+      if (ld_u64(a.key + c) != kMaxKey) { pull[i] = kClaimed; continue; }
+      int x, y, z;
+      unravel(c, a.d, x, y, z);
+      unsigned long long best = kMaxKey;
+      for (int k = 0; k < 26; ++k) {
+        int nx = x + kNX[k], ny = y + kNY[k], nz = z + kNZ[k];
+        if (!inside(nx, ny, nz, a.d)) continue;
+        int u = flat(nx, ny, nz, a.d);
+        if (lb.cc[u] != lb.L) continue;
+        unsigned long long ku = ld_u64(a.key + u);
+        if (ku == kMaxKey) continue;
+        unsigned su = (unsigned)ku;
+        float d = offer_distance(a, x, y, z, int(su));
+        if (d >= a.scale * a.dbf[su] + a.constant) continue;
+        unsigned mb = max(__float_as_uint(d), (unsigned)(ku >> 32));
+        unsigned long long cand = ((unsigned long long)mb << 32) | su;
+        if (cand < best) best = cand;
+      }
+      pull[i] = best;
+      if (best != kMaxKey) atomicMin(&sh.lowest, (unsigned)(best >> 32));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && sh.lowest != 0x7f800000u) sh.band = band_of(sh.lowest, a.delta, sh.band + 1);
+    __syncthreads();
+    hi = __float_as_uint((float)(sh.band + 1) * a.delta);
+    for (int i = threadIdx.x; i < nf; i += blockDim.x) {  // promote / keep waiting / drop
+      int c = ld_i(far + i);
+      unsigned long long p = pull[i];
+      if (p == kClaimed) continue;
+      if (p == kMaxKey) { atomicAnd(a.state + c, ~kIW); continue; }
+      if ((unsigned)(p >> 32) < hi) {
+        st_u64(a.key + c, p);
+        touched[atomicAdd(&sh.n_touched, 1)] = c;
+        cur[atomicAdd(&sh.n_cur, 1)] = c;
+        atomicOr(a.state + c, kIQ);
+        atomicAnd(a.state + c, ~kIW);
+      } else {
+        keep[atomicAdd(&sh.n_keep, 1)] = c;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) { sh.n_far = sh.n_keep; sh.n_keep = 0; }
+    int *tmp = far; far = keep; keep = tmp;
+    __syncthreads();
+  }
+  const int nt = sh.n_touched;
+  if (threadIdx.x == 0) atomicAdd(reinterpret_cast<unsigned long long *>(a.counters + 6), (unsigned long long)nt);
+  for (int j = threadIdx.x; j < nt; j += blockDim.x) {  // the claimed set dies
+    int v = touched[j];
+    st_u8(a.alive + v, 0);
+    st_u64(a.key + v, kMaxKey);
+    atomicAnd(a.state + v, ~(kIQ | kIW));
+  }
+  __syncthreads();
+}
+
+// ---- target selection -----------------------------------------------------------------------
+template <class CC> __device__ int next_target(const Label<CC> &lb, Shared &sh) {
+  const TraceArgs &a = lb.a;
+  if (threadIdx.x == 0) {
+    int L = int(lb.L), pc = sh.pcount;
+    if (pc >= a.maxp[L]) sh.target = -1;
+    else if (pc < a.mcount[L]) sh.target = a.mflat[a.mstart[L] + pc];
+    else sh.target = -2;
+  }
+  __syncthreads();
+  if (sh.target != -2) return sh.target;
+  const int per = 4, window = per * blockDim.x;
+  for (;;) {
+    long long c = sh.cursor, e = sh.seg_end;
+    if (c >= e) {
+      __syncthreads();
+      if (threadIdx.x == 0) sh.target = -1;
+      __syncthreads();
+      return -1;
+    }
+    if (threadIdx.x == 0) sh.found = INT32_MAX;
+    __syncthreads();
+    for (int r = 0; r < per; ++r) {
+      long long j = c + r * blockDim.x + threadIdx.x;
+      if (j < e) {
+        int v = int(0xFFFFFFFFu - (unsigned)a.order[j]);
+        if (ld_u8(a.alive + v)) atomicMin(&sh.found, int(j - c));
+      }
+    }
+    __syncthreads();
+    if (sh.found != INT32_MAX) {
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        sh.cursor = c + sh.found;
+        sh.target = int(0xFFFFFFFFu - (unsigned)a.order[c + sh.found]);
+      }
+      __syncthreads();
+      return sh.target;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) sh.cursor = c + window;
+    __syncthreads();
+  }
+}
+
+template <class CC>
+__global__ void __launch_bounds__(kBlock) trace_labels(TraceArgs a) {
+  __shared__ Shared sh;
+  __shared__ int item;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) item = atomicAdd(a.next, 1);
+    __syncthreads();
+    if (item >= a.nqueue) return;
+    const int L = a.queue[item];
+    Label<CC> lb{a, static_cast<const CC *>(a.cc), CC(L), a.seg_start[L], a.seg_count[L]};
+    int *l0 = a.lists[0] + lb.seg, *l1 = a.lists[1] + lb.seg, *l2 = a.lists[2] + lb.seg, *l3 = a.lists[3] + lb.seg,
+        *l4 = a.lists[4] + lb.seg, *walk = a.lists[5] + lb.seg;
+    unsigned long long *s0 = a.snaps[0] + lb.seg, *s1 = a.snaps[1] + lb.seg;
+    if (threadIdx.x == 0) {
+      sh.cursor = lb.seg;
+      sh.seg_end = (long long)lb.seg + lb.count;
+      sh.pcount = 0;
+    }
+    __syncthreads();
+    for (;;) {
+      const int t = next_target(lb, sh);
+      if (t < 0) break;
+      const int status = railroad(lb, sh, t, l0, l1, l2, l3, l4);
+      if (status == 1) backtrace(lb, sh, t, walk);
+      __syncthreads();
+      const int nt = sh.n_touched;
+      for (int j = threadIdx.x; j < nt; j += blockDim.x) {  // restore the railroad state
+        int v = l4[j];
+        st_f(a.dist + v, kInf);
+        atomicAnd(a.state + v, ~(kDQ | kDF));
+      }
+      if (status != 1 || sh.path_len < 0) {
+        if (threadIdx.x == 0) a.ejected[atomicAdd(a.counters + 2, 1) % a.ejected_cap] = L;
+        __syncthreads();
+        break;
+      }
+      const int len = sh.path_len;
+      if (threadIdx.x == 0) {
+        atomicAdd(reinterpret_cast<unsigned long long *>(a.counters + 4), (unsigned long long)nt);
+        int pos = atomicAdd(a.counters + 0, len);
+        int rec = atomicAdd(a.counters + 1, 1);
+        if ((long long)pos + len > a.store_cap || rec >= a.rec_cap) {
+          atomicOr(a.counters + 3, 1);  // overflow: the host retries with larger stores
+          sh.store_pos = -1;
+        } else {
+          sh.store_pos = pos;
+          a.rec_label[rec] = L;
+          a.rec_len[rec] = len;
+          a.rec_start[rec] = pos;
+        }
+      }
+      __syncthreads();
+      const int pos = sh.store_pos;
+      for (int j = threadIdx.x; j < len; j += blockDim.x) {
+        int v = walk[j];
+        if (pos >= 0) a.store[pos + (len - 1 - j)] = v;  // stored target-first
+        st_f(a.pdrf + v, 0.0f);                          // the path becomes rail
+      }
+      __syncthreads();
+      invalidate(lb, sh, walk, len, l0, l1, l2, l3, l4, s0, s1);
+      if (threadIdx.x == 0) sh.pcount += 1;
+      __syncthreads();
+    }
+  }
+}
+
+}  // namespace
+
+void trace_all(Context &ctx, int cc_bytes, const TraceArgs &args) {
+  if (args.nqueue == 0) return;
+  const int grid = std::min(args.nqueue, ctx.compute_units() * 2);
+  KernelTimer kt(ctx, "trace_labels");
+  if (cc_bytes == 1) trace_labels<uint8_t><<<grid, kBlock, 0, ctx.stream()>>>(args);
+  else if (cc_bytes == 2) trace_labels<uint16_t><<<grid, kBlock, 0, ctx.stream()>>>(args);
+  else trace_labels<uint32_t><<<grid, kBlock, 0, ctx.stream()>>>(args);
+  BROOK_LAUNCHED();
+}
+
+}  // namespace brook

@@ -1,0 +1,38 @@
+// The per-label trace loop (target, railroad, backtrace, rails, invalidation).
+#pragma once
+#include "stages.hpp"
+
+namespace brook {
+
+struct TraceArgs {
+  Dims d{0, 0, 0};
+  const void *cc = nullptr;
+  const float *dbf = nullptr;
+  float *pdrf = nullptr;          // rails become 0 in place
+  float *dist = nullptr;          // +inf volume (restored after every railroad)
+  int *state = nullptr;           // 0 volume: railroad / invalidation queue bits
+  unsigned long long *key = nullptr;  // ~0 volume: invalidation claims (restored)
+  unsigned char *alive = nullptr;
+  const int *seg_start = nullptr, *seg_count = nullptr;
+  const unsigned long long *order = nullptr;  // sorted order keys in segment layout (voxel = ~low 32 bits)
+  const int *mstart = nullptr, *mcount = nullptr, *mflat = nullptr, *maxp = nullptr;
+  int *lists[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // int scratch, segment layout
+  unsigned long long *snaps[2] = {nullptr, nullptr};                       // u64 scratch, segment layout
+  int *store = nullptr;
+  long long store_cap = 0;
+  int *rec_label = nullptr, *rec_len = nullptr, *rec_start = nullptr;
+  int rec_cap = 0;
+  // [0] store used [1] records [2] ejected labels [3] overflow; [4..5] u64 railroad-reached voxels,
+  // [6..7] u64 invalidated voxels
+  int *counters = nullptr;
+  int *ejected = nullptr;
+  int ejected_cap = 1;
+  const int *queue = nullptr;
+  int nqueue = 0;
+  int *next = nullptr;
+  float wx = 1, wy = 1, wz = 1, scale = 1, constant = 0, delta = 1;
+};
+
+void trace_all(Context &ctx, int cc_bytes, const TraceArgs &args);
+
+}  // namespace brook

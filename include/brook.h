@@ -1,0 +1,161 @@
+/* brook_amd C ABI: the skeletonization path of giorgioangel/brook's C API on one AMD Instinct
+ * MI355X (gfx950). Calls block; errors return a brook_status and brook_last_error() holds the
+ * text (thread-local, valid until the next call on that thread). Inputs are borrowed for the call;
+ * results are owned by their handles and views stay valid until the handle is destroyed.
+ *
+ * Each declaration names the reference interface it replaces (cpp/include/brook/brook.h:line).
+ * Entry points marked EXTENSION have no counterpart in the reference's C API. */
+#ifndef BROOK_AMD_BROOK_H
+#define BROOK_AMD_BROOK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BROOK_ABI_VERSION 1u
+
+typedef struct brook_context brook_context;
+typedef struct brook_array brook_array;
+typedef struct brook_components brook_components;
+typedef struct brook_skeletons brook_skeletons;
+
+/* brook.h:26-29 (BROOK_CUDA_ERROR there: a device (HIP) error here) */
+typedef enum brook_status {
+  BROOK_SUCCESS = 0, BROOK_INVALID_ARGUMENT = 1, BROOK_OUT_OF_MEMORY = 2,
+  BROOK_DEVICE_ERROR = 3, BROOK_INTERNAL_ERROR = 4
+} brook_status;
+
+/* brook.h:31-35 */
+typedef enum brook_dtype {
+  BROOK_U8 = 0, BROOK_U16 = 1, BROOK_U32 = 2, BROOK_U64 = 3,
+  BROOK_I8 = 4, BROOK_I16 = 5, BROOK_I32 = 6, BROOK_I64 = 7,
+  BROOK_F32 = 8, BROOK_F64 = 9
+} brook_dtype;
+
+/* brook.h:37 */
+typedef enum brook_memory { BROOK_HOST = 0, BROOK_DEVICE = 1 } brook_memory;
+
+/* brook.h:42-50: data points at logical (0,0,0); byte strides (any on the host; device input is
+ * gathered into Fortran order on the GPU when it is not Fortran-contiguous). Zero-initialize and
+ * set struct_size/abi_version. Device input must be on the context's device and ready. */
+typedef struct brook_volume {
+  size_t struct_size;
+  uint32_t abi_version;
+  const void *data;
+  int64_t shape[3];
+  int64_t strides[3];
+  brook_dtype dtype;
+  brook_memory memory;
+} brook_volume;
+
+/* brook.h:52-58 */
+typedef struct brook_teasar_parameters {
+  double scale, constant, pdrf_scale, pdrf_exponent;
+  double soma_detection_threshold, soma_acceptance_threshold;
+  double soma_invalidation_scale, soma_invalidation_const;
+  int64_t max_paths;
+  uint32_t has_max_paths;
+} brook_teasar_parameters;
+
+/* brook.h:60-67 */
+enum brook_skeletonize_flags {
+  BROOK_FIX_BRANCHING = 1u, BROOK_FIX_BORDERS = 2u, BROOK_FILTER_OBJECTS = 4u,
+  BROOK_DEVICE_OUTPUT = 8u, BROOK_FILL_HOLES = 16u, BROOK_FIX_AVOCADOS = 32u
+};
+
+/* brook.h:68-84 */
+typedef struct brook_skeletonize_options {
+  size_t struct_size;
+  uint32_t abi_version;
+  brook_teasar_parameters teasar;
+  float anisotropy[3];
+  double dust_threshold;
+  uint32_t flags;
+  const int64_t *object_ids;
+  size_t object_ids_count;
+  const int64_t *targets_before; /* (x,y,z) int64 triples, caller order */
+  size_t targets_before_count;
+  const int64_t *targets_after;
+  size_t targets_after_count;
+  const brook_volume *voxel_graph;
+} brook_skeletonize_options;
+
+/* brook.h:88-94: host packed result; offsets have skeleton_count+1 entries; edges local. */
+typedef struct brook_skeletons_view {
+  size_t skeleton_count, vertex_count, edge_count;
+  const int64_t *labels, *vertex_offsets, *edge_offsets;
+  const float *vertices; /* vertex_count x 3, physical coordinates */
+  const uint32_t *edges; /* edge_count x 2 */
+  const float *radii;
+  float anisotropy[3];
+} brook_skeletons_view;
+
+/* brook.h:147-150 */
+uint32_t brook_abi_version(void);
+const char *brook_version(void);
+const char *brook_last_error(void);
+
+/* brook.h:154-156: a negative or out-of-range device returns BROOK_INVALID_ARGUMENT. */
+brook_status brook_context_create(int device, brook_context **out);
+void brook_context_destroy(brook_context *context);
+brook_status brook_context_synchronize(brook_context *context);
+
+/* brook.h:159-162: device arrays owned by their handle. */
+brook_status brook_array_upload(brook_context *context, const brook_volume *input, brook_array **out);
+void brook_array_destroy(brook_array *array);
+brook_status brook_array_view(const brook_array *array, brook_volume *out);
+brook_status brook_array_copy_to_host(const brook_array *array, void *out, size_t bytes);
+
+/* brook.h:169-182: 26-connected components, ids 1..N by each component's first voxel in
+ * Fortran order, the smallest unsigned dtype that holds N; mapping has N+1 entries
+ * (mapping[id] = input value), roots N (the first voxel of each id). */
+brook_status brook_connected_components(brook_context *context, const brook_volume *labels, brook_components **out);
+void brook_components_destroy(brook_components *components);
+brook_status brook_components_view(const brook_components *components, brook_volume *labels,
+                                   const int64_t **mapping, const int64_t **roots, size_t *count);
+brook_status brook_components_copy_to_host(const brook_components *components, void *out, size_t bytes);
+brook_status brook_components_labels(const brook_components *components, brook_array **out);
+
+/* brook.h:183-184: multi-label anisotropic EDT (float32, Fortran order, background 0). */
+brook_status brook_edt(brook_context *context, const brook_volume *labels, const float anisotropy[3], int black_border,
+                       brook_array **out);
+
+/* brook.h:198-201 */
+void brook_skeletonize_options_init(brook_skeletonize_options *options);
+brook_status brook_skeletonize(brook_context *context, const brook_volume *labels,
+                               const brook_skeletonize_options *options, brook_skeletons **out);
+
+/* brook.h:230,252 */
+brook_status brook_skeletons_get_view(const brook_skeletons *result, brook_skeletons_view *out);
+void brook_skeletons_destroy(brook_skeletons *result);
+
+/* EXTENSION (the reference batches only in Python/C++, batch.hpp:20-22): skeletonize `count`
+ * volumes in one call; out receives `count` result handles, each equal to a separate
+ * brook_skeletonize of that volume. */
+brook_status brook_skeletonize_batch(brook_context *context, const brook_volume *const *volumes, size_t count,
+                                     const brook_skeletonize_options *options, brook_skeletons **out);
+
+/* EXTENSION: the preamble's fields for stage-level checks: DBF, DAF and PDRF (before the roots
+ * become rails) as n-voxel float32 host arrays (any may be NULL), and per-component tables of
+ * table_capacity entries (component id index; NULL to skip): traced (0/1), root, target (flat
+ * indices), dbf_max, max_daf. *count receives the component count. */
+brook_status brook_prepare_fields(brook_context *context, const brook_volume *labels,
+                                  const brook_skeletonize_options *options, float *dbf, float *daf, float *pdrf,
+                                  size_t table_capacity, uint8_t *traced, int64_t *root, int64_t *target,
+                                  float *dbf_max, float *max_daf, int64_t *count);
+
+/* EXTENSION: a timing/counter of the context's last call (e.g. "trace_ms", "paths"; with kernel
+ * timing on, "k_<kernel>_ms" and "k_<kernel>_n" from events on the context's stream). */
+brook_status brook_context_stat(const brook_context *context, const char *name, double *value);
+
+/* EXTENSION: time the engine's main kernels with HIP events on the context's stream (adds one
+ * host wait per timed kernel). */
+brook_status brook_context_set_kernel_timing(brook_context *context, int enable);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,212 @@
+"""GPU parity: the HIP path against the oracle and the reference's golden vectors, through the
+C ABI. Integer work (components, paths, vertex/edge sets) must be bit-exact; EDT values are
+float32 and must equal the oracle's restatement of the reference's rounding rule bit for bit,
+and lie within 1e-5 relative of the float64 EDT."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from tests.golden import shapes
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = json.load(open(os.path.join(HERE, "golden", "golden_digests.json")))
+CFG1_PARAMS = dict(scale=1.5, const=300, pdrf_scale=100000, pdrf_exponent=4)
+
+
+@pytest.fixture(scope="module")
+def brook():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import brook_amd
+    return brook_amd
+
+
+def _assert_packed_equal(got, want, what=""):
+    g, w = got.as_dict() if hasattr(got, "as_dict") else got, want
+    assert list(g["labels"]) == list(w["labels"]), what
+    for name in ("v_off", "e_off"):
+        assert np.array_equal(np.asarray(g[name]), np.asarray(w[name])), (what, name)
+    for name in ("vertices", "edges", "radii"):
+        assert np.array_equal(np.asarray(g[name]), np.asarray(w[name])), (what, name)
+
+
+@pytest.mark.parametrize("name", [c[0] for c in shapes.golden_cases()])
+def test_golden_digests(brook, name):
+    import torch
+    case = {c[0]: c for c in shapes.golden_cases()}[name]
+    options = dict(shapes.GOLDEN_OPTIONS, **case[2])
+    expected = {int(k): v for k, v in GOLDEN[name].items()}
+    for layout in (np.ascontiguousarray, np.asfortranarray):
+        packed = brook.skeletonize(layout(case[1]), output="packed", progress=False, **options)
+        assert shapes.summary_from_packed(packed.as_dict()) == expected, layout.__name__
+    dev = torch.from_numpy(np.ascontiguousarray(case[1])).cuda()  # C-ordered device input (gathered on the GPU)
+    packed = brook.skeletonize(dev, output="packed", progress=False, **options)
+    assert shapes.summary_from_packed(packed.as_dict()) == expected, "device"
+
+
+def test_ccl_known_answer(brook):
+    import ctypes
+    from brook_amd import _native as native
+    from brook_amd._intake import volume_struct
+    labels = np.array([1, 1, 0, 0, 0, 0, 2, 2], np.uint8)[:, None, None]
+    keep = []
+    vol, _ = volume_struct(labels, keep)
+    h = ctypes.c_void_p()
+    native.check(native.lib().brook_connected_components(native.context().handle, ctypes.byref(vol), ctypes.byref(h)))
+    mapping, roots, count = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_size_t()
+    native.check(native.lib().brook_components_view(h, None, ctypes.byref(mapping), ctypes.byref(roots), ctypes.byref(count)))
+    m = np.ctypeslib.as_array(ctypes.cast(mapping, ctypes.POINTER(ctypes.c_int64)), (count.value + 1,)).copy()
+    r = np.ctypeslib.as_array(ctypes.cast(roots, ctypes.POINTER(ctypes.c_int64)), (count.value,)).copy()
+    out = np.zeros(8, np.uint8)
+    native.check(native.lib().brook_components_copy_to_host(h, out.ctypes.data, 8))
+    native.lib().brook_components_destroy(h)
+    assert count.value == 2 and list(m) == [0, 1, 2] and list(r) == [0, 6]
+    assert out.tolist() == [1, 1, 0, 0, 0, 0, 2, 2]
+
+
+def _gpu_ccl(labels):
+    import ctypes
+    from brook_amd import _native as native
+    from brook_amd._intake import volume_struct
+    keep = []
+    vol, _ = volume_struct(labels, keep)
+    h = ctypes.c_void_p()
+    native.check(native.lib().brook_connected_components(native.context().handle, ctypes.byref(vol), ctypes.byref(h)))
+    view = native.Volume()
+    mapping, roots, count = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_size_t()
+    native.check(native.lib().brook_components_view(h, ctypes.byref(view), ctypes.byref(mapping), ctypes.byref(roots),
+                                                    ctypes.byref(count)))
+    dt = {0: np.uint8, 1: np.uint16, 2: np.uint32, 3: np.uint64}[view.dtype]
+    out = np.zeros(labels.shape, dt, order="F")
+    native.check(native.lib().brook_components_copy_to_host(h, out.ctypes.data, out.nbytes))
+    n = count.value
+    m = np.ctypeslib.as_array(ctypes.cast(mapping, ctypes.POINTER(ctypes.c_int64)), (n + 1,)).copy()
+    r = np.ctypeslib.as_array(ctypes.cast(roots, ctypes.POINTER(ctypes.c_int64)), (max(n, 1),))[:n].copy()
+    native.lib().brook_components_destroy(h)
+    return out, m, r
+
+
+@pytest.mark.parametrize("seed,shape,nl", [(0, (37, 29, 23), 3), (1, (64, 64, 64), 7), (2, (130, 70, 9), 300),
+                                           (3, (20, 1, 33), 2)])
+def test_ccl_matches_oracle(brook, seed, shape, nl):
+    rng = np.random.default_rng(seed)
+    labels = np.asfortranarray(rng.integers(0, nl + 1, size=shape).astype(np.uint32) * (rng.random(shape) < 0.7))
+    got, gm, gr = _gpu_ccl(labels)
+    cc, m, r = oracle.ccl(labels)
+    assert np.array_equal(got.astype(np.uint32), cc) and np.array_equal(gm, m) and np.array_equal(gr, r)
+    assert got.dtype == (np.uint8 if len(r) <= 255 else np.uint16 if len(r) <= 65535 else np.uint32)
+
+
+def test_ccl_synthetic_tubes(brook):
+    from brook_amd import synth
+    labels = synth.tubes_128()
+    got, gm, gr = _gpu_ccl(labels)
+    cc, m, r = oracle.ccl(labels)
+    assert np.array_equal(got.astype(np.uint32), cc) and np.array_equal(gm, m) and np.array_equal(gr, r)
+
+
+def _gpu_edt(labels, aniso, bb):
+    import ctypes
+    from brook_amd import _native as native
+    from brook_amd._intake import volume_struct
+    keep = []
+    vol, _ = volume_struct(labels, keep)
+    an = np.asarray(aniso, np.float32)
+    h = ctypes.c_void_p()
+    native.check(native.lib().brook_edt(native.context().handle, ctypes.byref(vol), an.ctypes.data, int(bb), ctypes.byref(h)))
+    out = np.zeros(labels.shape, np.float32, order="F")
+    native.check(native.lib().brook_array_copy_to_host(h, out.ctypes.data, out.nbytes))
+    native.lib().brook_array_destroy(h)
+    return out
+
+
+@pytest.mark.parametrize("seed,aniso,bb", [(0, (1, 1, 1), False), (1, (16, 16, 40), False), (2, (4.5, 7.91, 40), True),
+                                           (3, (8, 8, 40), False)])
+def test_edt_bit_exact_and_tolerance(brook, seed, aniso, bb):
+    rng = np.random.default_rng(seed)
+    shape = (70, 50, 40)
+    labels = np.zeros(shape, np.uint32)
+    x, y, z = np.ogrid[:shape[0], :shape[1], :shape[2]]
+    for lab in range(1, 9):
+        c = rng.integers(0, 50, 3)
+        r = rng.integers(3, 15)
+        labels[((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2 <= r * r) & (labels == 0)] = lab
+    labels = np.asfortranarray(labels)
+    got = _gpu_edt(labels, aniso, bb)
+    cc, _, _ = oracle.ccl(labels)
+    want = oracle.edt(cc, aniso, bb)
+    assert np.array_equal(got, want)
+    exact = oracle.edt_f64(cc, aniso, bb)
+    fg = labels > 0
+    rel = np.abs(got[fg].astype(np.float64) - exact[fg]) / exact[fg]
+    assert rel.max() <= 1e-5, (rel.max(), rel.mean())
+
+
+def test_edt_known_answer(brook):
+    labels = np.array([1, 1, 0, 0, 0, 0, 2, 2], np.uint8)[:, None, None]
+    assert _gpu_edt(labels, (1, 1, 1), False).ravel().tolist() == [2, 1, 0, 0, 0, 0, 1, 2]
+
+
+def test_fields_match_oracle(brook):
+    from brook_amd import synth
+    labels = synth.tubes_128()
+    aniso = (16, 16, 40)
+    got = brook.prepare_fields(labels, CFG1_PARAMS, aniso, 1000)
+    want = oracle.prepare(labels, CFG1_PARAMS, aniso, 1000)
+    for name in ("dbf", "daf", "pdrf"):
+        assert np.array_equal(got[name], want[name]), name
+    act = want["active"].astype(bool)
+    assert np.array_equal(got["active"], want["active"])
+    for name in ("root", "target", "dbf_max", "max_daf"):
+        assert np.array_equal(got[name][act], want[name][act]), name
+
+
+@pytest.mark.parametrize("seed", [1001, 1002])
+def test_skeletons_match_oracle_tubes(brook, seed):
+    from brook_amd import synth
+    labels = synth.tubes_128(seed)
+    aniso = (16, 16, 40)
+    got = brook.skeletonize(labels, CFG1_PARAMS, aniso, output="packed", progress=False)
+    want = oracle.skeletonize(labels, CFG1_PARAMS, aniso, 1000, threads=8)
+    _assert_packed_equal(got, want, f"seed {seed}")
+
+
+def test_batch_equals_separate(brook):
+    from brook_amd import synth
+    vols = [synth.volume("tubes", s, n, 70 + i) for i, (s, n) in enumerate([((64, 80, 72), 6), ((96, 64, 64), 9), ((64, 64, 100), 4)])]
+    opts = dict(teasar_params=CFG1_PARAMS, anisotropy=(16, 16, 40), dust_threshold=500)
+    batch = brook.skeletonize_batch(vols, output="packed", **opts)
+    assert len(batch) == 3
+    for i, v in enumerate(vols):
+        single = brook.skeletonize(v, output="packed", progress=False, **opts)
+        _assert_packed_equal(batch[i], single.as_dict(), f"sample {i}")
+
+
+def test_edge_cases(brook):
+    # empty, all background, volume at or below the dust threshold, a single voxel label
+    assert brook.skeletonize(np.zeros((0, 4, 4), np.uint32), progress=False) == {}
+    assert brook.skeletonize(np.zeros((20, 20, 20), np.uint32), progress=False, dust_threshold=0) == {}
+    assert brook.skeletonize(np.ones((5, 5, 5), np.uint32), progress=False) == {}
+    one = np.zeros((9, 9, 9), np.uint32)
+    one[4, 4, 4] = 3
+    assert brook.skeletonize(one, dust_threshold=0, progress=False) == {}
+    # one label filling the whole volume: black border
+    full = np.ones((24, 20, 18), np.uint16)
+    got = brook.skeletonize(full, dict(scale=1, const=2), dust_threshold=0, output="packed", progress=False)
+    want = oracle.skeletonize(full, dict(scale=1, const=2), dust_threshold=0)
+    _assert_packed_equal(got, want, "full volume")
+
+
+def test_unsupported_options_raise(brook):
+    labels = np.zeros((20, 20, 20), np.uint32)
+    labels[5:15, 5:15, 5:15] = 1
+    with pytest.raises(NotImplementedError):
+        brook.skeletonize(labels, fix_branching=False, dust_threshold=0, progress=False)
+    with pytest.raises(ValueError):
+        brook.skeletonize(labels, dict(pdrf_exponent=3), dust_threshold=0, progress=False)
