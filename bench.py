@@ -1,0 +1,181 @@
+"""Benchmark: Mvoxels/s skeletonized on one MI355X, device-resident input.
+
+A step is one `brook_amd.skeletonize` call (CCL, EDT, border targets, floods, PDRF, target order,
+trace, assembly; packed host result) on the 512^3 uint32 volume of 200 branching-tree labels
+(~40% occupancy, anisotropy (8,8,40), scale 1.5, const 300, pdrf_scale 1e5, pdrf_exponent 4,
+dust 1000, fix_borders) already resident in HBM as a Fortran-ordered tensor.
+
+Prints one JSON line (rank 0): throughput, p50/p99 step latency, the roofline of the dominant
+kernel (algorithmic bytes / event-timed kernel duration on the engine's own stream) and the
+CPU oracle timed on a bounded crop of the same volume on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mvoxels/s skeletonized (512^3, 200 labels); batched/served vs plain; p99 ms"
+PARAMS = dict(scale=1.5, const=300, pdrf_scale=100000, pdrf_exponent=4)
+ANISO = (8, 8, 40)
+HBM_PEAK_GBS = 8000.0
+
+
+def _device_volume(vol):
+    import torch
+    flat = torch.from_numpy(np.ravel(vol, order="F")).cuda()
+    sx, sy, sz = vol.shape
+    return flat.view(sz, sy, sx).permute(2, 1, 0)  # shape (sx, sy, sz), Fortran strides
+
+
+def _kernel_table(stats, n, b_in, b_cc, F):
+    """Per-kernel event time and algorithmic bytes per launch (per-voxel figures of the path's
+    stage table; trace and floods per reached/claimed voxel)."""
+    reached, claimed, flood = stats.get("railroad_reached_voxels", 0), stats.get("invalidated_voxels", 0), stats.get("flood_voxels", 0)
+    algo = {
+        "ccl": (["ccl_init", "ccl_union", "ccl_count", "ccl_mark", "ccl_relabel"], (b_in + b_cc) * n),
+        "edt": (["edt_x", "edt_y", "edt_z"], (3 * b_cc + 4 + 8 + 8) * n),
+        "prepare_labels": (["prepare_labels"], 12 * flood),
+        "pdrf_alive": (["pdrf_alive"], (4 + 4 + b_cc + 4 + 1) * n),
+        "trace_labels": (["trace_labels"], 12 * reached + 13 * claimed),
+    }
+    rows = {}
+    for name, (kernels, nbytes) in algo.items():
+        ms = sum(stats.get(f"k_{k}_ms", 0.0) for k in kernels)
+        launches = max(1.0, stats.get(f"k_{kernels[-1]}_n", 1.0))
+        if ms <= 0:
+            continue
+        achieved = nbytes / (ms / 1e3) / 1e9
+        rows[name] = dict(ms=ms, launches=launches, bytes=nbytes, achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS)
+    return rows
+
+
+def _traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary, if any."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        return json.load(open(path)).get(kernel, {}).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(vol, threads):
+    """The oracle (a C++ port of the reference's algorithm) on a 256^3 crop of the same volume."""
+    import oracle
+    crop = np.asfortranarray(vol[:256, :256, :256])
+    t0 = time.perf_counter()
+    oracle.skeletonize(crop, PARAMS, ANISO, 1000, threads=threads)
+    dt = time.perf_counter() - t0
+    return dict(value=crop.size / dt / 1e6, unit="Mvoxels/s", cores=threads, kind="port",
+                sample=f"oracle on the 256^3 corner crop of the 512^3 volume ({crop.size} voxels), "
+                       f"{threads} threads, {dt:.2f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the path would shard by volume and by label, see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import torch
+
+    import brook_amd
+    from brook_amd import _native as native
+    from brook_amd import synth
+
+    t0 = time.perf_counter()
+    vol = synth.trees_512()
+    gen_s = time.perf_counter() - t0
+    n = vol.size
+    dev = _device_volume(vol)
+    torch.cuda.synchronize()
+    ctx = native.context()
+    run = lambda: brook_amd.skeletonize(dev, PARAMS, ANISO, dust_threshold=1000, output="packed", progress=False)
+    for _ in range(args.warmup):
+        run()
+    native.check(native.lib().brook_context_set_kernel_timing(ctx.handle, 1))
+    names = ["ccl_init", "ccl_union", "ccl_count", "ccl_mark", "ccl_relabel", "edt_x", "edt_y", "edt_z",
+             "prepare_labels", "pdrf_alive", "trace_labels"]
+    stat_names = ["railroad_reached_voxels", "invalidated_voxels", "flood_voxels", "foreground_voxels", "cc_bytes",
+                  "paths", "components", "prep_ccl_ms", "prep_analyze_ms", "prep_edt_ms", "prep_borders_ms",
+                  "prep_floods_ms", "prep_pdrf_order_ms", "trace_ms", "assembly_ms"]
+    acc = {}
+    times = []
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        result = run()
+        times.append((time.perf_counter() - s0) * 1e3)
+        for k in names:
+            for suffix in ("_ms", "_n"):
+                acc[f"k_{k}{suffix}"] = acc.get(f"k_{k}{suffix}", 0.0) + ctx.stat(f"k_{k}{suffix}")
+        for k in stat_names:
+            acc[k] = acc.get(k, 0.0) + ctx.stat(k)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - start
+    native.check(native.lib().brook_context_set_kernel_timing(ctx.handle, 0))
+    K = args.steps
+    per = {k: v / K for k, v in acc.items()}
+    b_cc = int(per.get("cc_bytes", 2))
+    kernels = _kernel_table(per, n, 4, b_cc, per.get("foreground_voxels", 0))
+    dominant = max(kernels, key=lambda k: kernels[k]["ms"]) if kernels else None
+    roof = None
+    if dominant:
+        row = kernels[dominant]
+        roof = dict(bound="hbm", kernel=dominant, achieved=round(row["achieved_gbs"], 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(row["frac"], 5), traffic=_traffic(dominant),
+                    algorithmic_bytes_per_launch=int(row["bytes"]), ms_per_launch=round(row["ms"], 3))
+    out = dict(
+        metric=METRIC,
+        value=round(n * K / elapsed / 1e6, 3),
+        unit="Mvoxels/s",
+        n_gpus=1,
+        steps=K,
+        warmup=args.warmup,
+        ms_per_step=round(elapsed / K * 1e3, 3),
+        p50_ms=round(float(np.percentile(times, 50)), 3),
+        p99_ms=round(float(np.percentile(times, 99)), 3),
+        higher_is_better=True,
+        scaling="weak",
+        vs_baseline=None,
+        dtype="f32+int32",
+        data="synthetic",
+        config=dict(workload="trees_512: 512^3 uint32 volume, 200 branching-tree labels, ~40% occupancy, "
+                             "anisotropy (8,8,40), scale 1.5, const 300, pdrf_scale 1e5, pdrf_exponent 4, "
+                             "dust 1000, fix_borders; device-resident input, packed host output",
+                    voxels=int(n), skeletons=len(result), vertices=int(result.vertices.shape[0]),
+                    paths=int(per.get("paths", 0)), components=int(per.get("components", 0)),
+                    parallelism="1 GPU"),
+        stages_ms={k: round(per.get(k, 0.0), 3) for k in stat_names if k.endswith("_ms")},
+        kernels={k: dict(ms=round(v["ms"], 3), achieved_gbs=round(v["achieved_gbs"], 2), frac=round(v["frac"], 5))
+                 for k, v in kernels.items()},
+        roofline=roof,
+        generator_s=round(gen_s, 2),
+    )
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(vol, args.cpu_threads)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

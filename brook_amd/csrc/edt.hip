@@ -203,16 +203,18 @@ void run_edt(Context &ctx, const CC *cc, Dims dm, const float aniso[3], bool bb,
   for (size_t k = 1; k < table.size(); ++k) table[k] = table[k - 1] + aniso[0];
   Buffer<float> acc(ctx, table.size());
   acc.upload(ctx, table.data(), table.size());
-  KernelTimer x_timer(ctx, "edt_x");
-  if (dm.sx <= 128)
-    edt_x<CC, 2><<<(rows + 3) / 4, 256, 0, s>>>(cc, dm.sx, rows, acc.data(), bb, out);
-  else if (dm.sx <= 512)
-    edt_x<CC, 8><<<(rows + 3) / 4, 256, 0, s>>>(cc, dm.sx, rows, acc.data(), bb, out);
-  else if (dm.sx <= 2048)
-    edt_x<CC, 32><<<(rows + 3) / 4, 256, 0, s>>>(cc, dm.sx, rows, acc.data(), bb, out);
-  else
-    edt_x_serial<CC><<<(rows + 255) / 256, 256, 0, s>>>(cc, dm.sx, rows, acc.data(), bb, out);
-  BROOK_LAUNCHED();
+  {
+    KernelTimer x_timer(ctx, "edt_x");
+    if (dm.sx <= 128)
+      edt_x<CC, 2><<<(rows + 3) / 4, 256, 0, s>>>(cc, dm.sx, rows, acc.data(), bb, out);
+    else if (dm.sx <= 512)
+      edt_x<CC, 8><<<(rows + 3) / 4, 256, 0, s>>>(cc, dm.sx, rows, acc.data(), bb, out);
+    else if (dm.sx <= 2048)
+      edt_x<CC, 32><<<(rows + 3) / 4, 256, 0, s>>>(cc, dm.sx, rows, acc.data(), bb, out);
+    else
+      edt_x_serial<CC><<<(rows + 255) / 256, 256, 0, s>>>(cc, dm.sx, rows, acc.data(), bb, out);
+    BROOK_LAUNCHED();
+  }
   const int last = dims_used - 1;
   if (last == 0) {
     long long n = (long long)dm.sx * dm.sy * dm.sz;

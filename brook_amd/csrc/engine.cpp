@@ -118,6 +118,12 @@ Packed skeletonize(Context &ctx, const LabelVolume &in, const SkeletonizeOptions
   for (size_t id = 1; id < k; ++id) first[id] = cc.roots[id - 1];
   for (int id : labels)
     if (!borders[id].empty()) root[id] = flat_of(borders[id].back(), d);
+  int64_t flood_voxels = 0;  // voxels swept by the root-search and DAF floods
+  for (int id : labels) flood_voxels += boxes[id].count * (root[id] < 0 ? 2 : 1);
+  ctx.stats["flood_voxels"] = double(flood_voxels);
+  ctx.stats["foreground_voxels"] = double(F);
+  ctx.stats["cc_bytes"] = cc.cc_bytes;
+  ctx.stats["components"] = double(count);
   std::vector<int> queue = labels;
   std::stable_sort(queue.begin(), queue.end(), [&](int a, int b) { return boxes[a].count > boxes[b].count; });
   Buffer<long long> first_dev(ctx, k), root_dev(ctx, k), target_dev(ctx, k);
