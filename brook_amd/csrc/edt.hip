@@ -97,89 +97,127 @@ __global__ void __launch_bounds__(256) edt_x_serial(const CC *cc, int sx, int ro
   }
 }
 
-// Lower envelope along y (axis 1) or z (axis 2), one thread per line. Scratch entry k of thread t
-// lives at [k * pitch + t] so the stack pushes of a wave land on neighbouring addresses.
+// Lower envelope along y (axis 1) or z (axis 2), one thread per line, neighbouring threads on
+// neighbouring x. The line is read in chunks of kChunk voxels (all loads of a chunk in flight at
+// once, staged in the thread's own LDS slots). The envelope stack (position, height) lives in LDS
+// for its first kStack entries and spills to a global column for deeper stacks; the float64
+// intersection Z[j] of entries j-1 and j is recomputed from them when needed -- the same
+// expression on the same operands as when the entry was pushed, so the same bits.
+constexpr int kStack = 20, kChunk = 8, kFhBlock = 256;
+
 template <class CC>
-__global__ void __launch_bounds__(256) edt_fh(const CC *cc, Dims dm, int axis, float w, int bb, int line0, int nlines,
-                                              int pitch, int *Vs, float *Hs, double *Zs, float *D, int finish,
-                                              unsigned *label_max) {
-  int t = blockIdx.x * 256 + threadIdx.x;
+__global__ void __launch_bounds__(kFhBlock) edt_fh(const CC *cc, Dims dm, int axis, float w, int bb, int nlines,
+                                                   long long spill_pitch, int *spill_v, float *spill_h, float *D,
+                                                   int finish, unsigned *label_max) {
+  __shared__ int sV[kStack * kFhBlock];
+  __shared__ float sH[kStack * kFhBlock];
+  __shared__ CC sL[kChunk * kFhBlock];
+  __shared__ float sD[kChunk * kFhBlock];
+  const int tid = threadIdx.x;
+  const int t = blockIdx.x * kFhBlock + tid;
   if (t >= nlines) return;
-  int line = line0 + t;
   long long base, stride;
   int n;
   if (axis == 1) {
-    int x = line % dm.sx, z = line / dm.sx;
+    int x = t % dm.sx, z = t / dm.sx;
     base = x + (long long)dm.sx * dm.sy * z;
     n = dm.sy;
     stride = dm.sx;
   } else {
-    base = line;
+    base = t;
     n = dm.sz;
     stride = (long long)dm.sx * dm.sy;
   }
   const double w2 = (double)(w * w);
-  int *V = Vs + t;
-  float *H = Hs + t;
-  double *Z = Zs + t;
-  int i = 0;
-  while (i < n) {
-    CC l = cc[base + i * stride];
-    if (l == CC(0)) {
-      if (finish) D[base + i * stride] = 0.0f;
-      ++i;
-      continue;
-    }
-    const int c = i;
-    int k = -1;
-    if (c > 0 || bb) { k = 0; V[0] = c - 1; H[0] = 0.0f; Z[0] = -1e300; }
-    auto place = [&](int pp, double ph) {
-      bool placed = false;
-      while (k >= 0 && !placed) {
-        long long o = (long long)k * pitch;
-        double vk = (double)V[o];
-        double s = ((ph + w2 * (double)pp * (double)pp) - ((double)H[o] + w2 * vk * vk)) / (2.0 * w2 * ((double)pp - vk));
-        if (s <= Z[o]) {
-          k--;
-        } else {
-          k++;
-          long long q = (long long)k * pitch;
-          V[q] = pp; H[q] = (float)ph; Z[q] = s;
-          placed = true;
-        }
+  auto getV = [&](int k) -> int { return k < kStack ? sV[k * kFhBlock + tid] : spill_v[(k - kStack) * spill_pitch + t]; };
+  auto getH = [&](int k) -> float { return k < kStack ? sH[k * kFhBlock + tid] : spill_h[(k - kStack) * spill_pitch + t]; };
+  auto set = [&](int k, int v, float h) {
+    if (k < kStack) { sV[k * kFhBlock + tid] = v; sH[k * kFhBlock + tid] = h; }
+    else { spill_v[(k - kStack) * spill_pitch + t] = v; spill_h[(k - kStack) * spill_pitch + t] = h; }
+  };
+  // intersection of entries j-1 and j (j >= 1), as computed when entry j was placed
+  auto inter = [&](int j) -> double {
+    double v1 = (double)getV(j - 1), h1 = (double)getH(j - 1), v2 = (double)getV(j), h2 = (double)getH(j);
+    return ((h2 + w2 * v2 * v2) - (h1 + w2 * v1 * v1)) / (2.0 * w2 * (v2 - v1));
+  };
+  int k = -1, c = 0;
+  bool active = false;
+  CC cur = CC(0);
+  auto place = [&](int pp, double ph) {
+    bool placed = false;
+    while (k >= 0 && !placed) {
+      double vk = (double)getV(k), hk = (double)getH(k);
+      double s = ((ph + w2 * (double)pp * (double)pp) - (hk + w2 * vk * vk)) / (2.0 * w2 * ((double)pp - vk));
+      double zk = k == 0 ? -1e300 : inter(k);
+      if (s <= zk) {
+        k--;
+      } else {
+        k++;
+        set(k, pp, (float)ph);
+        placed = true;
       }
-      if (!placed) { k = 0; V[0] = pp; H[0] = (float)ph; Z[0] = -1e300; }
-    };
-    int d = c;
-    for (;;) {
-      float hf = D[base + d * stride];
-      if (hf < kInf) place(d, (double)hf);
-      if (d + 1 < n && cc[base + (d + 1) * stride] == l) ++d;
-      else break;
     }
-    if (d < n - 1 || bb) place(d + 1, 0.0);
+    if (!placed) { k = 0; set(0, pp, (float)ph); }
+  };
+  auto close_run = [&](int d, bool right_wall) {
+    if (right_wall) place(d + 1, 0.0);
     float run_max = 0.0f;
     if (k < 0) {
       for (int q = c; q <= d; ++q) D[base + q * stride] = kInf;
       run_max = kInf;
     } else {
       int kk = 0;
+      double znext = k >= 1 ? inter(1) : 0.0;
+      int vkk = getV(0);
+      float hkk = getH(0);
       for (int q = c; q <= d; ++q) {
-        while (kk < k && Z[(long long)(kk + 1) * pitch] < (double)q) kk++;
-        long long o = (long long)kk * pitch;
-        double dq = (double)q - (double)V[o];
-        float v = (float)(w2 * dq * dq + (double)H[o]);
+        while (kk < k && znext < (double)q) {
+          kk++;
+          vkk = getV(kk);
+          hkk = getH(kk);
+          if (kk < k) znext = inter(kk + 1);
+        }
+        double dq = (double)q - (double)vkk;
+        float v = (float)(w2 * dq * dq + (double)hkk);
         if (finish) { v = sqrtf(v); run_max = fmaxf(run_max, v); }
         D[base + q * stride] = v;
       }
     }
     if (finish && label_max && run_max < kInf) {
       unsigned bits = __float_as_uint(run_max);
-      unsigned *slot = label_max + (unsigned long long)l;
+      unsigned *slot = label_max + (unsigned long long)cur;
       if (ld_u(slot) < bits) atomicMax(slot, bits);
     }
-    i = d + 1;
+  };
+  for (int y0 = 0; y0 < n; y0 += kChunk) {
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) {
+      int y = y0 + j;
+      CC l = y < n ? cc[base + y * stride] : CC(0);
+      sL[j * kFhBlock + tid] = l;
+      sD[j * kFhBlock + tid] = (y < n && l != CC(0)) ? D[base + y * stride] : kInf;
+    }
+    const int m = min(kChunk, n - y0);
+    for (int j = 0; j < m; ++j) {
+      const int y = y0 + j;
+      const CC l = sL[j * kFhBlock + tid];
+      if (active && l != cur) { close_run(y - 1, true); active = false; }
+      if (l == CC(0)) {
+        if (finish) D[base + y * stride] = 0.0f;
+        continue;
+      }
+      if (!active) {
+        active = true;
+        cur = l;
+        c = y;
+        k = -1;
+        if (y > 0 || bb) { k = 0; set(0, y - 1, 0.0f); }
+      }
+      const float hf = sD[j * kFhBlock + tid];
+      if (hf < kInf) place(y, (double)hf);
+    }
   }
+  if (active) close_run(n - 1, bb != 0);
 }
 
 // finish for a single pass (x only): background 0, else sqrt.
@@ -222,24 +260,17 @@ void run_edt(Context &ctx, const CC *cc, Dims dm, const float aniso[3], bool bb,
     BROOK_LAUNCHED();
     return;
   }
-  // scratch for the envelopes: lines are launched in batches under a fixed budget
-  const size_t budget = size_t(512) << 20;
+  // envelope stacks deeper than the LDS part spill to one global column per line
   for (int axis = 1; axis <= last; ++axis) {
     int n = axis == 1 ? dm.sy : dm.sz;
     int nlines = axis == 1 ? dm.sx * dm.sz : dm.sx * dm.sy;
-    size_t per_line = size_t(n + 3) * 16;
-    int batch = int(std::max<size_t>(256, std::min<size_t>(size_t(nlines), budget / per_line)));
-    batch = (batch + 255) / 256 * 256;
-    Buffer<int> V(ctx, size_t(batch) * (n + 3));
-    Buffer<float> H(ctx, size_t(batch) * (n + 3));
-    Buffer<double> Z(ctx, size_t(batch) * (n + 3));
+    size_t spill_rows = size_t(std::max(1, n + 3 - kStack));
+    Buffer<int> V(ctx, spill_rows * nlines);
+    Buffer<float> H(ctx, spill_rows * nlines);
     KernelTimer fh_timer(ctx, axis == 1 ? "edt_y" : "edt_z");
-    for (int l0 = 0; l0 < nlines; l0 += batch) {
-      int count = std::min(batch, nlines - l0);
-      edt_fh<CC><<<(count + 255) / 256, 256, 0, s>>>(cc, dm, axis, aniso[axis], bb, l0, count, batch, V.data(), H.data(),
-                                                     Z.data(), out, axis == last, label_max);
-      BROOK_LAUNCHED();
-    }
+    edt_fh<CC><<<(nlines + kFhBlock - 1) / kFhBlock, kFhBlock, 0, s>>>(cc, dm, axis, aniso[axis], bb, nlines, nlines,
+                                                                      V.data(), H.data(), out, axis == last, label_max);
+    BROOK_LAUNCHED();
   }
 }
 

@@ -257,6 +257,9 @@ Packed skeletonize(Context &ctx, const LabelVolume &in, const SkeletonizeOptions
   ta.rec_start = rec_start.data();
   ta.rec_cap = rec_cap;
   ta.counters = counters.data();
+  Buffer<unsigned long long> label_stats(ctx, 8 * k);
+  label_stats.clear(ctx);
+  ta.label_stats = label_stats.data();
   ta.ejected = ejected.data();
   ta.ejected_cap = 64;
   ta.queue = queue_dev.data();
@@ -280,6 +283,31 @@ Packed skeletonize(Context &ctx, const LabelVolume &in, const SkeletonizeOptions
   std::memcpy(&claimed, &cnt[6], 8);
   ctx.stats["railroad_reached_voxels"] = double(reached);
   ctx.stats["invalidated_voxels"] = double(claimed);
+  {  // per-label trace profile (device wall clock)
+    int khz = 0;
+    BROOK_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx.device()));
+    const double tick_ms = khz > 0 ? 1.0 / khz : 0.0;
+    std::vector<unsigned long long> ls = label_stats.download(ctx);
+    double sums[8] = {0}, worst = 0;
+    size_t worst_id = 0;
+    for (int id : labels) {
+      for (int j = 0; j < 8; ++j) sums[j] += double(ls[8 * size_t(id) + j]);
+      if (double(ls[8 * size_t(id)]) > worst) { worst = double(ls[8 * size_t(id)]); worst_id = size_t(id); }
+    }
+    const double nl = std::max<double>(1.0, double(labels.size())), paths = std::max(1.0, sums[7]);
+    ctx.stats["trace_label_max_ms"] = worst * tick_ms;
+    ctx.stats["trace_label_mean_ms"] = sums[0] * tick_ms / nl;
+    const char *phase[5] = {"total", "target", "railroad", "backtrace", "invalidate"};
+    for (int j = 1; j < 5; ++j) ctx.stats[std::string("trace_") + phase[j] + "_ms_per_label"] = sums[j] * tick_ms / nl;
+    ctx.stats["railroad_rounds_per_path"] = sums[5] / paths;
+    ctx.stats["invalidate_rounds_per_path"] = sums[6] / paths;
+    for (int j = 1; j < 5; ++j)
+      ctx.stats[std::string("slowest_") + phase[j] + "_ms"] = double(ls[8 * worst_id + j]) * tick_ms;
+    ctx.stats["slowest_paths"] = double(ls[8 * worst_id + 7]);
+    ctx.stats["slowest_voxels"] = double(boxes[worst_id].count);
+    ctx.stats["slowest_railroad_rounds"] = double(ls[8 * worst_id + 5]);
+    ctx.stats["slowest_invalidate_rounds"] = double(ls[8 * worst_id + 6]);
+  }
   ctx.stats["labels_traced"] = double(labels.size());
 
   // ---- assembly: groups = input labels in first-skeleton-component order -----------------------

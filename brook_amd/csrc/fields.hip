@@ -66,15 +66,35 @@ __device__ void flood_label(const CC *cc, Dims d, CC L, int src, const float *W,
       float du = ld_f(dist + u);
       int x, y, z;
       unravel(u, d, x, y, z);
+      // all 26 neighbours' loads, then all their atomics, in flight together
+      int v[26];
+      bool ok[26];
+#pragma unroll
       for (int k = 0; k < 26; ++k) {
         int nx = x + kNX[k], ny = y + kNY[k], nz = z + kNZ[k];
-        if (!inside(nx, ny, nz, d)) continue;
-        int v = flat(nx, ny, nz, d);
-        if (cc[v] != L) continue;
-        float cand = du + W[k];
-        float old = atomic_min_pos(dist + v, cand);
-        if (old > cand && atomicExch(inq + v, 1) == 0) nxt[atomicAdd(&counts[1], 1)] = v;
+        ok[k] = inside(nx, ny, nz, d);
+        v[k] = ok[k] ? flat(nx, ny, nz, d) : u;
       }
+      float seen[26];
+#pragma unroll
+      for (int k = 0; k < 26; ++k) {
+        ok[k] = ok[k] && cc[v[k]] == L;
+        seen[k] = ld_f(dist + v[k]);
+      }
+      bool better[26];
+#pragma unroll
+      for (int k = 0; k < 26; ++k) {
+        float cand = du + W[k];
+        better[k] = false;
+        if (ok[k] && cand < seen[k])
+          better[k] = __int_as_float(atomicMin(reinterpret_cast<int *>(dist + v[k]), __float_as_int(cand))) > cand;
+      }
+      bool fresh[26];
+#pragma unroll
+      for (int k = 0; k < 26; ++k) fresh[k] = better[k] && atomicExch(inq + v[k], 1) == 0;
+#pragma unroll
+      for (int k = 0; k < 26; ++k)
+        if (fresh[k]) nxt[atomicAdd(&counts[1], 1)] = v[k];
     }
     __syncthreads();
     if (threadIdx.x == 0) { counts[0] = counts[1]; counts[1] = 0; }

@@ -54,7 +54,19 @@ struct Shared {
   int store_pos;
   float lane_du[kLaneFront];
   int lane_u[kLaneFront];
+  // per-label profile: [0] total [1] target [2] railroad [3] backtrace [4] invalidation (wall-clock
+  // ticks), [5] railroad rounds, [6] invalidation rounds, [7] paths
+  unsigned long long prof[8];
+  long long tick;
 };
+
+__device__ __forceinline__ void lap(Shared &sh, int slot) {
+  if (threadIdx.x == 0) {
+    long long now = wall_clock64();
+    sh.prof[slot] += (unsigned long long)(now - sh.tick);
+    sh.tick = now;
+  }
+}
 
 // ---- railroad -------------------------------------------------------------------------------
 template <class CC>
@@ -79,6 +91,58 @@ __device__ __forceinline__ void relax(const Label<CC> &lb, Shared &sh, int u, in
   } else {
     atomicMin(&sh.minfar, ((unsigned long long)__float_as_uint(cand) << 32) | __float_as_uint(fv));
     if (!(atomicOr(a.state + v, kDF) & kDF)) far[atomicAdd(&sh.n_far, 1)] = v;
+  }
+}
+
+// relax() for all 26 neighbours of u at once: every load, then every atomic, is issued before
+// any result is waited for (one voxel costs a few memory round trips instead of 26 chains).
+template <class CC>
+__device__ __forceinline__ void relax_all(const Label<CC> &lb, Shared &sh, int u, float du, float bound, int *nxt,
+                                          int *far, int *touched) {
+  const TraceArgs &a = lb.a;
+  int x, y, z;
+  unravel(u, a.d, x, y, z);
+  int v[26];
+  bool ok[26];
+#pragma unroll
+  for (int k = 0; k < 26; ++k) {
+    int nx = x + kNX[k], ny = y + kNY[k], nz = z + kNZ[k];
+    ok[k] = inside(nx, ny, nz, a.d);
+    v[k] = ok[k] ? flat(nx, ny, nz, a.d) : u;
+  }
+  float fv[26], seen[26];
+#pragma unroll
+  for (int k = 0; k < 26; ++k) {
+    ok[k] = ok[k] && lb.cc[v[k]] == lb.L;
+    fv[k] = ld_f(a.pdrf + v[k]);
+    seen[k] = ld_f(a.dist + v[k]);
+  }
+  float cand[26], old[26];
+#pragma unroll
+  for (int k = 0; k < 26; ++k) {
+    cand[k] = vw_step(du, fv[k]);
+    old[k] = cand[k];
+    if (ok[k] && cand[k] < seen[k])
+      old[k] = __int_as_float(atomicMin(reinterpret_cast<int *>(a.dist + v[k]), __float_as_int(cand[k])));
+  }
+  int was[26];
+#pragma unroll
+  for (int k = 0; k < 26; ++k) {
+    was[k] = -1;
+    if (old[k] > cand[k] && fv[k] != 0.0f) was[k] = atomicOr(a.state + v[k], cand[k] <= bound ? kDQ : kDF);
+  }
+#pragma unroll
+  for (int k = 0; k < 26; ++k) {
+    if (!(old[k] > cand[k])) continue;
+    if (old[k] >= kInf) touched[atomicAdd(&sh.n_touched, 1)] = v[k];
+    if (fv[k] == 0.0f) {  // a rail: fixes the cost and the last road voxel; never expanded
+      atomicMin(&sh.rail_best, ((unsigned long long)__float_as_uint(cand[k]) << 32) | (unsigned)v[k]);
+    } else if (cand[k] <= bound) {
+      if (!(was[k] & kDQ)) nxt[atomicAdd(&sh.n_nxt, 1)] = v[k];
+    } else {
+      atomicMin(&sh.minfar, ((unsigned long long)__float_as_uint(cand[k]) << 32) | __float_as_uint(fv[k]));
+      if (!(was[k] & kDF)) far[atomicAdd(&sh.n_far, 1)] = v[k];
+    }
   }
 }
 
@@ -141,16 +205,14 @@ __device__ int railroad(const Label<CC> &lb, Shared &sh, int t, int *near_a, int
       } else {
         for (int i = threadIdx.x; i < nc; i += blockDim.x) {
           int u = ld_i(cur + i);
-          atomicAnd(a.state + u, ~kDQ);
+          atomicAnd(a.state + u, ~kDQ);  // clear, THEN read the distance
           float du = ld_f(a.dist + u);
           if (ld_f(a.pdrf + u) == 0.0f || du > bound) continue;
-          int x, y, z;
-          unravel(u, a.d, x, y, z);
-          for (int k = 0; k < 26; ++k) relax(lb, sh, u, x, y, z, k, du, bound, nxt, fcur, touched);
+          relax_all(lb, sh, u, du, bound, nxt, fcur, touched);
         }
       }
       __syncthreads();
-      if (threadIdx.x == 0) { sh.n_cur = sh.n_nxt; sh.n_nxt = 0; }
+      if (threadIdx.x == 0) { sh.n_cur = sh.n_nxt; sh.n_nxt = 0; sh.prof[5] += 1; }
       int *tmp = cur; cur = nxt; nxt = tmp;
       __syncthreads();
     }
@@ -284,6 +346,67 @@ __device__ __forceinline__ void offer(const Label<CC> &lb, Shared &sh, int x, in
   }
 }
 
+// offer() to all 26 neighbours of u at once (loads, then claims, then queue flags in flight together).
+template <class CC>
+__device__ __forceinline__ void offer_all(const Label<CC> &lb, Shared &sh, int u, unsigned su, unsigned mu, float maxd,
+                                          unsigned hi, int *nxt, int *far, int *touched) {
+  const TraceArgs &a = lb.a;
+  int x, y, z, qx, qy, qz;
+  unravel(u, a.d, x, y, z);
+  unravel(int(su), a.d, qx, qy, qz);
+  int v[26];
+  bool ok[26];
+#pragma unroll
+  for (int k = 0; k < 26; ++k) {
+    int nx = x + kNX[k], ny = y + kNY[k], nz = z + kNZ[k];
+    ok[k] = inside(nx, ny, nz, a.d);
+    v[k] = ok[k] ? flat(nx, ny, nz, a.d) : u;
+  }
+  unsigned char al[26];
+  unsigned long long kv[26];
+#pragma unroll
+  for (int k = 0; k < 26; ++k) {
+    ok[k] = ok[k] && lb.cc[v[k]] == lb.L;
+    al[k] = ld_u8(a.alive + v[k]);
+    kv[k] = ld_u64(a.key + v[k]);
+  }
+  unsigned mb[26];
+#pragma unroll
+  for (int k = 0; k < 26; ++k) {
+    float dx = a.wx * (float)(x + kNX[k] - qx), dy = a.wy * (float)(y + kNY[k] - qy), dz = a.wz * (float)(z + kNZ[k] - qz);
+    float d = sqrtf(dx * dx + dy * dy + dz * dz);
+    ok[k] = ok[k] && al[k] && d < maxd;
+    mb[k] = max(__float_as_uint(d), mu);  // M = max(M[u], d)
+  }
+  unsigned long long old[26];
+#pragma unroll
+  for (int k = 0; k < 26; ++k) {
+    unsigned long long cand = ((unsigned long long)mb[k] << 32) | su;
+    old[k] = cand;
+    if (ok[k] && mb[k] < hi && kv[k] > cand) old[k] = atomicMin(a.key + v[k], cand);
+  }
+#pragma unroll
+  for (int k = 0; k < 26; ++k) {
+    if (!ok[k]) continue;
+    unsigned long long cand = ((unsigned long long)mb[k] << 32) | su;
+    if (mb[k] < hi) {  // inside the band: claim now
+      if (old[k] > cand) {
+        if (old[k] == kMaxKey) touched[atomicAdd(&sh.n_touched, 1)] = v[k];
+        int was = atomicOr(a.state + v[k], kIQ);
+        if (was & kIW) atomicAnd(a.state + v[k], ~kIW);
+        if (!(was & kIQ)) nxt[atomicAdd(&sh.n_nxt, 1)] = v[k];
+      }
+    } else if (ld_u64(a.key + v[k]) == kMaxKey) {  // beyond it: wait for its band
+      int seen = ld_i(a.state + v[k]);
+      while (!(seen & (kIQ | kIW))) {
+        int got = atomicCAS(a.state + v[k], seen, seen | kIW);
+        if (got == seen) { far[atomicAdd(&sh.n_far, 1)] = v[k]; break; }
+        seen = got;
+      }
+    }
+  }
+}
+
 __device__ __forceinline__ int band_of(unsigned mbits, float delta, int kmin) {
   float m = __uint_as_float(mbits);
   int k = (int)(m / delta);
@@ -330,12 +453,16 @@ __device__ void invalidate(const Label<CC> &lb, Shared &sh, const int *path, int
         unsigned long long ku = snap[i];
         unsigned su = (unsigned)ku, mu = (unsigned)(ku >> 32);
         float maxd = a.scale * a.dbf[su] + a.constant;
+        if (!lanes) {
+          offer_all(lb, sh, u, su, mu, maxd, hi, nxt, far, touched);
+          continue;
+        }
         int x, y, z;
         unravel(u, a.d, x, y, z);
-        for (int k = k0; k < k1; ++k) offer(lb, sh, x, y, z, k, su, mu, maxd, hi, nxt, far, touched);
+        offer(lb, sh, x, y, z, k0, su, mu, maxd, hi, nxt, far, touched);
       }
       __syncthreads();
-      if (threadIdx.x == 0) { sh.n_cur = sh.n_nxt; sh.n_nxt = 0; }
+      if (threadIdx.x == 0) { sh.n_cur = sh.n_nxt; sh.n_nxt = 0; sh.prof[6] += 1; }
       int *tmp = cur; cur = nxt; nxt = tmp;
       __syncthreads();
     }
@@ -464,14 +591,20 @@ __global__ void __launch_bounds__(kBlock) trace_labels(TraceArgs a) {
       sh.cursor = lb.seg;
       sh.seg_end = (long long)lb.seg + lb.count;
       sh.pcount = 0;
+      for (int j = 0; j < 8; ++j) sh.prof[j] = 0;
+      sh.tick = wall_clock64();
     }
+    const long long label_begin = wall_clock64();
     __syncthreads();
     for (;;) {
       const int t = next_target(lb, sh);
+      lap(sh, 1);
       if (t < 0) break;
       const int status = railroad(lb, sh, t, l0, l1, l2, l3, l4);
+      lap(sh, 2);
       if (status == 1) backtrace(lb, sh, t, walk);
       __syncthreads();
+      lap(sh, 3);
       const int nt = sh.n_touched;
       for (int j = threadIdx.x; j < nt; j += blockDim.x) {  // restore the railroad state
         int v = l4[j];
@@ -507,8 +640,13 @@ __global__ void __launch_bounds__(kBlock) trace_labels(TraceArgs a) {
       }
       __syncthreads();
       invalidate(lb, sh, walk, len, l0, l1, l2, l3, l4, s0, s1);
-      if (threadIdx.x == 0) sh.pcount += 1;
+      lap(sh, 4);
+      if (threadIdx.x == 0) { sh.pcount += 1; sh.prof[7] += 1; }
       __syncthreads();
+    }
+    if (threadIdx.x == 0 && a.label_stats) {
+      sh.prof[0] = (unsigned long long)(wall_clock64() - label_begin);
+      for (int j = 0; j < 8; ++j) a.label_stats[8LL * L + j] = sh.prof[j];
     }
   }
 }

@@ -25,6 +25,9 @@ struct TraceArgs {
   // [0] store used [1] records [2] ejected labels [3] overflow; [4..5] u64 railroad-reached voxels,
   // [6..7] u64 invalidated voxels
   int *counters = nullptr;
+  // per label (8 u64): wall-clock ticks total / target / railroad / backtrace / invalidation,
+  // railroad rounds, invalidation rounds, paths (null: not recorded)
+  unsigned long long *label_stats = nullptr;
   int *ejected = nullptr;
   int ejected_cap = 1;
   const int *queue = nullptr;
