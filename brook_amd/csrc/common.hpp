@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace brook {
 
@@ -15,6 +16,11 @@ __constant__ static const int kNZ[26] = {0, 0, 0, 0, -1, 1, 0, 0, 0, 0, -1, 1, -
 __constant__ static const int kD3Rank[26] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 21, 20, 24, 19, 23, 22, 25};
 __constant__ static const int kD3Inv[26] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 22, 20, 19, 24, 23, 21, 25};
 __constant__ static const int kOpp[26] = {1, 0, 3, 2, 5, 4, 9, 8, 7, 6, 13, 12, 11, 10, 17, 16, 15, 14, 25, 24, 23, 22, 21, 20, 19, 18};
+
+// The same table as compile-time constants, for fully unrolled neighbour loops.
+constexpr int kDX[26] = {-1, 1, 0, 0, 0, 0, -1, -1, 1, 1, 0, 0, 0, 0, -1, -1, 1, 1, -1, -1, -1, -1, 1, 1, 1, 1};
+constexpr int kDY[26] = {0, 0, -1, 1, 0, 0, -1, 1, -1, 1, -1, -1, 1, 1, 0, 0, 0, 0, -1, -1, 1, 1, -1, -1, 1, 1};
+constexpr int kDZ[26] = {0, 0, 0, 0, -1, 1, 0, 0, 0, 0, -1, 1, -1, 1, -1, 1, -1, 1, -1, 1, -1, 1, -1, 1, -1, 1};
 
 constexpr float kInf = __builtin_huge_valf();
 constexpr unsigned long long kMaxKey = ~0ULL;
@@ -34,6 +40,20 @@ __device__ __forceinline__ bool inside(int x, int y, int z, const Dims &d) {
   return (unsigned)x < (unsigned)d.sx && (unsigned)y < (unsigned)d.sy && (unsigned)z < (unsigned)d.sz;
 }
 __device__ __forceinline__ int flat(int x, int y, int z, const Dims &d) { return x + d.sx * (y + d.sy * z); }
+// Flat offset of neighbour k (only meaningful where its mask bit says it is inside the volume).
+template <int K> __device__ __forceinline__ int nb_offset(const Dims &d) {
+  return kDX[K] + d.sx * (kDY[K] + d.sy * kDZ[K]);
+}
+__device__ __forceinline__ int nb_offset(int k, const Dims &d) { return kNX[k] + d.sx * (kNY[k] + d.sy * kNZ[k]); }
+
+// Calls f(integral_constant k) for k = B..E-1 with k known at compile time.
+template <int B, int E, class F> __device__ __forceinline__ void for_nb(F &&f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    for_nb<B + 1, E>(f);
+  }
+}
+template <class F> __device__ __forceinline__ void for_each_nb(F &&f) { for_nb<0, 26>(f); }
 
 // Loads of memory that other threads change inside the same kernel: relaxed, agent scope, so the
 // access is served by L2 (the CU's vector L1 may hold a stale line otherwise).

@@ -143,6 +143,8 @@ Packed skeletonize(Context &ctx, const LabelVolume &in, const SkeletonizeOptions
   Buffer<int> state(ctx, size_t(n));
   fill_value(ctx, daf.data(), n, kInf);
   state.clear(ctx);
+  Buffer<unsigned> nbr(ctx, size_t(n));
+  neighbour_masks(ctx, cc.labels.data(), cc.cc_bytes, d, nbr.data());
   std::vector<Buffer<int>> lists;
   for (int i = 0; i < 6; ++i) lists.emplace_back(ctx, size_t(std::max<int64_t>(F, 1)));
   LabelTables tables;
@@ -154,6 +156,7 @@ Packed skeletonize(Context &ctx, const LabelVolume &in, const SkeletonizeOptions
   tables.target = target_dev.data();
   tables.max_daf = max_daf_dev.data();
   tables.weights = weights.data();
+  tables.nbr = nbr.data();
   prepare_fields(ctx, cc.labels.data(), cc.cc_bytes, d, tables, queue_dev.data(), int(queue.size()), daf.data(),
                  state.data(), lists[0].data(), lists[1].data());
   root = root_dev.download(ctx);
@@ -233,7 +236,7 @@ Packed skeletonize(Context &ctx, const LabelVolume &in, const SkeletonizeOptions
   next.clear(ctx);
   TraceArgs ta;
   ta.d = d;
-  ta.cc = cc.labels.data();
+  ta.nbr = nbr.data();
   ta.dbf = dbf.data();
   ta.pdrf = pdrf.data();
   ta.dist = daf.data();
@@ -257,7 +260,7 @@ Packed skeletonize(Context &ctx, const LabelVolume &in, const SkeletonizeOptions
   ta.rec_start = rec_start.data();
   ta.rec_cap = rec_cap;
   ta.counters = counters.data();
-  Buffer<unsigned long long> label_stats(ctx, 8 * k);
+  Buffer<unsigned long long> label_stats(ctx, size_t(kTraceProfileSlots) * k);
   label_stats.clear(ctx);
   ta.label_stats = label_stats.data();
   ta.ejected = ejected.data();
@@ -288,11 +291,18 @@ Packed skeletonize(Context &ctx, const LabelVolume &in, const SkeletonizeOptions
     BROOK_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx.device()));
     const double tick_ms = khz > 0 ? 1.0 / khz : 0.0;
     std::vector<unsigned long long> ls = label_stats.download(ctx);
-    double sums[8] = {0}, worst = 0;
+    const size_t P = kTraceProfileSlots;
+    double sums[kTraceProfileSlots] = {0}, worst = 0;
     size_t worst_id = 0;
     for (int id : labels) {
-      for (int j = 0; j < 8; ++j) sums[j] += double(ls[8 * size_t(id) + j]);
-      if (double(ls[8 * size_t(id)]) > worst) { worst = double(ls[8 * size_t(id)]); worst_id = size_t(id); }
+      for (size_t j = 0; j < P; ++j) sums[j] += double(ls[P * size_t(id) + j]);
+      if (double(ls[P * size_t(id)]) > worst) { worst = double(ls[P * size_t(id)]); worst_id = size_t(id); }
+    }
+    const char *work[6] = {"railroad_phases", "far_scanned", "near_expanded", "invalidate_expanded",
+                           "waiting_pulled", "bands"};
+    for (int j = 0; j < 6; ++j) {
+      ctx.stats[std::string(work[j]) + "_per_path"] = sums[8 + j] / std::max(1.0, sums[7]);
+      ctx.stats[std::string("slowest_") + work[j]] = double(ls[P * worst_id + 8 + j]);
     }
     const double nl = std::max<double>(1.0, double(labels.size())), paths = std::max(1.0, sums[7]);
     ctx.stats["trace_label_max_ms"] = worst * tick_ms;
@@ -302,11 +312,23 @@ Packed skeletonize(Context &ctx, const LabelVolume &in, const SkeletonizeOptions
     ctx.stats["railroad_rounds_per_path"] = sums[5] / paths;
     ctx.stats["invalidate_rounds_per_path"] = sums[6] / paths;
     for (int j = 1; j < 5; ++j)
-      ctx.stats[std::string("slowest_") + phase[j] + "_ms"] = double(ls[8 * worst_id + j]) * tick_ms;
-    ctx.stats["slowest_paths"] = double(ls[8 * worst_id + 7]);
+      ctx.stats[std::string("slowest_") + phase[j] + "_ms"] = double(ls[P * worst_id + j]) * tick_ms;
+    ctx.stats["slowest_paths"] = double(ls[P * worst_id + 7]);
     ctx.stats["slowest_voxels"] = double(boxes[worst_id].count);
-    ctx.stats["slowest_railroad_rounds"] = double(ls[8 * worst_id + 5]);
-    ctx.stats["slowest_invalidate_rounds"] = double(ls[8 * worst_id + 6]);
+    ctx.stats["slowest_railroad_rounds"] = double(ls[P * worst_id + 5]);
+    ctx.stats["slowest_invalidate_rounds"] = double(ls[P * worst_id + 6]);
+    ctx.stats["slowest_id"] = double(worst_id);
+    if (ctx.kernel_timing)  // per-label detail for diagnostics
+      for (int id : labels) {
+        const std::string s = std::to_string(id);
+        ctx.stats["label_ms:" + s] = double(ls[P * id]) * tick_ms;
+        ctx.stats["label_railroad_ms:" + s] = double(ls[P * id + 2]) * tick_ms;
+        ctx.stats["label_paths:" + s] = double(ls[P * id + 7]);
+        ctx.stats["label_near:" + s] = double(ls[P * id + 10]);
+        ctx.stats["label_inval:" + s] = double(ls[P * id + 11]);
+        ctx.stats["label_voxels:" + s] = double(boxes[id].count);
+        ctx.stats["label_dbfmax:" + s] = double(dbf_max[id]);
+      }
   }
   ctx.stats["labels_traced"] = double(labels.size());
 

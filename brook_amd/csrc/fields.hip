@@ -40,6 +40,24 @@ __global__ void __launch_bounds__(256) fill_segments(const CC *cc, int n, Dims d
   if (valid && l != CC(0)) vox[seg_start[l] + base + (lane - my_start)] = i;
 }
 
+// Neighbour mask: bit k of mask[v] = neighbour k of v is inside the volume and has v's (nonzero) id.
+// Every later neighbour walk reads this one word instead of 26 ids and bounds checks.
+template <class CC>
+__global__ void __launch_bounds__(256) mask_kernel(const CC *cc, Dims d, long long n, unsigned *mask) {
+  long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  CC l = cc[i];
+  if (l == CC(0)) { mask[i] = 0u; return; }
+  int x, y, z;
+  unravel(int(i), d, x, y, z);
+  unsigned m = 0;
+  for_each_nb([&](auto K) {
+    constexpr int k = decltype(K)::value;
+    if (inside(x + kDX[k], y + kDY[k], z + kDZ[k], d) && cc[i + nb_offset<k>(d)] == l) m |= 1u << k;
+  });
+  mask[i] = m;
+}
+
 __global__ void fill_float(float *p, long long n, float v) {
   long long i = (long long)blockIdx.x * 256 + threadIdx.x;
   if (i < n) p[i] = v;
@@ -47,8 +65,7 @@ __global__ void fill_float(float *p, long long n, float v) {
 
 // Geometric flood of label L from one source inside one workgroup. dist must be +inf on the
 // label's voxels and inq 0 on entry; on exit inq is 0 again.
-template <class CC>
-__device__ void flood_label(const CC *cc, Dims d, CC L, int src, const float *W, float *dist, int *inq, int *cur,
+__device__ void flood_label(const unsigned *nbr, Dims d, int src, const float *W, float *dist, int *inq, int *cur,
                             int *nxt, int *counts /*LDS [2]*/) {
   if (threadIdx.x == 0) {
     st_f(dist + src, 0.0f);
@@ -64,37 +81,30 @@ __device__ void flood_label(const CC *cc, Dims d, CC L, int src, const float *W,
       int u = ld_i(cur + i);
       atomicExch(inq + u, 0);  // clear, THEN read the distance
       float du = ld_f(dist + u);
-      int x, y, z;
-      unravel(u, d, x, y, z);
-      // all 26 neighbours' loads, then all their atomics, in flight together
-      int v[26];
-      bool ok[26];
-#pragma unroll
-      for (int k = 0; k < 26; ++k) {
-        int nx = x + kNX[k], ny = y + kNY[k], nz = z + kNZ[k];
-        ok[k] = inside(nx, ny, nz, d);
-        v[k] = ok[k] ? flat(nx, ny, nz, d) : u;
-      }
+      const unsigned m = nbr[u];
+      // all neighbours' loads, then all their atomics, in flight together
       float seen[26];
-#pragma unroll
-      for (int k = 0; k < 26; ++k) {
-        ok[k] = ok[k] && cc[v[k]] == L;
-        seen[k] = ld_f(dist + v[k]);
-      }
-      bool better[26];
-#pragma unroll
-      for (int k = 0; k < 26; ++k) {
-        float cand = du + W[k];
-        better[k] = false;
-        if (ok[k] && cand < seen[k])
-          better[k] = __int_as_float(atomicMin(reinterpret_cast<int *>(dist + v[k]), __float_as_int(cand))) > cand;
-      }
-      bool fresh[26];
-#pragma unroll
-      for (int k = 0; k < 26; ++k) fresh[k] = better[k] && atomicExch(inq + v[k], 1) == 0;
-#pragma unroll
-      for (int k = 0; k < 26; ++k)
-        if (fresh[k]) nxt[atomicAdd(&counts[1], 1)] = v[k];
+      for_each_nb([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        seen[k] = (m >> k) & 1u ? ld_f(dist + u + nb_offset<k>(d)) : 0.0f;
+      });
+      float old[26];
+      for_each_nb([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        const float cand = du + W[k];
+        old[k] = cand;
+        if (((m >> k) & 1u) && cand < seen[k])
+          old[k] = __int_as_float(atomicMin(reinterpret_cast<int *>(dist + u + nb_offset<k>(d)), __float_as_int(cand)));
+      });
+      int was[26];
+      for_each_nb([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        was[k] = old[k] > du + W[k] ? atomicExch(inq + u + nb_offset<k>(d), 1) : 1;
+      });
+      for_each_nb([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        if (was[k] == 0) nxt[atomicAdd(&counts[1], 1)] = u + nb_offset<k>(d);
+      });
     }
     __syncthreads();
     if (threadIdx.x == 0) { counts[0] = counts[1]; counts[1] = 0; }
@@ -146,14 +156,14 @@ __global__ void __launch_bounds__(512) prepare_labels(const CC *cc, Dims d, Labe
     int *a = lists_a + s0, *b = lists_b + s0;
     long long root = t.root[L];
     if (root < 0) {  // no border target: flood from the component's first voxel, root = argmax
-      flood_label<CC>(cc, d, CC(L), int(t.first[L]), W, dist, inq, a, b, counts);
+      flood_label(t.nbr, d, int(t.first[L]), W, dist, inq, a, b, counts);
       unsigned long long key = label_argmax(dist, list, cnt, &best);
       root = key ? (long long)(0xFFFFFFFFu - (unsigned)key) : -1;
       for (int j = threadIdx.x; j < cnt; j += blockDim.x) st_f(dist + list[j], kInf);
       __syncthreads();
       if (threadIdx.x == 0) t.root[L] = root;
     }
-    flood_label<CC>(cc, d, CC(L), int(root), W, dist, inq, a, b, counts);
+    flood_label(t.nbr, d, int(root), W, dist, inq, a, b, counts);
     unsigned long long key = label_argmax(dist, list, cnt, &best);
     if (threadIdx.x == 0) {
       t.target[L] = key ? (long long)(0xFFFFFFFFu - (unsigned)key) : -1;
@@ -223,6 +233,19 @@ void build_segments(Context &ctx, const void *cc, int cc_bytes, Dims d, const st
   else
     fill_segments<uint32_t><<<grid_for(n), 256, 0, ctx.stream()>>>(static_cast<const uint32_t *>(cc), n, d,
                                                                    seg_start_dev.data(), fill.data(), vox.data());
+  BROOK_LAUNCHED();
+}
+
+void neighbour_masks(Context &ctx, const void *cc, int cc_bytes, Dims d, unsigned *mask) {
+  const long long n = (long long)d.sx * d.sy * d.sz;
+  if (n == 0) return;
+  KernelTimer kt(ctx, "neighbour_masks");
+  if (cc_bytes == 1)
+    mask_kernel<uint8_t><<<grid_for(n), 256, 0, ctx.stream()>>>(static_cast<const uint8_t *>(cc), d, n, mask);
+  else if (cc_bytes == 2)
+    mask_kernel<uint16_t><<<grid_for(n), 256, 0, ctx.stream()>>>(static_cast<const uint16_t *>(cc), d, n, mask);
+  else
+    mask_kernel<uint32_t><<<grid_for(n), 256, 0, ctx.stream()>>>(static_cast<const uint32_t *>(cc), d, n, mask);
   BROOK_LAUNCHED();
 }
 

@@ -16,7 +16,11 @@ struct LabelTables {
   long long *target = nullptr;     // out: DAF argmax (lowest flat index on ties)
   float *max_daf = nullptr;        // out: DAF maximum
   const float *weights = nullptr;  // 26 geometric step weights
+  const unsigned *nbr = nullptr;   // per-voxel same-label neighbour masks
 };
+
+// mask[v] bit k = neighbour k of v is inside the volume and has v's nonzero id.
+void neighbour_masks(Context &ctx, const void *cc, int cc_bytes, Dims d, unsigned *mask);
 
 void build_segments(Context &ctx, const void *cc, int cc_bytes, Dims d, const std::vector<int> &seg_start,
                     Buffer<int> &seg_start_dev, Buffer<int> &vox);

@@ -16,6 +16,7 @@
 //   invalidate rolling ball, bands of the bottleneck M on a fixed grid, synchronous rounds from
 //              a snapshot inside a band, exact pulls at band boundaries
 //                                                         (invalidation_kernels.cuh:105-299)
+// Neighbour walks read the per-voxel same-label mask (one word instead of 26 ids).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -30,33 +31,35 @@ constexpr int kDQ = 1, kDF = 2, kIQ = 4, kIW = 8;
 constexpr unsigned long long kMinfarNone = 0x7f8000007f800000ULL;
 constexpr unsigned long long kClaimed = 0xFFFFFFFFFFFFFFFEULL;
 constexpr int kLaneFront = 16;  // fronts up to this size relax with one lane per neighbour
+constexpr int kProf = kTraceProfileSlots;
+#ifndef BROOK_NB_BATCH
+#define BROOK_NB_BATCH 13
+#endif
+// Neighbours whose memory operations one thread keeps in flight together (13: two halves; 26: all).
+constexpr int kNbBatch = BROOK_NB_BATCH;
 
 __device__ __forceinline__ int wave_min(int v) {
   for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
   return v;
 }
 
-template <class CC> struct Label {
-  const TraceArgs &a;
-  const CC *cc;
-  CC L;
-  int seg, count;
-};
-
 struct Shared {
   int n_cur, n_nxt, n_far, n_farnxt, n_touched, n_keep;
-  int done, target, status, path_len, found, band;
+  int done, target, path_len, found, band;
   long long cursor, seg_end;
   int pcount;
   unsigned lowest;
   float T;
   unsigned long long rail_best, minfar;
   int store_pos;
-  float lane_du[kLaneFront];
-  int lane_u[kLaneFront];
+  float lane_du[kLaneFront * 2];
+  int lane_u[kLaneFront * 2];
+  unsigned lane_m[kLaneFront * 2];
   // per-label profile: [0] total [1] target [2] railroad [3] backtrace [4] invalidation (wall-clock
-  // ticks), [5] railroad rounds, [6] invalidation rounds, [7] paths
-  unsigned long long prof[8];
+  // ticks), [5] railroad rounds, [6] invalidation rounds, [7] paths, [8] railroad phases,
+  // [9] far entries scanned, [10] near entries expanded, [11] invalidation entries expanded,
+  // [12] waiting entries pulled, [13] bands
+  unsigned long long prof[kProf];
   long long tick;
 };
 
@@ -69,17 +72,13 @@ __device__ __forceinline__ void lap(Shared &sh, int slot) {
 }
 
 // ---- railroad -------------------------------------------------------------------------------
-template <class CC>
-__device__ __forceinline__ void relax(const Label<CC> &lb, Shared &sh, int u, int x, int y, int z, int k, float du,
-                                      float bound, int *nxt, int *far, int *touched) {
-  const TraceArgs &a = lb.a;
-  int nx = x + kNX[k], ny = y + kNY[k], nz = z + kNZ[k];
-  if (!inside(nx, ny, nz, a.d)) return;
-  int v = flat(nx, ny, nz, a.d);
-  if (lb.cc[v] != lb.L) return;
-  float fv = ld_f(a.pdrf + v);
-  float cand = vw_step(du, fv);
-  float old = atomic_min_pos(a.dist + v, cand);
+// Relaxation of neighbour k (a set bit of u's mask) of u, which holds distance du.
+__device__ __forceinline__ void relax_one(const TraceArgs &a, Shared &sh, int u, int k, float du, float bound,
+                                          int *nxt, int *far, int *touched) {
+  const int v = u + nb_offset(k, a.d);
+  const float fv = ld_f(a.pdrf + v);
+  const float cand = vw_step(du, fv);
+  const float old = atomic_min_pos(a.dist + v, cand);
   if (!(old > cand)) return;
   if (old >= kInf) touched[atomicAdd(&sh.n_touched, 1)] = v;
   if (fv == 0.0f) {  // a rail: fixes the cost and the last road voxel; never expanded
@@ -94,55 +93,65 @@ __device__ __forceinline__ void relax(const Label<CC> &lb, Shared &sh, int u, in
   }
 }
 
-// relax() for all 26 neighbours of u at once: every load, then every atomic, is issued before
-// any result is waited for (one voxel costs a few memory round trips instead of 26 chains).
-template <class CC>
-__device__ __forceinline__ void relax_all(const Label<CC> &lb, Shared &sh, int u, float du, float bound, int *nxt,
-                                          int *far, int *touched) {
-  const TraceArgs &a = lb.a;
-  int x, y, z;
-  unravel(u, a.d, x, y, z);
-  int v[26];
-  bool ok[26];
-#pragma unroll
-  for (int k = 0; k < 26; ++k) {
-    int nx = x + kNX[k], ny = y + kNY[k], nz = z + kNZ[k];
-    ok[k] = inside(nx, ny, nz, a.d);
-    v[k] = ok[k] ? flat(nx, ny, nz, a.d) : u;
-  }
-  float fv[26], seen[26];
-#pragma unroll
-  for (int k = 0; k < 26; ++k) {
-    ok[k] = ok[k] && lb.cc[v[k]] == lb.L;
-    fv[k] = ld_f(a.pdrf + v[k]);
-    seen[k] = ld_f(a.dist + v[k]);
-  }
-  float cand[26], old[26];
-#pragma unroll
-  for (int k = 0; k < 26; ++k) {
-    cand[k] = vw_step(du, fv[k]);
-    old[k] = cand[k];
-    if (ok[k] && cand[k] < seen[k])
-      old[k] = __int_as_float(atomicMin(reinterpret_cast<int *>(a.dist + v[k]), __float_as_int(cand[k])));
-  }
-  int was[26];
-#pragma unroll
-  for (int k = 0; k < 26; ++k) {
-    was[k] = -1;
-    if (old[k] > cand[k] && fv[k] != 0.0f) was[k] = atomicOr(a.state + v[k], cand[k] <= bound ? kDQ : kDF);
-  }
-#pragma unroll
-  for (int k = 0; k < 26; ++k) {
-    if (!(old[k] > cand[k])) continue;
-    if (old[k] >= kInf) touched[atomicAdd(&sh.n_touched, 1)] = v[k];
-    if (fv[k] == 0.0f) {  // a rail: fixes the cost and the last road voxel; never expanded
-      atomicMin(&sh.rail_best, ((unsigned long long)__float_as_uint(cand[k]) << 32) | (unsigned)v[k]);
-    } else if (cand[k] <= bound) {
-      if (!(was[k] & kDQ)) nxt[atomicAdd(&sh.n_nxt, 1)] = v[k];
+// Masked neighbours B..E-1 of u at once: every load, then every atomic, is issued before any result
+// is waited for (a voxel costs a few memory round trips instead of one chain per neighbour).
+// A rail is recognised by cand == du (vw_step adds nothing only for a zero cost).
+template <int B, int E>
+__device__ __forceinline__ void relax_part(const TraceArgs &a, Shared &sh, int u, unsigned m, float du, float bound,
+                                           int *nxt, int *far, int *touched) {
+  float cand[E - B], seen[E - B];
+  for_nb<B, E>([&](auto K) {
+    constexpr int k = decltype(K)::value, i = k - B;
+    const bool on = (m >> k) & 1u;
+    cand[i] = on ? ld_f(a.pdrf + u + nb_offset<k>(a.d)) : kInf;
+    seen[i] = on ? ld_f(a.dist + u + nb_offset<k>(a.d)) : 0.0f;
+  });
+  for_nb<B, E>([&](auto K) {  // seen[] becomes the previous value, or cand when nothing improves
+    constexpr int k = decltype(K)::value, i = k - B;
+    const float c = vw_step(du, cand[i]);
+    cand[i] = c;
+    const bool go = ((m >> k) & 1u) && c < seen[i];
+    seen[i] = go ? __int_as_float(atomicMin(reinterpret_cast<int *>(a.dist + u + nb_offset<k>(a.d)), __float_as_int(c)))
+                 : c;
+  });
+  unsigned near = 0, farm = 0;
+  for_nb<B, E>([&](auto K) {
+    constexpr int k = decltype(K)::value, i = k - B;
+    const float c = cand[i];
+    if (!(seen[i] > c)) return;
+    const int v = u + nb_offset<k>(a.d);
+    if (seen[i] >= kInf) touched[atomicAdd(&sh.n_touched, 1)] = v;
+    if (c == du) {  // rail
+      atomicMin(&sh.rail_best, ((unsigned long long)__float_as_uint(c) << 32) | (unsigned)v);
+    } else if (c <= bound) {
+      near |= 1u << k;
     } else {
-      atomicMin(&sh.minfar, ((unsigned long long)__float_as_uint(cand[k]) << 32) | __float_as_uint(fv[k]));
-      if (!(was[k] & kDF)) far[atomicAdd(&sh.n_far, 1)] = v[k];
+      atomicMin(&sh.minfar, ((unsigned long long)__float_as_uint(c) << 32) | __float_as_uint(c - du));
+      farm |= 1u << k;
     }
+  });
+  if (!(near | farm)) return;
+  int was[E - B];
+  for_nb<B, E>([&](auto K) {
+    constexpr int k = decltype(K)::value, i = k - B;
+    was[i] = -1;
+    if ((near | farm) >> k & 1u) was[i] = atomicOr(a.state + u + nb_offset<k>(a.d), (near >> k & 1u) ? kDQ : kDF);
+  });
+  for_nb<B, E>([&](auto K) {
+    constexpr int k = decltype(K)::value, i = k - B;
+    const int v = u + nb_offset<k>(a.d);
+    if ((near >> k & 1u) && !(was[i] & kDQ)) nxt[atomicAdd(&sh.n_nxt, 1)] = v;
+    if ((farm >> k & 1u) && !(was[i] & kDF)) far[atomicAdd(&sh.n_far, 1)] = v;
+  });
+}
+
+__device__ __forceinline__ void relax_all(const TraceArgs &a, Shared &sh, int u, unsigned m, float du, float bound,
+                                          int *nxt, int *far, int *touched) {
+  if constexpr (kNbBatch >= 26) {
+    relax_part<0, 26>(a, sh, u, m, du, bound, nxt, far, touched);
+  } else {
+    if (m & 0x1FFFu) relax_part<0, 13>(a, sh, u, m, du, bound, nxt, far, touched);
+    if (m >> 13) relax_part<13, 26>(a, sh, u, m, du, bound, nxt, far, touched);
   }
 }
 
@@ -154,10 +163,8 @@ __device__ __forceinline__ float rail_bound(const Shared &sh) {
 }
 
 // Returns 1 when the nearest rail is settled (sh.rail_best), 2 when no rail is reachable.
-template <class CC>
-__device__ int railroad(const Label<CC> &lb, Shared &sh, int t, int *near_a, int *near_b, int *far_a, int *far_b,
+__device__ __forceinline__ int railroad(const TraceArgs &a, Shared &sh, int t, int *near_a, int *near_b, int *far_a, int *far_b,
                         int *touched) {
-  const TraceArgs &a = lb.a;
   if (threadIdx.x == 0) {
     st_f(a.dist + t, 0.0f);
     touched[0] = t;
@@ -189,30 +196,28 @@ __device__ int railroad(const Label<CC> &lb, Shared &sh, int t, int *near_a, int
           int u = ld_i(cur + threadIdx.x);
           atomicAnd(a.state + u, ~kDQ);  // clear, THEN read the distance
           float du = ld_f(a.dist + u);
-          bool skip = ld_f(a.pdrf + u) == 0.0f || du > bound;
-          sh.lane_du[threadIdx.x] = skip ? -1.0f : du;
+          sh.lane_du[threadIdx.x] = du > bound ? -1.0f : du;
           sh.lane_u[threadIdx.x] = u;
+          sh.lane_m[threadIdx.x] = a.nbr[u];
         }
         __syncthreads();
         for (int j = threadIdx.x; j < nc * 32; j += blockDim.x) {
           int i = j >> 5, k = j & 31;
           float du = sh.lane_du[i];
-          if (k >= 26 || du < 0.0f) continue;
-          int u = sh.lane_u[i], x, y, z;
-          unravel(u, a.d, x, y, z);
-          relax(lb, sh, u, x, y, z, k, du, bound, nxt, fcur, touched);
+          if (k >= 26 || du < 0.0f || !((sh.lane_m[i] >> k) & 1u)) continue;
+          relax_one(a, sh, sh.lane_u[i], k, du, bound, nxt, fcur, touched);
         }
       } else {
         for (int i = threadIdx.x; i < nc; i += blockDim.x) {
           int u = ld_i(cur + i);
           atomicAnd(a.state + u, ~kDQ);  // clear, THEN read the distance
           float du = ld_f(a.dist + u);
-          if (ld_f(a.pdrf + u) == 0.0f || du > bound) continue;
-          relax_all(lb, sh, u, du, bound, nxt, fcur, touched);
+          if (du > bound) continue;
+          relax_all(a, sh, u, a.nbr[u], du, bound, nxt, fcur, touched);
         }
       }
       __syncthreads();
-      if (threadIdx.x == 0) { sh.n_cur = sh.n_nxt; sh.n_nxt = 0; sh.prof[5] += 1; }
+      if (threadIdx.x == 0) { sh.n_cur = sh.n_nxt; sh.n_nxt = 0; sh.prof[5] += 1; sh.prof[10] += nc; }
       int *tmp = cur; cur = nxt; nxt = tmp;
       __syncthreads();
     }
@@ -227,6 +232,8 @@ __device__ int railroad(const Label<CC> &lb, Shared &sh, int t, int *near_a, int
         sh.T = dmin + w;
         sh.minfar = kMinfarNone;
       }
+      sh.prof[8] += 1;
+      sh.prof[9] += sh.n_far;
     }
     __syncthreads();
     if (sh.done) break;
@@ -253,90 +260,69 @@ __device__ int railroad(const Label<CC> &lb, Shared &sh, int t, int *near_a, int
 }
 
 // ---- backtrace (wave 0) ---------------------------------------------------------------------
-template <class CC>
-__device__ void backtrace(const Label<CC> &lb, Shared &sh, int t, int *walk) {
-  const TraceArgs &a = lb.a;
+__device__ __forceinline__ void backtrace(const TraceArgs &a, Shared &sh, int t, int count, int *walk) {
   if (threadIdx.x >= 64) return;
   const int lane = threadIdx.x;
+  const int ok_lane = lane < 26 ? lane : 0;
   int cur = (int)(unsigned)(sh.rail_best & 0xFFFFFFFFULL);
-  int x, y, z;
-  unravel(cur, a.d, x, y, z);
-  int cnt = 0, kept = 0;
+  int kept = 0;
+  bool first = true;
   for (;;) {
     int kb = -1;
+    const unsigned m = a.nbr[cur];
     if (cur != t) {
       float du = kInf;
-      if (lane < 26) {
-        int nx = x + kNX[lane], ny = y + kNY[lane], nz = z + kNZ[lane];
-        if (inside(nx, ny, nz, a.d)) {
-          int u = flat(nx, ny, nz, a.d);
-          if (lb.cc[u] == lb.L && ld_f(a.pdrf + u) != 0.0f) du = ld_f(a.dist + u);
-        }
+      if (lane < 26 && ((m >> lane) & 1u)) {
+        int u = cur + nb_offset(ok_lane, a.d);
+        if (ld_f(a.pdrf + u) != 0.0f) du = ld_f(a.dist + u);
       }
       int db = __float_as_int(du);
       int mb = wave_min(db);
       if (mb < 0x7f800000) kb = wave_min(db == mb ? lane : 64);
     }
-    if (cnt == 0 && kb >= 0) {  // the join at u = cur + N[kb]: u's first rail in dijkstra3d order
-      int ux = x + kNX[kb], uy = y + kNY[kb], uz = z + kNZ[kb];
+    if (first && kb >= 0) {  // the join at u = cur + N[kb]: u's first rail in dijkstra3d order
+      const int u = cur + nb_offset(kb, a.d);
+      const unsigned mu = a.nbr[u];
       int key = 64;
-      if (lane < 26) {
-        int nx = ux + kNX[lane], ny = uy + kNY[lane], nz = uz + kNZ[lane];
-        if (inside(nx, ny, nz, a.d)) {
-          int v = flat(nx, ny, nz, a.d);
-          if (lb.cc[v] == lb.L && ld_f(a.pdrf + v) == 0.0f) key = kD3Rank[lane];
-        }
-      }
+      if (lane < 26 && ((mu >> lane) & 1u) && ld_f(a.pdrf + u + nb_offset(ok_lane, a.d)) == 0.0f) key = kD3Rank[lane];
       int r = wave_min(key);
       if (r < 26) {
         int ks = kD3Inv[r];
-        x = ux + kNX[ks]; y = uy + kNY[ks]; z = uz + kNZ[ks];
-        cur = flat(x, y, z, a.d);
+        cur = u + nb_offset(ks, a.d);
         kb = kOpp[ks];
       }
     }
-    if (kept >= lb.count) { kept = -1; break; }  // longer than the label: cannot happen on a valid field
+    first = false;
+    if (kept >= count) { kept = -1; break; }  // longer than the label: cannot happen on a valid field
     if (lane == 0) walk[kept] = cur;
     ++kept;
-    ++cnt;
     if (kb < 0) break;
-    x += kNX[kb]; y += kNY[kb]; z += kNZ[kb];
-    cur = flat(x, y, z, a.d);
+    cur += nb_offset(kb, a.d);
   }
   if (lane == 0) sh.path_len = kept;
 }
 
 // ---- invalidation ---------------------------------------------------------------------------
-__device__ __forceinline__ float offer_distance(const TraceArgs &a, int x, int y, int z, int src) {
-  int qx, qy, qz;
-  unravel(src, a.d, qx, qy, qz);
+__device__ __forceinline__ float offer_distance(const TraceArgs &a, int x, int y, int z, int qx, int qy, int qz) {
   float dx = a.wx * (float)(x - qx), dy = a.wy * (float)(y - qy), dz = a.wz * (float)(z - qz);
   return sqrtf(dx * dx + dy * dy + dz * dz);
 }
 
-template <class CC>
-__device__ __forceinline__ void offer(const Label<CC> &lb, Shared &sh, int x, int y, int z, int k, unsigned su,
-                                      unsigned mu, float maxd, unsigned hi, int *nxt, int *far, int *touched) {
-  const TraceArgs &a = lb.a;
-  int nx = x + kNX[k], ny = y + kNY[k], nz = z + kNZ[k];
-  if (!inside(nx, ny, nz, a.d)) return;
-  int v = flat(nx, ny, nz, a.d);
-  if (lb.cc[v] != lb.L || !ld_u8(a.alive + v)) return;
-  float d = offer_distance(a, nx, ny, nz, int(su));
-  if (d >= maxd) return;
-  unsigned mb = max(__float_as_uint(d), mu);  // M = max(M[u], d)
-  if (mb < hi) {  // inside the band: claim now
-    unsigned long long cand = ((unsigned long long)mb << 32) | su;
-    unsigned long long cur = ld_u64(a.key + v);
-    if (cur <= cand) return;
-    unsigned long long old = atomicMin(a.key + v, cand);
+// Claim or wait for neighbour v, at distance d from the source su, from a voxel with bottleneck mu.
+__device__ __forceinline__ void offer_to(const TraceArgs &a, Shared &sh, int v, float d, unsigned su, unsigned mu,
+                                         unsigned hi, unsigned long long old_or_none, bool issued, int *nxt, int *far,
+                                         int *touched) {
+  const unsigned mb = max(__float_as_uint(d), mu);  // M = max(M[u], d)
+  if (mb < hi) {
+    const unsigned long long cand = ((unsigned long long)mb << 32) | su;
+    const unsigned long long old = issued ? old_or_none : (ld_u64(a.key + v) <= cand ? cand : atomicMin(a.key + v, cand));
     if (old > cand) {
       if (old == kMaxKey) touched[atomicAdd(&sh.n_touched, 1)] = v;
       int was = atomicOr(a.state + v, kIQ);
       if (was & kIW) atomicAnd(a.state + v, ~kIW);
       if (!(was & kIQ)) nxt[atomicAdd(&sh.n_nxt, 1)] = v;
     }
-  } else if (ld_u64(a.key + v) == kMaxKey) {  // beyond it: wait for its band
+  } else if (ld_u64(a.key + v) == kMaxKey) {  // beyond the band: wait for its band
     int seen = ld_i(a.state + v);
     while (!(seen & (kIQ | kIW))) {
       int got = atomicCAS(a.state + v, seen, seen | kIW);
@@ -346,64 +332,55 @@ __device__ __forceinline__ void offer(const Label<CC> &lb, Shared &sh, int x, in
   }
 }
 
-// offer() to all 26 neighbours of u at once (loads, then claims, then queue flags in flight together).
-template <class CC>
-__device__ __forceinline__ void offer_all(const Label<CC> &lb, Shared &sh, int u, unsigned su, unsigned mu, float maxd,
-                                          unsigned hi, int *nxt, int *far, int *touched) {
-  const TraceArgs &a = lb.a;
+// Masked neighbours B..E-1 of u at once: alive and key loads, then claim atomics, in flight
+// together. Only the key's high word (the bottleneck) is pre-loaded: a claim whose bottleneck is
+// above it cannot win and issues no atomic.
+template <int B, int E>
+__device__ __forceinline__ void offer_part(const TraceArgs &a, Shared &sh, int u, unsigned m, int x, int y, int z,
+                                           int qx, int qy, int qz, unsigned su, unsigned mu, float maxd, unsigned hi,
+                                           int *nxt, int *far, int *touched) {
+  const unsigned *key_hi = reinterpret_cast<const unsigned *>(a.key) + 1;
+  unsigned char al[E - B];
+  unsigned kh[E - B];
+  for_nb<B, E>([&](auto K) {
+    constexpr int k = decltype(K)::value, i = k - B;
+    const bool on = (m >> k) & 1u;
+    al[i] = on ? ld_u8(a.alive + u + nb_offset<k>(a.d)) : (unsigned char)0;
+    kh[i] = on ? ld_u(key_hi + 2 * (u + nb_offset<k>(a.d))) : 0u;
+  });
+  float dist[E - B];
+  unsigned ok = 0, claim = 0;
+  unsigned long long old[E - B];
+  for_nb<B, E>([&](auto K) {
+    constexpr int k = decltype(K)::value, i = k - B;
+    dist[i] = offer_distance(a, x + kDX[k], y + kDY[k], z + kDZ[k], qx, qy, qz);
+    const unsigned mb = max(__float_as_uint(dist[i]), mu);
+    old[i] = 0;  // "no improvement" unless the atomic says otherwise
+    if (al[i] && dist[i] < maxd) {
+      ok |= 1u << k;
+      if (mb < hi) {
+        claim |= 1u << k;
+        if (kh[i] >= mb) old[i] = atomicMin(a.key + u + nb_offset<k>(a.d), ((unsigned long long)mb << 32) | su);
+      }
+    }
+  });
+  for_nb<B, E>([&](auto K) {
+    constexpr int k = decltype(K)::value, i = k - B;
+    if ((ok >> k) & 1u)
+      offer_to(a, sh, u + nb_offset<k>(a.d), dist[i], su, mu, hi, old[i], (claim >> k) & 1u, nxt, far, touched);
+  });
+}
+
+__device__ __forceinline__ void offer_all(const TraceArgs &a, Shared &sh, int u, unsigned m, unsigned su, unsigned mu,
+                                          float maxd, unsigned hi, int *nxt, int *far, int *touched) {
   int x, y, z, qx, qy, qz;
   unravel(u, a.d, x, y, z);
   unravel(int(su), a.d, qx, qy, qz);
-  int v[26];
-  bool ok[26];
-#pragma unroll
-  for (int k = 0; k < 26; ++k) {
-    int nx = x + kNX[k], ny = y + kNY[k], nz = z + kNZ[k];
-    ok[k] = inside(nx, ny, nz, a.d);
-    v[k] = ok[k] ? flat(nx, ny, nz, a.d) : u;
-  }
-  unsigned char al[26];
-  unsigned long long kv[26];
-#pragma unroll
-  for (int k = 0; k < 26; ++k) {
-    ok[k] = ok[k] && lb.cc[v[k]] == lb.L;
-    al[k] = ld_u8(a.alive + v[k]);
-    kv[k] = ld_u64(a.key + v[k]);
-  }
-  unsigned mb[26];
-#pragma unroll
-  for (int k = 0; k < 26; ++k) {
-    float dx = a.wx * (float)(x + kNX[k] - qx), dy = a.wy * (float)(y + kNY[k] - qy), dz = a.wz * (float)(z + kNZ[k] - qz);
-    float d = sqrtf(dx * dx + dy * dy + dz * dz);
-    ok[k] = ok[k] && al[k] && d < maxd;
-    mb[k] = max(__float_as_uint(d), mu);  // M = max(M[u], d)
-  }
-  unsigned long long old[26];
-#pragma unroll
-  for (int k = 0; k < 26; ++k) {
-    unsigned long long cand = ((unsigned long long)mb[k] << 32) | su;
-    old[k] = cand;
-    if (ok[k] && mb[k] < hi && kv[k] > cand) old[k] = atomicMin(a.key + v[k], cand);
-  }
-#pragma unroll
-  for (int k = 0; k < 26; ++k) {
-    if (!ok[k]) continue;
-    unsigned long long cand = ((unsigned long long)mb[k] << 32) | su;
-    if (mb[k] < hi) {  // inside the band: claim now
-      if (old[k] > cand) {
-        if (old[k] == kMaxKey) touched[atomicAdd(&sh.n_touched, 1)] = v[k];
-        int was = atomicOr(a.state + v[k], kIQ);
-        if (was & kIW) atomicAnd(a.state + v[k], ~kIW);
-        if (!(was & kIQ)) nxt[atomicAdd(&sh.n_nxt, 1)] = v[k];
-      }
-    } else if (ld_u64(a.key + v[k]) == kMaxKey) {  // beyond it: wait for its band
-      int seen = ld_i(a.state + v[k]);
-      while (!(seen & (kIQ | kIW))) {
-        int got = atomicCAS(a.state + v[k], seen, seen | kIW);
-        if (got == seen) { far[atomicAdd(&sh.n_far, 1)] = v[k]; break; }
-        seen = got;
-      }
-    }
+  if constexpr (kNbBatch >= 26) {
+    offer_part<0, 26>(a, sh, u, m, x, y, z, qx, qy, qz, su, mu, maxd, hi, nxt, far, touched);
+  } else {
+    if (m & 0x1FFFu) offer_part<0, 13>(a, sh, u, m, x, y, z, qx, qy, qz, su, mu, maxd, hi, nxt, far, touched);
+    if (m >> 13) offer_part<13, 26>(a, sh, u, m, x, y, z, qx, qy, qz, su, mu, maxd, hi, nxt, far, touched);
   }
 }
 
@@ -415,10 +392,8 @@ __device__ __forceinline__ int band_of(unsigned mbits, float delta, int kmin) {
   return k < kmin ? kmin : k;
 }
 
-template <class CC>
-__device__ void invalidate(const Label<CC> &lb, Shared &sh, const int *path, int len, int *la, int *lb_, int *lfar,
+__device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const int *path, int len, int *la, int *lb_, int *lfar,
                            int *lkeep, int *touched, unsigned long long *snap, unsigned long long *pull) {
-  const TraceArgs &a = lb.a;
   if (threadIdx.x == 0) { sh.n_cur = sh.n_nxt = sh.n_far = sh.n_keep = sh.n_touched = 0; sh.band = 0; }
   __syncthreads();
   int *cur = la, *nxt = lb_, *far = lfar, *keep = lkeep;
@@ -446,45 +421,53 @@ __device__ void invalidate(const Label<CC> &lb, Shared &sh, const int *path, int
       const bool lanes = nc <= kLaneFront * 2;
       const int items = lanes ? nc * 32 : nc;
       for (int j = threadIdx.x; j < items; j += blockDim.x) {
-        int i = lanes ? (j >> 5) : j;
-        int k0 = lanes ? (j & 31) : 0, k1 = lanes ? k0 + 1 : 26;
-        if (k0 >= 26) continue;
-        int u = ld_i(cur + i);
-        unsigned long long ku = snap[i];
-        unsigned su = (unsigned)ku, mu = (unsigned)(ku >> 32);
-        float maxd = a.scale * a.dbf[su] + a.constant;
+        const int i = lanes ? (j >> 5) : j;
+        const int k = lanes ? (j & 31) : 0;
+        if (k >= 26) continue;
+        const int u = ld_i(cur + i);
+        const unsigned m = a.nbr[u];
+        const unsigned long long ku = snap[i];
+        const unsigned su = (unsigned)ku, mu = (unsigned)(ku >> 32);
+        const float maxd = a.scale * a.dbf[su] + a.constant;
         if (!lanes) {
-          offer_all(lb, sh, u, su, mu, maxd, hi, nxt, far, touched);
+          offer_all(a, sh, u, m, su, mu, maxd, hi, nxt, far, touched);
           continue;
         }
-        int x, y, z;
-        unravel(u, a.d, x, y, z);
-        offer(lb, sh, x, y, z, k0, su, mu, maxd, hi, nxt, far, touched);
+        if (!((m >> k) & 1u)) continue;
+        const int v = u + nb_offset(k, a.d);
+        if (!ld_u8(a.alive + v)) continue;
+        int x, y, z, qx, qy, qz;
+        unravel(v, a.d, x, y, z);
+        unravel(int(su), a.d, qx, qy, qz);
+        const float d = offer_distance(a, x, y, z, qx, qy, qz);
+        if (d >= maxd) continue;
+        offer_to(a, sh, v, d, su, mu, hi, 0, false, nxt, far, touched);
       }
       __syncthreads();
-      if (threadIdx.x == 0) { sh.n_cur = sh.n_nxt; sh.n_nxt = 0; sh.prof[6] += 1; }
+      if (threadIdx.x == 0) { sh.n_cur = sh.n_nxt; sh.n_nxt = 0; sh.prof[6] += 1; sh.prof[11] += nc; }
       int *tmp = cur; cur = nxt; nxt = tmp;
       __syncthreads();
     }
     const int nf = sh.n_far;
     if (nf == 0) break;
-    if (threadIdx.x == 0) sh.lowest = 0x7f800000u;
+    if (threadIdx.x == 0) { sh.lowest = 0x7f800000u; sh.prof[12] += nf; sh.prof[13] += 1; }
     __syncthreads();
     for (int i = threadIdx.x; i < nf; i += blockDim.x) {  // band boundary: exact pulls
       int c = ld_i(far + i);
       if (ld_u64(a.key + c) != kMaxKey) { pull[i] = kClaimed; continue; }
+      const unsigned mc = a.nbr[c];
       int x, y, z;
       unravel(c, a.d, x, y, z);
       unsigned long long best = kMaxKey;
       for (int k = 0; k < 26; ++k) {
-        int nx = x + kNX[k], ny = y + kNY[k], nz = z + kNZ[k];
-        if (!inside(nx, ny, nz, a.d)) continue;
-        int u = flat(nx, ny, nz, a.d);
-        if (lb.cc[u] != lb.L) continue;
+        if (!((mc >> k) & 1u)) continue;
+        const int u = c + nb_offset(k, a.d);
         unsigned long long ku = ld_u64(a.key + u);
         if (ku == kMaxKey) continue;
         unsigned su = (unsigned)ku;
-        float d = offer_distance(a, x, y, z, int(su));
+        int qx, qy, qz;
+        unravel(int(su), a.d, qx, qy, qz);
+        float d = offer_distance(a, x, y, z, qx, qy, qz);
         if (d >= a.scale * a.dbf[su] + a.constant) continue;
         unsigned mb = max(__float_as_uint(d), (unsigned)(ku >> 32));
         unsigned long long cand = ((unsigned long long)mb << 32) | su;
@@ -529,10 +512,9 @@ __device__ void invalidate(const Label<CC> &lb, Shared &sh, const int *path, int
 }
 
 // ---- target selection -----------------------------------------------------------------------
-template <class CC> __device__ int next_target(const Label<CC> &lb, Shared &sh) {
-  const TraceArgs &a = lb.a;
+__device__ __forceinline__ int next_target(const TraceArgs &a, Shared &sh, int L) {
   if (threadIdx.x == 0) {
-    int L = int(lb.L), pc = sh.pcount;
+    int pc = sh.pcount;
     if (pc >= a.maxp[L]) sh.target = -1;
     else if (pc < a.mcount[L]) sh.target = a.mflat[a.mstart[L] + pc];
     else sh.target = -2;
@@ -573,7 +555,6 @@ template <class CC> __device__ int next_target(const Label<CC> &lb, Shared &sh) 
   }
 }
 
-template <class CC>
 __global__ void __launch_bounds__(kBlock) trace_labels(TraceArgs a) {
   __shared__ Shared sh;
   __shared__ int item;
@@ -583,26 +564,26 @@ __global__ void __launch_bounds__(kBlock) trace_labels(TraceArgs a) {
     __syncthreads();
     if (item >= a.nqueue) return;
     const int L = a.queue[item];
-    Label<CC> lb{a, static_cast<const CC *>(a.cc), CC(L), a.seg_start[L], a.seg_count[L]};
-    int *l0 = a.lists[0] + lb.seg, *l1 = a.lists[1] + lb.seg, *l2 = a.lists[2] + lb.seg, *l3 = a.lists[3] + lb.seg,
-        *l4 = a.lists[4] + lb.seg, *walk = a.lists[5] + lb.seg;
-    unsigned long long *s0 = a.snaps[0] + lb.seg, *s1 = a.snaps[1] + lb.seg;
+    const int seg = a.seg_start[L], count = a.seg_count[L];
+    int *l0 = a.lists[0] + seg, *l1 = a.lists[1] + seg, *l2 = a.lists[2] + seg, *l3 = a.lists[3] + seg,
+        *l4 = a.lists[4] + seg, *walk = a.lists[5] + seg;
+    unsigned long long *s0 = a.snaps[0] + seg, *s1 = a.snaps[1] + seg;
     if (threadIdx.x == 0) {
-      sh.cursor = lb.seg;
-      sh.seg_end = (long long)lb.seg + lb.count;
+      sh.cursor = seg;
+      sh.seg_end = (long long)seg + count;
       sh.pcount = 0;
-      for (int j = 0; j < 8; ++j) sh.prof[j] = 0;
+      for (int j = 0; j < kProf; ++j) sh.prof[j] = 0;
       sh.tick = wall_clock64();
     }
     const long long label_begin = wall_clock64();
     __syncthreads();
     for (;;) {
-      const int t = next_target(lb, sh);
+      const int t = next_target(a, sh, L);
       lap(sh, 1);
       if (t < 0) break;
-      const int status = railroad(lb, sh, t, l0, l1, l2, l3, l4);
+      const int status = railroad(a, sh, t, l0, l1, l2, l3, l4);
       lap(sh, 2);
-      if (status == 1) backtrace(lb, sh, t, walk);
+      if (status == 1) backtrace(a, sh, t, count, walk);
       __syncthreads();
       lap(sh, 3);
       const int nt = sh.n_touched;
@@ -622,7 +603,7 @@ __global__ void __launch_bounds__(kBlock) trace_labels(TraceArgs a) {
         int pos = atomicAdd(a.counters + 0, len);
         int rec = atomicAdd(a.counters + 1, 1);
         if ((long long)pos + len > a.store_cap || rec >= a.rec_cap) {
-          atomicOr(a.counters + 3, 1);  // overflow: the host retries with larger stores
+          atomicOr(a.counters + 3, 1);  // overflow: reported to the host
           sh.store_pos = -1;
         } else {
           sh.store_pos = pos;
@@ -639,14 +620,14 @@ __global__ void __launch_bounds__(kBlock) trace_labels(TraceArgs a) {
         st_f(a.pdrf + v, 0.0f);                          // the path becomes rail
       }
       __syncthreads();
-      invalidate(lb, sh, walk, len, l0, l1, l2, l3, l4, s0, s1);
+      invalidate(a, sh, walk, len, l0, l1, l2, l3, l4, s0, s1);
       lap(sh, 4);
       if (threadIdx.x == 0) { sh.pcount += 1; sh.prof[7] += 1; }
       __syncthreads();
     }
     if (threadIdx.x == 0 && a.label_stats) {
       sh.prof[0] = (unsigned long long)(wall_clock64() - label_begin);
-      for (int j = 0; j < 8; ++j) a.label_stats[8LL * L + j] = sh.prof[j];
+      for (int j = 0; j < kProf; ++j) a.label_stats[(long long)kProf * L + j] = sh.prof[j];
     }
   }
 }
@@ -654,12 +635,11 @@ __global__ void __launch_bounds__(kBlock) trace_labels(TraceArgs a) {
 }  // namespace
 
 void trace_all(Context &ctx, int cc_bytes, const TraceArgs &args) {
+  (void)cc_bytes;
   if (args.nqueue == 0) return;
   const int grid = std::min(args.nqueue, ctx.compute_units() * 2);
   KernelTimer kt(ctx, "trace_labels");
-  if (cc_bytes == 1) trace_labels<uint8_t><<<grid, kBlock, 0, ctx.stream()>>>(args);
-  else if (cc_bytes == 2) trace_labels<uint16_t><<<grid, kBlock, 0, ctx.stream()>>>(args);
-  else trace_labels<uint32_t><<<grid, kBlock, 0, ctx.stream()>>>(args);
+  trace_labels<<<grid, kBlock, 0, ctx.stream()>>>(args);
   BROOK_LAUNCHED();
 }
 

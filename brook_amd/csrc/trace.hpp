@@ -4,9 +4,11 @@
 
 namespace brook {
 
+constexpr int kTraceProfileSlots = 16;
+
 struct TraceArgs {
   Dims d{0, 0, 0};
-  const void *cc = nullptr;
+  const unsigned *nbr = nullptr;  // per-voxel same-label neighbour masks (bit k = neighbour k)
   const float *dbf = nullptr;
   float *pdrf = nullptr;          // rails become 0 in place
   float *dist = nullptr;          // +inf volume (restored after every railroad)
@@ -25,8 +27,10 @@ struct TraceArgs {
   // [0] store used [1] records [2] ejected labels [3] overflow; [4..5] u64 railroad-reached voxels,
   // [6..7] u64 invalidated voxels
   int *counters = nullptr;
-  // per label (8 u64): wall-clock ticks total / target / railroad / backtrace / invalidation,
-  // railroad rounds, invalidation rounds, paths (null: not recorded)
+  // per label (kTraceProfileSlots u64): wall-clock ticks total / target / railroad / backtrace /
+  // invalidation, railroad rounds, invalidation rounds, paths, railroad phases, far entries
+  // scanned, near entries expanded, invalidation entries expanded, waiting entries pulled, bands
+  // (null: not recorded)
   unsigned long long *label_stats = nullptr;
   int *ejected = nullptr;
   int ejected_cap = 1;
