@@ -92,6 +92,34 @@ __device__ __forceinline__ float atomic_min_pos(float *addr, float val) {
   return __int_as_float(atomicMin(reinterpret_cast<int *>(addr), __float_as_int(val)));
 }
 
+// Slot in a list whose length lives at *counter (any address space), for every lane that calls
+// this at the same point: one atomic per wave, lanes numbered in lane order.
+__device__ __forceinline__ int wave_slot(int *counter) {
+  const unsigned long long act = __ballot(1);
+  const int lane = __lane_id();
+  const int leader = __ffsll((long long)act) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(counter, __popcll(act));
+  base = __shfl(base, leader, 64);
+  return base + __popcll(act & ((1ULL << lane) - 1ULL));
+}
+
+// Minimum over the wave (all 64 lanes active).
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    unsigned long long o = __shfl_xor(v, off, 64);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+// A 64-bit voxel word of the distance floods: high half = the distance's float bits, low half bit 0
+// = "not queued". ~0 (the shared sentinel) reads as +inf, not queued.
+__device__ __forceinline__ unsigned word_dist_bits(unsigned long long w) {
+  const unsigned hi = (unsigned)(w >> 32);
+  return hi == 0xFFFFFFFFu ? 0x7f800000u : hi;
+}
+
 // Shuffle of any integer width up by `delta` lanes (wave64).
 template <class T> __device__ __forceinline__ T shfl_up_any(T v, unsigned delta) {
   if constexpr (sizeof(T) == 8) return T(__shfl_up((unsigned long long)v, delta, 64));

@@ -157,8 +157,10 @@ Packed skeletonize(Context &ctx, const LabelVolume &in, const SkeletonizeOptions
   tables.max_daf = max_daf_dev.data();
   tables.weights = weights.data();
   tables.nbr = nbr.data();
-  prepare_fields(ctx, cc.labels.data(), cc.cc_bytes, d, tables, queue_dev.data(), int(queue.size()), daf.data(),
-                 state.data(), lists[0].data(), lists[1].data());
+  Buffer<unsigned long long> key(ctx, size_t(n));  // packed distance words / invalidation claims, ~0 between uses
+  key.fill_bytes(ctx, 0xff);
+  prepare_fields(ctx, d, tables, queue_dev.data(), int(queue.size()), key.data(), daf.data(), lists[0].data(),
+                 lists[1].data());
   root = root_dev.download(ctx);
   std::vector<long long> target = target_dev.download(ctx);
   std::vector<float> max_daf = max_daf_dev.download(ctx);
@@ -225,9 +227,7 @@ Packed skeletonize(Context &ctx, const LabelVolume &in, const SkeletonizeOptions
   mflat_dev.upload(ctx, mflat.data(), mflat.size());
 
   // ---- trace -----------------------------------------------------------------------------------
-  fill_value(ctx, daf.data(), n, kInf);  // the railroad distance field from here on
-  Buffer<unsigned long long> key(ctx, size_t(n)), snap_b(ctx, size_t(std::max<int64_t>(F, 1)));
-  key.fill_bytes(ctx, 0xff);
+  Buffer<unsigned long long> snap_b(ctx, size_t(std::max<int64_t>(F, 1)));
   const long long store_cap = std::max<long long>(int64_t(1) << 22, 4 * F);
   const int rec_cap = int(std::max<int64_t>(int64_t(1) << 16, F / 8 + 4 * int64_t(labels.size())));
   Buffer<int> store(ctx, size_t(store_cap)), rec_label(ctx, rec_cap), rec_len(ctx, rec_cap), rec_start(ctx, rec_cap);
@@ -239,7 +239,6 @@ Packed skeletonize(Context &ctx, const LabelVolume &in, const SkeletonizeOptions
   ta.nbr = nbr.data();
   ta.dbf = dbf.data();
   ta.pdrf = pdrf.data();
-  ta.dist = daf.data();
   ta.state = state.data();
   ta.key = key.data();
   ta.alive = alive.data();

@@ -63,47 +63,45 @@ __global__ void fill_float(float *p, long long n, float v) {
   if (i < n) p[i] = v;
 }
 
-// Geometric flood of label L from one source inside one workgroup. dist must be +inf on the
-// label's voxels and inq 0 on entry; on exit inq is 0 again.
-__device__ void flood_label(const unsigned *nbr, Dims d, int src, const float *W, float *dist, int *inq, int *cur,
+// Geometric flood of one label from one source inside one workgroup, on the packed voxel words
+// (distance bits high, "not queued" bit 0 low; ~0 = +inf, not queued). Words of the label must be
+// ~0 on entry; on exit they hold the distances, every voxel dequeued (bit 0 set).
+// Label-correcting rounds: a voxel is dequeued by one atomic that also returns its distance, and
+// an improvement that finds the voxel not queued queues it, so no update is ever lost.
+__device__ void flood_label(const unsigned *nbr, Dims d, int src, const float *W, unsigned long long *words, int *cur,
                             int *nxt, int *counts /*LDS [2]*/) {
+  const unsigned *hi_words = reinterpret_cast<const unsigned *>(words) + 1;
   if (threadIdx.x == 0) {
-    st_f(dist + src, 0.0f);
+    st_u64(words + src, 0ULL);  // distance 0, queued
     cur[0] = src;
     counts[0] = 1;
     counts[1] = 0;
   }
   __syncthreads();
   for (;;) {
-    int nc = counts[0];
+    const int nc = counts[0];
     if (nc == 0) break;
     for (int i = threadIdx.x; i < nc; i += blockDim.x) {
-      int u = ld_i(cur + i);
-      atomicExch(inq + u, 0);  // clear, THEN read the distance
-      float du = ld_f(dist + u);
+      const int u = ld_i(cur + i);
       const unsigned m = nbr[u];
+      const float du = __uint_as_float((unsigned)(atomicOr(words + u, 1ULL) >> 32));
       // all neighbours' loads, then all their atomics, in flight together
-      float seen[26];
+      unsigned seen[26];
       for_each_nb([&](auto K) {
         constexpr int k = decltype(K)::value;
-        seen[k] = (m >> k) & 1u ? ld_f(dist + u + nb_offset<k>(d)) : 0.0f;
+        seen[k] = (m >> k) & 1u ? ld_u(hi_words + 2 * (u + nb_offset<k>(d))) : 0u;
       });
-      float old[26];
+      unsigned long long old[26];
       for_each_nb([&](auto K) {
         constexpr int k = decltype(K)::value;
-        const float cand = du + W[k];
-        old[k] = cand;
-        if (((m >> k) & 1u) && cand < seen[k])
-          old[k] = __int_as_float(atomicMin(reinterpret_cast<int *>(dist + u + nb_offset<k>(d)), __float_as_int(cand)));
-      });
-      int was[26];
-      for_each_nb([&](auto K) {
-        constexpr int k = decltype(K)::value;
-        was[k] = old[k] > du + W[k] ? atomicExch(inq + u + nb_offset<k>(d), 1) : 1;
+        const unsigned cb = __float_as_uint(du + W[k]);
+        old[k] = 0;
+        if (cb < seen[k]) old[k] = atomicMin(words + u + nb_offset<k>(d), (unsigned long long)cb << 32);
       });
       for_each_nb([&](auto K) {
         constexpr int k = decltype(K)::value;
-        if (was[k] == 0) nxt[atomicAdd(&counts[1], 1)] = u + nb_offset<k>(d);
+        const unsigned cb = __float_as_uint(du + W[k]);
+        if ((old[k] & 1ULL) && old[k] > ((unsigned long long)cb << 32)) nxt[wave_slot(&counts[1])] = u + nb_offset<k>(d);
       });
     }
     __syncthreads();
@@ -113,17 +111,18 @@ __device__ void flood_label(const unsigned *nbr, Dims d, int src, const float *W
   }
 }
 
-// Argmax of dist over the label's voxel list: largest finite value, ties -> lowest flat index
-// (the packed key of batched_regular.inc:9-22). Returns the key (0: none).
-__device__ unsigned long long label_argmax(const float *dist, const int *list, int count, unsigned long long *lds) {
+// Argmax of the flooded distance over the label's voxel list: largest finite value, ties -> lowest
+// flat index (the packed key of batched_regular.inc:9-22). Returns the key (0: none).
+__device__ unsigned long long label_argmax(const unsigned long long *words, const int *list, int count,
+                                           unsigned long long *lds) {
   if (threadIdx.x == 0) *lds = 0;
   __syncthreads();
   unsigned long long best = 0;
   for (int j = threadIdx.x; j < count; j += blockDim.x) {
     int v = list[j];
-    float f = ld_f(dist + v);
-    if (!(f < kInf)) continue;
-    unsigned long long key = ((unsigned long long)__float_as_uint(f) << 32) | (unsigned long long)(0xFFFFFFFFu - (unsigned)v);
+    unsigned fb = word_dist_bits(ld_u64(words + v));
+    if (!(fb < 0x7f800000u)) continue;
+    unsigned long long key = ((unsigned long long)fb << 32) | (unsigned long long)(0xFFFFFFFFu - (unsigned)v);
     best = key > best ? key : best;
   }
   for (int off = 32; off > 0; off >>= 1) {
@@ -137,9 +136,8 @@ __device__ unsigned long long label_argmax(const float *dist, const int *list, i
   return r;
 }
 
-template <class CC>
-__global__ void __launch_bounds__(512) prepare_labels(const CC *cc, Dims d, LabelTables t, const int *queue, int nqueue,
-                                                      int *next, float *dist, int *inq, int *lists_a, int *lists_b) {
+__global__ void __launch_bounds__(512) prepare_labels(Dims d, LabelTables t, const int *queue, int nqueue, int *next,
+                                                      unsigned long long *words, float *daf, int *lists_a, int *lists_b) {
   __shared__ float W[26];
   __shared__ int counts[2];
   __shared__ unsigned long long best;
@@ -156,19 +154,25 @@ __global__ void __launch_bounds__(512) prepare_labels(const CC *cc, Dims d, Labe
     int *a = lists_a + s0, *b = lists_b + s0;
     long long root = t.root[L];
     if (root < 0) {  // no border target: flood from the component's first voxel, root = argmax
-      flood_label(t.nbr, d, int(t.first[L]), W, dist, inq, a, b, counts);
-      unsigned long long key = label_argmax(dist, list, cnt, &best);
+      flood_label(t.nbr, d, int(t.first[L]), W, words, a, b, counts);
+      unsigned long long key = label_argmax(words, list, cnt, &best);
       root = key ? (long long)(0xFFFFFFFFu - (unsigned)key) : -1;
-      for (int j = threadIdx.x; j < cnt; j += blockDim.x) st_f(dist + list[j], kInf);
+      for (int j = threadIdx.x; j < cnt; j += blockDim.x) st_u64(words + list[j], kMaxKey);
       __syncthreads();
       if (threadIdx.x == 0) t.root[L] = root;
     }
-    flood_label(t.nbr, d, int(root), W, dist, inq, a, b, counts);
-    unsigned long long key = label_argmax(dist, list, cnt, &best);
+    flood_label(t.nbr, d, int(root), W, words, a, b, counts);
+    unsigned long long key = label_argmax(words, list, cnt, &best);
     if (threadIdx.x == 0) {
       t.target[L] = key ? (long long)(0xFFFFFFFFu - (unsigned)key) : -1;
       t.max_daf[L] = key ? __uint_as_float((unsigned)(key >> 32)) : 0.0f;
     }
+    for (int j = threadIdx.x; j < cnt; j += blockDim.x) {  // DAF out, words back to ~0
+      const int v = list[j];
+      daf[v] = __uint_as_float(word_dist_bits(ld_u64(words + v)));
+      st_u64(words + v, kMaxKey);
+    }
+    __syncthreads();
   }
 }
 
@@ -255,22 +259,14 @@ void fill_value(Context &ctx, float *p, long long n, float v) {
   BROOK_LAUNCHED();
 }
 
-void prepare_fields(Context &ctx, const void *cc, int cc_bytes, Dims d, const LabelTables &t, const int *queue, int nqueue,
-                    float *dist, int *inq, int *lists_a, int *lists_b) {
+void prepare_fields(Context &ctx, Dims d, const LabelTables &t, const int *queue, int nqueue, unsigned long long *words,
+                    float *daf, int *lists_a, int *lists_b) {
   if (nqueue == 0) return;
   Buffer<int> next(ctx, 1);
   next.clear(ctx);
   int grid = std::min(nqueue, ctx.compute_units() * 2);
   KernelTimer kt(ctx, "prepare_labels");
-  if (cc_bytes == 1)
-    prepare_labels<uint8_t><<<grid, 512, 0, ctx.stream()>>>(static_cast<const uint8_t *>(cc), d, t, queue, nqueue,
-                                                            next.data(), dist, inq, lists_a, lists_b);
-  else if (cc_bytes == 2)
-    prepare_labels<uint16_t><<<grid, 512, 0, ctx.stream()>>>(static_cast<const uint16_t *>(cc), d, t, queue, nqueue,
-                                                             next.data(), dist, inq, lists_a, lists_b);
-  else
-    prepare_labels<uint32_t><<<grid, 512, 0, ctx.stream()>>>(static_cast<const uint32_t *>(cc), d, t, queue, nqueue,
-                                                             next.data(), dist, inq, lists_a, lists_b);
+  prepare_labels<<<grid, 512, 0, ctx.stream()>>>(d, t, queue, nqueue, next.data(), words, daf, lists_a, lists_b);
   BROOK_LAUNCHED();
   ctx.synchronize();
 }

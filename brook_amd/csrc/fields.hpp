@@ -26,9 +26,10 @@ void build_segments(Context &ctx, const void *cc, int cc_bytes, Dims d, const st
                     Buffer<int> &seg_start_dev, Buffer<int> &vox);
 void fill_value(Context &ctx, float *p, long long n, float v);
 // Root search and DAF floods for the ids in `queue` (one workgroup per id, largest first).
-// dist: +inf volume in, DAF out; inq: zero volume in and out; lists_*: scratch in segment layout.
-void prepare_fields(Context &ctx, const void *cc, int cc_bytes, Dims d, const LabelTables &t, const int *queue, int nqueue,
-                    float *dist, int *inq, int *lists_a, int *lists_b);
+// words: ~0 volume in and out (the packed distance words); daf: DAF written on the ids' voxels;
+// lists_*: scratch in segment layout.
+void prepare_fields(Context &ctx, Dims d, const LabelTables &t, const int *queue, int nqueue, unsigned long long *words,
+                    float *daf, int *lists_a, int *lists_b);
 void pdrf_and_alive(Context &ctx, const void *cc, int cc_bytes, long long n, const float *dbf, const float *daf,
                     const float *M, const float *inv, const unsigned char *use, int squarings, float scale, float *pdrf,
                     unsigned char *alive);

@@ -27,7 +27,7 @@ namespace brook {
 namespace {
 
 constexpr int kBlock = 512;
-constexpr int kDQ = 1, kDF = 2, kIQ = 4, kIW = 8;
+constexpr int kDF = 2, kIQ = 4, kIW = 8;  // state bits: far list, invalidation queue / wait
 constexpr unsigned long long kMinfarNone = 0x7f8000007f800000ULL;
 constexpr unsigned long long kClaimed = 0xFFFFFFFFFFFFFFFEULL;
 constexpr int kLaneFront = 16;  // fronts up to this size relax with one lane per neighbour
@@ -72,86 +72,98 @@ __device__ __forceinline__ void lap(Shared &sh, int slot) {
 }
 
 // ---- railroad -------------------------------------------------------------------------------
+// Distances live in the packed voxel words (a.key): high half the distance bits, bit 0 "not
+// queued". A front voxel is dequeued by one atomicOr that also returns its distance; an improvement
+// is one atomicMin whose old value says whether the voxel still has to be queued.
+// Rails (PDRF 0) and far candidates keep "not queued"; near candidates store "queued".
+// Per-thread minima of the reached rails and far distances are folded into LDS once per round.
+struct RoadMins {
+  unsigned long long rail = kMaxKey, far = kMinfarNone;
+};
+
+__device__ __forceinline__ void road_fold(Shared &sh, RoadMins &mins) {
+  const unsigned long long r = wave_min_u64(mins.rail), f = wave_min_u64(mins.far);
+  if (__lane_id() == 0) {
+    if (r != kMaxKey) atomicMin(&sh.rail_best, r);
+    if (f != kMinfarNone) atomicMin(&sh.minfar, f);
+  }
+  mins = RoadMins{};
+}
+
+// The outcome of one improvement attempt of v to cand: old is the word before the atomic.
+__device__ __forceinline__ void road_update(const TraceArgs &a, Shared &sh, RoadMins &mins, int v, float cand,
+                                            bool rail, bool near, float f, unsigned long long old, int *nxt, int *far,
+                                            int *touched) {
+  const unsigned cb = __float_as_uint(cand);
+  const unsigned long long nv = ((unsigned long long)cb << 32) | (near ? 0ULL : 1ULL);
+  if (!(old > nv)) return;  // nothing changed
+  if ((unsigned)(old >> 32) == 0xFFFFFFFFu) touched[wave_slot(&sh.n_touched)] = v;
+  if (rail) {  // a rail: fixes the cost and the last road voxel; never expanded
+    const unsigned long long key = ((unsigned long long)cb << 32) | (unsigned)v;
+    mins.rail = key < mins.rail ? key : mins.rail;
+  } else if (near) {
+    if (old & 1ULL) nxt[wave_slot(&sh.n_nxt)] = v;
+  } else {
+    const unsigned long long key = ((unsigned long long)cb << 32) | __float_as_uint(f);
+    mins.far = key < mins.far ? key : mins.far;
+    if (!(atomicOr(a.state + v, kDF) & kDF)) far[wave_slot(&sh.n_far)] = v;
+  }
+}
+
 // Relaxation of neighbour k (a set bit of u's mask) of u, which holds distance du.
-__device__ __forceinline__ void relax_one(const TraceArgs &a, Shared &sh, int u, int k, float du, float bound,
-                                          int *nxt, int *far, int *touched) {
+__device__ __forceinline__ void relax_one(const TraceArgs &a, Shared &sh, RoadMins &mins, int u, int k, float du,
+                                          float bound, int *nxt, int *far, int *touched) {
+  const unsigned *hi_words = reinterpret_cast<const unsigned *>(a.key) + 1;
   const int v = u + nb_offset(k, a.d);
   const float fv = ld_f(a.pdrf + v);
   const float cand = vw_step(du, fv);
-  const float old = atomic_min_pos(a.dist + v, cand);
-  if (!(old > cand)) return;
-  if (old >= kInf) touched[atomicAdd(&sh.n_touched, 1)] = v;
-  if (fv == 0.0f) {  // a rail: fixes the cost and the last road voxel; never expanded
-    atomicMin(&sh.rail_best, ((unsigned long long)__float_as_uint(cand) << 32) | (unsigned)v);
-    return;
-  }
-  if (cand <= bound) {
-    if (!(atomicOr(a.state + v, kDQ) & kDQ)) nxt[atomicAdd(&sh.n_nxt, 1)] = v;
-  } else {
-    atomicMin(&sh.minfar, ((unsigned long long)__float_as_uint(cand) << 32) | __float_as_uint(fv));
-    if (!(atomicOr(a.state + v, kDF) & kDF)) far[atomicAdd(&sh.n_far, 1)] = v;
-  }
+  const unsigned cb = __float_as_uint(cand);
+  if (ld_u(hi_words + 2 * v) <= cb || !(cand < kInf)) return;
+  const bool rail = fv == 0.0f, near = !rail && cand <= bound;
+  const unsigned long long old = atomicMin(a.key + v, ((unsigned long long)cb << 32) | (near ? 0ULL : 1ULL));
+  road_update(a, sh, mins, v, cand, rail, near, fv, old, nxt, far, touched);
 }
 
 // Masked neighbours B..E-1 of u at once: every load, then every atomic, is issued before any result
 // is waited for (a voxel costs a few memory round trips instead of one chain per neighbour).
 // A rail is recognised by cand == du (vw_step adds nothing only for a zero cost).
 template <int B, int E>
-__device__ __forceinline__ void relax_part(const TraceArgs &a, Shared &sh, int u, unsigned m, float du, float bound,
-                                           int *nxt, int *far, int *touched) {
-  float cand[E - B], seen[E - B];
+__device__ __forceinline__ void relax_part(const TraceArgs &a, Shared &sh, RoadMins &mins, int u, unsigned m, float du,
+                                           float bound, int *nxt, int *far, int *touched) {
+  const unsigned *hi_words = reinterpret_cast<const unsigned *>(a.key) + 1;
+  float cand[E - B];
+  unsigned seen[E - B];
   for_nb<B, E>([&](auto K) {
     constexpr int k = decltype(K)::value, i = k - B;
     const bool on = (m >> k) & 1u;
     cand[i] = on ? ld_f(a.pdrf + u + nb_offset<k>(a.d)) : kInf;
-    seen[i] = on ? ld_f(a.dist + u + nb_offset<k>(a.d)) : 0.0f;
+    seen[i] = on ? ld_u(hi_words + 2 * (u + nb_offset<k>(a.d))) : 0u;
   });
-  for_nb<B, E>([&](auto K) {  // seen[] becomes the previous value, or cand when nothing improves
+  unsigned long long old[E - B];
+  for_nb<B, E>([&](auto K) {
     constexpr int k = decltype(K)::value, i = k - B;
     const float c = vw_step(du, cand[i]);
     cand[i] = c;
-    const bool go = ((m >> k) & 1u) && c < seen[i];
-    seen[i] = go ? __int_as_float(atomicMin(reinterpret_cast<int *>(a.dist + u + nb_offset<k>(a.d)), __float_as_int(c)))
-                 : c;
+    const unsigned cb = __float_as_uint(c);
+    const bool near = c != du && c <= bound;
+    old[i] = 0;  // "nothing changed" unless the atomic says otherwise
+    if (cb < seen[i] && c < kInf) old[i] = atomicMin(a.key + u + nb_offset<k>(a.d), ((unsigned long long)cb << 32) | (near ? 0ULL : 1ULL));
   });
-  unsigned near = 0, farm = 0;
   for_nb<B, E>([&](auto K) {
     constexpr int k = decltype(K)::value, i = k - B;
     const float c = cand[i];
-    if (!(seen[i] > c)) return;
-    const int v = u + nb_offset<k>(a.d);
-    if (seen[i] >= kInf) touched[atomicAdd(&sh.n_touched, 1)] = v;
-    if (c == du) {  // rail
-      atomicMin(&sh.rail_best, ((unsigned long long)__float_as_uint(c) << 32) | (unsigned)v);
-    } else if (c <= bound) {
-      near |= 1u << k;
-    } else {
-      atomicMin(&sh.minfar, ((unsigned long long)__float_as_uint(c) << 32) | __float_as_uint(c - du));
-      farm |= 1u << k;
-    }
-  });
-  if (!(near | farm)) return;
-  int was[E - B];
-  for_nb<B, E>([&](auto K) {
-    constexpr int k = decltype(K)::value, i = k - B;
-    was[i] = -1;
-    if ((near | farm) >> k & 1u) was[i] = atomicOr(a.state + u + nb_offset<k>(a.d), (near >> k & 1u) ? kDQ : kDF);
-  });
-  for_nb<B, E>([&](auto K) {
-    constexpr int k = decltype(K)::value, i = k - B;
-    const int v = u + nb_offset<k>(a.d);
-    if ((near >> k & 1u) && !(was[i] & kDQ)) nxt[atomicAdd(&sh.n_nxt, 1)] = v;
-    if ((farm >> k & 1u) && !(was[i] & kDF)) far[atomicAdd(&sh.n_far, 1)] = v;
+    const bool rail = c == du, near = !rail && c <= bound;
+    road_update(a, sh, mins, u + nb_offset<k>(a.d), c, rail, near, c - du, old[i], nxt, far, touched);
   });
 }
 
-__device__ __forceinline__ void relax_all(const TraceArgs &a, Shared &sh, int u, unsigned m, float du, float bound,
-                                          int *nxt, int *far, int *touched) {
+__device__ __forceinline__ void relax_all(const TraceArgs &a, Shared &sh, RoadMins &mins, int u, unsigned m, float du,
+                                          float bound, int *nxt, int *far, int *touched) {
   if constexpr (kNbBatch >= 26) {
-    relax_part<0, 26>(a, sh, u, m, du, bound, nxt, far, touched);
+    relax_part<0, 26>(a, sh, mins, u, m, du, bound, nxt, far, touched);
   } else {
-    if (m & 0x1FFFu) relax_part<0, 13>(a, sh, u, m, du, bound, nxt, far, touched);
-    if (m >> 13) relax_part<13, 26>(a, sh, u, m, du, bound, nxt, far, touched);
+    if (m & 0x1FFFu) relax_part<0, 13>(a, sh, mins, u, m, du, bound, nxt, far, touched);
+    if (m >> 13) relax_part<13, 26>(a, sh, mins, u, m, du, bound, nxt, far, touched);
   }
 }
 
@@ -162,29 +174,35 @@ __device__ __forceinline__ float rail_bound(const Shared &sh) {
   return b;
 }
 
+// Dequeues front voxel u: marks it "not queued" and returns its distance, in one atomic.
+__device__ __forceinline__ float dequeue(const TraceArgs &a, int u) {
+  return __uint_as_float((unsigned)(atomicOr(a.key + u, 1ULL) >> 32));
+}
+
 // Returns 1 when the nearest rail is settled (sh.rail_best), 2 when no rail is reachable.
-__device__ __forceinline__ int railroad(const TraceArgs &a, Shared &sh, int t, int *near_a, int *near_b, int *far_a, int *far_b,
-                        int *touched) {
+__device__ __forceinline__ int railroad(const TraceArgs &a, Shared &sh, int t, int *near_a, int *near_b, int *far_a,
+                                        int *far_b, int *touched) {
   if (threadIdx.x == 0) {
-    st_f(a.dist + t, 0.0f);
     touched[0] = t;
     sh.n_touched = 1;
     sh.n_cur = sh.n_nxt = sh.n_far = sh.n_farnxt = 0;
     sh.minfar = kMinfarNone;
     float ft = ld_f(a.pdrf + t);
     if (ft == 0.0f) {
+      st_u64(a.key + t, 1ULL);  // distance 0, not queued
       sh.rail_best = (unsigned)t;
       sh.done = 1;
     } else {
+      st_u64(a.key + t, 0ULL);  // distance 0, queued
       sh.rail_best = kMaxKey;
       sh.done = 0;
       near_a[0] = t;
       sh.n_cur = 1;
-      atomicOr(a.state + t, kDQ);
       sh.T = fmaxf(2.0f * ft, 1e-30f);
     }
   }
   __syncthreads();
+  RoadMins mins;
   int *cur = near_a, *nxt = near_b, *fcur = far_a, *fnxt = far_b;
   while (sh.done == 0) {
     for (;;) {  // settle the near band
@@ -193,29 +211,29 @@ __device__ __forceinline__ int railroad(const TraceArgs &a, Shared &sh, int t, i
       const float bound = rail_bound(sh);
       if (nc <= kLaneFront) {
         if ((int)threadIdx.x < nc) {
-          int u = ld_i(cur + threadIdx.x);
-          atomicAnd(a.state + u, ~kDQ);  // clear, THEN read the distance
-          float du = ld_f(a.dist + u);
+          const int u = ld_i(cur + threadIdx.x);
+          const float du = dequeue(a, u);
           sh.lane_du[threadIdx.x] = du > bound ? -1.0f : du;
           sh.lane_u[threadIdx.x] = u;
           sh.lane_m[threadIdx.x] = a.nbr[u];
         }
         __syncthreads();
         for (int j = threadIdx.x; j < nc * 32; j += blockDim.x) {
-          int i = j >> 5, k = j & 31;
-          float du = sh.lane_du[i];
+          const int i = j >> 5, k = j & 31;
+          const float du = sh.lane_du[i];
           if (k >= 26 || du < 0.0f || !((sh.lane_m[i] >> k) & 1u)) continue;
-          relax_one(a, sh, sh.lane_u[i], k, du, bound, nxt, fcur, touched);
+          relax_one(a, sh, mins, sh.lane_u[i], k, du, bound, nxt, fcur, touched);
         }
       } else {
         for (int i = threadIdx.x; i < nc; i += blockDim.x) {
-          int u = ld_i(cur + i);
-          atomicAnd(a.state + u, ~kDQ);  // clear, THEN read the distance
-          float du = ld_f(a.dist + u);
+          const int u = ld_i(cur + i);
+          const unsigned m = a.nbr[u];
+          const float du = dequeue(a, u);
           if (du > bound) continue;
-          relax_all(a, sh, u, a.nbr[u], du, bound, nxt, fcur, touched);
+          relax_all(a, sh, mins, u, m, du, bound, nxt, fcur, touched);
         }
       }
+      road_fold(sh, mins);
       __syncthreads();
       if (threadIdx.x == 0) { sh.n_cur = sh.n_nxt; sh.n_nxt = 0; sh.prof[5] += 1; sh.prof[10] += nc; }
       int *tmp = cur; cur = nxt; nxt = tmp;
@@ -240,17 +258,19 @@ __device__ __forceinline__ int railroad(const TraceArgs &a, Shared &sh, int t, i
     const int nf = sh.n_far;
     const float bound = rail_bound(sh);
     for (int i = threadIdx.x; i < nf; i += blockDim.x) {  // promote far voxels inside the new band
-      int w = ld_i(fcur + i);
+      const int w = ld_i(fcur + i);
       if (!(ld_i(a.state + w) & kDF)) continue;
-      float dw = ld_f(a.dist + w);
+      const float dw = __uint_as_float((unsigned)(ld_u64(a.key + w) >> 32));
       if (dw <= bound) {
         atomicAnd(a.state + w, ~kDF);
-        if (!(atomicOr(a.state + w, kDQ) & kDQ)) cur[atomicAdd(&sh.n_cur, 1)] = w;
+        if (atomicAnd(a.key + w, ~1ULL) & 1ULL) cur[wave_slot(&sh.n_cur)] = w;
       } else {
-        atomicMin(&sh.minfar, ((unsigned long long)__float_as_uint(dw) << 32) | __float_as_uint(ld_f(a.pdrf + w)));
-        fnxt[atomicAdd(&sh.n_farnxt, 1)] = w;
+        const unsigned long long key = ((unsigned long long)__float_as_uint(dw) << 32) | __float_as_uint(ld_f(a.pdrf + w));
+        mins.far = key < mins.far ? key : mins.far;
+        fnxt[wave_slot(&sh.n_farnxt)] = w;
       }
     }
+    road_fold(sh, mins);
     __syncthreads();
     if (threadIdx.x == 0) { sh.n_far = sh.n_farnxt; sh.n_farnxt = 0; }
     int *tmp = fcur; fcur = fnxt; fnxt = tmp;
@@ -274,7 +294,7 @@ __device__ __forceinline__ void backtrace(const TraceArgs &a, Shared &sh, int t,
       float du = kInf;
       if (lane < 26 && ((m >> lane) & 1u)) {
         int u = cur + nb_offset(ok_lane, a.d);
-        if (ld_f(a.pdrf + u) != 0.0f) du = ld_f(a.dist + u);
+        if (ld_f(a.pdrf + u) != 0.0f) du = __uint_as_float(word_dist_bits(ld_u64(a.key + u)));
       }
       int db = __float_as_int(du);
       int mb = wave_min(db);
@@ -317,16 +337,16 @@ __device__ __forceinline__ void offer_to(const TraceArgs &a, Shared &sh, int v, 
     const unsigned long long cand = ((unsigned long long)mb << 32) | su;
     const unsigned long long old = issued ? old_or_none : (ld_u64(a.key + v) <= cand ? cand : atomicMin(a.key + v, cand));
     if (old > cand) {
-      if (old == kMaxKey) touched[atomicAdd(&sh.n_touched, 1)] = v;
+      if (old == kMaxKey) touched[wave_slot(&sh.n_touched)] = v;
       int was = atomicOr(a.state + v, kIQ);
       if (was & kIW) atomicAnd(a.state + v, ~kIW);
-      if (!(was & kIQ)) nxt[atomicAdd(&sh.n_nxt, 1)] = v;
+      if (!(was & kIQ)) nxt[wave_slot(&sh.n_nxt)] = v;
     }
   } else if (ld_u64(a.key + v) == kMaxKey) {  // beyond the band: wait for its band
     int seen = ld_i(a.state + v);
     while (!(seen & (kIQ | kIW))) {
       int got = atomicCAS(a.state + v, seen, seen | kIW);
-      if (got == seen) { far[atomicAdd(&sh.n_far, 1)] = v; break; }
+      if (got == seen) { far[wave_slot(&sh.n_far)] = v; break; }
       seen = got;
     }
   }
@@ -402,8 +422,8 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const
     if (!ld_u8(a.alive + p)) continue;
     if (atomicCAS(a.key + p, kMaxKey, (unsigned long long)(unsigned)p) == kMaxKey) {
       atomicOr(a.state + p, kIQ);
-      cur[atomicAdd(&sh.n_cur, 1)] = p;
-      touched[atomicAdd(&sh.n_touched, 1)] = p;
+      cur[wave_slot(&sh.n_cur)] = p;
+      touched[wave_slot(&sh.n_touched)] = p;
     }
   }
   __syncthreads();
@@ -487,12 +507,12 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const
       if (p == kMaxKey) { atomicAnd(a.state + c, ~kIW); continue; }
       if ((unsigned)(p >> 32) < hi) {
         st_u64(a.key + c, p);
-        touched[atomicAdd(&sh.n_touched, 1)] = c;
-        cur[atomicAdd(&sh.n_cur, 1)] = c;
+        touched[wave_slot(&sh.n_touched)] = c;
+        cur[wave_slot(&sh.n_cur)] = c;
         atomicOr(a.state + c, kIQ);
         atomicAnd(a.state + c, ~kIW);
       } else {
-        keep[atomicAdd(&sh.n_keep, 1)] = c;
+        keep[wave_slot(&sh.n_keep)] = c;
       }
     }
     __syncthreads();
@@ -589,8 +609,8 @@ __global__ void __launch_bounds__(kBlock) trace_labels(TraceArgs a) {
       const int nt = sh.n_touched;
       for (int j = threadIdx.x; j < nt; j += blockDim.x) {  // restore the railroad state
         int v = l4[j];
-        st_f(a.dist + v, kInf);
-        atomicAnd(a.state + v, ~(kDQ | kDF));
+        st_u64(a.key + v, kMaxKey);
+        st_i(a.state + v, 0);
       }
       if (status != 1 || sh.path_len < 0) {
         if (threadIdx.x == 0) a.ejected[atomicAdd(a.counters + 2, 1) % a.ejected_cap] = L;

@@ -11,9 +11,10 @@ struct TraceArgs {
   const unsigned *nbr = nullptr;  // per-voxel same-label neighbour masks (bit k = neighbour k)
   const float *dbf = nullptr;
   float *pdrf = nullptr;          // rails become 0 in place
-  float *dist = nullptr;          // +inf volume (restored after every railroad)
   int *state = nullptr;           // 0 volume: railroad / invalidation queue bits
-  unsigned long long *key = nullptr;  // ~0 volume: invalidation claims (restored)
+  // ~0 volume, restored after every use: railroad distance words (distance bits high, bit 0 "not
+  // queued") and invalidation claims (bottleneck bits high, source low)
+  unsigned long long *key = nullptr;
   unsigned char *alive = nullptr;
   const int *seg_start = nullptr, *seg_count = nullptr;
   const unsigned long long *order = nullptr;  // sorted order keys in segment layout (voxel = ~low 32 bits)
