@@ -104,6 +104,32 @@ __device__ __forceinline__ int wave_slot(int *counter) {
   return base + __popcll(act & ((1ULL << lane) - 1ULL));
 }
 
+// Reserves n slots per lane in a list whose length lives at *counter; returns the lane's first
+// slot. All 64 lanes must be active: one inclusive scan and one atomic per wave.
+__device__ __forceinline__ int wave_reserve(int *counter, int n) {
+  const int lane = __lane_id();
+  int x = n;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  const int total = __shfl(x, 63, 64);
+  int base = 0;
+  if (lane == 63 && total) base = atomicAdd(counter, total);
+  base = __shfl(base, 63, 64);
+  return base + x - n;
+}
+
+// Appends u + off[k] for every set bit k of mask (all 64 lanes active; off: neighbour offsets).
+__device__ __forceinline__ void emit_neighbours(int *list, int *counter, unsigned mask, int u, const int *off) {
+  int pos = wave_reserve(counter, __popc(mask));
+  while (mask) {
+    const int k = __ffs(mask) - 1;
+    mask &= mask - 1;
+    list[pos++] = u + off[k];
+  }
+}
+
 // Minimum over the wave (all 64 lanes active).
 __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v) {
   for (int off = 32; off > 0; off >>= 1) {

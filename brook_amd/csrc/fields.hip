@@ -68,8 +68,8 @@ __global__ void fill_float(float *p, long long n, float v) {
 // ~0 on entry; on exit they hold the distances, every voxel dequeued (bit 0 set).
 // Label-correcting rounds: a voxel is dequeued by one atomic that also returns its distance, and
 // an improvement that finds the voxel not queued queues it, so no update is ever lost.
-__device__ void flood_label(const unsigned *nbr, Dims d, int src, const float *W, unsigned long long *words, int *cur,
-                            int *nxt, int *counts /*LDS [2]*/) {
+__device__ void flood_label(const unsigned *nbr, Dims d, int src, const float *W, const int *off, unsigned long long *words,
+                            int *cur, int *nxt, int *counts /*LDS [2]*/) {
   const unsigned *hi_words = reinterpret_cast<const unsigned *>(words) + 1;
   if (threadIdx.x == 0) {
     st_u64(words + src, 0ULL);  // distance 0, queued
@@ -81,28 +81,34 @@ __device__ void flood_label(const unsigned *nbr, Dims d, int src, const float *W
   for (;;) {
     const int nc = counts[0];
     if (nc == 0) break;
-    for (int i = threadIdx.x; i < nc; i += blockDim.x) {
-      const int u = ld_i(cur + i);
-      const unsigned m = nbr[u];
-      const float du = __uint_as_float((unsigned)(atomicOr(words + u, 1ULL) >> 32));
-      // all neighbours' loads, then all their atomics, in flight together
-      unsigned seen[26];
-      for_each_nb([&](auto K) {
-        constexpr int k = decltype(K)::value;
-        seen[k] = (m >> k) & 1u ? ld_u(hi_words + 2 * (u + nb_offset<k>(d))) : 0u;
-      });
-      unsigned long long old[26];
-      for_each_nb([&](auto K) {
-        constexpr int k = decltype(K)::value;
-        const unsigned cb = __float_as_uint(du + W[k]);
-        old[k] = 0;
-        if (cb < seen[k]) old[k] = atomicMin(words + u + nb_offset<k>(d), (unsigned long long)cb << 32);
-      });
-      for_each_nb([&](auto K) {
-        constexpr int k = decltype(K)::value;
-        const unsigned cb = __float_as_uint(du + W[k]);
-        if ((old[k] & 1ULL) && old[k] > ((unsigned long long)cb << 32)) nxt[wave_slot(&counts[1])] = u + nb_offset<k>(d);
-      });
+    for (int base = 0; base < nc; base += blockDim.x) {  // uniform: every lane reaches the append
+      const int i = base + threadIdx.x;
+      unsigned push = 0;
+      int u = 0;
+      if (i < nc) {
+        u = ld_i(cur + i);
+        const unsigned m = nbr[u];
+        const float du = __uint_as_float((unsigned)(atomicOr(words + u, 1ULL) >> 32));
+        // all neighbours' loads, then all their atomics, in flight together
+        unsigned seen[26];
+        for_each_nb([&](auto K) {
+          constexpr int k = decltype(K)::value;
+          seen[k] = (m >> k) & 1u ? ld_u(hi_words + 2 * (u + nb_offset<k>(d))) : 0u;
+        });
+        unsigned long long old[26];
+        for_each_nb([&](auto K) {
+          constexpr int k = decltype(K)::value;
+          const unsigned cb = __float_as_uint(du + W[k]);
+          old[k] = 0;
+          if (cb < seen[k]) old[k] = atomicMin(words + u + nb_offset<k>(d), (unsigned long long)cb << 32);
+        });
+        for_each_nb([&](auto K) {
+          constexpr int k = decltype(K)::value;
+          const unsigned cb = __float_as_uint(du + W[k]);
+          if ((old[k] & 1ULL) && old[k] > ((unsigned long long)cb << 32)) push |= 1u << k;
+        });
+      }
+      emit_neighbours(nxt, &counts[1], push, u, off);
     }
     __syncthreads();
     if (threadIdx.x == 0) { counts[0] = counts[1]; counts[1] = 0; }
@@ -139,10 +145,14 @@ __device__ unsigned long long label_argmax(const unsigned long long *words, cons
 __global__ void __launch_bounds__(512) prepare_labels(Dims d, LabelTables t, const int *queue, int nqueue, int *next,
                                                       unsigned long long *words, float *daf, int *lists_a, int *lists_b) {
   __shared__ float W[26];
+  __shared__ int off[26];
   __shared__ int counts[2];
   __shared__ unsigned long long best;
   __shared__ int item;
-  if (threadIdx.x < 26) W[threadIdx.x] = t.weights[threadIdx.x];
+  if (threadIdx.x < 26) {
+    W[threadIdx.x] = t.weights[threadIdx.x];
+    off[threadIdx.x] = nb_offset(int(threadIdx.x), d);
+  }
   for (;;) {
     __syncthreads();
     if (threadIdx.x == 0) item = atomicAdd(next, 1);
@@ -154,14 +164,14 @@ __global__ void __launch_bounds__(512) prepare_labels(Dims d, LabelTables t, con
     int *a = lists_a + s0, *b = lists_b + s0;
     long long root = t.root[L];
     if (root < 0) {  // no border target: flood from the component's first voxel, root = argmax
-      flood_label(t.nbr, d, int(t.first[L]), W, words, a, b, counts);
+      flood_label(t.nbr, d, int(t.first[L]), W, off, words, a, b, counts);
       unsigned long long key = label_argmax(words, list, cnt, &best);
       root = key ? (long long)(0xFFFFFFFFu - (unsigned)key) : -1;
       for (int j = threadIdx.x; j < cnt; j += blockDim.x) st_u64(words + list[j], kMaxKey);
       __syncthreads();
       if (threadIdx.x == 0) t.root[L] = root;
     }
-    flood_label(t.nbr, d, int(root), W, words, a, b, counts);
+    flood_label(t.nbr, d, int(root), W, off, words, a, b, counts);
     unsigned long long key = label_argmax(words, list, cnt, &best);
     if (threadIdx.x == 0) {
       t.target[L] = key ? (long long)(0xFFFFFFFFu - (unsigned)key) : -1;

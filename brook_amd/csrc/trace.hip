@@ -52,6 +52,7 @@ struct Shared {
   float T;
   unsigned long long rail_best, minfar;
   int store_pos;
+  int off[26];  // flat offsets of the 26 neighbours
   float lane_du[kLaneFront * 2];
   int lane_u[kLaneFront * 2];
   unsigned lane_m[kLaneFront * 2];
@@ -81,6 +82,11 @@ struct RoadMins {
   unsigned long long rail = kMaxKey, far = kMinfarNone;
 };
 
+// Neighbours (bit k = neighbour k) to append to the touched, near and far lists.
+struct Pushes {
+  unsigned touched = 0, near = 0, far = 0;
+};
+
 __device__ __forceinline__ void road_fold(Shared &sh, RoadMins &mins) {
   const unsigned long long r = wave_min_u64(mins.rail), f = wave_min_u64(mins.far);
   if (__lane_id() == 0) {
@@ -90,12 +96,19 @@ __device__ __forceinline__ void road_fold(Shared &sh, RoadMins &mins) {
   mins = RoadMins{};
 }
 
-// The outcome of one improvement attempt of v to cand: old is the word before the atomic.
-__device__ __forceinline__ void road_update(const TraceArgs &a, Shared &sh, RoadMins &mins, int v, float cand,
-                                            bool rail, bool near, float f, unsigned long long old, int *nxt, int *far,
-                                            int *touched) {
+// Relaxation of neighbour k (a set bit of u's mask) of u, which holds distance du; one lane per
+// neighbour, list slots claimed per wave.
+__device__ __forceinline__ void relax_one(const TraceArgs &a, Shared &sh, RoadMins &mins, int u, int k, float du,
+                                          float bound, int *nxt, int *far, int *touched) {
+  const unsigned *hi_words = reinterpret_cast<const unsigned *>(a.key) + 1;
+  const int v = u + sh.off[k];
+  const float fv = ld_f(a.pdrf + v);
+  const float cand = vw_step(du, fv);
   const unsigned cb = __float_as_uint(cand);
+  if (ld_u(hi_words + 2 * v) <= cb || !(cand < kInf)) return;
+  const bool rail = fv == 0.0f, near = !rail && cand <= bound;
   const unsigned long long nv = ((unsigned long long)cb << 32) | (near ? 0ULL : 1ULL);
+  const unsigned long long old = atomicMin(a.key + v, nv);
   if (!(old > nv)) return;  // nothing changed
   if ((unsigned)(old >> 32) == 0xFFFFFFFFu) touched[wave_slot(&sh.n_touched)] = v;
   if (rail) {  // a rail: fixes the cost and the last road voxel; never expanded
@@ -104,66 +117,78 @@ __device__ __forceinline__ void road_update(const TraceArgs &a, Shared &sh, Road
   } else if (near) {
     if (old & 1ULL) nxt[wave_slot(&sh.n_nxt)] = v;
   } else {
-    const unsigned long long key = ((unsigned long long)cb << 32) | __float_as_uint(f);
+    const unsigned long long key = ((unsigned long long)cb << 32) | __float_as_uint(fv);
     mins.far = key < mins.far ? key : mins.far;
     if (!(atomicOr(a.state + v, kDF) & kDF)) far[wave_slot(&sh.n_far)] = v;
   }
 }
 
-// Relaxation of neighbour k (a set bit of u's mask) of u, which holds distance du.
-__device__ __forceinline__ void relax_one(const TraceArgs &a, Shared &sh, RoadMins &mins, int u, int k, float du,
-                                          float bound, int *nxt, int *far, int *touched) {
-  const unsigned *hi_words = reinterpret_cast<const unsigned *>(a.key) + 1;
-  const int v = u + nb_offset(k, a.d);
-  const float fv = ld_f(a.pdrf + v);
-  const float cand = vw_step(du, fv);
-  const unsigned cb = __float_as_uint(cand);
-  if (ld_u(hi_words + 2 * v) <= cb || !(cand < kInf)) return;
-  const bool rail = fv == 0.0f, near = !rail && cand <= bound;
-  const unsigned long long old = atomicMin(a.key + v, ((unsigned long long)cb << 32) | (near ? 0ULL : 1ULL));
-  road_update(a, sh, mins, v, cand, rail, near, fv, old, nxt, far, touched);
-}
-
 // Masked neighbours B..E-1 of u at once: every load, then every atomic, is issued before any result
-// is waited for (a voxel costs a few memory round trips instead of one chain per neighbour).
+// is waited for; the outcome is a set of bit masks (list appends happen once per loop iteration).
 // A rail is recognised by cand == du (vw_step adds nothing only for a zero cost).
 template <int B, int E>
-__device__ __forceinline__ void relax_part(const TraceArgs &a, Shared &sh, RoadMins &mins, int u, unsigned m, float du,
-                                           float bound, int *nxt, int *far, int *touched) {
+__device__ __forceinline__ void relax_part(const TraceArgs &a, RoadMins &mins, int u, unsigned m, float du, float bound,
+                                           Pushes &p) {
   const unsigned *hi_words = reinterpret_cast<const unsigned *>(a.key) + 1;
-  float cand[E - B];
+  float c[E - B];
   unsigned seen[E - B];
   for_nb<B, E>([&](auto K) {
     constexpr int k = decltype(K)::value, i = k - B;
     const bool on = (m >> k) & 1u;
-    cand[i] = on ? ld_f(a.pdrf + u + nb_offset<k>(a.d)) : kInf;
+    c[i] = on ? ld_f(a.pdrf + u + nb_offset<k>(a.d)) : kInf;
     seen[i] = on ? ld_u(hi_words + 2 * (u + nb_offset<k>(a.d))) : 0u;
   });
   unsigned long long old[E - B];
   for_nb<B, E>([&](auto K) {
     constexpr int k = decltype(K)::value, i = k - B;
-    const float c = vw_step(du, cand[i]);
-    cand[i] = c;
-    const unsigned cb = __float_as_uint(c);
-    const bool near = c != du && c <= bound;
+    const float cc = vw_step(du, c[i]);
+    c[i] = cc;
+    const unsigned cb = __float_as_uint(cc);
+    const bool near = cc != du && cc <= bound;
     old[i] = 0;  // "nothing changed" unless the atomic says otherwise
-    if (cb < seen[i] && c < kInf) old[i] = atomicMin(a.key + u + nb_offset<k>(a.d), ((unsigned long long)cb << 32) | (near ? 0ULL : 1ULL));
+    if (cb < seen[i] && cc < kInf)
+      old[i] = atomicMin(a.key + u + nb_offset<k>(a.d), ((unsigned long long)cb << 32) | (near ? 0ULL : 1ULL));
   });
+  unsigned farc = 0;
   for_nb<B, E>([&](auto K) {
     constexpr int k = decltype(K)::value, i = k - B;
-    const float c = cand[i];
-    const bool rail = c == du, near = !rail && c <= bound;
-    road_update(a, sh, mins, u + nb_offset<k>(a.d), c, rail, near, c - du, old[i], nxt, far, touched);
+    const float cc = c[i];
+    const unsigned cb = __float_as_uint(cc);
+    const bool rail = cc == du, near = !rail && cc <= bound;
+    const unsigned long long nv = ((unsigned long long)cb << 32) | (near ? 0ULL : 1ULL);
+    if (!(old[i] > nv)) return;
+    if ((unsigned)(old[i] >> 32) == 0xFFFFFFFFu) p.touched |= 1u << k;
+    if (rail) {
+      const unsigned long long key = ((unsigned long long)cb << 32) | (unsigned)(u + nb_offset<k>(a.d));
+      mins.rail = key < mins.rail ? key : mins.rail;
+    } else if (near) {
+      if (old[i] & 1ULL) p.near |= 1u << k;
+    } else {
+      const unsigned long long key = ((unsigned long long)cb << 32) | __float_as_uint(cc - du);
+      mins.far = key < mins.far ? key : mins.far;
+      farc |= 1u << k;
+    }
   });
+  if (farc) {
+    int was[E - B];
+    for_nb<B, E>([&](auto K) {
+      constexpr int k = decltype(K)::value, i = k - B;
+      was[i] = (farc >> k) & 1u ? atomicOr(a.state + u + nb_offset<k>(a.d), kDF) : kDF;
+    });
+    for_nb<B, E>([&](auto K) {
+      constexpr int k = decltype(K)::value, i = k - B;
+      if (!(was[i] & kDF)) p.far |= 1u << k;
+    });
+  }
 }
 
-__device__ __forceinline__ void relax_all(const TraceArgs &a, Shared &sh, RoadMins &mins, int u, unsigned m, float du,
-                                          float bound, int *nxt, int *far, int *touched) {
+__device__ __forceinline__ void relax_all(const TraceArgs &a, RoadMins &mins, int u, unsigned m, float du, float bound,
+                                          Pushes &p) {
   if constexpr (kNbBatch >= 26) {
-    relax_part<0, 26>(a, sh, mins, u, m, du, bound, nxt, far, touched);
+    relax_part<0, 26>(a, mins, u, m, du, bound, p);
   } else {
-    if (m & 0x1FFFu) relax_part<0, 13>(a, sh, mins, u, m, du, bound, nxt, far, touched);
-    if (m >> 13) relax_part<13, 26>(a, sh, mins, u, m, du, bound, nxt, far, touched);
+    if (m & 0x1FFFu) relax_part<0, 13>(a, mins, u, m, du, bound, p);
+    if (m >> 13) relax_part<13, 26>(a, mins, u, m, du, bound, p);
   }
 }
 
@@ -225,12 +250,19 @@ __device__ __forceinline__ int railroad(const TraceArgs &a, Shared &sh, int t, i
           relax_one(a, sh, mins, sh.lane_u[i], k, du, bound, nxt, fcur, touched);
         }
       } else {
-        for (int i = threadIdx.x; i < nc; i += blockDim.x) {
-          const int u = ld_i(cur + i);
-          const unsigned m = a.nbr[u];
-          const float du = dequeue(a, u);
-          if (du > bound) continue;
-          relax_all(a, sh, mins, u, m, du, bound, nxt, fcur, touched);
+        for (int base = 0; base < nc; base += blockDim.x) {  // uniform: every lane reaches the appends
+          const int i = base + threadIdx.x;
+          Pushes p;
+          int u = 0;
+          if (i < nc) {
+            u = ld_i(cur + i);
+            const unsigned m = a.nbr[u];
+            const float du = dequeue(a, u);
+            if (du <= bound) relax_all(a, mins, u, m, du, bound, p);
+          }
+          emit_neighbours(touched, &sh.n_touched, p.touched, u, sh.off);
+          emit_neighbours(nxt, &sh.n_nxt, p.near, u, sh.off);
+          emit_neighbours(fcur, &sh.n_far, p.far, u, sh.off);
         }
       }
       road_fold(sh, mins);
@@ -354,11 +386,12 @@ __device__ __forceinline__ void offer_to(const TraceArgs &a, Shared &sh, int v, 
 
 // Masked neighbours B..E-1 of u at once: alive and key loads, then claim atomics, in flight
 // together. Only the key's high word (the bottleneck) is pre-loaded: a claim whose bottleneck is
-// above it cannot win and issues no atomic.
+// above it cannot win and issues no atomic. The outcome is a set of bit masks; the rarer waits
+// beyond the band run as a rolled loop.
 template <int B, int E>
-__device__ __forceinline__ void offer_part(const TraceArgs &a, Shared &sh, int u, unsigned m, int x, int y, int z,
+__device__ __forceinline__ void offer_part(const TraceArgs &a, const int *off, int u, unsigned m, int x, int y, int z,
                                            int qx, int qy, int qz, unsigned su, unsigned mu, float maxd, unsigned hi,
-                                           int *nxt, int *far, int *touched) {
+                                           Pushes &p) {
   const unsigned *key_hi = reinterpret_cast<const unsigned *>(a.key) + 1;
   unsigned char al[E - B];
   unsigned kh[E - B];
@@ -368,39 +401,71 @@ __device__ __forceinline__ void offer_part(const TraceArgs &a, Shared &sh, int u
     al[i] = on ? ld_u8(a.alive + u + nb_offset<k>(a.d)) : (unsigned char)0;
     kh[i] = on ? ld_u(key_hi + 2 * (u + nb_offset<k>(a.d))) : 0u;
   });
-  float dist[E - B];
-  unsigned ok = 0, claim = 0;
+  unsigned mbs[E - B];
+  unsigned wait = 0;
   unsigned long long old[E - B];
   for_nb<B, E>([&](auto K) {
     constexpr int k = decltype(K)::value, i = k - B;
-    dist[i] = offer_distance(a, x + kDX[k], y + kDY[k], z + kDZ[k], qx, qy, qz);
-    const unsigned mb = max(__float_as_uint(dist[i]), mu);
+    const float d = offer_distance(a, x + kDX[k], y + kDY[k], z + kDZ[k], qx, qy, qz);
+    const unsigned mb = max(__float_as_uint(d), mu);  // M = max(M[u], d)
+    mbs[i] = mb;
     old[i] = 0;  // "no improvement" unless the atomic says otherwise
-    if (al[i] && dist[i] < maxd) {
-      ok |= 1u << k;
-      if (mb < hi) {
-        claim |= 1u << k;
-        if (kh[i] >= mb) old[i] = atomicMin(a.key + u + nb_offset<k>(a.d), ((unsigned long long)mb << 32) | su);
-      }
+    if (al[i] && d < maxd) {
+      if (mb >= hi) wait |= 1u << k;
+      else if (kh[i] >= mb) old[i] = atomicMin(a.key + u + nb_offset<k>(a.d), ((unsigned long long)mb << 32) | su);
     }
   });
+  unsigned won = 0;
   for_nb<B, E>([&](auto K) {
     constexpr int k = decltype(K)::value, i = k - B;
-    if ((ok >> k) & 1u)
-      offer_to(a, sh, u + nb_offset<k>(a.d), dist[i], su, mu, hi, old[i], (claim >> k) & 1u, nxt, far, touched);
+    if (old[i] > (((unsigned long long)mbs[i] << 32) | su)) {
+      won |= 1u << k;
+      if (old[i] == kMaxKey) p.touched |= 1u << k;
+    }
   });
+  if (won) {
+    int was[E - B];
+    for_nb<B, E>([&](auto K) {
+      constexpr int k = decltype(K)::value, i = k - B;
+      was[i] = (won >> k) & 1u ? atomicOr(a.state + u + nb_offset<k>(a.d), kIQ) : kIQ;
+    });
+    unsigned clear = 0;
+    for_nb<B, E>([&](auto K) {
+      constexpr int k = decltype(K)::value, i = k - B;
+      if (!((won >> k) & 1u)) return;
+      if (was[i] & kIW) clear |= 1u << k;
+      if (!(was[i] & kIQ)) p.near |= 1u << k;
+    });
+    while (clear) {
+      const int k = __ffs(clear) - 1;
+      clear &= clear - 1;
+      atomicAnd(a.state + u + off[k], ~kIW);
+    }
+  }
+  while (wait) {  // beyond the band: wait for its band
+    const int k = __ffs(wait) - 1;
+    wait &= wait - 1;
+    const int v = u + off[k];
+    if (ld_u64(a.key + v) != kMaxKey) continue;
+    int seen = ld_i(a.state + v);
+    while (!(seen & (kIQ | kIW))) {
+      const int got = atomicCAS(a.state + v, seen, seen | kIW);
+      if (got == seen) { p.far |= 1u << k; break; }
+      seen = got;
+    }
+  }
 }
 
-__device__ __forceinline__ void offer_all(const TraceArgs &a, Shared &sh, int u, unsigned m, unsigned su, unsigned mu,
-                                          float maxd, unsigned hi, int *nxt, int *far, int *touched) {
+__device__ __forceinline__ void offer_all(const TraceArgs &a, const int *off, int u, unsigned m, unsigned su, unsigned mu,
+                                          float maxd, unsigned hi, Pushes &p) {
   int x, y, z, qx, qy, qz;
   unravel(u, a.d, x, y, z);
   unravel(int(su), a.d, qx, qy, qz);
   if constexpr (kNbBatch >= 26) {
-    offer_part<0, 26>(a, sh, u, m, x, y, z, qx, qy, qz, su, mu, maxd, hi, nxt, far, touched);
+    offer_part<0, 26>(a, off, u, m, x, y, z, qx, qy, qz, su, mu, maxd, hi, p);
   } else {
-    if (m & 0x1FFFu) offer_part<0, 13>(a, sh, u, m, x, y, z, qx, qy, qz, su, mu, maxd, hi, nxt, far, touched);
-    if (m >> 13) offer_part<13, 26>(a, sh, u, m, x, y, z, qx, qy, qz, su, mu, maxd, hi, nxt, far, touched);
+    if (m & 0x1FFFu) offer_part<0, 13>(a, off, u, m, x, y, z, qx, qy, qz, su, mu, maxd, hi, p);
+    if (m >> 13) offer_part<13, 26>(a, off, u, m, x, y, z, qx, qy, qz, su, mu, maxd, hi, p);
   }
 }
 
@@ -438,30 +503,39 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const
         atomicAnd(a.state + u, ~kIQ);
       }
       __syncthreads();
-      const bool lanes = nc <= kLaneFront * 2;
-      const int items = lanes ? nc * 32 : nc;
-      for (int j = threadIdx.x; j < items; j += blockDim.x) {
-        const int i = lanes ? (j >> 5) : j;
-        const int k = lanes ? (j & 31) : 0;
-        if (k >= 26) continue;
-        const int u = ld_i(cur + i);
-        const unsigned m = a.nbr[u];
-        const unsigned long long ku = snap[i];
-        const unsigned su = (unsigned)ku, mu = (unsigned)(ku >> 32);
-        const float maxd = a.scale * a.dbf[su] + a.constant;
-        if (!lanes) {
-          offer_all(a, sh, u, m, su, mu, maxd, hi, nxt, far, touched);
-          continue;
+      if (nc <= kLaneFront * 2) {  // small front: one lane per neighbour
+        for (int j = threadIdx.x; j < nc * 32; j += blockDim.x) {
+          const int i = j >> 5, k = j & 31;
+          if (k >= 26) continue;
+          const int u = ld_i(cur + i);
+          if (!((a.nbr[u] >> k) & 1u)) continue;
+          const int v = u + sh.off[k];
+          if (!ld_u8(a.alive + v)) continue;
+          const unsigned long long ku = snap[i];
+          const unsigned su = (unsigned)ku, mu = (unsigned)(ku >> 32);
+          int x, y, z, qx, qy, qz;
+          unravel(v, a.d, x, y, z);
+          unravel(int(su), a.d, qx, qy, qz);
+          const float d = offer_distance(a, x, y, z, qx, qy, qz);
+          if (d >= a.scale * a.dbf[su] + a.constant) continue;
+          offer_to(a, sh, v, d, su, mu, hi, 0, false, nxt, far, touched);
         }
-        if (!((m >> k) & 1u)) continue;
-        const int v = u + nb_offset(k, a.d);
-        if (!ld_u8(a.alive + v)) continue;
-        int x, y, z, qx, qy, qz;
-        unravel(v, a.d, x, y, z);
-        unravel(int(su), a.d, qx, qy, qz);
-        const float d = offer_distance(a, x, y, z, qx, qy, qz);
-        if (d >= maxd) continue;
-        offer_to(a, sh, v, d, su, mu, hi, 0, false, nxt, far, touched);
+      } else {
+        for (int base = 0; base < nc; base += blockDim.x) {  // uniform: every lane reaches the appends
+          const int i = base + threadIdx.x;
+          Pushes p;
+          int u = 0;
+          if (i < nc) {
+            u = ld_i(cur + i);
+            const unsigned m = a.nbr[u];
+            const unsigned long long ku = snap[i];
+            const unsigned su = (unsigned)ku, mu = (unsigned)(ku >> 32);
+            offer_all(a, sh.off, u, m, su, mu, a.scale * a.dbf[su] + a.constant, hi, p);
+          }
+          emit_neighbours(touched, &sh.n_touched, p.touched, u, sh.off);
+          emit_neighbours(nxt, &sh.n_nxt, p.near, u, sh.off);
+          emit_neighbours(far, &sh.n_far, p.far, u, sh.off);
+        }
       }
       __syncthreads();
       if (threadIdx.x == 0) { sh.n_cur = sh.n_nxt; sh.n_nxt = 0; sh.prof[6] += 1; sh.prof[11] += nc; }
@@ -578,6 +652,7 @@ __device__ __forceinline__ int next_target(const TraceArgs &a, Shared &sh, int L
 __global__ void __launch_bounds__(kBlock) trace_labels(TraceArgs a) {
   __shared__ Shared sh;
   __shared__ int item;
+  if (threadIdx.x < 26) sh.off[threadIdx.x] = nb_offset(int(threadIdx.x), a.d);
   for (;;) {
     __syncthreads();
     if (threadIdx.x == 0) item = atomicAdd(a.next, 1);
