@@ -75,6 +75,23 @@ __device__ __forceinline__ void st_u64(unsigned long long *p, unsigned long long
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Loads and stores of per-voxel data that, during one kernel, only the workgroup owning the voxel's
+// component label touches (floods, tracer): workgroup scope is all the memory model needs, and the
+// accesses may then be served by the CU's vector L1 instead of always going to L2.
+#ifndef BROOK_LABEL_SCOPE
+#define BROOK_LABEL_SCOPE __HIP_MEMORY_SCOPE_WORKGROUP
+#endif
+__device__ __forceinline__ float lp_f(const float *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, BROOK_LABEL_SCOPE); }
+__device__ __forceinline__ int lp_i(const int *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, BROOK_LABEL_SCOPE); }
+__device__ __forceinline__ unsigned lp_u(const unsigned *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, BROOK_LABEL_SCOPE); }
+__device__ __forceinline__ unsigned long long lp_u64(const unsigned long long *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, BROOK_LABEL_SCOPE);
+}
+__device__ __forceinline__ void sp_f(float *p, float v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, BROOK_LABEL_SCOPE); }
+__device__ __forceinline__ void sp_u64(unsigned long long *p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, BROOK_LABEL_SCOPE);
+}
+
 // One vertex-weighted step fl(d + f) that never returns d itself for a positive cost: one ulp up
 // instead (the rule of the reference's brook_vw_step, common.cuh:123-127), which keeps distances
 // strictly increasing along every step so predecessor walks terminate.
@@ -137,6 +154,27 @@ __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
     v = o < v ? o : v;
   }
   return v;
+}
+
+// Everything the floods and the tracer touch per voxel, in one 16-byte record (one cache line holds
+// eight voxels of a row): a neighbour costs one line instead of one per array.
+//   pdrf  the penalty field (rails 0)
+//   meta  bits [0,26) the same-label neighbour mask, then kAlive, kFarListed (railroad far list),
+//         kQueued / kWaiting (invalidation queue, waiting for a later band)
+//   word  a distance word (see word_dist_bits) or an invalidation claim; ~0 between uses
+struct alignas(16) VoxelRec {
+  float pdrf;
+  unsigned meta;
+  unsigned long long word;
+};
+constexpr unsigned kNbrBits = (1u << 26) - 1u;
+constexpr unsigned kAlive = 1u << 26, kFarListed = 1u << 27, kQueued = 1u << 28, kWaiting = 1u << 29;
+
+__device__ __forceinline__ float rec_pdrf(const VoxelRec *r, int v) { return lp_f(&r[v].pdrf); }
+__device__ __forceinline__ unsigned rec_meta(const VoxelRec *r, int v) { return lp_u(&r[v].meta); }
+// The high half of the word (distance bits or claim bottleneck) without the 64-bit load.
+__device__ __forceinline__ unsigned rec_word_hi(const VoxelRec *r, int v) {
+  return lp_u(reinterpret_cast<const unsigned *>(&r[v].word) + 1);
 }
 
 // A 64-bit voxel word of the distance floods: high half = the distance's float bits, low half bit 0

@@ -140,11 +140,9 @@ Packed skeletonize(Context &ctx, const LabelVolume &in, const SkeletonizeOptions
   Buffer<float> weights(ctx, 26);
   weights.upload(ctx, W.data(), 26);
   Buffer<float> daf(ctx, size_t(n));
-  Buffer<int> state(ctx, size_t(n));
   fill_value(ctx, daf.data(), n, kInf);
-  state.clear(ctx);
-  Buffer<unsigned> nbr(ctx, size_t(n));
-  neighbour_masks(ctx, cc.labels.data(), cc.cc_bytes, d, nbr.data());
+  Buffer<VoxelRec> rec(ctx, size_t(n));  // 16 bytes per voxel: PDRF, neighbour mask + flags, distance word
+  voxel_records(ctx, cc.labels.data(), cc.cc_bytes, d, rec.data());
   std::vector<Buffer<int>> lists;
   for (int i = 0; i < 6; ++i) lists.emplace_back(ctx, size_t(std::max<int64_t>(F, 1)));
   LabelTables tables;
@@ -156,11 +154,8 @@ Packed skeletonize(Context &ctx, const LabelVolume &in, const SkeletonizeOptions
   tables.target = target_dev.data();
   tables.max_daf = max_daf_dev.data();
   tables.weights = weights.data();
-  tables.nbr = nbr.data();
-  Buffer<unsigned long long> key(ctx, size_t(n));  // packed distance words / invalidation claims, ~0 between uses
-  key.fill_bytes(ctx, 0xff);
-  prepare_fields(ctx, d, tables, queue_dev.data(), int(queue.size()), key.data(), daf.data(), lists[0].data(),
-                 lists[1].data());
+  tables.rec = rec.data();
+  prepare_fields(ctx, d, tables, queue_dev.data(), int(queue.size()), daf.data(), lists[0].data(), lists[1].data());
   root = root_dev.download(ctx);
   std::vector<long long> target = target_dev.download(ctx);
   std::vector<float> max_daf = max_daf_dev.download(ctx);
@@ -182,9 +177,8 @@ Packed skeletonize(Context &ctx, const LabelVolume &in, const SkeletonizeOptions
   inv_dev.upload(ctx, inv.data(), k);
   use_dev.upload(ctx, use.data(), k);
   Buffer<float> pdrf(ctx, size_t(n));
-  Buffer<uint8_t> alive(ctx, size_t(n));
   pdrf_and_alive(ctx, cc.labels.data(), cc.cc_bytes, n, dbf.data(), daf.data(), M_dev.data(), inv_dev.data(),
-                 use_dev.data(), squarings, float(o.teasar.pdrf_scale), pdrf.data(), alive.data());
+                 use_dev.data(), squarings, float(o.teasar.pdrf_scale), pdrf.data(), rec.data());
   Buffer<int> labels_dev(ctx, std::max<size_t>(labels.size(), 1));
   labels_dev.upload(ctx, labels.data(), labels.size());
   if (fields) {
@@ -200,7 +194,7 @@ Packed skeletonize(Context &ctx, const LabelVolume &in, const SkeletonizeOptions
     if (fields->pdrf) BROOK_HIP(hipMemcpy(fields->pdrf, pdrf.data(), size_t(n) * 4, hipMemcpyDeviceToHost));
     if (fields->stop) return empty_result(o);
   }
-  root_rails(ctx, pdrf.data(), root_dev.data(), labels_dev.data(), int(labels.size()));
+  root_rails(ctx, rec.data(), root_dev.data(), labels_dev.data(), int(labels.size()));
   Buffer<unsigned long long> order(ctx, size_t(std::max<int64_t>(F, 1))), snap_a(ctx, size_t(std::max<int64_t>(F, 1)));
   target_order_keys(ctx, vox.data(), F, daf.data(), snap_a.data());
   sort_segments(ctx, snap_a.data(), order.data(), F, seg_start_dev.data(), seg_start_dev.data() + 1, int(k));
@@ -236,12 +230,8 @@ Packed skeletonize(Context &ctx, const LabelVolume &in, const SkeletonizeOptions
   next.clear(ctx);
   TraceArgs ta;
   ta.d = d;
-  ta.nbr = nbr.data();
+  ta.rec = rec.data();
   ta.dbf = dbf.data();
-  ta.pdrf = pdrf.data();
-  ta.state = state.data();
-  ta.key = key.data();
-  ta.alive = alive.data();
   ta.seg_start = seg_start_dev.data();
   ta.seg_count = seg_count_dev.data();
   ta.order = order.data();

@@ -2,7 +2,7 @@
 //   * label voxel lists (segments laid out by id, from one wave-aggregated atomic per run),
 //   * one workgroup per label from a device work queue: root-search flood + argmax, DAF flood +
 //     argmax (label-gated geometric SSSP, the unique float32 min-plus fixpoint), no grid barriers,
-//   * PDRF and the alive mask in one streaming pass, the root rails, the target-order keys.
+//   * PDRF and the alive bit in one streaming pass, the root rails, the target-order keys.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -40,22 +40,28 @@ __global__ void __launch_bounds__(256) fill_segments(const CC *cc, int n, Dims d
   if (valid && l != CC(0)) vox[seg_start[l] + base + (lane - my_start)] = i;
 }
 
-// Neighbour mask: bit k of mask[v] = neighbour k of v is inside the volume and has v's (nonzero) id.
-// Every later neighbour walk reads this one word instead of 26 ids and bounds checks.
+// Voxel records: the neighbour mask (bit k = neighbour k of v is inside the volume and has v's
+// nonzero id; every later neighbour walk reads this instead of 26 ids and bounds checks),
+// PDRF +inf, word ~0.
 template <class CC>
-__global__ void __launch_bounds__(256) mask_kernel(const CC *cc, Dims d, long long n, unsigned *mask) {
+__global__ void __launch_bounds__(256) init_records(const CC *cc, Dims d, long long n, VoxelRec *rec) {
   long long i = (long long)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   CC l = cc[i];
-  if (l == CC(0)) { mask[i] = 0u; return; }
-  int x, y, z;
-  unravel(int(i), d, x, y, z);
   unsigned m = 0;
-  for_each_nb([&](auto K) {
-    constexpr int k = decltype(K)::value;
-    if (inside(x + kDX[k], y + kDY[k], z + kDZ[k], d) && cc[i + nb_offset<k>(d)] == l) m |= 1u << k;
-  });
-  mask[i] = m;
+  if (l != CC(0)) {
+    int x, y, z;
+    unravel(int(i), d, x, y, z);
+    for_each_nb([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      if (inside(x + kDX[k], y + kDY[k], z + kDZ[k], d) && cc[i + nb_offset<k>(d)] == l) m |= 1u << k;
+    });
+  }
+  VoxelRec r;
+  r.pdrf = kInf;
+  r.meta = m;
+  r.word = kMaxKey;
+  rec[i] = r;
 }
 
 __global__ void fill_float(float *p, long long n, float v) {
@@ -63,16 +69,15 @@ __global__ void fill_float(float *p, long long n, float v) {
   if (i < n) p[i] = v;
 }
 
-// Geometric flood of one label from one source inside one workgroup, on the packed voxel words
+// Geometric flood of one label from one source inside one workgroup, on the records' packed words
 // (distance bits high, "not queued" bit 0 low; ~0 = +inf, not queued). Words of the label must be
 // ~0 on entry; on exit they hold the distances, every voxel dequeued (bit 0 set).
 // Label-correcting rounds: a voxel is dequeued by one atomic that also returns its distance, and
 // an improvement that finds the voxel not queued queues it, so no update is ever lost.
-__device__ void flood_label(const unsigned *nbr, Dims d, int src, const float *W, const int *off, unsigned long long *words,
-                            int *cur, int *nxt, int *counts /*LDS [2]*/) {
-  const unsigned *hi_words = reinterpret_cast<const unsigned *>(words) + 1;
+__device__ void flood_label(VoxelRec *rec, Dims d, int src, const float *W, const int *off, int *cur, int *nxt,
+                            int *counts /*LDS [2]*/) {
   if (threadIdx.x == 0) {
-    st_u64(words + src, 0ULL);  // distance 0, queued
+    sp_u64(&rec[src].word, 0ULL);  // distance 0, queued
     cur[0] = src;
     counts[0] = 1;
     counts[1] = 0;
@@ -86,21 +91,21 @@ __device__ void flood_label(const unsigned *nbr, Dims d, int src, const float *W
       unsigned push = 0;
       int u = 0;
       if (i < nc) {
-        u = ld_i(cur + i);
-        const unsigned m = nbr[u];
-        const float du = __uint_as_float((unsigned)(atomicOr(words + u, 1ULL) >> 32));
+        u = lp_i(cur + i);
+        const unsigned m = rec_meta(rec, u) & kNbrBits;
+        const float du = __uint_as_float((unsigned)(atomicOr(&rec[u].word, 1ULL) >> 32));
         // all neighbours' loads, then all their atomics, in flight together
         unsigned seen[26];
         for_each_nb([&](auto K) {
           constexpr int k = decltype(K)::value;
-          seen[k] = (m >> k) & 1u ? ld_u(hi_words + 2 * (u + nb_offset<k>(d))) : 0u;
+          seen[k] = (m >> k) & 1u ? rec_word_hi(rec, u + nb_offset<k>(d)) : 0u;
         });
         unsigned long long old[26];
         for_each_nb([&](auto K) {
           constexpr int k = decltype(K)::value;
           const unsigned cb = __float_as_uint(du + W[k]);
           old[k] = 0;
-          if (cb < seen[k]) old[k] = atomicMin(words + u + nb_offset<k>(d), (unsigned long long)cb << 32);
+          if (cb < seen[k]) old[k] = atomicMin(&rec[u + nb_offset<k>(d)].word, (unsigned long long)cb << 32);
         });
         for_each_nb([&](auto K) {
           constexpr int k = decltype(K)::value;
@@ -119,14 +124,13 @@ __device__ void flood_label(const unsigned *nbr, Dims d, int src, const float *W
 
 // Argmax of the flooded distance over the label's voxel list: largest finite value, ties -> lowest
 // flat index (the packed key of batched_regular.inc:9-22). Returns the key (0: none).
-__device__ unsigned long long label_argmax(const unsigned long long *words, const int *list, int count,
-                                           unsigned long long *lds) {
+__device__ unsigned long long label_argmax(const VoxelRec *rec, const int *list, int count, unsigned long long *lds) {
   if (threadIdx.x == 0) *lds = 0;
   __syncthreads();
   unsigned long long best = 0;
   for (int j = threadIdx.x; j < count; j += blockDim.x) {
     int v = list[j];
-    unsigned fb = word_dist_bits(ld_u64(words + v));
+    unsigned fb = word_dist_bits(lp_u64(&rec[v].word));
     if (!(fb < 0x7f800000u)) continue;
     unsigned long long key = ((unsigned long long)fb << 32) | (unsigned long long)(0xFFFFFFFFu - (unsigned)v);
     best = key > best ? key : best;
@@ -143,7 +147,7 @@ __device__ unsigned long long label_argmax(const unsigned long long *words, cons
 }
 
 __global__ void __launch_bounds__(512) prepare_labels(Dims d, LabelTables t, const int *queue, int nqueue, int *next,
-                                                      unsigned long long *words, float *daf, int *lists_a, int *lists_b) {
+                                                      float *daf, int *lists_a, int *lists_b) {
   __shared__ float W[26];
   __shared__ int off[26];
   __shared__ int counts[2];
@@ -164,23 +168,23 @@ __global__ void __launch_bounds__(512) prepare_labels(Dims d, LabelTables t, con
     int *a = lists_a + s0, *b = lists_b + s0;
     long long root = t.root[L];
     if (root < 0) {  // no border target: flood from the component's first voxel, root = argmax
-      flood_label(t.nbr, d, int(t.first[L]), W, off, words, a, b, counts);
-      unsigned long long key = label_argmax(words, list, cnt, &best);
+      flood_label(t.rec, d, int(t.first[L]), W, off, a, b, counts);
+      unsigned long long key = label_argmax(t.rec, list, cnt, &best);
       root = key ? (long long)(0xFFFFFFFFu - (unsigned)key) : -1;
-      for (int j = threadIdx.x; j < cnt; j += blockDim.x) st_u64(words + list[j], kMaxKey);
+      for (int j = threadIdx.x; j < cnt; j += blockDim.x) sp_u64(&t.rec[list[j]].word, kMaxKey);
       __syncthreads();
       if (threadIdx.x == 0) t.root[L] = root;
     }
-    flood_label(t.nbr, d, int(root), W, off, words, a, b, counts);
-    unsigned long long key = label_argmax(words, list, cnt, &best);
+    flood_label(t.rec, d, int(root), W, off, a, b, counts);
+    unsigned long long key = label_argmax(t.rec, list, cnt, &best);
     if (threadIdx.x == 0) {
       t.target[L] = key ? (long long)(0xFFFFFFFFu - (unsigned)key) : -1;
       t.max_daf[L] = key ? __uint_as_float((unsigned)(key >> 32)) : 0.0f;
     }
     for (int j = threadIdx.x; j < cnt; j += blockDim.x) {  // DAF out, words back to ~0
       const int v = list[j];
-      daf[v] = __uint_as_float(word_dist_bits(ld_u64(words + v)));
-      st_u64(words + v, kMaxKey);
+      daf[v] = __uint_as_float(word_dist_bits(lp_u64(&t.rec[v].word)));
+      sp_u64(&t.rec[v].word, kMaxKey);
     }
     __syncthreads();
   }
@@ -190,12 +194,11 @@ __global__ void __launch_bounds__(512) prepare_labels(Dims d, LabelTables t, con
 template <class CC>
 __global__ void __launch_bounds__(256) pdrf_alive(const CC *cc, const float *dbf, const float *daf, const float *M,
                                                   const float *inv, const unsigned char *use, int squarings, float scale,
-                                                  long long n, float *pdrf, unsigned char *alive) {
+                                                  long long n, float *pdrf, VoxelRec *rec) {
   long long i = (long long)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   CC l = cc[i];
   unsigned char u = use[l];
-  alive[i] = (l != CC(0) && u) ? 1 : 0;
   if (l == CC(0) || u == 0) { pdrf[i] = kInf; return; }
   float dv = dbf[i];
   if (dv == 0.0f) dv = kInf;
@@ -208,11 +211,13 @@ __global__ void __launch_bounds__(256) pdrf_alive(const CC *cc, const float *dbf
     p = p + a * inv[l];
   }
   pdrf[i] = p;
+  rec[i].pdrf = p;
+  rec[i].meta |= kAlive;
 }
 
-__global__ void set_rails(float *pdrf, const long long *root, const int *labels, int count) {
+__global__ void set_rails(VoxelRec *rec, const long long *root, const int *labels, int count) {
   int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < count) pdrf[root[labels[i]]] = 0.0f;
+  if (i < count) rec[root[labels[i]]].pdrf = 0.0f;
 }
 
 // Target-order key of each listed voxel: ascending keys = DAF descending (inf and 0 as 0),
@@ -250,16 +255,16 @@ void build_segments(Context &ctx, const void *cc, int cc_bytes, Dims d, const st
   BROOK_LAUNCHED();
 }
 
-void neighbour_masks(Context &ctx, const void *cc, int cc_bytes, Dims d, unsigned *mask) {
+void voxel_records(Context &ctx, const void *cc, int cc_bytes, Dims d, VoxelRec *rec) {
   const long long n = (long long)d.sx * d.sy * d.sz;
   if (n == 0) return;
-  KernelTimer kt(ctx, "neighbour_masks");
+  KernelTimer kt(ctx, "init_records");
   if (cc_bytes == 1)
-    mask_kernel<uint8_t><<<grid_for(n), 256, 0, ctx.stream()>>>(static_cast<const uint8_t *>(cc), d, n, mask);
+    init_records<uint8_t><<<grid_for(n), 256, 0, ctx.stream()>>>(static_cast<const uint8_t *>(cc), d, n, rec);
   else if (cc_bytes == 2)
-    mask_kernel<uint16_t><<<grid_for(n), 256, 0, ctx.stream()>>>(static_cast<const uint16_t *>(cc), d, n, mask);
+    init_records<uint16_t><<<grid_for(n), 256, 0, ctx.stream()>>>(static_cast<const uint16_t *>(cc), d, n, rec);
   else
-    mask_kernel<uint32_t><<<grid_for(n), 256, 0, ctx.stream()>>>(static_cast<const uint32_t *>(cc), d, n, mask);
+    init_records<uint32_t><<<grid_for(n), 256, 0, ctx.stream()>>>(static_cast<const uint32_t *>(cc), d, n, rec);
   BROOK_LAUNCHED();
 }
 
@@ -269,37 +274,37 @@ void fill_value(Context &ctx, float *p, long long n, float v) {
   BROOK_LAUNCHED();
 }
 
-void prepare_fields(Context &ctx, Dims d, const LabelTables &t, const int *queue, int nqueue, unsigned long long *words,
-                    float *daf, int *lists_a, int *lists_b) {
+void prepare_fields(Context &ctx, Dims d, const LabelTables &t, const int *queue, int nqueue, float *daf, int *lists_a,
+                    int *lists_b) {
   if (nqueue == 0) return;
   Buffer<int> next(ctx, 1);
   next.clear(ctx);
   int grid = std::min(nqueue, ctx.compute_units() * 2);
   KernelTimer kt(ctx, "prepare_labels");
-  prepare_labels<<<grid, 512, 0, ctx.stream()>>>(d, t, queue, nqueue, next.data(), words, daf, lists_a, lists_b);
+  prepare_labels<<<grid, 512, 0, ctx.stream()>>>(d, t, queue, nqueue, next.data(), daf, lists_a, lists_b);
   BROOK_LAUNCHED();
   ctx.synchronize();
 }
 
 void pdrf_and_alive(Context &ctx, const void *cc, int cc_bytes, long long n, const float *dbf, const float *daf,
                     const float *M, const float *inv, const unsigned char *use, int squarings, float scale, float *pdrf,
-                    unsigned char *alive) {
+                    VoxelRec *rec) {
   KernelTimer kt(ctx, "pdrf_alive");
   if (cc_bytes == 1)
     pdrf_alive<uint8_t><<<grid_for(n), 256, 0, ctx.stream()>>>(static_cast<const uint8_t *>(cc), dbf, daf, M, inv, use,
-                                                               squarings, scale, n, pdrf, alive);
+                                                               squarings, scale, n, pdrf, rec);
   else if (cc_bytes == 2)
     pdrf_alive<uint16_t><<<grid_for(n), 256, 0, ctx.stream()>>>(static_cast<const uint16_t *>(cc), dbf, daf, M, inv, use,
-                                                                squarings, scale, n, pdrf, alive);
+                                                                squarings, scale, n, pdrf, rec);
   else
     pdrf_alive<uint32_t><<<grid_for(n), 256, 0, ctx.stream()>>>(static_cast<const uint32_t *>(cc), dbf, daf, M, inv, use,
-                                                                squarings, scale, n, pdrf, alive);
+                                                                squarings, scale, n, pdrf, rec);
   BROOK_LAUNCHED();
 }
 
-void root_rails(Context &ctx, float *pdrf, const long long *root, const int *labels, int count) {
+void root_rails(Context &ctx, VoxelRec *rec, const long long *root, const int *labels, int count) {
   if (!count) return;
-  set_rails<<<grid_for(count), 256, 0, ctx.stream()>>>(pdrf, root, labels, count);
+  set_rails<<<grid_for(count), 256, 0, ctx.stream()>>>(rec, root, labels, count);
   BROOK_LAUNCHED();
 }
 

@@ -16,24 +16,24 @@ struct LabelTables {
   long long *target = nullptr;     // out: DAF argmax (lowest flat index on ties)
   float *max_daf = nullptr;        // out: DAF maximum
   const float *weights = nullptr;  // 26 geometric step weights
-  const unsigned *nbr = nullptr;   // per-voxel same-label neighbour masks
+  VoxelRec *rec = nullptr;         // voxel records (masks in; words ~0 in and out)
 };
 
-// mask[v] bit k = neighbour k of v is inside the volume and has v's nonzero id.
-void neighbour_masks(Context &ctx, const void *cc, int cc_bytes, Dims d, unsigned *mask);
+// rec[v]: neighbour mask (bit k = neighbour k of v is inside the volume and has v's nonzero id),
+// PDRF +inf, word ~0, no flags.
+void voxel_records(Context &ctx, const void *cc, int cc_bytes, Dims d, VoxelRec *rec);
 
 void build_segments(Context &ctx, const void *cc, int cc_bytes, Dims d, const std::vector<int> &seg_start,
                     Buffer<int> &seg_start_dev, Buffer<int> &vox);
 void fill_value(Context &ctx, float *p, long long n, float v);
 // Root search and DAF floods for the ids in `queue` (one workgroup per id, largest first).
-// words: ~0 volume in and out (the packed distance words); daf: DAF written on the ids' voxels;
-// lists_*: scratch in segment layout.
-void prepare_fields(Context &ctx, Dims d, const LabelTables &t, const int *queue, int nqueue, unsigned long long *words,
-                    float *daf, int *lists_a, int *lists_b);
+// t.rec words ~0 in and out; daf: DAF written on the ids' voxels; lists_*: scratch in segment layout.
+void prepare_fields(Context &ctx, Dims d, const LabelTables &t, const int *queue, int nqueue, float *daf, int *lists_a,
+                    int *lists_b);
 void pdrf_and_alive(Context &ctx, const void *cc, int cc_bytes, long long n, const float *dbf, const float *daf,
                     const float *M, const float *inv, const unsigned char *use, int squarings, float scale, float *pdrf,
-                    unsigned char *alive);
-void root_rails(Context &ctx, float *pdrf, const long long *root, const int *labels, int count);
+                    VoxelRec *rec);
+void root_rails(Context &ctx, VoxelRec *rec, const long long *root, const int *labels, int count);
 void target_order_keys(Context &ctx, const int *vox, long long total, const float *daf, unsigned long long *keys);
 // Segmented ascending sort of 64-bit keys (segments = id voxel lists).
 void sort_segments(Context &ctx, const unsigned long long *in, unsigned long long *out, long long total,

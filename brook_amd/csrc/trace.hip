@@ -26,17 +26,36 @@
 namespace brook {
 namespace {
 
-constexpr int kBlock = 512;
-constexpr int kDF = 2, kIQ = 4, kIW = 8;  // state bits: far list, invalidation queue / wait
+#ifndef BROOK_TRACE_BLOCK
+#define BROOK_TRACE_BLOCK 1024
+#endif
+constexpr int kBlock = BROOK_TRACE_BLOCK;  // threads per label workgroup
 constexpr unsigned long long kMinfarNone = 0x7f8000007f800000ULL;
 constexpr unsigned long long kClaimed = 0xFFFFFFFFFFFFFFFEULL;
 constexpr int kLaneFront = 16;  // fronts up to this size relax with one lane per neighbour
 constexpr int kProf = kTraceProfileSlots;
 #ifndef BROOK_NB_BATCH
-#define BROOK_NB_BATCH 13
+#define BROOK_NB_BATCH 9
 #endif
-// Neighbours whose memory operations one thread keeps in flight together (13: two halves; 26: all).
+// Neighbours whose memory operations one thread keeps in flight together (26: all; 13: two parts;
+// 9: three parts). Fewer in flight, fewer registers.
 constexpr int kNbBatch = BROOK_NB_BATCH;
+
+// Calls f(integral_constant B, integral_constant E) for each part [B, E) of the 26 neighbours that
+// has a set bit in m.
+template <class F> __device__ __forceinline__ void for_parts(unsigned m, F &&f) {
+  using std::integral_constant;
+  if constexpr (kNbBatch >= 26) {
+    if (m) f(integral_constant<int, 0>{}, integral_constant<int, 26>{});
+  } else if constexpr (kNbBatch >= 13) {
+    if (m & 0x1FFFu) f(integral_constant<int, 0>{}, integral_constant<int, 13>{});
+    if (m >> 13) f(integral_constant<int, 13>{}, integral_constant<int, 26>{});
+  } else {
+    if (m & 0x1FFu) f(integral_constant<int, 0>{}, integral_constant<int, 9>{});
+    if ((m >> 9) & 0x1FFu) f(integral_constant<int, 9>{}, integral_constant<int, 18>{});
+    if (m >> 18) f(integral_constant<int, 18>{}, integral_constant<int, 26>{});
+  }
+}
 
 __device__ __forceinline__ int wave_min(int v) {
   for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
@@ -73,7 +92,7 @@ __device__ __forceinline__ void lap(Shared &sh, int slot) {
 }
 
 // ---- railroad -------------------------------------------------------------------------------
-// Distances live in the packed voxel words (a.key): high half the distance bits, bit 0 "not
+// Distances live in the records' packed words: high half the distance bits, bit 0 "not
 // queued". A front voxel is dequeued by one atomicOr that also returns its distance; an improvement
 // is one atomicMin whose old value says whether the voxel still has to be queued.
 // Rails (PDRF 0) and far candidates keep "not queued"; near candidates store "queued".
@@ -100,15 +119,14 @@ __device__ __forceinline__ void road_fold(Shared &sh, RoadMins &mins) {
 // neighbour, list slots claimed per wave.
 __device__ __forceinline__ void relax_one(const TraceArgs &a, Shared &sh, RoadMins &mins, int u, int k, float du,
                                           float bound, int *nxt, int *far, int *touched) {
-  const unsigned *hi_words = reinterpret_cast<const unsigned *>(a.key) + 1;
   const int v = u + sh.off[k];
-  const float fv = ld_f(a.pdrf + v);
+  const float fv = rec_pdrf(a.rec, v);
   const float cand = vw_step(du, fv);
   const unsigned cb = __float_as_uint(cand);
-  if (ld_u(hi_words + 2 * v) <= cb || !(cand < kInf)) return;
+  if (rec_word_hi(a.rec, v) <= cb || !(cand < kInf)) return;
   const bool rail = fv == 0.0f, near = !rail && cand <= bound;
   const unsigned long long nv = ((unsigned long long)cb << 32) | (near ? 0ULL : 1ULL);
-  const unsigned long long old = atomicMin(a.key + v, nv);
+  const unsigned long long old = atomicMin(&a.rec[v].word, nv);
   if (!(old > nv)) return;  // nothing changed
   if ((unsigned)(old >> 32) == 0xFFFFFFFFu) touched[wave_slot(&sh.n_touched)] = v;
   if (rail) {  // a rail: fixes the cost and the last road voxel; never expanded
@@ -119,7 +137,7 @@ __device__ __forceinline__ void relax_one(const TraceArgs &a, Shared &sh, RoadMi
   } else {
     const unsigned long long key = ((unsigned long long)cb << 32) | __float_as_uint(fv);
     mins.far = key < mins.far ? key : mins.far;
-    if (!(atomicOr(a.state + v, kDF) & kDF)) far[wave_slot(&sh.n_far)] = v;
+    if (!(atomicOr(&a.rec[v].meta, kFarListed) & kFarListed)) far[wave_slot(&sh.n_far)] = v;
   }
 }
 
@@ -129,14 +147,13 @@ __device__ __forceinline__ void relax_one(const TraceArgs &a, Shared &sh, RoadMi
 template <int B, int E>
 __device__ __forceinline__ void relax_part(const TraceArgs &a, RoadMins &mins, int u, unsigned m, float du, float bound,
                                            Pushes &p) {
-  const unsigned *hi_words = reinterpret_cast<const unsigned *>(a.key) + 1;
   float c[E - B];
   unsigned seen[E - B];
   for_nb<B, E>([&](auto K) {
     constexpr int k = decltype(K)::value, i = k - B;
     const bool on = (m >> k) & 1u;
-    c[i] = on ? ld_f(a.pdrf + u + nb_offset<k>(a.d)) : kInf;
-    seen[i] = on ? ld_u(hi_words + 2 * (u + nb_offset<k>(a.d))) : 0u;
+    c[i] = on ? rec_pdrf(a.rec, u + nb_offset<k>(a.d)) : kInf;
+    seen[i] = on ? rec_word_hi(a.rec, (u + nb_offset<k>(a.d))) : 0u;
   });
   unsigned long long old[E - B];
   for_nb<B, E>([&](auto K) {
@@ -147,7 +164,7 @@ __device__ __forceinline__ void relax_part(const TraceArgs &a, RoadMins &mins, i
     const bool near = cc != du && cc <= bound;
     old[i] = 0;  // "nothing changed" unless the atomic says otherwise
     if (cb < seen[i] && cc < kInf)
-      old[i] = atomicMin(a.key + u + nb_offset<k>(a.d), ((unsigned long long)cb << 32) | (near ? 0ULL : 1ULL));
+      old[i] = atomicMin(&a.rec[u + nb_offset<k>(a.d)].word, ((unsigned long long)cb << 32) | (near ? 0ULL : 1ULL));
   });
   unsigned farc = 0;
   for_nb<B, E>([&](auto K) {
@@ -170,26 +187,21 @@ __device__ __forceinline__ void relax_part(const TraceArgs &a, RoadMins &mins, i
     }
   });
   if (farc) {
-    int was[E - B];
+    unsigned was[E - B];
     for_nb<B, E>([&](auto K) {
       constexpr int k = decltype(K)::value, i = k - B;
-      was[i] = (farc >> k) & 1u ? atomicOr(a.state + u + nb_offset<k>(a.d), kDF) : kDF;
+      was[i] = (farc >> k) & 1u ? atomicOr(&a.rec[u + nb_offset<k>(a.d)].meta, kFarListed) : kFarListed;
     });
     for_nb<B, E>([&](auto K) {
       constexpr int k = decltype(K)::value, i = k - B;
-      if (!(was[i] & kDF)) p.far |= 1u << k;
+      if (!(was[i] & kFarListed)) p.far |= 1u << k;
     });
   }
 }
 
 __device__ __forceinline__ void relax_all(const TraceArgs &a, RoadMins &mins, int u, unsigned m, float du, float bound,
                                           Pushes &p) {
-  if constexpr (kNbBatch >= 26) {
-    relax_part<0, 26>(a, mins, u, m, du, bound, p);
-  } else {
-    if (m & 0x1FFFu) relax_part<0, 13>(a, mins, u, m, du, bound, p);
-    if (m >> 13) relax_part<13, 26>(a, mins, u, m, du, bound, p);
-  }
+  for_parts(m, [&](auto B, auto E) { relax_part<decltype(B)::value, decltype(E)::value>(a, mins, u, m, du, bound, p); });
 }
 
 __device__ __forceinline__ float rail_bound(const Shared &sh) {
@@ -201,7 +213,7 @@ __device__ __forceinline__ float rail_bound(const Shared &sh) {
 
 // Dequeues front voxel u: marks it "not queued" and returns its distance, in one atomic.
 __device__ __forceinline__ float dequeue(const TraceArgs &a, int u) {
-  return __uint_as_float((unsigned)(atomicOr(a.key + u, 1ULL) >> 32));
+  return __uint_as_float((unsigned)(atomicOr(&a.rec[u].word, 1ULL) >> 32));
 }
 
 // Returns 1 when the nearest rail is settled (sh.rail_best), 2 when no rail is reachable.
@@ -212,13 +224,13 @@ __device__ __forceinline__ int railroad(const TraceArgs &a, Shared &sh, int t, i
     sh.n_touched = 1;
     sh.n_cur = sh.n_nxt = sh.n_far = sh.n_farnxt = 0;
     sh.minfar = kMinfarNone;
-    float ft = ld_f(a.pdrf + t);
+    float ft = rec_pdrf(a.rec, t);
     if (ft == 0.0f) {
-      st_u64(a.key + t, 1ULL);  // distance 0, not queued
+      sp_u64(&a.rec[t].word, 1ULL);  // distance 0, not queued
       sh.rail_best = (unsigned)t;
       sh.done = 1;
     } else {
-      st_u64(a.key + t, 0ULL);  // distance 0, queued
+      sp_u64(&a.rec[t].word, 0ULL);  // distance 0, queued
       sh.rail_best = kMaxKey;
       sh.done = 0;
       near_a[0] = t;
@@ -236,11 +248,11 @@ __device__ __forceinline__ int railroad(const TraceArgs &a, Shared &sh, int t, i
       const float bound = rail_bound(sh);
       if (nc <= kLaneFront) {
         if ((int)threadIdx.x < nc) {
-          const int u = ld_i(cur + threadIdx.x);
+          const int u = lp_i(cur + threadIdx.x);
           const float du = dequeue(a, u);
           sh.lane_du[threadIdx.x] = du > bound ? -1.0f : du;
           sh.lane_u[threadIdx.x] = u;
-          sh.lane_m[threadIdx.x] = a.nbr[u];
+          sh.lane_m[threadIdx.x] = (rec_meta(a.rec, u) & kNbrBits);
         }
         __syncthreads();
         for (int j = threadIdx.x; j < nc * 32; j += blockDim.x) {
@@ -255,8 +267,8 @@ __device__ __forceinline__ int railroad(const TraceArgs &a, Shared &sh, int t, i
           Pushes p;
           int u = 0;
           if (i < nc) {
-            u = ld_i(cur + i);
-            const unsigned m = a.nbr[u];
+            u = lp_i(cur + i);
+            const unsigned m = (rec_meta(a.rec, u) & kNbrBits);
             const float du = dequeue(a, u);
             if (du <= bound) relax_all(a, mins, u, m, du, bound, p);
           }
@@ -290,14 +302,14 @@ __device__ __forceinline__ int railroad(const TraceArgs &a, Shared &sh, int t, i
     const int nf = sh.n_far;
     const float bound = rail_bound(sh);
     for (int i = threadIdx.x; i < nf; i += blockDim.x) {  // promote far voxels inside the new band
-      const int w = ld_i(fcur + i);
-      if (!(ld_i(a.state + w) & kDF)) continue;
-      const float dw = __uint_as_float((unsigned)(ld_u64(a.key + w) >> 32));
+      const int w = lp_i(fcur + i);
+      if (!(rec_meta(a.rec, w) & kFarListed)) continue;
+      const float dw = __uint_as_float((unsigned)(lp_u64(&a.rec[w].word) >> 32));
       if (dw <= bound) {
-        atomicAnd(a.state + w, ~kDF);
-        if (atomicAnd(a.key + w, ~1ULL) & 1ULL) cur[wave_slot(&sh.n_cur)] = w;
+        atomicAnd(&a.rec[w].meta, ~kFarListed);
+        if (atomicAnd(&a.rec[w].word, ~1ULL) & 1ULL) cur[wave_slot(&sh.n_cur)] = w;
       } else {
-        const unsigned long long key = ((unsigned long long)__float_as_uint(dw) << 32) | __float_as_uint(ld_f(a.pdrf + w));
+        const unsigned long long key = ((unsigned long long)__float_as_uint(dw) << 32) | __float_as_uint(rec_pdrf(a.rec, w));
         mins.far = key < mins.far ? key : mins.far;
         fnxt[wave_slot(&sh.n_farnxt)] = w;
       }
@@ -321,12 +333,12 @@ __device__ __forceinline__ void backtrace(const TraceArgs &a, Shared &sh, int t,
   bool first = true;
   for (;;) {
     int kb = -1;
-    const unsigned m = a.nbr[cur];
+    const unsigned m = (rec_meta(a.rec, cur) & kNbrBits);
     if (cur != t) {
       float du = kInf;
       if (lane < 26 && ((m >> lane) & 1u)) {
         int u = cur + nb_offset(ok_lane, a.d);
-        if (ld_f(a.pdrf + u) != 0.0f) du = __uint_as_float(word_dist_bits(ld_u64(a.key + u)));
+        if (rec_pdrf(a.rec, u) != 0.0f) du = __uint_as_float(word_dist_bits(lp_u64(&a.rec[u].word)));
       }
       int db = __float_as_int(du);
       int mb = wave_min(db);
@@ -334,9 +346,9 @@ __device__ __forceinline__ void backtrace(const TraceArgs &a, Shared &sh, int t,
     }
     if (first && kb >= 0) {  // the join at u = cur + N[kb]: u's first rail in dijkstra3d order
       const int u = cur + nb_offset(kb, a.d);
-      const unsigned mu = a.nbr[u];
+      const unsigned mu = (rec_meta(a.rec, u) & kNbrBits);
       int key = 64;
-      if (lane < 26 && ((mu >> lane) & 1u) && ld_f(a.pdrf + u + nb_offset(ok_lane, a.d)) == 0.0f) key = kD3Rank[lane];
+      if (lane < 26 && ((mu >> lane) & 1u) && rec_pdrf(a.rec, u + nb_offset(ok_lane, a.d)) == 0.0f) key = kD3Rank[lane];
       int r = wave_min(key);
       if (r < 26) {
         int ks = kD3Inv[r];
@@ -367,17 +379,17 @@ __device__ __forceinline__ void offer_to(const TraceArgs &a, Shared &sh, int v, 
   const unsigned mb = max(__float_as_uint(d), mu);  // M = max(M[u], d)
   if (mb < hi) {
     const unsigned long long cand = ((unsigned long long)mb << 32) | su;
-    const unsigned long long old = issued ? old_or_none : (ld_u64(a.key + v) <= cand ? cand : atomicMin(a.key + v, cand));
+    const unsigned long long old = issued ? old_or_none : (lp_u64(&a.rec[v].word) <= cand ? cand : atomicMin(&a.rec[v].word, cand));
     if (old > cand) {
       if (old == kMaxKey) touched[wave_slot(&sh.n_touched)] = v;
-      int was = atomicOr(a.state + v, kIQ);
-      if (was & kIW) atomicAnd(a.state + v, ~kIW);
-      if (!(was & kIQ)) nxt[wave_slot(&sh.n_nxt)] = v;
+      const unsigned was = atomicOr(&a.rec[v].meta, kQueued);
+      if (was & kWaiting) atomicAnd(&a.rec[v].meta, ~kWaiting);
+      if (!(was & kQueued)) nxt[wave_slot(&sh.n_nxt)] = v;
     }
-  } else if (ld_u64(a.key + v) == kMaxKey) {  // beyond the band: wait for its band
-    int seen = ld_i(a.state + v);
-    while (!(seen & (kIQ | kIW))) {
-      int got = atomicCAS(a.state + v, seen, seen | kIW);
+  } else if (lp_u64(&a.rec[v].word) == kMaxKey) {  // beyond the band: wait for its band
+    unsigned seen = rec_meta(a.rec, v);
+    while (!(seen & (kQueued | kWaiting))) {
+      unsigned got = atomicCAS(&a.rec[v].meta, seen, seen | kWaiting);
       if (got == seen) { far[wave_slot(&sh.n_far)] = v; break; }
       seen = got;
     }
@@ -392,14 +404,13 @@ template <int B, int E>
 __device__ __forceinline__ void offer_part(const TraceArgs &a, const int *off, int u, unsigned m, int x, int y, int z,
                                            int qx, int qy, int qz, unsigned su, unsigned mu, float maxd, unsigned hi,
                                            Pushes &p) {
-  const unsigned *key_hi = reinterpret_cast<const unsigned *>(a.key) + 1;
-  unsigned char al[E - B];
+  bool al[E - B];
   unsigned kh[E - B];
   for_nb<B, E>([&](auto K) {
     constexpr int k = decltype(K)::value, i = k - B;
     const bool on = (m >> k) & 1u;
-    al[i] = on ? ld_u8(a.alive + u + nb_offset<k>(a.d)) : (unsigned char)0;
-    kh[i] = on ? ld_u(key_hi + 2 * (u + nb_offset<k>(a.d))) : 0u;
+    al[i] = on && (rec_meta(a.rec, u + nb_offset<k>(a.d)) & kAlive);
+    kh[i] = on ? rec_word_hi(a.rec, (u + nb_offset<k>(a.d))) : 0u;
   });
   unsigned mbs[E - B];
   unsigned wait = 0;
@@ -412,7 +423,7 @@ __device__ __forceinline__ void offer_part(const TraceArgs &a, const int *off, i
     old[i] = 0;  // "no improvement" unless the atomic says otherwise
     if (al[i] && d < maxd) {
       if (mb >= hi) wait |= 1u << k;
-      else if (kh[i] >= mb) old[i] = atomicMin(a.key + u + nb_offset<k>(a.d), ((unsigned long long)mb << 32) | su);
+      else if (kh[i] >= mb) old[i] = atomicMin(&a.rec[u + nb_offset<k>(a.d)].word, ((unsigned long long)mb << 32) | su);
     }
   });
   unsigned won = 0;
@@ -424,32 +435,32 @@ __device__ __forceinline__ void offer_part(const TraceArgs &a, const int *off, i
     }
   });
   if (won) {
-    int was[E - B];
+    unsigned was[E - B];
     for_nb<B, E>([&](auto K) {
       constexpr int k = decltype(K)::value, i = k - B;
-      was[i] = (won >> k) & 1u ? atomicOr(a.state + u + nb_offset<k>(a.d), kIQ) : kIQ;
+      was[i] = (won >> k) & 1u ? atomicOr(&a.rec[u + nb_offset<k>(a.d)].meta, kQueued) : kQueued;
     });
     unsigned clear = 0;
     for_nb<B, E>([&](auto K) {
       constexpr int k = decltype(K)::value, i = k - B;
       if (!((won >> k) & 1u)) return;
-      if (was[i] & kIW) clear |= 1u << k;
-      if (!(was[i] & kIQ)) p.near |= 1u << k;
+      if (was[i] & kWaiting) clear |= 1u << k;
+      if (!(was[i] & kQueued)) p.near |= 1u << k;
     });
     while (clear) {
       const int k = __ffs(clear) - 1;
       clear &= clear - 1;
-      atomicAnd(a.state + u + off[k], ~kIW);
+      atomicAnd(&a.rec[u + off[k]].meta, ~kWaiting);
     }
   }
   while (wait) {  // beyond the band: wait for its band
     const int k = __ffs(wait) - 1;
     wait &= wait - 1;
     const int v = u + off[k];
-    if (ld_u64(a.key + v) != kMaxKey) continue;
-    int seen = ld_i(a.state + v);
-    while (!(seen & (kIQ | kIW))) {
-      const int got = atomicCAS(a.state + v, seen, seen | kIW);
+    if (lp_u64(&a.rec[v].word) != kMaxKey) continue;
+    unsigned seen = rec_meta(a.rec, v);
+    while (!(seen & (kQueued | kWaiting))) {
+      const unsigned got = atomicCAS(&a.rec[v].meta, seen, seen | kWaiting);
       if (got == seen) { p.far |= 1u << k; break; }
       seen = got;
     }
@@ -461,12 +472,39 @@ __device__ __forceinline__ void offer_all(const TraceArgs &a, const int *off, in
   int x, y, z, qx, qy, qz;
   unravel(u, a.d, x, y, z);
   unravel(int(su), a.d, qx, qy, qz);
-  if constexpr (kNbBatch >= 26) {
-    offer_part<0, 26>(a, off, u, m, x, y, z, qx, qy, qz, su, mu, maxd, hi, p);
-  } else {
-    if (m & 0x1FFFu) offer_part<0, 13>(a, off, u, m, x, y, z, qx, qy, qz, su, mu, maxd, hi, p);
-    if (m >> 13) offer_part<13, 26>(a, off, u, m, x, y, z, qx, qy, qz, su, mu, maxd, hi, p);
-  }
+  for_parts(m, [&](auto B, auto E) {
+    offer_part<decltype(B)::value, decltype(E)::value>(a, off, u, m, x, y, z, qx, qy, qz, su, mu, maxd, hi, p);
+  });
+}
+
+// The best claim waiting voxel c (at x, y, z) can pull from its claimed neighbours B..E-1: the
+// smallest (max(bottleneck, distance to the neighbour's source), source) among those whose source's
+// ball holds c. Claim loads, then the sources' DBF loads, each issued together.
+template <int B, int E>
+__device__ __forceinline__ void pull_part(const TraceArgs &a, int c, unsigned mc, int x, int y, int z,
+                                          unsigned long long &best) {
+  unsigned long long ku[E - B];
+  for_nb<B, E>([&](auto K) {
+    constexpr int k = decltype(K)::value, i = k - B;
+    ku[i] = (mc >> k) & 1u ? lp_u64(&a.rec[c + nb_offset<k>(a.d)].word) : kMaxKey;
+  });
+  float r[E - B];
+  for_nb<B, E>([&](auto K) {
+    constexpr int k = decltype(K)::value, i = k - B;
+    r[i] = ku[i] != kMaxKey ? a.dbf[(unsigned)ku[i]] : 0.0f;
+  });
+  for_nb<B, E>([&](auto K) {
+    constexpr int k = decltype(K)::value, i = k - B;
+    if (ku[i] == kMaxKey) return;
+    const unsigned su = (unsigned)ku[i];
+    int qx, qy, qz;
+    unravel(int(su), a.d, qx, qy, qz);
+    const float d = offer_distance(a, x, y, z, qx, qy, qz);
+    if (d >= a.scale * r[i] + a.constant) return;
+    const unsigned mb = max(__float_as_uint(d), (unsigned)(ku[i] >> 32));
+    const unsigned long long cand = ((unsigned long long)mb << 32) | su;
+    if (cand < best) best = cand;
+  });
 }
 
 __device__ __forceinline__ int band_of(unsigned mbits, float delta, int kmin) {
@@ -484,9 +522,9 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const
   int *cur = la, *nxt = lb_, *far = lfar, *keep = lkeep;
   for (int j = threadIdx.x; j < len; j += blockDim.x) {  // sources: path voxels still alive
     int p = path[j];
-    if (!ld_u8(a.alive + p)) continue;
-    if (atomicCAS(a.key + p, kMaxKey, (unsigned long long)(unsigned)p) == kMaxKey) {
-      atomicOr(a.state + p, kIQ);
+    if (!(rec_meta(a.rec, p) & kAlive)) continue;
+    if (atomicCAS(&a.rec[p].word, kMaxKey, (unsigned long long)(unsigned)p) == kMaxKey) {
+      atomicOr(&a.rec[p].meta, kQueued);
       cur[wave_slot(&sh.n_cur)] = p;
       touched[wave_slot(&sh.n_touched)] = p;
     }
@@ -498,19 +536,19 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const
       const int nc = sh.n_cur;
       if (nc == 0) break;
       for (int i = threadIdx.x; i < nc; i += blockDim.x) {  // snapshot, re-arm
-        int u = ld_i(cur + i);
-        snap[i] = ld_u64(a.key + u);
-        atomicAnd(a.state + u, ~kIQ);
+        int u = lp_i(cur + i);
+        snap[i] = lp_u64(&a.rec[u].word);
+        atomicAnd(&a.rec[u].meta, ~kQueued);
       }
       __syncthreads();
       if (nc <= kLaneFront * 2) {  // small front: one lane per neighbour
         for (int j = threadIdx.x; j < nc * 32; j += blockDim.x) {
           const int i = j >> 5, k = j & 31;
           if (k >= 26) continue;
-          const int u = ld_i(cur + i);
-          if (!((a.nbr[u] >> k) & 1u)) continue;
+          const int u = lp_i(cur + i);
+          if (!(((rec_meta(a.rec, u) & kNbrBits) >> k) & 1u)) continue;
           const int v = u + sh.off[k];
-          if (!ld_u8(a.alive + v)) continue;
+          if (!(rec_meta(a.rec, v) & kAlive)) continue;
           const unsigned long long ku = snap[i];
           const unsigned su = (unsigned)ku, mu = (unsigned)(ku >> 32);
           int x, y, z, qx, qy, qz;
@@ -526,8 +564,8 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const
           Pushes p;
           int u = 0;
           if (i < nc) {
-            u = ld_i(cur + i);
-            const unsigned m = a.nbr[u];
+            u = lp_i(cur + i);
+            const unsigned m = (rec_meta(a.rec, u) & kNbrBits);
             const unsigned long long ku = snap[i];
             const unsigned su = (unsigned)ku, mu = (unsigned)(ku >> 32);
             offer_all(a, sh.off, u, m, su, mu, a.scale * a.dbf[su] + a.constant, hi, p);
@@ -547,26 +585,13 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const
     if (threadIdx.x == 0) { sh.lowest = 0x7f800000u; sh.prof[12] += nf; sh.prof[13] += 1; }
     __syncthreads();
     for (int i = threadIdx.x; i < nf; i += blockDim.x) {  // band boundary: exact pulls
-      int c = ld_i(far + i);
-      if (ld_u64(a.key + c) != kMaxKey) { pull[i] = kClaimed; continue; }
-      const unsigned mc = a.nbr[c];
+      int c = lp_i(far + i);
+      if (lp_u64(&a.rec[c].word) != kMaxKey) { pull[i] = kClaimed; continue; }
+      const unsigned mc = (rec_meta(a.rec, c) & kNbrBits);
       int x, y, z;
       unravel(c, a.d, x, y, z);
       unsigned long long best = kMaxKey;
-      for (int k = 0; k < 26; ++k) {
-        if (!((mc >> k) & 1u)) continue;
-        const int u = c + nb_offset(k, a.d);
-        unsigned long long ku = ld_u64(a.key + u);
-        if (ku == kMaxKey) continue;
-        unsigned su = (unsigned)ku;
-        int qx, qy, qz;
-        unravel(int(su), a.d, qx, qy, qz);
-        float d = offer_distance(a, x, y, z, qx, qy, qz);
-        if (d >= a.scale * a.dbf[su] + a.constant) continue;
-        unsigned mb = max(__float_as_uint(d), (unsigned)(ku >> 32));
-        unsigned long long cand = ((unsigned long long)mb << 32) | su;
-        if (cand < best) best = cand;
-      }
+      for_parts(mc, [&](auto B, auto E) { pull_part<decltype(B)::value, decltype(E)::value>(a, c, mc, x, y, z, best); });
       pull[i] = best;
       if (best != kMaxKey) atomicMin(&sh.lowest, (unsigned)(best >> 32));
     }
@@ -575,16 +600,16 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const
     __syncthreads();
     hi = __float_as_uint((float)(sh.band + 1) * a.delta);
     for (int i = threadIdx.x; i < nf; i += blockDim.x) {  // promote / keep waiting / drop
-      int c = ld_i(far + i);
+      int c = lp_i(far + i);
       unsigned long long p = pull[i];
       if (p == kClaimed) continue;
-      if (p == kMaxKey) { atomicAnd(a.state + c, ~kIW); continue; }
+      if (p == kMaxKey) { atomicAnd(&a.rec[c].meta, ~kWaiting); continue; }
       if ((unsigned)(p >> 32) < hi) {
-        st_u64(a.key + c, p);
+        sp_u64(&a.rec[c].word, p);
         touched[wave_slot(&sh.n_touched)] = c;
         cur[wave_slot(&sh.n_cur)] = c;
-        atomicOr(a.state + c, kIQ);
-        atomicAnd(a.state + c, ~kIW);
+        atomicOr(&a.rec[c].meta, kQueued);
+        atomicAnd(&a.rec[c].meta, ~kWaiting);
       } else {
         keep[wave_slot(&sh.n_keep)] = c;
       }
@@ -598,9 +623,8 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const
   if (threadIdx.x == 0) atomicAdd(reinterpret_cast<unsigned long long *>(a.counters + 6), (unsigned long long)nt);
   for (int j = threadIdx.x; j < nt; j += blockDim.x) {  // the claimed set dies
     int v = touched[j];
-    st_u8(a.alive + v, 0);
-    st_u64(a.key + v, kMaxKey);
-    atomicAnd(a.state + v, ~(kIQ | kIW));
+    sp_u64(&a.rec[v].word, kMaxKey);
+    atomicAnd(&a.rec[v].meta, ~(kAlive | kQueued | kWaiting));
   }
   __syncthreads();
 }
@@ -630,7 +654,7 @@ __device__ __forceinline__ int next_target(const TraceArgs &a, Shared &sh, int L
       long long j = c + r * blockDim.x + threadIdx.x;
       if (j < e) {
         int v = int(0xFFFFFFFFu - (unsigned)a.order[j]);
-        if (ld_u8(a.alive + v)) atomicMin(&sh.found, int(j - c));
+        if ((rec_meta(a.rec, v) & kAlive)) atomicMin(&sh.found, int(j - c));
       }
     }
     __syncthreads();
@@ -684,8 +708,8 @@ __global__ void __launch_bounds__(kBlock) trace_labels(TraceArgs a) {
       const int nt = sh.n_touched;
       for (int j = threadIdx.x; j < nt; j += blockDim.x) {  // restore the railroad state
         int v = l4[j];
-        st_u64(a.key + v, kMaxKey);
-        st_i(a.state + v, 0);
+        sp_u64(&a.rec[v].word, kMaxKey);
+        atomicAnd(&a.rec[v].meta, ~kFarListed);
       }
       if (status != 1 || sh.path_len < 0) {
         if (threadIdx.x == 0) a.ejected[atomicAdd(a.counters + 2, 1) % a.ejected_cap] = L;
@@ -712,7 +736,7 @@ __global__ void __launch_bounds__(kBlock) trace_labels(TraceArgs a) {
       for (int j = threadIdx.x; j < len; j += blockDim.x) {
         int v = walk[j];
         if (pos >= 0) a.store[pos + (len - 1 - j)] = v;  // stored target-first
-        st_f(a.pdrf + v, 0.0f);                          // the path becomes rail
+        sp_f(&a.rec[v].pdrf, 0.0f);                          // the path becomes rail
       }
       __syncthreads();
       invalidate(a, sh, walk, len, l0, l1, l2, l3, l4, s0, s1);

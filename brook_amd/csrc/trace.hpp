@@ -8,14 +8,8 @@ constexpr int kTraceProfileSlots = 16;
 
 struct TraceArgs {
   Dims d{0, 0, 0};
-  const unsigned *nbr = nullptr;  // per-voxel same-label neighbour masks (bit k = neighbour k)
+  VoxelRec *rec = nullptr;  // voxel records: PDRF (rails become 0 in place), flags, words (~0 between uses)
   const float *dbf = nullptr;
-  float *pdrf = nullptr;          // rails become 0 in place
-  int *state = nullptr;           // 0 volume: railroad / invalidation queue bits
-  // ~0 volume, restored after every use: railroad distance words (distance bits high, bit 0 "not
-  // queued") and invalidation claims (bottleneck bits high, source low)
-  unsigned long long *key = nullptr;
-  unsigned char *alive = nullptr;
   const int *seg_start = nullptr, *seg_count = nullptr;
   const unsigned long long *order = nullptr;  // sorted order keys in segment layout (voxel = ~low 32 bits)
   const int *mstart = nullptr, *mcount = nullptr, *mflat = nullptr, *maxp = nullptr;
