@@ -68,6 +68,46 @@ def _traffic(kernel):
         return None
 
 
+def host_io(vol, calls=3):
+    """The same workload with host input (Fortran numpy array) and host output: includes the
+    host-to-device copy of the labels."""
+    import brook_amd
+    times = []
+    for _ in range(calls):
+        t0 = time.perf_counter()
+        brook_amd.skeletonize(vol, PARAMS, ANISO, dust_threshold=1000, output="packed", progress=False)
+        times.append(time.perf_counter() - t0)
+    return dict(value=round(vol.size / float(np.median(times)) / 1e6, 3), unit="Mvoxels/s", calls=calls,
+                p50_ms=round(float(np.median(times)) * 1e3, 3))
+
+
+def batched():
+    """64 mixed-shape volumes (64..256 per axis, 1-50 tube labels each, ~254 M voxels in all),
+    device-resident: one skeletonize_batch call against 64 plain calls."""
+    import torch
+
+    import brook_amd
+    from brook_amd import synth
+    vols = synth.batch_64()
+    dev = [_device_volume(v) for v in vols]
+    total = sum(v.size for v in vols)
+    params, aniso = PARAMS, (16, 16, 40)
+    brook_amd.skeletonize_batch(dev[:4], params, aniso, dust_threshold=1000, output="packed")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    brook_amd.skeletonize_batch(dev, params, aniso, dust_threshold=1000, output="packed")
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for d in dev:
+        brook_amd.skeletonize(d, params, aniso, dust_threshold=1000, output="packed", progress=False)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    return dict(voxels=int(total), samples=len(vols), batch_mvox_s=round(total / t_batch / 1e6, 3),
+                plain_mvox_s=round(total / t_plain / 1e6, 3), batch_ms=round(t_batch * 1e3, 3),
+                plain_ms=round(t_plain * 1e3, 3), speedup=round(t_plain / t_batch, 3))
+
+
 def cpu_baseline(vol, threads):
     """The oracle (a C++ port of the reference's algorithm) on a 256^3 crop of the same volume."""
     import oracle
@@ -83,9 +123,10 @@ def cpu_baseline(vol, threads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the host-input and batched measurements")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy three times (the known-size reference "
@@ -182,6 +223,9 @@ def main():
         roofline=roof,
         generator_s=round(gen_s, 2),
     )
+    if not args.no_extras:
+        out["host_io"] = host_io(vol)
+        out["batched"] = batched()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(vol, args.cpu_threads)
     else:

@@ -194,19 +194,28 @@ brook::SkeletonizeOptions convert(const brook_skeletonize_options *p, std::vecto
   return o;
 }
 
-brook::Packed run_one(brook::Context &ctx, const brook_volume *v, const brook_skeletonize_options *p,
-                      brook::FieldsOut *fields = nullptr) {
-  std::vector<uint64_t> ids;
-  brook::SkeletonizeOptions o = convert(p, &ids);
+// A caller's volume as the engine's device label volume (owned copies, if any, live in `in` and
+// `masked`), with the object filter applied.
+brook::LabelVolume engine_input(brook::Context &ctx, const brook_volume *v, const brook_skeletonize_options *p,
+                                const std::vector<uint64_t> &ids, Input &in, brook::Buffer<uint8_t> &masked) {
   require(integer_dtype(v ? v->dtype : BROOK_F32), "skeletonize requires integer labels");
-  Input in = normalize(ctx, v);
-  brook::Buffer<uint8_t> masked;
+  in = normalize(ctx, v);
   if (p->flags & BROOK_FILTER_OBJECTS) {
     masked = brook::Buffer<uint8_t>(ctx, size_t(in.view.size()) * in.view.bytes);
     brook::mask_labels(ctx, in.view.data, in.view.bytes, in.view.size(), ids, masked.data());
     in.view.data = masked.data();
   }
-  return brook::skeletonize(ctx, in.view, o, fields);
+  return in.view;
+}
+
+brook::Packed run_one(brook::Context &ctx, const brook_volume *v, const brook_skeletonize_options *p,
+                      brook::FieldsOut *fields = nullptr) {
+  std::vector<uint64_t> ids;
+  brook::SkeletonizeOptions o = convert(p, &ids);
+  Input in;
+  brook::Buffer<uint8_t> masked;
+  const brook::LabelVolume view = engine_input(ctx, v, p, ids, in, masked);
+  return brook::skeletonize(ctx, view, o, fields);
 }
 
 }  // namespace
@@ -386,10 +395,17 @@ brook_status brook_skeletonize_batch(brook_context *ctx, const brook_volume *con
     require(ctx && (volumes || !count) && options && (out || !count));
     for (size_t i = 0; i < count; ++i) out[i] = nullptr;
     brook::ContextCall call(ctx->value);
+    std::vector<uint64_t> ids;
+    brook::SkeletonizeOptions o = convert(options, &ids);
+    std::vector<Input> inputs(count);
+    std::vector<brook::Buffer<uint8_t>> masked(count);
+    std::vector<brook::LabelVolume> views;
+    for (size_t i = 0; i < count; ++i) views.push_back(engine_input(ctx->value, volumes[i], options, ids, inputs[i], masked[i]));
+    std::vector<brook::Packed> packed = brook::skeletonize_batch(ctx->value, views, o);
     std::vector<std::unique_ptr<brook_skeletons>> results;
     for (size_t i = 0; i < count; ++i) {
       auto r = std::make_unique<brook_skeletons>();
-      r->packed = run_one(ctx->value, volumes[i], options);
+      r->packed = std::move(packed[i]);
       results.push_back(std::move(r));
     }
     for (size_t i = 0; i < count; ++i) out[i] = results[i].release();

@@ -55,6 +55,22 @@ template <int B, int E, class F> __device__ __forceinline__ void for_nb(F &&f) {
 }
 template <class F> __device__ __forceinline__ void for_each_nb(F &&f) { for_nb<0, 26>(f); }
 
+// Calls f(integral_constant B, integral_constant E) for each part [B, E) of the 26 neighbours that
+// has a set bit in m; Batch = neighbours per part (26, 13 or 9).
+template <int Batch, class F> __device__ __forceinline__ void for_parts(unsigned m, F &&f) {
+  using std::integral_constant;
+  if constexpr (Batch >= 26) {
+    if (m) f(integral_constant<int, 0>{}, integral_constant<int, 26>{});
+  } else if constexpr (Batch >= 13) {
+    if (m & 0x1FFFu) f(integral_constant<int, 0>{}, integral_constant<int, 13>{});
+    if (m >> 13) f(integral_constant<int, 13>{}, integral_constant<int, 26>{});
+  } else {
+    if (m & 0x1FFu) f(integral_constant<int, 0>{}, integral_constant<int, 9>{});
+    if ((m >> 9) & 0x1FFu) f(integral_constant<int, 9>{}, integral_constant<int, 18>{});
+    if (m >> 18) f(integral_constant<int, 18>{}, integral_constant<int, 26>{});
+  }
+}
+
 // Loads of memory that other threads change inside the same kernel: relaxed, agent scope, so the
 // access is served by L2 (the CU's vector L1 may hold a stale line otherwise).
 __device__ __forceinline__ float ld_f(const float *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <numeric>
 
 #include "fields.hpp"
@@ -49,16 +50,9 @@ Packed empty_result(const SkeletonizeOptions &o) {
   return p;
 }
 
-}  // namespace
-
-Packed skeletonize(Context &ctx, const LabelVolume &in, const SkeletonizeOptions &o, FieldsOut *fields) {
-  ctx.stats.clear();
-  Lap lap{ctx};
-  const Dims d = in.dims;
-  const int64_t n = in.size();
+void check_options(const SkeletonizeOptions &o) {
   for (float a : o.aniso)
     if (!(a > 0) || !std::isfinite(a)) throw std::invalid_argument("anisotropy must be finite and positive");
-  if (n == 0 || double(n) <= o.dust) return empty_result(o);
   if (!o.fix_branching) throw Unsupported("fix_branching=False is not supported by this build");
   if (o.fill_holes || o.fix_avocados || o.voxel_graph)
     throw Unsupported("fill_holes, fix_avocados and voxel_graph are not supported by this build");
@@ -68,289 +62,442 @@ Packed skeletonize(Context &ctx, const LabelVolume &in, const SkeletonizeOptions
     throw std::invalid_argument("odd pdrf_exponent makes the background penalty negative");
   if (!(exponent > 0 && exponent < 65536 && exponent == std::floor(exponent) && !(int(exponent) & (int(exponent) - 1))))
     throw Unsupported("pdrf_exponent must be a power of two (1..32768) in this build");
+}
 
-  // ---- components, analysis, distance field -------------------------------------------------
-  Components cc = connected_components(ctx, in);
-  lap("prep_ccl_ms");
-  const int64_t count = cc.count;
-  if (count == 0) return empty_result(o);
-  const size_t k = size_t(count) + 1;
-  std::vector<Box> boxes = analyze(ctx, cc, d);
-  lap("prep_analyze_ms");
-  const bool black_border = count == 1 && boxes[1].count == n;
-  Buffer<float> dbf(ctx, size_t(n));
-  Buffer<unsigned> label_max(ctx, k);
-  label_max.clear(ctx);
-  edt(ctx, cc.labels.data(), cc.cc_bytes, d, o.aniso, black_border, 3, dbf.data(), label_max.data());
-  std::vector<unsigned> max_bits = label_max.download(ctx);
-  std::vector<float> dbf_max(k);
-  for (size_t i = 0; i < k; ++i) std::memcpy(&dbf_max[i], &max_bits[i], 4);
-  lap("prep_edt_ms");
-  std::vector<std::vector<Point>> borders(k);
-  if (o.fix_borders) borders = border_targets(ctx, cc.labels.data(), cc.cc_bytes, d, count, o.aniso);
-  lap("prep_borders_ms");
-  std::vector<std::vector<Point>> before(k);
-  for (const Point &p : o.before) before.at(size_t(component_at(ctx, cc, d, p))).push_back(p);
-
-  // ---- labels to trace (skeletonize.cu:128-133, batched.cu:186-203) ---------------------------
-  std::vector<uint8_t> active(k, 0);
-  std::vector<int> labels;
-  for (size_t id = 1; id < k; ++id) {
-    const Box &b = boxes[id];
-    int64_t ext = int64_t(b.hi[0] - b.lo[0] + 1) * (b.hi[1] - b.lo[1] + 1) * (b.hi[2] - b.lo[2] + 1);
-    if (double(b.count) <= o.dust || ext <= 1) continue;
-    if (double(dbf_max[id]) > o.teasar.soma_detection)
-      throw Unsupported("a component exceeds soma_detection_threshold; soma processing is not supported by this build");
-    active[id] = 1;
-    labels.push_back(int(id));
-  }
-  std::vector<int> seg_start(k + 1, 0), seg_count(k, 0);
-  for (size_t id = 1; id < k; ++id) seg_count[id] = int(boxes[id].count);
-  for (size_t id = 0; id < k; ++id) seg_start[id + 1] = seg_start[id] + seg_count[id];
-  const int64_t F = seg_start[k];
-  Buffer<int> seg_start_dev, vox;
-  build_segments(ctx, cc.labels.data(), cc.cc_bytes, d, seg_start, seg_start_dev, vox);
-  Buffer<int> seg_count_dev(ctx, k);
-  seg_count_dev.upload(ctx, seg_count.data(), k);
-
-  // ---- roots, DAF, targets -------------------------------------------------------------------
-  std::vector<long long> first(k, -1), root(k, -1);
-  for (size_t id = 1; id < k; ++id) first[id] = cc.roots[id - 1];
-  for (int id : labels)
-    if (!borders[id].empty()) root[id] = flat_of(borders[id].back(), d);
-  int64_t flood_voxels = 0;  // voxels swept by the root-search and DAF floods
-  for (int id : labels) flood_voxels += boxes[id].count * (root[id] < 0 ? 2 : 1);
-  ctx.stats["flood_voxels"] = double(flood_voxels);
-  ctx.stats["foreground_voxels"] = double(F);
-  ctx.stats["cc_bytes"] = cc.cc_bytes;
-  ctx.stats["components"] = double(count);
-  std::vector<int> queue = labels;
-  std::stable_sort(queue.begin(), queue.end(), [&](int a, int b) { return boxes[a].count > boxes[b].count; });
-  Buffer<long long> first_dev(ctx, k), root_dev(ctx, k), target_dev(ctx, k);
-  Buffer<float> max_daf_dev(ctx, k);
-  first_dev.upload(ctx, first.data(), k);
-  root_dev.upload(ctx, root.data(), k);
-  Buffer<int> queue_dev(ctx, std::max<size_t>(queue.size(), 1));
-  queue_dev.upload(ctx, queue.data(), queue.size());
-  const double ax = o.aniso[0], ay = o.aniso[1], az = o.aniso[2];
+// Geometric step weights of the 26 neighbours (float of the double length, batched.cu:186-188).
+std::vector<float> step_weights(const float aniso[3]) {
+  const double ax = aniso[0], ay = aniso[1], az = aniso[2];
   std::vector<float> W = {float(ax), float(ax), float(ay), float(ay), float(az), float(az)};
   for (double e : {std::sqrt(ax * ax + ay * ay), std::sqrt(ay * ay + az * az), std::sqrt(ax * ax + az * az)})
     for (int i = 0; i < 4; ++i) W.push_back(float(e));
   for (int i = 0; i < 8; ++i) W.push_back(float(std::sqrt(ax * ax + ay * ay + az * az)));
-  Buffer<float> weights(ctx, 26);
-  weights.upload(ctx, W.data(), 26);
-  Buffer<float> daf(ctx, size_t(n));
-  fill_value(ctx, daf.data(), n, kInf);
-  Buffer<VoxelRec> rec(ctx, size_t(n));  // 16 bytes per voxel: PDRF, neighbour mask + flags, distance word
-  voxel_records(ctx, cc.labels.data(), cc.cc_bytes, d, rec.data());
-  std::vector<Buffer<int>> lists;
-  for (int i = 0; i < 6; ++i) lists.emplace_back(ctx, size_t(std::max<int64_t>(F, 1)));
-  LabelTables tables;
-  tables.seg_start = seg_start_dev.data();
-  tables.seg_count = seg_count_dev.data();
-  tables.vox = vox.data();
-  tables.first = first_dev.data();
-  tables.root = root_dev.data();
-  tables.target = target_dev.data();
-  tables.max_daf = max_daf_dev.data();
-  tables.weights = weights.data();
-  tables.rec = rec.data();
-  prepare_fields(ctx, d, tables, queue_dev.data(), int(queue.size()), daf.data(), lists[0].data(), lists[1].data());
-  root = root_dev.download(ctx);
-  std::vector<long long> target = target_dev.download(ctx);
-  std::vector<float> max_daf = max_daf_dev.download(ctx);
-  lap("prep_floods_ms");
+  return W;
+}
 
-  // ---- PDRF (exponent a power of two: squarings), alive mask, rails, target order -------------
-  std::vector<float> M(k, 0.0f), inv(k, 0.0f);
-  std::vector<uint8_t> use(k, 0);
-  for (int id : labels) {
-    M[id] = float(1.0 / std::pow(double(dbf_max[id]), 1.01));
-    if (max_daf[id] != 0) { inv[id] = float(1.0 / double(max_daf[id])); use[id] = 2; }
-    else use[id] = 1;
-  }
-  int squarings = 0;
-  for (int e = int(exponent); e > 1; e >>= 1) ++squarings;
-  Buffer<float> M_dev(ctx, k), inv_dev(ctx, k);
-  Buffer<uint8_t> use_dev(ctx, k);
-  M_dev.upload(ctx, M.data(), k);
-  inv_dev.upload(ctx, inv.data(), k);
-  use_dev.upload(ctx, use.data(), k);
-  Buffer<float> pdrf(ctx, size_t(n));
-  pdrf_and_alive(ctx, cc.labels.data(), cc.cc_bytes, n, dbf.data(), daf.data(), M_dev.data(), inv_dev.data(),
-                 use_dev.data(), squarings, float(o.teasar.pdrf_scale), pdrf.data(), rec.data());
-  Buffer<int> labels_dev(ctx, std::max<size_t>(labels.size(), 1));
-  labels_dev.upload(ctx, labels.data(), labels.size());
-  if (fields) {
-    fields->count = count;
-    fields->root.assign(root.begin(), root.end());
-    fields->target.assign(target.begin(), target.end());
-    fields->dbf_max = dbf_max;
-    fields->max_daf = max_daf;
-    fields->active = active;
-    ctx.synchronize();
-    if (fields->dbf) BROOK_HIP(hipMemcpy(fields->dbf, dbf.data(), size_t(n) * 4, hipMemcpyDeviceToHost));
-    if (fields->daf) BROOK_HIP(hipMemcpy(fields->daf, daf.data(), size_t(n) * 4, hipMemcpyDeviceToHost));
-    if (fields->pdrf) BROOK_HIP(hipMemcpy(fields->pdrf, pdrf.data(), size_t(n) * 4, hipMemcpyDeviceToHost));
-    if (fields->stop) return empty_result(o);
-  }
-  root_rails(ctx, rec.data(), root_dev.data(), labels_dev.data(), int(labels.size()));
-  Buffer<unsigned long long> order(ctx, size_t(std::max<int64_t>(F, 1))), snap_a(ctx, size_t(std::max<int64_t>(F, 1)));
-  target_order_keys(ctx, vox.data(), F, daf.data(), snap_a.data());
-  sort_segments(ctx, snap_a.data(), order.data(), F, seg_start_dev.data(), seg_start_dev.data() + 1, int(k));
-  lap("prep_pdrf_order_ms");
+// One volume through the pipeline, stage by stage, so that several volumes can share the flood
+// and trace launches: preamble (CCL, analysis, EDT, border targets, voxel records), floods,
+// PDRF and target order, trace, assembly. Device buffers live as long as the object.
+class Sample {
+ public:
+  Sample(Context &ctx, const LabelVolume &in, const SkeletonizeOptions &o, FieldsOut *fields)
+      : ctx_(ctx), in_(in), o_(o), fields_(fields), d_(in.dims), n_(in.size()) {}
 
-  // ---- manual targets and path budgets (lockstep.cu:116-143) ----------------------------------
-  std::vector<int> mstart(k, 0), mcount(k, 0), mflat, maxp(k, 0);
-  for (int id : labels) {
-    std::vector<Point> manual = borders[id];
-    if (!manual.empty()) manual.pop_back();
-    manual.insert(manual.end(), before[id].begin(), before[id].end());
-    mstart[id] = int(mflat.size());
-    if (manual.empty()) mflat.push_back(int(target[id]));
-    else for (auto it = manual.rbegin(); it != manual.rend(); ++it) mflat.push_back(int(flat_of(*it, d)));
-    mcount[id] = int(mflat.size()) - mstart[id];
-    int64_t budget = o.teasar.max_paths >= 0 ? o.teasar.max_paths : boxes[id].count;
-    maxp[id] = mcount[id] >= budget ? 0 : int(std::min<int64_t>(INT32_MAX, budget));
-  }
-  mflat.push_back(0);
-  Buffer<int> mstart_dev(ctx, k), mcount_dev(ctx, k), maxp_dev(ctx, k), mflat_dev(ctx, mflat.size());
-  mstart_dev.upload(ctx, mstart.data(), k);
-  mcount_dev.upload(ctx, mcount.data(), k);
-  maxp_dev.upload(ctx, maxp.data(), k);
-  mflat_dev.upload(ctx, mflat.data(), mflat.size());
+  bool done() const { return done_; }
+  Packed take_result() { return std::move(result_); }
+  // Rough device bytes a sample holds from its preamble to its assembly (40 B per voxel, 56 B per
+  // foreground voxel at the usual 40% occupancy, plus slack).
+  static int64_t footprint(int64_t n) { return n * 40 + (n * 2 / 5) * 56 + (int64_t(64) << 20); }
 
-  // ---- trace -----------------------------------------------------------------------------------
-  Buffer<unsigned long long> snap_b(ctx, size_t(std::max<int64_t>(F, 1)));
-  const long long store_cap = std::max<long long>(int64_t(1) << 22, 4 * F);
-  const int rec_cap = int(std::max<int64_t>(int64_t(1) << 16, F / 8 + 4 * int64_t(labels.size())));
-  Buffer<int> store(ctx, size_t(store_cap)), rec_label(ctx, rec_cap), rec_len(ctx, rec_cap), rec_start(ctx, rec_cap);
-  Buffer<int> counters(ctx, 8), ejected(ctx, 64), next(ctx, 1);
-  counters.clear(ctx);
-  next.clear(ctx);
-  TraceArgs ta;
-  ta.d = d;
-  ta.rec = rec.data();
-  ta.dbf = dbf.data();
-  ta.seg_start = seg_start_dev.data();
-  ta.seg_count = seg_count_dev.data();
-  ta.order = order.data();
-  ta.mstart = mstart_dev.data();
-  ta.mcount = mcount_dev.data();
-  ta.mflat = mflat_dev.data();
-  ta.maxp = maxp_dev.data();
-  for (int i = 0; i < 6; ++i) ta.lists[i] = lists[i].data();
-  ta.snaps[0] = snap_a.data();
-  ta.snaps[1] = snap_b.data();
-  ta.store = store.data();
-  ta.store_cap = store_cap;
-  ta.rec_label = rec_label.data();
-  ta.rec_len = rec_len.data();
-  ta.rec_start = rec_start.data();
-  ta.rec_cap = rec_cap;
-  ta.counters = counters.data();
-  Buffer<unsigned long long> label_stats(ctx, size_t(kTraceProfileSlots) * k);
-  label_stats.clear(ctx);
-  ta.label_stats = label_stats.data();
-  ta.ejected = ejected.data();
-  ta.ejected_cap = 64;
-  ta.queue = queue_dev.data();
-  ta.nqueue = int(queue.size());
-  ta.next = next.data();
-  ta.wx = o.aniso[0];
-  ta.wy = o.aniso[1];
-  ta.wz = o.aniso[2];
-  ta.scale = float(o.teasar.scale);
-  ta.constant = float(o.teasar.constant);
-  ta.delta = std::min({o.aniso[0], o.aniso[1], o.aniso[2]});
-  trace_all(ctx, cc.cc_bytes, ta);
-  std::vector<int> cnt = counters.download(ctx);
-  lap("trace_ms");
-  if (cnt[3]) throw std::runtime_error("path store capacity exceeded");
-  if (cnt[2]) throw Unsupported("a label has no reachable rail (the per-component fallback tracer is not part of this build)");
-  ctx.stats["paths"] = cnt[1];
-  ctx.stats["path_vertices"] = cnt[0];
-  uint64_t reached = 0, claimed = 0;
-  std::memcpy(&reached, &cnt[4], 8);
-  std::memcpy(&claimed, &cnt[6], 8);
-  ctx.stats["railroad_reached_voxels"] = double(reached);
-  ctx.stats["invalidated_voxels"] = double(claimed);
-  {  // per-label trace profile (device wall clock)
+  void preamble(Lap &lap) {
+    if (n_ == 0 || double(n_) <= o_.dust) return finish_empty();
+    cc_ = connected_components(ctx_, in_);
+    lap("prep_ccl_ms");
+    count_ = cc_.count;
+    if (count_ == 0) return finish_empty();
+    k_ = size_t(count_) + 1;
+    boxes_ = analyze(ctx_, cc_, d_);
+    lap("prep_analyze_ms");
+    const bool black_border = count_ == 1 && boxes_[1].count == n_;
+    dbf_ = Buffer<float>(ctx_, size_t(n_));
+    Buffer<unsigned> label_max(ctx_, k_);
+    label_max.clear(ctx_);
+    edt(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, o_.aniso, black_border, 3, dbf_.data(), label_max.data());
+    std::vector<unsigned> max_bits = label_max.download(ctx_);
+    dbf_max_.resize(k_);
+    for (size_t i = 0; i < k_; ++i) std::memcpy(&dbf_max_[i], &max_bits[i], 4);
+    lap("prep_edt_ms");
+    borders_.assign(k_, {});
+    if (o_.fix_borders) borders_ = border_targets(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, count_, o_.aniso);
+    lap("prep_borders_ms");
+    before_.assign(k_, {});
+    for (const Point &p : o_.before) before_.at(size_t(component_at(ctx_, cc_, d_, p))).push_back(p);
+
+    // labels to trace (skeletonize.cu:128-133, batched.cu:186-203)
+    active_.assign(k_, 0);
+    for (size_t id = 1; id < k_; ++id) {
+      const Box &b = boxes_[id];
+      int64_t ext = int64_t(b.hi[0] - b.lo[0] + 1) * (b.hi[1] - b.lo[1] + 1) * (b.hi[2] - b.lo[2] + 1);
+      if (double(b.count) <= o_.dust || ext <= 1) continue;
+      if (double(dbf_max_[id]) > o_.teasar.soma_detection)
+        throw Unsupported("a component exceeds soma_detection_threshold; soma processing is not supported by this build");
+      active_[id] = 1;
+      labels_.push_back(int(id));
+    }
+    std::vector<int> seg_start(k_ + 1, 0), seg_count(k_, 0);
+    for (size_t id = 1; id < k_; ++id) seg_count[id] = int(boxes_[id].count);
+    for (size_t id = 0; id < k_; ++id) seg_start[id + 1] = seg_start[id] + seg_count[id];
+    F_ = seg_start[k_];
+    build_segments(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, seg_start, seg_start_dev_, vox_);
+    seg_count_dev_ = Buffer<int>(ctx_, k_);
+    seg_count_dev_.upload(ctx_, seg_count.data(), k_);
+
+    // roots, DAF, targets: inputs of the floods
+    std::vector<long long> first(k_, -1);
+    root_.assign(k_, -1);
+    for (size_t id = 1; id < k_; ++id) first[id] = cc_.roots[id - 1];
+    for (int id : labels_)
+      if (!borders_[id].empty()) root_[id] = flat_of(borders_[id].back(), d_);
+    flood_voxels_ = 0;
+    for (int id : labels_) flood_voxels_ += boxes_[id].count * (root_[id] < 0 ? 2 : 1);
+    first_dev_ = Buffer<long long>(ctx_, k_);
+    root_dev_ = Buffer<long long>(ctx_, k_);
+    target_dev_ = Buffer<long long>(ctx_, k_);
+    max_daf_dev_ = Buffer<float>(ctx_, k_);
+    first_dev_.upload(ctx_, first.data(), k_);
+    root_dev_.upload(ctx_, root_.data(), k_);
+    std::vector<float> W = step_weights(o_.aniso);
+    weights_ = Buffer<float>(ctx_, 26);
+    weights_.upload(ctx_, W.data(), 26);
+    daf_ = Buffer<float>(ctx_, size_t(n_));
+    fill_value(ctx_, daf_.data(), n_, kInf);
+    rec_ = Buffer<VoxelRec>(ctx_, size_t(n_));  // 16 bytes per voxel: PDRF, neighbour mask + flags, distance word
+    voxel_records(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, rec_.data());
+    for (int i = 0; i < 6; ++i) lists_.emplace_back(ctx_, size_t(std::max<int64_t>(F_, 1)));
+  }
+
+  FloodJob flood_job() const {
+    FloodJob j;
+    j.d = d_;
+    j.t.seg_start = seg_start_dev_.data();
+    j.t.seg_count = seg_count_dev_.data();
+    j.t.vox = vox_.data();
+    j.t.first = first_dev_.data();
+    j.t.root = root_dev_.data();
+    j.t.target = target_dev_.data();
+    j.t.max_daf = max_daf_dev_.data();
+    j.t.weights = weights_.data();
+    j.t.rec = rec_.data();
+    j.daf = daf_.data();
+    j.lists_a = lists_[0].data();
+    j.lists_b = lists_[1].data();
+    return j;
+  }
+
+  // The sample's labels as work items (sorted largest first over all samples of a group).
+  void items(int s, std::vector<LabelItem> &q) const {
+    if (done_) return;
+    for (int id : labels_) q.push_back(LabelItem{s, id, boxes_[id].count});
+  }
+
+  void after_floods(Lap &lap) {
+    if (done_) return;
+    root_ = root_dev_.download(ctx_);
+    target_ = target_dev_.download(ctx_);
+    max_daf_ = max_daf_dev_.download(ctx_);
+
+    // PDRF (exponent a power of two: squarings), alive bits, rails, target order
+    std::vector<float> M(k_, 0.0f), inv(k_, 0.0f);
+    std::vector<uint8_t> use(k_, 0);
+    for (int id : labels_) {
+      M[id] = float(1.0 / std::pow(double(dbf_max_[id]), 1.01));
+      if (max_daf_[id] != 0) { inv[id] = float(1.0 / double(max_daf_[id])); use[id] = 2; }
+      else use[id] = 1;
+    }
+    int squarings = 0;
+    for (int e = int(o_.teasar.pdrf_exponent); e > 1; e >>= 1) ++squarings;
+    Buffer<float> M_dev(ctx_, k_), inv_dev(ctx_, k_);
+    Buffer<uint8_t> use_dev(ctx_, k_);
+    M_dev.upload(ctx_, M.data(), k_);
+    inv_dev.upload(ctx_, inv.data(), k_);
+    use_dev.upload(ctx_, use.data(), k_);
+    Buffer<float> pdrf(ctx_, size_t(n_));
+    pdrf_and_alive(ctx_, cc_.labels.data(), cc_.cc_bytes, n_, dbf_.data(), daf_.data(), M_dev.data(), inv_dev.data(),
+                   use_dev.data(), squarings, float(o_.teasar.pdrf_scale), pdrf.data(), rec_.data());
+    Buffer<int> labels_dev(ctx_, std::max<size_t>(labels_.size(), 1));
+    labels_dev.upload(ctx_, labels_.data(), labels_.size());
+    if (fields_) {
+      fields_->count = count_;
+      fields_->root.assign(root_.begin(), root_.end());
+      fields_->target.assign(target_.begin(), target_.end());
+      fields_->dbf_max = dbf_max_;
+      fields_->max_daf = max_daf_;
+      fields_->active = active_;
+      ctx_.synchronize();
+      if (fields_->dbf) BROOK_HIP(hipMemcpy(fields_->dbf, dbf_.data(), size_t(n_) * 4, hipMemcpyDeviceToHost));
+      if (fields_->daf) BROOK_HIP(hipMemcpy(fields_->daf, daf_.data(), size_t(n_) * 4, hipMemcpyDeviceToHost));
+      if (fields_->pdrf) BROOK_HIP(hipMemcpy(fields_->pdrf, pdrf.data(), size_t(n_) * 4, hipMemcpyDeviceToHost));
+      if (fields_->stop) return finish_empty();
+    }
+    root_rails(ctx_, rec_.data(), root_dev_.data(), labels_dev.data(), int(labels_.size()));
+    const size_t fl = size_t(std::max<int64_t>(F_, 1));
+    order_ = Buffer<unsigned long long>(ctx_, fl);
+    snap_a_ = Buffer<unsigned long long>(ctx_, fl);
+    target_order_keys(ctx_, vox_.data(), F_, daf_.data(), snap_a_.data());
+    sort_segments(ctx_, snap_a_.data(), order_.data(), F_, seg_start_dev_.data(), seg_start_dev_.data() + 1, int(k_));
+    lap("prep_pdrf_order_ms");
+
+    // manual targets and path budgets (lockstep.cu:116-143)
+    std::vector<int> mstart(k_, 0), mcount(k_, 0), mflat, maxp(k_, 0);
+    for (int id : labels_) {
+      std::vector<Point> manual = borders_[id];
+      if (!manual.empty()) manual.pop_back();
+      manual.insert(manual.end(), before_[id].begin(), before_[id].end());
+      mstart[id] = int(mflat.size());
+      if (manual.empty()) mflat.push_back(int(target_[id]));
+      else for (auto it = manual.rbegin(); it != manual.rend(); ++it) mflat.push_back(int(flat_of(*it, d_)));
+      mcount[id] = int(mflat.size()) - mstart[id];
+      int64_t budget = o_.teasar.max_paths >= 0 ? o_.teasar.max_paths : boxes_[id].count;
+      maxp[id] = mcount[id] >= budget ? 0 : int(std::min<int64_t>(INT32_MAX, budget));
+    }
+    mflat.push_back(0);
+    mstart_dev_ = Buffer<int>(ctx_, k_);
+    mcount_dev_ = Buffer<int>(ctx_, k_);
+    maxp_dev_ = Buffer<int>(ctx_, k_);
+    mflat_dev_ = Buffer<int>(ctx_, mflat.size());
+    mstart_dev_.upload(ctx_, mstart.data(), k_);
+    mcount_dev_.upload(ctx_, mcount.data(), k_);
+    maxp_dev_.upload(ctx_, maxp.data(), k_);
+    mflat_dev_.upload(ctx_, mflat.data(), mflat.size());
+
+    // trace buffers
+    snap_b_ = Buffer<unsigned long long>(ctx_, fl);
+    store_cap_ = std::max<long long>(int64_t(1) << 22, 4 * F_);
+    rec_cap_ = int(std::max<int64_t>(int64_t(1) << 16, F_ / 8 + 4 * int64_t(labels_.size())));
+    store_ = Buffer<int>(ctx_, size_t(store_cap_));
+    rec_label_ = Buffer<int>(ctx_, rec_cap_);
+    rec_len_ = Buffer<int>(ctx_, rec_cap_);
+    rec_start_ = Buffer<int>(ctx_, rec_cap_);
+    counters_ = Buffer<int>(ctx_, 8);
+    ejected_ = Buffer<int>(ctx_, 64);
+    counters_.clear(ctx_);
+    label_stats_ = Buffer<unsigned long long>(ctx_, size_t(kTraceProfileSlots) * k_);
+    label_stats_.clear(ctx_);
+  }
+
+  TraceArgs trace_args() const {
+    TraceArgs ta;
+    ta.d = d_;
+    ta.rec = rec_.data();
+    ta.dbf = dbf_.data();
+    ta.seg_start = seg_start_dev_.data();
+    ta.seg_count = seg_count_dev_.data();
+    ta.order = order_.data();
+    ta.mstart = mstart_dev_.data();
+    ta.mcount = mcount_dev_.data();
+    ta.mflat = mflat_dev_.data();
+    ta.maxp = maxp_dev_.data();
+    for (int i = 0; i < 6; ++i) ta.lists[i] = lists_[i].data();
+    ta.snaps[0] = snap_a_.data();
+    ta.snaps[1] = snap_b_.data();
+    ta.store = store_.data();
+    ta.store_cap = store_cap_;
+    ta.rec_label = rec_label_.data();
+    ta.rec_len = rec_len_.data();
+    ta.rec_start = rec_start_.data();
+    ta.rec_cap = rec_cap_;
+    ta.counters = counters_.data();
+    ta.label_stats = label_stats_.data();
+    ta.ejected = ejected_.data();
+    ta.ejected_cap = 64;
+    ta.wx = o_.aniso[0];
+    ta.wy = o_.aniso[1];
+    ta.wz = o_.aniso[2];
+    ta.scale = float(o_.teasar.scale);
+    ta.constant = float(o_.teasar.constant);
+    ta.delta = std::min({o_.aniso[0], o_.aniso[1], o_.aniso[2]});
+    return ta;
+  }
+
+  // Trace outcome, statistics (per-label profiles only when `profile`) and assembly.
+  void finish(Lap &lap, bool profile) {
+    if (done_) return;
+    std::vector<int> cnt = counters_.download(ctx_);
+    if (cnt[3]) throw std::runtime_error("path store capacity exceeded");
+    if (cnt[2]) throw Unsupported("a label has no reachable rail (the per-component fallback tracer is not part of this build)");
+    ctx_.stats["paths"] += cnt[1];
+    ctx_.stats["path_vertices"] += cnt[0];
+    uint64_t reached = 0, claimed = 0;
+    std::memcpy(&reached, &cnt[4], 8);
+    std::memcpy(&claimed, &cnt[6], 8);
+    ctx_.stats["railroad_reached_voxels"] += double(reached);
+    ctx_.stats["invalidated_voxels"] += double(claimed);
+    ctx_.stats["labels_traced"] += double(labels_.size());
+    if (profile) record_profile();
+
+    // assembly: groups = input labels in first-skeleton-component order
+    const int records = cnt[1];
+    std::vector<int> rl = rec_label_.download(ctx_, records), rn = rec_len_.download(ctx_, records);
+    std::vector<uint8_t> has_edges(k_, 0);
+    for (int r = 0; r < records; ++r)
+      if (rn[r] >= 2) has_edges[rl[r]] = 1;
+    AssemblyInput ai;
+    ai.cc = cc_.labels.data();
+    ai.d = d_;
+    ai.dbf = dbf_.data();
+    ai.store = store_.data();
+    ai.store_used = cnt[0];
+    ai.rec_start = rec_start_.data();
+    ai.rec_len = rec_len_.data();
+    ai.records = records;
+    ai.group_of.assign(k_, 0);
+    std::map<int64_t, int> group_index;
+    for (size_t id = 1; id < k_; ++id) {
+      if (!has_edges[id]) continue;
+      int64_t label = cc_.mapping[id];
+      auto it = group_index.find(label);
+      if (it == group_index.end()) {
+        it = group_index.emplace(label, int(ai.group_labels.size())).first;
+        ai.group_labels.push_back(label);
+      }
+      ai.group_of[id] = it->second;
+    }
+    std::copy(o_.aniso, o_.aniso + 3, ai.aniso);
+    result_ = assemble(ctx_, cc_.cc_bytes, ai);
+    lap("assembly_ms");
+    done_ = true;
+  }
+
+  void record_counts() const {
+    if (!count_) return;
+    ctx_.stats["flood_voxels"] += double(flood_voxels_);
+    ctx_.stats["foreground_voxels"] += double(F_);
+    ctx_.stats["cc_bytes"] = cc_.cc_bytes;
+    ctx_.stats["components"] += double(count_);
+  }
+
+ private:
+  void finish_empty() {
+    result_ = empty_result(o_);
+    done_ = true;
+  }
+
+  void record_profile() {  // per-label trace profile (device wall clock)
     int khz = 0;
-    BROOK_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx.device()));
+    BROOK_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx_.device()));
     const double tick_ms = khz > 0 ? 1.0 / khz : 0.0;
-    std::vector<unsigned long long> ls = label_stats.download(ctx);
+    std::vector<unsigned long long> ls = label_stats_.download(ctx_);
     const size_t P = kTraceProfileSlots;
     double sums[kTraceProfileSlots] = {0}, worst = 0;
     size_t worst_id = 0;
-    for (int id : labels) {
+    for (int id : labels_) {
       for (size_t j = 0; j < P; ++j) sums[j] += double(ls[P * size_t(id) + j]);
       if (double(ls[P * size_t(id)]) > worst) { worst = double(ls[P * size_t(id)]); worst_id = size_t(id); }
     }
+    auto &st = ctx_.stats;
     const char *work[6] = {"railroad_phases", "far_scanned", "near_expanded", "invalidate_expanded",
                            "waiting_pulled", "bands"};
     for (int j = 0; j < 6; ++j) {
-      ctx.stats[std::string(work[j]) + "_per_path"] = sums[8 + j] / std::max(1.0, sums[7]);
-      ctx.stats[std::string("slowest_") + work[j]] = double(ls[P * worst_id + 8 + j]);
+      st[std::string(work[j]) + "_per_path"] = sums[8 + j] / std::max(1.0, sums[7]);
+      st[std::string("slowest_") + work[j]] = double(ls[P * worst_id + 8 + j]);
     }
-    const double nl = std::max<double>(1.0, double(labels.size())), paths = std::max(1.0, sums[7]);
-    ctx.stats["trace_label_max_ms"] = worst * tick_ms;
-    ctx.stats["trace_label_mean_ms"] = sums[0] * tick_ms / nl;
+    const double nl = std::max<double>(1.0, double(labels_.size())), paths = std::max(1.0, sums[7]);
+    st["trace_label_max_ms"] = worst * tick_ms;
+    st["trace_label_mean_ms"] = sums[0] * tick_ms / nl;
     const char *phase[5] = {"total", "target", "railroad", "backtrace", "invalidate"};
-    for (int j = 1; j < 5; ++j) ctx.stats[std::string("trace_") + phase[j] + "_ms_per_label"] = sums[j] * tick_ms / nl;
-    ctx.stats["railroad_rounds_per_path"] = sums[5] / paths;
-    ctx.stats["invalidate_rounds_per_path"] = sums[6] / paths;
-    for (int j = 1; j < 5; ++j)
-      ctx.stats[std::string("slowest_") + phase[j] + "_ms"] = double(ls[P * worst_id + j]) * tick_ms;
-    ctx.stats["slowest_paths"] = double(ls[P * worst_id + 7]);
-    ctx.stats["slowest_voxels"] = double(boxes[worst_id].count);
-    ctx.stats["slowest_railroad_rounds"] = double(ls[P * worst_id + 5]);
-    ctx.stats["slowest_invalidate_rounds"] = double(ls[P * worst_id + 6]);
-    ctx.stats["slowest_id"] = double(worst_id);
-    if (ctx.kernel_timing)  // per-label detail for diagnostics
-      for (int id : labels) {
+    for (int j = 1; j < 5; ++j) st[std::string("trace_") + phase[j] + "_ms_per_label"] = sums[j] * tick_ms / nl;
+    st["railroad_rounds_per_path"] = sums[5] / paths;
+    st["invalidate_rounds_per_path"] = sums[6] / paths;
+    for (int j = 1; j < 5; ++j) st[std::string("slowest_") + phase[j] + "_ms"] = double(ls[P * worst_id + j]) * tick_ms;
+    st["slowest_paths"] = double(ls[P * worst_id + 7]);
+    st["slowest_voxels"] = double(boxes_[worst_id].count);
+    st["slowest_railroad_rounds"] = double(ls[P * worst_id + 5]);
+    st["slowest_invalidate_rounds"] = double(ls[P * worst_id + 6]);
+    st["slowest_id"] = double(worst_id);
+    if (ctx_.kernel_timing)  // per-label detail for diagnostics
+      for (int id : labels_) {
         const std::string s = std::to_string(id);
-        ctx.stats["label_ms:" + s] = double(ls[P * id]) * tick_ms;
-        ctx.stats["label_railroad_ms:" + s] = double(ls[P * id + 2]) * tick_ms;
-        ctx.stats["label_paths:" + s] = double(ls[P * id + 7]);
-        ctx.stats["label_near:" + s] = double(ls[P * id + 10]);
-        ctx.stats["label_inval:" + s] = double(ls[P * id + 11]);
-        ctx.stats["label_voxels:" + s] = double(boxes[id].count);
-        ctx.stats["label_dbfmax:" + s] = double(dbf_max[id]);
+        st["label_ms:" + s] = double(ls[P * id]) * tick_ms;
+        st["label_railroad_ms:" + s] = double(ls[P * id + 2]) * tick_ms;
+        st["label_paths:" + s] = double(ls[P * id + 7]);
+        st["label_near:" + s] = double(ls[P * id + 10]);
+        st["label_inval:" + s] = double(ls[P * id + 11]);
+        st["label_voxels:" + s] = double(boxes_[id].count);
+        st["label_dbfmax:" + s] = double(dbf_max_[id]);
       }
   }
-  ctx.stats["labels_traced"] = double(labels.size());
 
-  // ---- assembly: groups = input labels in first-skeleton-component order -----------------------
-  const int records = cnt[1];
-  std::vector<int> rl = rec_label.download(ctx, records), rn = rec_len.download(ctx, records);
-  std::vector<uint8_t> has_edges(k, 0);
-  for (int r = 0; r < records; ++r)
-    if (rn[r] >= 2) has_edges[rl[r]] = 1;
-  AssemblyInput ai;
-  ai.cc = cc.labels.data();
-  ai.d = d;
-  ai.dbf = dbf.data();
-  ai.store = store.data();
-  ai.store_used = cnt[0];
-  ai.rec_start = rec_start.data();
-  ai.rec_len = rec_len.data();
-  ai.records = records;
-  ai.group_of.assign(k, 0);
-  std::map<int64_t, int> group_index;
-  for (size_t id = 1; id < k; ++id) {
-    if (!has_edges[id]) continue;
-    int64_t label = cc.mapping[id];
-    auto it = group_index.find(label);
-    if (it == group_index.end()) {
-      it = group_index.emplace(label, int(ai.group_labels.size())).first;
-      ai.group_labels.push_back(label);
+  Context &ctx_;
+  LabelVolume in_;
+  const SkeletonizeOptions &o_;
+  FieldsOut *fields_;
+  const Dims d_;
+  const int64_t n_;
+  bool done_ = false;
+  Packed result_;
+
+  Components cc_;
+  int64_t count_ = 0, F_ = 0, flood_voxels_ = 0;
+  size_t k_ = 0;
+  std::vector<Box> boxes_;
+  std::vector<float> dbf_max_, max_daf_;
+  std::vector<std::vector<Point>> borders_, before_;
+  std::vector<uint8_t> active_;
+  std::vector<int> labels_;
+  std::vector<long long> root_, target_;
+  Buffer<float> dbf_, daf_, weights_, max_daf_dev_;
+  Buffer<int> seg_start_dev_, vox_, seg_count_dev_;
+  Buffer<long long> first_dev_, root_dev_, target_dev_;
+  Buffer<VoxelRec> rec_;
+  std::vector<Buffer<int>> lists_;
+  Buffer<unsigned long long> order_, snap_a_, snap_b_;
+  Buffer<int> mstart_dev_, mcount_dev_, maxp_dev_, mflat_dev_;
+  long long store_cap_ = 0;
+  int rec_cap_ = 0;
+  Buffer<int> store_, rec_label_, rec_len_, rec_start_, counters_, ejected_;
+  Buffer<unsigned long long> label_stats_;
+};
+
+bool larger_first(const LabelItem &a, const LabelItem &b) { return a.size > b.size; }
+
+// Runs a group of samples together: every preamble, one flood launch over all their labels
+// (largest first), every PDRF/order pass, one trace launch, every assembly.
+void run_group(Context &ctx, std::vector<std::unique_ptr<Sample>> &group, Lap &lap) {
+  for (auto &s : group) s->preamble(lap);
+  for (auto &s : group) s->record_counts();
+  std::vector<LabelItem> items;
+  for (size_t i = 0; i < group.size(); ++i) group[i]->items(int(i), items);
+  std::stable_sort(items.begin(), items.end(), larger_first);
+  std::vector<FloodJob> jobs;
+  for (auto &s : group) jobs.push_back(s->done() ? FloodJob{} : s->flood_job());
+  prepare_fields(ctx, jobs, items);
+  lap("prep_floods_ms");
+  for (auto &s : group) s->after_floods(lap);
+  items.clear();
+  for (size_t i = 0; i < group.size(); ++i) group[i]->items(int(i), items);
+  std::stable_sort(items.begin(), items.end(), larger_first);
+  std::vector<TraceArgs> args;
+  for (auto &s : group) args.push_back(s->done() ? TraceArgs{} : s->trace_args());
+  trace_all(ctx, args, items);
+  lap("trace_ms");
+  const bool profile = group.size() == 1;
+  for (auto &s : group) s->finish(lap, profile);
+}
+
+}  // namespace
+
+Packed skeletonize(Context &ctx, const LabelVolume &in, const SkeletonizeOptions &o, FieldsOut *fields) {
+  ctx.stats.clear();
+  check_options(o);
+  Lap lap{ctx};
+  std::vector<std::unique_ptr<Sample>> group;
+  group.push_back(std::make_unique<Sample>(ctx, in, o, fields));
+  run_group(ctx, group, lap);
+  return group[0]->take_result();
+}
+
+std::vector<Packed> skeletonize_batch(Context &ctx, const std::vector<LabelVolume> &volumes, const SkeletonizeOptions &o) {
+  ctx.stats.clear();
+  check_options(o);
+  Lap lap{ctx};
+  size_t free_bytes = 0, total_bytes = 0;
+  BROOK_HIP(hipMemGetInfo(&free_bytes, &total_bytes));
+  const int64_t budget = std::max<int64_t>(int64_t(free_bytes) / 2, int64_t(1) << 30);
+  std::vector<Packed> out;
+  out.reserve(volumes.size());
+  size_t i = 0;
+  while (i < volumes.size()) {  // groups within half of the free device memory (at least one sample)
+    std::vector<std::unique_ptr<Sample>> group;
+    int64_t bytes = 0;
+    while (i < volumes.size()) {
+      const int64_t need = Sample::footprint(volumes[i].size());
+      if (!group.empty() && bytes + need > budget) break;
+      group.push_back(std::make_unique<Sample>(ctx, volumes[i], o, nullptr));
+      bytes += need;
+      ++i;
     }
-    ai.group_of[id] = it->second;
+    run_group(ctx, group, lap);
+    for (auto &s : group) out.push_back(s->take_result());
   }
-  std::copy(o.aniso, o.aniso + 3, ai.aniso);
-  Packed out = assemble(ctx, cc.cc_bytes, ai);
-  lap("assembly_ms");
   return out;
 }
 

@@ -34,5 +34,10 @@ struct FieldsOut {
 };
 
 Packed skeletonize(Context &ctx, const LabelVolume &labels, const SkeletonizeOptions &options, FieldsOut *fields = nullptr);
+// Independent volumes with shared options: result[i] equals skeletonize(volumes[i]). Volumes are
+// run in groups that fit half of the free device memory; a group's labels share one flood launch
+// and one trace launch.
+std::vector<Packed> skeletonize_batch(Context &ctx, const std::vector<LabelVolume> &volumes,
+                                      const SkeletonizeOptions &options);
 
 }  // namespace brook

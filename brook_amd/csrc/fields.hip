@@ -69,12 +69,43 @@ __global__ void fill_float(float *p, long long n, float v) {
   if (i < n) p[i] = v;
 }
 
+#ifndef BROOK_FLOOD_BLOCK
+#define BROOK_FLOOD_BLOCK 1024
+#endif
+constexpr int kFloodBlock = BROOK_FLOOD_BLOCK;  // threads per label workgroup
+constexpr int kFloodBatch = kFloodBlock > 512 ? 9 : 26;  // neighbours in flight per thread
+
+// Relaxes masked neighbours B..E-1 of u (distance du): all loads, then all atomics, in flight
+// together. Returns the neighbours this thread must queue (improved and found not queued).
+template <int B, int E>
+__device__ __forceinline__ unsigned flood_part(VoxelRec *rec, Dims d, int u, unsigned m, float du, const float *W) {
+  unsigned seen[E - B];
+  for_nb<B, E>([&](auto K) {
+    constexpr int k = decltype(K)::value, i = k - B;
+    seen[i] = (m >> k) & 1u ? rec_word_hi(rec, u + nb_offset<k>(d)) : 0u;
+  });
+  unsigned long long old[E - B];
+  for_nb<B, E>([&](auto K) {
+    constexpr int k = decltype(K)::value, i = k - B;
+    const unsigned cb = __float_as_uint(du + W[k]);
+    old[i] = 0;
+    if (cb < seen[i]) old[i] = atomicMin(&rec[u + nb_offset<k>(d)].word, (unsigned long long)cb << 32);
+  });
+  unsigned push = 0;
+  for_nb<B, E>([&](auto K) {
+    constexpr int k = decltype(K)::value, i = k - B;
+    const unsigned cb = __float_as_uint(du + W[k]);
+    if ((old[i] & 1ULL) && old[i] > ((unsigned long long)cb << 32)) push |= 1u << k;
+  });
+  return push;
+}
+
 // Geometric flood of one label from one source inside one workgroup, on the records' packed words
 // (distance bits high, "not queued" bit 0 low; ~0 = +inf, not queued). Words of the label must be
 // ~0 on entry; on exit they hold the distances, every voxel dequeued (bit 0 set).
 // Label-correcting rounds: a voxel is dequeued by one atomic that also returns its distance, and
 // an improvement that finds the voxel not queued queues it, so no update is ever lost.
-__device__ void flood_label(VoxelRec *rec, Dims d, int src, const float *W, const int *off, int *cur, int *nxt,
+__device__ __forceinline__ void flood_label(VoxelRec *rec, Dims d, int src, const float *W, const int *off, int *cur, int *nxt,
                             int *counts /*LDS [2]*/) {
   if (threadIdx.x == 0) {
     sp_u64(&rec[src].word, 0ULL);  // distance 0, queued
@@ -94,23 +125,8 @@ __device__ void flood_label(VoxelRec *rec, Dims d, int src, const float *W, cons
         u = lp_i(cur + i);
         const unsigned m = rec_meta(rec, u) & kNbrBits;
         const float du = __uint_as_float((unsigned)(atomicOr(&rec[u].word, 1ULL) >> 32));
-        // all neighbours' loads, then all their atomics, in flight together
-        unsigned seen[26];
-        for_each_nb([&](auto K) {
-          constexpr int k = decltype(K)::value;
-          seen[k] = (m >> k) & 1u ? rec_word_hi(rec, u + nb_offset<k>(d)) : 0u;
-        });
-        unsigned long long old[26];
-        for_each_nb([&](auto K) {
-          constexpr int k = decltype(K)::value;
-          const unsigned cb = __float_as_uint(du + W[k]);
-          old[k] = 0;
-          if (cb < seen[k]) old[k] = atomicMin(&rec[u + nb_offset<k>(d)].word, (unsigned long long)cb << 32);
-        });
-        for_each_nb([&](auto K) {
-          constexpr int k = decltype(K)::value;
-          const unsigned cb = __float_as_uint(du + W[k]);
-          if ((old[k] & 1ULL) && old[k] > ((unsigned long long)cb << 32)) push |= 1u << k;
+        for_parts<kFloodBatch>(m, [&](auto B, auto E) {
+          push |= flood_part<decltype(B)::value, decltype(E)::value>(rec, d, u, m, du, W);
         });
       }
       emit_neighbours(nxt, &counts[1], push, u, off);
@@ -124,7 +140,7 @@ __device__ void flood_label(VoxelRec *rec, Dims d, int src, const float *W, cons
 
 // Argmax of the flooded distance over the label's voxel list: largest finite value, ties -> lowest
 // flat index (the packed key of batched_regular.inc:9-22). Returns the key (0: none).
-__device__ unsigned long long label_argmax(const VoxelRec *rec, const int *list, int count, unsigned long long *lds) {
+__device__ __forceinline__ unsigned long long label_argmax(const VoxelRec *rec, const int *list, int count, unsigned long long *lds) {
   if (threadIdx.x == 0) *lds = 0;
   __syncthreads();
   unsigned long long best = 0;
@@ -146,26 +162,31 @@ __device__ unsigned long long label_argmax(const VoxelRec *rec, const int *list,
   return r;
 }
 
-__global__ void __launch_bounds__(512) prepare_labels(Dims d, LabelTables t, const int *queue, int nqueue, int *next,
-                                                      float *daf, int *lists_a, int *lists_b) {
+__global__ void __launch_bounds__(kFloodBlock) prepare_labels(const FloodJob *__restrict__ jobs,
+                                                              const int2 *__restrict__ queue, int nqueue, int *next) {
   __shared__ float W[26];
   __shared__ int off[26];
   __shared__ int counts[2];
   __shared__ unsigned long long best;
   __shared__ int item;
-  if (threadIdx.x < 26) {
-    W[threadIdx.x] = t.weights[threadIdx.x];
-    off[threadIdx.x] = nb_offset(int(threadIdx.x), d);
-  }
   for (;;) {
     __syncthreads();
     if (threadIdx.x == 0) item = atomicAdd(next, 1);
     __syncthreads();
     if (item >= nqueue) return;
-    const int L = queue[item];
+    const int2 q = queue[item];
+    const FloodJob job = jobs[__builtin_amdgcn_readfirstlane(q.x)];  // uniform: scalar registers
+    const int L = __builtin_amdgcn_readfirstlane(q.y);
+    const Dims d = job.d;
+    const LabelTables &t = job.t;
+    if (threadIdx.x < 26) {
+      W[threadIdx.x] = t.weights[threadIdx.x];
+      off[threadIdx.x] = nb_offset(int(threadIdx.x), d);
+    }
+    __syncthreads();
     const int s0 = t.seg_start[L], cnt = t.seg_count[L];
     const int *list = t.vox + s0;
-    int *a = lists_a + s0, *b = lists_b + s0;
+    int *a = job.lists_a + s0, *b = job.lists_b + s0;
     long long root = t.root[L];
     if (root < 0) {  // no border target: flood from the component's first voxel, root = argmax
       flood_label(t.rec, d, int(t.first[L]), W, off, a, b, counts);
@@ -183,10 +204,9 @@ __global__ void __launch_bounds__(512) prepare_labels(Dims d, LabelTables t, con
     }
     for (int j = threadIdx.x; j < cnt; j += blockDim.x) {  // DAF out, words back to ~0
       const int v = list[j];
-      daf[v] = __uint_as_float(word_dist_bits(lp_u64(&t.rec[v].word)));
+      job.daf[v] = __uint_as_float(word_dist_bits(lp_u64(&t.rec[v].word)));
       sp_u64(&t.rec[v].word, kMaxKey);
     }
-    __syncthreads();
   }
 }
 
@@ -274,14 +294,20 @@ void fill_value(Context &ctx, float *p, long long n, float v) {
   BROOK_LAUNCHED();
 }
 
-void prepare_fields(Context &ctx, Dims d, const LabelTables &t, const int *queue, int nqueue, float *daf, int *lists_a,
-                    int *lists_b) {
-  if (nqueue == 0) return;
+void prepare_fields(Context &ctx, const std::vector<FloodJob> &jobs, const std::vector<LabelItem> &items) {
+  if (items.empty()) return;
+  Buffer<FloodJob> jobs_dev(ctx, jobs.size());
+  jobs_dev.upload(ctx, jobs.data(), jobs.size());
+  std::vector<int2> q(items.size());
+  for (size_t i = 0; i < items.size(); ++i) q[i] = make_int2(items[i].sample, items[i].label);
+  Buffer<int2> queue(ctx, q.size());
+  queue.upload(ctx, q.data(), q.size());
   Buffer<int> next(ctx, 1);
   next.clear(ctx);
-  int grid = std::min(nqueue, ctx.compute_units() * 2);
+  const int nq = int(q.size());
+  const int grid = std::min(nq, ctx.compute_units() * 2);
   KernelTimer kt(ctx, "prepare_labels");
-  prepare_labels<<<grid, 512, 0, ctx.stream()>>>(d, t, queue, nqueue, next.data(), daf, lists_a, lists_b);
+  prepare_labels<<<grid, kFloodBlock, 0, ctx.stream()>>>(jobs_dev.data(), queue.data(), nq, next.data());
   BROOK_LAUNCHED();
   ctx.synchronize();
 }

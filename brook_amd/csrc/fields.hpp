@@ -26,10 +26,16 @@ void voxel_records(Context &ctx, const void *cc, int cc_bytes, Dims d, VoxelRec 
 void build_segments(Context &ctx, const void *cc, int cc_bytes, Dims d, const std::vector<int> &seg_start,
                     Buffer<int> &seg_start_dev, Buffer<int> &vox);
 void fill_value(Context &ctx, float *p, long long n, float v);
-// Root search and DAF floods for the ids in `queue` (one workgroup per id, largest first).
-// t.rec words ~0 in and out; daf: DAF written on the ids' voxels; lists_*: scratch in segment layout.
-void prepare_fields(Context &ctx, Dims d, const LabelTables &t, const int *queue, int nqueue, float *daf, int *lists_a,
-                    int *lists_b);
+// One sample's flood inputs: t.rec words ~0 in and out; daf receives the DAF on the traced ids'
+// voxels; lists_*: scratch in segment layout.
+struct FloodJob {
+  Dims d{0, 0, 0};
+  LabelTables t;
+  float *daf = nullptr;
+  int *lists_a = nullptr, *lists_b = nullptr;
+};
+// Root search and DAF floods of every item (one workgroup per item, in item order).
+void prepare_fields(Context &ctx, const std::vector<FloodJob> &jobs, const std::vector<LabelItem> &items);
 void pdrf_and_alive(Context &ctx, const void *cc, int cc_bytes, long long n, const float *dbf, const float *daf,
                     const float *M, const float *inv, const unsigned char *use, int squarings, float scale, float *pdrf,
                     VoxelRec *rec);

@@ -27,6 +27,12 @@ struct Components {
   std::vector<int64_t> roots;    // [id - 1] = minimum flat index of the component
 };
 
+// One component label of one sample of a group (sample index, component id, voxel count).
+struct LabelItem {
+  int sample = 0, label = 0;
+  int64_t size = 0;
+};
+
 struct Box {
   int lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {-1, -1, -1};
   int64_t count = 0;

@@ -41,21 +41,6 @@ constexpr int kProf = kTraceProfileSlots;
 // 9: three parts). Fewer in flight, fewer registers.
 constexpr int kNbBatch = BROOK_NB_BATCH;
 
-// Calls f(integral_constant B, integral_constant E) for each part [B, E) of the 26 neighbours that
-// has a set bit in m.
-template <class F> __device__ __forceinline__ void for_parts(unsigned m, F &&f) {
-  using std::integral_constant;
-  if constexpr (kNbBatch >= 26) {
-    if (m) f(integral_constant<int, 0>{}, integral_constant<int, 26>{});
-  } else if constexpr (kNbBatch >= 13) {
-    if (m & 0x1FFFu) f(integral_constant<int, 0>{}, integral_constant<int, 13>{});
-    if (m >> 13) f(integral_constant<int, 13>{}, integral_constant<int, 26>{});
-  } else {
-    if (m & 0x1FFu) f(integral_constant<int, 0>{}, integral_constant<int, 9>{});
-    if ((m >> 9) & 0x1FFu) f(integral_constant<int, 9>{}, integral_constant<int, 18>{});
-    if (m >> 18) f(integral_constant<int, 18>{}, integral_constant<int, 26>{});
-  }
-}
 
 __device__ __forceinline__ int wave_min(int v) {
   for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
@@ -201,7 +186,7 @@ __device__ __forceinline__ void relax_part(const TraceArgs &a, RoadMins &mins, i
 
 __device__ __forceinline__ void relax_all(const TraceArgs &a, RoadMins &mins, int u, unsigned m, float du, float bound,
                                           Pushes &p) {
-  for_parts(m, [&](auto B, auto E) { relax_part<decltype(B)::value, decltype(E)::value>(a, mins, u, m, du, bound, p); });
+  for_parts<kNbBatch>(m, [&](auto B, auto E) { relax_part<decltype(B)::value, decltype(E)::value>(a, mins, u, m, du, bound, p); });
 }
 
 __device__ __forceinline__ float rail_bound(const Shared &sh) {
@@ -472,7 +457,7 @@ __device__ __forceinline__ void offer_all(const TraceArgs &a, const int *off, in
   int x, y, z, qx, qy, qz;
   unravel(u, a.d, x, y, z);
   unravel(int(su), a.d, qx, qy, qz);
-  for_parts(m, [&](auto B, auto E) {
+  for_parts<kNbBatch>(m, [&](auto B, auto E) {
     offer_part<decltype(B)::value, decltype(E)::value>(a, off, u, m, x, y, z, qx, qy, qz, su, mu, maxd, hi, p);
   });
 }
@@ -591,7 +576,7 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const
       int x, y, z;
       unravel(c, a.d, x, y, z);
       unsigned long long best = kMaxKey;
-      for_parts(mc, [&](auto B, auto E) { pull_part<decltype(B)::value, decltype(E)::value>(a, c, mc, x, y, z, best); });
+      for_parts<kNbBatch>(mc, [&](auto B, auto E) { pull_part<decltype(B)::value, decltype(E)::value>(a, c, mc, x, y, z, best); });
       pull[i] = best;
       if (best != kMaxKey) atomicMin(&sh.lowest, (unsigned)(best >> 32));
     }
@@ -673,16 +658,19 @@ __device__ __forceinline__ int next_target(const TraceArgs &a, Shared &sh, int L
   }
 }
 
-__global__ void __launch_bounds__(kBlock) trace_labels(TraceArgs a) {
+__global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restrict__ all, const int2 *__restrict__ queue,
+                                                      int nqueue, int *next) {
   __shared__ Shared sh;
   __shared__ int item;
-  if (threadIdx.x < 26) sh.off[threadIdx.x] = nb_offset(int(threadIdx.x), a.d);
   for (;;) {
     __syncthreads();
-    if (threadIdx.x == 0) item = atomicAdd(a.next, 1);
+    if (threadIdx.x == 0) item = atomicAdd(next, 1);
     __syncthreads();
-    if (item >= a.nqueue) return;
-    const int L = a.queue[item];
+    if (item >= nqueue) return;
+    const int2 q = queue[item];
+    const TraceArgs a = all[__builtin_amdgcn_readfirstlane(q.x)];  // uniform: scalar registers
+    const int L = __builtin_amdgcn_readfirstlane(q.y);
+    if (threadIdx.x < 26) sh.off[threadIdx.x] = nb_offset(int(threadIdx.x), a.d);
     const int seg = a.seg_start[L], count = a.seg_count[L];
     int *l0 = a.lists[0] + seg, *l1 = a.lists[1] + seg, *l2 = a.lists[2] + seg, *l3 = a.lists[3] + seg,
         *l4 = a.lists[4] + seg, *walk = a.lists[5] + seg;
@@ -753,13 +741,22 @@ __global__ void __launch_bounds__(kBlock) trace_labels(TraceArgs a) {
 
 }  // namespace
 
-void trace_all(Context &ctx, int cc_bytes, const TraceArgs &args) {
-  (void)cc_bytes;
-  if (args.nqueue == 0) return;
-  const int grid = std::min(args.nqueue, ctx.compute_units() * 2);
+void trace_all(Context &ctx, const std::vector<TraceArgs> &args, const std::vector<LabelItem> &items) {
+  if (items.empty()) return;
+  Buffer<TraceArgs> args_dev(ctx, args.size());
+  args_dev.upload(ctx, args.data(), args.size());
+  std::vector<int2> q(items.size());
+  for (size_t i = 0; i < items.size(); ++i) q[i] = make_int2(items[i].sample, items[i].label);
+  Buffer<int2> queue(ctx, q.size());
+  queue.upload(ctx, q.data(), q.size());
+  Buffer<int> next(ctx, 1);
+  next.clear(ctx);
+  const int nq = int(q.size());
+  const int grid = std::min(nq, ctx.compute_units() * 2);
   KernelTimer kt(ctx, "trace_labels");
-  trace_labels<<<grid, kBlock, 0, ctx.stream()>>>(args);
+  trace_labels<<<grid, kBlock, 0, ctx.stream()>>>(args_dev.data(), queue.data(), nq, next.data());
   BROOK_LAUNCHED();
+  ctx.synchronize();
 }
 
 }  // namespace brook

@@ -1,5 +1,7 @@
 // The per-label trace loop (target, railroad, backtrace, rails, invalidation).
 #pragma once
+#include <vector>
+
 #include "stages.hpp"
 
 namespace brook {
@@ -29,12 +31,11 @@ struct TraceArgs {
   unsigned long long *label_stats = nullptr;
   int *ejected = nullptr;
   int ejected_cap = 1;
-  const int *queue = nullptr;
-  int nqueue = 0;
-  int *next = nullptr;
   float wx = 1, wy = 1, wz = 1, scale = 1, constant = 0, delta = 1;
 };
 
-void trace_all(Context &ctx, int cc_bytes, const TraceArgs &args);
+// Traces every item (sample index into args, component id) with one workgroup per item, in item
+// order, all in one launch.
+void trace_all(Context &ctx, const std::vector<TraceArgs> &args, const std::vector<LabelItem> &items);
 
 }  // namespace brook
