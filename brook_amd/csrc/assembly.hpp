@@ -33,6 +33,5 @@ struct AssemblyInput {
 };
 
 Packed assemble(Context &ctx, int cc_bytes, const AssemblyInput &in);
-void exclusive_sum(Context &ctx, const int *in, int *out, long long n);
 
 }  // namespace brook

@@ -3,9 +3,9 @@
 // table, y/z lower envelopes with float64 intersections, sqrtf at the end.
 //
 // Layout for HBM: the x pass gives each row to one wave (64 consecutive voxels per load, run
-// boundaries from two ballots, no atomics); the y and z passes give each line to one thread with
-// neighbouring threads on neighbouring x, so every step of every thread is one coalesced wave
-// load. Background runs are never evaluated (their values are discarded by the finish rule) and
+// boundaries from two ballots, no atomics); the y and z passes first list the label runs of every
+// line (one thread per line, coalesced across neighbouring x) and then give each run its own
+// thread. Background runs are never evaluated (their values are discarded by the finish rule) and
 // the last pass fuses sqrt, the background zero and the per-label maximum.
 #include <hip/hip_runtime.h>
 
@@ -97,52 +97,112 @@ __global__ void __launch_bounds__(256) edt_x_serial(const CC *cc, int sx, int ro
   }
 }
 
-// Lower envelope along y (axis 1) or z (axis 2), one thread per line, neighbouring threads on
-// neighbouring x. The line is read in chunks of kChunk voxels (all loads of a chunk in flight at
-// once, staged in the thread's own LDS slots). The envelope stack (position, height) lives in LDS
-// for its first kStack entries and spills to a global column for deeper stacks; the float64
-// intersection Z[j] of entries j-1 and j is recomputed from them when needed -- the same
-// expression on the same operands as when the entry was pushed, so the same bits.
-constexpr int kStack = 20, kChunk = 8, kFhBlock = 256;
+// Lower envelopes along y (axis 1) or z (axis 2), run-parallel. A run is a maximal stretch of one
+// nonzero id along a line; runs are independent (the walls at their ends stand for the
+// neighbouring id or the border), so each run gets its own thread and no wave waits on another
+// line's runs.
+//   edt_runs:   one thread per line: lists the line's runs (count-only when runs == nullptr) and
+//               writes the background zeros of the last pass;
+//   edt_run_fh: one thread per run: the envelope of the run's parabolas (heights from the previous
+//               pass) with float64 intersections, then the run's values.
+// The envelope stack keeps its first kRunStack entries in LDS; deeper stacks spill to a per-run
+// global column. The intersection of entries j-1 and j is recomputed from them when needed -- the
+// same expression on the same operands as when entry j was placed, so the same bits.
+constexpr int kRunStack = 24, kRunBlock = 256, kLineChunk = 8;
 
-template <class CC>
-__global__ void __launch_bounds__(kFhBlock) edt_fh(const CC *cc, Dims dm, int axis, float w, int bb, int nlines,
-                                                   long long spill_pitch, int *spill_v, float *spill_h, float *D,
-                                                   int finish, unsigned *label_max) {
-  __shared__ int sV[kStack * kFhBlock];
-  __shared__ float sH[kStack * kFhBlock];
-  __shared__ CC sL[kChunk * kFhBlock];
-  __shared__ float sD[kChunk * kFhBlock];
-  const int tid = threadIdx.x;
-  const int t = blockIdx.x * kFhBlock + tid;
-  if (t >= nlines) return;
+struct LineGeom {
   long long base, stride;
   int n;
+};
+__device__ __forceinline__ LineGeom line_geom(Dims dm, int axis, int t) {
+  LineGeom g;
   if (axis == 1) {
-    int x = t % dm.sx, z = t / dm.sx;
-    base = x + (long long)dm.sx * dm.sy * z;
-    n = dm.sy;
-    stride = dm.sx;
+    const int x = t % dm.sx, z = t / dm.sx;
+    g.base = x + (long long)dm.sx * dm.sy * z;
+    g.n = dm.sy;
+    g.stride = dm.sx;
   } else {
-    base = t;
-    n = dm.sz;
-    stride = (long long)dm.sx * dm.sy;
+    g.base = t;
+    g.n = dm.sz;
+    g.stride = (long long)dm.sx * dm.sy;
   }
+  return g;
+}
+
+// Run record: x = line, y = first position, z = last position | left wall << 29 | right wall << 30,
+// w = first entry of the run's spill column. The count pass stores each line's number of runs and
+// of spill entries; the list pass writes the line's runs from the scanned offsets (no shared
+// counters).
+template <class CC>
+__global__ void __launch_bounds__(256) edt_runs(const CC *cc, Dims dm, int axis, int bb, int nlines, int finish, float *D,
+                                                int *line_runs, long long *line_spill, const int *run_at,
+                                                const long long *spill_at, int4 *runs) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= nlines) return;
+  const LineGeom g = line_geom(dm, axis, t);
+  const bool list = runs != nullptr;
+  int nr = 0;
+  long long ns = 0, at = list ? spill_at[t] : 0;
+  const int first = list ? run_at[t] : 0;
+  auto emit = [&](int c, int d, bool right) {
+    const int spill = max(0, d - c + 3 - kRunStack);
+    if (list) runs[first + nr] = make_int4(t, c, d | ((c > 0 || bb) ? 1 << 29 : 0) | (right ? 1 << 30 : 0), int(at + ns));
+    ++nr;
+    ns += spill;
+  };
+  CC cur = CC(0);
+  int c = 0;
+  for (int y0 = 0; y0 < g.n; y0 += kLineChunk) {
+    CC lab[kLineChunk];
+#pragma unroll
+    for (int j = 0; j < kLineChunk; ++j) lab[j] = y0 + j < g.n ? cc[g.base + (y0 + j) * g.stride] : CC(0);
+    const int m = min(kLineChunk, g.n - y0);
+    for (int j = 0; j < m; ++j) {
+      const int y = y0 + j;
+      const CC l = lab[j];
+      if (l != cur) {
+        if (cur != CC(0)) emit(c, y - 1, true);
+        c = y;
+        cur = l;
+      }
+      if (l == CC(0) && finish && list) D[g.base + y * g.stride] = 0.0f;
+    }
+  }
+  if (cur != CC(0)) emit(c, g.n - 1, bb != 0);
+  if (!list) {
+    line_runs[t] = nr;
+    line_spill[t] = ns;
+  }
+}
+
+template <class CC>
+__global__ void __launch_bounds__(kRunBlock) edt_run_fh(const CC *cc, Dims dm, int axis, float w, int nruns,
+                                                        const int4 *runs, int *spill_v, float *spill_h, float *D,
+                                                        int finish, unsigned *label_max) {
+  __shared__ int sV[kRunStack * kRunBlock];
+  __shared__ float sH[kRunStack * kRunBlock];
+  const int tid = threadIdx.x;
+  const int i = blockIdx.x * kRunBlock + tid;
+  if (i >= nruns) return;
+  const int4 r = runs[i];
+  const LineGeom g = line_geom(dm, axis, r.x);
+  const int c = r.y, d = r.z & ((1 << 29) - 1);
+  const bool left = (r.z >> 29) & 1, right = (r.z >> 30) & 1;
+  int *sv = spill_v + r.w;
+  float *sh = spill_h + r.w;
   const double w2 = (double)(w * w);
-  auto getV = [&](int k) -> int { return k < kStack ? sV[k * kFhBlock + tid] : spill_v[(k - kStack) * spill_pitch + t]; };
-  auto getH = [&](int k) -> float { return k < kStack ? sH[k * kFhBlock + tid] : spill_h[(k - kStack) * spill_pitch + t]; };
+  auto getV = [&](int k) -> int { return k < kRunStack ? sV[k * kRunBlock + tid] : sv[k - kRunStack]; };
+  auto getH = [&](int k) -> float { return k < kRunStack ? sH[k * kRunBlock + tid] : sh[k - kRunStack]; };
   auto set = [&](int k, int v, float h) {
-    if (k < kStack) { sV[k * kFhBlock + tid] = v; sH[k * kFhBlock + tid] = h; }
-    else { spill_v[(k - kStack) * spill_pitch + t] = v; spill_h[(k - kStack) * spill_pitch + t] = h; }
+    if (k < kRunStack) { sV[k * kRunBlock + tid] = v; sH[k * kRunBlock + tid] = h; }
+    else { sv[k - kRunStack] = v; sh[k - kRunStack] = h; }
   };
   // intersection of entries j-1 and j (j >= 1), as computed when entry j was placed
   auto inter = [&](int j) -> double {
     double v1 = (double)getV(j - 1), h1 = (double)getH(j - 1), v2 = (double)getV(j), h2 = (double)getH(j);
     return ((h2 + w2 * v2 * v2) - (h1 + w2 * v1 * v1)) / (2.0 * w2 * (v2 - v1));
   };
-  int k = -1, c = 0;
-  bool active = false;
-  CC cur = CC(0);
+  int k = -1;
   auto place = [&](int pp, double ph) {
     bool placed = false;
     while (k >= 0 && !placed) {
@@ -159,65 +219,43 @@ __global__ void __launch_bounds__(kFhBlock) edt_fh(const CC *cc, Dims dm, int ax
     }
     if (!placed) { k = 0; set(0, pp, (float)ph); }
   };
-  auto close_run = [&](int d, bool right_wall) {
-    if (right_wall) place(d + 1, 0.0);
-    float run_max = 0.0f;
-    if (k < 0) {
-      for (int q = c; q <= d; ++q) D[base + q * stride] = kInf;
-      run_max = kInf;
-    } else {
-      int kk = 0;
-      double znext = k >= 1 ? inter(1) : 0.0;
-      int vkk = getV(0);
-      float hkk = getH(0);
-      for (int q = c; q <= d; ++q) {
-        while (kk < k && znext < (double)q) {
-          kk++;
-          vkk = getV(kk);
-          hkk = getH(kk);
-          if (kk < k) znext = inter(kk + 1);
-        }
-        double dq = (double)q - (double)vkk;
-        float v = (float)(w2 * dq * dq + (double)hkk);
-        if (finish) { v = sqrtf(v); run_max = fmaxf(run_max, v); }
-        D[base + q * stride] = v;
-      }
-    }
-    if (finish && label_max && run_max < kInf) {
-      unsigned bits = __float_as_uint(run_max);
-      unsigned *slot = label_max + (unsigned long long)cur;
-      if (ld_u(slot) < bits) atomicMax(slot, bits);
-    }
-  };
-  for (int y0 = 0; y0 < n; y0 += kChunk) {
+  if (left) { k = 0; set(0, c - 1, 0.0f); }
+  for (int q0 = c; q0 <= d; q0 += kLineChunk) {  // heights of the previous pass, a chunk in flight
+    float hv[kLineChunk];
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j) {
-      int y = y0 + j;
-      CC l = y < n ? cc[base + y * stride] : CC(0);
-      sL[j * kFhBlock + tid] = l;
-      sD[j * kFhBlock + tid] = (y < n && l != CC(0)) ? D[base + y * stride] : kInf;
-    }
-    const int m = min(kChunk, n - y0);
-    for (int j = 0; j < m; ++j) {
-      const int y = y0 + j;
-      const CC l = sL[j * kFhBlock + tid];
-      if (active && l != cur) { close_run(y - 1, true); active = false; }
-      if (l == CC(0)) {
-        if (finish) D[base + y * stride] = 0.0f;
-        continue;
+    for (int j = 0; j < kLineChunk; ++j) hv[j] = q0 + j <= d ? D[g.base + (q0 + j) * g.stride] : kInf;
+    const int m = min(kLineChunk, d - q0 + 1);
+    for (int j = 0; j < m; ++j)
+      if (hv[j] < kInf) place(q0 + j, (double)hv[j]);
+  }
+  if (right) place(d + 1, 0.0);
+  float run_max = 0.0f;
+  if (k < 0) {
+    for (int q = c; q <= d; ++q) D[g.base + q * g.stride] = kInf;
+    run_max = kInf;
+  } else {
+    int kk = 0;
+    double znext = k >= 1 ? inter(1) : 0.0;
+    int vkk = getV(0);
+    float hkk = getH(0);
+    for (int q = c; q <= d; ++q) {
+      while (kk < k && znext < (double)q) {
+        kk++;
+        vkk = getV(kk);
+        hkk = getH(kk);
+        if (kk < k) znext = inter(kk + 1);
       }
-      if (!active) {
-        active = true;
-        cur = l;
-        c = y;
-        k = -1;
-        if (y > 0 || bb) { k = 0; set(0, y - 1, 0.0f); }
-      }
-      const float hf = sD[j * kFhBlock + tid];
-      if (hf < kInf) place(y, (double)hf);
+      double dq = (double)q - (double)vkk;
+      float v = (float)(w2 * dq * dq + (double)hkk);
+      if (finish) { v = sqrtf(v); run_max = fmaxf(run_max, v); }
+      D[g.base + q * g.stride] = v;
     }
   }
-  if (active) close_run(n - 1, bb != 0);
+  if (finish && label_max && run_max < kInf) {
+    unsigned bits = __float_as_uint(run_max);
+    unsigned *slot = label_max + (unsigned long long)cc[g.base + c * g.stride];
+    if (ld_u(slot) < bits) atomicMax(slot, bits);
+  }
 }
 
 // finish for a single pass (x only): background 0, else sqrt.
@@ -260,17 +298,39 @@ void run_edt(Context &ctx, const CC *cc, Dims dm, const float aniso[3], bool bb,
     BROOK_LAUNCHED();
     return;
   }
-  // envelope stacks deeper than the LDS part spill to one global column per line
   for (int axis = 1; axis <= last; ++axis) {
-    int n = axis == 1 ? dm.sy : dm.sz;
-    int nlines = axis == 1 ? dm.sx * dm.sz : dm.sx * dm.sy;
-    size_t spill_rows = size_t(std::max(1, n + 3 - kStack));
-    Buffer<int> V(ctx, spill_rows * nlines);
-    Buffer<float> H(ctx, spill_rows * nlines);
+    const int nlines = axis == 1 ? dm.sx * dm.sz : dm.sx * dm.sy;
+    const unsigned lgrid = unsigned((nlines + 255) / 256);
     KernelTimer fh_timer(ctx, axis == 1 ? "edt_y" : "edt_z");
-    edt_fh<CC><<<(nlines + kFhBlock - 1) / kFhBlock, kFhBlock, 0, s>>>(cc, dm, axis, aniso[axis], bb, nlines, nlines,
-                                                                      V.data(), H.data(), out, axis == last, label_max);
+    Buffer<int> line_runs(ctx, size_t(nlines)), run_at(ctx, size_t(nlines));
+    Buffer<long long> line_spill(ctx, size_t(nlines)), spill_at(ctx, size_t(nlines));
+    edt_runs<CC><<<lgrid, 256, 0, s>>>(cc, dm, axis, bb, nlines, 0, out, line_runs.data(), line_spill.data(), nullptr,
+                                       nullptr, nullptr);
     BROOK_LAUNCHED();
+    exclusive_sum(ctx, line_runs.data(), run_at.data(), nlines);
+    exclusive_sum(ctx, line_spill.data(), spill_at.data(), nlines);
+    int last_runs = 0, last_at = 0;
+    long long last_spill = 0, last_spill_at = 0;
+    BROOK_HIP(hipMemcpyAsync(&last_runs, line_runs.data() + nlines - 1, 4, hipMemcpyDeviceToHost, s));
+    BROOK_HIP(hipMemcpyAsync(&last_at, run_at.data() + nlines - 1, 4, hipMemcpyDeviceToHost, s));
+    BROOK_HIP(hipMemcpyAsync(&last_spill, line_spill.data() + nlines - 1, 8, hipMemcpyDeviceToHost, s));
+    BROOK_HIP(hipMemcpyAsync(&last_spill_at, spill_at.data() + nlines - 1, 8, hipMemcpyDeviceToHost, s));
+    ctx.synchronize();
+    const long long total_runs = (long long)last_at + last_runs, total_spill = last_spill_at + last_spill;
+    if (total_runs > INT32_MAX || total_spill > INT32_MAX)
+      throw Unsupported("EDT envelope spill exceeds 2^31 entries (runs too long for this build)");
+    const int nruns = int(total_runs);
+    Buffer<int4> runs(ctx, size_t(std::max(nruns, 1)));
+    Buffer<int> V(ctx, size_t(std::max<long long>(total_spill, 1)));
+    Buffer<float> H(ctx, size_t(std::max<long long>(total_spill, 1)));
+    edt_runs<CC><<<lgrid, 256, 0, s>>>(cc, dm, axis, bb, nlines, axis == last, out, nullptr, nullptr, run_at.data(),
+                                       spill_at.data(), runs.data());
+    BROOK_LAUNCHED();
+    if (nruns) {
+      edt_run_fh<CC><<<unsigned((nruns + kRunBlock - 1) / kRunBlock), kRunBlock, 0, s>>>(
+          cc, dm, axis, aniso[axis], nruns, runs.data(), V.data(), H.data(), out, axis == last, label_max);
+      BROOK_LAUNCHED();
+    }
   }
 }
 

@@ -16,6 +16,14 @@ void exclusive_sum(Context &ctx, const int *in, int *out, long long n) {
   BROOK_HIP(hipcub::DeviceScan::ExclusiveSum(temp.data(), bytes, in, out, int(n), ctx.stream()));
 }
 
+void exclusive_sum(Context &ctx, const long long *in, long long *out, long long n) {
+  if (n <= 0) return;
+  size_t bytes = 0;
+  BROOK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, int(n), ctx.stream()));
+  Buffer<unsigned char> temp(ctx, std::max<size_t>(bytes, 1));
+  BROOK_HIP(hipcub::DeviceScan::ExclusiveSum(temp.data(), bytes, in, out, int(n), ctx.stream()));
+}
+
 void sort_segments(Context &ctx, const unsigned long long *in, unsigned long long *out, long long total,
                    const int *seg_begin, const int *seg_end, int segments) {
   if (total <= 0 || segments <= 0) return;

@@ -49,6 +49,9 @@ std::vector<Box> analyze(Context &ctx, const Components &cc, Dims dims);
 // When `label_max` is given (count+1 entries, zeroed), it receives the per-id maximum (float bits).
 void edt(Context &ctx, const void *cc, int cc_bytes, Dims dims, const float aniso[3], bool black_border, int dims_used,
          float *out, unsigned *label_max = nullptr);
+// Exclusive prefix sums on the context's stream (int and 64-bit element forms).
+void exclusive_sum(Context &ctx, const int *in, int *out, long long n);
+void exclusive_sum(Context &ctx, const long long *in, long long *out, long long n);
 // Label-dependent copy of one volume face into a plane of shape (fy, fx, 1) (borders.cu:14-20).
 void extract_face(Context &ctx, const void *cc, int cc_bytes, Dims dims, int a, int b, int c, int fixed, void *plane);
 
