@@ -47,6 +47,34 @@ __device__ __forceinline__ int wave_min(int v) {
   return v;
 }
 
+// One open railroad round on the help board (global memory).
+struct HelpJob {
+  int state;  // 0 free, 3 being filled, 1 open, 2 closing
+  int epoch;  // generation: a helper joins only the job it saw before announcing itself
+  int helpers;
+  int sample, nc, nchunks, next_chunk, done_chunks;
+  int n_nxt, n_far, n_touched;
+  float bound;
+  const int *cur;
+  int *nxt, *far, *touched;
+  unsigned long long rail, minfar;
+};
+constexpr int kHelpBoard = 64;
+// Helping is compiled out: its device-scope fences (an L2 writeback and invalidate per wave and
+// chunk on gfx950) stalled the whole chip, and the path stays off until it synchronizes within an
+// XCD instead.
+#ifndef BROOK_TRACE_HELP
+#define BROOK_TRACE_HELP 0
+#endif
+constexpr bool kTraceHelp = BROOK_TRACE_HELP != 0;
+
+struct Team {
+  HelpJob *board;  // null: no helping
+  const int *next;  // the label queue cursor
+  int nqueue, sample;
+  int *error;
+};
+
 struct Shared {
   int n_cur, n_nxt, n_far, n_farnxt, n_touched, n_keep;
   int done, target, path_len, found, band;
@@ -57,6 +85,7 @@ struct Shared {
   unsigned long long rail_best, minfar;
   int store_pos;
   int off[26];  // flat offsets of the 26 neighbours
+  int chunk, job, quit;
   float lane_du[kLaneFront * 2];
   int lane_u[kLaneFront * 2];
   unsigned lane_m[kLaneFront * 2];
@@ -189,6 +218,141 @@ __device__ __forceinline__ void relax_all(const TraceArgs &a, RoadMins &mins, in
   for_parts<kNbBatch>(m, [&](auto B, auto E) { relax_part<decltype(B)::value, decltype(E)::value>(a, mins, u, m, du, bound, p); });
 }
 
+// Dequeues front voxel u: marks it "not queued" and returns its distance, in one atomic.
+__device__ __forceinline__ float dequeue(const TraceArgs &a, int u) {
+  return __uint_as_float((unsigned)(atomicOr(&a.rec[u].word, 1ULL) >> 32));
+}
+
+// ---- help with large railroad rounds ---------------------------------------------------------
+// Once the label queue is empty, workgroups without a label help the ones still tracing: a label
+// whose railroad front reaches kHelpMin entries posts the round on a small board; owner and helpers
+// claim chunks of the front, append to the round's lists through global counters and fold their
+// minima into the job. The owner waits only for chunks already claimed (never for a helper to
+// arrive); helpers join a job only if its generation is unchanged across their join, and leave the
+// kernel when every label is done. Every spin is capped: past the cap the kernel reports an error
+// instead of waiting further.
+constexpr int kHelpMin = 4096, kHelpChunk = 1024;
+constexpr long long kSpinCap = 1LL << 24;       // owner waits (bounded by one chunk's work)
+constexpr long long kIdleCap = 1LL << 22;       // an idle helper leaves after this many empty polls
+
+__device__ __forceinline__ void relax_front(const TraceArgs &a, Shared &sh, RoadMins &mins, const int *cur, int lo, int hi,
+                                            float bound, int *nxt, int *far, int *touched, int *n_nxt, int *n_far,
+                                            int *n_touched) {
+  for (int base = lo; base < hi; base += blockDim.x) {  // uniform: every lane reaches the appends
+    const int i = base + threadIdx.x;
+    Pushes p;
+    int u = 0;
+    if (i < hi) {
+      u = lp_i(cur + i);
+      const unsigned m = (rec_meta(a.rec, u) & kNbrBits);
+      const float du = dequeue(a, u);
+      if (du <= bound) relax_all(a, mins, u, m, du, bound, p);
+    }
+    emit_neighbours(touched, n_touched, p.touched, u, sh.off);
+    emit_neighbours(nxt, n_nxt, p.near, u, sh.off);
+    emit_neighbours(far, n_far, p.far, u, sh.off);
+  }
+}
+
+// Claims and relaxes chunks of an open job until none is left (owner and helpers alike).
+__device__ __forceinline__ void help_chunks(const TraceArgs &a, Shared &sh, HelpJob *job) {
+  const int nc = ld_i(&job->nc), nchunks = ld_i(&job->nchunks);
+  const float bound = ld_f(&job->bound);
+  const int *cur = job->cur;
+  int *nxt = job->nxt, *far = job->far, *touched = job->touched;
+  for (;;) {
+    if (threadIdx.x == 0) sh.chunk = atomicAdd(&job->next_chunk, 1);
+    __syncthreads();
+    const int c = sh.chunk;
+    __syncthreads();
+    if (c >= nchunks) break;
+    RoadMins mins;
+    relax_front(a, sh, mins, cur, c * kHelpChunk, min(nc, (c + 1) * kHelpChunk), bound, nxt, far, touched, &job->n_nxt,
+                &job->n_far, &job->n_touched);
+    const unsigned long long r = wave_min_u64(mins.rail), f = wave_min_u64(mins.far);
+    if (__lane_id() == 0) {
+      if (r != kMaxKey) atomicMin(&job->rail, r);
+      if (f != kMinfarNone) atomicMin(&job->minfar, f);
+    }
+    __threadfence();  // this thread's list entries and record updates are visible device-wide
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(&job->done_chunks, 1);
+  }
+}
+
+// Spins until *p >= want (or the cap, which records an error); returns false on the cap.
+__device__ __forceinline__ bool spin_until_at_least(const int *p, int want, int *error) {
+  for (long long i = 0; ld_i(p) < want; ++i) {
+    if (i > kSpinCap) { atomicExch(error, 1); return false; }
+    __builtin_amdgcn_s_sleep(2);
+  }
+  return true;
+}
+// Spins until *p <= 0 (or the cap, which records an error).
+__device__ __forceinline__ bool spin_until_none(const int *p, int *error) {
+  for (long long i = 0; ld_i(p) > 0; ++i) {
+    if (i > kSpinCap) { atomicExch(error, 1); return false; }
+    __builtin_amdgcn_s_sleep(2);
+  }
+  return true;
+}
+
+// Owner side: posts the round [0, nc) of `cur` when a board slot is free; returns false (nothing
+// done) otherwise. On return true the round is complete and sh's list lengths and minima include
+// every helper's work.
+__device__ bool team_round(const TraceArgs &a, Shared &sh, const Team &team, const int *cur, int nc, float bound,
+                           int *nxt, int *far, int *touched) {
+  if (threadIdx.x == 0) {
+    sh.job = -1;
+    if (ld_i(team.next) >= team.nqueue)  // the label queue is empty: idle workgroups exist
+      for (int s = 0; s < kHelpBoard; ++s)
+        if (atomicCAS(&team.board[s].state, 0, 3) == 0) { sh.job = s; break; }
+  }
+  __syncthreads();
+  if (sh.job < 0) return false;
+  HelpJob *job = team.board + sh.job;
+  __threadfence();  // every wave: its front entries and record updates are visible device-wide
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    job->sample = team.sample;
+    job->nc = nc;
+    job->nchunks = (nc + kHelpChunk - 1) / kHelpChunk;
+    job->next_chunk = 0;
+    job->done_chunks = 0;
+    job->bound = bound;
+    job->cur = cur;
+    job->nxt = nxt;
+    job->far = far;
+    job->touched = touched;
+    job->n_nxt = sh.n_nxt;
+    job->n_far = sh.n_far;
+    job->n_touched = sh.n_touched;
+    job->rail = kMaxKey;
+    job->minfar = kMinfarNone;
+    job->epoch += 1;
+    __threadfence();
+    atomicExch(&job->state, 1);  // open
+  }
+  __syncthreads();
+  help_chunks(a, sh, job);
+  if (threadIdx.x == 0) {
+    spin_until_at_least(&job->done_chunks, ld_i(&job->nchunks), team.error);
+    atomicExch(&job->state, 2);  // closing: no new joins
+    spin_until_none(&job->helpers, team.error);
+    __threadfence();
+    sh.n_nxt = ld_i(&job->n_nxt);
+    sh.n_far = ld_i(&job->n_far);
+    sh.n_touched = ld_i(&job->n_touched);
+    const unsigned long long r = ld_u64(&job->rail), f = ld_u64(&job->minfar);
+    if (r < sh.rail_best) sh.rail_best = r;
+    if (f < sh.minfar) sh.minfar = f;
+    atomicExch(&job->state, 0);  // free
+  }
+  __threadfence();  // every wave: the helpers' list entries and record updates
+  __syncthreads();
+  return true;
+}
+
 __device__ __forceinline__ float rail_bound(const Shared &sh) {
   float b = sh.T;
   unsigned long long rb = sh.rail_best;
@@ -196,14 +360,9 @@ __device__ __forceinline__ float rail_bound(const Shared &sh) {
   return b;
 }
 
-// Dequeues front voxel u: marks it "not queued" and returns its distance, in one atomic.
-__device__ __forceinline__ float dequeue(const TraceArgs &a, int u) {
-  return __uint_as_float((unsigned)(atomicOr(&a.rec[u].word, 1ULL) >> 32));
-}
-
 // Returns 1 when the nearest rail is settled (sh.rail_best), 2 when no rail is reachable.
-__device__ __forceinline__ int railroad(const TraceArgs &a, Shared &sh, int t, int *near_a, int *near_b, int *far_a,
-                                        int *far_b, int *touched) {
+__device__ __forceinline__ int railroad(const TraceArgs &a, Shared &sh, const Team &team, int t, int *near_a,
+                                        int *near_b, int *far_a, int *far_b, int *touched) {
   if (threadIdx.x == 0) {
     touched[0] = t;
     sh.n_touched = 1;
@@ -246,21 +405,8 @@ __device__ __forceinline__ int railroad(const TraceArgs &a, Shared &sh, int t, i
           if (k >= 26 || du < 0.0f || !((sh.lane_m[i] >> k) & 1u)) continue;
           relax_one(a, sh, mins, sh.lane_u[i], k, du, bound, nxt, fcur, touched);
         }
-      } else {
-        for (int base = 0; base < nc; base += blockDim.x) {  // uniform: every lane reaches the appends
-          const int i = base + threadIdx.x;
-          Pushes p;
-          int u = 0;
-          if (i < nc) {
-            u = lp_i(cur + i);
-            const unsigned m = (rec_meta(a.rec, u) & kNbrBits);
-            const float du = dequeue(a, u);
-            if (du <= bound) relax_all(a, mins, u, m, du, bound, p);
-          }
-          emit_neighbours(touched, &sh.n_touched, p.touched, u, sh.off);
-          emit_neighbours(nxt, &sh.n_nxt, p.near, u, sh.off);
-          emit_neighbours(fcur, &sh.n_far, p.far, u, sh.off);
-        }
+      } else if (!kTraceHelp || nc < kHelpMin || !team.board || !team_round(a, sh, team, cur, nc, bound, nxt, fcur, touched)) {
+        relax_front(a, sh, mins, cur, 0, nc, bound, nxt, fcur, touched, &sh.n_nxt, &sh.n_far, &sh.n_touched);
       }
       road_fold(sh, mins);
       __syncthreads();
@@ -658,18 +804,62 @@ __device__ __forceinline__ int next_target(const TraceArgs &a, Shared &sh, int L
   }
 }
 
+// A workgroup without a label: joins open rounds until every label is done.
+__device__ __forceinline__ void help_loop(const TraceArgs *__restrict__ all, Shared &sh, HelpJob *board, const int *labels_done,
+                          int nqueue, int *error) {
+  long long idle = 0;
+  for (;;) {
+    __syncthreads();  // every wave has read the previous iteration's sh.quit / sh.job
+    if (threadIdx.x == 0) {
+      sh.quit = ld_i(labels_done) >= nqueue;
+      sh.job = -1;
+      for (int s = 0; s < kHelpBoard && !sh.quit && sh.job < 0; ++s) {
+        HelpJob *j = board + s;
+        const int e = ld_i(&j->epoch);
+        if (ld_i(&j->state) != 1) continue;
+        atomicAdd(&j->helpers, 1);
+        __threadfence();
+        if (ld_i(&j->state) == 1 && ld_i(&j->epoch) == e) sh.job = s;
+        else atomicSub(&j->helpers, 1);
+      }
+      if (!sh.quit && sh.job < 0) {
+        if (++idle > kIdleCap) sh.quit = 1;  // leaving is always safe: owners never wait for helpers to arrive
+        __builtin_amdgcn_s_sleep(8);
+      }
+    }
+    __syncthreads();
+    if (sh.quit) return;
+    if (sh.job < 0) continue;
+    __threadfence();  // records and lists as the owner left them (no stale L1 lines)
+    HelpJob *job = board + sh.job;
+    const TraceArgs a = all[__builtin_amdgcn_readfirstlane(ld_i(&job->sample))];
+    if (threadIdx.x < 26) sh.off[threadIdx.x] = nb_offset(int(threadIdx.x), a.d);
+    __syncthreads();
+    help_chunks(a, sh, job);
+    if (threadIdx.x == 0) atomicSub(&job->helpers, 1);
+    __syncthreads();
+    idle = 0;
+  }
+}
+
 __global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restrict__ all, const int2 *__restrict__ queue,
-                                                      int nqueue, int *next) {
+                                                      int nqueue, int *next, HelpJob *board, int *labels_done, int *error) {
   __shared__ Shared sh;
   __shared__ int item;
   for (;;) {
     __syncthreads();
     if (threadIdx.x == 0) item = atomicAdd(next, 1);
     __syncthreads();
-    if (item >= nqueue) return;
+    if (item >= nqueue) {
+      if constexpr (kTraceHelp)
+        if (board) help_loop(all, sh, board, labels_done, nqueue, error);
+      return;
+    }
     const int2 q = queue[item];
-    const TraceArgs a = all[__builtin_amdgcn_readfirstlane(q.x)];  // uniform: scalar registers
+    const int sample = __builtin_amdgcn_readfirstlane(q.x);
+    const TraceArgs a = all[sample];  // uniform: scalar registers
     const int L = __builtin_amdgcn_readfirstlane(q.y);
+    const Team team{board, next, nqueue, sample, error};
     if (threadIdx.x < 26) sh.off[threadIdx.x] = nb_offset(int(threadIdx.x), a.d);
     const int seg = a.seg_start[L], count = a.seg_count[L];
     int *l0 = a.lists[0] + seg, *l1 = a.lists[1] + seg, *l2 = a.lists[2] + seg, *l3 = a.lists[3] + seg,
@@ -688,7 +878,7 @@ __global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restri
       const int t = next_target(a, sh, L);
       lap(sh, 1);
       if (t < 0) break;
-      const int status = railroad(a, sh, t, l0, l1, l2, l3, l4);
+      const int status = railroad(a, sh, team, t, l0, l1, l2, l3, l4);
       lap(sh, 2);
       if (status == 1) backtrace(a, sh, t, count, walk);
       __syncthreads();
@@ -736,6 +926,11 @@ __global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restri
       sh.prof[0] = (unsigned long long)(wall_clock64() - label_begin);
       for (int j = 0; j < kProf; ++j) a.label_stats[(long long)kProf * L + j] = sh.prof[j];
     }
+    if constexpr (kTraceHelp)
+      if (threadIdx.x == 0 && board) {
+        __threadfence();
+        atomicAdd(labels_done, 1);
+      }
   }
 }
 
@@ -749,14 +944,20 @@ void trace_all(Context &ctx, const std::vector<TraceArgs> &args, const std::vect
   for (size_t i = 0; i < items.size(); ++i) q[i] = make_int2(items[i].sample, items[i].label);
   Buffer<int2> queue(ctx, q.size());
   queue.upload(ctx, q.data(), q.size());
-  Buffer<int> next(ctx, 1);
+  Buffer<int> next(ctx, 1), done(ctx, 2);  // queue cursor; labels done, error flag
   next.clear(ctx);
+  done.clear(ctx);
+  Buffer<HelpJob> board(ctx, kHelpBoard);
+  board.clear(ctx);
   const int nq = int(q.size());
   const int grid = std::min(nq, ctx.compute_units() * 2);
-  KernelTimer kt(ctx, "trace_labels");
-  trace_labels<<<grid, kBlock, 0, ctx.stream()>>>(args_dev.data(), queue.data(), nq, next.data());
-  BROOK_LAUNCHED();
-  ctx.synchronize();
+  {
+    KernelTimer kt(ctx, "trace_labels");
+    trace_labels<<<grid, kBlock, 0, ctx.stream()>>>(args_dev.data(), queue.data(), nq, next.data(),
+                                                    kTraceHelp ? board.data() : nullptr, done.data(), done.data() + 1);
+    BROOK_LAUNCHED();
+  }
+  if (done.download(ctx)[1]) throw std::runtime_error("trace: a helper wait exceeded its cap");
 }
 
 }  // namespace brook
