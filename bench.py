@@ -108,6 +108,49 @@ def batched():
                 plain_ms=round(t_plain * 1e3, 3), speedup=round(t_plain / t_batch, 3))
 
 
+def served(rates=(10.0,), count=24, seed=5005):
+    """Serving form: Poisson request streams, each request a host uint32 volume with every edge in
+    [128, 512] and 5-50 tube labels (8 distinct volumes drawn at random per request). The same
+    arrival sequence is served by the continuous-batching server and by a serial server (plain
+    calls, max_batch=1). Latency = result ready - arrival."""
+    import brook_amd
+    from brook_amd import synth
+    rng = np.random.default_rng(seed)
+    pool = []
+    for i in range(8):
+        shape = tuple(int(x) for x in rng.integers(128, 513, 3))
+        pool.append(synth.volume("tubes", shape, int(rng.integers(5, 51)), seed * 10000 + i))
+    params, aniso = PARAMS, (16, 16, 40)
+    brook_amd.skeletonize(pool[0], params, aniso, dust_threshold=1000, output="packed", progress=False)
+    out = {}
+    for lam in rates:
+        gaps = rng.exponential(1.0 / lam, count)
+        picks = rng.integers(0, len(pool), count)
+        voxels = sum(pool[j].size for j in picks)
+        row = {}
+        for mode, max_batch in (("batched", 64), ("serial", 1)):
+            with brook_amd.Server(params, aniso, 1000, max_batch=max_batch) as srv:
+                t0 = time.perf_counter()
+                at = t0
+                futures = []
+                for g, j in zip(gaps, picks):
+                    at += g
+                    wait = at - time.perf_counter()
+                    if wait > 0:
+                        time.sleep(wait)
+                    futures.append(srv.submit(pool[j]))
+                for f in futures:
+                    f.result()
+                lat = [(f.request.done - f.request.arrival) * 1e3 for f in futures]
+                span = max(f.request.done for f in futures) - t0
+                row[mode] = dict(p50_ms=round(float(np.percentile(lat, 50)), 2),
+                                 p99_ms=round(float(np.percentile(lat, 99)), 2),
+                                 req_per_s=round(count / span, 3), mvox_s=round(voxels / span / 1e6, 3),
+                                 mean_batch=round(float(np.mean(srv.batches)), 2))
+        out[f"{lam:g}"] = dict(requests=count, **row)
+    return out
+
+
 def cpu_baseline(vol, threads):
     """The oracle (a C++ port of the reference's algorithm) on a 256^3 crop of the same volume."""
     import oracle
@@ -126,7 +169,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-extras", action="store_true", help="skip the host-input and batched measurements")
+    ap.add_argument("--no-extras", action="store_true", help="skip the host-input, batched and served measurements")
+    ap.add_argument("--served-sweep", action="store_true", help="serve at 2, 10, 20 and 50 requests/s (default: 10)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy three times (the known-size reference "
@@ -226,6 +270,7 @@ def main():
     if not args.no_extras:
         out["host_io"] = host_io(vol)
         out["batched"] = batched()
+        out["served"] = served((2.0, 10.0, 20.0, 50.0) if args.served_sweep else (10.0,))
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(vol, args.cpu_threads)
     else:

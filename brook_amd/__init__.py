@@ -11,6 +11,8 @@ from ._intake import DEFAULT_TEASAR_PARAMS, DimensionError, format_labels, prepa
 from ._native import DeviceError  # noqa: E402
 from ._packed import BatchedSkeletons, PackedSkeletons, Skeleton  # noqa: E402
 from .batch import skeletonize_batch  # noqa: E402
+from .serve import Server  # noqa: E402
 
 __all__ = ["skeletonize", "skeletonize_batch", "BatchedSkeletons", "PackedSkeletons", "Skeleton",
-           "DEFAULT_TEASAR_PARAMS", "DimensionError", "DeviceError", "format_labels", "prepare_fields", "__version__"]
+           "DEFAULT_TEASAR_PARAMS", "DimensionError", "DeviceError", "format_labels", "prepare_fields", "Server",
+           "__version__"]

@@ -30,6 +30,35 @@ def _per_sample(entries, count, name):
     return [[] if e is None else [tuple(int(c) for c in p) for p in e] for e in entries]
 
 
+def batch_parts(items, teasar_params, anisotropy, dust_threshold, fix_branching=True, fix_borders=True,
+                fill_holes=False, fix_avocados=False, before=None, after=None):
+    """One PackedSkeletons per sample (host arrays): samples without per-sample targets share one
+    engine batch (one flood and one trace launch per group); otherwise one engine call each."""
+    before = before or [[] for _ in items]
+    after = after or [[] for _ in items]
+    keep = []
+    ctx = native.context()
+    if all(not b for b in before) and all(not a for a in after):
+        if not items:
+            return []
+        vols = [volume_struct(s, keep)[0] for s in items]
+        opts = options_struct(teasar_params, anisotropy, dust_threshold, fix_branching, fix_borders, fill_holes,
+                              fix_avocados, None, (), (), keep)
+        arr = (ctypes.POINTER(native.Volume) * len(vols))(*[ctypes.pointer(v) for v in vols])
+        handles = (ctypes.c_void_p * len(vols))()
+        native.check(native.lib().brook_skeletonize_batch(ctx.handle, arr, len(vols), ctypes.byref(opts), handles))
+        return [_packed_from_handle(ctypes.c_void_p(h)) for h in handles]
+    parts = []
+    for s, b, a in zip(items, before, after):
+        vol, _ = volume_struct(s, keep)
+        opts = options_struct(teasar_params, anisotropy, dust_threshold, fix_branching, fix_borders, fill_holes,
+                              fix_avocados, None, b, a, keep)
+        h = ctypes.c_void_p()
+        native.check(native.lib().brook_skeletonize(ctx.handle, ctypes.byref(vol), ctypes.byref(opts), ctypes.byref(h)))
+        parts.append(_packed_from_handle(h))
+    return parts
+
+
 def skeletonize_batch(samples, teasar_params=DEFAULT_TEASAR_PARAMS, anisotropy=(1, 1, 1), dust_threshold=1000,
                       fix_branching=True, fix_borders=True, fill_holes=False, fix_avocados=False,
                       extra_targets_before=None, extra_targets_after=None, progress=False, output="packed_device",
@@ -45,26 +74,8 @@ def skeletonize_batch(samples, teasar_params=DEFAULT_TEASAR_PARAMS, anisotropy=(
     before = _per_sample(extra_targets_before, len(items), "extra_targets_before")
     after = _per_sample(extra_targets_after, len(items), "extra_targets_after")
     anisotropy = tuple(float(a) for a in anisotropy)
-    keep = []
-    ctx = native.context()
-    parts = []
-    uniform = all(not b for b in before) and all(not a for a in after)
-    if uniform and items:
-        vols = [volume_struct(s, keep)[0] for s in items]
-        opts = options_struct(teasar_params, anisotropy, dust_threshold, fix_branching, fix_borders, fill_holes,
-                              fix_avocados, None, (), (), keep)
-        arr = (ctypes.POINTER(native.Volume) * len(vols))(*[ctypes.pointer(v) for v in vols])
-        handles = (ctypes.c_void_p * len(vols))()
-        native.check(native.lib().brook_skeletonize_batch(ctx.handle, arr, len(vols), ctypes.byref(opts), handles))
-        parts = [_packed_from_handle(ctypes.c_void_p(h)) for h in handles]
-    else:
-        for s, b, a in zip(items, before, after):
-            vol, _ = volume_struct(s, keep)
-            opts = options_struct(teasar_params, anisotropy, dust_threshold, fix_branching, fix_borders, fill_holes,
-                                  fix_avocados, None, b, a, keep)
-            h = ctypes.c_void_p()
-            native.check(native.lib().brook_skeletonize(ctx.handle, ctypes.byref(vol), ctypes.byref(opts), ctypes.byref(h)))
-            parts.append(_packed_from_handle(h))
+    parts = batch_parts(items, teasar_params, anisotropy, dust_threshold, fix_branching, fix_borders, fill_holes,
+                        fix_avocados, before, after)
     parts = [p if len(p) else PackedSkeletons.from_skeletons({}, anisotropy) for p in parts]
     if output == "skeletons":
         return [p.to_skeletons() for p in parts]

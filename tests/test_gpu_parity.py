@@ -188,6 +188,21 @@ def test_batch_equals_separate(brook):
         _assert_packed_equal(batch[i], single.as_dict(), f"sample {i}")
 
 
+def test_served_results_equal_plain_calls(brook):
+    """Continuous batching: requests that share a batch give the results of plain calls."""
+    from brook_amd import synth
+    shapes = [((64, 80, 72), 6), ((96, 64, 64), 9), ((64, 64, 100), 4), ((72, 72, 72), 7), ((80, 64, 96), 5)]
+    vols = [synth.volume("tubes", s, n, 90 + i) for i, (s, n) in enumerate(shapes)]
+    opts = dict(teasar_params=CFG1_PARAMS, anisotropy=(16, 16, 40), dust_threshold=500)
+    plain = [brook.skeletonize(v, output="packed", progress=False, **opts) for v in vols]
+    with brook.Server(**opts) as srv:
+        futures = [srv.submit(v) for v in vols]
+        got = [f.result(120) for f in futures]
+    assert sum(srv.batches) == len(vols) and max(srv.batches) > 1, srv.batches
+    for i, (g, p) in enumerate(zip(got, plain)):
+        _assert_packed_equal(g, p.as_dict(), f"request {i} (batches {srv.batches})")
+
+
 def test_edge_cases(brook):
     # empty, all background, volume at or below the dust threshold, a single voxel label
     assert brook.skeletonize(np.zeros((0, 4, 4), np.uint32), progress=False) == {}
