@@ -68,6 +68,11 @@ __device__ __forceinline__ void unite(int *parent, int a, int b) {
   }
 }
 
+// Unions between x-runs of equal label in the four backward rows ((y-1, z) and (y-1..y+1, z-1);
+// x-neighbours in the voxel's own row are one run already). Every voxel of a run shares the run's
+// root, so one union per pair of touching runs suffices: voxel x of run [c, d] unites with the
+// row's run at x where that run begins at x or x == c (centre contacts), and the run ends take
+// the diagonal contacts (x-1 at c, x+1 at d when the centre differs).
 template <class T>
 __global__ void __launch_bounds__(256) ccl_union(const T *lab, int *parent, Dims d, int n) {
   int i = blockIdx.x * 256 + threadIdx.x;
@@ -76,17 +81,18 @@ __global__ void __launch_bounds__(256) ccl_union(const T *lab, int *parent, Dims
   if (l == T(0)) return;
   int x, y, z;
   unravel(i, d, x, y, z);
-  // rows whose voxels precede i: (y-1, z) and (y-1..y+1, z-1); x-neighbours in i's own row are
-  // one run already. A row's centre voxel of equal label joins its whole run; else both sides.
+  const bool first = x == 0 || lab[i - 1] != l, last = x + 1 == d.sx || lab[i + 1] != l;
   const int rows[4][2] = {{-1, 0}, {-1, -1}, {0, -1}, {1, -1}};
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     int yy = y + rows[r][0], zz = z + rows[r][1];
     if (yy < 0 || yy >= d.sy || zz < 0) continue;
     int row = d.sx * (yy + d.sy * zz);
-    if (lab[row + x] == l) { unite(parent, i, row + x); continue; }
-    if (x > 0 && lab[row + x - 1] == l) unite(parent, i, row + x - 1);
-    if (x + 1 < d.sx && lab[row + x + 1] == l) unite(parent, i, row + x + 1);
+    const bool centre = lab[row + x] == l;
+    const bool left = x > 0 && lab[row + x - 1] == l;
+    if (centre && (first || !left)) unite(parent, i, row + x);
+    if (first && left && !centre) unite(parent, i, row + x - 1);
+    if (last && !centre && x + 1 < d.sx && lab[row + x + 1] == l) unite(parent, i, row + x + 1);
   }
 }
 
