@@ -18,10 +18,6 @@ namespace {
 constexpr int kBG = INT32_MIN;  // parent of a background voxel
 constexpr int kTile = 4096;     // voxels per root-count tile (256 threads x 16)
 
-__device__ __forceinline__ unsigned long long lanes_upto(int lane) {
-  return lane == 63 ? ~0ULL : ((2ULL << lane) - 1ULL);
-}
-
 template <class T>
 __global__ void __launch_bounds__(256) ccl_init(const T *lab, int *parent, Dims d, int n) {
   int i = blockIdx.x * 256 + threadIdx.x;
@@ -172,31 +168,62 @@ __global__ void __launch_bounds__(256) ccl_relabel(const int *parent, int n, CC 
   cc[i] = CC(-p);
 }
 
-// ---- analysis: per-id bbox and count, one atomic group per run segment of a wave ----------
+// ---- analysis: per-id bbox and count ---------------------------------------------------------
+// A workgroup walks kRunChunk consecutive voxels; each x-run within a wave updates its label's
+// LDS slot, and the workgroup then sends one count/bbox update per label it saw. (Few labels over
+// many runs otherwise serialize on the same global addresses.)
+constexpr int kRunChunk = 16384;
+
 template <class CC>
 __global__ void __launch_bounds__(256) analyze_runs(const CC *cc, Dims d, int n, int *lo, int *hi,
                                                     unsigned long long *counts) {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  int lane = threadIdx.x & 63;
-  bool valid = i < n;
-  CC l = valid ? cc[i] : CC(0);
-  int x = valid ? i % d.sx : 0;
-  CC prev = shfl_up_any(l, 1);
-  bool start = !valid || lane == 0 || x == 0 || prev != l;
-  unsigned long long starts = __ballot(start);
-  if (!valid || l == CC(0) || !start) return;
-  unsigned long long after = starts & ~lanes_upto(lane);
-  int end_lane = after ? __ffsll((long long)after) - 2 : 63;
-  int len = end_lane - lane + 1;
-  int y = (i / d.sx) % d.sy, z = i / (d.sx * d.sy);
-  atomicAdd(counts + l, (unsigned long long)len);
-  int *L = lo + 3 * int(l), *H = hi + 3 * int(l);
-  if (ld_i(L) > x) atomicMin(L, x);
-  if (ld_i(H) < x + len - 1) atomicMax(H, x + len - 1);
-  if (ld_i(L + 1) > y) atomicMin(L + 1, y);
-  if (ld_i(H + 1) < y) atomicMax(H + 1, y);
-  if (ld_i(L + 2) > z) atomicMin(L + 2, z);
-  if (ld_i(H + 2) < z) atomicMax(H + 2, z);
+  __shared__ LabelSlots slots;
+  __shared__ unsigned cnt[kLabelSlots];
+  __shared__ int box[6][kLabelSlots];  // lo x,y,z then hi x,y,z
+  slots.init();
+  for (int s = threadIdx.x; s < kLabelSlots; s += 256) {
+    cnt[s] = 0;
+    for (int a = 0; a < 3; ++a) { box[a][s] = INT32_MAX; box[3 + a][s] = -1; }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const long long begin = (long long)blockIdx.x * kRunChunk;
+  for (int t = 0; t < kRunChunk; t += 256) {
+    const long long g = begin + t + threadIdx.x;
+    const bool valid = g < n;
+    const int i = valid ? int(g) : 0;
+    const CC l = valid ? cc[i] : CC(0);
+    const int x = valid ? i % d.sx : 0;
+    const WaveRun r = wave_run(l, valid, x, lane);
+    if (!valid || l == CC(0) || !r.head) continue;
+    const int y = (i / d.sx) % d.sy, z = i / (d.sx * d.sy), xe = x + r.len - 1;
+    const int s = slots.insert(int(l));
+    if (s >= 0) {
+      atomicAdd(&cnt[s], unsigned(r.len));
+      atomicMin(&box[0][s], x);
+      atomicMin(&box[1][s], y);
+      atomicMin(&box[2][s], z);
+      atomicMax(&box[3][s], xe);
+      atomicMax(&box[4][s], y);
+      atomicMax(&box[5][s], z);
+      continue;
+    }
+    atomicAdd(counts + l, (unsigned long long)r.len);
+    int *L = lo + 3 * int(l), *H = hi + 3 * int(l);
+    atomicMin(L, x); atomicMin(L + 1, y); atomicMin(L + 2, z);
+    atomicMax(H, xe); atomicMax(H + 1, y); atomicMax(H + 2, z);
+  }
+  __syncthreads();
+  for (int s = threadIdx.x; s < kLabelSlots; s += 256) {
+    const int l = slots.tag[s];
+    if (l < 0) continue;
+    atomicAdd(counts + l, (unsigned long long)cnt[s]);
+    int *L = lo + 3 * l, *H = hi + 3 * l;
+    for (int a = 0; a < 3; ++a) {
+      if (ld_i(L + a) > box[a][s]) atomicMin(L + a, box[a][s]);
+      if (ld_i(H + a) < box[3 + a][s]) atomicMax(H + a, box[3 + a][s]);
+    }
+  }
 }
 
 template <class CC>
@@ -284,7 +311,7 @@ template <class CC> void run_analyze(Context &ctx, const Components &cc, Dims d,
   lo.upload(ctx, l0.data(), l0.size());
   hi.upload(ctx, h0.data(), h0.size());
   counts.clear(ctx);
-  analyze_runs<CC><<<blocks_for(n, 256), 256, 0, ctx.stream()>>>(reinterpret_cast<const CC *>(cc.labels.data()), d, n,
+  analyze_runs<CC><<<blocks_for(n, kRunChunk), 256, 0, ctx.stream()>>>(reinterpret_cast<const CC *>(cc.labels.data()), d, n,
                                                                   lo.data(), hi.data(), counts.data());
   BROOK_LAUNCHED();
   auto L = lo.download(ctx), H = hi.download(ctx);

@@ -206,4 +206,54 @@ template <class T> __device__ __forceinline__ T shfl_up_any(T v, unsigned delta)
   else return T(__shfl_up((unsigned)v, delta, 64));
 }
 
+// Mask of lanes 0..lane.
+__device__ __forceinline__ unsigned long long lanes_upto(int lane) {
+  return lane == 63 ? ~0ULL : ((2ULL << lane) - 1ULL);
+}
+
+// Runs of one label among a wave's 64 consecutive voxels of one x-row: every lane learns the lane
+// its run starts at; a run's first lane (head) also learns the run's length. Invalid lanes are
+// runs of their own. Every lane of the wave must call it.
+struct WaveRun {
+  bool head;
+  int start_lane, len;
+};
+template <class T> __device__ __forceinline__ WaveRun wave_run(T l, bool valid, int x, int lane) {
+  const T prev = shfl_up_any(l, 1);
+  const bool start = !valid || lane == 0 || x == 0 || prev != l;
+  const unsigned long long starts = __ballot(start);
+  const unsigned long long after = starts & ~lanes_upto(lane);
+  WaveRun r;
+  r.head = start;
+  r.start_lane = 63 - __clzll((long long)(starts & lanes_upto(lane)));
+  r.len = (after ? __ffsll((long long)after) - 2 : 63) - lane + 1;
+  return r;
+}
+
+// A workgroup's LDS table of labels (open addressing, bounded probing): run updates of one label
+// fold into its slot, and the workgroup sends one global update per label it saw instead of one
+// per run. A label whose probe chain is full gets -1 for good (slots are never freed), and its
+// runs go to global memory directly.
+constexpr int kLabelSlots = 256, kLabelProbes = 8;
+struct LabelSlots {
+  int tag[kLabelSlots];  // label, or -1
+  __device__ void init() {
+    for (int s = threadIdx.x; s < kLabelSlots; s += blockDim.x) tag[s] = -1;
+  }
+  static __device__ __forceinline__ int home(int l) { return int((unsigned(l) * 0x9E3779B1u) >> 24); }
+  __device__ int insert(int l) {
+    for (int p = 0, s = home(l); p < kLabelProbes; ++p, s = (s + 1) & (kLabelSlots - 1)) {
+      const int old = atomicCAS(&tag[s], -1, l);
+      if (old == -1 || old == l) return s;
+    }
+    return -1;
+  }
+  __device__ int find(int l) const {
+    for (int p = 0, s = home(l); p < kLabelProbes; ++p, s = (s + 1) & (kLabelSlots - 1))
+      if (tag[s] == l) return s;
+    return -1;
+  }
+};
+static_assert(kLabelSlots == 256, "home() keeps the top 8 bits of the hash");
+
 }  // namespace brook

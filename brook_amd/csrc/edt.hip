@@ -18,8 +18,6 @@
 namespace brook {
 namespace {
 
-__device__ __forceinline__ unsigned long long mask_upto(int lane) { return lane == 63 ? ~0ULL : ((2ULL << lane) - 1ULL); }
-
 template <class CC, int MAXCH>
 __global__ void __launch_bounds__(256) edt_x(const CC *cc, int sx, int rows, const float *acc, int bb, float *D) {
   const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -41,7 +39,7 @@ __global__ void __launch_bounds__(256) edt_x(const CC *cc, int sx, int rows, con
       if (lane == 0) prev = tail;
       bool start = valid && (p == 0 || prev != l);
       unsigned long long S = __ballot(start);
-      unsigned long long below = S & mask_upto(lane);
+      unsigned long long below = S & lanes_upto(lane);
       cst[j] = below ? j * 64 + 63 - __clzll((long long)below) : carry;
       if (S) carry = j * 64 + 63 - __clzll((long long)S);
       lab[j] = l;

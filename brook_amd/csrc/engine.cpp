@@ -220,10 +220,9 @@ class Sample {
     }
     root_rails(ctx_, rec_.data(), root_dev_.data(), labels_dev.data(), int(labels_.size()));
     const size_t fl = size_t(std::max<int64_t>(F_, 1));
-    order_ = Buffer<unsigned long long>(ctx_, fl);
+    order_ = Buffer<int>(ctx_, fl);
     snap_a_ = Buffer<unsigned long long>(ctx_, fl);
-    target_order_keys(ctx_, vox_.data(), F_, daf_.data(), snap_a_.data());
-    sort_segments(ctx_, snap_a_.data(), order_.data(), F_, seg_start_dev_.data(), seg_start_dev_.data() + 1, int(k_));
+    target_order(ctx_, cc_.labels.data(), cc_.cc_bytes, n_, daf_.data(), F_, count_, order_.data());
     lap("prep_pdrf_order_ms");
 
     // manual targets and path budgets (lockstep.cu:116-143)
@@ -429,7 +428,8 @@ class Sample {
   Buffer<long long> first_dev_, root_dev_, target_dev_;
   Buffer<VoxelRec> rec_;
   std::vector<Buffer<int>> lists_;
-  Buffer<unsigned long long> order_, snap_a_, snap_b_;
+  Buffer<int> order_;
+  Buffer<unsigned long long> snap_a_, snap_b_;
   Buffer<int> mstart_dev_, mcount_dev_, maxp_dev_, mflat_dev_;
   long long store_cap_ = 0;
   int rec_cap_ = 0;

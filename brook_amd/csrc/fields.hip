@@ -13,31 +13,53 @@
 namespace brook {
 namespace {
 
-__device__ __forceinline__ unsigned long long upto(int lane) { return lane == 63 ? ~0ULL : ((2ULL << lane) - 1ULL); }
-
 // Voxel lists: vox[seg_start[l] + slot] = every voxel of id l (slot order is free; consumers
-// that need an order sort the list or reduce over it).
+// that need an order sort the list or reduce over it). A workgroup walks kListChunk consecutive
+// voxels twice: the first pass totals each label's voxels in LDS and reserves them with one global
+// add per label, the second hands out places within the reservation per x-run.
+constexpr int kListChunk = 16384;
+
 template <class CC>
 __global__ void __launch_bounds__(256) fill_segments(const CC *cc, int n, Dims d, const int *seg_start, int *fill,
                                                      int *vox) {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  int lane = threadIdx.x & 63;
-  bool valid = i < n;
-  CC l = valid ? cc[i] : CC(0);
-  int x = valid ? i % d.sx : 0;
-  CC prev = shfl_up_any(l, 1);
-  bool start = !valid || lane == 0 || x == 0 || prev != l;
-  unsigned long long starts = __ballot(start);
-  unsigned long long mine = starts & upto(lane);
-  int my_start = 63 - __clzll((long long)mine);  // start lane of this lane's segment
-  int base = 0;
-  if (start && valid && l != CC(0)) {
-    unsigned long long after = starts & ~upto(lane);
-    int end_lane = after ? __ffsll((long long)after) - 2 : 63;
-    base = atomicAdd(fill + l, end_lane - lane + 1);
+  __shared__ LabelSlots slots;
+  __shared__ int cnt[kLabelSlots], base[kLabelSlots];
+  slots.init();
+  for (int s = threadIdx.x; s < kLabelSlots; s += 256) cnt[s] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const long long begin = (long long)blockIdx.x * kListChunk;
+  for (int t = 0; t < kListChunk; t += 256) {
+    const long long g = begin + t + threadIdx.x;
+    const bool valid = g < n;
+    const int i = valid ? int(g) : 0;
+    const CC l = valid ? cc[i] : CC(0);
+    const WaveRun r = wave_run(l, valid, valid ? i % d.sx : 0, lane);
+    if (!valid || l == CC(0) || !r.head) continue;
+    const int s = slots.insert(int(l));
+    if (s >= 0) atomicAdd(&cnt[s], r.len);
   }
-  base = __shfl(base, my_start, 64);
-  if (valid && l != CC(0)) vox[seg_start[l] + base + (lane - my_start)] = i;
+  __syncthreads();
+  for (int s = threadIdx.x; s < kLabelSlots; s += 256) {
+    const int l = slots.tag[s];
+    if (l >= 0) base[s] = atomicAdd(fill + l, cnt[s]);
+    cnt[s] = 0;
+  }
+  __syncthreads();
+  for (int t = 0; t < kListChunk; t += 256) {
+    const long long g = begin + t + threadIdx.x;
+    const bool valid = g < n;
+    const int i = valid ? int(g) : 0;
+    const CC l = valid ? cc[i] : CC(0);
+    const WaveRun r = wave_run(l, valid, valid ? i % d.sx : 0, lane);
+    int at = 0;
+    if (valid && l != CC(0) && r.head) {
+      const int s = slots.find(int(l));
+      at = s >= 0 ? base[s] + atomicAdd(&cnt[s], r.len) : atomicAdd(fill + l, r.len);
+    }
+    at = __shfl(at, r.start_lane, 64);
+    if (valid && l != CC(0)) vox[seg_start[l] + at + (lane - r.start_lane)] = i;
+  }
 }
 
 // Voxel records: the neighbour mask (bit k = neighbour k of v is inside the volume and has v's
@@ -240,18 +262,8 @@ __global__ void set_rails(VoxelRec *rec, const long long *root, const int *label
   if (i < count) rec[root[labels[i]]].pdrf = 0.0f;
 }
 
-// Target-order key of each listed voxel: ascending keys = DAF descending (inf and 0 as 0),
-// flat index descending (lockstep_order.cu:21-43).
-__global__ void order_keys(const int *vox, long long total, const float *daf, unsigned long long *keys) {
-  long long j = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (j >= total) return;
-  int v = vox[j];
-  float f = daf[v];
-  if (isinf(f) || f == 0.0f) f = 0.0f;
-  keys[j] = ((unsigned long long)(0xFFFFFFFFu - __float_as_uint(f)) << 32) | (unsigned long long)(0xFFFFFFFFu - (unsigned)v);
-}
-
 unsigned grid_for(long long n) { return unsigned((n + 255) / 256); }
+unsigned blocks_of(long long n, int per) { return unsigned((n + per - 1) / per); }
 
 }  // namespace
 
@@ -264,13 +276,13 @@ void build_segments(Context &ctx, const void *cc, int cc_bytes, Dims d, const st
   fill.clear(ctx);
   vox = Buffer<int>(ctx, std::max(1, seg_start.back()));
   if (cc_bytes == 1)
-    fill_segments<uint8_t><<<grid_for(n), 256, 0, ctx.stream()>>>(static_cast<const uint8_t *>(cc), n, d,
+    fill_segments<uint8_t><<<blocks_of(n, kListChunk), 256, 0, ctx.stream()>>>(static_cast<const uint8_t *>(cc), n, d,
                                                                   seg_start_dev.data(), fill.data(), vox.data());
   else if (cc_bytes == 2)
-    fill_segments<uint16_t><<<grid_for(n), 256, 0, ctx.stream()>>>(static_cast<const uint16_t *>(cc), n, d,
+    fill_segments<uint16_t><<<blocks_of(n, kListChunk), 256, 0, ctx.stream()>>>(static_cast<const uint16_t *>(cc), n, d,
                                                                    seg_start_dev.data(), fill.data(), vox.data());
   else
-    fill_segments<uint32_t><<<grid_for(n), 256, 0, ctx.stream()>>>(static_cast<const uint32_t *>(cc), n, d,
+    fill_segments<uint32_t><<<blocks_of(n, kListChunk), 256, 0, ctx.stream()>>>(static_cast<const uint32_t *>(cc), n, d,
                                                                    seg_start_dev.data(), fill.data(), vox.data());
   BROOK_LAUNCHED();
 }
@@ -331,12 +343,6 @@ void pdrf_and_alive(Context &ctx, const void *cc, int cc_bytes, long long n, con
 void root_rails(Context &ctx, VoxelRec *rec, const long long *root, const int *labels, int count) {
   if (!count) return;
   set_rails<<<grid_for(count), 256, 0, ctx.stream()>>>(rec, root, labels, count);
-  BROOK_LAUNCHED();
-}
-
-void target_order_keys(Context &ctx, const int *vox, long long total, const float *daf, unsigned long long *keys) {
-  if (!total) return;
-  order_keys<<<grid_for(total), 256, 0, ctx.stream()>>>(vox, total, daf, keys);
   BROOK_LAUNCHED();
 }
 

@@ -40,10 +40,10 @@ void pdrf_and_alive(Context &ctx, const void *cc, int cc_bytes, long long n, con
                     const float *M, const float *inv, const unsigned char *use, int squarings, float scale, float *pdrf,
                     VoxelRec *rec);
 void root_rails(Context &ctx, VoxelRec *rec, const long long *root, const int *labels, int count);
-void target_order_keys(Context &ctx, const int *vox, long long total, const float *daf, unsigned long long *keys);
-// Segmented ascending sort of 64-bit keys (segments = id voxel lists).
-void sort_segments(Context &ctx, const unsigned long long *in, unsigned long long *out, long long total,
-                   const int *seg_begin, const int *seg_end, int segments);
+// Per-id target order in segment layout: order[seg_start[id] + j] = the id's voxels by descending
+// DAF (inf and 0 as 0), ties by descending flat index (lockstep_order.cu:21-43); F foreground voxels.
+void target_order(Context &ctx, const void *cc, int cc_bytes, long long n, const float *daf, long long F, int64_t count,
+                  int *order);
 // Ascending sort of 64-bit keys over [0, end_bit).
 void sort_keys(Context &ctx, unsigned long long *keys, unsigned long long *scratch, long long count, int end_bit);
 

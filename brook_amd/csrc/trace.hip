@@ -784,7 +784,7 @@ __device__ __forceinline__ int next_target(const TraceArgs &a, Shared &sh, int L
     for (int r = 0; r < per; ++r) {
       long long j = c + r * blockDim.x + threadIdx.x;
       if (j < e) {
-        int v = int(0xFFFFFFFFu - (unsigned)a.order[j]);
+        int v = a.order[j];
         if ((rec_meta(a.rec, v) & kAlive)) atomicMin(&sh.found, int(j - c));
       }
     }
@@ -793,7 +793,7 @@ __device__ __forceinline__ int next_target(const TraceArgs &a, Shared &sh, int L
       __syncthreads();
       if (threadIdx.x == 0) {
         sh.cursor = c + sh.found;
-        sh.target = int(0xFFFFFFFFu - (unsigned)a.order[c + sh.found]);
+        sh.target = a.order[c + sh.found];
       }
       __syncthreads();
       return sh.target;

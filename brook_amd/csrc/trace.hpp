@@ -13,7 +13,7 @@ struct TraceArgs {
   VoxelRec *rec = nullptr;  // voxel records: PDRF (rails become 0 in place), flags, words (~0 between uses)
   const float *dbf = nullptr;
   const int *seg_start = nullptr, *seg_count = nullptr;
-  const unsigned long long *order = nullptr;  // sorted order keys in segment layout (voxel = ~low 32 bits)
+  const int *order = nullptr;  // per-label target order (voxels) in segment layout
   const int *mstart = nullptr, *mcount = nullptr, *mflat = nullptr, *maxp = nullptr;
   int *lists[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // int scratch, segment layout
   unsigned long long *snaps[2] = {nullptr, nullptr};                       // u64 scratch, segment layout
