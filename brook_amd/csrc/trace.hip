@@ -107,10 +107,17 @@ __device__ __forceinline__ void lap(Shared &sh, int slot) {
 
 // ---- railroad -------------------------------------------------------------------------------
 // Distances live in the records' packed words: high half the distance bits, bit 0 "not
-// queued". A front voxel is dequeued by one atomicOr that also returns its distance; an improvement
-// is one atomicMin whose old value says whether the voxel still has to be queued.
-// Rails (PDRF 0) and far candidates keep "not queued"; near candidates store "queued".
+// queued", bit 1 "far listed". A front voxel is dequeued by one atomicOr that also returns its
+// distance; an improvement is one atomicMin whose old value says whether the voxel still has to
+// be queued or far-listed. Near candidates store 00 (queued), rails 01, far candidates 11; a near
+// improvement of a far-listed voxel clears its far bit, so its far-list entry goes stale.
 // Per-thread minima of the reached rails and far distances are folded into LDS once per round.
+constexpr unsigned long long kRoadNear = 0, kRoadRail = 1, kRoadFar = 3;
+// A word whose far bit is set by a far candidate (the unreached sentinel ~0 is not far-listed).
+__device__ __forceinline__ bool far_listed(unsigned long long w) {
+  return (w & 2ULL) && (unsigned)(w >> 32) != 0xFFFFFFFFu;
+}
+
 struct RoadMins {
   unsigned long long rail = kMaxKey, far = kMinfarNone;
 };
@@ -139,7 +146,7 @@ __device__ __forceinline__ void relax_one(const TraceArgs &a, Shared &sh, RoadMi
   const unsigned cb = __float_as_uint(cand);
   if (rec_word_hi(a.rec, v) <= cb || !(cand < kInf)) return;
   const bool rail = fv == 0.0f, near = !rail && cand <= bound;
-  const unsigned long long nv = ((unsigned long long)cb << 32) | (near ? 0ULL : 1ULL);
+  const unsigned long long nv = ((unsigned long long)cb << 32) | (near ? kRoadNear : rail ? kRoadRail : kRoadFar);
   const unsigned long long old = atomicMin(&a.rec[v].word, nv);
   if (!(old > nv)) return;  // nothing changed
   if ((unsigned)(old >> 32) == 0xFFFFFFFFu) touched[wave_slot(&sh.n_touched)] = v;
@@ -151,7 +158,7 @@ __device__ __forceinline__ void relax_one(const TraceArgs &a, Shared &sh, RoadMi
   } else {
     const unsigned long long key = ((unsigned long long)cb << 32) | __float_as_uint(fv);
     mins.far = key < mins.far ? key : mins.far;
-    if (!(atomicOr(&a.rec[v].meta, kFarListed) & kFarListed)) far[wave_slot(&sh.n_far)] = v;
+    if (!far_listed(old)) far[wave_slot(&sh.n_far)] = v;
   }
 }
 
@@ -175,18 +182,18 @@ __device__ __forceinline__ void relax_part(const TraceArgs &a, RoadMins &mins, i
     const float cc = vw_step(du, c[i]);
     c[i] = cc;
     const unsigned cb = __float_as_uint(cc);
-    const bool near = cc != du && cc <= bound;
+    const bool rail = cc == du, near = !rail && cc <= bound;
     old[i] = 0;  // "nothing changed" unless the atomic says otherwise
     if (cb < seen[i] && cc < kInf)
-      old[i] = atomicMin(&a.rec[u + nb_offset<k>(a.d)].word, ((unsigned long long)cb << 32) | (near ? 0ULL : 1ULL));
+      old[i] = atomicMin(&a.rec[u + nb_offset<k>(a.d)].word,
+                         ((unsigned long long)cb << 32) | (near ? kRoadNear : rail ? kRoadRail : kRoadFar));
   });
-  unsigned farc = 0;
   for_nb<B, E>([&](auto K) {
     constexpr int k = decltype(K)::value, i = k - B;
     const float cc = c[i];
     const unsigned cb = __float_as_uint(cc);
     const bool rail = cc == du, near = !rail && cc <= bound;
-    const unsigned long long nv = ((unsigned long long)cb << 32) | (near ? 0ULL : 1ULL);
+    const unsigned long long nv = ((unsigned long long)cb << 32) | (near ? kRoadNear : rail ? kRoadRail : kRoadFar);
     if (!(old[i] > nv)) return;
     if ((unsigned)(old[i] >> 32) == 0xFFFFFFFFu) p.touched |= 1u << k;
     if (rail) {
@@ -197,20 +204,9 @@ __device__ __forceinline__ void relax_part(const TraceArgs &a, RoadMins &mins, i
     } else {
       const unsigned long long key = ((unsigned long long)cb << 32) | __float_as_uint(cc - du);
       mins.far = key < mins.far ? key : mins.far;
-      farc |= 1u << k;
+      if (!far_listed(old[i])) p.far |= 1u << k;
     }
   });
-  if (farc) {
-    unsigned was[E - B];
-    for_nb<B, E>([&](auto K) {
-      constexpr int k = decltype(K)::value, i = k - B;
-      was[i] = (farc >> k) & 1u ? atomicOr(&a.rec[u + nb_offset<k>(a.d)].meta, kFarListed) : kFarListed;
-    });
-    for_nb<B, E>([&](auto K) {
-      constexpr int k = decltype(K)::value, i = k - B;
-      if (!(was[i] & kFarListed)) p.far |= 1u << k;
-    });
-  }
 }
 
 __device__ __forceinline__ void relax_all(const TraceArgs &a, RoadMins &mins, int u, unsigned m, float du, float bound,
@@ -432,14 +428,20 @@ __device__ __forceinline__ int railroad(const TraceArgs &a, Shared &sh, const Te
     if (sh.done) break;
     const int nf = sh.n_far;
     const float bound = rail_bound(sh);
-    for (int i = threadIdx.x; i < nf; i += blockDim.x) {  // promote far voxels inside the new band
+    // Promote far voxels inside the new band. Entries whose far bit is clear are stale (the voxel
+    // went near); of several entries of one voxel, the one whose atomic clears the far bit acts.
+    // A far-listed word is never queued, so the claim also queues it; a voxel still beyond the
+    // band gets both bits back.
+    for (int i = threadIdx.x; i < nf; i += blockDim.x) {
       const int w = lp_i(fcur + i);
-      if (!(rec_meta(a.rec, w) & kFarListed)) continue;
-      const float dw = __uint_as_float((unsigned)(lp_u64(&a.rec[w].word) >> 32));
+      if (!far_listed(lp_u64(&a.rec[w].word))) continue;
+      const unsigned long long was = atomicAnd(&a.rec[w].word, ~3ULL);
+      if (!far_listed(was)) continue;
+      const float dw = __uint_as_float((unsigned)(was >> 32));
       if (dw <= bound) {
-        atomicAnd(&a.rec[w].meta, ~kFarListed);
-        if (atomicAnd(&a.rec[w].word, ~1ULL) & 1ULL) cur[wave_slot(&sh.n_cur)] = w;
+        cur[wave_slot(&sh.n_cur)] = w;
       } else {
+        atomicOr(&a.rec[w].word, kRoadFar);
         const unsigned long long key = ((unsigned long long)__float_as_uint(dw) << 32) | __float_as_uint(rec_pdrf(a.rec, w));
         mins.far = key < mins.far ? key : mins.far;
         fnxt[wave_slot(&sh.n_farnxt)] = w;
@@ -885,9 +887,7 @@ __global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restri
       lap(sh, 3);
       const int nt = sh.n_touched;
       for (int j = threadIdx.x; j < nt; j += blockDim.x) {  // restore the railroad state
-        int v = l4[j];
-        sp_u64(&a.rec[v].word, kMaxKey);
-        atomicAnd(&a.rec[v].meta, ~kFarListed);
+        sp_u64(&a.rec[l4[j]].word, kMaxKey);
       }
       if (status != 1 || sh.path_len < 0) {
         if (threadIdx.x == 0) a.ejected[atomicAdd(a.counters + 2, 1) % a.ejected_cap] = L;
