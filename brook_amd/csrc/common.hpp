@@ -175,17 +175,17 @@ __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
 // Everything the floods and the tracer touch per voxel, in one 16-byte record (one cache line holds
 // eight voxels of a row): a neighbour costs one line instead of one per array.
 //   pdrf  the penalty field (rails 0)
-//   meta  bits [0,26) the same-label neighbour mask, then kAlive, kQueued / kWaiting (invalidation
-//         queue, waiting for a later band)
+//   meta  bits [0,26) the same-label neighbour mask, then kAlive and kWaiting (invalidation: waiting
+//         for a later band)
 //   word  a distance word (see word_dist_bits; the railroad also keeps bit 1 = "far listed") or an
-//         invalidation claim; ~0 between uses
+//         invalidation claim (bottleneck, source, queued bit); ~0 between uses
 struct alignas(16) VoxelRec {
   float pdrf;
   unsigned meta;
   unsigned long long word;
 };
 constexpr unsigned kNbrBits = (1u << 26) - 1u;
-constexpr unsigned kAlive = 1u << 26, kQueued = 1u << 28, kWaiting = 1u << 29;
+constexpr unsigned kAlive = 1u << 26, kWaiting = 1u << 29;
 
 __device__ __forceinline__ float rec_pdrf(const VoxelRec *r, int v) { return lp_f(&r[v].pdrf); }
 __device__ __forceinline__ unsigned rec_meta(const VoxelRec *r, int v) { return lp_u(&r[v].meta); }

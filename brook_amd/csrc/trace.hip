@@ -500,6 +500,17 @@ __device__ __forceinline__ void backtrace(const TraceArgs &a, Shared &sh, int t,
 }
 
 // ---- invalidation ---------------------------------------------------------------------------
+// A claim word is (bottleneck bits << 32) | (source << 1) | queued: claims order by (bottleneck,
+// source), and the improving atomicMin returns whether the voxel is already queued for the next
+// round (an equal claim never re-queues). A round's snapshot clears the queued bit with the atomic
+// that reads the claim. ~0 is unclaimed (and not queued).
+__device__ __forceinline__ unsigned long long claim_word(unsigned mb, unsigned su) {
+  return ((unsigned long long)mb << 32) | ((unsigned long long)su << 1) | 1ULL;
+}
+__device__ __forceinline__ unsigned claim_source(unsigned long long w) { return (unsigned)w >> 1; }
+// An improving claim whose old word says the voxel still has to be queued.
+__device__ __forceinline__ bool needs_queue(unsigned long long old) { return old == kMaxKey || !(old & 1ULL); }
+
 __device__ __forceinline__ float offer_distance(const TraceArgs &a, int x, int y, int z, int qx, int qy, int qz) {
   float dx = a.wx * (float)(x - qx), dy = a.wy * (float)(y - qy), dz = a.wz * (float)(z - qz);
   return sqrtf(dx * dx + dy * dy + dz * dz);
@@ -511,17 +522,15 @@ __device__ __forceinline__ void offer_to(const TraceArgs &a, Shared &sh, int v, 
                                          int *touched) {
   const unsigned mb = max(__float_as_uint(d), mu);  // M = max(M[u], d)
   if (mb < hi) {
-    const unsigned long long cand = ((unsigned long long)mb << 32) | su;
+    const unsigned long long cand = claim_word(mb, su);
     const unsigned long long old = issued ? old_or_none : (lp_u64(&a.rec[v].word) <= cand ? cand : atomicMin(&a.rec[v].word, cand));
     if (old > cand) {
       if (old == kMaxKey) touched[wave_slot(&sh.n_touched)] = v;
-      const unsigned was = atomicOr(&a.rec[v].meta, kQueued);
-      if (was & kWaiting) atomicAnd(&a.rec[v].meta, ~kWaiting);
-      if (!(was & kQueued)) nxt[wave_slot(&sh.n_nxt)] = v;
+      if (needs_queue(old)) nxt[wave_slot(&sh.n_nxt)] = v;
     }
   } else if (lp_u64(&a.rec[v].word) == kMaxKey) {  // beyond the band: wait for its band
     unsigned seen = rec_meta(a.rec, v);
-    while (!(seen & (kQueued | kWaiting))) {
+    while (!(seen & kWaiting)) {
       unsigned got = atomicCAS(&a.rec[v].meta, seen, seen | kWaiting);
       if (got == seen) { far[wave_slot(&sh.n_far)] = v; break; }
       seen = got;
@@ -532,7 +541,8 @@ __device__ __forceinline__ void offer_to(const TraceArgs &a, Shared &sh, int v, 
 // Masked neighbours B..E-1 of u at once: alive and key loads, then claim atomics, in flight
 // together. Only the key's high word (the bottleneck) is pre-loaded: a claim whose bottleneck is
 // above it cannot win and issues no atomic. The outcome is a set of bit masks; the rarer waits
-// beyond the band run as a rolled loop.
+// beyond the band run as a rolled loop. (A waiting voxel that is claimed keeps its waiting flag:
+// the band boundary skips claimed entries, and the flag is cleared when the claimed set dies.)
 template <int B, int E>
 __device__ __forceinline__ void offer_part(const TraceArgs &a, const int *off, int u, unsigned m, int x, int y, int z,
                                            int qx, int qy, int qz, unsigned su, unsigned mu, float maxd, unsigned hi,
@@ -556,43 +566,23 @@ __device__ __forceinline__ void offer_part(const TraceArgs &a, const int *off, i
     old[i] = 0;  // "no improvement" unless the atomic says otherwise
     if (al[i] && d < maxd) {
       if (mb >= hi) wait |= 1u << k;
-      else if (kh[i] >= mb) old[i] = atomicMin(&a.rec[u + nb_offset<k>(a.d)].word, ((unsigned long long)mb << 32) | su);
+      else if (kh[i] >= mb) old[i] = atomicMin(&a.rec[u + nb_offset<k>(a.d)].word, claim_word(mb, su));
     }
   });
-  unsigned won = 0;
   for_nb<B, E>([&](auto K) {
     constexpr int k = decltype(K)::value, i = k - B;
-    if (old[i] > (((unsigned long long)mbs[i] << 32) | su)) {
-      won |= 1u << k;
+    if (old[i] > claim_word(mbs[i], su)) {  // won (old is 0 where no atomic was issued)
       if (old[i] == kMaxKey) p.touched |= 1u << k;
+      if (needs_queue(old[i])) p.near |= 1u << k;
     }
   });
-  if (won) {
-    unsigned was[E - B];
-    for_nb<B, E>([&](auto K) {
-      constexpr int k = decltype(K)::value, i = k - B;
-      was[i] = (won >> k) & 1u ? atomicOr(&a.rec[u + nb_offset<k>(a.d)].meta, kQueued) : kQueued;
-    });
-    unsigned clear = 0;
-    for_nb<B, E>([&](auto K) {
-      constexpr int k = decltype(K)::value, i = k - B;
-      if (!((won >> k) & 1u)) return;
-      if (was[i] & kWaiting) clear |= 1u << k;
-      if (!(was[i] & kQueued)) p.near |= 1u << k;
-    });
-    while (clear) {
-      const int k = __ffs(clear) - 1;
-      clear &= clear - 1;
-      atomicAnd(&a.rec[u + off[k]].meta, ~kWaiting);
-    }
-  }
   while (wait) {  // beyond the band: wait for its band
     const int k = __ffs(wait) - 1;
     wait &= wait - 1;
     const int v = u + off[k];
     if (lp_u64(&a.rec[v].word) != kMaxKey) continue;
     unsigned seen = rec_meta(a.rec, v);
-    while (!(seen & (kQueued | kWaiting))) {
+    while (!(seen & kWaiting)) {
       const unsigned got = atomicCAS(&a.rec[v].meta, seen, seen | kWaiting);
       if (got == seen) { p.far |= 1u << k; break; }
       seen = got;
@@ -624,12 +614,12 @@ __device__ __forceinline__ void pull_part(const TraceArgs &a, int c, unsigned mc
   float r[E - B];
   for_nb<B, E>([&](auto K) {
     constexpr int k = decltype(K)::value, i = k - B;
-    r[i] = ku[i] != kMaxKey ? a.dbf[(unsigned)ku[i]] : 0.0f;
+    r[i] = ku[i] != kMaxKey ? a.dbf[claim_source(ku[i])] : 0.0f;
   });
   for_nb<B, E>([&](auto K) {
     constexpr int k = decltype(K)::value, i = k - B;
     if (ku[i] == kMaxKey) return;
-    const unsigned su = (unsigned)ku[i];
+    const unsigned su = claim_source(ku[i]);
     int qx, qy, qz;
     unravel(int(su), a.d, qx, qy, qz);
     const float d = offer_distance(a, x, y, z, qx, qy, qz);
@@ -656,8 +646,7 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const
   for (int j = threadIdx.x; j < len; j += blockDim.x) {  // sources: path voxels still alive
     int p = path[j];
     if (!(rec_meta(a.rec, p) & kAlive)) continue;
-    if (atomicCAS(&a.rec[p].word, kMaxKey, (unsigned long long)(unsigned)p) == kMaxKey) {
-      atomicOr(&a.rec[p].meta, kQueued);
+    if (atomicCAS(&a.rec[p].word, kMaxKey, claim_word(0u, unsigned(p))) == kMaxKey) {
       cur[wave_slot(&sh.n_cur)] = p;
       touched[wave_slot(&sh.n_touched)] = p;
     }
@@ -668,11 +657,8 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const
     for (;;) {  // synchronous rounds inside the band
       const int nc = sh.n_cur;
       if (nc == 0) break;
-      for (int i = threadIdx.x; i < nc; i += blockDim.x) {  // snapshot, re-arm
-        int u = lp_i(cur + i);
-        snap[i] = lp_u64(&a.rec[u].word);
-        atomicAnd(&a.rec[u].meta, ~kQueued);
-      }
+      for (int i = threadIdx.x; i < nc; i += blockDim.x)  // snapshot and re-arm in one atomic
+        snap[i] = atomicAnd(&a.rec[lp_i(cur + i)].word, ~1ULL);
       __syncthreads();
       if (nc <= kLaneFront * 2) {  // small front: one lane per neighbour
         for (int j = threadIdx.x; j < nc * 32; j += blockDim.x) {
@@ -683,7 +669,7 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const
           const int v = u + sh.off[k];
           if (!(rec_meta(a.rec, v) & kAlive)) continue;
           const unsigned long long ku = snap[i];
-          const unsigned su = (unsigned)ku, mu = (unsigned)(ku >> 32);
+          const unsigned su = claim_source(ku), mu = (unsigned)(ku >> 32);
           int x, y, z, qx, qy, qz;
           unravel(v, a.d, x, y, z);
           unravel(int(su), a.d, qx, qy, qz);
@@ -700,7 +686,7 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const
             u = lp_i(cur + i);
             const unsigned m = (rec_meta(a.rec, u) & kNbrBits);
             const unsigned long long ku = snap[i];
-            const unsigned su = (unsigned)ku, mu = (unsigned)(ku >> 32);
+            const unsigned su = claim_source(ku), mu = (unsigned)(ku >> 32);
             offer_all(a, sh.off, u, m, su, mu, a.scale * a.dbf[su] + a.constant, hi, p);
           }
           emit_neighbours(touched, &sh.n_touched, p.touched, u, sh.off);
@@ -738,10 +724,9 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const
       if (p == kClaimed) continue;
       if (p == kMaxKey) { atomicAnd(&a.rec[c].meta, ~kWaiting); continue; }
       if ((unsigned)(p >> 32) < hi) {
-        sp_u64(&a.rec[c].word, p);
+        sp_u64(&a.rec[c].word, claim_word((unsigned)(p >> 32), (unsigned)p));
         touched[wave_slot(&sh.n_touched)] = c;
         cur[wave_slot(&sh.n_cur)] = c;
-        atomicOr(&a.rec[c].meta, kQueued);
         atomicAnd(&a.rec[c].meta, ~kWaiting);
       } else {
         keep[wave_slot(&sh.n_keep)] = c;
@@ -757,7 +742,7 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const
   for (int j = threadIdx.x; j < nt; j += blockDim.x) {  // the claimed set dies
     int v = touched[j];
     sp_u64(&a.rec[v].word, kMaxKey);
-    atomicAnd(&a.rec[v].meta, ~(kAlive | kQueued | kWaiting));
+    atomicAnd(&a.rec[v].meta, ~(kAlive | kWaiting));
   }
   __syncthreads();
 }
