@@ -21,6 +21,25 @@ __constant__ static const int kOpp[26] = {1, 0, 3, 2, 5, 4, 9, 8, 7, 6, 13, 12, 
 constexpr int kDX[26] = {-1, 1, 0, 0, 0, 0, -1, -1, 1, 1, 0, 0, 0, 0, -1, -1, 1, 1, -1, -1, -1, -1, 1, 1, 1, 1};
 constexpr int kDY[26] = {0, 0, -1, 1, 0, 0, -1, 1, -1, 1, -1, -1, 1, 1, 0, 0, 0, 0, -1, -1, 1, 1, -1, -1, 1, 1};
 constexpr int kDZ[26] = {0, 0, 0, 0, -1, 1, 0, 0, 0, 0, -1, 1, -1, 1, -1, 1, -1, 1, -1, 1, -1, 1, -1, 1, -1, 1};
+// The same deltas packed two bits per neighbour (delta + 1), for loops whose k is a run-time value:
+// shifts of an immediate instead of table loads.
+constexpr unsigned long long pack_deltas(const int (&t)[26]) {
+  unsigned long long p = 0;
+  for (int k = 0; k < 26; ++k) p |= (unsigned long long)(t[k] + 1) << (2 * k);
+  return p;
+}
+constexpr unsigned long long kPackDX = pack_deltas(kDX), kPackDY = pack_deltas(kDY), kPackDZ = pack_deltas(kDZ);
+constexpr int unpack_delta(unsigned long long p, int k) { return int((p >> (2 * k)) & 3ULL) - 1; }
+constexpr bool deltas_match() {
+  for (int k = 0; k < 26; ++k)
+    if (unpack_delta(kPackDX, k) != kDX[k] || unpack_delta(kPackDY, k) != kDY[k] || unpack_delta(kPackDZ, k) != kDZ[k])
+      return false;
+  return true;
+}
+static_assert(deltas_match(), "packed neighbour deltas");
+__device__ __forceinline__ int nb_dx(int k) { return unpack_delta(kPackDX, k); }
+__device__ __forceinline__ int nb_dy(int k) { return unpack_delta(kPackDY, k); }
+__device__ __forceinline__ int nb_dz(int k) { return unpack_delta(kPackDZ, k); }
 
 constexpr float kInf = __builtin_huge_valf();
 constexpr unsigned long long kMaxKey = ~0ULL;
@@ -71,24 +90,65 @@ template <int Batch, class F> __device__ __forceinline__ void for_parts(unsigned
   }
 }
 
+// ---- device-memory access in the global address space -------------------------------------------
+// Pointers that reach a kernel inside a struct are generic, and generic accesses compile to FLAT
+// instructions, which also count against the LDS counter (so every LDS wait waits for them too)
+// and cannot use a scalar base. The kernels access device memory through the forms below, which
+// name the global address space: plain loads and stores, and relaxed agent-scope atomics (the
+// semantics of HIP's atomicX functions).
+template <class T> __device__ __forceinline__ __attribute__((address_space(1))) T *gp(T *p) {
+  return (__attribute__((address_space(1))) T *)p;
+}
+template <class T> struct same_type { using type = T; };
+template <class T> using same_t = typename same_type<T>::type;
+template <class T> __device__ __forceinline__ T g_ld(const T *p) { return *gp(p); }
+template <class T> __device__ __forceinline__ void g_st(T *p, same_t<T> v) { *gp(p) = v; }
+template <class T> __device__ __forceinline__ T g_min(T *p, same_t<T> v) {
+  return __hip_atomic_fetch_min(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T> __device__ __forceinline__ T g_max(T *p, same_t<T> v) {
+  return __hip_atomic_fetch_max(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T> __device__ __forceinline__ T g_add(T *p, same_t<T> v) {
+  return __hip_atomic_fetch_add(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T> __device__ __forceinline__ T g_sub(T *p, same_t<T> v) {
+  return __hip_atomic_fetch_sub(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T> __device__ __forceinline__ T g_or(T *p, same_t<T> v) {
+  return __hip_atomic_fetch_or(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T> __device__ __forceinline__ T g_and(T *p, same_t<T> v) {
+  return __hip_atomic_fetch_and(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T> __device__ __forceinline__ T g_xchg(T *p, same_t<T> v) {
+  return __hip_atomic_exchange(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Compare-and-swap; returns the previous value.
+template <class T> __device__ __forceinline__ T g_cas(T *p, same_t<T> expected, same_t<T> desired) {
+  __hip_atomic_compare_exchange_strong(gp(p), &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+  return expected;
+}
+
 // Loads of memory that other threads change inside the same kernel: relaxed, agent scope, so the
 // access is served by L2 (the CU's vector L1 may hold a stale line otherwise).
-__device__ __forceinline__ float ld_f(const float *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-__device__ __forceinline__ int ld_i(const int *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-__device__ __forceinline__ unsigned ld_u(const unsigned *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ float ld_f(const float *p) { return __hip_atomic_load(gp(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ int ld_i(const int *p) { return __hip_atomic_load(gp(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ unsigned ld_u(const unsigned *p) { return __hip_atomic_load(gp(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 __device__ __forceinline__ unsigned long long ld_u64(const unsigned long long *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(gp(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ unsigned char ld_u8(const unsigned char *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(gp(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void st_f(float *p, float v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-__device__ __forceinline__ void st_i(int *p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ void st_f(float *p, float v) { __hip_atomic_store(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ void st_i(int *p, int v) { __hip_atomic_store(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 __device__ __forceinline__ void st_u8(unsigned char *p, unsigned char v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void st_u64(unsigned long long *p, unsigned long long v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Loads and stores of per-voxel data that, during one kernel, only the workgroup owning the voxel's
@@ -97,15 +157,15 @@ __device__ __forceinline__ void st_u64(unsigned long long *p, unsigned long long
 #ifndef BROOK_LABEL_SCOPE
 #define BROOK_LABEL_SCOPE __HIP_MEMORY_SCOPE_WORKGROUP
 #endif
-__device__ __forceinline__ float lp_f(const float *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, BROOK_LABEL_SCOPE); }
-__device__ __forceinline__ int lp_i(const int *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, BROOK_LABEL_SCOPE); }
-__device__ __forceinline__ unsigned lp_u(const unsigned *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, BROOK_LABEL_SCOPE); }
+__device__ __forceinline__ float lp_f(const float *p) { return __hip_atomic_load(gp(p), __ATOMIC_RELAXED, BROOK_LABEL_SCOPE); }
+__device__ __forceinline__ int lp_i(const int *p) { return __hip_atomic_load(gp(p), __ATOMIC_RELAXED, BROOK_LABEL_SCOPE); }
+__device__ __forceinline__ unsigned lp_u(const unsigned *p) { return __hip_atomic_load(gp(p), __ATOMIC_RELAXED, BROOK_LABEL_SCOPE); }
 __device__ __forceinline__ unsigned long long lp_u64(const unsigned long long *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, BROOK_LABEL_SCOPE);
+  return __hip_atomic_load(gp(p), __ATOMIC_RELAXED, BROOK_LABEL_SCOPE);
 }
-__device__ __forceinline__ void sp_f(float *p, float v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, BROOK_LABEL_SCOPE); }
+__device__ __forceinline__ void sp_f(float *p, float v) { __hip_atomic_store(gp(p), v, __ATOMIC_RELAXED, BROOK_LABEL_SCOPE); }
 __device__ __forceinline__ void sp_u64(unsigned long long *p, unsigned long long v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, BROOK_LABEL_SCOPE);
+  __hip_atomic_store(gp(p), v, __ATOMIC_RELAXED, BROOK_LABEL_SCOPE);
 }
 
 // One vertex-weighted step fl(d + f) that never returns d itself for a positive cost: one ulp up
@@ -122,7 +182,7 @@ __device__ __forceinline__ float vw_step(float d, float f) {
 __device__ __forceinline__ float atomic_min_pos(float *addr, float val) {
   float cur = ld_f(addr);
   if (cur <= val) return cur;
-  return __int_as_float(atomicMin(reinterpret_cast<int *>(addr), __float_as_int(val)));
+  return __int_as_float(g_min(reinterpret_cast<int *>(addr), __float_as_int(val)));
 }
 
 // Slot in a list whose length lives at *counter (any address space), for every lane that calls
@@ -159,7 +219,7 @@ __device__ __forceinline__ void emit_neighbours(int *list, int *counter, unsigne
   while (mask) {
     const int k = __ffs(mask) - 1;
     mask &= mask - 1;
-    list[pos++] = u + off[k];
+    g_st(list + pos++, u + off[k]);
   }
 }
 

@@ -111,7 +111,7 @@ __device__ __forceinline__ unsigned flood_part(VoxelRec *rec, Dims d, int u, uns
     constexpr int k = decltype(K)::value, i = k - B;
     const unsigned cb = __float_as_uint(du + W[k]);
     old[i] = 0;
-    if (cb < seen[i]) old[i] = atomicMin(&rec[u + nb_offset<k>(d)].word, (unsigned long long)cb << 32);
+    if (cb < seen[i]) old[i] = g_min(&rec[u + nb_offset<k>(d)].word, (unsigned long long)cb << 32);
   });
   unsigned push = 0;
   for_nb<B, E>([&](auto K) {
@@ -131,7 +131,7 @@ __device__ __forceinline__ void flood_label(VoxelRec *rec, Dims d, int src, cons
                             int *counts /*LDS [2]*/) {
   if (threadIdx.x == 0) {
     sp_u64(&rec[src].word, 0ULL);  // distance 0, queued
-    cur[0] = src;
+    g_st(cur, src);
     counts[0] = 1;
     counts[1] = 0;
   }
@@ -146,7 +146,7 @@ __device__ __forceinline__ void flood_label(VoxelRec *rec, Dims d, int src, cons
       if (i < nc) {
         u = lp_i(cur + i);
         const unsigned m = rec_meta(rec, u) & kNbrBits;
-        const float du = __uint_as_float((unsigned)(atomicOr(&rec[u].word, 1ULL) >> 32));
+        const float du = __uint_as_float((unsigned)(g_or(&rec[u].word, 1ULL) >> 32));
         for_parts<kFloodBatch>(m, [&](auto B, auto E) {
           push |= flood_part<decltype(B)::value, decltype(E)::value>(rec, d, u, m, du, W);
         });
@@ -167,7 +167,7 @@ __device__ __forceinline__ unsigned long long label_argmax(const VoxelRec *rec, 
   __syncthreads();
   unsigned long long best = 0;
   for (int j = threadIdx.x; j < count; j += blockDim.x) {
-    int v = list[j];
+    int v = g_ld(list + j);
     unsigned fb = word_dist_bits(lp_u64(&rec[v].word));
     if (!(fb < 0x7f800000u)) continue;
     unsigned long long key = ((unsigned long long)fb << 32) | (unsigned long long)(0xFFFFFFFFu - (unsigned)v);
@@ -202,31 +202,31 @@ __global__ void __launch_bounds__(kFloodBlock) prepare_labels(const FloodJob *__
     const Dims d = job.d;
     const LabelTables &t = job.t;
     if (threadIdx.x < 26) {
-      W[threadIdx.x] = t.weights[threadIdx.x];
+      W[threadIdx.x] = g_ld(t.weights + threadIdx.x);
       off[threadIdx.x] = nb_offset(int(threadIdx.x), d);
     }
     __syncthreads();
-    const int s0 = t.seg_start[L], cnt = t.seg_count[L];
+    const int s0 = g_ld(t.seg_start + L), cnt = g_ld(t.seg_count + L);
     const int *list = t.vox + s0;
     int *a = job.lists_a + s0, *b = job.lists_b + s0;
-    long long root = t.root[L];
+    long long root = g_ld(t.root + L);
     if (root < 0) {  // no border target: flood from the component's first voxel, root = argmax
-      flood_label(t.rec, d, int(t.first[L]), W, off, a, b, counts);
+      flood_label(t.rec, d, int(g_ld(t.first + L)), W, off, a, b, counts);
       unsigned long long key = label_argmax(t.rec, list, cnt, &best);
       root = key ? (long long)(0xFFFFFFFFu - (unsigned)key) : -1;
-      for (int j = threadIdx.x; j < cnt; j += blockDim.x) sp_u64(&t.rec[list[j]].word, kMaxKey);
+      for (int j = threadIdx.x; j < cnt; j += blockDim.x) sp_u64(&t.rec[g_ld(list + j)].word, kMaxKey);
       __syncthreads();
-      if (threadIdx.x == 0) t.root[L] = root;
+      if (threadIdx.x == 0) g_st(t.root + L, root);
     }
     flood_label(t.rec, d, int(root), W, off, a, b, counts);
     unsigned long long key = label_argmax(t.rec, list, cnt, &best);
     if (threadIdx.x == 0) {
-      t.target[L] = key ? (long long)(0xFFFFFFFFu - (unsigned)key) : -1;
-      t.max_daf[L] = key ? __uint_as_float((unsigned)(key >> 32)) : 0.0f;
+      g_st(t.target + L, key ? (long long)(0xFFFFFFFFu - (unsigned)key) : -1LL);
+      g_st(t.max_daf + L, key ? __uint_as_float((unsigned)(key >> 32)) : 0.0f);
     }
     for (int j = threadIdx.x; j < cnt; j += blockDim.x) {  // DAF out, words back to ~0
-      const int v = list[j];
-      job.daf[v] = __uint_as_float(word_dist_bits(lp_u64(&t.rec[v].word)));
+      const int v = g_ld(list + j);
+      g_st(job.daf + v, __uint_as_float(word_dist_bits(lp_u64(&t.rec[v].word))));
       sp_u64(&t.rec[v].word, kMaxKey);
     }
   }

@@ -32,7 +32,6 @@ namespace {
 constexpr int kBlock = BROOK_TRACE_BLOCK;  // threads per label workgroup
 constexpr unsigned long long kMinfarNone = 0x7f8000007f800000ULL;
 constexpr unsigned long long kClaimed = 0xFFFFFFFFFFFFFFFEULL;
-constexpr int kLaneFront = 16;  // fronts up to this size relax with one lane per neighbour
 constexpr int kProf = kTraceProfileSlots;
 #ifndef BROOK_NB_BATCH
 #define BROOK_NB_BATCH 9
@@ -86,9 +85,6 @@ struct Shared {
   int store_pos;
   int off[26];  // flat offsets of the 26 neighbours
   int chunk, job, quit;
-  float lane_du[kLaneFront * 2];
-  int lane_u[kLaneFront * 2];
-  unsigned lane_m[kLaneFront * 2];
   // per-label profile: [0] total [1] target [2] railroad [3] backtrace [4] invalidation (wall-clock
   // ticks), [5] railroad rounds, [6] invalidation rounds, [7] paths, [8] railroad phases,
   // [9] far entries scanned, [10] near entries expanded, [11] invalidation entries expanded,
@@ -136,32 +132,6 @@ __device__ __forceinline__ void road_fold(Shared &sh, RoadMins &mins) {
   mins = RoadMins{};
 }
 
-// Relaxation of neighbour k (a set bit of u's mask) of u, which holds distance du; one lane per
-// neighbour, list slots claimed per wave.
-__device__ __forceinline__ void relax_one(const TraceArgs &a, Shared &sh, RoadMins &mins, int u, int k, float du,
-                                          float bound, int *nxt, int *far, int *touched) {
-  const int v = u + sh.off[k];
-  const float fv = rec_pdrf(a.rec, v);
-  const float cand = vw_step(du, fv);
-  const unsigned cb = __float_as_uint(cand);
-  if (rec_word_hi(a.rec, v) <= cb || !(cand < kInf)) return;
-  const bool rail = fv == 0.0f, near = !rail && cand <= bound;
-  const unsigned long long nv = ((unsigned long long)cb << 32) | (near ? kRoadNear : rail ? kRoadRail : kRoadFar);
-  const unsigned long long old = atomicMin(&a.rec[v].word, nv);
-  if (!(old > nv)) return;  // nothing changed
-  if ((unsigned)(old >> 32) == 0xFFFFFFFFu) touched[wave_slot(&sh.n_touched)] = v;
-  if (rail) {  // a rail: fixes the cost and the last road voxel; never expanded
-    const unsigned long long key = ((unsigned long long)cb << 32) | (unsigned)v;
-    mins.rail = key < mins.rail ? key : mins.rail;
-  } else if (near) {
-    if (old & 1ULL) nxt[wave_slot(&sh.n_nxt)] = v;
-  } else {
-    const unsigned long long key = ((unsigned long long)cb << 32) | __float_as_uint(fv);
-    mins.far = key < mins.far ? key : mins.far;
-    if (!far_listed(old)) far[wave_slot(&sh.n_far)] = v;
-  }
-}
-
 // Masked neighbours B..E-1 of u at once: every load, then every atomic, is issued before any result
 // is waited for; the outcome is a set of bit masks (list appends happen once per loop iteration).
 // A rail is recognised by cand == du (vw_step adds nothing only for a zero cost).
@@ -185,7 +155,7 @@ __device__ __forceinline__ void relax_part(const TraceArgs &a, RoadMins &mins, i
     const bool rail = cc == du, near = !rail && cc <= bound;
     old[i] = 0;  // "nothing changed" unless the atomic says otherwise
     if (cb < seen[i] && cc < kInf)
-      old[i] = atomicMin(&a.rec[u + nb_offset<k>(a.d)].word,
+      old[i] = g_min(&a.rec[u + nb_offset<k>(a.d)].word,
                          ((unsigned long long)cb << 32) | (near ? kRoadNear : rail ? kRoadRail : kRoadFar));
   });
   for_nb<B, E>([&](auto K) {
@@ -216,7 +186,101 @@ __device__ __forceinline__ void relax_all(const TraceArgs &a, RoadMins &mins, in
 
 // Dequeues front voxel u: marks it "not queued" and returns its distance, in one atomic.
 __device__ __forceinline__ float dequeue(const TraceArgs &a, int u) {
-  return __uint_as_float((unsigned)(atomicOr(&a.rec[u].word, 1ULL) >> 32));
+  return __uint_as_float((unsigned)(g_or(&a.rec[u].word, 1ULL) >> 32));
+}
+
+// ---- fronts of up to a quarter workgroup: a group of lanes per entry --------------------------
+// Each entry gets G lanes (the largest power of two up to 32 that fits every entry into the
+// workgroup, at least 4); lane g of a group handles neighbours [g*S, g*S + S), S = ceil(26 / G) <=
+// kSpan, so all of an entry's loads, then all of its atomics, are in flight together instead of
+// part after part. One code path serves every G (a span is kSpan slots, the tail predicated off),
+// which keeps the kernel's register pressure at the level of the one-thread-per-entry path.
+constexpr int kGroupFront = kBlock / 4, kSpan = 7;
+struct Group {
+  int G, i, g, lead, kb, cnt;  // lanes per entry, entry, lane in group, first lane, span
+};
+__device__ __forceinline__ Group group_of(int n) {
+  Group q;
+  q.G = 32;
+  while (q.G > 4 && n * q.G > kBlock) q.G >>= 1;
+  const int shift = __ffs(q.G) - 1, span = (26 + q.G - 1) / q.G;
+  q.i = threadIdx.x >> shift;
+  q.g = threadIdx.x & (q.G - 1);
+  q.lead = __lane_id() & ~(q.G - 1);
+  q.kb = q.g * span;
+  q.cnt = min(span, 26 - q.kb);  // may be <= 0: nothing to do
+  return q;
+}
+
+// Neighbours kb..kb+cnt-1 (those set in m; cnt <= kSpan) of u: relax_part with a run-time span.
+__device__ __forceinline__ void relax_span(const TraceArgs &a, RoadMins &mins, int u, unsigned m, int kb, int cnt, float du,
+                                           float bound, const int *off, Pushes &p) {
+  constexpr int S = kSpan;
+  float c[S];
+  unsigned seen[S];
+  int v[S];
+#pragma unroll
+  for (int j = 0; j < S; ++j) {
+    const int k = kb + j;
+    const bool on = j < cnt && ((m >> k) & 1u);
+    v[j] = u + off[on ? k : 0];
+    c[j] = on ? rec_pdrf(a.rec, v[j]) : kInf;
+    seen[j] = on ? rec_word_hi(a.rec, v[j]) : 0u;
+  }
+  unsigned long long old[S];
+#pragma unroll
+  for (int j = 0; j < S; ++j) {
+    const float cc = vw_step(du, c[j]);
+    c[j] = cc;
+    const unsigned cb = __float_as_uint(cc);
+    const bool rail = cc == du, near = !rail && cc <= bound;
+    old[j] = 0;  // "nothing changed" unless the atomic says otherwise
+    if (cb < seen[j] && cc < kInf)
+      old[j] = g_min(&a.rec[v[j]].word, ((unsigned long long)cb << 32) | (near ? kRoadNear : rail ? kRoadRail : kRoadFar));
+  }
+#pragma unroll
+  for (int j = 0; j < S; ++j) {
+    const int k = kb + j;
+    const float cc = c[j];
+    const unsigned cb = __float_as_uint(cc);
+    const bool rail = cc == du, near = !rail && cc <= bound;
+    const unsigned long long nv = ((unsigned long long)cb << 32) | (near ? kRoadNear : rail ? kRoadRail : kRoadFar);
+    if (!(old[j] > nv)) continue;
+    if ((unsigned)(old[j] >> 32) == 0xFFFFFFFFu) p.touched |= 1u << k;
+    if (rail) {
+      const unsigned long long key = ((unsigned long long)cb << 32) | (unsigned)v[j];
+      mins.rail = key < mins.rail ? key : mins.rail;
+    } else if (near) {
+      if (old[j] & 1ULL) p.near |= 1u << k;
+    } else {
+      const unsigned long long key = ((unsigned long long)cb << 32) | __float_as_uint(cc - du);
+      mins.far = key < mins.far ? key : mins.far;
+      if (!far_listed(old[j])) p.far |= 1u << k;
+    }
+  }
+}
+
+// One railroad round over a front of at most kGroupFront entries, a group of lanes per entry: the
+// group's first lane dequeues the entry and shares it.
+__device__ __forceinline__ void relax_groups(const TraceArgs &a, Shared &sh, RoadMins &mins, const int *cur, int nc,
+                                             float bound, int *nxt, int *far, int *touched) {
+  const Group q = group_of(nc);
+  int u = 0;
+  unsigned m = 0;
+  float du = kInf;
+  if (q.g == 0 && q.i < nc) {
+    u = lp_i(cur + q.i);
+    m = rec_meta(a.rec, u) & kNbrBits;
+    du = dequeue(a, u);
+  }
+  u = __shfl(u, q.lead, 64);
+  m = __shfl(m, q.lead, 64);
+  du = __shfl(du, q.lead, 64);
+  Pushes p;
+  if (q.i < nc && du <= bound) relax_span(a, mins, u, m, q.kb, q.cnt, du, bound, sh.off, p);
+  emit_neighbours(touched, &sh.n_touched, p.touched, u, sh.off);
+  emit_neighbours(nxt, &sh.n_nxt, p.near, u, sh.off);
+  emit_neighbours(far, &sh.n_far, p.far, u, sh.off);
 }
 
 // ---- help with large railroad rounds ---------------------------------------------------------
@@ -257,7 +321,7 @@ __device__ __forceinline__ void help_chunks(const TraceArgs &a, Shared &sh, Help
   const int *cur = job->cur;
   int *nxt = job->nxt, *far = job->far, *touched = job->touched;
   for (;;) {
-    if (threadIdx.x == 0) sh.chunk = atomicAdd(&job->next_chunk, 1);
+    if (threadIdx.x == 0) sh.chunk = g_add(&job->next_chunk, 1);
     __syncthreads();
     const int c = sh.chunk;
     __syncthreads();
@@ -267,19 +331,19 @@ __device__ __forceinline__ void help_chunks(const TraceArgs &a, Shared &sh, Help
                 &job->n_far, &job->n_touched);
     const unsigned long long r = wave_min_u64(mins.rail), f = wave_min_u64(mins.far);
     if (__lane_id() == 0) {
-      if (r != kMaxKey) atomicMin(&job->rail, r);
-      if (f != kMinfarNone) atomicMin(&job->minfar, f);
+      if (r != kMaxKey) g_min(&job->rail, r);
+      if (f != kMinfarNone) g_min(&job->minfar, f);
     }
     __threadfence();  // this thread's list entries and record updates are visible device-wide
     __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(&job->done_chunks, 1);
+    if (threadIdx.x == 0) g_add(&job->done_chunks, 1);
   }
 }
 
 // Spins until *p >= want (or the cap, which records an error); returns false on the cap.
 __device__ __forceinline__ bool spin_until_at_least(const int *p, int want, int *error) {
   for (long long i = 0; ld_i(p) < want; ++i) {
-    if (i > kSpinCap) { atomicExch(error, 1); return false; }
+    if (i > kSpinCap) { g_xchg(error, 1); return false; }
     __builtin_amdgcn_s_sleep(2);
   }
   return true;
@@ -287,7 +351,7 @@ __device__ __forceinline__ bool spin_until_at_least(const int *p, int want, int 
 // Spins until *p <= 0 (or the cap, which records an error).
 __device__ __forceinline__ bool spin_until_none(const int *p, int *error) {
   for (long long i = 0; ld_i(p) > 0; ++i) {
-    if (i > kSpinCap) { atomicExch(error, 1); return false; }
+    if (i > kSpinCap) { g_xchg(error, 1); return false; }
     __builtin_amdgcn_s_sleep(2);
   }
   return true;
@@ -302,7 +366,7 @@ __device__ bool team_round(const TraceArgs &a, Shared &sh, const Team &team, con
     sh.job = -1;
     if (ld_i(team.next) >= team.nqueue)  // the label queue is empty: idle workgroups exist
       for (int s = 0; s < kHelpBoard; ++s)
-        if (atomicCAS(&team.board[s].state, 0, 3) == 0) { sh.job = s; break; }
+        if (g_cas(&team.board[s].state, 0, 3) == 0) { sh.job = s; break; }
   }
   __syncthreads();
   if (sh.job < 0) return false;
@@ -327,13 +391,13 @@ __device__ bool team_round(const TraceArgs &a, Shared &sh, const Team &team, con
     job->minfar = kMinfarNone;
     job->epoch += 1;
     __threadfence();
-    atomicExch(&job->state, 1);  // open
+    g_xchg(&job->state, 1);  // open
   }
   __syncthreads();
   help_chunks(a, sh, job);
   if (threadIdx.x == 0) {
     spin_until_at_least(&job->done_chunks, ld_i(&job->nchunks), team.error);
-    atomicExch(&job->state, 2);  // closing: no new joins
+    g_xchg(&job->state, 2);  // closing: no new joins
     spin_until_none(&job->helpers, team.error);
     __threadfence();
     sh.n_nxt = ld_i(&job->n_nxt);
@@ -342,7 +406,7 @@ __device__ bool team_round(const TraceArgs &a, Shared &sh, const Team &team, con
     const unsigned long long r = ld_u64(&job->rail), f = ld_u64(&job->minfar);
     if (r < sh.rail_best) sh.rail_best = r;
     if (f < sh.minfar) sh.minfar = f;
-    atomicExch(&job->state, 0);  // free
+    g_xchg(&job->state, 0);  // free
   }
   __threadfence();  // every wave: the helpers' list entries and record updates
   __syncthreads();
@@ -360,7 +424,7 @@ __device__ __forceinline__ float rail_bound(const Shared &sh) {
 __device__ __forceinline__ int railroad(const TraceArgs &a, Shared &sh, const Team &team, int t, int *near_a,
                                         int *near_b, int *far_a, int *far_b, int *touched) {
   if (threadIdx.x == 0) {
-    touched[0] = t;
+    g_st(touched, t);
     sh.n_touched = 1;
     sh.n_cur = sh.n_nxt = sh.n_far = sh.n_farnxt = 0;
     sh.minfar = kMinfarNone;
@@ -373,7 +437,7 @@ __device__ __forceinline__ int railroad(const TraceArgs &a, Shared &sh, const Te
       sp_u64(&a.rec[t].word, 0ULL);  // distance 0, queued
       sh.rail_best = kMaxKey;
       sh.done = 0;
-      near_a[0] = t;
+      g_st(near_a, t);
       sh.n_cur = 1;
       sh.T = fmaxf(2.0f * ft, 1e-30f);
     }
@@ -386,21 +450,8 @@ __device__ __forceinline__ int railroad(const TraceArgs &a, Shared &sh, const Te
       const int nc = sh.n_cur;
       if (nc == 0) break;
       const float bound = rail_bound(sh);
-      if (nc <= kLaneFront) {
-        if ((int)threadIdx.x < nc) {
-          const int u = lp_i(cur + threadIdx.x);
-          const float du = dequeue(a, u);
-          sh.lane_du[threadIdx.x] = du > bound ? -1.0f : du;
-          sh.lane_u[threadIdx.x] = u;
-          sh.lane_m[threadIdx.x] = (rec_meta(a.rec, u) & kNbrBits);
-        }
-        __syncthreads();
-        for (int j = threadIdx.x; j < nc * 32; j += blockDim.x) {
-          const int i = j >> 5, k = j & 31;
-          const float du = sh.lane_du[i];
-          if (k >= 26 || du < 0.0f || !((sh.lane_m[i] >> k) & 1u)) continue;
-          relax_one(a, sh, mins, sh.lane_u[i], k, du, bound, nxt, fcur, touched);
-        }
+      if (nc <= kGroupFront) {
+        relax_groups(a, sh, mins, cur, nc, bound, nxt, fcur, touched);
       } else if (!kTraceHelp || nc < kHelpMin || !team.board || !team_round(a, sh, team, cur, nc, bound, nxt, fcur, touched)) {
         relax_front(a, sh, mins, cur, 0, nc, bound, nxt, fcur, touched, &sh.n_nxt, &sh.n_far, &sh.n_touched);
       }
@@ -435,16 +486,16 @@ __device__ __forceinline__ int railroad(const TraceArgs &a, Shared &sh, const Te
     for (int i = threadIdx.x; i < nf; i += blockDim.x) {
       const int w = lp_i(fcur + i);
       if (!far_listed(lp_u64(&a.rec[w].word))) continue;
-      const unsigned long long was = atomicAnd(&a.rec[w].word, ~3ULL);
+      const unsigned long long was = g_and(&a.rec[w].word, ~3ULL);
       if (!far_listed(was)) continue;
       const float dw = __uint_as_float((unsigned)(was >> 32));
       if (dw <= bound) {
-        cur[wave_slot(&sh.n_cur)] = w;
+        g_st(cur + wave_slot(&sh.n_cur), w);
       } else {
-        atomicOr(&a.rec[w].word, kRoadFar);
+        g_or(&a.rec[w].word, kRoadFar);
         const unsigned long long key = ((unsigned long long)__float_as_uint(dw) << 32) | __float_as_uint(rec_pdrf(a.rec, w));
         mins.far = key < mins.far ? key : mins.far;
-        fnxt[wave_slot(&sh.n_farnxt)] = w;
+        g_st(fnxt + wave_slot(&sh.n_farnxt), w);
       }
     }
     road_fold(sh, mins);
@@ -491,7 +542,7 @@ __device__ __forceinline__ void backtrace(const TraceArgs &a, Shared &sh, int t,
     }
     first = false;
     if (kept >= count) { kept = -1; break; }  // longer than the label: cannot happen on a valid field
-    if (lane == 0) walk[kept] = cur;
+    if (lane == 0) g_st(walk + kept, cur);
     ++kept;
     if (kb < 0) break;
     cur += nb_offset(kb, a.d);
@@ -516,23 +567,18 @@ __device__ __forceinline__ float offer_distance(const TraceArgs &a, int x, int y
   return sqrtf(dx * dx + dy * dy + dz * dz);
 }
 
-// Claim or wait for neighbour v, at distance d from the source su, from a voxel with bottleneck mu.
-__device__ __forceinline__ void offer_to(const TraceArgs &a, Shared &sh, int v, float d, unsigned su, unsigned mu,
-                                         unsigned hi, unsigned long long old_or_none, bool issued, int *nxt, int *far,
-                                         int *touched) {
-  const unsigned mb = max(__float_as_uint(d), mu);  // M = max(M[u], d)
-  if (mb < hi) {
-    const unsigned long long cand = claim_word(mb, su);
-    const unsigned long long old = issued ? old_or_none : (lp_u64(&a.rec[v].word) <= cand ? cand : atomicMin(&a.rec[v].word, cand));
-    if (old > cand) {
-      if (old == kMaxKey) touched[wave_slot(&sh.n_touched)] = v;
-      if (needs_queue(old)) nxt[wave_slot(&sh.n_nxt)] = v;
-    }
-  } else if (lp_u64(&a.rec[v].word) == kMaxKey) {  // beyond the band: wait for its band
+// Neighbours in `wait` are beyond the band: each unclaimed one not yet waiting is marked waiting
+// (one CAS winner) and reported in p.far, to be pulled at the band boundary.
+__device__ __forceinline__ void wait_for_band(const TraceArgs &a, const int *off, int u, unsigned wait, Pushes &p) {
+  while (wait) {
+    const int k = __ffs(wait) - 1;
+    wait &= wait - 1;
+    const int v = u + off[k];
+    if (lp_u64(&a.rec[v].word) != kMaxKey) continue;
     unsigned seen = rec_meta(a.rec, v);
     while (!(seen & kWaiting)) {
-      unsigned got = atomicCAS(&a.rec[v].meta, seen, seen | kWaiting);
-      if (got == seen) { far[wave_slot(&sh.n_far)] = v; break; }
+      const unsigned got = g_cas(&a.rec[v].meta, seen, seen | kWaiting);
+      if (got == seen) { p.far |= 1u << k; break; }
       seen = got;
     }
   }
@@ -566,7 +612,7 @@ __device__ __forceinline__ void offer_part(const TraceArgs &a, const int *off, i
     old[i] = 0;  // "no improvement" unless the atomic says otherwise
     if (al[i] && d < maxd) {
       if (mb >= hi) wait |= 1u << k;
-      else if (kh[i] >= mb) old[i] = atomicMin(&a.rec[u + nb_offset<k>(a.d)].word, claim_word(mb, su));
+      else if (kh[i] >= mb) old[i] = g_min(&a.rec[u + nb_offset<k>(a.d)].word, claim_word(mb, su));
     }
   });
   for_nb<B, E>([&](auto K) {
@@ -576,18 +622,83 @@ __device__ __forceinline__ void offer_part(const TraceArgs &a, const int *off, i
       if (needs_queue(old[i])) p.near |= 1u << k;
     }
   });
-  while (wait) {  // beyond the band: wait for its band
-    const int k = __ffs(wait) - 1;
-    wait &= wait - 1;
-    const int v = u + off[k];
-    if (lp_u64(&a.rec[v].word) != kMaxKey) continue;
-    unsigned seen = rec_meta(a.rec, v);
-    while (!(seen & kWaiting)) {
-      const unsigned got = atomicCAS(&a.rec[v].meta, seen, seen | kWaiting);
-      if (got == seen) { p.far |= 1u << k; break; }
-      seen = got;
+  wait_for_band(a, off, u, wait, p);
+}
+
+// Neighbours kb..kb+cnt-1 (those set in m; cnt <= kSpan) of u: offer_part with a run-time span.
+__device__ __forceinline__ void offer_span(const TraceArgs &a, const int *off, int u, unsigned m, int kb, int cnt, int x,
+                                           int y, int z, int qx, int qy, int qz, unsigned su, unsigned mu, float maxd,
+                                           unsigned hi, Pushes &p) {
+  constexpr int S = kSpan;
+  bool al[S];
+  unsigned kh[S];
+  int v[S];
+#pragma unroll
+  for (int j = 0; j < S; ++j) {
+    const int k = kb + j;
+    const bool on = j < cnt && ((m >> k) & 1u);
+    v[j] = u + off[on ? k : 0];
+    al[j] = on && (rec_meta(a.rec, v[j]) & kAlive);
+    kh[j] = on ? rec_word_hi(a.rec, v[j]) : 0u;
+  }
+  unsigned mbs[S];
+  unsigned wait = 0;
+  unsigned long long old[S];
+#pragma unroll
+  for (int j = 0; j < S; ++j) {
+    const int k = j < cnt ? kb + j : 0;  // slots past the span are off (al[j] false)
+    const float d = offer_distance(a, x + nb_dx(k), y + nb_dy(k), z + nb_dz(k), qx, qy, qz);
+    const unsigned mb = max(__float_as_uint(d), mu);  // M = max(M[u], d)
+    mbs[j] = mb;
+    old[j] = 0;  // "no improvement" unless the atomic says otherwise
+    if (al[j] && d < maxd) {
+      if (mb >= hi) wait |= 1u << k;
+      else if (kh[j] >= mb) old[j] = g_min(&a.rec[v[j]].word, claim_word(mb, su));
     }
   }
+#pragma unroll
+  for (int j = 0; j < S; ++j) {
+    const int k = kb + j;
+    if (old[j] > claim_word(mbs[j], su)) {  // won
+      if (old[j] == kMaxKey) p.touched |= 1u << k;
+      if (needs_queue(old[j])) p.near |= 1u << k;
+    }
+  }
+  wait_for_band(a, off, u, wait, p);
+}
+
+// One invalidation round over a front of at most kGroupFront entries, a group of lanes per entry.
+// The group's first lane takes the entry's snapshot (re-arming it) and its ball radius before the
+// barrier that separates the round's snapshots from its offers; the values stay in registers.
+__device__ __forceinline__ void offer_groups(const TraceArgs &a, Shared &sh, const int *cur, int nc, unsigned hi, int *nxt,
+                                             int *far, int *touched) {
+  const Group q = group_of(nc);
+  int u = 0;
+  unsigned m = 0;
+  unsigned long long ku = kMaxKey;
+  float maxd = 0.0f;
+  if (q.i < nc) {
+    u = lp_i(cur + q.i);
+    m = rec_meta(a.rec, u) & kNbrBits;
+    if (q.g == 0) {
+      ku = g_and(&a.rec[u].word, ~1ULL);
+      maxd = a.scale * g_ld(a.dbf + claim_source(ku)) + a.constant;
+    }
+  }
+  __syncthreads();
+  ku = __shfl(ku, q.lead, 64);
+  maxd = __shfl(maxd, q.lead, 64);
+  Pushes p;
+  if (q.i < nc) {
+    const unsigned su = claim_source(ku), mu = (unsigned)(ku >> 32);
+    int x, y, z, qx, qy, qz;
+    unravel(u, a.d, x, y, z);
+    unravel(int(su), a.d, qx, qy, qz);
+    offer_span(a, sh.off, u, m, q.kb, q.cnt, x, y, z, qx, qy, qz, su, mu, maxd, hi, p);
+  }
+  emit_neighbours(touched, &sh.n_touched, p.touched, u, sh.off);
+  emit_neighbours(nxt, &sh.n_nxt, p.near, u, sh.off);
+  emit_neighbours(far, &sh.n_far, p.far, u, sh.off);
 }
 
 __device__ __forceinline__ void offer_all(const TraceArgs &a, const int *off, int u, unsigned m, unsigned su, unsigned mu,
@@ -614,7 +725,7 @@ __device__ __forceinline__ void pull_part(const TraceArgs &a, int c, unsigned mc
   float r[E - B];
   for_nb<B, E>([&](auto K) {
     constexpr int k = decltype(K)::value, i = k - B;
-    r[i] = ku[i] != kMaxKey ? a.dbf[claim_source(ku[i])] : 0.0f;
+    r[i] = ku[i] != kMaxKey ? g_ld(a.dbf + claim_source(ku[i])) : 0.0f;
   });
   for_nb<B, E>([&](auto K) {
     constexpr int k = decltype(K)::value, i = k - B;
@@ -628,6 +739,62 @@ __device__ __forceinline__ void pull_part(const TraceArgs &a, int c, unsigned mc
     const unsigned long long cand = ((unsigned long long)mb << 32) | su;
     if (cand < best) best = cand;
   });
+}
+
+// Neighbours kb..kb+cnt-1 (those set in mc; cnt <= kSpan) of waiting voxel c: pull_part with a
+// run-time span.
+__device__ __forceinline__ void pull_span(const TraceArgs &a, const int *off, int c, unsigned mc, int kb, int cnt, int x,
+                                          int y, int z, unsigned long long &best) {
+  constexpr int S = kSpan;
+  unsigned long long ku[S];
+#pragma unroll
+  for (int j = 0; j < S; ++j) {
+    const int k = kb + j;
+    ku[j] = j < cnt && ((mc >> k) & 1u) ? lp_u64(&a.rec[c + off[k]].word) : kMaxKey;
+  }
+  float r[S];
+#pragma unroll
+  for (int j = 0; j < S; ++j) r[j] = ku[j] != kMaxKey ? g_ld(a.dbf + claim_source(ku[j])) : 0.0f;
+#pragma unroll
+  for (int j = 0; j < S; ++j) {
+    if (ku[j] == kMaxKey) continue;
+    const unsigned su = claim_source(ku[j]);
+    int qx, qy, qz;
+    unravel(int(su), a.d, qx, qy, qz);
+    const float d = offer_distance(a, x, y, z, qx, qy, qz);
+    if (d >= a.scale * r[j] + a.constant) continue;
+    const unsigned mb = max(__float_as_uint(d), (unsigned)(ku[j] >> 32));
+    const unsigned long long cand = ((unsigned long long)mb << 32) | su;
+    if (cand < best) best = cand;
+  }
+}
+
+// Band-boundary pulls of at most kGroupFront waiting entries, a group of lanes per entry, the
+// group's minimum folded by shuffles: pull[i] = kClaimed (claimed meanwhile), the best claim, or
+// kMaxKey.
+__device__ __forceinline__ void pull_groups(const TraceArgs &a, Shared &sh, const int *far, int nf, unsigned long long *pull) {
+  const Group q = group_of(nf);
+  unsigned long long best = kMaxKey;
+  bool claimed = false;
+  if (q.i < nf) {
+    const int c = lp_i(far + q.i);
+    const unsigned long long w = lp_u64(&a.rec[c].word);
+    const unsigned mc = rec_meta(a.rec, c) & kNbrBits;
+    claimed = w != kMaxKey;
+    if (!claimed) {
+      int x, y, z;
+      unravel(c, a.d, x, y, z);
+      pull_span(a, sh.off, c, mc, q.kb, q.cnt, x, y, z, best);
+    }
+  }
+  for (int o = 1; o < q.G; o <<= 1) {
+    const unsigned long long other = __shfl_xor(best, o, 64);
+    best = other < best ? other : best;
+  }
+  if (q.g == 0 && q.i < nf) {
+    g_st(pull + q.i, claimed ? kClaimed : best);
+    if (!claimed && best != kMaxKey) atomicMin(&sh.lowest, (unsigned)(best >> 32));
+  }
 }
 
 __device__ __forceinline__ int band_of(unsigned mbits, float delta, int kmin) {
@@ -644,11 +811,11 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const
   __syncthreads();
   int *cur = la, *nxt = lb_, *far = lfar, *keep = lkeep;
   for (int j = threadIdx.x; j < len; j += blockDim.x) {  // sources: path voxels still alive
-    int p = path[j];
+    int p = g_ld(path + j);
     if (!(rec_meta(a.rec, p) & kAlive)) continue;
-    if (atomicCAS(&a.rec[p].word, kMaxKey, claim_word(0u, unsigned(p))) == kMaxKey) {
-      cur[wave_slot(&sh.n_cur)] = p;
-      touched[wave_slot(&sh.n_touched)] = p;
+    if (g_cas(&a.rec[p].word, kMaxKey, claim_word(0u, unsigned(p))) == kMaxKey) {
+      g_st(cur + wave_slot(&sh.n_cur), p);
+      g_st(touched + wave_slot(&sh.n_touched), p);
     }
   }
   __syncthreads();
@@ -657,27 +824,12 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const
     for (;;) {  // synchronous rounds inside the band
       const int nc = sh.n_cur;
       if (nc == 0) break;
-      for (int i = threadIdx.x; i < nc; i += blockDim.x)  // snapshot and re-arm in one atomic
-        snap[i] = atomicAnd(&a.rec[lp_i(cur + i)].word, ~1ULL);
-      __syncthreads();
-      if (nc <= kLaneFront * 2) {  // small front: one lane per neighbour
-        for (int j = threadIdx.x; j < nc * 32; j += blockDim.x) {
-          const int i = j >> 5, k = j & 31;
-          if (k >= 26) continue;
-          const int u = lp_i(cur + i);
-          if (!(((rec_meta(a.rec, u) & kNbrBits) >> k) & 1u)) continue;
-          const int v = u + sh.off[k];
-          if (!(rec_meta(a.rec, v) & kAlive)) continue;
-          const unsigned long long ku = snap[i];
-          const unsigned su = claim_source(ku), mu = (unsigned)(ku >> 32);
-          int x, y, z, qx, qy, qz;
-          unravel(v, a.d, x, y, z);
-          unravel(int(su), a.d, qx, qy, qz);
-          const float d = offer_distance(a, x, y, z, qx, qy, qz);
-          if (d >= a.scale * a.dbf[su] + a.constant) continue;
-          offer_to(a, sh, v, d, su, mu, hi, 0, false, nxt, far, touched);
-        }
+      if (nc <= kGroupFront) {
+        offer_groups(a, sh, cur, nc, hi, nxt, far, touched);
       } else {
+        for (int i = threadIdx.x; i < nc; i += blockDim.x)  // snapshot and re-arm in one atomic
+          g_st(snap + i, g_and(&a.rec[lp_i(cur + i)].word, ~1ULL));
+        __syncthreads();
         for (int base = 0; base < nc; base += blockDim.x) {  // uniform: every lane reaches the appends
           const int i = base + threadIdx.x;
           Pushes p;
@@ -685,7 +837,7 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const
           if (i < nc) {
             u = lp_i(cur + i);
             const unsigned m = (rec_meta(a.rec, u) & kNbrBits);
-            const unsigned long long ku = snap[i];
+            const unsigned long long ku = g_ld(snap + i);
             const unsigned su = claim_source(ku), mu = (unsigned)(ku >> 32);
             offer_all(a, sh.off, u, m, su, mu, a.scale * a.dbf[su] + a.constant, hi, p);
           }
@@ -703,16 +855,20 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const
     if (nf == 0) break;
     if (threadIdx.x == 0) { sh.lowest = 0x7f800000u; sh.prof[12] += nf; sh.prof[13] += 1; }
     __syncthreads();
-    for (int i = threadIdx.x; i < nf; i += blockDim.x) {  // band boundary: exact pulls
-      int c = lp_i(far + i);
-      if (lp_u64(&a.rec[c].word) != kMaxKey) { pull[i] = kClaimed; continue; }
-      const unsigned mc = (rec_meta(a.rec, c) & kNbrBits);
-      int x, y, z;
-      unravel(c, a.d, x, y, z);
-      unsigned long long best = kMaxKey;
-      for_parts<kNbBatch>(mc, [&](auto B, auto E) { pull_part<decltype(B)::value, decltype(E)::value>(a, c, mc, x, y, z, best); });
-      pull[i] = best;
-      if (best != kMaxKey) atomicMin(&sh.lowest, (unsigned)(best >> 32));
+    if (nf <= kGroupFront) {  // band boundary: exact pulls
+      pull_groups(a, sh, far, nf, pull);
+    } else {
+      for (int i = threadIdx.x; i < nf; i += blockDim.x) {
+        int c = lp_i(far + i);
+        if (lp_u64(&a.rec[c].word) != kMaxKey) { g_st(pull + i, kClaimed); continue; }
+        const unsigned mc = (rec_meta(a.rec, c) & kNbrBits);
+        int x, y, z;
+        unravel(c, a.d, x, y, z);
+        unsigned long long best = kMaxKey;
+        for_parts<kNbBatch>(mc, [&](auto B, auto E) { pull_part<decltype(B)::value, decltype(E)::value>(a, c, mc, x, y, z, best); });
+        g_st(pull + i, best);
+        if (best != kMaxKey) atomicMin(&sh.lowest, (unsigned)(best >> 32));
+      }
     }
     __syncthreads();
     if (threadIdx.x == 0 && sh.lowest != 0x7f800000u) sh.band = band_of(sh.lowest, a.delta, sh.band + 1);
@@ -720,16 +876,16 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const
     hi = __float_as_uint((float)(sh.band + 1) * a.delta);
     for (int i = threadIdx.x; i < nf; i += blockDim.x) {  // promote / keep waiting / drop
       int c = lp_i(far + i);
-      unsigned long long p = pull[i];
+      unsigned long long p = g_ld(pull + i);
       if (p == kClaimed) continue;
-      if (p == kMaxKey) { atomicAnd(&a.rec[c].meta, ~kWaiting); continue; }
+      if (p == kMaxKey) { g_and(&a.rec[c].meta, ~kWaiting); continue; }
       if ((unsigned)(p >> 32) < hi) {
         sp_u64(&a.rec[c].word, claim_word((unsigned)(p >> 32), (unsigned)p));
-        touched[wave_slot(&sh.n_touched)] = c;
-        cur[wave_slot(&sh.n_cur)] = c;
-        atomicAnd(&a.rec[c].meta, ~kWaiting);
+        g_st(touched + wave_slot(&sh.n_touched), c);
+        g_st(cur + wave_slot(&sh.n_cur), c);
+        g_and(&a.rec[c].meta, ~kWaiting);
       } else {
-        keep[wave_slot(&sh.n_keep)] = c;
+        g_st(keep + wave_slot(&sh.n_keep), c);
       }
     }
     __syncthreads();
@@ -738,11 +894,11 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const
     __syncthreads();
   }
   const int nt = sh.n_touched;
-  if (threadIdx.x == 0) atomicAdd(reinterpret_cast<unsigned long long *>(a.counters + 6), (unsigned long long)nt);
+  if (threadIdx.x == 0) g_add(reinterpret_cast<unsigned long long *>(a.counters + 6), (unsigned long long)nt);
   for (int j = threadIdx.x; j < nt; j += blockDim.x) {  // the claimed set dies
-    int v = touched[j];
+    int v = g_ld(touched + j);
     sp_u64(&a.rec[v].word, kMaxKey);
-    atomicAnd(&a.rec[v].meta, ~(kAlive | kWaiting));
+    g_and(&a.rec[v].meta, ~(kAlive | kWaiting));
   }
   __syncthreads();
 }
@@ -751,8 +907,8 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const
 __device__ __forceinline__ int next_target(const TraceArgs &a, Shared &sh, int L) {
   if (threadIdx.x == 0) {
     int pc = sh.pcount;
-    if (pc >= a.maxp[L]) sh.target = -1;
-    else if (pc < a.mcount[L]) sh.target = a.mflat[a.mstart[L] + pc];
+    if (pc >= g_ld(a.maxp + L)) sh.target = -1;
+    else if (pc < g_ld(a.mcount + L)) sh.target = g_ld(a.mflat + g_ld(a.mstart + L) + pc);
     else sh.target = -2;
   }
   __syncthreads();
@@ -771,7 +927,7 @@ __device__ __forceinline__ int next_target(const TraceArgs &a, Shared &sh, int L
     for (int r = 0; r < per; ++r) {
       long long j = c + r * blockDim.x + threadIdx.x;
       if (j < e) {
-        int v = a.order[j];
+        int v = g_ld(a.order + j);
         if ((rec_meta(a.rec, v) & kAlive)) atomicMin(&sh.found, int(j - c));
       }
     }
@@ -780,7 +936,7 @@ __device__ __forceinline__ int next_target(const TraceArgs &a, Shared &sh, int L
       __syncthreads();
       if (threadIdx.x == 0) {
         sh.cursor = c + sh.found;
-        sh.target = a.order[c + sh.found];
+        sh.target = g_ld(a.order + c + sh.found);
       }
       __syncthreads();
       return sh.target;
@@ -804,10 +960,10 @@ __device__ __forceinline__ void help_loop(const TraceArgs *__restrict__ all, Sha
         HelpJob *j = board + s;
         const int e = ld_i(&j->epoch);
         if (ld_i(&j->state) != 1) continue;
-        atomicAdd(&j->helpers, 1);
+        g_add(&j->helpers, 1);
         __threadfence();
         if (ld_i(&j->state) == 1 && ld_i(&j->epoch) == e) sh.job = s;
-        else atomicSub(&j->helpers, 1);
+        else g_sub(&j->helpers, 1);
       }
       if (!sh.quit && sh.job < 0) {
         if (++idle > kIdleCap) sh.quit = 1;  // leaving is always safe: owners never wait for helpers to arrive
@@ -823,7 +979,7 @@ __device__ __forceinline__ void help_loop(const TraceArgs *__restrict__ all, Sha
     if (threadIdx.x < 26) sh.off[threadIdx.x] = nb_offset(int(threadIdx.x), a.d);
     __syncthreads();
     help_chunks(a, sh, job);
-    if (threadIdx.x == 0) atomicSub(&job->helpers, 1);
+    if (threadIdx.x == 0) g_sub(&job->helpers, 1);
     __syncthreads();
     idle = 0;
   }
@@ -835,7 +991,7 @@ __global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restri
   __shared__ int item;
   for (;;) {
     __syncthreads();
-    if (threadIdx.x == 0) item = atomicAdd(next, 1);
+    if (threadIdx.x == 0) item = g_add(next, 1);
     __syncthreads();
     if (item >= nqueue) {
       if constexpr (kTraceHelp)
@@ -848,7 +1004,7 @@ __global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restri
     const int L = __builtin_amdgcn_readfirstlane(q.y);
     const Team team{board, next, nqueue, sample, error};
     if (threadIdx.x < 26) sh.off[threadIdx.x] = nb_offset(int(threadIdx.x), a.d);
-    const int seg = a.seg_start[L], count = a.seg_count[L];
+    const int seg = g_ld(a.seg_start + L), count = g_ld(a.seg_count + L);
     int *l0 = a.lists[0] + seg, *l1 = a.lists[1] + seg, *l2 = a.lists[2] + seg, *l3 = a.lists[3] + seg,
         *l4 = a.lists[4] + seg, *walk = a.lists[5] + seg;
     unsigned long long *s0 = a.snaps[0] + seg, *s1 = a.snaps[1] + seg;
@@ -872,33 +1028,33 @@ __global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restri
       lap(sh, 3);
       const int nt = sh.n_touched;
       for (int j = threadIdx.x; j < nt; j += blockDim.x) {  // restore the railroad state
-        sp_u64(&a.rec[l4[j]].word, kMaxKey);
+        sp_u64(&a.rec[g_ld(l4 + j)].word, kMaxKey);
       }
       if (status != 1 || sh.path_len < 0) {
-        if (threadIdx.x == 0) a.ejected[atomicAdd(a.counters + 2, 1) % a.ejected_cap] = L;
+        if (threadIdx.x == 0) g_st(a.ejected + g_add(a.counters + 2, 1) % a.ejected_cap, L);
         __syncthreads();
         break;
       }
       const int len = sh.path_len;
       if (threadIdx.x == 0) {
-        atomicAdd(reinterpret_cast<unsigned long long *>(a.counters + 4), (unsigned long long)nt);
-        int pos = atomicAdd(a.counters + 0, len);
-        int rec = atomicAdd(a.counters + 1, 1);
+        g_add(reinterpret_cast<unsigned long long *>(a.counters + 4), (unsigned long long)nt);
+        int pos = g_add(a.counters + 0, len);
+        int rec = g_add(a.counters + 1, 1);
         if ((long long)pos + len > a.store_cap || rec >= a.rec_cap) {
-          atomicOr(a.counters + 3, 1);  // overflow: reported to the host
+          g_or(a.counters + 3, 1);  // overflow: reported to the host
           sh.store_pos = -1;
         } else {
           sh.store_pos = pos;
-          a.rec_label[rec] = L;
-          a.rec_len[rec] = len;
-          a.rec_start[rec] = pos;
+          g_st(a.rec_label + rec, L);
+          g_st(a.rec_len + rec, len);
+          g_st(a.rec_start + rec, pos);
         }
       }
       __syncthreads();
       const int pos = sh.store_pos;
       for (int j = threadIdx.x; j < len; j += blockDim.x) {
-        int v = walk[j];
-        if (pos >= 0) a.store[pos + (len - 1 - j)] = v;  // stored target-first
+        int v = g_ld(walk + j);
+        if (pos >= 0) g_st(a.store + pos + (len - 1 - j), v);  // stored target-first
         sp_f(&a.rec[v].pdrf, 0.0f);                          // the path becomes rail
       }
       __syncthreads();
@@ -909,12 +1065,12 @@ __global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restri
     }
     if (threadIdx.x == 0 && a.label_stats) {
       sh.prof[0] = (unsigned long long)(wall_clock64() - label_begin);
-      for (int j = 0; j < kProf; ++j) a.label_stats[(long long)kProf * L + j] = sh.prof[j];
+      for (int j = 0; j < kProf; ++j) g_st(a.label_stats + (long long)kProf * L + j, sh.prof[j]);
     }
     if constexpr (kTraceHelp)
       if (threadIdx.x == 0 && board) {
         __threadfence();
-        atomicAdd(labels_done, 1);
+        g_add(labels_done, 1);
       }
   }
 }
