@@ -4,7 +4,9 @@
 // host work, as in the reference's serial plan (borders.cu:53-116, point_set.hpp:11-55).
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cstring>
+#include <future>
 #include <limits>
 
 #include "borders.hpp"
@@ -97,11 +99,62 @@ std::array<float, 4> tie_keys(int64_t ix, int64_t iy, float cx, float cy, int64_
   return {sq(x, y, cx, cy, wx, wy), sq(x, y, mid_x, mid_y, wx, wy), corner, edge};
 }
 
-template <class T> std::vector<int64_t> widen(const std::vector<uint8_t> &raw) {
-  size_t n = raw.size() / sizeof(T);
-  std::vector<int64_t> out(n);
-  const T *p = reinterpret_cast<const T *>(raw.data());
-  for (size_t i = 0; i < n; ++i) out[i] = int64_t(p[i]);
+// One face's host pass (borders.cu:53-116): float32 centroid sums in Kimimaro's order (x outer),
+// the farthest pixels' tie keys, and the picks ordered by each component's first pixel with a
+// positive distance (x + fx*y). Pixel i = y + fy*x, so both loops walk memory in order; equal keys
+// keep the pixel first in (y, x) order, as the reference's y-outer scan does.
+struct FacePick {
+  int64_t first;
+  int64_t id;  // the 2-D component
+  int64_t x, y;
+};
+template <class T>
+std::vector<FacePick> face_picks(const std::vector<uint8_t> &raw, const std::vector<float> &dv, size_t k, int64_t fx,
+                                 int64_t fy, float wx, float wy) {
+  const T *lab = reinterpret_cast<const T *>(raw.data());
+  std::vector<float> maxdist(k, 0.0f), sumx(k, 0.0f), sumy(k, 0.0f);
+  std::vector<uint32_t> size(k, 0);
+  std::vector<int64_t> first(k, std::numeric_limits<int64_t>::max());
+  for (int64_t x = 0; x < fx; ++x)
+    for (int64_t y = 0; y < fy; ++y) {
+      const size_t i = size_t(y + fy * x);
+      const size_t l = size_t(lab[i]);
+      if (!l) continue;
+      sumx[l] += float(x);
+      sumy[l] += float(y);
+      ++size[l];
+      if (dv[i] > 0) {
+        maxdist[l] = std::max(maxdist[l], dv[i]);
+        first[l] = std::min(first[l], x + fx * y);
+      }
+    }
+  std::vector<float> cxs(k, 0.0f), cys(k, 0.0f);
+  for (size_t l = 1; l < k; ++l) {
+    if (!size[l]) continue;
+    float px = (wx * sumx[l]) / float(size[l]), py = (wy * sumy[l]) / float(size[l]);
+    if (px - (wx * float(fx)) / 2.0f < 0) px = px + wx;
+    if (py - (wy * float(fy)) / 2.0f < 0) py = py + wy;
+    cxs[l] = float(int(px / wx));
+    cys[l] = float(int(py / wy));
+  }
+  std::vector<std::array<float, 4>> best(k);
+  std::vector<int64_t> bx(k, -1), by(k, -1);
+  for (int64_t x = 0; x < fx; ++x)
+    for (int64_t y = 0; y < fy; ++y) {
+      const size_t i = size_t(y + fy * x);
+      const size_t l = size_t(lab[i]);
+      if (!l || dv[i] == 0 || dv[i] != maxdist[l]) continue;
+      const auto key = tie_keys(x, y, cxs[l], cys[l], fx, fy, wx, wy);
+      if (bx[l] < 0 || key < best[l] || (key == best[l] && (y < by[l] || (y == by[l] && x < bx[l])))) {
+        best[l] = key;
+        bx[l] = x;
+        by[l] = y;
+      }
+    }
+  std::vector<FacePick> out;
+  for (size_t l = 1; l < k; ++l)
+    if (bx[l] >= 0) out.push_back(FacePick{first[l], int64_t(l), bx[l], by[l]});
+  std::sort(out.begin(), out.end(), [](const FacePick &u, const FacePick &v) { return u.first < v.first; });
   return out;
 }
 
@@ -112,6 +165,17 @@ std::vector<std::vector<Point>> border_targets(Context &ctx, const void *cc, int
   std::vector<TupleSet> sets(size_t(count) + 1);
   const int axes[3][3] = {{0, 1, 2}, {0, 2, 1}, {1, 2, 0}};
   const int dims[3] = {d.sx, d.sy, d.sz};
+  using Clock = std::chrono::steady_clock;
+  auto ms = [](Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+  // The device work of the faces runs in order on the context's stream; each face's host pass runs
+  // on a thread of its own meanwhile. The point sets are filled in face order afterwards.
+  struct Face {
+    std::future<std::vector<FacePick>> picks;
+    std::vector<int64_t> mapping;  // 2-D component -> volume component id
+    int a, b, c, fixed;
+  };
+  std::vector<Face> faces;
+  const auto t0 = Clock::now();
   for (int face = 0; face < 6; ++face) {
     const int a = axes[face / 2][0], b = axes[face / 2][1], c = axes[face / 2][2];
     const int fixed = face % 2 ? dims[c] - 1 : 0, fx = dims[a], fy = dims[b];
@@ -128,56 +192,34 @@ std::vector<std::vector<Point>> border_targets(Context &ctx, const void *cc, int
     const float pa[3] = {aniso[b], aniso[a], 1.0f};
     edt(ctx, c2.labels.data(), c2.cc_bytes, pv.dims, pa, true, 2, dist.data());
     std::vector<uint8_t> raw = c2.labels.download(ctx, size_t(np) * c2.cc_bytes);
-    std::vector<int64_t> lab = c2.cc_bytes == 1 ? widen<uint8_t>(raw) : c2.cc_bytes == 2 ? widen<uint16_t>(raw) : widen<uint32_t>(raw);
     std::vector<float> dv = dist.download(ctx, size_t(np));
     const size_t k = size_t(c2.count) + 1;
-    std::vector<float> maxdist(k, 0.0f), sumx(k, 0.0f), sumy(k, 0.0f);
-    std::vector<uint32_t> size(k, 0);
-    std::vector<int64_t> first(k, std::numeric_limits<int64_t>::max()), pick(k, -1);
-    for (int64_t x = 0; x < fx; ++x)  // float32 centroid sums in Kimimaro's order (x outer)
-      for (int64_t y = 0; y < fy; ++y) {
-        size_t i = size_t(y + int64_t(fy) * x);
-        int64_t l = lab[i];
-        if (!l) continue;
-        sumx[l] += float(x);
-        sumy[l] += float(y);
-        ++size[l];
-        if (dv[i] > 0) {
-          maxdist[l] = std::max(maxdist[l], dv[i]);
-          first[l] = std::min(first[l], x + int64_t(fx) * y);
-        }
-      }
     const float wx = aniso[a], wy = aniso[b];
-    std::vector<float> cxs(k, 0.0f), cys(k, 0.0f);
-    for (size_t l = 1; l < k; ++l) {
-      if (!size[l]) continue;
-      float px = (wx * sumx[l]) / float(size[l]), py = (wy * sumy[l]) / float(size[l]);
-      if (px - (wx * float(fx)) / 2.0f < 0) px = px + wx;
-      if (py - (wy * float(fy)) / 2.0f < 0) py = py + wy;
-      cxs[l] = float(int(px / wx));
-      cys[l] = float(int(py / wy));
-    }
-    std::vector<std::array<float, 4>> best(k);
-    for (int64_t y = 0; y < fy; ++y)
-      for (int64_t x = 0; x < fx; ++x) {
-        size_t i = size_t(y + int64_t(fy) * x);
-        int64_t l = lab[i];
-        if (!l || dv[i] == 0 || dv[i] != maxdist[l]) continue;
-        auto key = tie_keys(x, y, cxs[l], cys[l], fx, fy, wx, wy);
-        if (pick[l] < 0 || key < best[l]) { best[l] = key; pick[l] = x + int64_t(fx) * y; }
-      }
-    std::vector<size_t> order;
-    for (size_t l = 1; l < k; ++l)
-      if (pick[l] >= 0) order.push_back(l);
-    std::sort(order.begin(), order.end(), [&](size_t u, size_t v) { return first[u] < first[v]; });
-    for (size_t l : order) {
-      Point p{};
-      p[a] = pick[l] % fx;
-      p[b] = pick[l] / fx;
-      p[c] = fixed;
-      sets.at(size_t(c2.mapping[l])).add(p);
-    }
+    const int bytes = c2.cc_bytes;
+    Face f;
+    f.picks = std::async(std::launch::async, [raw = std::move(raw), dv = std::move(dv), k, fx, fy, wx, wy, bytes]() {
+      if (bytes == 1) return face_picks<uint8_t>(raw, dv, k, fx, fy, wx, wy);
+      if (bytes == 2) return face_picks<uint16_t>(raw, dv, k, fx, fy, wx, wy);
+      return face_picks<uint32_t>(raw, dv, k, fx, fy, wx, wy);
+    });
+    f.mapping.assign(c2.mapping.begin(), c2.mapping.end());
+    f.a = a;
+    f.b = b;
+    f.c = c;
+    f.fixed = fixed;
+    faces.push_back(std::move(f));
   }
+  const auto t1 = Clock::now();
+  ctx.stats["borders_device_ms"] += ms(t0, t1);  // face extraction, 2-D CCL and EDT, downloads
+  for (Face &f : faces)
+    for (const FacePick &q : f.picks.get()) {
+      Point p{};
+      p[f.a] = q.x;
+      p[f.b] = q.y;
+      p[f.c] = f.fixed;
+      sets.at(size_t(f.mapping[size_t(q.id)])).add(p);
+    }
+  ctx.stats["borders_host_ms"] += ms(t1, Clock::now());  // what the host passes add after the device work
   std::vector<std::vector<Point>> out;
   out.reserve(sets.size());
   for (auto &s : sets) out.push_back(s.items());

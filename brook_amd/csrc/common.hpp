@@ -267,6 +267,29 @@ template <class T> __device__ __forceinline__ T shfl_up_any(T v, unsigned delta)
   else return T(__shfl_up((unsigned)v, delta, 64));
 }
 
+// A group of lanes per front entry, for fronts of at most a quarter of the workgroup: each entry
+// gets G lanes (the largest power of two up to 32 that fits every entry into the workgroup, at
+// least 4); lane g of a group handles neighbours [g*S, g*S + S), S = ceil(26 / G) <= kSpan, so all
+// of an entry's loads, then all of its atomics, are in flight together instead of part after part.
+// One code path serves every G (a span is kSpan slots, the tail predicated off), which keeps a
+// kernel's register pressure at the level of its one-thread-per-entry path.
+constexpr int kSpan = 7;
+struct Group {
+  int G, i, g, lead, kb, cnt;  // lanes per entry, entry, lane in group, group's first lane, span
+};
+__device__ __forceinline__ Group group_of(int n, int block) {
+  Group q;
+  q.G = 32;
+  while (q.G > 4 && n * q.G > block) q.G >>= 1;
+  const int shift = __ffs(q.G) - 1, span = (26 + q.G - 1) / q.G;
+  q.i = threadIdx.x >> shift;
+  q.g = threadIdx.x & (q.G - 1);
+  q.lead = __lane_id() & ~(q.G - 1);
+  q.kb = q.g * span;
+  q.cnt = min(span, 26 - q.kb);  // may be <= 0: nothing to do
+  return q;
+}
+
 // Mask of lanes 0..lane.
 __device__ __forceinline__ unsigned long long lanes_upto(int lane) {
   return lane == 63 ? ~0ULL : ((2ULL << lane) - 1ULL);

@@ -122,6 +122,34 @@ __device__ __forceinline__ unsigned flood_part(VoxelRec *rec, Dims d, int u, uns
   return push;
 }
 
+// Neighbours kb..kb+cnt-1 (those set in m; cnt <= kSpan) of u: flood_part with a run-time span.
+__device__ __forceinline__ unsigned flood_span(VoxelRec *rec, int u, unsigned m, int kb, int cnt, float du, const float *W,
+                                               const int *off) {
+  constexpr int S = kSpan;
+  unsigned seen[S];
+  int v[S];
+#pragma unroll
+  for (int j = 0; j < S; ++j) {
+    const bool on = j < cnt && ((m >> (kb + j)) & 1u);
+    v[j] = u + off[on ? kb + j : 0];
+    seen[j] = on ? rec_word_hi(rec, v[j]) : 0u;
+  }
+  unsigned long long old[S];
+#pragma unroll
+  for (int j = 0; j < S; ++j) {
+    const unsigned cb = __float_as_uint(du + W[j < cnt ? kb + j : 0]);
+    old[j] = 0;
+    if (cb < seen[j]) old[j] = g_min(&rec[v[j]].word, (unsigned long long)cb << 32);
+  }
+  unsigned push = 0;
+#pragma unroll
+  for (int j = 0; j < S; ++j) {
+    const unsigned cb = __float_as_uint(du + W[j < cnt ? kb + j : 0]);
+    if ((old[j] & 1ULL) && old[j] > ((unsigned long long)cb << 32)) push |= 1u << (kb + j);
+  }
+  return push;
+}
+
 // Geometric flood of one label from one source inside one workgroup, on the records' packed words
 // (distance bits high, "not queued" bit 0 low; ~0 = +inf, not queued). Words of the label must be
 // ~0 on entry; on exit they hold the distances, every voxel dequeued (bit 0 set).
@@ -139,19 +167,36 @@ __device__ __forceinline__ void flood_label(VoxelRec *rec, Dims d, int src, cons
   for (;;) {
     const int nc = counts[0];
     if (nc == 0) break;
-    for (int base = 0; base < nc; base += blockDim.x) {  // uniform: every lane reaches the append
-      const int i = base + threadIdx.x;
-      unsigned push = 0;
+    if (nc <= kFloodBlock / 4) {  // a group of lanes per front voxel; its first lane dequeues
+      const Group q = group_of(nc, kFloodBlock);
       int u = 0;
-      if (i < nc) {
-        u = lp_i(cur + i);
-        const unsigned m = rec_meta(rec, u) & kNbrBits;
-        const float du = __uint_as_float((unsigned)(g_or(&rec[u].word, 1ULL) >> 32));
-        for_parts<kFloodBatch>(m, [&](auto B, auto E) {
-          push |= flood_part<decltype(B)::value, decltype(E)::value>(rec, d, u, m, du, W);
-        });
+      unsigned m = 0;
+      float du = 0.0f;
+      if (q.g == 0 && q.i < nc) {
+        u = lp_i(cur + q.i);
+        m = rec_meta(rec, u) & kNbrBits;
+        du = __uint_as_float((unsigned)(g_or(&rec[u].word, 1ULL) >> 32));
       }
+      u = __shfl(u, q.lead, 64);
+      m = __shfl(m, q.lead, 64);
+      du = __shfl(du, q.lead, 64);
+      const unsigned push = q.i < nc ? flood_span(rec, u, m, q.kb, q.cnt, du, W, off) : 0u;
       emit_neighbours(nxt, &counts[1], push, u, off);
+    } else {
+      for (int base = 0; base < nc; base += blockDim.x) {  // uniform: every lane reaches the append
+        const int i = base + threadIdx.x;
+        unsigned push = 0;
+        int u = 0;
+        if (i < nc) {
+          u = lp_i(cur + i);
+          const unsigned m = rec_meta(rec, u) & kNbrBits;
+          const float du = __uint_as_float((unsigned)(g_or(&rec[u].word, 1ULL) >> 32));
+          for_parts<kFloodBatch>(m, [&](auto B, auto E) {
+            push |= flood_part<decltype(B)::value, decltype(E)::value>(rec, d, u, m, du, W);
+          });
+        }
+        emit_neighbours(nxt, &counts[1], push, u, off);
+      }
     }
     __syncthreads();
     if (threadIdx.x == 0) { counts[0] = counts[1]; counts[1] = 0; }

@@ -46,36 +46,16 @@ __device__ __forceinline__ int wave_min(int v) {
   return v;
 }
 
-// One open railroad round on the help board (global memory).
-struct HelpJob {
-  int state;  // 0 free, 3 being filled, 1 open, 2 closing
-  int epoch;  // generation: a helper joins only the job it saw before announcing itself
-  int helpers;
-  int sample, nc, nchunks, next_chunk, done_chunks;
-  int n_nxt, n_far, n_touched;
-  float bound;
-  const int *cur;
-  int *nxt, *far, *touched;
-  unsigned long long rail, minfar;
-};
-constexpr int kHelpBoard = 64;
-// Helping is compiled out: its device-scope fences (an L2 writeback and invalidate per wave and
-// chunk on gfx950) stalled the whole chip, and the path stays off until it synchronizes within an
-// XCD instead.
-#ifndef BROOK_TRACE_HELP
-#define BROOK_TRACE_HELP 0
-#endif
-constexpr bool kTraceHelp = BROOK_TRACE_HELP != 0;
-
-struct Team {
-  HelpJob *board;  // null: no helping
-  const int *next;  // the label queue cursor
-  int nqueue, sample;
-  int *error;
-};
+// Front lists (the near rounds of the railroad and the invalidation rounds): entry j lives in LDS for
+// j < kLdsFront and in the label's global list beyond, so a round's appends and the next round's
+// reads of a small front never wait for device memory.
+constexpr int kLdsFront = 4096;
 
 struct Shared {
-  int n_cur, n_nxt, n_far, n_farnxt, n_touched, n_keep;
+  // Front-list lengths rotate over three counters: round r reads cnt[r], appends to cnt[r+1] and
+  // clears cnt[r+2] (read in round r-1), which saves the barrier that a swap of two would need.
+  int cnt[3];
+  int n_far, n_farnxt, n_touched, n_keep;
   int done, target, path_len, found, band;
   long long cursor, seg_end;
   int pcount;
@@ -84,14 +64,36 @@ struct Shared {
   unsigned long long rail_best, minfar;
   int store_pos;
   int off[26];  // flat offsets of the 26 neighbours
-  int chunk, job, quit;
   // per-label profile: [0] total [1] target [2] railroad [3] backtrace [4] invalidation (wall-clock
   // ticks), [5] railroad rounds, [6] invalidation rounds, [7] paths, [8] railroad phases,
   // [9] far entries scanned, [10] near entries expanded, [11] invalidation entries expanded,
   // [12] waiting entries pulled, [13] bands
   unsigned long long prof[kProf];
   long long tick;
+  int front[2][kLdsFront];
 };
+
+__device__ __forceinline__ int rot_next(int r) { return r == 2 ? 0 : r + 1; }
+__device__ __forceinline__ int rot_prev(int r) { return r == 0 ? 2 : r - 1; }
+
+struct Front {
+  int *g;  // the label's list in device memory (entries >= kLdsFront)
+  int *s;  // LDS (entries < kLdsFront)
+};
+__device__ __forceinline__ int front_get(const Front &f, int j) { return j < kLdsFront ? f.s[j] : lp_i(f.g + j); }
+__device__ __forceinline__ void front_put(const Front &f, int j, int v) {
+  if (j < kLdsFront) f.s[j] = v;
+  else g_st(f.g + j, v);
+}
+// emit_neighbours into a front list.
+__device__ __forceinline__ void emit_front(const Front &f, int *counter, unsigned mask, int u, const int *off) {
+  int pos = wave_reserve(counter, __popc(mask));
+  while (mask) {
+    const int k = __ffs(mask) - 1;
+    mask &= mask - 1;
+    front_put(f, pos++, u + off[k]);
+  }
+}
 
 __device__ __forceinline__ void lap(Shared &sh, int slot) {
   if (threadIdx.x == 0) {
@@ -189,28 +191,9 @@ __device__ __forceinline__ float dequeue(const TraceArgs &a, int u) {
   return __uint_as_float((unsigned)(g_or(&a.rec[u].word, 1ULL) >> 32));
 }
 
-// ---- fronts of up to a quarter workgroup: a group of lanes per entry --------------------------
-// Each entry gets G lanes (the largest power of two up to 32 that fits every entry into the
-// workgroup, at least 4); lane g of a group handles neighbours [g*S, g*S + S), S = ceil(26 / G) <=
-// kSpan, so all of an entry's loads, then all of its atomics, are in flight together instead of
-// part after part. One code path serves every G (a span is kSpan slots, the tail predicated off),
-// which keeps the kernel's register pressure at the level of the one-thread-per-entry path.
-constexpr int kGroupFront = kBlock / 4, kSpan = 7;
-struct Group {
-  int G, i, g, lead, kb, cnt;  // lanes per entry, entry, lane in group, first lane, span
-};
-__device__ __forceinline__ Group group_of(int n) {
-  Group q;
-  q.G = 32;
-  while (q.G > 4 && n * q.G > kBlock) q.G >>= 1;
-  const int shift = __ffs(q.G) - 1, span = (26 + q.G - 1) / q.G;
-  q.i = threadIdx.x >> shift;
-  q.g = threadIdx.x & (q.G - 1);
-  q.lead = __lane_id() & ~(q.G - 1);
-  q.kb = q.g * span;
-  q.cnt = min(span, 26 - q.kb);  // may be <= 0: nothing to do
-  return q;
-}
+// ---- fronts of up to a quarter workgroup: a group of lanes per entry (group_of) ----------------
+constexpr int kGroupFront = kBlock / 4;
+__device__ __forceinline__ Group group_of(int n) { return brook::group_of(n, kBlock); }
 
 // Neighbours kb..kb+cnt-1 (those set in m; cnt <= kSpan) of u: relax_part with a run-time span.
 __device__ __forceinline__ void relax_span(const TraceArgs &a, RoadMins &mins, int u, unsigned m, int kb, int cnt, float du,
@@ -262,14 +245,14 @@ __device__ __forceinline__ void relax_span(const TraceArgs &a, RoadMins &mins, i
 
 // One railroad round over a front of at most kGroupFront entries, a group of lanes per entry: the
 // group's first lane dequeues the entry and shares it.
-__device__ __forceinline__ void relax_groups(const TraceArgs &a, Shared &sh, RoadMins &mins, const int *cur, int nc,
-                                             float bound, int *nxt, int *far, int *touched) {
+__device__ __forceinline__ void relax_groups(const TraceArgs &a, Shared &sh, RoadMins &mins, const Front &cur, int nc,
+                                             float bound, const Front &nxt, int *n_nxt, int *far, int *touched) {
   const Group q = group_of(nc);
   int u = 0;
   unsigned m = 0;
   float du = kInf;
   if (q.g == 0 && q.i < nc) {
-    u = lp_i(cur + q.i);
+    u = front_get(cur, q.i);
     m = rec_meta(a.rec, u) & kNbrBits;
     du = dequeue(a, u);
   }
@@ -279,138 +262,27 @@ __device__ __forceinline__ void relax_groups(const TraceArgs &a, Shared &sh, Roa
   Pushes p;
   if (q.i < nc && du <= bound) relax_span(a, mins, u, m, q.kb, q.cnt, du, bound, sh.off, p);
   emit_neighbours(touched, &sh.n_touched, p.touched, u, sh.off);
-  emit_neighbours(nxt, &sh.n_nxt, p.near, u, sh.off);
+  emit_front(nxt, n_nxt, p.near, u, sh.off);
   emit_neighbours(far, &sh.n_far, p.far, u, sh.off);
 }
 
-// ---- help with large railroad rounds ---------------------------------------------------------
-// Once the label queue is empty, workgroups without a label help the ones still tracing: a label
-// whose railroad front reaches kHelpMin entries posts the round on a small board; owner and helpers
-// claim chunks of the front, append to the round's lists through global counters and fold their
-// minima into the job. The owner waits only for chunks already claimed (never for a helper to
-// arrive); helpers join a job only if its generation is unchanged across their join, and leave the
-// kernel when every label is done. Every spin is capped: past the cap the kernel reports an error
-// instead of waiting further.
-constexpr int kHelpMin = 4096, kHelpChunk = 1024;
-constexpr long long kSpinCap = 1LL << 24;       // owner waits (bounded by one chunk's work)
-constexpr long long kIdleCap = 1LL << 22;       // an idle helper leaves after this many empty polls
-
-__device__ __forceinline__ void relax_front(const TraceArgs &a, Shared &sh, RoadMins &mins, const int *cur, int lo, int hi,
-                                            float bound, int *nxt, int *far, int *touched, int *n_nxt, int *n_far,
-                                            int *n_touched) {
-  for (int base = lo; base < hi; base += blockDim.x) {  // uniform: every lane reaches the appends
+// One railroad round over a larger front: a thread per entry, its neighbours in parts.
+__device__ __forceinline__ void relax_front(const TraceArgs &a, Shared &sh, RoadMins &mins, const Front &cur, int nc,
+                                            float bound, const Front &nxt, int *n_nxt, int *far, int *touched) {
+  for (int base = 0; base < nc; base += blockDim.x) {  // uniform: every lane reaches the appends
     const int i = base + threadIdx.x;
     Pushes p;
     int u = 0;
-    if (i < hi) {
-      u = lp_i(cur + i);
+    if (i < nc) {
+      u = front_get(cur, i);
       const unsigned m = (rec_meta(a.rec, u) & kNbrBits);
       const float du = dequeue(a, u);
       if (du <= bound) relax_all(a, mins, u, m, du, bound, p);
     }
-    emit_neighbours(touched, n_touched, p.touched, u, sh.off);
-    emit_neighbours(nxt, n_nxt, p.near, u, sh.off);
-    emit_neighbours(far, n_far, p.far, u, sh.off);
+    emit_neighbours(touched, &sh.n_touched, p.touched, u, sh.off);
+    emit_front(nxt, n_nxt, p.near, u, sh.off);
+    emit_neighbours(far, &sh.n_far, p.far, u, sh.off);
   }
-}
-
-// Claims and relaxes chunks of an open job until none is left (owner and helpers alike).
-__device__ __forceinline__ void help_chunks(const TraceArgs &a, Shared &sh, HelpJob *job) {
-  const int nc = ld_i(&job->nc), nchunks = ld_i(&job->nchunks);
-  const float bound = ld_f(&job->bound);
-  const int *cur = job->cur;
-  int *nxt = job->nxt, *far = job->far, *touched = job->touched;
-  for (;;) {
-    if (threadIdx.x == 0) sh.chunk = g_add(&job->next_chunk, 1);
-    __syncthreads();
-    const int c = sh.chunk;
-    __syncthreads();
-    if (c >= nchunks) break;
-    RoadMins mins;
-    relax_front(a, sh, mins, cur, c * kHelpChunk, min(nc, (c + 1) * kHelpChunk), bound, nxt, far, touched, &job->n_nxt,
-                &job->n_far, &job->n_touched);
-    const unsigned long long r = wave_min_u64(mins.rail), f = wave_min_u64(mins.far);
-    if (__lane_id() == 0) {
-      if (r != kMaxKey) g_min(&job->rail, r);
-      if (f != kMinfarNone) g_min(&job->minfar, f);
-    }
-    __threadfence();  // this thread's list entries and record updates are visible device-wide
-    __syncthreads();
-    if (threadIdx.x == 0) g_add(&job->done_chunks, 1);
-  }
-}
-
-// Spins until *p >= want (or the cap, which records an error); returns false on the cap.
-__device__ __forceinline__ bool spin_until_at_least(const int *p, int want, int *error) {
-  for (long long i = 0; ld_i(p) < want; ++i) {
-    if (i > kSpinCap) { g_xchg(error, 1); return false; }
-    __builtin_amdgcn_s_sleep(2);
-  }
-  return true;
-}
-// Spins until *p <= 0 (or the cap, which records an error).
-__device__ __forceinline__ bool spin_until_none(const int *p, int *error) {
-  for (long long i = 0; ld_i(p) > 0; ++i) {
-    if (i > kSpinCap) { g_xchg(error, 1); return false; }
-    __builtin_amdgcn_s_sleep(2);
-  }
-  return true;
-}
-
-// Owner side: posts the round [0, nc) of `cur` when a board slot is free; returns false (nothing
-// done) otherwise. On return true the round is complete and sh's list lengths and minima include
-// every helper's work.
-__device__ bool team_round(const TraceArgs &a, Shared &sh, const Team &team, const int *cur, int nc, float bound,
-                           int *nxt, int *far, int *touched) {
-  if (threadIdx.x == 0) {
-    sh.job = -1;
-    if (ld_i(team.next) >= team.nqueue)  // the label queue is empty: idle workgroups exist
-      for (int s = 0; s < kHelpBoard; ++s)
-        if (g_cas(&team.board[s].state, 0, 3) == 0) { sh.job = s; break; }
-  }
-  __syncthreads();
-  if (sh.job < 0) return false;
-  HelpJob *job = team.board + sh.job;
-  __threadfence();  // every wave: its front entries and record updates are visible device-wide
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    job->sample = team.sample;
-    job->nc = nc;
-    job->nchunks = (nc + kHelpChunk - 1) / kHelpChunk;
-    job->next_chunk = 0;
-    job->done_chunks = 0;
-    job->bound = bound;
-    job->cur = cur;
-    job->nxt = nxt;
-    job->far = far;
-    job->touched = touched;
-    job->n_nxt = sh.n_nxt;
-    job->n_far = sh.n_far;
-    job->n_touched = sh.n_touched;
-    job->rail = kMaxKey;
-    job->minfar = kMinfarNone;
-    job->epoch += 1;
-    __threadfence();
-    g_xchg(&job->state, 1);  // open
-  }
-  __syncthreads();
-  help_chunks(a, sh, job);
-  if (threadIdx.x == 0) {
-    spin_until_at_least(&job->done_chunks, ld_i(&job->nchunks), team.error);
-    g_xchg(&job->state, 2);  // closing: no new joins
-    spin_until_none(&job->helpers, team.error);
-    __threadfence();
-    sh.n_nxt = ld_i(&job->n_nxt);
-    sh.n_far = ld_i(&job->n_far);
-    sh.n_touched = ld_i(&job->n_touched);
-    const unsigned long long r = ld_u64(&job->rail), f = ld_u64(&job->minfar);
-    if (r < sh.rail_best) sh.rail_best = r;
-    if (f < sh.minfar) sh.minfar = f;
-    g_xchg(&job->state, 0);  // free
-  }
-  __threadfence();  // every wave: the helpers' list entries and record updates
-  __syncthreads();
-  return true;
 }
 
 __device__ __forceinline__ float rail_bound(const Shared &sh) {
@@ -421,12 +293,12 @@ __device__ __forceinline__ float rail_bound(const Shared &sh) {
 }
 
 // Returns 1 when the nearest rail is settled (sh.rail_best), 2 when no rail is reachable.
-__device__ __forceinline__ int railroad(const TraceArgs &a, Shared &sh, const Team &team, int t, int *near_a,
-                                        int *near_b, int *far_a, int *far_b, int *touched) {
+__device__ __forceinline__ int railroad(const TraceArgs &a, Shared &sh, int t, const Front &near_a, const Front &near_b,
+                                        int *far_a, int *far_b, int *touched) {
   if (threadIdx.x == 0) {
     g_st(touched, t);
     sh.n_touched = 1;
-    sh.n_cur = sh.n_nxt = sh.n_far = sh.n_farnxt = 0;
+    sh.cnt[0] = sh.cnt[1] = sh.cnt[2] = sh.n_far = sh.n_farnxt = 0;
     sh.minfar = kMinfarNone;
     float ft = rec_pdrf(a.rec, t);
     if (ft == 0.0f) {
@@ -437,29 +309,29 @@ __device__ __forceinline__ int railroad(const TraceArgs &a, Shared &sh, const Te
       sp_u64(&a.rec[t].word, 0ULL);  // distance 0, queued
       sh.rail_best = kMaxKey;
       sh.done = 0;
-      g_st(near_a, t);
-      sh.n_cur = 1;
+      front_put(near_a, 0, t);
+      sh.cnt[0] = 1;
       sh.T = fmaxf(2.0f * ft, 1e-30f);
     }
   }
   __syncthreads();
   RoadMins mins;
-  int *cur = near_a, *nxt = near_b, *fcur = far_a, *fnxt = far_b;
+  Front cur = near_a, nxt = near_b;
+  int *fcur = far_a, *fnxt = far_b;
+  int r = 0;  // the near list's length is sh.cnt[r]
   while (sh.done == 0) {
     for (;;) {  // settle the near band
-      const int nc = sh.n_cur;
+      const int nc = sh.cnt[r];
       if (nc == 0) break;
+      const int rn = rot_next(r);
+      if (threadIdx.x == 0) { sh.cnt[rot_prev(r)] = 0; sh.prof[5] += 1; sh.prof[10] += nc; }
       const float bound = rail_bound(sh);
-      if (nc <= kGroupFront) {
-        relax_groups(a, sh, mins, cur, nc, bound, nxt, fcur, touched);
-      } else if (!kTraceHelp || nc < kHelpMin || !team.board || !team_round(a, sh, team, cur, nc, bound, nxt, fcur, touched)) {
-        relax_front(a, sh, mins, cur, 0, nc, bound, nxt, fcur, touched, &sh.n_nxt, &sh.n_far, &sh.n_touched);
-      }
+      if (nc <= kGroupFront) relax_groups(a, sh, mins, cur, nc, bound, nxt, &sh.cnt[rn], fcur, touched);
+      else relax_front(a, sh, mins, cur, nc, bound, nxt, &sh.cnt[rn], fcur, touched);
       road_fold(sh, mins);
       __syncthreads();
-      if (threadIdx.x == 0) { sh.n_cur = sh.n_nxt; sh.n_nxt = 0; sh.prof[5] += 1; sh.prof[10] += nc; }
-      int *tmp = cur; cur = nxt; nxt = tmp;
-      __syncthreads();
+      r = rn;
+      const Front tmp = cur; cur = nxt; nxt = tmp;
     }
     if (threadIdx.x == 0) {  // phase end: nearest rail final? else advance the band
       unsigned long long rb = sh.rail_best, mf = sh.minfar;
@@ -490,7 +362,7 @@ __device__ __forceinline__ int railroad(const TraceArgs &a, Shared &sh, const Te
       if (!far_listed(was)) continue;
       const float dw = __uint_as_float((unsigned)(was >> 32));
       if (dw <= bound) {
-        g_st(cur + wave_slot(&sh.n_cur), w);
+        front_put(cur, wave_slot(&sh.cnt[r]), w);
       } else {
         g_or(&a.rec[w].word, kRoadFar);
         const unsigned long long key = ((unsigned long long)__float_as_uint(dw) << 32) | __float_as_uint(rec_pdrf(a.rec, w));
@@ -670,15 +542,15 @@ __device__ __forceinline__ void offer_span(const TraceArgs &a, const int *off, i
 // One invalidation round over a front of at most kGroupFront entries, a group of lanes per entry.
 // The group's first lane takes the entry's snapshot (re-arming it) and its ball radius before the
 // barrier that separates the round's snapshots from its offers; the values stay in registers.
-__device__ __forceinline__ void offer_groups(const TraceArgs &a, Shared &sh, const int *cur, int nc, unsigned hi, int *nxt,
-                                             int *far, int *touched) {
+__device__ __forceinline__ void offer_groups(const TraceArgs &a, Shared &sh, const Front &cur, int nc, unsigned hi,
+                                             const Front &nxt, int *n_nxt, int *far, int *touched) {
   const Group q = group_of(nc);
   int u = 0;
   unsigned m = 0;
   unsigned long long ku = kMaxKey;
   float maxd = 0.0f;
   if (q.i < nc) {
-    u = lp_i(cur + q.i);
+    u = front_get(cur, q.i);
     m = rec_meta(a.rec, u) & kNbrBits;
     if (q.g == 0) {
       ku = g_and(&a.rec[u].word, ~1ULL);
@@ -697,7 +569,7 @@ __device__ __forceinline__ void offer_groups(const TraceArgs &a, Shared &sh, con
     offer_span(a, sh.off, u, m, q.kb, q.cnt, x, y, z, qx, qy, qz, su, mu, maxd, hi, p);
   }
   emit_neighbours(touched, &sh.n_touched, p.touched, u, sh.off);
-  emit_neighbours(nxt, &sh.n_nxt, p.near, u, sh.off);
+  emit_front(nxt, n_nxt, p.near, u, sh.off);
   emit_neighbours(far, &sh.n_far, p.far, u, sh.off);
 }
 
@@ -805,16 +677,22 @@ __device__ __forceinline__ int band_of(unsigned mbits, float delta, int kmin) {
   return k < kmin ? kmin : k;
 }
 
-__device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const int *path, int len, int *la, int *lb_, int *lfar,
-                           int *lkeep, int *touched, unsigned long long *snap, unsigned long long *pull) {
-  if (threadIdx.x == 0) { sh.n_cur = sh.n_nxt = sh.n_far = sh.n_keep = sh.n_touched = 0; sh.band = 0; }
+__device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const int *path, int len, const Front &la,
+                                           const Front &lb, int *lfar, int *lkeep, int *touched, unsigned long long *snap,
+                                           unsigned long long *pull) {
+  if (threadIdx.x == 0) {
+    sh.cnt[0] = sh.cnt[1] = sh.cnt[2] = sh.n_far = sh.n_keep = sh.n_touched = 0;
+    sh.band = 0;
+  }
   __syncthreads();
-  int *cur = la, *nxt = lb_, *far = lfar, *keep = lkeep;
+  Front cur = la, nxt = lb;
+  int *far = lfar, *keep = lkeep;
+  int r = 0;  // the current list's length is sh.cnt[r]
   for (int j = threadIdx.x; j < len; j += blockDim.x) {  // sources: path voxels still alive
     int p = g_ld(path + j);
     if (!(rec_meta(a.rec, p) & kAlive)) continue;
     if (g_cas(&a.rec[p].word, kMaxKey, claim_word(0u, unsigned(p))) == kMaxKey) {
-      g_st(cur + wave_slot(&sh.n_cur), p);
+      front_put(cur, wave_slot(&sh.cnt[0]), p);
       g_st(touched + wave_slot(&sh.n_touched), p);
     }
   }
@@ -822,34 +700,35 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const
   for (;;) {
     unsigned hi = __float_as_uint((float)(sh.band + 1) * a.delta);
     for (;;) {  // synchronous rounds inside the band
-      const int nc = sh.n_cur;
+      const int nc = sh.cnt[r];
       if (nc == 0) break;
+      const int rn = rot_next(r);
+      if (threadIdx.x == 0) { sh.cnt[rot_prev(r)] = 0; sh.prof[6] += 1; sh.prof[11] += nc; }
       if (nc <= kGroupFront) {
-        offer_groups(a, sh, cur, nc, hi, nxt, far, touched);
+        offer_groups(a, sh, cur, nc, hi, nxt, &sh.cnt[rn], far, touched);
       } else {
         for (int i = threadIdx.x; i < nc; i += blockDim.x)  // snapshot and re-arm in one atomic
-          g_st(snap + i, g_and(&a.rec[lp_i(cur + i)].word, ~1ULL));
+          g_st(snap + i, g_and(&a.rec[front_get(cur, i)].word, ~1ULL));
         __syncthreads();
         for (int base = 0; base < nc; base += blockDim.x) {  // uniform: every lane reaches the appends
           const int i = base + threadIdx.x;
           Pushes p;
           int u = 0;
           if (i < nc) {
-            u = lp_i(cur + i);
+            u = front_get(cur, i);
             const unsigned m = (rec_meta(a.rec, u) & kNbrBits);
             const unsigned long long ku = g_ld(snap + i);
             const unsigned su = claim_source(ku), mu = (unsigned)(ku >> 32);
             offer_all(a, sh.off, u, m, su, mu, a.scale * a.dbf[su] + a.constant, hi, p);
           }
           emit_neighbours(touched, &sh.n_touched, p.touched, u, sh.off);
-          emit_neighbours(nxt, &sh.n_nxt, p.near, u, sh.off);
+          emit_front(nxt, &sh.cnt[rn], p.near, u, sh.off);
           emit_neighbours(far, &sh.n_far, p.far, u, sh.off);
         }
       }
       __syncthreads();
-      if (threadIdx.x == 0) { sh.n_cur = sh.n_nxt; sh.n_nxt = 0; sh.prof[6] += 1; sh.prof[11] += nc; }
-      int *tmp = cur; cur = nxt; nxt = tmp;
-      __syncthreads();
+      r = rn;
+      const Front tmp = cur; cur = nxt; nxt = tmp;
     }
     const int nf = sh.n_far;
     if (nf == 0) break;
@@ -882,7 +761,7 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const
       if ((unsigned)(p >> 32) < hi) {
         sp_u64(&a.rec[c].word, claim_word((unsigned)(p >> 32), (unsigned)p));
         g_st(touched + wave_slot(&sh.n_touched), c);
-        g_st(cur + wave_slot(&sh.n_cur), c);
+        front_put(cur, wave_slot(&sh.cnt[r]), c);
         g_and(&a.rec[c].meta, ~kWaiting);
       } else {
         g_st(keep + wave_slot(&sh.n_keep), c);
@@ -947,66 +826,23 @@ __device__ __forceinline__ int next_target(const TraceArgs &a, Shared &sh, int L
   }
 }
 
-// A workgroup without a label: joins open rounds until every label is done.
-__device__ __forceinline__ void help_loop(const TraceArgs *__restrict__ all, Shared &sh, HelpJob *board, const int *labels_done,
-                          int nqueue, int *error) {
-  long long idle = 0;
-  for (;;) {
-    __syncthreads();  // every wave has read the previous iteration's sh.quit / sh.job
-    if (threadIdx.x == 0) {
-      sh.quit = ld_i(labels_done) >= nqueue;
-      sh.job = -1;
-      for (int s = 0; s < kHelpBoard && !sh.quit && sh.job < 0; ++s) {
-        HelpJob *j = board + s;
-        const int e = ld_i(&j->epoch);
-        if (ld_i(&j->state) != 1) continue;
-        g_add(&j->helpers, 1);
-        __threadfence();
-        if (ld_i(&j->state) == 1 && ld_i(&j->epoch) == e) sh.job = s;
-        else g_sub(&j->helpers, 1);
-      }
-      if (!sh.quit && sh.job < 0) {
-        if (++idle > kIdleCap) sh.quit = 1;  // leaving is always safe: owners never wait for helpers to arrive
-        __builtin_amdgcn_s_sleep(8);
-      }
-    }
-    __syncthreads();
-    if (sh.quit) return;
-    if (sh.job < 0) continue;
-    __threadfence();  // records and lists as the owner left them (no stale L1 lines)
-    HelpJob *job = board + sh.job;
-    const TraceArgs a = all[__builtin_amdgcn_readfirstlane(ld_i(&job->sample))];
-    if (threadIdx.x < 26) sh.off[threadIdx.x] = nb_offset(int(threadIdx.x), a.d);
-    __syncthreads();
-    help_chunks(a, sh, job);
-    if (threadIdx.x == 0) g_sub(&job->helpers, 1);
-    __syncthreads();
-    idle = 0;
-  }
-}
-
 __global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restrict__ all, const int2 *__restrict__ queue,
-                                                      int nqueue, int *next, HelpJob *board, int *labels_done, int *error) {
+                                                      int nqueue, int *next) {
   __shared__ Shared sh;
   __shared__ int item;
   for (;;) {
     __syncthreads();
     if (threadIdx.x == 0) item = g_add(next, 1);
     __syncthreads();
-    if (item >= nqueue) {
-      if constexpr (kTraceHelp)
-        if (board) help_loop(all, sh, board, labels_done, nqueue, error);
-      return;
-    }
+    if (item >= nqueue) return;
     const int2 q = queue[item];
     const int sample = __builtin_amdgcn_readfirstlane(q.x);
     const TraceArgs a = all[sample];  // uniform: scalar registers
     const int L = __builtin_amdgcn_readfirstlane(q.y);
-    const Team team{board, next, nqueue, sample, error};
     if (threadIdx.x < 26) sh.off[threadIdx.x] = nb_offset(int(threadIdx.x), a.d);
     const int seg = g_ld(a.seg_start + L), count = g_ld(a.seg_count + L);
-    int *l0 = a.lists[0] + seg, *l1 = a.lists[1] + seg, *l2 = a.lists[2] + seg, *l3 = a.lists[3] + seg,
-        *l4 = a.lists[4] + seg, *walk = a.lists[5] + seg;
+    int *l2 = a.lists[2] + seg, *l3 = a.lists[3] + seg, *l4 = a.lists[4] + seg, *walk = a.lists[5] + seg;
+    const Front na{a.lists[0] + seg, sh.front[0]}, nb{a.lists[1] + seg, sh.front[1]};
     unsigned long long *s0 = a.snaps[0] + seg, *s1 = a.snaps[1] + seg;
     if (threadIdx.x == 0) {
       sh.cursor = seg;
@@ -1021,7 +857,7 @@ __global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restri
       const int t = next_target(a, sh, L);
       lap(sh, 1);
       if (t < 0) break;
-      const int status = railroad(a, sh, team, t, l0, l1, l2, l3, l4);
+      const int status = railroad(a, sh, t, na, nb, l2, l3, l4);
       lap(sh, 2);
       if (status == 1) backtrace(a, sh, t, count, walk);
       __syncthreads();
@@ -1058,7 +894,7 @@ __global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restri
         sp_f(&a.rec[v].pdrf, 0.0f);                          // the path becomes rail
       }
       __syncthreads();
-      invalidate(a, sh, walk, len, l0, l1, l2, l3, l4, s0, s1);
+      invalidate(a, sh, walk, len, na, nb, l2, l3, l4, s0, s1);
       lap(sh, 4);
       if (threadIdx.x == 0) { sh.pcount += 1; sh.prof[7] += 1; }
       __syncthreads();
@@ -1067,11 +903,6 @@ __global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restri
       sh.prof[0] = (unsigned long long)(wall_clock64() - label_begin);
       for (int j = 0; j < kProf; ++j) g_st(a.label_stats + (long long)kProf * L + j, sh.prof[j]);
     }
-    if constexpr (kTraceHelp)
-      if (threadIdx.x == 0 && board) {
-        __threadfence();
-        g_add(labels_done, 1);
-      }
   }
 }
 
@@ -1085,20 +916,15 @@ void trace_all(Context &ctx, const std::vector<TraceArgs> &args, const std::vect
   for (size_t i = 0; i < items.size(); ++i) q[i] = make_int2(items[i].sample, items[i].label);
   Buffer<int2> queue(ctx, q.size());
   queue.upload(ctx, q.data(), q.size());
-  Buffer<int> next(ctx, 1), done(ctx, 2);  // queue cursor; labels done, error flag
+  Buffer<int> next(ctx, 1);  // queue cursor
   next.clear(ctx);
-  done.clear(ctx);
-  Buffer<HelpJob> board(ctx, kHelpBoard);
-  board.clear(ctx);
   const int nq = int(q.size());
   const int grid = std::min(nq, ctx.compute_units() * 2);
   {
     KernelTimer kt(ctx, "trace_labels");
-    trace_labels<<<grid, kBlock, 0, ctx.stream()>>>(args_dev.data(), queue.data(), nq, next.data(),
-                                                    kTraceHelp ? board.data() : nullptr, done.data(), done.data() + 1);
+    trace_labels<<<grid, kBlock, 0, ctx.stream()>>>(args_dev.data(), queue.data(), nq, next.data());
     BROOK_LAUNCHED();
   }
-  if (done.download(ctx)[1]) throw std::runtime_error("trace: a helper wait exceeded its cap");
 }
 
 }  // namespace brook
