@@ -108,6 +108,45 @@ def batched():
                 plain_ms=round(t_plain * 1e3, 3), speedup=round(t_plain / t_batch, 3))
 
 
+def stress(calls=2):
+    """The stress workload: 1024x1024x512 uint64 Voronoi cells (Poisson(5000) seeds under anisotropy
+    (4,4,40), eroded by one voxel) holding 1-4 tubes each, device-resident; Mvoxels/s and
+    skeletons (objects) per second over `calls` timed calls after one warm call."""
+    import torch
+
+    import brook_amd
+    from brook_amd import _native as native
+    from brook_amd import synth
+    t0 = time.perf_counter()
+    vol, cells, painted = synth.cells_1024()
+    gen_s = time.perf_counter() - t0
+    dev = _device_volume(vol.view(np.int64))  # ids < 2^63: the same values as signed integers
+    del vol
+    aniso = (4, 4, 40)
+    run = lambda: brook_amd.skeletonize(dev, PARAMS, aniso, dust_threshold=1000, output="packed", progress=False)
+    result = run()
+    torch.cuda.synchronize()
+    ctx = native.context()
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        result = run()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / calls
+    n = dev.numel()
+    out = dict(workload="cells_1024: 1024x1024x512 uint64, Poisson(5000) Voronoi cells under (4,4,40) eroded by one "
+                        "voxel, 1-4 tubes of radius U[2,6] per cell, ids cell<<32|tube; dust 1000, fix_borders",
+               voxels=int(n), cells=cells, painted_voxels=painted, components=int(ctx.stat("components")),
+               skeletons=len(result), calls=calls, ms_per_call=round(wall * 1e3, 3),
+               mvox_s=round(n / wall / 1e6, 3), objects_s=round(len(result) / wall, 1),
+               stages_ms={k: round(ctx.stat(k), 3) for k in ("prep_ccl_ms", "prep_edt_ms", "prep_borders_ms",
+                                                              "prep_floods_ms", "prep_pdrf_order_ms", "trace_ms",
+                                                              "assembly_ms")},
+               generator_s=round(gen_s, 2))
+    del dev
+    torch.cuda.empty_cache()
+    return out
+
+
 def served(rates=(10.0,), count=24, seed=5005):
     """Serving form: Poisson request streams, each request a host uint32 volume with every edge in
     [128, 512] and 5-50 tube labels (8 distinct volumes drawn at random per request). The same
@@ -169,7 +208,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-extras", action="store_true", help="skip the host-input, batched and served measurements")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the host-input, batched, stress and served measurements")
     ap.add_argument("--served-sweep", action="store_true", help="serve at 2, 10, 20 and 50 requests/s (default: 10)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--copy-calibration", action="store_true",
@@ -270,6 +310,7 @@ def main():
     if not args.no_extras:
         out["host_io"] = host_io(vol)
         out["batched"] = batched()
+        out["stress"] = stress()
         out["served"] = served((2.0, 10.0, 20.0, 50.0) if args.served_sweep else (10.0,))
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(vol, args.cpu_threads)

@@ -177,6 +177,19 @@ def test_skeletons_match_oracle_tubes(brook, seed):
     _assert_packed_equal(got, want, f"seed {seed}")
 
 
+def test_skeletons_match_oracle_cells(brook):
+    """The stress workload's shape at a small size: uint64 ids above 2^32 (cell << 32 | tube), a few
+    hundred small objects clipped by eroded Voronoi cells, anisotropy (4, 4, 40)."""
+    from brook_amd import synth
+    labels, cells, _ = synth.cells((192, 192, 96), 120, 4004)
+    assert labels.dtype == np.uint64 and cells > 100
+    aniso = (4, 4, 40)
+    got = brook.skeletonize(labels, CFG1_PARAMS, aniso, dust_threshold=100, output="packed", progress=False)
+    want = oracle.skeletonize(labels, CFG1_PARAMS, aniso, 100, threads=8)
+    assert len(want["labels"]) > 200
+    _assert_packed_equal(got, want, "cells")
+
+
 def test_batch_equals_separate(brook):
     from brook_amd import synth
     vols = [synth.volume("tubes", s, n, 70 + i) for i, (s, n) in enumerate([((64, 80, 72), 6), ((96, 64, 64), 9), ((64, 64, 100), 4)])]
