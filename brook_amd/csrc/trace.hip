@@ -422,6 +422,24 @@ __device__ __forceinline__ void backtrace(const TraceArgs &a, Shared &sh, int t,
   if (lane == 0) sh.path_len = kept;
 }
 
+// ---- fix_branching=False: parent walk (one thread) -------------------------------------------
+// backtrace_parent_lab (lockstep_regular.inc:166-202): from the target along the parent field,
+// keeping every vertex up to and including the first without a parent (-1 at the root, STOP -2
+// at a vertex an earlier path emitted). The walk is a pointer chase; its length is the path's.
+__device__ __forceinline__ void parent_walk(const TraceArgs &a, Shared &sh, int t, int count, int *walk) {
+  if (threadIdx.x != 0) return;
+  int cur = t, kept = 0;
+  for (;;) {
+    const int p = lp_i(a.parent + cur);
+    if (kept >= count) { kept = -1; break; }  // longer than the label: cannot happen on a valid field
+    g_st(walk + kept, cur);
+    ++kept;
+    if (p < 0) break;
+    cur = p;
+  }
+  sh.path_len = kept;
+}
+
 // ---- invalidation ---------------------------------------------------------------------------
 // A claim word is (bottleneck bits << 32) | (source << 1) | queued: claims order by (bottleneck,
 // source), and the improving atomicMin returns whether the voxel is already queued for the next
@@ -857,9 +875,15 @@ __global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restri
       const int t = next_target(a, sh, L);
       lap(sh, 1);
       if (t < 0) break;
-      const int status = railroad(a, sh, t, na, nb, l2, l3, l4);
-      lap(sh, 2);
-      if (status == 1) backtrace(a, sh, t, count, walk);
+      int status = 1;
+      if (a.parent) {
+        parent_walk(a, sh, t, count, walk);
+        if (threadIdx.x == 0) sh.n_touched = 0;
+      } else {
+        status = railroad(a, sh, t, na, nb, l2, l3, l4);
+        lap(sh, 2);
+        if (status == 1) backtrace(a, sh, t, count, walk);
+      }
       __syncthreads();
       lap(sh, 3);
       const int nt = sh.n_touched;
@@ -890,8 +914,9 @@ __global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restri
       const int pos = sh.store_pos;
       for (int j = threadIdx.x; j < len; j += blockDim.x) {
         int v = g_ld(walk + j);
-        if (pos >= 0) g_st(a.store + pos + (len - 1 - j), v);  // stored target-first
-        sp_f(&a.rec[v].pdrf, 0.0f);                          // the path becomes rail
+        if (pos >= 0) g_st(a.store + pos + (len - 1 - j), v);
+        if (a.parent) g_st(a.parent + v, -2);  // STOP: later walks end on this vertex
+        else sp_f(&a.rec[v].pdrf, 0.0f);       // the path becomes rail
       }
       __syncthreads();
       invalidate(a, sh, walk, len, na, nb, l2, l3, l4, s0, s1);

@@ -57,6 +57,7 @@ def lib():
         L.oracle_railroad.argtypes = [_P, _P, _I64, _I64, _I64, _I64, _P, _I64]
         L.oracle_prepare.restype = _I64
         L.oracle_prepare.argtypes = [_P, _I64, _I64, _I64, _P, _P, _D, _I, _I, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P]
+        L.oracle_vw_parents.argtypes = [_P, _I64, _I64, _I64, _P, _P, _I64, _P, _P]
         L.oracle_skeletonize.restype = _P
         L.oracle_skeletonize.argtypes = [_P, _I64, _I64, _I64, _P, _I64, _P, _D, _I, _I, _P, _I64, _I]
         L.oracle_result_sizes.argtypes = [_P, _P]
@@ -186,9 +187,25 @@ def prepare(labels, teasar_params=None, anisotropy=(1, 1, 1), dust_threshold=100
                 max_daf=max_daf[:k].copy(), root=root[:k].copy(), target=target[:k].copy())
 
 
+def vw_parents(labels, vw, sources):
+    """Vertex-weighted flood over the float32 weights `vw` from the flat `sources` (label-gated by the
+    26-connected components of `labels`) and the parent field: (dist f32, parent int64), Fortran
+    order."""
+    a, shape = _fortran_int64(labels)
+    w = np.asfortranarray(np.asarray(vw, np.float32))
+    src = np.ascontiguousarray(np.asarray(sources, np.int64))
+    dist = np.zeros(shape, np.float32, order="F")
+    parent = np.zeros(shape, np.int64, order="F")
+    if lib().oracle_vw_parents(_ptr(a), *shape, _ptr(w), _ptr(src), len(src), _ptr(dist), _ptr(parent)) != 0:
+        _fail("vw_parents")
+    return dist, parent
+
+
 def skeletonize(labels, teasar_params=None, anisotropy=(1, 1, 1), dust_threshold=1000, fix_branching=True,
-                fix_borders=True, extra_targets_before=(), threads=1):
-    """Packed result dict: labels, v_off, e_off, vertices (V,3) f32, edges (E,2) u32, radii f32, stats."""
+                fix_borders=True, extra_targets_before=(), threads=1, stop_emitted=True):
+    """Packed result dict: labels, v_off, e_off, vertices (V,3) f32, edges (E,2) u32, radii f32, stats.
+    fix_branching=False walks the parent field; stop_emitted=False makes every walk run to the root
+    (Kimimaro's paths) instead of ending on the first emitted vertex."""
     a, shape = _fortran_int64(labels)
     pv = params_vector(teasar_params)
     mp = (teasar_params or {}).get("max_paths")
@@ -196,7 +213,8 @@ def skeletonize(labels, teasar_params=None, anisotropy=(1, 1, 1), dust_threshold
     tb = np.ascontiguousarray(np.asarray(list(extra_targets_before), np.int64).reshape(-1, 3))
     L = lib()
     h = L.oracle_skeletonize(_ptr(a), *shape, _ptr(pv), -1 if mp is None else int(mp), _ptr(an), float(dust_threshold),
-                             int(fix_branching), int(fix_borders), _ptr(tb), len(tb), threads)
+                             1 if fix_branching else (0 if stop_emitted else -1), int(fix_borders), _ptr(tb), len(tb),
+                             threads)
     if not h:
         _fail("skeletonize")
     try:

@@ -21,6 +21,10 @@
 //   railroad/join  test_package.py:751-811 known answers (26 singles, 325 pairs, all 26, 9^3 road)
 //   invalidation   golden skeleton digests; plain (M, source) Dijkstra cross-check on random balls
 //   whole path     golden digests test_package.py:2079-2106 (Kimimaro 5.8.1 results)
+//   vertex-weighted flood and parents (fix_branching=False)  brute-force float32 Bellman-Ford of
+//                  the vw_step fixpoint; every parent chain satisfies d[v] = vw_step(d[parent], pdrf[v])
+//                  and reaches the root; the STOP rule's union of vertices and edges equals the
+//                  union of full walks to the root (Kimimaro's paths) on random volumes
 #include <algorithm>
 #include <array>
 #include <atomic>
@@ -455,6 +459,51 @@ void geometric_flood(const std::vector<uint32_t> &cc, const Shape &s, const floa
   }
 }
 
+// Vertex-weighted flood (batched.cu:246-251 with flood mode 1): the unique least fixpoint of
+// d[v] = min_u vw_step(d[u], vw[v]) over same-label 26-neighbours. vw_step is monotone
+// non-decreasing in d and adds at least one ulp for a positive weight, so Dijkstra reproduces it.
+void vw_flood(const std::vector<uint32_t> &cc, const Shape &s, const std::vector<float> &vw,
+              const std::vector<i64> &sources, std::vector<float> &dist) {
+  MinHeap heap;
+  for (i64 src : sources) { dist[src] = 0.0f; heap.push({0.0f, src}); }
+  while (!heap.empty()) {
+    HeapItem it = heap.top(); heap.pop();
+    if (it.d > dist[it.v]) continue;
+    i64 x, y, z; s.unravel(it.v, x, y, z);
+    uint32_t l = cc[it.v];
+    for (int k = 0; k < 26; ++k) {
+      i64 nx = x + NX[k], ny = y + NY[k], nz = z + NZ[k];
+      if (!s.inside(nx, ny, nz)) continue;
+      i64 u = s.at(nx, ny, nz);
+      if (cc[u] != l) continue;
+      float c = vw_step(it.d, vw[u]);
+      if (c < dist[u]) { dist[u] = c; heap.push({c, u}); }
+    }
+  }
+}
+
+// parents_lab, batched_regular.inc:113-134: the same-label neighbour with the smallest distance,
+// first k on ties; -1 for background, inactive labels, unreached voxels and the sources (d 0).
+void parent_field(const std::vector<uint32_t> &cc, const Shape &s, const std::vector<uint8_t> &active,
+                  const std::vector<float> &dist, std::vector<int64_t> &parent) {
+  parent.assign(size_t(s.n()), -1);
+  for (i64 v = 0; v < s.n(); ++v) {
+    uint32_t l = cc[v];
+    if (!l || !active[l] || !(dist[v] < FINF) || dist[v] == 0.0f) continue;
+    i64 x, y, z; s.unravel(v, x, y, z);
+    float best = FINF;
+    i64 arg = -1;
+    for (int k = 0; k < 26; ++k) {
+      i64 nx = x + NX[k], ny = y + NY[k], nz = z + NZ[k];
+      if (!s.inside(nx, ny, nz)) continue;
+      i64 u = s.at(nx, ny, nz);
+      if (cc[u] != l) continue;
+      if (dist[u] < best) { best = dist[u]; arg = u; }
+    }
+    parent[v] = arg;
+  }
+}
+
 // Geometric weights, batched.cu:211-214.
 void geometric_weights(const float aniso[3], float W[26]) {
   double x = aniso[0], y = aniso[1], z = aniso[2];
@@ -473,6 +522,7 @@ struct Params {
   float aniso[3] = {1, 1, 1};
   double dust = 1000;
   bool fix_branching = true, fix_borders = true;
+  bool stop_emitted = true;  // fix_branching=False: walks end on the first emitted vertex (lockstep_regular.inc:160-165)
 };
 
 struct Prepared {
@@ -860,7 +910,6 @@ Result skeletonize(const i64 *labels, const Shape &s, const Params &p, const std
                    int threads) {
   Result res;
   if (!s.n() || double(s.n()) <= p.dust) return res;
-  if (!p.fix_branching) throw std::runtime_error("fix_branching=False is not restated by the oracle");
   Prepared pr = prepare(labels, s, p, threads);
   const size_t count = pr.comp.roots.size(), k = count + 1;
   res.components = i64(count);
@@ -900,6 +949,16 @@ Result skeletonize(const i64 *labels, const Shape &s, const Params &p, const std
   std::vector<u64> key(s.n(), MAXKEY);
   std::vector<uint8_t> alive(s.n(), 0), inq(s.n(), 0);
   for (i64 v = 0; v < s.n(); ++v) alive[v] = pr.active[pr.comp.cc[v]] ? 1 : 0;
+  // fix_branching=False: one vertex-weighted flood from the roots over the PDRF and the parent
+  // field (batched.cu:246-251); paths are parent walks (lockstep_regular.inc:166-202).
+  std::vector<int64_t> parent;
+  if (!p.fix_branching) {
+    std::vector<float> vdist(s.n(), FINF);
+    std::vector<i64> roots;
+    for (size_t id : labels_list) roots.push_back(pr.root[id]);
+    vw_flood(pr.comp.cc, s, pr.pdrf, roots, vdist);
+    parent_field(pr.comp.cc, s, pr.active, vdist, parent);
+  }
   // the root becomes a rail (lockstep.cu:183)
   for (size_t id : labels_list) pr.pdrf[pr.root[id]] = 0.0f;
   const float aniso_min = std::min({p.aniso[0], p.aniso[1], p.aniso[2]});
@@ -928,10 +987,23 @@ Result skeletonize(const i64 *labels, const Shape &s, const Params &p, const std
         if (cursor >= ord.size()) break;
         t = ord[cursor];
       }
-      std::vector<i64> path = tr.railroad(L, t);
-      if (path.empty()) throw std::runtime_error("no rail reachable (the per-component fallback is not restated)");
+      std::vector<i64> path;
+      if (p.fix_branching) {
+        path = tr.railroad(L, t);
+        if (path.empty()) throw std::runtime_error("no rail reachable (the per-component fallback is not restated)");
+      } else {
+        // backtrace_parent_lab (lockstep_regular.inc:166-202): the walk keeps every vertex up to
+        // and including the first with no parent (-1 at the root, STOP -2 at an emitted vertex).
+        for (i64 cur = t;;) {
+          path.push_back(cur);
+          i64 par = cur == pr.root[L] ? -1 : parent[cur];
+          if (par < 0) break;
+          cur = par;
+        }
+      }
       records[L].push_back({L, path});
-      for (i64 v : path) pr.pdrf[v] = 0.0f;
+      if (p.fix_branching) for (i64 v : path) pr.pdrf[v] = 0.0f;
+      else if (p.stop_emitted) for (i64 v : path) parent[v] = -2;
       inval_counts[L] += tr.invalidate(L, path);
       pcount++;
     }
@@ -1124,8 +1196,27 @@ int64_t oracle_prepare(const int64_t *labels, int64_t sx, int64_t sy, int64_t sz
   return count;
 }
 
+// Vertex-weighted flood from `sources` over the weights `vw` and the parent field (all labels active).
+int oracle_vw_parents(const int64_t *labels, int64_t sx, int64_t sy, int64_t sz, const float *vw, const int64_t *sources,
+                      int64_t nsources, float *dist_out, int64_t *parent_out) {
+  return guarded([&] {
+    Shape s{sx, sy, sz};
+    Components c = ccl(labels, s);
+    std::vector<float> w(vw, vw + s.n()), dist(s.n(), FINF);
+    std::vector<uint8_t> active(c.roots.size() + 1, 1);
+    active[0] = 0;
+    vw_flood(c.cc, s, w, std::vector<i64>(sources, sources + nsources), dist);
+    std::vector<int64_t> parent;
+    parent_field(c.cc, s, active, dist, parent);
+    std::copy(dist.begin(), dist.end(), dist_out);
+    std::copy(parent.begin(), parent.end(), parent_out);
+  });
+}
+
 // Whole path. params: scale, const, pdrf_scale, pdrf_exponent, soma_detection, soma_acceptance,
-// soma_scale, soma_const; max_paths < 0 = none. Returns an opaque handle (null on error).
+// soma_scale, soma_const; max_paths < 0 = none; fix_branching 1, or 0 (parent walks with the
+// STOP rule), or -1 (parent walks to the root, Kimimaro's paths; for tests). Returns an opaque
+// handle (null on error).
 void *oracle_skeletonize(const int64_t *labels, int64_t sx, int64_t sy, int64_t sz, const double *params,
                          int64_t max_paths, const float *aniso, double dust, int fix_branching, int fix_borders,
                          const int64_t *targets_before, int64_t ntargets, int threads) {
@@ -1137,7 +1228,8 @@ void *oracle_skeletonize(const int64_t *labels, int64_t sx, int64_t sy, int64_t 
     p.max_paths = max_paths;
     std::copy(aniso, aniso + 3, p.aniso);
     p.dust = dust;
-    p.fix_branching = fix_branching != 0;
+    p.fix_branching = fix_branching > 0;  // 1 fix, 0 parent walks with the STOP rule, -1 full walks
+    p.stop_emitted = fix_branching == 0;
     p.fix_borders = fix_borders != 0;
     std::vector<std::array<i64, 3>> before;
     for (int64_t i = 0; i < ntargets; ++i) before.push_back({targets_before[3 * i], targets_before[3 * i + 1], targets_before[3 * i + 2]});

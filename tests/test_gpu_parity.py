@@ -177,6 +177,28 @@ def test_skeletons_match_oracle_tubes(brook, seed):
     _assert_packed_equal(got, want, f"seed {seed}")
 
 
+@pytest.mark.parametrize("seed", [1001, 1002])
+def test_skeletons_match_oracle_tubes_parent_paths(brook, seed):
+    """fix_branching=False: one vertex-weighted field and parent walks with the STOP rule."""
+    from brook_amd import synth
+    labels = synth.tubes_128(seed)
+    aniso = (16, 16, 40)
+    got = brook.skeletonize(labels, CFG1_PARAMS, aniso, fix_branching=False, output="packed", progress=False)
+    want = oracle.skeletonize(labels, CFG1_PARAMS, aniso, 1000, fix_branching=False, threads=8)
+    _assert_packed_equal(got, want, f"seed {seed}")
+
+
+def test_batch_parent_paths_equal_oracle(brook):
+    """fix_branching=False in the batched engine (one parent-field launch for every sample)."""
+    from brook_amd import synth
+    vols = [synth.volume("tubes", s, n, 170 + i) for i, (s, n) in enumerate([((64, 80, 72), 6), ((96, 64, 64), 9)])]
+    opts = dict(teasar_params=CFG1_PARAMS, anisotropy=(16, 16, 40), dust_threshold=500)
+    batch = brook.skeletonize_batch(vols, output="packed", fix_branching=False, **opts)
+    for i, v in enumerate(vols):
+        want = oracle.skeletonize(v, CFG1_PARAMS, (16, 16, 40), 500, fix_branching=False, threads=8)
+        _assert_packed_equal(batch[i], want, f"sample {i}")
+
+
 def test_skeletons_match_oracle_cells(brook):
     """The stress workload's shape at a small size: uint64 ids above 2^32 (cell << 32 | tube), a few
     hundred small objects clipped by eroded Voronoi cells, anisotropy (4, 4, 40)."""
@@ -235,6 +257,6 @@ def test_unsupported_options_raise(brook):
     labels = np.zeros((20, 20, 20), np.uint32)
     labels[5:15, 5:15, 5:15] = 1
     with pytest.raises(NotImplementedError):
-        brook.skeletonize(labels, fix_branching=False, dust_threshold=0, progress=False)
+        brook.skeletonize(labels, fill_holes=True, dust_threshold=0, progress=False)
     with pytest.raises(ValueError):
         brook.skeletonize(labels, dict(pdrf_exponent=3), dust_threshold=0, progress=False)

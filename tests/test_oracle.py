@@ -170,6 +170,78 @@ def test_geometric_flood_matches_bellman_ford():
     assert np.array_equal(got, dist)
 
 
+def _vw_step(d, f):
+    """brook_vw_step (common.cuh:123-127): fl(d + f), one ulp up where a positive f adds nothing."""
+    c = np.float32(d + f)
+    if c == d and f > 0 and d < np.inf:
+        c = np.nextafter(np.float32(d), np.float32(np.inf))
+    return np.float32(c)
+
+
+def test_vw_parents_match_bellman_ford_and_chain_rule():
+    """fix_branching=False field: the vertex-weighted flood equals a brute-force float32 Bellman-Ford
+    fixpoint bit for bit (weights with ties and one-ulp steps), and every parent is the first
+    same-label 26-neighbour (table order) of smallest distance, whose step gives the voxel's
+    distance; chains reach a source."""
+    rng = np.random.default_rng(5)
+    labels = (rng.random((8, 7, 6)) < 0.75).astype(np.uint32) * rng.integers(1, 3, (8, 7, 6)).astype(np.uint32)
+    cc, _, roots = oracle.ccl(labels)
+    vw = rng.choice(np.float32([0.5, 1.0, 1.0, 2.0, 1e-8]), size=cc.shape).astype(np.float32)
+    dist, parent = oracle.vw_parents(labels, vw, roots)
+    offs = [(-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1), (0, 0, 1), (-1, -1, 0), (-1, 1, 0), (1, -1, 0),
+            (1, 1, 0), (0, -1, -1), (0, -1, 1), (0, 1, -1), (0, 1, 1), (-1, 0, -1), (-1, 0, 1), (1, 0, -1), (1, 0, 1)]
+    offs += list(itertools.product((-1, 1), repeat=3))
+    want = np.full(cc.shape, np.inf, np.float32)
+    sources = {np.unravel_index(r, cc.shape, order="F") for r in roots}
+    for p in sources:
+        want[p] = 0
+    nbrs = lambda p: [q for q in (tuple(a + b for a, b in zip(p, o)) for o in offs)
+                      if all(0 <= q[i] < cc.shape[i] for i in range(3)) and cc[q] == cc[p]]
+    changed = True
+    while changed:
+        changed = False
+        for p in np.argwhere(want < np.inf):
+            p = tuple(p)
+            for q in nbrs(p):
+                c = _vw_step(want[p], vw[q])
+                if c < want[q]:
+                    want[q] = c
+                    changed = True
+    assert np.array_equal(dist, want)
+    flat = lambda q: int(np.ravel_multi_index(q, cc.shape, order="F"))
+    for v in np.argwhere(cc > 0):
+        v = tuple(v)
+        if v in sources:
+            assert parent[v] == -1
+            continue
+        best = None
+        for q in nbrs(v):
+            if best is None or dist[q] < dist[best]:
+                best = q
+        assert parent[v] == flat(best)
+        assert dist[v] == _vw_step(dist[best], vw[v])
+        steps, cur = 0, v
+        while parent[cur] >= 0:
+            nxt = np.unravel_index(int(parent[cur]), cc.shape, order="F")
+            assert dist[nxt] < dist[cur]
+            cur, steps = nxt, steps + 1
+        assert cur in sources and steps < cc.size
+
+
+@pytest.mark.parametrize("seed", [1001, 1002])
+def test_parent_walk_stop_rule_equals_full_walks(seed):
+    """The STOP rule of the parent walks (lockstep_regular.inc:160-165) gives the same skeletons as
+    Kimimaro's fix_branching=False paths, every walk running to the root."""
+    from brook_amd import synth
+    labels = synth.volume("tubes", (96, 96, 64), 8, seed)
+    params = dict(scale=1.5, const=300, pdrf_scale=100000, pdrf_exponent=4)
+    stop = oracle.skeletonize(labels, params, (16, 16, 40), 1000, fix_branching=False, threads=8)
+    full = oracle.skeletonize(labels, params, (16, 16, 40), 1000, fix_branching=False, stop_emitted=False, threads=8)
+    assert stop["stats"] == full["stats"] and len(stop["labels"]) > 3
+    for name in ("labels", "v_off", "e_off", "vertices", "edges", "radii"):
+        assert np.array_equal(stop[name], full[name]), name
+
+
 def test_railroad_road_known_answer():
     # test_package.py:751-770: a 9^3 field whose z = 0 plane is rail.
     field = np.ones((9, 9, 9), np.float32, order="F")
