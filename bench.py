@@ -81,6 +81,29 @@ def host_io(vol, calls=3):
                 p50_ms=round(float(np.median(times)) * 1e3, 3))
 
 
+def parent_paths(dev, n, calls=3):
+    """The same workload with fix_branching=False (one vertex-weighted field and parent walks per
+    label instead of a railroad per path): Mvoxels/s over `calls` timed calls after a warm one."""
+    import torch
+
+    import brook_amd
+    from brook_amd import _native as native
+    run = lambda: brook_amd.skeletonize(dev, PARAMS, ANISO, dust_threshold=1000, output="packed", progress=False,
+                                        fix_branching=False)
+    result = run()
+    torch.cuda.synchronize()
+    ctx = native.context()
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        result = run()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / calls
+    return dict(value=round(n / wall / 1e6, 3), unit="Mvoxels/s", calls=calls, ms_per_call=round(wall * 1e3, 3),
+                skeletons=len(result), vertices=int(result.vertices.shape[0]),
+                stages_ms={k: round(ctx.stat(k), 3) for k in ("prep_floods_ms", "prep_pdrf_order_ms",
+                                                              "trace_ms")})
+
+
 def batched():
     """64 mixed-shape volumes (64..256 per axis, 1-50 tube labels each, ~254 M voxels in all),
     device-resident: one skeletonize_batch call against 64 plain calls."""
@@ -209,7 +232,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the host-input, batched, stress and served measurements")
+                    help="skip the host-input, fix_branching=False, batched, stress and served measurements")
     ap.add_argument("--served-sweep", action="store_true", help="serve at 2, 10, 20 and 50 requests/s (default: 10)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--copy-calibration", action="store_true",
@@ -309,6 +332,7 @@ def main():
     )
     if not args.no_extras:
         out["host_io"] = host_io(vol)
+        out["fix_branching_false"] = parent_paths(dev, n)
         out["batched"] = batched()
         out["stress"] = stress()
         out["served"] = served((2.0, 10.0, 20.0, 50.0) if args.served_sweep else (10.0,))

@@ -82,7 +82,6 @@ class Sample {
       : ctx_(ctx), in_(in), o_(o), fields_(fields), d_(in.dims), n_(in.size()) {}
 
   bool done() const { return done_; }
-  const SkeletonizeOptions &options() const { return o_; }
   Packed take_result() { return std::move(result_); }
   // Rough device bytes a sample holds from its preamble to its assembly (40 B per voxel, 56 B per
   // foreground voxel at the usual 40% occupancy, plus slack).
@@ -171,7 +170,6 @@ class Sample {
     j.daf = daf_.data();
     j.lists_a = lists_[0].data();
     j.lists_b = lists_[1].data();
-    j.parent = parent_.data();
     return j;
   }
 
@@ -220,7 +218,8 @@ class Sample {
       if (fields_->pdrf) BROOK_HIP(hipMemcpy(fields_->pdrf, pdrf.data(), size_t(n_) * 4, hipMemcpyDeviceToHost));
       if (fields_->stop) return finish_empty();
     }
-    root_rails(ctx_, rec_.data(), root_dev_.data(), labels_dev.data(), int(labels_.size()));
+    // the root becomes a rail (lockstep.cu:183); the fix_branching=False field floods from it
+    if (o_.fix_branching) root_rails(ctx_, rec_.data(), root_dev_.data(), labels_dev.data(), int(labels_.size()));
     const size_t fl = size_t(std::max<int64_t>(F_, 1));
     order_ = Buffer<int>(ctx_, fl);
     snap_a_ = Buffer<unsigned long long>(ctx_, fl);
@@ -297,6 +296,7 @@ class Sample {
     ta.constant = float(o_.teasar.constant);
     ta.delta = std::min({o_.aniso[0], o_.aniso[1], o_.aniso[2]});
     ta.parent = o_.fix_branching ? nullptr : parent_.data();
+    ta.root = root_dev_.data();
     return ta;
   }
 
@@ -459,12 +459,6 @@ void run_group(Context &ctx, std::vector<std::unique_ptr<Sample>> &group, Lap &l
   items.clear();
   for (size_t i = 0; i < group.size(); ++i) group[i]->items(int(i), items);
   std::stable_sort(items.begin(), items.end(), larger_first);
-  if (!group.empty() && !group[0]->options().fix_branching) {  // one parent field per label
-    jobs.clear();
-    for (auto &s : group) jobs.push_back(s->done() ? FloodJob{} : s->flood_job());
-    prepare_parent_field(ctx, jobs, items);
-    lap("prep_parents_ms");
-  }
   std::vector<TraceArgs> args;
   for (auto &s : group) args.push_back(s->done() ? TraceArgs{} : s->trace_args());
   trace_all(ctx, args, items);

@@ -97,66 +97,54 @@ __global__ void fill_float(float *p, long long n, float v) {
 constexpr int kFloodBlock = BROOK_FLOOD_BLOCK;  // threads per label workgroup
 constexpr int kFloodBatch = kFloodBlock > 512 ? 9 : 26;  // neighbours in flight per thread
 
-// The step of a flood into neighbour k: geometric (du + W[k], batched_coop.inc:10-113 mode 0) or
-// vertex-weighted over the PDRF held in the neighbour's record (vw_step(du, pdrf), mode 1).
-template <bool kVertex> __device__ __forceinline__ float flood_step(float du, const float *W, int k, float pv) {
-  if constexpr (kVertex) return vw_step(du, pv);
-  else return du + W[k];
-}
-
 // Relaxes masked neighbours B..E-1 of u (distance du): all loads, then all atomics, in flight
 // together. Returns the neighbours this thread must queue (improved and found not queued).
-template <int B, int E, bool kVertex>
+template <int B, int E>
 __device__ __forceinline__ unsigned flood_part(VoxelRec *rec, Dims d, int u, unsigned m, float du, const float *W) {
   unsigned seen[E - B];
-  float pv[E - B] = {};
   for_nb<B, E>([&](auto K) {
     constexpr int k = decltype(K)::value, i = k - B;
     seen[i] = (m >> k) & 1u ? rec_word_hi(rec, u + nb_offset<k>(d)) : 0u;
-    if constexpr (kVertex) pv[i] = (m >> k) & 1u ? rec_pdrf(rec, u + nb_offset<k>(d)) : kInf;
   });
   unsigned long long old[E - B];
   for_nb<B, E>([&](auto K) {
     constexpr int k = decltype(K)::value, i = k - B;
-    const unsigned cb = __float_as_uint(flood_step<kVertex>(du, W, k, pv[i]));
+    const unsigned cb = __float_as_uint(du + W[k]);
     old[i] = 0;
     if (cb < seen[i]) old[i] = g_min(&rec[u + nb_offset<k>(d)].word, (unsigned long long)cb << 32);
   });
   unsigned push = 0;
   for_nb<B, E>([&](auto K) {
     constexpr int k = decltype(K)::value, i = k - B;
-    const unsigned cb = __float_as_uint(flood_step<kVertex>(du, W, k, pv[i]));
+    const unsigned cb = __float_as_uint(du + W[k]);
     if ((old[i] & 1ULL) && old[i] > ((unsigned long long)cb << 32)) push |= 1u << k;
   });
   return push;
 }
 
 // Neighbours kb..kb+cnt-1 (those set in m; cnt <= kSpan) of u: flood_part with a run-time span.
-template <bool kVertex>
 __device__ __forceinline__ unsigned flood_span(VoxelRec *rec, int u, unsigned m, int kb, int cnt, float du, const float *W,
                                                const int *off) {
   constexpr int S = kSpan;
   unsigned seen[S];
-  float pv[S] = {};
   int v[S];
 #pragma unroll
   for (int j = 0; j < S; ++j) {
     const bool on = j < cnt && ((m >> (kb + j)) & 1u);
     v[j] = u + off[on ? kb + j : 0];
     seen[j] = on ? rec_word_hi(rec, v[j]) : 0u;
-    if constexpr (kVertex) pv[j] = on ? rec_pdrf(rec, v[j]) : kInf;
   }
   unsigned long long old[S];
 #pragma unroll
   for (int j = 0; j < S; ++j) {
-    const unsigned cb = __float_as_uint(flood_step<kVertex>(du, W, j < cnt ? kb + j : 0, pv[j]));
+    const unsigned cb = __float_as_uint(du + W[j < cnt ? kb + j : 0]);
     old[j] = 0;
     if (cb < seen[j]) old[j] = g_min(&rec[v[j]].word, (unsigned long long)cb << 32);
   }
   unsigned push = 0;
 #pragma unroll
   for (int j = 0; j < S; ++j) {
-    const unsigned cb = __float_as_uint(flood_step<kVertex>(du, W, j < cnt ? kb + j : 0, pv[j]));
+    const unsigned cb = __float_as_uint(du + W[j < cnt ? kb + j : 0]);
     if ((old[j] & 1ULL) && old[j] > ((unsigned long long)cb << 32)) push |= 1u << (kb + j);
   }
   return push;
@@ -167,7 +155,6 @@ __device__ __forceinline__ unsigned flood_span(VoxelRec *rec, int u, unsigned m,
 // ~0 on entry; on exit they hold the distances, every voxel dequeued (bit 0 set).
 // Label-correcting rounds: a voxel is dequeued by one atomic that also returns its distance, and
 // an improvement that finds the voxel not queued queues it, so no update is ever lost.
-template <bool kVertex = false>
 __device__ __forceinline__ void flood_label(VoxelRec *rec, Dims d, int src, const float *W, const int *off, int *cur, int *nxt,
                             int *counts /*LDS [2]*/) {
   if (threadIdx.x == 0) {
@@ -193,7 +180,7 @@ __device__ __forceinline__ void flood_label(VoxelRec *rec, Dims d, int src, cons
       u = __shfl(u, q.lead, 64);
       m = __shfl(m, q.lead, 64);
       du = __shfl(du, q.lead, 64);
-      const unsigned push = q.i < nc ? flood_span<kVertex>(rec, u, m, q.kb, q.cnt, du, W, off) : 0u;
+      const unsigned push = q.i < nc ? flood_span(rec, u, m, q.kb, q.cnt, du, W, off) : 0u;
       emit_neighbours(nxt, &counts[1], push, u, off);
     } else {
       for (int base = 0; base < nc; base += blockDim.x) {  // uniform: every lane reaches the append
@@ -205,7 +192,7 @@ __device__ __forceinline__ void flood_label(VoxelRec *rec, Dims d, int src, cons
           const unsigned m = rec_meta(rec, u) & kNbrBits;
           const float du = __uint_as_float((unsigned)(g_or(&rec[u].word, 1ULL) >> 32));
           for_parts<kFloodBatch>(m, [&](auto B, auto E) {
-            push |= flood_part<decltype(B)::value, decltype(E)::value, kVertex>(rec, d, u, m, du, W);
+            push |= flood_part<decltype(B)::value, decltype(E)::value>(rec, d, u, m, du, W);
           });
         }
         emit_neighbours(nxt, &counts[1], push, u, off);
@@ -287,56 +274,6 @@ __global__ void __launch_bounds__(kFloodBlock) prepare_labels(const FloodJob *__
       g_st(job.daf + v, __uint_as_float(word_dist_bits(lp_u64(&t.rec[v].word))));
       sp_u64(&t.rec[v].word, kMaxKey);
     }
-  }
-}
-
-// fix_branching=False field (batched.cu:246-251): per label, the vertex-weighted flood from the
-// root over the PDRF in the records (vw_step, the one-ulp rule) and the parent field of
-// parents_lab (batched_regular.inc:113-134): parent[v] = the same-label neighbour of smallest
-// distance, first k on ties; -1 at the root. The label's words return to ~0.
-__global__ void __launch_bounds__(kFloodBlock) prepare_parents(const FloodJob *__restrict__ jobs,
-                                                               const int2 *__restrict__ queue, int nqueue, int *next) {
-  __shared__ float W[26];
-  __shared__ int off[26];
-  __shared__ int counts[2];
-  __shared__ int item;
-  for (;;) {
-    __syncthreads();
-    if (threadIdx.x == 0) item = g_add(next, 1);
-    __syncthreads();
-    if (item >= nqueue) return;
-    const int2 q = queue[item];
-    const FloodJob job = jobs[__builtin_amdgcn_readfirstlane(q.x)];  // uniform: scalar registers
-    const int L = __builtin_amdgcn_readfirstlane(q.y);
-    const Dims d = job.d;
-    const LabelTables &t = job.t;
-    if (threadIdx.x < 26) {
-      W[threadIdx.x] = 0.0f;  // unused: vertex weights
-      off[threadIdx.x] = nb_offset(int(threadIdx.x), d);
-    }
-    __syncthreads();
-    const int s0 = g_ld(t.seg_start + L), cnt = g_ld(t.seg_count + L);
-    const int *list = t.vox + s0;
-    const int root = int(g_ld(t.root + L));
-    flood_label<true>(t.rec, d, root, W, off, job.lists_a + s0, job.lists_b + s0, counts);
-    for (int j = threadIdx.x; j < cnt; j += blockDim.x) {
-      const int v = g_ld(list + j);
-      const unsigned dv = word_dist_bits(lp_u64(&t.rec[v].word));
-      int p = -1;
-      if (v != root && dv != 0u && dv < 0x7f800000u) {
-        unsigned m = rec_meta(t.rec, v) & kNbrBits, best = 0x7f800000u;
-        while (m) {  // ascending k: the first of equal distances stays
-          const int k = __ffs(m) - 1;
-          m &= m - 1;
-          const int u = v + off[k];
-          const unsigned du = word_dist_bits(lp_u64(&t.rec[u].word));
-          if (du < best) { best = du; p = u; }
-        }
-      }
-      g_st(job.parent + v, p);
-    }
-    __syncthreads();
-    for (int j = threadIdx.x; j < cnt; j += blockDim.x) sp_u64(&t.rec[g_ld(list + j)].word, kMaxKey);
   }
 }
 
@@ -428,24 +365,6 @@ void prepare_fields(Context &ctx, const std::vector<FloodJob> &jobs, const std::
   const int grid = std::min(nq, ctx.compute_units() * 2);
   KernelTimer kt(ctx, "prepare_labels");
   prepare_labels<<<grid, kFloodBlock, 0, ctx.stream()>>>(jobs_dev.data(), queue.data(), nq, next.data());
-  BROOK_LAUNCHED();
-  ctx.synchronize();
-}
-
-void prepare_parent_field(Context &ctx, const std::vector<FloodJob> &jobs, const std::vector<LabelItem> &items) {
-  if (items.empty()) return;
-  Buffer<FloodJob> jobs_dev(ctx, jobs.size());
-  jobs_dev.upload(ctx, jobs.data(), jobs.size());
-  std::vector<int2> q(items.size());
-  for (size_t i = 0; i < items.size(); ++i) q[i] = make_int2(items[i].sample, items[i].label);
-  Buffer<int2> queue(ctx, q.size());
-  queue.upload(ctx, q.data(), q.size());
-  Buffer<int> next(ctx, 1);
-  next.clear(ctx);
-  const int nq = int(q.size());
-  const int grid = std::min(nq, ctx.compute_units() * 2);
-  KernelTimer kt(ctx, "prepare_parents");
-  prepare_parents<<<grid, kFloodBlock, 0, ctx.stream()>>>(jobs_dev.data(), queue.data(), nq, next.data());
   BROOK_LAUNCHED();
   ctx.synchronize();
 }

@@ -33,13 +33,9 @@ struct FloodJob {
   LabelTables t;
   float *daf = nullptr;
   int *lists_a = nullptr, *lists_b = nullptr;
-  int *parent = nullptr;  // fix_branching=False: the parent field (out)
 };
 // Root search and DAF floods of every item (one workgroup per item, in item order).
 void prepare_fields(Context &ctx, const std::vector<FloodJob> &jobs, const std::vector<LabelItem> &items);
-// fix_branching=False: the vertex-weighted flood from each item's root over the PDRF in the records
-// and the parent field (one workgroup per item; words ~0 in and out).
-void prepare_parent_field(Context &ctx, const std::vector<FloodJob> &jobs, const std::vector<LabelItem> &items);
 void pdrf_and_alive(Context &ctx, const void *cc, int cc_bytes, long long n, const float *dbf, const float *daf,
                     const float *M, const float *inv, const unsigned char *use, int squarings, float scale, float *pdrf,
                     VoxelRec *rec);

@@ -292,16 +292,18 @@ __device__ __forceinline__ float rail_bound(const Shared &sh) {
   return b;
 }
 
-// Returns 1 when the nearest rail is settled (sh.rail_best), 2 when no rail is reachable.
+// Returns 1 when the nearest rail is settled (sh.rail_best), 2 when no rail is reachable. A
+// `source` run never takes t itself for a rail: with no rails in the label it settles every
+// voxel reachable from t (the vertex-weighted distances of the fix_branching=False field).
 __device__ __forceinline__ int railroad(const TraceArgs &a, Shared &sh, int t, const Front &near_a, const Front &near_b,
-                                        int *far_a, int *far_b, int *touched) {
+                                        int *far_a, int *far_b, int *touched, bool source = false) {
   if (threadIdx.x == 0) {
     g_st(touched, t);
     sh.n_touched = 1;
     sh.cnt[0] = sh.cnt[1] = sh.cnt[2] = sh.n_far = sh.n_farnxt = 0;
     sh.minfar = kMinfarNone;
     float ft = rec_pdrf(a.rec, t);
-    if (ft == 0.0f) {
+    if (ft == 0.0f && !source) {
       sp_u64(&a.rec[t].word, 1ULL);  // distance 0, not queued
       sh.rail_best = (unsigned)t;
       sh.done = 1;
@@ -422,7 +424,40 @@ __device__ __forceinline__ void backtrace(const TraceArgs &a, Shared &sh, int t,
   if (lane == 0) sh.path_len = kept;
 }
 
-// ---- fix_branching=False: parent walk (one thread) -------------------------------------------
+// ---- fix_branching=False: parent field and parent walk ---------------------------------------
+// The field (batched.cu:246-251): the railroad from the root in `source` mode (the near-far
+// bands of flood_lab_nf, batched_coop.inc:125-267) gives every voxel of the label its
+// vertex-weighted distance over the PDRF; parents_lab (batched_regular.inc:113-134) then takes
+// the same-label neighbour of smallest distance, first k on ties (-1 at the root), and the words
+// go back to ~0. Returns false when the run met a voxel of PDRF 0 besides the root (it would stop
+// there as at a rail; PDRF underflows to 0 only for extreme exponents): the label is ejected.
+__device__ __forceinline__ bool parent_field(const TraceArgs &a, Shared &sh, int root, const Front &na, const Front &nb,
+                                             int *l2, int *l3, int *touched) {
+  const int status = railroad(a, sh, root, na, nb, l2, l3, touched, true);
+  const int nt = sh.n_touched;
+  for (int j = threadIdx.x; j < nt; j += blockDim.x) {
+    const int v = g_ld(touched + j);
+    const unsigned dv = word_dist_bits(lp_u64(&a.rec[v].word));
+    int p = -1;
+    if (v != root && dv != 0u && dv < 0x7f800000u) {
+      unsigned m = rec_meta(a.rec, v) & kNbrBits, best = 0x7f800000u;
+      while (m) {  // ascending k: the first of equal distances stays
+        const int k = __ffs(m) - 1;
+        m &= m - 1;
+        const int u = v + sh.off[k];
+        const unsigned du = word_dist_bits(lp_u64(&a.rec[u].word));
+        if (du < best) { best = du; p = u; }
+      }
+    }
+    g_st(a.parent + v, p);
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < nt; j += blockDim.x) sp_u64(&a.rec[g_ld(touched + j)].word, kMaxKey);
+  __syncthreads();
+  return status == 2;
+}
+
+// The walk (one thread):
 // backtrace_parent_lab (lockstep_regular.inc:166-202): from the target along the parent field,
 // keeping every vertex up to and including the first without a parent (-1 at the root, STOP -2
 // at a vertex an earlier path emitted). The walk is a pointer chase; its length is the path's.
@@ -871,7 +906,13 @@ __global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restri
     }
     const long long label_begin = wall_clock64();
     __syncthreads();
-    for (;;) {
+    bool traceable = true;
+    if (a.parent) {
+      traceable = parent_field(a, sh, int(g_ld(a.root + L)), na, nb, l2, l3, l4);
+      lap(sh, 2);
+      if (!traceable && threadIdx.x == 0) g_st(a.ejected + g_add(a.counters + 2, 1) % a.ejected_cap, L);
+    }
+    for (; traceable;) {
       const int t = next_target(a, sh, L);
       lap(sh, 1);
       if (t < 0) break;

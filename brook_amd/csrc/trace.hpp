@@ -32,9 +32,11 @@ struct TraceArgs {
   int *ejected = nullptr;
   int ejected_cap = 1;
   float wx = 1, wy = 1, wz = 1, scale = 1, constant = 0, delta = 1;
-  // fix_branching=False: the parent field (an emitted vertex's entry becomes STOP, -2); paths are
-  // parent walks instead of railroads. Null: fix_branching=True.
+  // fix_branching=False: the parent field, built per label from `root` at the label's start (an
+  // emitted vertex's entry becomes STOP, -2); paths are parent walks instead of railroads.
+  // Null: fix_branching=True.
   int *parent = nullptr;
+  const long long *root = nullptr;
 };
 
 // Traces every item (sample index into args, component id) with one workgroup per item, in item
