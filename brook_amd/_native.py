@@ -90,6 +90,8 @@ def lib():
                                                 _c.POINTER(SkeletonizeOptions), _c.POINTER(_P)]),
                 "brook_skeletons_get_view": (S, [_P, _c.POINTER(SkeletonsView)]),
                 "brook_skeletons_destroy": (None, [_P]),
+                "brook_merge_skeletons": (S, [_P, _c.POINTER(_P), _P, _c.c_size_t, _c.POINTER(_P)]),
+                "brook_merge_views": (S, [_P, _c.POINTER(SkeletonsView), _P, _c.c_size_t, _c.POINTER(_P)]),
                 "brook_prepare_fields": (S, [_P, _c.POINTER(Volume), _c.POINTER(SkeletonizeOptions), _P, _P, _P,
                                              _c.c_size_t, _P, _P, _P, _P, _P, _P]),
                 "brook_context_stat": (S, [_P, _c.c_char_p, _c.POINTER(_c.c_double)]),

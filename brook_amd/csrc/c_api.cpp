@@ -9,6 +9,7 @@
 #include "../../include/brook.h"
 #include "engine.hpp"
 #include "layout.hpp"
+#include "merge.hpp"
 
 namespace {
 
@@ -430,6 +431,74 @@ brook_status brook_skeletons_get_view(const brook_skeletons *r, brook_skeletons_
   });
 }
 void brook_skeletons_destroy(brook_skeletons *r) { delete r; }
+
+namespace {
+brook::PackedView view_of(const brook_skeletons_view &v) {
+  require(v.vertex_offsets && v.edge_offsets && (v.labels || !v.skeleton_count));
+  require((v.vertices && v.radii) || !v.vertex_count);
+  require(v.edges || !v.edge_count);
+  require(v.vertex_offsets[v.skeleton_count] == int64_t(v.vertex_count) &&
+          v.edge_offsets[v.skeleton_count] == int64_t(v.edge_count));
+  brook::PackedView p;
+  p.skeletons = v.skeleton_count;
+  p.labels = v.labels;
+  p.v_off = v.vertex_offsets;
+  p.e_off = v.edge_offsets;
+  p.vertices = v.vertices;
+  p.edges = v.edges;
+  p.radii = v.radii;
+  for (int a = 0; a < 3; ++a) p.anisotropy[a] = v.anisotropy[a];
+  return p;
+}
+brook_skeletons *merged(brook::Context &ctx, const std::vector<brook::PackedView> &parts, const float *origins) {
+  std::vector<std::array<float, 3>> at(parts.size(), std::array<float, 3>{0, 0, 0});
+  if (origins)
+    for (size_t i = 0; i < parts.size(); ++i) at[i] = {origins[3 * i], origins[3 * i + 1], origins[3 * i + 2]};
+  auto r = std::make_unique<brook_skeletons>();
+  r->packed = brook::merge_packed(ctx, parts, at);
+  return r.release();
+}
+}  // namespace
+
+brook_status brook_merge_skeletons(brook_context *ctx, const brook_skeletons *const *fragments, const float *origins,
+                                   size_t count, brook_skeletons **out) {
+  return protect([&] {
+    require(ctx && out && (fragments || !count));
+    *out = nullptr;
+    brook::ContextCall call(ctx->value);
+    std::vector<brook::PackedView> parts;
+    std::vector<brook_skeletons_view> views(count);
+    for (size_t i = 0; i < count; ++i) {
+      require(fragments[i]);
+      const brook::Packed &p = fragments[i]->packed;
+      brook_skeletons_view &v = views[i];
+      v.skeleton_count = p.labels.size();
+      v.vertex_count = p.radii.size();
+      v.edge_count = p.edges.size() / 2;
+      v.labels = p.labels.data();
+      v.vertex_offsets = reinterpret_cast<const int64_t *>(p.v_off.data());
+      v.edge_offsets = reinterpret_cast<const int64_t *>(p.e_off.data());
+      v.vertices = p.vertices.data();
+      v.edges = p.edges.data();
+      v.radii = p.radii.data();
+      std::copy(p.anisotropy.begin(), p.anisotropy.end(), v.anisotropy);
+      parts.push_back(view_of(v));
+    }
+    *out = merged(ctx->value, parts, origins);
+  });
+}
+
+brook_status brook_merge_views(brook_context *ctx, const brook_skeletons_view *fragments, const float *origins, size_t count,
+                               brook_skeletons **out) {
+  return protect([&] {
+    require(ctx && out && (fragments || !count));
+    *out = nullptr;
+    brook::ContextCall call(ctx->value);
+    std::vector<brook::PackedView> parts;
+    for (size_t i = 0; i < count; ++i) parts.push_back(view_of(fragments[i]));
+    *out = merged(ctx->value, parts, origins);
+  });
+}
 
 brook_status brook_prepare_fields(brook_context *ctx, const brook_volume *labels, const brook_skeletonize_options *options,
                                   float *dbf, float *daf, float *pdrf, size_t cap, uint8_t *traced, int64_t *root,

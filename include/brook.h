@@ -132,6 +132,17 @@ brook_status brook_skeletonize(brook_context *context, const brook_volume *label
 brook_status brook_skeletons_get_view(const brook_skeletons *result, brook_skeletons_view *out);
 void brook_skeletons_destroy(brook_skeletons *result);
 
+/* brook.h:234-237: one result from `count` fragments, fragment i offset by origins[3i..3i+2]
+ * (count x 3 float32, NULL for zero offsets): one vertex per (label, position), the first
+ * occurrence's radius, edges remapped without self-loops or duplicates, vertices on no edge
+ * dropped, labels ascending (packed_merge.cu:88-166). The result is a host-output handle. */
+brook_status brook_merge_skeletons(brook_context *context, const brook_skeletons *const *fragments,
+                                   const float *origins, size_t count, brook_skeletons **out);
+/* EXTENSION: brook_merge_skeletons over borrowed views (packed results from any source): labels and
+ * offsets in host memory; vertices, edges and radii in host memory or on the context's device. */
+brook_status brook_merge_views(brook_context *context, const brook_skeletons_view *fragments, const float *origins,
+                               size_t count, brook_skeletons **out);
+
 /* EXTENSION (the reference batches only in Python/C++, batch.hpp:20-22): skeletonize `count`
  * volumes in one call; out receives `count` result handles, each equal to a separate
  * brook_skeletonize of that volume. */

@@ -230,3 +230,5 @@ def skeletonize(labels, teasar_params=None, anisotropy=(1, 1, 1), dust_threshold
         return out
     finally:
         L.oracle_result_free(h)
+
+from .merge import merge_fragments  # noqa: E402,F401
