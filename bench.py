@@ -104,6 +104,37 @@ def parent_paths(dev, n, calls=3):
                                                               "trace_ms")})
 
 
+def merge(result, fragments=8, calls=5):
+    """Packed merge of the workload's skeletons split into `fragments` fragments (each holds every
+    vertex of every skeleton and every `fragments`-th edge: the overlap of chunked results),
+    device-resident parts: input vertices merged per second over `calls` timed calls."""
+    import torch
+
+    from brook_amd import PackedSkeletons, packed
+    h = result.to_host()
+    parts = []
+    for f in range(fragments):
+        pieces = [h.edges[h.e_off[k]:h.e_off[k + 1]][f::fragments] for k in range(len(h))]
+        e_off = np.concatenate([[0], np.cumsum([len(p) for p in pieces])]).astype(np.int64)
+        parts.append(PackedSkeletons(h.labels, h.v_off, e_off, h.vertices, np.concatenate(pieces), h.radii,
+                                     h.anisotropy).to_device("cuda"))
+    origins = [(0.0, 0.0, 0.0)] * fragments
+    merged = packed.merge_fragments(parts, origins)
+    whole = packed.merge_fragments([h], [(0.0, 0.0, 0.0)])
+    same = all(np.array_equal(getattr(merged, k), getattr(whole, k)) for k in ("vertices", "edges", "radii", "v_off"))
+    times = []
+    for _ in range(calls):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        packed.merge_fragments(parts, origins)
+        times.append(time.perf_counter() - t0)
+    nv_in = fragments * int(h.vertices.shape[0])
+    wall = float(np.median(times))
+    return dict(value=round(nv_in / wall / 1e6, 3), unit="M input vertices/s", fragments=fragments, calls=calls,
+                p50_ms=round(wall * 1e3, 3), input_vertices=nv_in, input_edges=int(h.edges.shape[0]),
+                output_vertices=int(merged.vertices.shape[0]), split_equals_whole=bool(same))
+
+
 def batched():
     """64 mixed-shape volumes (64..256 per axis, 1-50 tube labels each, ~254 M voxels in all),
     device-resident: one skeletonize_batch call against 64 plain calls."""
@@ -232,7 +263,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the host-input, fix_branching=False, batched, stress and served measurements")
+                    help="skip the host-input, fix_branching=False, merge, batched, stress and served measurements")
     ap.add_argument("--served-sweep", action="store_true", help="serve at 2, 10, 20 and 50 requests/s (default: 10)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--copy-calibration", action="store_true",
@@ -333,6 +364,7 @@ def main():
     if not args.no_extras:
         out["host_io"] = host_io(vol)
         out["fix_branching_false"] = parent_paths(dev, n)
+        out["merge"] = merge(result)
         out["batched"] = batched()
         out["stress"] = stress()
         out["served"] = served((2.0, 10.0, 20.0, 50.0) if args.served_sweep else (10.0,))

@@ -9,6 +9,7 @@
 #include <numeric>
 
 #include "fields.hpp"
+#include "soma.hpp"
 #include "trace.hpp"
 
 namespace brook {
@@ -117,10 +118,21 @@ class Sample {
       const Box &b = boxes_[id];
       int64_t ext = int64_t(b.hi[0] - b.lo[0] + 1) * (b.hi[1] - b.lo[1] + 1) * (b.hi[2] - b.lo[2] + 1);
       if (double(b.count) <= o_.dust || ext <= 1) continue;
-      if (double(dbf_max_[id]) > o_.teasar.soma_detection)
-        throw Unsupported("a component exceeds soma_detection_threshold; soma processing is not supported by this build");
       active_[id] = 1;
       labels_.push_back(int(id));
+    }
+    // somata (batched.cu:194-205): a label above the detection threshold is checked for enclosed
+    // voids; without voids it is a soma when above the acceptance threshold. A label with voids
+    // needs the crop preparation of soma.cu (filled mask, refreshed DBF, ownership of the voids).
+    soma_.assign(k_, 0);
+    bool any_soma = false;
+    for (int id : labels_) {
+      if (!(double(dbf_max_[id]) > o_.teasar.soma_detection)) continue;
+      if (enclosed_voids(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, id, boxes_[id]))
+        throw Unsupported("a component above soma_detection_threshold encloses voids; filling them (soma.cu's crop "
+                          "preparation) is not supported by this build");
+      soma_[id] = double(dbf_max_[id]) > o_.teasar.soma_acceptance;
+      any_soma |= soma_[id] != 0;
     }
     std::vector<int> seg_start(k_ + 1, 0), seg_count(k_, 0);
     for (size_t id = 1; id < k_; ++id) seg_count[id] = int(boxes_[id].count);
@@ -135,7 +147,40 @@ class Sample {
     root_.assign(k_, -1);
     for (size_t id = 1; id < k_; ++id) first[id] = cc_.roots[id - 1];
     for (int id : labels_)
-      if (!borders_[id].empty()) root_[id] = flat_of(borders_[id].back(), d_);
+      if (!soma_[id] && !borders_[id].empty()) root_[id] = flat_of(borders_[id].back(), d_);
+    if (any_soma) {
+      // the soma root (batched.cu:58-108): among the label's DBF maxima, in box-relative
+      // coordinates sorted by (x, y, z), the first of least squared distance to their centroid
+      Buffer<uint8_t> soma_dev(ctx_, k_);
+      soma_dev.upload(ctx_, soma_.data(), k_);
+      std::vector<Maximum> maxima = soma_maxima(ctx_, cc_.labels.data(), cc_.cc_bytes, n_, dbf_.data(),
+                                                reinterpret_cast<const float *>(label_max.data()), soma_dev.data());
+      std::map<int64_t, std::vector<std::pair<Point, int64_t>>> by_label;
+      for (const Maximum &m : maxima) {
+        const Box &b = boxes_[size_t(m.id)];
+        const int64_t x = m.voxel % d_.sx, y = (m.voxel / d_.sx) % d_.sy, z = m.voxel / (int64_t(d_.sx) * d_.sy);
+        by_label[m.id].push_back({Point{x - b.lo[0], y - b.lo[1], z - b.lo[2]}, m.voxel});
+      }
+      for (auto &entry : by_label) {
+        auto &pts = entry.second;
+        std::sort(pts.begin(), pts.end());
+        int64_t sum[3] = {0, 0, 0};
+        for (auto &p : pts) for (int a = 0; a < 3; ++a) sum[a] += p.first[a];
+        float center[3];
+        for (int a = 0; a < 3; ++a) center[a] = float(double(sum[a]) / double(pts.size()));
+        double best = INFINITY;
+        for (auto &p : pts) {
+          double cost = 0;
+          for (int a = 0; a < 3; ++a) { const double dd = double(p.first[a]) - center[a]; cost += dd * dd; }
+          if (cost < best) { best = cost; root_[size_t(entry.first)] = p.second; }
+        }
+      }
+      std::vector<float> radius(k_, 0.0f);
+      for (int id : labels_)
+        if (soma_[id]) radius[id] = dbf_max_[id];
+      free_radius_dev_ = Buffer<float>(ctx_, k_);
+      free_radius_dev_.upload(ctx_, radius.data(), k_);
+    }
     flood_voxels_ = 0;
     for (int id : labels_) flood_voxels_ += boxes_[id].count * (root_[id] < 0 ? 2 : 1);
     first_dev_ = Buffer<long long>(ctx_, k_);
@@ -166,6 +211,7 @@ class Sample {
     j.t.target = target_dev_.data();
     j.t.max_daf = max_daf_dev_.data();
     j.t.weights = weights_.data();
+    j.t.free_radius = free_radius_dev_.size() ? free_radius_dev_.data() : nullptr;
     j.t.rec = rec_.data();
     j.daf = daf_.data();
     j.lists_a = lists_[0].data();
@@ -177,6 +223,11 @@ class Sample {
   void items(int s, std::vector<LabelItem> &q) const {
     if (done_) return;
     for (int id : labels_) q.push_back(LabelItem{s, id, boxes_[id].count});
+  }
+  void soma_items(int s, std::vector<LabelItem> &q) const {
+    if (done_ || soma_.empty()) return;
+    for (int id : labels_)
+      if (soma_[id]) q.push_back(LabelItem{s, id, boxes_[id].count});
   }
 
   void after_floods(Lap &lap) {
@@ -231,9 +282,10 @@ class Sample {
     for (int id : labels_) {
       std::vector<Point> manual = borders_[id];
       if (!manual.empty()) manual.pop_back();
+      if (soma_[id] && !borders_[id].empty()) manual.insert(manual.begin(), borders_[id].back());
       manual.insert(manual.end(), before_[id].begin(), before_[id].end());
       mstart[id] = int(mflat.size());
-      if (manual.empty()) mflat.push_back(int(target_[id]));
+      if (manual.empty() && !soma_[id]) mflat.push_back(int(target_[id]));
       else for (auto it = manual.rbegin(); it != manual.rend(); ++it) mflat.push_back(int(flat_of(*it, d_)));
       mcount[id] = int(mflat.size()) - mstart[id];
       int64_t budget = o_.teasar.max_paths >= 0 ? o_.teasar.max_paths : boxes_[id].count;
@@ -262,6 +314,17 @@ class Sample {
     counters_.clear(ctx_);
     label_stats_ = Buffer<unsigned long long>(ctx_, size_t(kTraceProfileSlots) * k_);
     label_stats_.clear(ctx_);
+    if (free_radius_dev_.size()) {  // the somata's trimming radii and balls (lockstep.cu:137-141)
+      std::vector<SomaLabel> table(k_);
+      for (int id : labels_)
+        if (soma_[id]) {
+          table[id].radius = double(dbf_max_[id]) * o_.teasar.soma_scale + o_.teasar.soma_constant;
+          table[id].ball = Ball{float(o_.teasar.soma_scale), float(o_.teasar.soma_constant)};
+          table[id].max_paths = o_.teasar.max_paths;
+        }
+      soma_dev_ = Buffer<SomaLabel>(ctx_, k_);
+      soma_dev_.upload(ctx_, table.data(), k_);
+    }
   }
 
   TraceArgs trace_args() const {
@@ -292,11 +355,11 @@ class Sample {
     ta.wx = o_.aniso[0];
     ta.wy = o_.aniso[1];
     ta.wz = o_.aniso[2];
-    ta.scale = float(o_.teasar.scale);
-    ta.constant = float(o_.teasar.constant);
+    ta.ball = Ball{float(o_.teasar.scale), float(o_.teasar.constant)};
     ta.delta = std::min({o_.aniso[0], o_.aniso[1], o_.aniso[2]});
     ta.parent = o_.fix_branching ? nullptr : parent_.data();
     ta.root = root_dev_.data();
+    if (soma_dev_.size()) ta.soma = soma_dev_.data();
     return ta;
   }
 
@@ -433,6 +496,9 @@ class Sample {
   std::vector<Buffer<int>> lists_;
   Buffer<int> order_;
   Buffer<int> parent_;  // fix_branching=False: parent field
+  std::vector<uint8_t> soma_;       // per id: a soma
+  Buffer<float> free_radius_dev_;   // per id: the soma's free-space radius (0: not a soma); empty: no somata
+  Buffer<SomaLabel> soma_dev_;      // per id: soma parameters; empty: no somata
   Buffer<unsigned long long> snap_a_, snap_b_;
   Buffer<int> mstart_dev_, mcount_dev_, maxp_dev_, mflat_dev_;
   long long store_cap_ = 0;
@@ -461,7 +527,9 @@ void run_group(Context &ctx, std::vector<std::unique_ptr<Sample>> &group, Lap &l
   std::stable_sort(items.begin(), items.end(), larger_first);
   std::vector<TraceArgs> args;
   for (auto &s : group) args.push_back(s->done() ? TraceArgs{} : s->trace_args());
-  trace_all(ctx, args, items);
+  std::vector<LabelItem> somata;
+  for (size_t i = 0; i < group.size(); ++i) group[i]->soma_items(int(i), somata);
+  trace_all(ctx, args, items, somata);
   lap("trace_ms");
   const bool profile = group.size() == 1;
   for (auto &s : group) s->finish(lap, profile);

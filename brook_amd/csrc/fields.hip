@@ -155,15 +155,9 @@ __device__ __forceinline__ unsigned flood_span(VoxelRec *rec, int u, unsigned m,
 // ~0 on entry; on exit they hold the distances, every voxel dequeued (bit 0 set).
 // Label-correcting rounds: a voxel is dequeued by one atomic that also returns its distance, and
 // an improvement that finds the voxel not queued queues it, so no update is ever lost.
-__device__ __forceinline__ void flood_label(VoxelRec *rec, Dims d, int src, const float *W, const int *off, int *cur, int *nxt,
-                            int *counts /*LDS [2]*/) {
-  if (threadIdx.x == 0) {
-    sp_u64(&rec[src].word, 0ULL);  // distance 0, queued
-    g_st(cur, src);
-    counts[0] = 1;
-    counts[1] = 0;
-  }
-  __syncthreads();
+// The front's words hold their start distances (queued) on entry; counts[0] is its length.
+__device__ __forceinline__ void flood_rounds(VoxelRec *rec, Dims d, const float *W, const int *off, int *cur, int *nxt,
+                                             int *counts /*LDS [2]*/) {
   for (;;) {
     const int nc = counts[0];
     if (nc == 0) break;
@@ -203,6 +197,54 @@ __device__ __forceinline__ void flood_label(VoxelRec *rec, Dims d, int src, cons
     int *t = cur; cur = nxt; nxt = t;
     __syncthreads();
   }
+}
+
+__device__ __forceinline__ void flood_label(VoxelRec *rec, Dims d, int src, const float *W, const int *off, int *cur, int *nxt,
+                                            int *counts /*LDS [2]*/) {
+  if (threadIdx.x == 0) {
+    sp_u64(&rec[src].word, 0ULL);  // distance 0, queued
+    g_st(cur, src);
+    counts[0] = 1;
+    counts[1] = 0;
+  }
+  __syncthreads();
+  flood_rounds(rec, d, W, off, cur, nxt, counts);
+}
+
+// The free-space length of a soma voxel at offset (dxi, dyi, dzi) from the root, when its physical
+// distance is within `radius`: the shortest 26-connected path through empty space, in float32 in
+// the order of freespace_seed_lab (batched_regular.inc:51-73).
+__device__ __forceinline__ bool free_space(int dxi, int dyi, int dzi, float wx, float wy, float wz, float radius, float &val) {
+  const float rdx = wx * (float)dxi, rdy = wy * (float)dyi, rdz = wz * (float)dzi;
+  if (rdx * rdx + rdy * rdy + rdz * rdz > radius * radius) return false;
+  const float dx = fabsf((float)dxi), dy = fabsf((float)dyi), dz = fabsf((float)dzi);
+  const float dxyz = fminf(fminf(dx, dy), dz);
+  const float dxy = fminf(dx, dy), dyz = fminf(dy, dz), dxz = fminf(dx, dz);
+  val = dxyz * sqrtf(wx * wx + wy * wy + wz * wz) + wx * (dx - dxyz) + wy * (dy - dxyz) + wz * (dz - dxyz) +
+        (dxy - dxyz) * (sqrtf(wx * wx + wy * wy) - wx - wy) + (dxz - dxyz) * (sqrtf(wx * wx + wz * wz) - wx - wz) +
+        (dyz - dxyz) * (sqrtf(wy * wy + wz * wz) - wy - wz);
+  return true;
+}
+
+// A soma's DAF flood (batched.cu:222-231): every voxel of the label inside the free-space ball
+// starts at its free-space length (the root at 0), then the label-correcting rounds.
+__device__ __forceinline__ void flood_free_space(VoxelRec *rec, Dims d, const int *list, int cnt, int root, float radius,
+                                                 const float *W, const int *off, int *cur, int *nxt, int *counts) {
+  if (threadIdx.x == 0) { counts[0] = 0; counts[1] = 0; }
+  __syncthreads();
+  int rx, ry, rz;
+  unravel(root, d, rx, ry, rz);
+  for (int j = threadIdx.x; j < cnt; j += blockDim.x) {
+    const int v = g_ld(list + j);
+    int x, y, z;
+    unravel(v, d, x, y, z);
+    float val;
+    if (!free_space(x - rx, y - ry, z - rz, W[0], W[2], W[4], radius, val)) continue;
+    sp_u64(&rec[v].word, (unsigned long long)__float_as_uint(val) << 32);  // queued
+    g_st(cur + wave_slot(&counts[0]), v);
+  }
+  __syncthreads();
+  flood_rounds(rec, d, W, off, cur, nxt, counts);
 }
 
 // Argmax of the flooded distance over the label's voxel list: largest finite value, ties -> lowest
@@ -263,7 +305,9 @@ __global__ void __launch_bounds__(kFloodBlock) prepare_labels(const FloodJob *__
       __syncthreads();
       if (threadIdx.x == 0) g_st(t.root + L, root);
     }
-    flood_label(t.rec, d, int(root), W, off, a, b, counts);
+    const float radius = t.free_radius ? g_ld(t.free_radius + L) : 0.0f;
+    if (radius > 0.0f) flood_free_space(t.rec, d, list, cnt, int(root), radius, W, off, a, b, counts);
+    else flood_label(t.rec, d, int(root), W, off, a, b, counts);
     unsigned long long key = label_argmax(t.rec, list, cnt, &best);
     if (threadIdx.x == 0) {
       g_st(t.target + L, key ? (long long)(0xFFFFFFFFu - (unsigned)key) : -1LL);

@@ -16,6 +16,9 @@ struct LabelTables {
   long long *target = nullptr;     // out: DAF argmax (lowest flat index on ties)
   float *max_daf = nullptr;        // out: DAF maximum
   const float *weights = nullptr;  // 26 geometric step weights
+  // > 0: the id is a soma; its DAF flood starts from the free-space ball of this radius around
+  // the root (null: no somata)
+  const float *free_radius = nullptr;
   VoxelRec *rec = nullptr;         // voxel records (masks in; words ~0 in and out)
 };
 

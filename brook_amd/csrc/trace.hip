@@ -595,7 +595,7 @@ __device__ __forceinline__ void offer_span(const TraceArgs &a, const int *off, i
 // One invalidation round over a front of at most kGroupFront entries, a group of lanes per entry.
 // The group's first lane takes the entry's snapshot (re-arming it) and its ball radius before the
 // barrier that separates the round's snapshots from its offers; the values stay in registers.
-__device__ __forceinline__ void offer_groups(const TraceArgs &a, Shared &sh, const Front &cur, int nc, unsigned hi,
+__device__ __forceinline__ void offer_groups(const TraceArgs &a, const Ball &ball, Shared &sh, const Front &cur, int nc, unsigned hi,
                                              const Front &nxt, int *n_nxt, int *far, int *touched) {
   const Group q = group_of(nc);
   int u = 0;
@@ -607,7 +607,7 @@ __device__ __forceinline__ void offer_groups(const TraceArgs &a, Shared &sh, con
     m = rec_meta(a.rec, u) & kNbrBits;
     if (q.g == 0) {
       ku = g_and(&a.rec[u].word, ~1ULL);
-      maxd = a.scale * g_ld(a.dbf + claim_source(ku)) + a.constant;
+      maxd = ball.scale * g_ld(a.dbf + claim_source(ku)) + ball.constant;
     }
   }
   __syncthreads();
@@ -640,7 +640,7 @@ __device__ __forceinline__ void offer_all(const TraceArgs &a, const int *off, in
 // smallest (max(bottleneck, distance to the neighbour's source), source) among those whose source's
 // ball holds c. Claim loads, then the sources' DBF loads, each issued together.
 template <int B, int E>
-__device__ __forceinline__ void pull_part(const TraceArgs &a, int c, unsigned mc, int x, int y, int z,
+__device__ __forceinline__ void pull_part(const TraceArgs &a, const Ball &ball, int c, unsigned mc, int x, int y, int z,
                                           unsigned long long &best) {
   unsigned long long ku[E - B];
   for_nb<B, E>([&](auto K) {
@@ -659,7 +659,7 @@ __device__ __forceinline__ void pull_part(const TraceArgs &a, int c, unsigned mc
     int qx, qy, qz;
     unravel(int(su), a.d, qx, qy, qz);
     const float d = offer_distance(a, x, y, z, qx, qy, qz);
-    if (d >= a.scale * r[i] + a.constant) return;
+    if (d >= ball.scale * r[i] + ball.constant) return;
     const unsigned mb = max(__float_as_uint(d), (unsigned)(ku[i] >> 32));
     const unsigned long long cand = ((unsigned long long)mb << 32) | su;
     if (cand < best) best = cand;
@@ -668,7 +668,7 @@ __device__ __forceinline__ void pull_part(const TraceArgs &a, int c, unsigned mc
 
 // Neighbours kb..kb+cnt-1 (those set in mc; cnt <= kSpan) of waiting voxel c: pull_part with a
 // run-time span.
-__device__ __forceinline__ void pull_span(const TraceArgs &a, const int *off, int c, unsigned mc, int kb, int cnt, int x,
+__device__ __forceinline__ void pull_span(const TraceArgs &a, const Ball &ball, const int *off, int c, unsigned mc, int kb, int cnt, int x,
                                           int y, int z, unsigned long long &best) {
   constexpr int S = kSpan;
   unsigned long long ku[S];
@@ -687,7 +687,7 @@ __device__ __forceinline__ void pull_span(const TraceArgs &a, const int *off, in
     int qx, qy, qz;
     unravel(int(su), a.d, qx, qy, qz);
     const float d = offer_distance(a, x, y, z, qx, qy, qz);
-    if (d >= a.scale * r[j] + a.constant) continue;
+    if (d >= ball.scale * r[j] + ball.constant) continue;
     const unsigned mb = max(__float_as_uint(d), (unsigned)(ku[j] >> 32));
     const unsigned long long cand = ((unsigned long long)mb << 32) | su;
     if (cand < best) best = cand;
@@ -697,7 +697,8 @@ __device__ __forceinline__ void pull_span(const TraceArgs &a, const int *off, in
 // Band-boundary pulls of at most kGroupFront waiting entries, a group of lanes per entry, the
 // group's minimum folded by shuffles: pull[i] = kClaimed (claimed meanwhile), the best claim, or
 // kMaxKey.
-__device__ __forceinline__ void pull_groups(const TraceArgs &a, Shared &sh, const int *far, int nf, unsigned long long *pull) {
+__device__ __forceinline__ void pull_groups(const TraceArgs &a, const Ball &ball, Shared &sh, const int *far, int nf,
+                                            unsigned long long *pull) {
   const Group q = group_of(nf);
   unsigned long long best = kMaxKey;
   bool claimed = false;
@@ -709,7 +710,7 @@ __device__ __forceinline__ void pull_groups(const TraceArgs &a, Shared &sh, cons
     if (!claimed) {
       int x, y, z;
       unravel(c, a.d, x, y, z);
-      pull_span(a, sh.off, c, mc, q.kb, q.cnt, x, y, z, best);
+      pull_span(a, ball, sh.off, c, mc, q.kb, q.cnt, x, y, z, best);
     }
   }
   for (int o = 1; o < q.G; o <<= 1) {
@@ -730,7 +731,8 @@ __device__ __forceinline__ int band_of(unsigned mbits, float delta, int kmin) {
   return k < kmin ? kmin : k;
 }
 
-__device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const int *path, int len, const Front &la,
+__device__ __forceinline__ void invalidate(const TraceArgs &a, const Ball &ball, Shared &sh, const int *path, int len,
+                                           const Front &la,
                                            const Front &lb, int *lfar, int *lkeep, int *touched, unsigned long long *snap,
                                            unsigned long long *pull) {
   if (threadIdx.x == 0) {
@@ -758,7 +760,7 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const
       const int rn = rot_next(r);
       if (threadIdx.x == 0) { sh.cnt[rot_prev(r)] = 0; sh.prof[6] += 1; sh.prof[11] += nc; }
       if (nc <= kGroupFront) {
-        offer_groups(a, sh, cur, nc, hi, nxt, &sh.cnt[rn], far, touched);
+        offer_groups(a, ball, sh, cur, nc, hi, nxt, &sh.cnt[rn], far, touched);
       } else {
         for (int i = threadIdx.x; i < nc; i += blockDim.x)  // snapshot and re-arm in one atomic
           g_st(snap + i, g_and(&a.rec[front_get(cur, i)].word, ~1ULL));
@@ -772,7 +774,7 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const
             const unsigned m = (rec_meta(a.rec, u) & kNbrBits);
             const unsigned long long ku = g_ld(snap + i);
             const unsigned su = claim_source(ku), mu = (unsigned)(ku >> 32);
-            offer_all(a, sh.off, u, m, su, mu, a.scale * a.dbf[su] + a.constant, hi, p);
+            offer_all(a, sh.off, u, m, su, mu, ball.scale * a.dbf[su] + ball.constant, hi, p);
           }
           emit_neighbours(touched, &sh.n_touched, p.touched, u, sh.off);
           emit_front(nxt, &sh.cnt[rn], p.near, u, sh.off);
@@ -788,7 +790,7 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const
     if (threadIdx.x == 0) { sh.lowest = 0x7f800000u; sh.prof[12] += nf; sh.prof[13] += 1; }
     __syncthreads();
     if (nf <= kGroupFront) {  // band boundary: exact pulls
-      pull_groups(a, sh, far, nf, pull);
+      pull_groups(a, ball, sh, far, nf, pull);
     } else {
       for (int i = threadIdx.x; i < nf; i += blockDim.x) {
         int c = lp_i(far + i);
@@ -797,7 +799,7 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, Shared &sh, const
         int x, y, z;
         unravel(c, a.d, x, y, z);
         unsigned long long best = kMaxKey;
-        for_parts<kNbBatch>(mc, [&](auto B, auto E) { pull_part<decltype(B)::value, decltype(E)::value>(a, c, mc, x, y, z, best); });
+        for_parts<kNbBatch>(mc, [&](auto B, auto E) { pull_part<decltype(B)::value, decltype(E)::value>(a, ball, c, mc, x, y, z, best); });
         g_st(pull + i, best);
         if (best != kMaxKey) atomicMin(&sh.lowest, (unsigned)(best >> 32));
       }
@@ -879,6 +881,67 @@ __device__ __forceinline__ int next_target(const TraceArgs &a, Shared &sh, int L
   }
 }
 
+// ---- somata ------------------------------------------------------------------------------------
+// Physical distance from the soma root above the soma radius, in float64 as numpy's
+// norm(anisotropy_f32 * (vertex - root)) (lockstep_regular.inc:35-45).
+__device__ __forceinline__ bool soma_far(const TraceArgs &a, int v, int rx, int ry, int rz, double radius) {
+  int x, y, z;
+  unravel(v, a.d, x, y, z);
+  const double dx = (double)a.wx * ((double)x - (double)rx), dy = (double)a.wy * ((double)y - (double)ry),
+               dz = (double)a.wz * ((double)z - (double)rz);
+  double ss = dx * dx + dy * dy;
+  ss = ss + dz * dz;
+  return sqrt(ss) > radius;
+}
+
+// One thread: a soma's walk compacted in place to the vertices Kimimaro keeps
+// (lockstep_regular.inc:128-147, 184-195). A railroad walk runs rail end -> target: the target
+// and the far vertices stay, and a rail end within the radius is replaced by the root. A parent
+// walk runs target -> root: the far vertices and the root stay. Every write lands at or before
+// the entry just read, except that a walk of one vertex gets two entries (a label has more voxels).
+__device__ __forceinline__ void soma_trim(const TraceArgs &a, Shared &sh, int *walk, int root, double radius) {
+  int rx, ry, rz;
+  unravel(root, a.d, rx, ry, rz);
+  const int n = sh.path_len;
+  int kept = 0;
+  for (int j = 0; j < n; ++j) {
+    const int v = g_ld(walk + j);
+    const bool far = soma_far(a, v, rx, ry, rz, radius);
+    if (!a.parent && j == 0 && !far) g_st(walk + kept++, root);
+    if (far || j == n - 1) g_st(walk + kept++, v);
+  }
+  sh.path_len = kept;
+}
+
+// The somata's own balls (lockstep.cu:197-206), one workgroup per soma label, before the trace:
+// the ball of the soma parameters around the root dies, and the label's path budget becomes its
+// voxel count minus what the ball took (or max_paths), none when the manual targets reach it.
+__global__ void __launch_bounds__(kBlock) soma_balls(const TraceArgs *__restrict__ all, const int2 *__restrict__ queue,
+                                                    int nqueue) {
+  __shared__ Shared sh;
+  const int2 q = queue[blockIdx.x];
+  const TraceArgs a = all[__builtin_amdgcn_readfirstlane(q.x)];
+  const int L = __builtin_amdgcn_readfirstlane(q.y);
+  if (threadIdx.x < 26) sh.off[threadIdx.x] = nb_offset(int(threadIdx.x), a.d);
+  const int seg = g_ld(a.seg_start + L), count = g_ld(a.seg_count + L);
+  int *l2 = a.lists[2] + seg, *l3 = a.lists[3] + seg, *l4 = a.lists[4] + seg, *walk = a.lists[5] + seg;
+  const Front na{a.lists[0] + seg, sh.front[0]}, nb{a.lists[1] + seg, sh.front[1]};
+  unsigned long long *s0 = a.snaps[0] + seg, *s1 = a.snaps[1] + seg;
+  const SomaLabel *soma = a.soma + L;
+  if (threadIdx.x == 0) {
+    for (int j = 0; j < kProf; ++j) sh.prof[j] = 0;
+    g_st(walk, int(g_ld(a.root + L)));
+  }
+  __syncthreads();
+  invalidate(a, soma->ball, sh, walk, 1, na, nb, l2, l3, l4, s0, s1);
+  if (threadIdx.x == 0) {
+    const long long limit = g_ld(&soma->max_paths);
+    const long long budget = limit >= 0 ? limit : (long long)count - sh.n_touched;
+    const int manual = g_ld(a.mcount + L);
+    g_st(a.maxp + L, manual >= budget ? 0 : int(budget < INT32_MAX ? budget : INT32_MAX));
+  }
+}
+
 __global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restrict__ all, const int2 *__restrict__ queue,
                                                       int nqueue, int *next) {
   __shared__ Shared sh;
@@ -912,6 +975,8 @@ __global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restri
       lap(sh, 2);
       if (!traceable && threadIdx.x == 0) g_st(a.ejected + g_add(a.counters + 2, 1) % a.ejected_cap, L);
     }
+    // A soma's paths keep only the vertices beyond its radius (its ball died in soma_balls).
+    const bool soma = a.soma && g_ld(&a.soma[L].radius) >= 0.0;
     for (; traceable;) {
       const int t = next_target(a, sh, L);
       lap(sh, 1);
@@ -936,6 +1001,8 @@ __global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restri
         __syncthreads();
         break;
       }
+      if (soma && threadIdx.x == 0) soma_trim(a, sh, walk, int(g_ld(a.root + L)), g_ld(&a.soma[L].radius));
+      __syncthreads();
       const int len = sh.path_len;
       if (threadIdx.x == 0) {
         g_add(reinterpret_cast<unsigned long long *>(a.counters + 4), (unsigned long long)nt);
@@ -956,11 +1023,13 @@ __global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restri
       for (int j = threadIdx.x; j < len; j += blockDim.x) {
         int v = g_ld(walk + j);
         if (pos >= 0) g_st(a.store + pos + (len - 1 - j), v);
-        if (a.parent) g_st(a.parent + v, -2);  // STOP: later walks end on this vertex
+        if (a.parent) {
+          if (!soma) g_st(a.parent + v, -2);  // STOP: later walks end on this vertex (not for somata)
+        }
         else sp_f(&a.rec[v].pdrf, 0.0f);       // the path becomes rail
       }
       __syncthreads();
-      invalidate(a, sh, walk, len, na, nb, l2, l3, l4, s0, s1);
+      invalidate(a, a.ball, sh, walk, len, na, nb, l2, l3, l4, s0, s1);
       lap(sh, 4);
       if (threadIdx.x == 0) { sh.pcount += 1; sh.prof[7] += 1; }
       __syncthreads();
@@ -974,7 +1043,8 @@ __global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restri
 
 }  // namespace
 
-void trace_all(Context &ctx, const std::vector<TraceArgs> &args, const std::vector<LabelItem> &items) {
+void trace_all(Context &ctx, const std::vector<TraceArgs> &args, const std::vector<LabelItem> &items,
+               const std::vector<LabelItem> &somata) {
   if (items.empty()) return;
   Buffer<TraceArgs> args_dev(ctx, args.size());
   args_dev.upload(ctx, args.data(), args.size());
@@ -986,6 +1056,15 @@ void trace_all(Context &ctx, const std::vector<TraceArgs> &args, const std::vect
   next.clear(ctx);
   const int nq = int(q.size());
   const int grid = std::min(nq, ctx.compute_units() * 2);
+  if (!somata.empty()) {
+    std::vector<int2> sq(somata.size());
+    for (size_t i = 0; i < somata.size(); ++i) sq[i] = make_int2(somata[i].sample, somata[i].label);
+    Buffer<int2> soma_queue(ctx, sq.size());
+    soma_queue.upload(ctx, sq.data(), sq.size());
+    KernelTimer kt(ctx, "soma_balls");
+    soma_balls<<<unsigned(sq.size()), kBlock, 0, ctx.stream()>>>(args_dev.data(), soma_queue.data(), int(sq.size()));
+    BROOK_LAUNCHED();
+  }
   {
     KernelTimer kt(ctx, "trace_labels");
     trace_labels<<<grid, kBlock, 0, ctx.stream()>>>(args_dev.data(), queue.data(), nq, next.data());

@@ -8,13 +8,19 @@ namespace brook {
 
 constexpr int kTraceProfileSlots = 16;
 
+// The invalidation ball of a source s: radius scale * DBF[s] + constant.
+struct Ball {
+  float scale = 1.0f, constant = 0.0f;
+};
+
 struct TraceArgs {
   Dims d{0, 0, 0};
   VoxelRec *rec = nullptr;  // voxel records: PDRF (rails become 0 in place), flags, words (~0 between uses)
   const float *dbf = nullptr;
   const int *seg_start = nullptr, *seg_count = nullptr;
   const int *order = nullptr;  // per-label target order (voxels) in segment layout
-  const int *mstart = nullptr, *mcount = nullptr, *mflat = nullptr, *maxp = nullptr;
+  const int *mstart = nullptr, *mcount = nullptr, *mflat = nullptr;
+  int *maxp = nullptr;  // path budgets (a soma's is set on the device, after its ball)
   int *lists[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // int scratch, segment layout
   unsigned long long *snaps[2] = {nullptr, nullptr};                       // u64 scratch, segment layout
   int *store = nullptr;
@@ -31,16 +37,28 @@ struct TraceArgs {
   unsigned long long *label_stats = nullptr;
   int *ejected = nullptr;
   int ejected_cap = 1;
-  float wx = 1, wy = 1, wz = 1, scale = 1, constant = 0, delta = 1;
+  float wx = 1, wy = 1, wz = 1, delta = 1;
+  Ball ball;  // the paths' invalidation ball (TEASAR scale and constant)
   // fix_branching=False: the parent field, built per label from `root` at the label's start (an
   // emitted vertex's entry becomes STOP, -2); paths are parent walks instead of railroads.
   // Null: fix_branching=True.
   int *parent = nullptr;
-  const long long *root = nullptr;
+  const long long *root = nullptr;  // per label (both modes)
+  // Somata, per label (null: none in the sample).
+  const struct SomaLabel *soma = nullptr;
+};
+
+// A label's soma parameters: the trimming radius (< 0: not a soma), the soma's own ball and the
+// path budget (< 0: the label's voxels minus those the soma ball took).
+struct SomaLabel {
+  double radius = -1.0;
+  Ball ball;
+  long long max_paths = -1;
 };
 
 // Traces every item (sample index into args, component id) with one workgroup per item, in item
-// order, all in one launch.
-void trace_all(Context &ctx, const std::vector<TraceArgs> &args, const std::vector<LabelItem> &items);
+// order, all in one launch; the soma labels among them (`somata`) lose their soma balls first.
+void trace_all(Context &ctx, const std::vector<TraceArgs> &args, const std::vector<LabelItem> &items,
+               const std::vector<LabelItem> &somata = {});
 
 }  // namespace brook

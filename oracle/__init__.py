@@ -56,7 +56,7 @@ def lib():
         L.oracle_railroad.restype = _I64
         L.oracle_railroad.argtypes = [_P, _P, _I64, _I64, _I64, _I64, _P, _I64]
         L.oracle_prepare.restype = _I64
-        L.oracle_prepare.argtypes = [_P, _I64, _I64, _I64, _P, _P, _D, _I, _I, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P]
+        L.oracle_prepare.argtypes = [_P, _I64, _I64, _I64, _P, _P, _D, _I, _I, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P]
         L.oracle_vw_parents.argtypes = [_P, _I64, _I64, _I64, _P, _P, _I64, _P, _P]
         L.oracle_skeletonize.restype = _P
         L.oracle_skeletonize.argtypes = [_P, _I64, _I64, _I64, _P, _I64, _P, _D, _I, _I, _P, _I64, _I]
@@ -175,16 +175,17 @@ def prepare(labels, teasar_params=None, anisotropy=(1, 1, 1), dust_threshold=100
     active = np.zeros(cap, np.uint8)
     dbf_max, max_daf = np.zeros(cap, np.float32), np.zeros(cap, np.float32)
     root, target = np.zeros(cap, np.int64), np.zeros(cap, np.int64)
+    soma = np.zeros(cap, np.uint8)
     pv = params_vector(teasar_params)
     an = np.asarray(anisotropy, np.float32)
     count = lib().oracle_prepare(_ptr(a), *shape, _ptr(pv), _ptr(an), float(dust_threshold), int(fix_borders), threads,
                                  _ptr(cc), _ptr(dbf), _ptr(daf), _ptr(pdrf), cap, _ptr(active), _ptr(dbf_max),
-                                 _ptr(max_daf), _ptr(root), _ptr(target))
+                                 _ptr(max_daf), _ptr(root), _ptr(target), _ptr(soma))
     if count < 0:
         _fail("prepare")
     k = count + 1
     return dict(cc=cc, dbf=dbf, daf=daf, pdrf=pdrf, count=count, active=active[:k].copy(), dbf_max=dbf_max[:k].copy(),
-                max_daf=max_daf[:k].copy(), root=root[:k].copy(), target=target[:k].copy())
+                max_daf=max_daf[:k].copy(), root=root[:k].copy(), target=target[:k].copy(), soma=soma[:k].copy())
 
 
 def vw_parents(labels, vw, sources):

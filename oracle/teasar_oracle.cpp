@@ -459,6 +459,104 @@ void geometric_flood(const std::vector<uint32_t> &cc, const Shape &s, const floa
   }
 }
 
+// The same fixpoint from sources with initial values (BatchedFlooder::flood seeded,
+// batched.cu:122-148): every finite seed starts the flood with its value.
+void geometric_flood_seeded(const std::vector<uint32_t> &cc, const Shape &s, const float W[26],
+                            const std::vector<std::pair<i64, float>> &seeds, std::vector<float> &dist) {
+  MinHeap heap;
+  for (auto &sd : seeds) { dist[sd.first] = sd.second; heap.push({sd.second, sd.first}); }
+  while (!heap.empty()) {
+    HeapItem it = heap.top(); heap.pop();
+    if (it.d > dist[it.v]) continue;
+    i64 x, y, z; s.unravel(it.v, x, y, z);
+    uint32_t l = cc[it.v];
+    for (int k = 0; k < 26; ++k) {
+      i64 nx = x + NX[k], ny = y + NY[k], nz = z + NZ[k];
+      if (!s.inside(nx, ny, nz)) continue;
+      i64 u = s.at(nx, ny, nz);
+      if (cc[u] != l) continue;
+      float c = it.d + W[k];
+      if (c < dist[u]) { dist[u] = c; heap.push({c, u}); }
+    }
+  }
+}
+
+// Free-space seed of a soma's DAF (freespace_seed_lab, batched_regular.inc:51-73): a voxel of the
+// label within `radius` (physical, float32) of the root starts at the closed-form length of the
+// shortest 26-connected free-space path to the root, evaluated in float32 in this order.
+static bool freespace_seed(i64 dx_i, i64 dy_i, i64 dz_i, float wx, float wy, float wz, float radius, float &val) {
+  float rdx = wx * (float)dx_i, rdy = wy * (float)dy_i, rdz = wz * (float)dz_i;
+  if (rdx * rdx + rdy * rdy + rdz * rdz > radius * radius) return false;
+  float dx = std::fabs((float)dx_i), dy = std::fabs((float)dy_i), dz = std::fabs((float)dz_i);
+  float dxyz = std::min(std::min(dx, dy), dz);
+  float dxy = std::min(dx, dy), dyz = std::min(dy, dz), dxz = std::min(dx, dz);
+  val = dxyz * std::sqrt(wx * wx + wy * wy + wz * wz)
+        + wx * (dx - dxyz) + wy * (dy - dxyz) + wz * (dz - dxyz)
+        + (dxy - dxyz) * (std::sqrt(wx * wx + wy * wy) - wx - wy)
+        + (dxz - dxyz) * (std::sqrt(wx * wx + wz * wz) - wx - wz)
+        + (dyz - dxyz) * (std::sqrt(wy * wy + wz * wz) - wy - wz);
+  return true;
+}
+
+// fill_voids' count on a label's bounding box (fillvoids_kernels.cuh, fillvoids_flood.cuh): the
+// voxels of the box that are not the label and are not 6-connected through such voxels to a face
+// of the box.
+i64 void_count(const std::vector<uint32_t> &cc, const Shape &s, uint32_t id, const Box &b) {
+  Shape c{b.hi[0] - b.lo[0] + 1, b.hi[1] - b.lo[1] + 1, b.hi[2] - b.lo[2] + 1};
+  auto fg = [&](i64 x, i64 y, i64 z) { return cc[s.at(x + b.lo[0], y + b.lo[1], z + b.lo[2])] == id; };
+  std::vector<uint8_t> seen(size_t(c.n()), 0);
+  std::vector<i64> stack;
+  for (i64 z = 0; z < c.sz; ++z)
+    for (i64 y = 0; y < c.sy; ++y)
+      for (i64 x = 0; x < c.sx; ++x)
+        if ((x == 0 || y == 0 || z == 0 || x == c.sx - 1 || y == c.sy - 1 || z == c.sz - 1) && !fg(x, y, z)) {
+          seen[c.at(x, y, z)] = 1;
+          stack.push_back(c.at(x, y, z));
+        }
+  while (!stack.empty()) {
+    i64 v = stack.back(); stack.pop_back();
+    i64 x, y, z; c.unravel(v, x, y, z);
+    for (int k = 0; k < 6; ++k) {
+      i64 nx = x + NX[k], ny = y + NY[k], nz = z + NZ[k];
+      if (!c.inside(nx, ny, nz) || fg(nx, ny, nz)) continue;
+      i64 u = c.at(nx, ny, nz);
+      if (!seen[u]) { seen[u] = 1; stack.push_back(u); }
+    }
+  }
+  i64 voids = 0;
+  for (i64 z = 0; z < c.sz; ++z)
+    for (i64 y = 0; y < c.sy; ++y)
+      for (i64 x = 0; x < c.sx; ++x)
+        if (!fg(x, y, z) && !seen[c.at(x, y, z)]) ++voids;
+  return voids;
+}
+
+// The soma root (find_soma_roots, batched.cu:58-108): among the label's voxels whose DBF equals
+// its maximum, in box-relative coordinates sorted by (x, y, z), the first of least squared
+// distance (float64) to their centroid (float32 of the float64 mean).
+i64 soma_root(const std::vector<uint32_t> &cc, const std::vector<float> &dbf, const Shape &s, uint32_t id, float maximum,
+              const Box &b) {
+  std::vector<std::pair<std::array<i64, 3>, i64>> pts;
+  for (i64 v = 0; v < s.n(); ++v)
+    if (cc[v] == id && dbf[v] == maximum) {
+      i64 x, y, z; s.unravel(v, x, y, z);
+      pts.push_back({{x - b.lo[0], y - b.lo[1], z - b.lo[2]}, v});
+    }
+  std::sort(pts.begin(), pts.end());
+  i64 sum[3] = {0, 0, 0};
+  for (auto &p : pts) for (int a = 0; a < 3; ++a) sum[a] += p.first[a];
+  float center[3];
+  for (int a = 0; a < 3; ++a) center[a] = float(double(sum[a]) / double(pts.size()));
+  double best = INFINITY;
+  i64 root = -1;
+  for (auto &p : pts) {
+    double cost = 0;
+    for (int a = 0; a < 3; ++a) { double d = double(p.first[a]) - center[a]; cost += d * d; }
+    if (cost < best) { best = cost; root = p.second; }
+  }
+  return root;
+}
+
 // Vertex-weighted flood (batched.cu:246-251 with flood mode 1): the unique least fixpoint of
 // d[v] = min_u vw_step(d[u], vw[v]) over same-label 26-neighbours. vw_step is monotone
 // non-decreasing in d and adds at least one ulp for a positive weight, so Dijkstra reproduces it.
@@ -530,7 +628,7 @@ struct Prepared {
   Components comp;
   std::vector<Box> boxes;
   std::vector<float> dbf, daf, pdrf;
-  std::vector<uint8_t> active;
+  std::vector<uint8_t> active, soma;
   std::vector<float> dbf_max, max_daf;
   std::vector<i64> root, target;
   std::vector<std::vector<std::array<i64, 3>>> borders;
@@ -573,7 +671,8 @@ static void parallel_for(size_t count, int threads, const std::function<void(siz
   if (error) std::rethrow_exception(error);
 }
 
-// prepare_sample (skeletonize.cu:70-116) + prepare_batched (batched.cu:177-255), non-soma labels.
+// prepare_sample (skeletonize.cu:70-116) + prepare_batched (batched.cu:177-255); somata without
+// enclosed voids (their roots, free-space seeded DAF).
 Prepared prepare(const i64 *labels, const Shape &s, const Params &p, int threads) {
   Prepared out;
   out.s = s;
@@ -589,20 +688,29 @@ Prepared prepare(const i64 *labels, const Shape &s, const Params &p, int threads
   std::vector<i64> unused;
   seg_argmax(out.dbf, out.comp.cc, k, out.dbf_max, unused);
   out.active.assign(k, 0);
+  out.soma.assign(k, 0);
   out.root.assign(k, -1);
   for (size_t id = 1; id < k; ++id) {
     const Box &b = out.boxes[id];
     i64 ext = (b.hi[0] - b.lo[0] + 1) * (b.hi[1] - b.lo[1] + 1) * (b.hi[2] - b.lo[2] + 1);
     if (double(b.count) <= p.dust || ext <= 1) continue;
     out.active[id] = 1;
-    if (double(out.dbf_max[id]) > p.soma_detection)
-      throw std::runtime_error("soma preparation (dbf_max above soma_detection_threshold) is not restated by the oracle");
+    // batched.cu:194-202: a label above the detection threshold is checked for enclosed voids;
+    // without voids it is a soma when above the acceptance threshold (voids: soma.cu's crop path,
+    // not restated)
+    if (double(out.dbf_max[id]) > p.soma_detection) {
+      if (void_count(out.comp.cc, s, uint32_t(id), b))
+        throw std::runtime_error("a soma label with enclosed voids (soma.cu crop preparation) is not restated by the oracle");
+      out.soma[id] = double(out.dbf_max[id]) > p.soma_acceptance;
+    }
   }
   float W[26];
   geometric_weights(p.aniso, W);
   std::vector<i64> search;
   for (size_t id = 1; id < k; ++id)
-    if (out.active[id]) {
+    if (out.soma[id]) out.root[id] = soma_root(out.comp.cc, out.dbf, s, uint32_t(id), out.dbf_max[id], out.boxes[id]);
+  for (size_t id = 1; id < k; ++id)
+    if (out.active[id] && !out.soma[id]) {
       if (!out.borders[id].empty()) {
         auto &pt = out.borders[id].back();
         out.root[id] = s.at(pt[0], pt[1], pt[2]);
@@ -622,8 +730,21 @@ Prepared prepare(const i64 *labels, const Shape &s, const Params &p, int threads
   }
   std::vector<size_t> act;
   for (size_t id = 1; id < k; ++id) if (out.active[id]) act.push_back(id);
+  // somata: the free-space ball around the root seeds the flood (batched.cu:222-231)
+  std::vector<std::vector<i64>> members(k);
+  for (i64 v = 0; v < s.n(); ++v)
+    if (out.soma[out.comp.cc[v]]) members[out.comp.cc[v]].push_back(v);
   parallel_for(act.size(), threads, [&](size_t i) {
-    geometric_flood(out.comp.cc, s, W, {out.root[act[i]]}, out.daf);
+    size_t id = act[i];
+    if (!out.soma[id]) { geometric_flood(out.comp.cc, s, W, {out.root[id]}, out.daf); return; }
+    i64 rx, ry, rz; s.unravel(out.root[id], rx, ry, rz);
+    std::vector<std::pair<i64, float>> seeds = {{out.root[id], 0.0f}};
+    for (i64 v : members[id]) {
+      i64 x, y, z; s.unravel(v, x, y, z);
+      float val;
+      if (freespace_seed(x - rx, y - ry, z - rz, p.aniso[0], p.aniso[1], p.aniso[2], out.dbf_max[id], val)) seeds.push_back({v, val});
+    }
+    geometric_flood_seeded(out.comp.cc, s, W, seeds, out.daf);
   });
   seg_argmax(out.daf, out.comp.cc, k, out.max_daf, out.target);
   double exponent = p.pdrf_exponent;
@@ -750,7 +871,8 @@ struct Tracer {
   // Rolling-ball invalidation (invalidation_kernels.cuh:105-299; per label as lockstep_coop.inc:550-845):
   // bands of the bottleneck M on the fixed grid [k delta, (k+1) delta), synchronous rounds from a
   // snapshot inside a band, exact pulls of the waiting voxels at each band boundary.
-  i64 invalidate(uint32_t L, const std::vector<i64> &path) {
+  i64 invalidate(uint32_t L, const std::vector<i64> &path) { return invalidate(L, path, scale, constant); }
+  i64 invalidate(uint32_t L, const std::vector<i64> &path, float ball_scale, float ball_constant) {
     std::vector<i64> touched, frontier, next, far, keep;
     for (i64 p : path) {
       if (!alive[p]) continue;
@@ -761,7 +883,7 @@ struct Tracer {
         touched.push_back(p);
       }
     }
-    auto ball = [&](i64 src) { return scale * dbf[src] + constant; };
+    auto ball = [&](i64 src) { return ball_scale * dbf[src] + ball_constant; };
     auto offer_d = [&](i64 x, i64 y, i64 z, i64 src) {
       i64 sxx, syy, szz; s.unravel(src, sxx, syy, szz);
       float dvx = wx * (float)(x - sxx), dvy = wy * (float)(y - syy), dvz = wz * (float)(z - szz);
@@ -933,8 +1055,9 @@ Result skeletonize(const i64 *labels, const Shape &s, const Params &p, const std
     labels_list.push_back(id);
     std::vector<std::array<i64, 3>> m = pr.borders[id];
     if (!m.empty()) m.pop_back();
+    if (pr.soma[id] && !pr.borders[id].empty()) m.insert(m.begin(), pr.borders[id].back());
     m.insert(m.end(), extra[id].begin(), extra[id].end());
-    if (m.empty()) manual[id].push_back(pr.target[id]);
+    if (m.empty() && !pr.soma[id]) manual[id].push_back(pr.target[id]);
     else for (auto it = m.rbegin(); it != m.rend(); ++it) manual[id].push_back(s.at((*it)[0], (*it)[1], (*it)[2]));
     i64 budget = p.max_paths >= 0 ? p.max_paths : pr.boxes[id].count;
     maxp[id] = i64(manual[id].size()) >= budget ? 0 : std::min<i64>(INT32_MAX, budget);
@@ -978,6 +1101,27 @@ Result skeletonize(const i64 *labels, const Shape &s, const Params &p, const std
               float(p.scale), float(p.constant), aniso_min};
     size_t cursor = 0;
     i64 pcount = 0;
+    // a soma: its ball dies before the first path and the budget shrinks by what it took
+    // (lockstep.cu:197-206); paths are trimmed to the vertices beyond the soma radius
+    // (lockstep_regular.inc:35-45, 128-147, 184-195)
+    const bool soma = pr.soma[L] != 0;
+    const double radius = soma ? double(pr.dbf_max[L]) * p.soma_scale + p.soma_constant : -1.0;
+    i64 rx = 0, ry = 0, rz = 0;
+    if (soma) {
+      s.unravel(pr.root[L], rx, ry, rz);
+      i64 removed = tr.invalidate(L, {pr.root[L]}, float(p.soma_scale), float(p.soma_constant));
+      inval_counts[L] += removed;
+      i64 budget = p.max_paths >= 0 ? p.max_paths : pr.boxes[L].count - removed;
+      maxp[L] = i64(manual[L].size()) >= budget ? 0 : std::min<i64>(INT32_MAX, budget);
+    }
+    auto far = [&](i64 v) {
+      i64 x, y, z; s.unravel(v, x, y, z);
+      double dx = double(p.aniso[0]) * (double(x) - double(rx)), dy = double(p.aniso[1]) * (double(y) - double(ry)),
+             dz = double(p.aniso[2]) * (double(z) - double(rz));
+      double ss = dx * dx + dy * dy;
+      ss = ss + dz * dz;
+      return std::sqrt(ss) > radius;
+    };
     for (;;) {
       if (pcount >= maxp[L]) break;
       i64 t;
@@ -1001,9 +1145,21 @@ Result skeletonize(const i64 *labels, const Shape &s, const Params &p, const std
           cur = par;
         }
       }
+      if (soma) {
+        std::vector<i64> kept;
+        if (p.fix_branching) {  // target-first: the target, the far vertices, the root for a near rail end
+          kept.push_back(path.front());
+          for (size_t j = 1; j < path.size(); ++j) if (far(path[j])) kept.push_back(path[j]);
+          if (!far(path.back())) kept.push_back(pr.root[L]);
+        } else {  // target .. root: the far vertices and the root
+          for (size_t j = 0; j + 1 < path.size(); ++j) if (far(path[j])) kept.push_back(path[j]);
+          kept.push_back(path.back());
+        }
+        path.swap(kept);
+      }
       records[L].push_back({L, path});
       if (p.fix_branching) for (i64 v : path) pr.pdrf[v] = 0.0f;
-      else if (p.stop_emitted) for (i64 v : path) parent[v] = -2;
+      else if (p.stop_emitted && !soma) for (i64 v : path) parent[v] = -2;
       inval_counts[L] += tr.invalidate(L, path);
       pcount++;
     }
@@ -1170,7 +1326,7 @@ int64_t oracle_railroad(const float *field, const uint32_t *cc, int64_t sx, int6
 int64_t oracle_prepare(const int64_t *labels, int64_t sx, int64_t sy, int64_t sz, const double *params /*8*/,
                        const float *aniso, double dust, int fix_borders, int threads, uint32_t *cc_out, float *dbf_out,
                        float *daf_out, float *pdrf_out, int64_t table_cap, uint8_t *active_out, float *dbf_max_out,
-                       float *max_daf_out, int64_t *root_out, int64_t *target_out) {
+                       float *max_daf_out, int64_t *root_out, int64_t *target_out, uint8_t *soma_out) {
   int64_t count = -1;
   guarded([&] {
     Params p;
@@ -1190,6 +1346,7 @@ int64_t oracle_prepare(const int64_t *labels, int64_t sx, int64_t sy, int64_t sz
     for (size_t i = 0; i < k; ++i) {
       active_out[i] = pr.active[i]; dbf_max_out[i] = pr.dbf_max[i]; max_daf_out[i] = pr.max_daf[i];
       root_out[i] = pr.root[i]; target_out[i] = pr.target[i];
+      if (soma_out) soma_out[i] = pr.soma[i];
     }
     count = int64_t(k - 1);
   });

@@ -91,3 +91,57 @@ def summary_from_packed(packed):
         verts, edges, radii = packed["vertices"][v0:v1], packed["edges"][e0:e1], packed["radii"][v0:v1]
         out[int(label)] = [v1 - v0, e1 - e0, float(np.max(radii)), digest(verts, edges, radii)]
     return out
+
+
+# ---- somata (the reference's test_package.py:347-367 input and variants) ----
+
+SOMA_PARAMS = dict(scale=1.5, const=2, pdrf_scale=100000, pdrf_exponent=4, soma_detection_threshold=6,
+                   soma_acceptance_threshold=9, soma_invalidation_scale=1.0, soma_invalidation_const=2)
+
+
+def soma_ball(size=72, centre=36, radius=12):
+    """A ball with six processes: three 3x3 tubes through the centre along x, y and z, from 2 to
+    size - 2 (test_package.py:351-357)."""
+    grid = np.indices((size,) * 3) - centre
+    labels = ((grid ** 2).sum(axis=0) <= radius ** 2).astype(np.uint8)
+    for axis in range(3):
+        tube = [slice(centre - 1, centre + 2)] * 3
+        tube[axis] = slice(2, size - 2)
+        labels[tuple(tube)] = 1
+    return np.asfortranarray(labels)
+
+
+def soma_cells(seed=7, shape=(96, 80, 64)):
+    """Several labels, some with an off-centre ball on a bent tube, some plain tubes, one ball
+    touching a face: somata beside ordinary labels in one volume."""
+    rng = np.random.default_rng(seed)
+    out = np.zeros(shape, np.uint32)
+    zz, yy, xx = None, None, None
+    grid = np.indices(shape)
+    for label in range(1, 7):
+        c = np.array([rng.integers(12, s - 12) for s in shape])
+        tube_axis = int(rng.integers(0, 3))
+        r_tube = 1 + int(rng.integers(0, 2))
+        d = np.abs(grid - c.reshape(3, 1, 1, 1))
+        others = [a for a in range(3) if a != tube_axis]
+        tube = (d[others[0]] <= r_tube) & (d[others[1]] <= r_tube)
+        bend = grid[others[0]] - c[others[0]] - np.round(6 * np.sin(grid[tube_axis] / 9.0)).astype(int)
+        tube = (np.abs(bend) <= r_tube) & (d[others[1]] <= r_tube)
+        body = tube
+        if label % 2:
+            r = 10 + int(rng.integers(0, 4))
+            body = body | ((((grid - c.reshape(3, 1, 1, 1)) ** 2).sum(axis=0)) <= r * r)
+        out[body & (out == 0)] = label
+    # a ball cut by the x = 0 face
+    ball = (((grid - np.array([2, shape[1] // 2, shape[2] // 2]).reshape(3, 1, 1, 1)) ** 2).sum(axis=0)) <= 49
+    out[ball & (out == 0)] = 9
+    return np.asfortranarray(out)
+
+
+def soma_with_void(size=40):
+    """A ball with a one-voxel cavity at its centre and a tube: a soma label with an enclosed void."""
+    grid = np.indices((size,) * 3) - size // 2
+    labels = ((grid ** 2).sum(axis=0) <= 10 ** 2).astype(np.uint8)
+    labels[size // 2 - 1:size // 2 + 2, size // 2 - 1:size // 2 + 2, 2:size - 2] = 1
+    labels[size // 2, size // 2, size // 2 - 5] = 0
+    return np.asfortranarray(labels)
