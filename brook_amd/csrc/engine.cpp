@@ -121,18 +121,41 @@ class Sample {
       active_[id] = 1;
       labels_.push_back(int(id));
     }
-    // somata (batched.cu:194-205): a label above the detection threshold is checked for enclosed
-    // voids; without voids it is a soma when above the acceptance threshold. A label with voids
-    // needs the crop preparation of soma.cu (filled mask, refreshed DBF, ownership of the voids).
+    // Somata (batched.cu:194-205, soma.cu:141-237), in ascending id: a label above the detection
+    // threshold whose box encloses voids takes them, evicting every traced label wholly inside
+    // them (one partly inside needs the reference's private arenas: not supported), and its DBF
+    // becomes the EDT of its filled box. A label is a soma when its (refreshed) DBF maximum
+    // exceeds the acceptance threshold.
     soma_.assign(k_, 0);
-    bool any_soma = false;
+    bool any_soma = false, any_fill = false;
+    std::vector<uint8_t> evicted(k_, 0);
     for (int id : labels_) {
-      if (!(double(dbf_max_[id]) > o_.teasar.soma_detection)) continue;
-      if (enclosed_voids(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, id, boxes_[id]))
-        throw Unsupported("a component above soma_detection_threshold encloses voids; filling them (soma.cu's crop "
-                          "preparation) is not supported by this build");
+      if (evicted[id] || !(double(dbf_max_[id]) > o_.teasar.soma_detection)) continue;
+      if (enclosed_voids(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, id, boxes_[id])) {
+        VoidFill fill(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, id, boxes_[id], k_);
+        const std::vector<int64_t> &taken = fill.taken();
+        for (size_t o = 1; o < k_; ++o) {
+          if (!taken[o] || !active_[o] || evicted[o]) continue;
+          if (taken[o] != boxes_[o].count)
+            throw Unsupported("a soma's voids hold part of another traced component (the reference's private "
+                              "arenas); not supported by this build");
+          evicted[o] = 1;
+          active_[o] = 0;
+        }
+        fill.paint(ctx_);
+        for (size_t o = 1; o < k_; ++o) boxes_[o].count -= taken[o];
+        boxes_[id].count += fill.voids();
+        dbf_max_[id] = fill.dbf_refresh(ctx_, o_.aniso, dbf_.data(), boxes_[id].count);
+        any_fill = true;
+      }
       soma_[id] = double(dbf_max_[id]) > o_.teasar.soma_acceptance;
       any_soma |= soma_[id] != 0;
+    }
+    if (any_fill) {
+      std::vector<int> kept;
+      for (int id : labels_)
+        if (!evicted[id]) kept.push_back(id);
+      labels_.swap(kept);
     }
     std::vector<int> seg_start(k_ + 1, 0), seg_count(k_, 0);
     for (size_t id = 1; id < k_; ++id) seg_count[id] = int(boxes_[id].count);
@@ -153,8 +176,10 @@ class Sample {
       // coordinates sorted by (x, y, z), the first of least squared distance to their centroid
       Buffer<uint8_t> soma_dev(ctx_, k_);
       soma_dev.upload(ctx_, soma_.data(), k_);
-      std::vector<Maximum> maxima = soma_maxima(ctx_, cc_.labels.data(), cc_.cc_bytes, n_, dbf_.data(),
-                                                reinterpret_cast<const float *>(label_max.data()), soma_dev.data());
+      Buffer<float> dbf_max_dev(ctx_, k_);
+      dbf_max_dev.upload(ctx_, dbf_max_.data(), k_);
+      std::vector<Maximum> maxima = soma_maxima(ctx_, cc_.labels.data(), cc_.cc_bytes, n_, dbf_.data(), dbf_max_dev.data(),
+                                                soma_dev.data());
       std::map<int64_t, std::vector<std::pair<Point, int64_t>>> by_label;
       for (const Maximum &m : maxima) {
         const Box &b = boxes_[size_t(m.id)];

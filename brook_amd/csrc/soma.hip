@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <cstring>
+
 #include "soma.hpp"
 
 namespace brook {
@@ -97,6 +99,74 @@ int64_t voids_of(Context &ctx, const CC *cc, Dims d, CC id, const Crop &c) {
   return int64_t(voids.download(ctx)[0]);
 }
 
+// Box voxels: mask = the label or a void; a void voxel's previous owner is counted.
+template <class CC>
+__global__ void void_owners(const CC *cc, Dims d, Crop c, CC id, const unsigned *reached, uint8_t *mask,
+                            unsigned long long *taken) {
+  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= c.size()) return;
+  const int x = int(i % c.ext[0]), y = int((i / c.ext[0]) % c.ext[1]), z = int(i / ((long long)c.ext[0] * c.ext[1]));
+  const CC l = cc[(c.lo[0] + x) + (long long)d.sx * ((c.lo[1] + y) + (long long)d.sy * (c.lo[2] + z))];
+  const bool hole = l != id && reached[i] == 0u;
+  mask[i] = (l == id || hole) ? 1 : 0;
+  if (hole && l != CC(0)) g_add(taken + (unsigned long long)l, 1ULL);
+}
+
+template <class CC> __global__ void paint_voids(CC *cc, Dims d, Crop c, CC id, const uint8_t *mask) {
+  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= c.size() || !mask[i]) return;
+  const int x = int(i % c.ext[0]), y = int((i / c.ext[0]) % c.ext[1]), z = int(i / ((long long)c.ext[0] * c.ext[1]));
+  cc[(c.lo[0] + x) + (long long)d.sx * ((c.lo[1] + y) + (long long)d.sy * (c.lo[2] + z))] = id;
+}
+
+__global__ void overlay_dbf(Dims d, Crop c, const uint8_t *mask, const float *box_dbf, float *dbf) {
+  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= c.size() || !mask[i]) return;
+  const int x = int(i % c.ext[0]), y = int((i / c.ext[0]) % c.ext[1]), z = int(i / ((long long)c.ext[0] * c.ext[1]));
+  dbf[(c.lo[0] + x) + (long long)d.sx * ((c.lo[1] + y) + (long long)d.sy * (c.lo[2] + z))] = box_dbf[i];
+}
+
+template <class CC>
+int64_t fill_of(Context &ctx, const CC *cc, Dims d, CC id, const Crop &c, size_t ids, Buffer<uint8_t> &mask,
+                std::vector<int64_t> &taken) {
+  auto s = ctx.stream();
+  const long long n = c.size();
+  if (n > INT32_MAX) throw std::invalid_argument("soma bounding box exceeds int32 capacity");
+  Buffer<unsigned> reached(ctx, size_t(n));
+  Buffer<int> a(ctx, size_t(n)), b(ctx, size_t(n)), counts(ctx, 2);
+  Buffer<unsigned long long> owners(ctx, ids);
+  reached.clear(ctx);
+  counts.clear(ctx);
+  owners.clear(ctx);
+  void_seed<CC><<<blocks_for(n), kThreads, 0, s>>>(cc, d, c, id, reached.data(), a.data(), counts.data());
+  BROOK_LAUNCHED();
+  int front = 0;
+  BROOK_HIP(hipMemcpyAsync(&front, counts.data(), 4, hipMemcpyDeviceToHost, s));
+  ctx.synchronize();
+  int slot = 0;
+  while (front > 0) {
+    const int other = 1 - slot;
+    BROOK_HIP(hipMemsetAsync(counts.data() + other, 0, 4, s));
+    void_round<CC><<<blocks_for(front), kThreads, 0, s>>>(cc, d, c, id, reached.data(), a.data(), front, b.data(),
+                                                          counts.data() + other);
+    BROOK_LAUNCHED();
+    BROOK_HIP(hipMemcpyAsync(&front, counts.data() + other, 4, hipMemcpyDeviceToHost, s));
+    ctx.synchronize();
+    std::swap(a, b);
+    slot = other;
+  }
+  mask = Buffer<uint8_t>(ctx, size_t(n));
+  void_owners<CC><<<blocks_for(n), kThreads, 0, s>>>(cc, d, c, id, reached.data(), mask.data(), owners.data());
+  BROOK_LAUNCHED();
+  const std::vector<unsigned long long> o = owners.download(ctx);
+  taken.assign(o.begin(), o.end());
+  Buffer<unsigned long long> holes(ctx, 1);  // the voids' count
+  holes.clear(ctx);
+  void_tally<CC><<<blocks_for(n), kThreads, 0, s>>>(cc, d, c, id, reached.data(), holes.data());
+  BROOK_LAUNCHED();
+  return int64_t(holes.download(ctx)[0]);
+}
+
 template <class CC> struct AtMaximum {
   const CC *cc;
   const float *dbf, *dbf_max;
@@ -146,6 +216,45 @@ int64_t enclosed_voids(Context &ctx, const void *cc, int cc_bytes, Dims d, int64
   if (cc_bytes == 1) return voids_of(ctx, static_cast<const uint8_t *>(cc), d, uint8_t(id), c);
   if (cc_bytes == 2) return voids_of(ctx, static_cast<const uint16_t *>(cc), d, uint16_t(id), c);
   return voids_of(ctx, static_cast<const uint32_t *>(cc), d, uint32_t(id), c);
+}
+
+VoidFill::VoidFill(Context &ctx, void *cc, int cc_bytes, Dims d, int64_t id, const Box &box, size_t ids)
+    : cc_(cc), cc_bytes_(cc_bytes), d_(d), id_(id) {
+  Crop c;
+  for (int a = 0; a < 3; ++a) {
+    c.lo[a] = lo_[a] = box.lo[a];
+    c.ext[a] = ext_[a] = box.hi[a] - box.lo[a] + 1;
+  }
+  if (cc_bytes == 1) voids_ = fill_of(ctx, static_cast<const uint8_t *>(cc), d, uint8_t(id), c, ids, mask_, taken_);
+  else if (cc_bytes == 2) voids_ = fill_of(ctx, static_cast<const uint16_t *>(cc), d, uint16_t(id), c, ids, mask_, taken_);
+  else voids_ = fill_of(ctx, static_cast<const uint32_t *>(cc), d, uint32_t(id), c, ids, mask_, taken_);
+}
+
+void VoidFill::paint(Context &ctx) {
+  Crop c;
+  for (int a = 0; a < 3; ++a) { c.lo[a] = lo_[a]; c.ext[a] = ext_[a]; }
+  const unsigned grid = blocks_for(c.size());
+  auto s = ctx.stream();
+  if (cc_bytes_ == 1) paint_voids<uint8_t><<<grid, kThreads, 0, s>>>(static_cast<uint8_t *>(cc_), d_, c, uint8_t(id_), mask_.data());
+  else if (cc_bytes_ == 2) paint_voids<uint16_t><<<grid, kThreads, 0, s>>>(static_cast<uint16_t *>(cc_), d_, c, uint16_t(id_), mask_.data());
+  else paint_voids<uint32_t><<<grid, kThreads, 0, s>>>(static_cast<uint32_t *>(cc_), d_, c, uint32_t(id_), mask_.data());
+  BROOK_LAUNCHED();
+}
+
+float VoidFill::dbf_refresh(Context &ctx, const float aniso[3], float *dbf, int64_t filled_count) {
+  Crop c;
+  for (int a = 0; a < 3; ++a) { c.lo[a] = lo_[a]; c.ext[a] = ext_[a]; }
+  const Dims box{ext_[0], ext_[1], ext_[2]};
+  Buffer<float> box_dbf(ctx, size_t(c.size()));
+  Buffer<unsigned> maximum(ctx, 2);
+  maximum.clear(ctx);
+  edt(ctx, mask_.data(), 1, box, aniso, filled_count == c.size(), 3, box_dbf.data(), maximum.data());
+  overlay_dbf<<<blocks_for(c.size()), kThreads, 0, ctx.stream()>>>(d_, c, mask_.data(), box_dbf.data(), dbf);
+  BROOK_LAUNCHED();
+  const unsigned bits = maximum.download(ctx)[1];
+  float m;
+  std::memcpy(&m, &bits, 4);
+  return m;
 }
 
 std::vector<Maximum> soma_maxima(Context &ctx, const void *cc, int cc_bytes, long long n, const float *dbf,

@@ -13,6 +13,30 @@ namespace brook {
 // face of the box (the voids fill_voids would fill).
 int64_t enclosed_voids(Context &ctx, const void *cc, int cc_bytes, Dims d, int64_t id, const Box &box);
 
+// The voids of id's box, found once and then applied (soma.cu:141-237 for a soma without private
+// arenas): taken()[o] = voxels of id o inside the voids; paint() writes id into every void voxel;
+// dbf_refresh() replaces the DBF of id's filled box by that box's own EDT (black border when the
+// filled mask fills the box) and returns its maximum.
+class VoidFill {
+ public:
+  VoidFill(Context &ctx, void *cc, int cc_bytes, Dims d, int64_t id, const Box &box, size_t ids);
+  int64_t voids() const { return voids_; }
+  const std::vector<int64_t> &taken() const { return taken_; }
+  void paint(Context &ctx);
+  // `filled_count`: the label's voxels after paint() (the box's black border when it is the box)
+  float dbf_refresh(Context &ctx, const float aniso[3], float *dbf, int64_t filled_count);
+
+ private:
+  void *cc_;
+  int cc_bytes_;
+  Dims d_;
+  int64_t id_;
+  int lo_[3], ext_[3];
+  int64_t voids_ = 0;
+  std::vector<int64_t> taken_;
+  Buffer<uint8_t> mask_;  // per box voxel: 1 = id or void
+};
+
 // Every voxel v (ascending flat index) of an id with soma[id] != 0 whose DBF equals dbf_max[id]
 // (device tables of count + 1 entries), with its id.
 struct Maximum {

@@ -531,6 +531,39 @@ i64 void_count(const std::vector<uint32_t> &cc, const Shape &s, uint32_t id, con
   return voids;
 }
 
+// The voids themselves (flat indices, ascending).
+std::vector<i64> void_voxels(const std::vector<uint32_t> &cc, const Shape &s, uint32_t id, const Box &b) {
+  std::vector<i64> out;
+  if (!void_count(cc, s, id, b)) return out;
+  Shape c{b.hi[0] - b.lo[0] + 1, b.hi[1] - b.lo[1] + 1, b.hi[2] - b.lo[2] + 1};
+  auto fg = [&](i64 x, i64 y, i64 z) { return cc[s.at(x + b.lo[0], y + b.lo[1], z + b.lo[2])] == id; };
+  std::vector<uint8_t> seen(size_t(c.n()), 0);
+  std::vector<i64> stack;
+  for (i64 z = 0; z < c.sz; ++z)
+    for (i64 y = 0; y < c.sy; ++y)
+      for (i64 x = 0; x < c.sx; ++x)
+        if ((x == 0 || y == 0 || z == 0 || x == c.sx - 1 || y == c.sy - 1 || z == c.sz - 1) && !fg(x, y, z)) {
+          seen[c.at(x, y, z)] = 1;
+          stack.push_back(c.at(x, y, z));
+        }
+  while (!stack.empty()) {
+    i64 v = stack.back(); stack.pop_back();
+    i64 x, y, z; c.unravel(v, x, y, z);
+    for (int k = 0; k < 6; ++k) {
+      i64 nx = x + NX[k], ny = y + NY[k], nz = z + NZ[k];
+      if (!c.inside(nx, ny, nz) || fg(nx, ny, nz)) continue;
+      i64 u = c.at(nx, ny, nz);
+      if (!seen[u]) { seen[u] = 1; stack.push_back(u); }
+    }
+  }
+  for (i64 z = 0; z < c.sz; ++z)
+    for (i64 y = 0; y < c.sy; ++y)
+      for (i64 x = 0; x < c.sx; ++x)
+        if (!fg(x, y, z) && !seen[c.at(x, y, z)]) out.push_back(s.at(x + b.lo[0], y + b.lo[1], z + b.lo[2]));
+  std::sort(out.begin(), out.end());
+  return out;
+}
+
 // The soma root (find_soma_roots, batched.cu:58-108): among the label's voxels whose DBF equals
 // its maximum, in box-relative coordinates sorted by (x, y, z), the first of least squared
 // distance (float64) to their centroid (float32 of the float64 mean).
@@ -671,8 +704,8 @@ static void parallel_for(size_t count, int threads, const std::function<void(siz
   if (error) std::rethrow_exception(error);
 }
 
-// prepare_sample (skeletonize.cu:70-116) + prepare_batched (batched.cu:177-255); somata without
-// enclosed voids (their roots, free-space seeded DAF).
+// prepare_sample (skeletonize.cu:70-116) + prepare_batched (batched.cu:177-255) + the somata of
+// soma.cu (voids filled, enclosed labels evicted, refreshed DBF; roots, free-space seeded DAF).
 Prepared prepare(const i64 *labels, const Shape &s, const Params &p, int threads) {
   Prepared out;
   out.s = s;
@@ -695,14 +728,50 @@ Prepared prepare(const i64 *labels, const Shape &s, const Params &p, int threads
     i64 ext = (b.hi[0] - b.lo[0] + 1) * (b.hi[1] - b.lo[1] + 1) * (b.hi[2] - b.lo[2] + 1);
     if (double(b.count) <= p.dust || ext <= 1) continue;
     out.active[id] = 1;
-    // batched.cu:194-202: a label above the detection threshold is checked for enclosed voids;
-    // without voids it is a soma when above the acceptance threshold (voids: soma.cu's crop path,
-    // not restated)
-    if (double(out.dbf_max[id]) > p.soma_detection) {
-      if (void_count(out.comp.cc, s, uint32_t(id), b))
-        throw std::runtime_error("a soma label with enclosed voids (soma.cu crop preparation) is not restated by the oracle");
-      out.soma[id] = double(out.dbf_max[id]) > p.soma_acceptance;
+  }
+  // Somata (batched.cu:194-205, soma.cu:141-237), in ascending id: a label above the detection
+  // threshold whose box encloses voids takes them (fill_voids), evicting every traced label wholly
+  // inside them (one partly inside is the reference's private-arena case: not restated); its DBF
+  // becomes the EDT of its filled box (black border when the filled mask fills the box). A label
+  // is a soma when its (refreshed) DBF maximum exceeds the acceptance threshold.
+  std::vector<uint8_t> evicted(k, 0);
+  for (size_t id = 1; id < k; ++id) {
+    if (!out.active[id] || evicted[id] || !(double(out.dbf_max[id]) > p.soma_detection)) continue;
+    Box &b = out.boxes[id];
+    std::vector<i64> voids = void_voxels(out.comp.cc, s, uint32_t(id), b);
+    if (!voids.empty()) {
+      std::vector<i64> taken(k, 0);
+      for (i64 v : voids) if (out.comp.cc[v]) taken[out.comp.cc[v]]++;
+      for (size_t o = 1; o < k; ++o) {
+        if (!taken[o] || !(double(out.boxes[o].count) > p.dust) || !out.active[o] || evicted[o]) continue;
+        if (taken[o] != out.boxes[o].count)
+          throw std::runtime_error("a soma's voids hold part of another traced label (soma.cu private arenas): not restated");
+        evicted[o] = 1;
+        out.active[o] = 0;
+      }
+      for (i64 v : voids) out.comp.cc[v] = uint32_t(id);
+      for (size_t o = 1; o < k; ++o) out.boxes[o].count -= taken[o];
+      b.count += i64(voids.size());
+      Shape c{b.hi[0] - b.lo[0] + 1, b.hi[1] - b.lo[1] + 1, b.hi[2] - b.lo[2] + 1};
+      std::vector<uint8_t> mask(size_t(c.n()));
+      for (i64 z = 0; z < c.sz; ++z)
+        for (i64 y = 0; y < c.sy; ++y)
+          for (i64 x = 0; x < c.sx; ++x)
+            mask[c.at(x, y, z)] = out.comp.cc[s.at(x + b.lo[0], y + b.lo[1], z + b.lo[2])] == id;
+      std::vector<float> D(size_t(c.n()));
+      edt_rule<uint8_t>(mask.data(), c, p.aniso, b.count == c.n(), 3, D.data());
+      float maximum = 0.0f;
+      for (i64 z = 0; z < c.sz; ++z)
+        for (i64 y = 0; y < c.sy; ++y)
+          for (i64 x = 0; x < c.sx; ++x) {
+            i64 i = c.at(x, y, z);
+            if (!mask[i]) continue;
+            out.dbf[s.at(x + b.lo[0], y + b.lo[1], z + b.lo[2])] = D[i];
+            maximum = std::max(maximum, D[i]);
+          }
+      out.dbf_max[id] = maximum;
     }
+    out.soma[id] = double(out.dbf_max[id]) > p.soma_acceptance;
   }
   float W[26];
   geometric_weights(p.aniso, W);

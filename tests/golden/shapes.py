@@ -145,3 +145,25 @@ def soma_with_void(size=40):
     labels[size // 2 - 1:size // 2 + 2, size // 2 - 1:size // 2 + 2, 2:size - 2] = 1
     labels[size // 2, size // 2, size // 2 - 5] = 0
     return np.asfortranarray(labels)
+
+
+def soma_cavity(size=48, nucleus=False, staircase=False):
+    """A ball (radius 12) with a 5x5x5 cavity beside its centre and 3x3 processes along x, y and z
+    through the whole volume, so the label's box is the volume. `nucleus`: a label 2 inside the
+    cavity. `staircase`: a label 3 running diagonally from the cavity through the wall to the
+    outside (26-connected, so part of it lies in the voids and part outside)."""
+    c = size // 2
+    grid = np.indices((size,) * 3) - c
+    labels = ((grid ** 2).sum(axis=0) <= 12 ** 2).astype(np.uint8)
+    for axis in range(3):
+        tube = [slice(c - 1, c + 2)] * 3
+        tube[axis] = slice(0, size)
+        labels[tuple(tube)] = 1
+    cav = (slice(c + 3, c + 8), slice(c - 2, c + 3), slice(c - 2, c + 3))
+    labels[cav] = 0
+    if nucleus:
+        labels[c + 4:c + 7, c - 1:c + 2, c - 1:c + 2] = 2
+    if staircase:
+        for i in range(12):
+            labels[c + 5 + i, c + i, c] = 3
+    return np.asfortranarray(labels)

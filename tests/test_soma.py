@@ -5,8 +5,10 @@ CPU tests pin the oracle: the reference's own soma test (test_package.py:347-367
 processes -- every process traced, the soma root a vertex, one connected skeleton) in both
 branching modes, the root rule on a plateau of maxima with a known centroid and tie, the seeded
 DAF against a brute-force float32 Bellman-Ford from the free-space seeds and the seed formula
-against its closed forms, trimming (no vertex within the soma radius but the root), and the void
-check. GPU tests compare brook_amd with the oracle bit for bit.
+against its closed forms, trimming (no vertex within the soma radius but the root), and filled
+voids: a soma whose box is the whole volume skeletonizes like the volume with its cavity filled
+(a label inside the cavity disappears), a label partly inside raises. GPU tests compare brook_amd
+with the oracle bit for bit.
 """
 import numpy as np
 import pytest
@@ -175,8 +177,36 @@ def test_oracle_soma_cells_trimming_and_voids():
         radius = float(p["dbf_max"][i]) * P["soma_invalidation_scale"] + P["soma_invalidation_const"]
         assert np.all((d > radius) | (d == 0)), label
         assert np.any(d == 0), label
-    with pytest.raises(RuntimeError, match="voids"):
-        oracle.skeletonize(shapes.soma_with_void(), P, (1, 1, 1), 0, fix_borders=False)
+
+
+def _prefilled(labels):
+    out = labels.copy()
+    c = labels.shape[0] // 2
+    out[c + 3:c + 8, c - 2:c + 3, c - 2:c + 3] = 1
+    return out
+
+
+@pytest.mark.parametrize("fix_branching", [True, False])
+@pytest.mark.parametrize("nucleus", [False, True])
+def test_oracle_soma_voids_equal_a_prefilled_volume(fix_branching, nucleus):
+    """The soma's box is the whole volume, so its refreshed DBF (the EDT of the filled box) is the
+    EDT of the volume with the cavity filled: the skeletons equal those of that volume, and a
+    label wholly inside the cavity is gone."""
+    labels = shapes.soma_cavity(nucleus=nucleus)
+    got = oracle.skeletonize(labels, P, (1, 1, 1), 0, fix_branching=fix_branching, fix_borders=False)
+    want = oracle.skeletonize(_prefilled(labels), P, (1, 1, 1), 0, fix_branching=fix_branching, fix_borders=False)
+    assert got["labels"].tolist() == [1]
+    for name in ("v_off", "e_off", "vertices", "edges", "radii"):
+        assert np.array_equal(got[name], want[name]), name
+    p = oracle.prepare(labels, P, (1, 1, 1), 0, fix_borders=False)
+    assert p["soma"][1] == 1 and (not nucleus or p["active"][2] == 0)
+
+
+def test_oracle_soma_voids_errors():
+    with pytest.raises(RuntimeError, match="private arenas"):
+        oracle.skeletonize(shapes.soma_cavity(staircase=True), P, (1, 1, 1), 0, fix_borders=False)
+    r = oracle.skeletonize(shapes.soma_with_void(), P, (1, 1, 1), 0, fix_borders=False)
+    assert r["labels"].tolist() == [1] and len(r["edges"]) > 0
 
 
 # ---- GPU ----
@@ -246,6 +276,23 @@ def test_gpu_soma_batch_and_max_paths(brook):
 
 
 @pytest.mark.gpu
-def test_gpu_soma_with_voids_is_unsupported(brook):
-    with pytest.raises(NotImplementedError, match="voids"):
-        brook.skeletonize(shapes.soma_with_void(), P, (1, 1, 1), dust_threshold=0, fix_borders=False, progress=False)
+@pytest.mark.parametrize("fix_branching", [True, False])
+@pytest.mark.parametrize("case", ["void", "cavity", "nucleus"])
+def test_gpu_soma_voids_equal_oracle(brook, fix_branching, case):
+    labels = {"void": shapes.soma_with_void(), "cavity": shapes.soma_cavity(),
+              "nucleus": shapes.soma_cavity(nucleus=True)}[case]
+    got = brook.skeletonize(labels, P, (1, 1, 1), dust_threshold=0, fix_branching=fix_branching, fix_borders=True,
+                            output="packed", progress=False)
+    want = oracle.skeletonize(labels, P, (1, 1, 1), 0, fix_branching=fix_branching, fix_borders=True)
+    _same(got, want, (case, fix_branching))
+    fields = brook.prepare_fields(labels, teasar_params=P, anisotropy=(1, 1, 1), dust_threshold=0, fix_borders=True)
+    ref = oracle.prepare(labels, P, (1, 1, 1), 0, fix_borders=True)
+    assert np.array_equal(fields["dbf"], ref["dbf"]) and np.array_equal(fields["daf"], ref["daf"])
+    assert np.array_equal(fields["active"], ref["active"])
+
+
+@pytest.mark.gpu
+def test_gpu_soma_partial_enclosure_is_unsupported(brook):
+    with pytest.raises(NotImplementedError, match="private"):
+        brook.skeletonize(shapes.soma_cavity(staircase=True), P, (1, 1, 1), dust_threshold=0, fix_borders=False,
+                          progress=False)
