@@ -54,8 +54,7 @@ Packed empty_result(const SkeletonizeOptions &o) {
 void check_options(const SkeletonizeOptions &o) {
   for (float a : o.aniso)
     if (!(a > 0) || !std::isfinite(a)) throw std::invalid_argument("anisotropy must be finite and positive");
-  if (o.fill_holes || o.fix_avocados || o.voxel_graph)
-    throw Unsupported("fill_holes, fix_avocados and voxel_graph are not supported by this build");
+  if (o.fix_avocados || o.voxel_graph) throw Unsupported("fix_avocados and voxel_graph are not supported by this build");
   if (!o.after.empty()) throw Unsupported("extra_targets_after is not supported by this build");
   const double exponent = o.teasar.pdrf_exponent;
   if (std::isfinite(exponent) && exponent == std::floor(exponent) && std::fmod(exponent, 2.0) == 1.0)
@@ -97,7 +96,11 @@ class Sample {
     k_ = size_t(count_) + 1;
     boxes_ = analyze(ctx_, cc_, d_);
     lap("prep_analyze_ms");
-    const bool black_border = count_ == 1 && boxes_[1].count == n_;
+    const bool black_border = count_ == 1 && boxes_[1].count == n_;  // of the volume before any fill
+    if (o_.fill_holes) {
+      fill_holes();
+      lap("prep_fill_holes_ms");
+    }
     dbf_ = Buffer<float>(ctx_, size_t(n_));
     Buffer<unsigned> label_max(ctx_, k_);
     label_max.clear(ctx_);
@@ -223,6 +226,30 @@ class Sample {
     voxel_records(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, rec_.data());
     for (int i = 0; i < 6; ++i) lists_.emplace_back(ctx_, size_t(std::max<int64_t>(F_, 1)));
     if (!o_.fix_branching) parent_ = Buffer<int>(ctx_, size_t(n_));
+  }
+
+  // fill_all_holes (holes.cu:37-62, skeletonize.cu:83-90): in ascending id and with the original
+  // boxes, a component's box voids become the component; a component painted over, even in part,
+  // fills nothing later; then the boxes and first voxels are recomputed.
+  void fill_holes() {
+    std::vector<uint8_t> skip(k_, 0);
+    bool painted = false;
+    for (size_t id = 1; id < k_; ++id) {
+      const Box &b = boxes_[id];
+      if (!b.count || skip[id]) continue;
+      const int64_t ex = b.hi[0] - b.lo[0] + 1, ey = b.hi[1] - b.lo[1] + 1, ez = b.hi[2] - b.lo[2] + 1;
+      if (ex <= 2 || ey <= 2 || ez <= 2 || ex * ey * ez == b.count) continue;  // no voxel can be enclosed
+      if (!enclosed_voids(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, int64_t(id), b)) continue;
+      VoidFill fill(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, int64_t(id), b, k_);
+      fill.paint(ctx_);
+      for (size_t o = 1; o < k_; ++o)
+        if (fill.taken()[o] && o != id) skip[o] = 1;
+      painted = true;
+    }
+    if (!painted) return;
+    boxes_ = analyze(ctx_, cc_, d_);
+    const std::vector<int64_t> first = first_voxels(ctx_, cc_.labels.data(), cc_.cc_bytes, n_, count_);
+    for (size_t id = 1; id < k_; ++id) cc_.roots[id - 1] = first[id];
   }
 
   FloodJob flood_job() const {

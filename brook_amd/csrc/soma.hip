@@ -35,48 +35,50 @@ __global__ void void_seed(const CC *cc, Dims d, Crop c, CC id, unsigned *reached
   g_st(front + wave_slot(count), int(i));
 }
 
-// One round: the face neighbours of the front that are not the label and not yet reached.
+// One launch of the flood: every front entry explores depth-first from itself (a small stack per
+// thread, a budget of pops), claiming box voxels that are not the label with an exchange; what a
+// full stack or a spent budget cannot hold goes to the next front. Each voxel is claimed once, so
+// a front never exceeds the box.
+constexpr int kDfsStack = 16, kDfsBudget = 512;
 template <class CC>
 __global__ void void_round(const CC *cc, Dims d, Crop c, CC id, unsigned *reached, const int *front, int n, int *next,
                            int *count) {
   const int j = blockIdx.x * kThreads + threadIdx.x;
   if (j >= n) return;
-  const long long i = front[j];
-  const int x = int(i % c.ext[0]), y = int((i / c.ext[0]) % c.ext[1]), z = int(i / ((long long)c.ext[0] * c.ext[1]));
   const long long sx = c.ext[0], sxy = (long long)c.ext[0] * c.ext[1];
-  const int step[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
-  for (int k = 0; k < 6; ++k) {
-    const int nx = x + step[k][0], ny = y + step[k][1], nz = z + step[k][2];
-    if (nx < 0 || ny < 0 || nz < 0 || nx >= c.ext[0] || ny >= c.ext[1] || nz >= c.ext[2]) continue;
-    if (is_label(cc, d, c, nx, ny, nz, id)) continue;
-    const long long v = nx + sx * ny + sxy * nz;
-    if (g_xchg(reached + v, 1u) == 0u) g_st(next + wave_slot(count), int(v));
+  int stack[kDfsStack];
+  int sp = 0, budget = kDfsBudget;
+  stack[sp++] = front[j];
+  while (sp > 0) {
+    const long long i = stack[--sp];
+    if (--budget < 0) {  // hand the rest back
+      g_st(next + atomicAdd(count, 1), int(i));
+      continue;
+    }
+    const int x = int(i % c.ext[0]), y = int((i / c.ext[0]) % c.ext[1]), z = int(i / sxy);
+    for (int k = 0; k < 6; ++k) {
+      const int nx = x + (k == 0 ? -1 : k == 1 ? 1 : 0), ny = y + (k == 2 ? -1 : k == 3 ? 1 : 0),
+                nz = z + (k == 4 ? -1 : k == 5 ? 1 : 0);
+      if (nx < 0 || ny < 0 || nz < 0 || nx >= c.ext[0] || ny >= c.ext[1] || nz >= c.ext[2]) continue;
+      if (is_label(cc, d, c, nx, ny, nz, id)) continue;
+      const long long v = nx + sx * ny + sxy * nz;
+      if (g_xchg(reached + v, 1u) != 0u) continue;
+      if (sp < kDfsStack) stack[sp++] = int(v);
+      else g_st(next + atomicAdd(count, 1), int(v));
+    }
   }
 }
 
-template <class CC>
-__global__ void void_tally(const CC *cc, Dims d, Crop c, CC id, const unsigned *reached, unsigned long long *voids) {
-  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
-  bool hole = false;
-  if (i < c.size()) {
-    const int x = int(i % c.ext[0]), y = int((i / c.ext[0]) % c.ext[1]), z = int(i / ((long long)c.ext[0] * c.ext[1]));
-    hole = !is_label(cc, d, c, x, y, z, id) && reached[i] == 0u;
-  }
-  const int n = __popcll(__ballot(hole));
-  if ((threadIdx.x & 63) == 0 && n) g_add(voids, (unsigned long long)n);
-}
-
-template <class CC>
-int64_t voids_of(Context &ctx, const CC *cc, Dims d, CC id, const Crop &c) {
+// reached[i] = 1 for the box voxels that are not the label and reach a face of the box through
+// such voxels (face neighbours).
+template <class CC> void reach_faces(Context &ctx, const CC *cc, Dims d, CC id, const Crop &c, Buffer<unsigned> &reached) {
   auto s = ctx.stream();
   const long long n = c.size();
-  if (n > INT32_MAX) throw std::invalid_argument("soma bounding box exceeds int32 capacity");
-  Buffer<unsigned> reached(ctx, size_t(n));
+  if (n > INT32_MAX) throw std::invalid_argument("label bounding box exceeds int32 capacity");
+  reached = Buffer<unsigned>(ctx, size_t(n));
   Buffer<int> a(ctx, size_t(n)), b(ctx, size_t(n)), counts(ctx, 2);
-  Buffer<unsigned long long> voids(ctx, 1);
   reached.clear(ctx);
   counts.clear(ctx);
-  voids.clear(ctx);
   void_seed<CC><<<blocks_for(n), kThreads, 0, s>>>(cc, d, c, id, reached.data(), a.data(), counts.data());
   BROOK_LAUNCHED();
   int front = 0;
@@ -94,7 +96,27 @@ int64_t voids_of(Context &ctx, const CC *cc, Dims d, CC id, const Crop &c) {
     std::swap(a, b);
     slot = other;
   }
-  void_tally<CC><<<blocks_for(n), kThreads, 0, s>>>(cc, d, c, id, reached.data(), voids.data());
+}
+
+template <class CC>
+__global__ void void_tally(const CC *cc, Dims d, Crop c, CC id, const unsigned *reached, unsigned long long *voids) {
+  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  bool hole = false;
+  if (i < c.size()) {
+    const int x = int(i % c.ext[0]), y = int((i / c.ext[0]) % c.ext[1]), z = int(i / ((long long)c.ext[0] * c.ext[1]));
+    hole = !is_label(cc, d, c, x, y, z, id) && reached[i] == 0u;
+  }
+  const int n = __popcll(__ballot(hole));
+  if ((threadIdx.x & 63) == 0 && n) g_add(voids, (unsigned long long)n);
+}
+
+template <class CC>
+int64_t voids_of(Context &ctx, const CC *cc, Dims d, CC id, const Crop &c) {
+  Buffer<unsigned> reached;
+  reach_faces(ctx, cc, d, id, c, reached);
+  Buffer<unsigned long long> voids(ctx, 1);
+  voids.clear(ctx);
+  void_tally<CC><<<blocks_for(c.size()), kThreads, 0, ctx.stream()>>>(cc, d, c, id, reached.data(), voids.data());
   BROOK_LAUNCHED();
   return int64_t(voids.download(ctx)[0]);
 }
@@ -131,40 +153,26 @@ int64_t fill_of(Context &ctx, const CC *cc, Dims d, CC id, const Crop &c, size_t
                 std::vector<int64_t> &taken) {
   auto s = ctx.stream();
   const long long n = c.size();
-  if (n > INT32_MAX) throw std::invalid_argument("soma bounding box exceeds int32 capacity");
-  Buffer<unsigned> reached(ctx, size_t(n));
-  Buffer<int> a(ctx, size_t(n)), b(ctx, size_t(n)), counts(ctx, 2);
-  Buffer<unsigned long long> owners(ctx, ids);
-  reached.clear(ctx);
-  counts.clear(ctx);
+  Buffer<unsigned> reached;
+  reach_faces(ctx, cc, d, id, c, reached);
+  Buffer<unsigned long long> owners(ctx, ids), holes(ctx, 1);
   owners.clear(ctx);
-  void_seed<CC><<<blocks_for(n), kThreads, 0, s>>>(cc, d, c, id, reached.data(), a.data(), counts.data());
-  BROOK_LAUNCHED();
-  int front = 0;
-  BROOK_HIP(hipMemcpyAsync(&front, counts.data(), 4, hipMemcpyDeviceToHost, s));
-  ctx.synchronize();
-  int slot = 0;
-  while (front > 0) {
-    const int other = 1 - slot;
-    BROOK_HIP(hipMemsetAsync(counts.data() + other, 0, 4, s));
-    void_round<CC><<<blocks_for(front), kThreads, 0, s>>>(cc, d, c, id, reached.data(), a.data(), front, b.data(),
-                                                          counts.data() + other);
-    BROOK_LAUNCHED();
-    BROOK_HIP(hipMemcpyAsync(&front, counts.data() + other, 4, hipMemcpyDeviceToHost, s));
-    ctx.synchronize();
-    std::swap(a, b);
-    slot = other;
-  }
+  holes.clear(ctx);
   mask = Buffer<uint8_t>(ctx, size_t(n));
   void_owners<CC><<<blocks_for(n), kThreads, 0, s>>>(cc, d, c, id, reached.data(), mask.data(), owners.data());
   BROOK_LAUNCHED();
-  const std::vector<unsigned long long> o = owners.download(ctx);
-  taken.assign(o.begin(), o.end());
-  Buffer<unsigned long long> holes(ctx, 1);  // the voids' count
-  holes.clear(ctx);
   void_tally<CC><<<blocks_for(n), kThreads, 0, s>>>(cc, d, c, id, reached.data(), holes.data());
   BROOK_LAUNCHED();
+  const std::vector<unsigned long long> o = owners.download(ctx);
+  taken.assign(o.begin(), o.end());
   return int64_t(holes.download(ctx)[0]);
+}
+
+template <class CC> __global__ void first_of(const CC *cc, long long n, unsigned long long *first) {
+  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const CC l = cc[i];
+  if (l != CC(0)) g_min(first + (unsigned long long)l, (unsigned long long)i);
 }
 
 template <class CC> struct AtMaximum {
@@ -255,6 +263,20 @@ float VoidFill::dbf_refresh(Context &ctx, const float aniso[3], float *dbf, int6
   float m;
   std::memcpy(&m, &bits, 4);
   return m;
+}
+
+std::vector<int64_t> first_voxels(Context &ctx, const void *cc, int cc_bytes, long long n, int64_t count) {
+  Buffer<unsigned long long> first(ctx, size_t(count + 1));
+  first.fill_bytes(ctx, 0xFF);
+  const unsigned grid = blocks_for(n);
+  auto s = ctx.stream();
+  if (cc_bytes == 1) first_of<uint8_t><<<grid, kThreads, 0, s>>>(static_cast<const uint8_t *>(cc), n, first.data());
+  else if (cc_bytes == 2) first_of<uint16_t><<<grid, kThreads, 0, s>>>(static_cast<const uint16_t *>(cc), n, first.data());
+  else first_of<uint32_t><<<grid, kThreads, 0, s>>>(static_cast<const uint32_t *>(cc), n, first.data());
+  BROOK_LAUNCHED();
+  std::vector<int64_t> out;
+  for (unsigned long long f : first.download(ctx)) out.push_back(f == ~0ULL ? -1 : int64_t(f));
+  return out;
 }
 
 std::vector<Maximum> soma_maxima(Context &ctx, const void *cc, int cc_bytes, long long n, const float *dbf,

@@ -37,6 +37,9 @@ class VoidFill {
   Buffer<uint8_t> mask_;  // per box voxel: 1 = id or void
 };
 
+// Per id (count + 1 entries): its smallest flat index (-1: no voxel).
+std::vector<int64_t> first_voxels(Context &ctx, const void *cc, int cc_bytes, long long n, int64_t count);
+
 // Every voxel v (ascending flat index) of an id with soma[id] != 0 whose DBF equals dbf_max[id]
 // (device tables of count + 1 entries), with its id.
 struct Maximum {

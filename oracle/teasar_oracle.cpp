@@ -652,7 +652,7 @@ struct Params {
   i64 max_paths = -1;  // < 0: none
   float aniso[3] = {1, 1, 1};
   double dust = 1000;
-  bool fix_branching = true, fix_borders = true;
+  bool fix_branching = true, fix_borders = true, fill_holes = false;
   bool stop_emitted = true;  // fix_branching=False: walks end on the first emitted vertex (lockstep_regular.inc:160-165)
 };
 
@@ -713,7 +713,30 @@ Prepared prepare(const i64 *labels, const Shape &s, const Params &p, int threads
   size_t count = out.comp.roots.size();
   size_t k = count + 1;
   out.boxes = analyze(out.comp.cc, s, count);
-  bool black_border = count == 1 && out.boxes[1].count == s.n();
+  bool black_border = count == 1 && out.boxes[1].count == s.n();  // of the volume before any fill
+  if (p.fill_holes) {
+    // fill_all_holes (holes.cu:37-62, skeletonize.cu:83-90): in ascending id and with the original
+    // boxes, a component's box voids become the component; a component painted over, even in
+    // part, fills nothing later; then the boxes and first voxels are recomputed.
+    std::vector<uint8_t> skip(k, 0);
+    bool painted = false;
+    for (size_t id = 1; id < k; ++id) {
+      if (!out.boxes[id].count || skip[id]) continue;
+      std::vector<i64> voids = void_voxels(out.comp.cc, s, uint32_t(id), out.boxes[id]);
+      for (i64 v : voids) {
+        uint32_t o = out.comp.cc[v];
+        if (o && o != id) skip[o] = 1;
+        out.comp.cc[v] = uint32_t(id);
+      }
+      painted |= !voids.empty();
+    }
+    if (painted) {
+      out.boxes = analyze(out.comp.cc, s, count);
+      std::vector<i64> first(k, -1);
+      for (i64 v = s.n() - 1; v >= 0; --v) first[out.comp.cc[v]] = v;
+      for (size_t id = 1; id < k; ++id) out.comp.roots[id - 1] = first[id];
+    }
+  }
   out.dbf.resize(s.n());
   edt_rule<uint32_t>(out.comp.cc.data(), s, p.aniso, black_border, 3, out.dbf.data());
   out.borders.assign(k, {});
@@ -1403,7 +1426,8 @@ int64_t oracle_prepare(const int64_t *labels, int64_t sx, int64_t sy, int64_t sz
     p.soma_detection = params[4]; p.soma_acceptance = params[5]; p.soma_scale = params[6]; p.soma_constant = params[7];
     std::copy(aniso, aniso + 3, p.aniso);
     p.dust = dust;
-    p.fix_borders = fix_borders != 0;
+    p.fix_borders = (fix_borders & 1) != 0;  // bit 0: fix_borders, bit 1: fill_holes
+    p.fill_holes = (fix_borders & 2) != 0;
     Shape s{sx, sy, sz};
     Prepared pr = prepare(labels, s, p, threads);
     size_t k = pr.comp.roots.size() + 1;
@@ -1456,7 +1480,8 @@ void *oracle_skeletonize(const int64_t *labels, int64_t sx, int64_t sy, int64_t 
     p.dust = dust;
     p.fix_branching = fix_branching > 0;  // 1 fix, 0 parent walks with the STOP rule, -1 full walks
     p.stop_emitted = fix_branching == 0;
-    p.fix_borders = fix_borders != 0;
+    p.fix_borders = (fix_borders & 1) != 0;  // bit 0: fix_borders, bit 1: fill_holes
+    p.fill_holes = (fix_borders & 2) != 0;
     std::vector<std::array<i64, 3>> before;
     for (int64_t i = 0; i < ntargets; ++i) before.push_back({targets_before[3 * i], targets_before[3 * i + 1], targets_before[3 * i + 2]});
     Result r = skeletonize(labels, Shape{sx, sy, sz}, p, before, threads);
