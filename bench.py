@@ -264,7 +264,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the host-input, fix_branching=False, merge, batched, stress and served measurements")
-    ap.add_argument("--served-sweep", action="store_true", help="serve at 2, 10, 20 and 50 requests/s (default: 10)")
+    ap.add_argument("--served-sweep", action="store_true",
+                    help="serve at 1, 2, 5, 10, 20 and 50 requests/s (default: 10 only)")
+    ap.add_argument("--served-requests", type=int, default=24, help="requests per arrival rate")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy three times (the known-size reference "
@@ -367,7 +369,8 @@ def main():
         out["merge"] = merge(result)
         out["batched"] = batched()
         out["stress"] = stress()
-        out["served"] = served((2.0, 10.0, 20.0, 50.0) if args.served_sweep else (10.0,))
+        out["served"] = served((1.0, 2.0, 5.0, 10.0, 20.0, 50.0) if args.served_sweep else (10.0,),
+                               count=args.served_requests)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(vol, args.cpu_threads)
     else:

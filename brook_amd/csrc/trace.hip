@@ -428,11 +428,14 @@ __device__ __forceinline__ void backtrace(const TraceArgs &a, Shared &sh, int t,
 // The field (batched.cu:246-251): the railroad from the root in `source` mode (the near-far
 // bands of flood_lab_nf, batched_coop.inc:125-267) gives every voxel of the label its
 // vertex-weighted distance over the PDRF; parents_lab (batched_regular.inc:113-134) then takes
-// the same-label neighbour of smallest distance, first k on ties (-1 at the root), and the words
-// go back to ~0. Returns false when the run met a voxel of PDRF 0 besides the root (it would stop
+// the same-label neighbour of smallest distance, first k on ties (-1 at the root and at voxels the
+// run does not reach, as a label that filled holes split has), and the words go back to ~0.
+// Returns false when the run met a voxel of PDRF 0 besides the root (it would stop
 // there as at a rail; PDRF underflows to 0 only for extreme exponents): the label is ejected.
-__device__ __forceinline__ bool parent_field(const TraceArgs &a, Shared &sh, int root, const Front &na, const Front &nb,
-                                             int *l2, int *l3, int *touched) {
+__device__ __forceinline__ bool parent_field(const TraceArgs &a, Shared &sh, int root, const int *voxels, int count,
+                                             const Front &na, const Front &nb, int *l2, int *l3, int *touched) {
+  for (int j = threadIdx.x; j < count; j += blockDim.x) g_st(a.parent + g_ld(voxels + j), -1);  // unreached: no parent
+  __syncthreads();
   const int status = railroad(a, sh, root, na, nb, l2, l3, touched, true);
   const int nt = sh.n_touched;
   for (int j = threadIdx.x; j < nt; j += blockDim.x) {
@@ -971,7 +974,7 @@ __global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restri
     __syncthreads();
     bool traceable = true;
     if (a.parent) {
-      traceable = parent_field(a, sh, int(g_ld(a.root + L)), na, nb, l2, l3, l4);
+      traceable = parent_field(a, sh, int(g_ld(a.root + L)), a.order + seg, count, na, nb, l2, l3, l4);
       lap(sh, 2);
       if (!traceable && threadIdx.x == 0) g_st(a.ejected + g_add(a.counters + 2, 1) % a.ejected_cap, L);
     }
@@ -996,7 +999,18 @@ __global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restri
       for (int j = threadIdx.x; j < nt; j += blockDim.x) {  // restore the railroad state
         sp_u64(&a.rec[g_ld(l4 + j)].word, kMaxKey);
       }
-      if (status != 1 || sh.path_len < 0) {
+      if (status == 2) {
+        // No rail reachable from t (a component that filled holes split): as the reference's
+        // per-component tracer does for such a label (trace.cu:198-215), an ordered target ends
+        // the label with the paths it has, a manual one counts as an empty path.
+        const bool manual = sh.pcount < g_ld(a.mcount + L);
+        __syncthreads();
+        if (!manual) break;
+        if (threadIdx.x == 0) sh.pcount += 1;
+        __syncthreads();
+        continue;
+      }
+      if (sh.path_len < 0) {
         if (threadIdx.x == 0) g_st(a.ejected + g_add(a.counters + 2, 1) % a.ejected_cap, L);
         __syncthreads();
         break;

@@ -1226,7 +1226,11 @@ Result skeletonize(const i64 *labels, const Shape &s, const Params &p, const std
       std::vector<i64> path;
       if (p.fix_branching) {
         path = tr.railroad(L, t);
-        if (path.empty()) throw std::runtime_error("no rail reachable (the per-component fallback is not restated)");
+        if (path.empty()) {  // no rail reachable (trace.cu:198-215): an ordered target ends the label,
+          if (pcount >= i64(manual[L].size())) break;  // a manual one is an empty path
+          pcount++;
+          continue;
+        }
       } else {
         // backtrace_parent_lab (lockstep_regular.inc:166-202): the walk keeps every vertex up to
         // and including the first with no parent (-1 at the root, STOP -2 at an emitted vertex).

@@ -18,9 +18,10 @@ from tests.golden import shapes
 P = dict(scale=1.5, const=2, pdrf_scale=100000, pdrf_exponent=4)
 
 
-def hollow_cells(seed=0, size=64):
+def hollow_cells(seed=0, size=64, split=False):
     """Shells with cavities (one holding a cube label, one a staircase label that leaves through
-    the wall diagonally), an open cup, and tubes."""
+    the wall diagonally -- on two sides when `split`, so that filling splits it), an open cup,
+    and tubes."""
     rng = np.random.default_rng(seed)
     out = np.zeros((size, size, size), np.uint32)
     grid = np.indices(out.shape)
@@ -33,6 +34,8 @@ def hollow_cells(seed=0, size=64):
     out[ball((44, 40, 20), 10) & ~ball((44, 40, 20), 5)] = 2     # shell ...
     for i in range(10):                                            # ... and a staircase through its wall
         out[44 + i, 40 + i, 20] = 8
+        if split:                                                  # the filled cavity and wall cut it in two
+            out[44 - i, 40 - i, 20] = 8
     cup = ball((20, 46, 46), 9) & ~ball((20, 46, 46), 5)
     cup[20:, :, :] &= False                                        # open on one side
     out[cup] = 3
@@ -66,9 +69,9 @@ def scipy_fill(cc):
     return cur
 
 
-@pytest.mark.parametrize("seed", [0, 1])
-def test_oracle_fill_equals_scipy(seed):
-    labels = hollow_cells(seed)
+@pytest.mark.parametrize("seed,split", [(0, False), (1, True)])
+def test_oracle_fill_equals_scipy(seed, split):
+    labels = hollow_cells(seed, split=split)
     cc, mapping, roots = oracle.ccl(labels)
     want = scipy_fill(cc)
     got = oracle.prepare(labels, P, (1, 1, 1), 0, fix_borders=False, fill_holes=True)["cc"]
@@ -104,13 +107,27 @@ def brook():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("seed", [0, 1])
+@pytest.mark.parametrize("seed,split", [(0, False), (1, True)])
 @pytest.mark.parametrize("fix_branching", [True, False])
-def test_gpu_fill_holes_equals_oracle(brook, seed, fix_branching):
-    labels = hollow_cells(seed)
+def test_gpu_fill_holes_equals_oracle(brook, seed, split, fix_branching):
+    """`split`: a label that filling cuts in two; its flood reaches one piece, and a target in the
+    other piece, which no path reaches, ends the label (the reference's per-component rule)."""
+    labels = hollow_cells(seed, split=split)
     got = brook.skeletonize(labels, P, (2, 2, 3), dust_threshold=0, fix_branching=fix_branching, fill_holes=True,
                             output="packed", progress=False).as_dict()
     want = oracle.skeletonize(labels, P, (2, 2, 3), 0, fix_branching=fix_branching, fill_holes=True)
+    assert got["labels"].tolist() == want["labels"].tolist()
+    for name in ("v_off", "e_off", "vertices", "edges", "radii"):
+        assert np.array_equal(np.asarray(got[name]), np.asarray(want[name])), name
+
+
+@pytest.mark.gpu
+def test_gpu_fill_holes_tubes_equal_oracle(brook):
+    from brook_amd import synth
+    labels = synth.tubes_128(1001)
+    params = dict(scale=1.5, const=300, pdrf_scale=100000, pdrf_exponent=4)
+    got = brook.skeletonize(labels, params, (16, 16, 40), fill_holes=True, output="packed", progress=False).as_dict()
+    want = oracle.skeletonize(labels, params, (16, 16, 40), 1000, fill_holes=True, threads=8)
     assert got["labels"].tolist() == want["labels"].tolist()
     for name in ("v_off", "e_off", "vertices", "edges", "radii"):
         assert np.array_equal(np.asarray(got[name]), np.asarray(want[name])), name
