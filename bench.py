@@ -43,6 +43,7 @@ def _kernel_table(stats, n, b_in, b_cc, F):
         "ccl": (["ccl_init", "ccl_union", "ccl_count", "ccl_mark", "ccl_relabel"], (b_in + b_cc) * n),
         "edt": (["edt_x", "edt_y", "edt_z"], (3 * b_cc + 4 + 8 + 8) * n),
         "prepare_labels": (["prepare_labels"], 12 * flood),
+        "init_records": (["init_records"], (b_cc + 16) * n),
         "pdrf_alive": (["pdrf_alive"], (4 + 4 + b_cc + 4 + 1) * n),
         "trace_labels": (["trace_labels"], 12 * reached + 13 * claimed),
     }
@@ -234,6 +235,7 @@ def served(rates=(10.0,), count=24, seed=5005):
                     futures.append(srv.submit(pool[j]))
                 for f in futures:
                     f.result()
+                _progress(f"served {lam:g} req/s, {mode} done")
                 lat = [(f.request.done - f.request.arrival) * 1e3 for f in futures]
                 span = max(f.request.done for f in futures) - t0
                 row[mode] = dict(p50_ms=round(float(np.percentile(lat, 50)), 2),
@@ -254,6 +256,11 @@ def cpu_baseline(vol, threads):
     return dict(value=crop.size / dt / 1e6, unit="Mvoxels/s", cores=threads, kind="port",
                 sample=f"oracle on the 256^3 corner crop of the 512^3 volume ({crop.size} voxels), "
                        f"{threads} threads, {dt:.2f} s")
+
+
+def _progress(what):
+    """One line on stderr per finished part (the JSON line stays the only stdout output)."""
+    print(f"bench: {what}", file=sys.stderr, flush=True)
 
 
 def main():
@@ -305,7 +312,7 @@ def main():
         run()
     native.check(native.lib().brook_context_set_kernel_timing(ctx.handle, 1))
     names = ["ccl_init", "ccl_union", "ccl_count", "ccl_mark", "ccl_relabel", "edt_x", "edt_y", "edt_z",
-             "prepare_labels", "pdrf_alive", "trace_labels"]
+             "init_records", "prepare_labels", "pdrf_alive", "trace_labels"]
     stat_names = ["railroad_reached_voxels", "invalidated_voxels", "flood_voxels", "foreground_voxels", "cc_bytes",
                   "paths", "components", "prep_ccl_ms", "prep_analyze_ms", "prep_edt_ms", "prep_borders_ms",
                   "prep_floods_ms", "prep_pdrf_order_ms", "trace_ms", "assembly_ms"]
@@ -363,16 +370,23 @@ def main():
         roofline=roof,
         generator_s=round(gen_s, 2),
     )
+    _progress(f"plain steps done ({out['value']} Mvoxels/s)")
     if not args.no_extras:
         out["host_io"] = host_io(vol)
+        _progress("host_io done")
         out["fix_branching_false"] = parent_paths(dev, n)
+        _progress("fix_branching_false done")
         out["merge"] = merge(result)
+        _progress("merge done")
         out["batched"] = batched()
+        _progress("batched done")
         out["stress"] = stress()
+        _progress("stress done")
         out["served"] = served((1.0, 2.0, 5.0, 10.0, 20.0, 50.0) if args.served_sweep else (10.0,),
                                count=args.served_requests)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(vol, args.cpu_threads)
+        _progress("cpu_baseline done")
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -228,13 +228,13 @@ def skeletonize(all_labels, teasar_params=DEFAULT_TEASAR_PARAMS, anisotropy=(1, 
 
 
 def prepare_fields(labels, teasar_params=DEFAULT_TEASAR_PARAMS, anisotropy=(1, 1, 1), dust_threshold=1000,
-                   fix_borders=True, extra_targets_before=()):
+                   fix_borders=True, extra_targets_before=(), fill_holes=False, fix_avocados=False):
     """The preamble's fields on the host (diagnostics): dbf, daf, pdrf (Fortran-ordered float32
     volumes, pdrf before the roots become rails) and per-component tables."""
     keep = []
     vol, size = volume_struct(labels, keep)
     shape = tuple(int(vol.shape[a]) for a in range(3))
-    opts = options_struct(teasar_params, anisotropy, dust_threshold, True, fix_borders, False, False, None,
+    opts = options_struct(teasar_params, anisotropy, dust_threshold, True, fix_borders, fill_holes, fix_avocados, None,
                           extra_targets_before, (), keep)
     dbf, daf, pdrf = (np.zeros(shape, np.float32, order="F") for _ in range(3))
     cap = size + 2

@@ -4,10 +4,12 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <memory>
 #include <numeric>
 
+#include "avocado.hpp"
 #include "fields.hpp"
 #include "soma.hpp"
 #include "trace.hpp"
@@ -16,6 +18,7 @@ namespace brook {
 namespace {
 
 using Clock = std::chrono::steady_clock;
+constexpr float kInfHost = std::numeric_limits<float>::infinity();
 
 struct Lap {
   Context &ctx;
@@ -54,7 +57,7 @@ Packed empty_result(const SkeletonizeOptions &o) {
 void check_options(const SkeletonizeOptions &o) {
   for (float a : o.aniso)
     if (!(a > 0) || !std::isfinite(a)) throw std::invalid_argument("anisotropy must be finite and positive");
-  if (o.fix_avocados || o.voxel_graph) throw Unsupported("fix_avocados and voxel_graph are not supported by this build");
+  if (o.voxel_graph) throw Unsupported("voxel_graph is not supported by this build");
   if (!o.after.empty()) throw Unsupported("extra_targets_after is not supported by this build");
   const double exponent = o.teasar.pdrf_exponent;
   if (std::isfinite(exponent) && exponent == std::floor(exponent) && std::fmod(exponent, 2.0) == 1.0)
@@ -109,6 +112,11 @@ class Sample {
     dbf_max_.resize(k_);
     for (size_t i = 0; i < k_; ++i) std::memcpy(&dbf_max_[i], &max_bits[i], 4);
     lap("prep_edt_ms");
+    if (o_.fix_avocados) {
+      fix_avocados(black_border);
+      lap("prep_avocados_ms");
+      if (count_ == 0) return finish_empty();
+    }
     borders_.assign(k_, {});
     if (o_.fix_borders) borders_ = border_targets(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, count_, o_.aniso);
     lap("prep_borders_ms");
@@ -226,6 +234,49 @@ class Sample {
     voxel_records(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, rec_.data());
     for (int i = 0; i < 6; ++i) lists_.emplace_back(ctx_, size_t(std::max<int64_t>(F_, 1)));
     if (!o_.fix_branching) parent_ = Buffer<int>(ctx_, size_t(n_));
+  }
+
+  // fix_avocados (skeletonize.cu:91-95): the correction runs on host copies of the ids and the DBF
+  // (as the reference's does); the DBF is refreshed on the device after every pass that changed
+  // labels; then the ids are renumbered by first voxel and the boxes recomputed.
+  void fix_avocados(bool black_border) {
+    const size_t n = size_t(n_);
+    std::vector<uint32_t> labels(n);
+    {
+      const std::vector<uint8_t> raw = cc_.labels.download(ctx_, n * size_t(cc_.cc_bytes));
+      for (size_t i = 0; i < n; ++i) {
+        uint32_t v = 0;
+        std::memcpy(&v, raw.data() + i * cc_.cc_bytes, size_t(cc_.cc_bytes));
+        labels[i] = v;
+      }
+    }
+    const std::vector<uint32_t> original = labels;
+    std::vector<float> distance = dbf_.download(ctx_, n);
+    Buffer<uint32_t> scratch;
+    auto refresh = [&](const std::vector<uint32_t> &current, std::vector<float> &field) {
+      if (!scratch.size()) scratch = Buffer<uint32_t>(ctx_, n);
+      scratch.upload(ctx_, current.data(), n);
+      edt(ctx_, scratch.data(), 4, d_, o_.aniso, black_border, 3, dbf_.data());
+      field = dbf_.download(ctx_, n);
+    };
+    ctx_.stats["avocado_passes"] = avocado_correct(labels, distance, d_, k_, o_.teasar.soma_detection, refresh);
+    std::vector<int64_t> mapping, roots;
+    const size_t count = avocado_renumber(labels, original, cc_.mapping, k_, mapping, roots);
+    const int bytes = id_bytes(int64_t(count));
+    std::vector<uint8_t> raw(n * size_t(bytes));
+    for (size_t i = 0; i < n; ++i) std::memcpy(raw.data() + i * bytes, &labels[i], size_t(bytes));  // little endian
+    cc_.labels = Buffer<uint8_t>(ctx_, std::max<size_t>(raw.size(), 1));
+    cc_.labels.upload(ctx_, raw.data(), raw.size());
+    cc_.cc_bytes = bytes;
+    cc_.count = int64_t(count);
+    cc_.mapping = mapping;
+    cc_.roots = roots;
+    count_ = int64_t(count);
+    k_ = count + 1;
+    dbf_max_.assign(k_, 0.0f);
+    for (size_t i = 0; i < n; ++i)
+      if (labels[i] && distance[i] < kInfHost) dbf_max_[labels[i]] = std::max(dbf_max_[labels[i]], distance[i]);
+    if (count) boxes_ = analyze(ctx_, cc_, d_);
   }
 
   // fill_all_holes (holes.cu:37-62, skeletonize.cu:83-90): in ascending id and with the original

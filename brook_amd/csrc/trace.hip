@@ -1003,14 +1003,10 @@ __global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restri
         // No rail reachable from t (a component that filled holes split): as the reference's
         // per-component tracer does for such a label (trace.cu:198-215), an ordered target ends
         // the label with the paths it has, a manual one counts as an empty path.
-        const bool manual = sh.pcount < g_ld(a.mcount + L);
+        if (sh.pcount >= g_ld(a.mcount + L)) break;
+        if (threadIdx.x == 0) sh.path_len = 0;
         __syncthreads();
-        if (!manual) break;
-        if (threadIdx.x == 0) sh.pcount += 1;
-        __syncthreads();
-        continue;
-      }
-      if (sh.path_len < 0) {
+      } else if (sh.path_len < 0) {
         if (threadIdx.x == 0) g_st(a.ejected + g_add(a.counters + 2, 1) % a.ejected_cap, L);
         __syncthreads();
         break;
@@ -1018,7 +1014,7 @@ __global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restri
       if (soma && threadIdx.x == 0) soma_trim(a, sh, walk, int(g_ld(a.root + L)), g_ld(&a.soma[L].radius));
       __syncthreads();
       const int len = sh.path_len;
-      if (threadIdx.x == 0) {
+      if (threadIdx.x == 0 && len > 0) {
         g_add(reinterpret_cast<unsigned long long *>(a.counters + 4), (unsigned long long)nt);
         int pos = g_add(a.counters + 0, len);
         int rec = g_add(a.counters + 1, 1);
