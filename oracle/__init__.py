@@ -167,7 +167,7 @@ def railroad(field, target, mask=None):
 
 
 def prepare(labels, teasar_params=None, anisotropy=(1, 1, 1), dust_threshold=1000, fix_borders=True, threads=1,
-            fill_holes=False):
+            fill_holes=False, fix_avocados=False):
     a, shape = _fortran_int64(labels)
     n = a.size
     cc = np.zeros(shape, np.uint32, order="F")
@@ -179,7 +179,7 @@ def prepare(labels, teasar_params=None, anisotropy=(1, 1, 1), dust_threshold=100
     soma = np.zeros(cap, np.uint8)
     pv = params_vector(teasar_params)
     an = np.asarray(anisotropy, np.float32)
-    flags = int(bool(fix_borders)) | (2 if fill_holes else 0)
+    flags = int(bool(fix_borders)) | (2 if fill_holes else 0) | (4 if fix_avocados else 0)
     count = lib().oracle_prepare(_ptr(a), *shape, _ptr(pv), _ptr(an), float(dust_threshold), flags, threads,
                                  _ptr(cc), _ptr(dbf), _ptr(daf), _ptr(pdrf), cap, _ptr(active), _ptr(dbf_max),
                                  _ptr(max_daf), _ptr(root), _ptr(target), _ptr(soma))
@@ -205,7 +205,8 @@ def vw_parents(labels, vw, sources):
 
 
 def skeletonize(labels, teasar_params=None, anisotropy=(1, 1, 1), dust_threshold=1000, fix_branching=True,
-                fix_borders=True, extra_targets_before=(), threads=1, stop_emitted=True, fill_holes=False):
+                fix_borders=True, extra_targets_before=(), threads=1, stop_emitted=True, fill_holes=False,
+                fix_avocados=False):
     """Packed result dict: labels, v_off, e_off, vertices (V,3) f32, edges (E,2) u32, radii f32, stats.
     fix_branching=False walks the parent field; stop_emitted=False makes every walk run to the root
     (Kimimaro's paths) instead of ending on the first emitted vertex."""
@@ -217,7 +218,8 @@ def skeletonize(labels, teasar_params=None, anisotropy=(1, 1, 1), dust_threshold
     L = lib()
     h = L.oracle_skeletonize(_ptr(a), *shape, _ptr(pv), -1 if mp is None else int(mp), _ptr(an), float(dust_threshold),
                              1 if fix_branching else (0 if stop_emitted else -1),
-                             int(bool(fix_borders)) | (2 if fill_holes else 0), _ptr(tb), len(tb),
+                             int(bool(fix_borders)) | (2 if fill_holes else 0) | (4 if fix_avocados else 0), _ptr(tb),
+                             len(tb),
                              threads)
     if not h:
         _fail("skeletonize")

@@ -25,6 +25,7 @@
 //                  the vw_step fixpoint; every parent chain satisfies d[v] = vw_step(d[parent], pdrf[v])
 //                  and reaches the root; the STOP rule's union of vertices and edges equals the
 //                  union of full walks to the root (Kimimaro's paths) on random volumes
+#include "avocado_oracle.hpp"
 #include <algorithm>
 #include <array>
 #include <atomic>
@@ -652,7 +653,7 @@ struct Params {
   i64 max_paths = -1;  // < 0: none
   float aniso[3] = {1, 1, 1};
   double dust = 1000;
-  bool fix_branching = true, fix_borders = true, fill_holes = false;
+  bool fix_branching = true, fix_borders = true, fill_holes = false, fix_avocados = false;
   bool stop_emitted = true;  // fix_branching=False: walks end on the first emitted vertex (lockstep_regular.inc:160-165)
 };
 
@@ -739,6 +740,34 @@ Prepared prepare(const i64 *labels, const Shape &s, const Params &p, int threads
   }
   out.dbf.resize(s.n());
   edt_rule<uint32_t>(out.comp.cc.data(), s, p.aniso, black_border, 3, out.dbf.data());
+  if (p.fix_avocados) {
+    // skeletonize.cu:91-95: the correction (avocado_oracle.hpp), the field refreshed after each
+    // changing pass, then fastremap.renumber with get_mapping's rule (the input label under each
+    // run start of a new id, the last one kept)
+    const std::vector<uint32_t> before = out.comp.cc;
+    oracle_avocado::correct(out.comp.cc, out.dbf, {s.sx, s.sy, s.sz}, k, p.soma_detection,
+                            [&](const std::vector<uint32_t> &lab, std::vector<float> &dist) {
+                              edt_rule<uint32_t>(lab.data(), s, p.aniso, black_border, 3, dist.data());
+                            });
+    std::vector<uint32_t> renum(k, 0);
+    std::vector<i64> mapping{0}, roots;
+    uint32_t prev = UINT32_MAX;
+    for (i64 v = 0; v < s.n(); ++v) {
+      const uint32_t l = out.comp.cc[v];
+      if (l && !renum[l]) { renum[l] = uint32_t(roots.size() + 1); roots.push_back(v); mapping.push_back(0); }
+      const uint32_t nl = l ? renum[l] : 0;
+      if (nl != prev) {
+        if (nl) mapping[nl] = before[v] ? out.comp.mapping[before[v]] : 0;
+        prev = nl;
+      }
+      out.comp.cc[v] = nl;
+    }
+    out.comp.mapping = mapping;
+    out.comp.roots = roots;
+    count = roots.size();
+    k = count + 1;
+    out.boxes = analyze(out.comp.cc, s, count);
+  }
   out.borders.assign(k, {});
   if (p.fix_borders) out.borders = border_targets(out.comp.cc, s, count, p.aniso);
   std::vector<i64> unused;
@@ -1430,8 +1459,9 @@ int64_t oracle_prepare(const int64_t *labels, int64_t sx, int64_t sy, int64_t sz
     p.soma_detection = params[4]; p.soma_acceptance = params[5]; p.soma_scale = params[6]; p.soma_constant = params[7];
     std::copy(aniso, aniso + 3, p.aniso);
     p.dust = dust;
-    p.fix_borders = (fix_borders & 1) != 0;  // bit 0: fix_borders, bit 1: fill_holes
+    p.fix_borders = (fix_borders & 1) != 0;  // bit 0: fix_borders, bit 1: fill_holes, bit 2: fix_avocados
     p.fill_holes = (fix_borders & 2) != 0;
+    p.fix_avocados = (fix_borders & 4) != 0;
     Shape s{sx, sy, sz};
     Prepared pr = prepare(labels, s, p, threads);
     size_t k = pr.comp.roots.size() + 1;
@@ -1484,8 +1514,9 @@ void *oracle_skeletonize(const int64_t *labels, int64_t sx, int64_t sy, int64_t 
     p.dust = dust;
     p.fix_branching = fix_branching > 0;  // 1 fix, 0 parent walks with the STOP rule, -1 full walks
     p.stop_emitted = fix_branching == 0;
-    p.fix_borders = (fix_borders & 1) != 0;  // bit 0: fix_borders, bit 1: fill_holes
+    p.fix_borders = (fix_borders & 1) != 0;  // bit 0: fix_borders, bit 1: fill_holes, bit 2: fix_avocados
     p.fill_holes = (fix_borders & 2) != 0;
+    p.fix_avocados = (fix_borders & 4) != 0;
     std::vector<std::array<i64, 3>> before;
     for (int64_t i = 0; i < ntargets; ++i) before.push_back({targets_before[3 * i], targets_before[3 * i + 1], targets_before[3 * i + 2]});
     Result r = skeletonize(labels, Shape{sx, sy, sz}, p, before, threads);
