@@ -58,7 +58,6 @@ void check_options(const SkeletonizeOptions &o) {
   for (float a : o.aniso)
     if (!(a > 0) || !std::isfinite(a)) throw std::invalid_argument("anisotropy must be finite and positive");
   if (o.voxel_graph) throw Unsupported("voxel_graph is not supported by this build");
-  if (!o.after.empty()) throw Unsupported("extra_targets_after is not supported by this build");
   const double exponent = o.teasar.pdrf_exponent;
   if (std::isfinite(exponent) && exponent == std::floor(exponent) && std::fmod(exponent, 2.0) == 1.0)
     throw std::invalid_argument("odd pdrf_exponent makes the background penalty negative");
@@ -120,8 +119,24 @@ class Sample {
     borders_.assign(k_, {});
     if (o_.fix_borders) borders_ = border_targets(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, count_, o_.aniso);
     lap("prep_borders_ms");
-    before_.assign(k_, {});
-    for (const Point &p : o_.before) before_.at(size_t(component_at(ctx_, cc_, d_, p))).push_back(p);
+    // Each point is filed under the component its wrapped coordinates fall on and traced at the
+    // coordinates as written (skeletonize.cu:59-65). A point whose written coordinates flatten off
+    // the volume or onto another component is rejected: the trace starts at that flat index.
+    auto file_points = [&](const std::vector<Point> &points, std::vector<std::vector<Point>> &out) {
+      out.assign(k_, {});
+      for (const Point &p : points) {
+        const int64_t id = component_at(ctx_, cc_, d_, p);
+        if (id) {
+          const int64_t flat = flat_of(p, d_), plane = int64_t(d_.sx) * d_.sy;
+          const Point at = {flat % d_.sx, (flat / d_.sx) % d_.sy, flat / plane};
+          if (flat < 0 || flat >= n_ || component_at(ctx_, cc_, d_, at) != id)
+            throw std::invalid_argument("extra target's coordinates do not flatten onto its component");
+        }
+        out.at(size_t(id)).push_back(p);
+      }
+    };
+    file_points(o_.before, before_);
+    file_points(o_.after, after_);
 
     // labels to trace (skeletonize.cu:128-133, batched.cu:186-203)
     active_.assign(k_, 0);
@@ -381,7 +396,7 @@ class Sample {
     lap("prep_pdrf_order_ms");
 
     // manual targets and path budgets (lockstep.cu:116-143)
-    std::vector<int> mstart(k_, 0), mcount(k_, 0), mflat, maxp(k_, 0);
+    std::vector<int> mstart(k_, 0), mcount(k_, 0), acount(k_, 0), mflat, maxp(k_, 0);
     for (int id : labels_) {
       std::vector<Point> manual = borders_[id];
       if (!manual.empty()) manual.pop_back();
@@ -391,8 +406,11 @@ class Sample {
       if (manual.empty() && !soma_[id]) mflat.push_back(int(target_[id]));
       else for (auto it = manual.rbegin(); it != manual.rend(); ++it) mflat.push_back(int(flat_of(*it, d_)));
       mcount[id] = int(mflat.size()) - mstart[id];
+      // the targets after the DAF order (per-component tracer, trace.cu:179-199): last given first
+      for (auto it = after_[id].rbegin(); it != after_[id].rend(); ++it) mflat.push_back(int(flat_of(*it, d_)));
+      acount[id] = int(after_[id].size());
       int64_t budget = o_.teasar.max_paths >= 0 ? o_.teasar.max_paths : boxes_[id].count;
-      maxp[id] = mcount[id] >= budget ? 0 : int(std::min<int64_t>(INT32_MAX, budget));
+      maxp[id] = mcount[id] + acount[id] >= budget ? 0 : int(std::min<int64_t>(INT32_MAX, budget));
     }
     mflat.push_back(0);
     mstart_dev_ = Buffer<int>(ctx_, k_);
@@ -401,6 +419,8 @@ class Sample {
     mflat_dev_ = Buffer<int>(ctx_, mflat.size());
     mstart_dev_.upload(ctx_, mstart.data(), k_);
     mcount_dev_.upload(ctx_, mcount.data(), k_);
+    acount_dev_ = Buffer<int>(ctx_, k_);
+    acount_dev_.upload(ctx_, acount.data(), k_);
     maxp_dev_.upload(ctx_, maxp.data(), k_);
     mflat_dev_.upload(ctx_, mflat.data(), mflat.size());
 
@@ -441,6 +461,7 @@ class Sample {
     ta.mstart = mstart_dev_.data();
     ta.mcount = mcount_dev_.data();
     ta.mflat = mflat_dev_.data();
+    ta.acount = acount_dev_.data();
     ta.maxp = maxp_dev_.data();
     for (int i = 0; i < 6; ++i) ta.lists[i] = lists_[i].data();
     ta.snaps[0] = snap_a_.data();
@@ -588,7 +609,7 @@ class Sample {
   size_t k_ = 0;
   std::vector<Box> boxes_;
   std::vector<float> dbf_max_, max_daf_;
-  std::vector<std::vector<Point>> borders_, before_;
+  std::vector<std::vector<Point>> borders_, before_, after_;
   std::vector<uint8_t> active_;
   std::vector<int> labels_;
   std::vector<long long> root_, target_;
@@ -603,7 +624,7 @@ class Sample {
   Buffer<float> free_radius_dev_;   // per id: the soma's free-space radius (0: not a soma); empty: no somata
   Buffer<SomaLabel> soma_dev_;      // per id: soma parameters; empty: no somata
   Buffer<unsigned long long> snap_a_, snap_b_;
-  Buffer<int> mstart_dev_, mcount_dev_, maxp_dev_, mflat_dev_;
+  Buffer<int> mstart_dev_, mcount_dev_, acount_dev_, maxp_dev_, mflat_dev_;
   long long store_cap_ = 0;
   int rec_cap_ = 0;
   Buffer<int> store_, rec_label_, rec_len_, rec_start_, counters_, ejected_;

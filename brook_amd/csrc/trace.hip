@@ -58,7 +58,7 @@ struct Shared {
   int n_far, n_farnxt, n_touched, n_keep;
   int done, target, path_len, found, band;
   long long cursor, seg_end;
-  int pcount;
+  int pcount, ordered, after_used;  // ordered: the current target came from the DAF order
   unsigned lowest;
   float T;
   unsigned long long rail_best, minfar;
@@ -844,6 +844,7 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, const Ball &ball,
 __device__ __forceinline__ int next_target(const TraceArgs &a, Shared &sh, int L) {
   if (threadIdx.x == 0) {
     int pc = sh.pcount;
+    sh.ordered = 0;
     if (pc >= g_ld(a.maxp + L)) sh.target = -1;
     else if (pc < g_ld(a.mcount + L)) sh.target = g_ld(a.mflat + g_ld(a.mstart + L) + pc);
     else sh.target = -2;
@@ -853,11 +854,19 @@ __device__ __forceinline__ int next_target(const TraceArgs &a, Shared &sh, int L
   const int per = 4, window = per * blockDim.x;
   for (;;) {
     long long c = sh.cursor, e = sh.seg_end;
-    if (c >= e) {
+    if (c >= e) {  // the order is spent: the extra targets after it, last given first (trace.cu:197-199)
       __syncthreads();
-      if (threadIdx.x == 0) sh.target = -1;
+      if (threadIdx.x == 0) {
+        const int after = a.acount ? g_ld(a.acount + L) : 0;
+        if (sh.after_used < after) {
+          sh.target = g_ld(a.mflat + g_ld(a.mstart + L) + g_ld(a.mcount + L) + sh.after_used);
+          sh.after_used += 1;
+        } else {
+          sh.target = -1;
+        }
+      }
       __syncthreads();
-      return -1;
+      return sh.target;
     }
     if (threadIdx.x == 0) sh.found = INT32_MAX;
     __syncthreads();
@@ -874,6 +883,7 @@ __device__ __forceinline__ int next_target(const TraceArgs &a, Shared &sh, int L
       if (threadIdx.x == 0) {
         sh.cursor = c + sh.found;
         sh.target = g_ld(a.order + c + sh.found);
+        sh.ordered = 1;
       }
       __syncthreads();
       return sh.target;
@@ -940,7 +950,7 @@ __global__ void __launch_bounds__(kBlock) soma_balls(const TraceArgs *__restrict
   if (threadIdx.x == 0) {
     const long long limit = g_ld(&soma->max_paths);
     const long long budget = limit >= 0 ? limit : (long long)count - sh.n_touched;
-    const int manual = g_ld(a.mcount + L);
+    const int manual = g_ld(a.mcount + L) + (a.acount ? g_ld(a.acount + L) : 0);
     g_st(a.maxp + L, manual >= budget ? 0 : int(budget < INT32_MAX ? budget : INT32_MAX));
   }
 }
@@ -967,6 +977,7 @@ __global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restri
       sh.cursor = seg;
       sh.seg_end = (long long)seg + count;
       sh.pcount = 0;
+      sh.after_used = 0;
       for (int j = 0; j < kProf; ++j) sh.prof[j] = 0;
       sh.tick = wall_clock64();
     }
@@ -1003,7 +1014,7 @@ __global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restri
         // No rail reachable from t (a component that filled holes split): as the reference's
         // per-component tracer does for such a label (trace.cu:198-215), an ordered target ends
         // the label with the paths it has, a manual one counts as an empty path.
-        if (sh.pcount >= g_ld(a.mcount + L)) break;
+        if (sh.ordered) break;
         if (threadIdx.x == 0) sh.path_len = 0;
         __syncthreads();
       } else if (sh.path_len < 0) {

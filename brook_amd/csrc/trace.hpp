@@ -19,7 +19,9 @@ struct TraceArgs {
   const float *dbf = nullptr;
   const int *seg_start = nullptr, *seg_count = nullptr;
   const int *order = nullptr;  // per-label target order (voxels) in segment layout
-  const int *mstart = nullptr, *mcount = nullptr, *mflat = nullptr;
+  // manual targets: label L's are mflat[mstart[L] ..][0, mcount[L]) (before the DAF order), then
+  // acount[L] more (after it; acount null: none)
+  const int *mstart = nullptr, *mcount = nullptr, *mflat = nullptr, *acount = nullptr;
   int *maxp = nullptr;  // path budgets (a soma's is set on the device, after its ball)
   int *lists[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // int scratch, segment layout
   unsigned long long *snaps[2] = {nullptr, nullptr};                       // u64 scratch, segment layout

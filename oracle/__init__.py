@@ -59,7 +59,7 @@ def lib():
         L.oracle_prepare.argtypes = [_P, _I64, _I64, _I64, _P, _P, _D, _I, _I, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P]
         L.oracle_vw_parents.argtypes = [_P, _I64, _I64, _I64, _P, _P, _I64, _P, _P]
         L.oracle_skeletonize.restype = _P
-        L.oracle_skeletonize.argtypes = [_P, _I64, _I64, _I64, _P, _I64, _P, _D, _I, _I, _P, _I64, _I]
+        L.oracle_skeletonize.argtypes = [_P, _I64, _I64, _I64, _P, _I64, _P, _D, _I, _I, _P, _I64, _P, _I64, _I]
         L.oracle_result_sizes.argtypes = [_P, _P]
         L.oracle_result_copy.argtypes = [_P, _P, _P, _P, _P, _P, _P]
         L.oracle_result_free.argtypes = [_P]
@@ -206,7 +206,7 @@ def vw_parents(labels, vw, sources):
 
 def skeletonize(labels, teasar_params=None, anisotropy=(1, 1, 1), dust_threshold=1000, fix_branching=True,
                 fix_borders=True, extra_targets_before=(), threads=1, stop_emitted=True, fill_holes=False,
-                fix_avocados=False):
+                fix_avocados=False, extra_targets_after=()):
     """Packed result dict: labels, v_off, e_off, vertices (V,3) f32, edges (E,2) u32, radii f32, stats.
     fix_branching=False walks the parent field; stop_emitted=False makes every walk run to the root
     (Kimimaro's paths) instead of ending on the first emitted vertex."""
@@ -215,12 +215,12 @@ def skeletonize(labels, teasar_params=None, anisotropy=(1, 1, 1), dust_threshold
     mp = (teasar_params or {}).get("max_paths")
     an = np.asarray(anisotropy, np.float32)
     tb = np.ascontiguousarray(np.asarray(list(extra_targets_before), np.int64).reshape(-1, 3))
+    ta = np.ascontiguousarray(np.asarray(list(extra_targets_after), np.int64).reshape(-1, 3))
     L = lib()
     h = L.oracle_skeletonize(_ptr(a), *shape, _ptr(pv), -1 if mp is None else int(mp), _ptr(an), float(dust_threshold),
                              1 if fix_branching else (0 if stop_emitted else -1),
                              int(bool(fix_borders)) | (2 if fill_holes else 0) | (4 if fix_avocados else 0), _ptr(tb),
-                             len(tb),
-                             threads)
+                             len(tb), _ptr(ta), len(ta), threads)
     if not h:
         _fail("skeletonize")
     try:

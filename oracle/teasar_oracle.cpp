@@ -1150,7 +1150,7 @@ struct Result {
 
 // skeletonize (skeletonize.cu:184-198): prepare, trace every eligible label, assemble and merge.
 Result skeletonize(const i64 *labels, const Shape &s, const Params &p, const std::vector<std::array<i64, 3>> &before,
-                   int threads) {
+                   const std::vector<std::array<i64, 3>> &after_points, int threads) {
   Result res;
   if (!s.n() || double(s.n()) <= p.dust) return res;
   Prepared pr = prepare(labels, s, p, threads);
@@ -1158,17 +1158,25 @@ Result skeletonize(const i64 *labels, const Shape &s, const Params &p, const std
   res.components = i64(count);
   if (!count) return res;
   // manual targets per label (lockstep.cu:116-137)
-  std::vector<std::vector<i64>> manual(k);
-  std::vector<std::vector<std::array<i64, 3>>> extra(k);
-  for (auto pt : before) {
-    std::array<i64, 3> q = pt;
-    i64 shp[3] = {s.sx, s.sy, s.sz};
-    for (int a = 0; a < 3; ++a) {
-      if (q[a] < 0) q[a] += shp[a];
-      if (q[a] < 0 || q[a] >= shp[a]) throw std::invalid_argument("extra target is outside the input volume");
+  std::vector<std::vector<i64>> manual(k), after(k);
+  std::vector<std::vector<std::array<i64, 3>>> extra(k), extra_after(k);
+  auto bucket = [&](const std::vector<std::array<i64, 3>> &points, std::vector<std::vector<std::array<i64, 3>>> &out) {
+    for (auto pt : points) {
+      std::array<i64, 3> q = pt;
+      i64 shp[3] = {s.sx, s.sy, s.sz};
+      for (int a = 0; a < 3; ++a) {
+        if (q[a] < 0) q[a] += shp[a];
+        if (q[a] < 0 || q[a] >= shp[a]) throw std::invalid_argument("extra target is outside the input volume");
+      }
+      const uint32_t id = pr.comp.cc[s.at(q[0], q[1], q[2])];
+      const i64 flat = s.at(pt[0], pt[1], pt[2]);  // traced at the coordinates as written
+      if (id && (flat < 0 || flat >= s.n() || pr.comp.cc[flat] != id))
+        throw std::invalid_argument("extra target's coordinates do not flatten onto its component");
+      out[id].push_back(pt);
     }
-    extra[pr.comp.cc[s.at(q[0], q[1], q[2])]].push_back(pt);
-  }
+  };
+  bucket(before, extra);
+  bucket(after_points, extra_after);
   std::vector<i64> maxp(k, 0);
   std::vector<size_t> labels_list;
   for (size_t id = 1; id < k; ++id) {
@@ -1180,8 +1188,11 @@ Result skeletonize(const i64 *labels, const Shape &s, const Params &p, const std
     m.insert(m.end(), extra[id].begin(), extra[id].end());
     if (m.empty() && !pr.soma[id]) manual[id].push_back(pr.target[id]);
     else for (auto it = m.rbegin(); it != m.rend(); ++it) manual[id].push_back(s.at((*it)[0], (*it)[1], (*it)[2]));
+    // the targets after the DAF order, taken from the back (trace.cu:206-212)
+    for (auto it = extra_after[id].rbegin(); it != extra_after[id].rend(); ++it)
+      after[id].push_back(s.at((*it)[0], (*it)[1], (*it)[2]));
     i64 budget = p.max_paths >= 0 ? p.max_paths : pr.boxes[id].count;
-    maxp[id] = i64(manual[id].size()) >= budget ? 0 : std::min<i64>(INT32_MAX, budget);
+    maxp[id] = i64(manual[id].size() + after[id].size()) >= budget ? 0 : std::min<i64>(INT32_MAX, budget);
   }
   // target order (lockstep_order.cu:21-43): DAF descending (inf and 0 as 0), flat index descending
   std::vector<std::vector<i64>> order(k);
@@ -1233,7 +1244,7 @@ Result skeletonize(const i64 *labels, const Shape &s, const Params &p, const std
       i64 removed = tr.invalidate(L, {pr.root[L]}, float(p.soma_scale), float(p.soma_constant));
       inval_counts[L] += removed;
       i64 budget = p.max_paths >= 0 ? p.max_paths : pr.boxes[L].count - removed;
-      maxp[L] = i64(manual[L].size()) >= budget ? 0 : std::min<i64>(INT32_MAX, budget);
+      maxp[L] = i64(manual[L].size() + after[L].size()) >= budget ? 0 : std::min<i64>(INT32_MAX, budget);
     }
     auto far = [&](i64 v) {
       i64 x, y, z; s.unravel(v, x, y, z);
@@ -1243,20 +1254,23 @@ Result skeletonize(const i64 *labels, const Shape &s, const Params &p, const std
       ss = ss + dz * dz;
       return std::sqrt(ss) > radius;
     };
+    size_t after_used = 0;
     for (;;) {
       if (pcount >= maxp[L]) break;
       i64 t;
+      bool ordered = false;
       if (pcount < i64(manual[L].size())) t = manual[L][pcount];
       else {
         while (cursor < ord.size() && !alive[ord[cursor]]) cursor++;
-        if (cursor >= ord.size()) break;
-        t = ord[cursor];
+        if (cursor < ord.size()) t = ord[cursor], ordered = true;
+        else if (after_used < after[L].size()) t = after[L][after_used++];  // nothing alive is left
+        else break;
       }
       std::vector<i64> path;
       if (p.fix_branching) {
         path = tr.railroad(L, t);
-        if (path.empty()) {  // no rail reachable (trace.cu:198-215): an ordered target ends the label,
-          if (pcount >= i64(manual[L].size())) break;  // a manual one is an empty path
+        if (path.empty()) {  // no rail reachable (trace.cu:218-224): an ordered target ends the label,
+          if (ordered) break;  // a manual one is an empty path
           pcount++;
           continue;
         }
@@ -1503,7 +1517,8 @@ int oracle_vw_parents(const int64_t *labels, int64_t sx, int64_t sy, int64_t sz,
 // handle (null on error).
 void *oracle_skeletonize(const int64_t *labels, int64_t sx, int64_t sy, int64_t sz, const double *params,
                          int64_t max_paths, const float *aniso, double dust, int fix_branching, int fix_borders,
-                         const int64_t *targets_before, int64_t ntargets, int threads) {
+                         const int64_t *targets_before, int64_t ntargets, const int64_t *targets_after,
+                         int64_t nafter, int threads) {
   PackedOut *out = nullptr;
   guarded([&] {
     Params p;
@@ -1519,7 +1534,9 @@ void *oracle_skeletonize(const int64_t *labels, int64_t sx, int64_t sy, int64_t 
     p.fix_avocados = (fix_borders & 4) != 0;
     std::vector<std::array<i64, 3>> before;
     for (int64_t i = 0; i < ntargets; ++i) before.push_back({targets_before[3 * i], targets_before[3 * i + 1], targets_before[3 * i + 2]});
-    Result r = skeletonize(labels, Shape{sx, sy, sz}, p, before, threads);
+    std::vector<std::array<i64, 3>> after;
+    for (int64_t i = 0; i < nafter; ++i) after.push_back({targets_after[3 * i], targets_after[3 * i + 1], targets_after[3 * i + 2]});
+    Result r = skeletonize(labels, Shape{sx, sy, sz}, p, before, after, threads);
     auto o = std::make_unique<PackedOut>();
     for (auto &sk : r.skeletons) {
       o->labels.push_back(sk.label);
