@@ -9,7 +9,8 @@ import os
 import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIBRARY = os.path.join(_HERE, "libbrook_amd.so")
+# BROOK_AMD_LIBRARY names another build of the same sources (brook_amd.tune times its variants so)
+LIBRARY = os.environ.get("BROOK_AMD_LIBRARY") or os.path.join(_HERE, "libbrook_amd.so")
 
 ABI_VERSION = 1
 SUCCESS, INVALID_ARGUMENT, OUT_OF_MEMORY, DEVICE_ERROR, INTERNAL_ERROR = range(5)

@@ -9,6 +9,7 @@
 #include <cmath>
 
 #include "fields.hpp"
+#include "tuned.hpp"
 
 namespace brook {
 namespace {
@@ -91,11 +92,8 @@ __global__ void fill_float(float *p, long long n, float v) {
   if (i < n) p[i] = v;
 }
 
-#ifndef BROOK_FLOOD_BLOCK
-#define BROOK_FLOOD_BLOCK 1024
-#endif
 constexpr int kFloodBlock = BROOK_FLOOD_BLOCK;  // threads per label workgroup
-constexpr int kFloodBatch = kFloodBlock > 512 ? 9 : 26;  // neighbours in flight per thread
+constexpr int kFloodBatch = BROOK_FLOOD_NB_BATCH;  // neighbours in flight per thread
 
 // Relaxes masked neighbours B..E-1 of u (distance du): all loads, then all atomics, in flight
 // together. Returns the neighbours this thread must queue (improved and found not queued).

@@ -22,20 +22,15 @@
 #include <algorithm>
 
 #include "trace.hpp"
+#include "tuned.hpp"
 
 namespace brook {
 namespace {
 
-#ifndef BROOK_TRACE_BLOCK
-#define BROOK_TRACE_BLOCK 1024
-#endif
 constexpr int kBlock = BROOK_TRACE_BLOCK;  // threads per label workgroup
 constexpr unsigned long long kMinfarNone = 0x7f8000007f800000ULL;
 constexpr unsigned long long kClaimed = 0xFFFFFFFFFFFFFFFEULL;
 constexpr int kProf = kTraceProfileSlots;
-#ifndef BROOK_NB_BATCH
-#define BROOK_NB_BATCH 9
-#endif
 // Neighbours whose memory operations one thread keeps in flight together (26: all; 13: two parts;
 // 9: three parts). Fewer in flight, fewer registers.
 constexpr int kNbBatch = BROOK_NB_BATCH;
