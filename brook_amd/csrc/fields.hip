@@ -404,7 +404,8 @@ void prepare_fields(Context &ctx, const std::vector<FloodJob> &jobs, const std::
   Buffer<int> next(ctx, 1);
   next.clear(ctx);
   const int nq = int(q.size());
-  const int grid = std::min(nq, ctx.compute_units() * 2);
+  // every workgroup takes labels from the queue until it is spent, so any grid drains it
+  const int grid = std::min(nq, BROOK_FLOOD_GRID > 0 ? BROOK_FLOOD_GRID : ctx.compute_units() * 2);
   KernelTimer kt(ctx, "prepare_labels");
   prepare_labels<<<grid, kFloodBlock, 0, ctx.stream()>>>(jobs_dev.data(), queue.data(), nq, next.data());
   BROOK_LAUNCHED();

@@ -1071,7 +1071,8 @@ void trace_all(Context &ctx, const std::vector<TraceArgs> &args, const std::vect
   Buffer<int> next(ctx, 1);  // queue cursor
   next.clear(ctx);
   const int nq = int(q.size());
-  const int grid = std::min(nq, ctx.compute_units() * 2);
+  // every workgroup takes labels from the queue until it is spent, so any grid drains it
+  const int grid = std::min(nq, BROOK_TRACE_GRID > 0 ? BROOK_TRACE_GRID : ctx.compute_units() * 2);
   if (!somata.empty()) {
     std::vector<int2> sq(somata.size());
     for (size_t i = 0; i < somata.size(); ++i) sq[i] = make_int2(somata[i].sample, somata[i].label);

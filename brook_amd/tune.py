@@ -1,9 +1,10 @@
 """Autotuner for the launch shapes and frontier chunk sizes of the persistent label kernels.
 
-The flood (`prepare_labels`) and trace (`trace_labels`) kernels take four compile-time shapes:
-threads per label workgroup for each, and how many of a voxel's 26 neighbours one thread keeps in
+The flood (`prepare_labels`) and trace (`trace_labels`) kernels take six compile-time shapes:
+threads per label workgroup for each, how many of a voxel's 26 neighbours one thread keeps in
 flight together (`for_parts` chunks: 9, 13 or 26; more in flight hides more latency and costs
-registers). They set speed only: every value gives bit-identical skeletons. This tuner searches
+registers), and how many label workgroups are resident at once (fewer: less contention for the
+labels on the critical path, later starts for the rest). They set speed only: every value gives bit-identical skeletons. This tuner searches
 them by coordinate descent from the current values in `csrc/tuned.hpp`: for each parameter in
 turn it builds the library with each candidate value (the others at their best so far), times it
 in a child process on the workloads below, rejects a candidate whose results differ from the
@@ -40,6 +41,8 @@ SPACE = {  # parameter -> candidate values, in the order coordinate descent visi
     "BROOK_NB_BATCH": [9, 13, 26],
     "BROOK_FLOOD_BLOCK": [512, 1024],
     "BROOK_FLOOD_NB_BATCH": [9, 13, 26],
+    "BROOK_TRACE_GRID": [0, 64, 128, 256],
+    "BROOK_FLOOD_GRID": [0, 64, 128, 256],
 }
 
 
@@ -120,7 +123,7 @@ def measure(repeats=3):
         for part in (result if name == "batch_ms" else [result]):
             absorb(part)
     out["digest"] = digest.hexdigest()
-    out["device"] = torch.cuda.get_device_name(0)
+    out["device"] = torch.cuda.get_device_properties(0).gcnArchName.split(":")[0]
     print(json.dumps(out), flush=True)
 
 

@@ -28,3 +28,5 @@ def test_search_space_is_what_the_kernels_accept():
         assert all(1 <= v <= 26 for v in tune.SPACE[name])
     for name in ("BROOK_TRACE_BLOCK", "BROOK_FLOOD_BLOCK"):
         assert all(v % 64 == 0 and v <= 1024 for v in tune.SPACE[name])  # whole waves, one workgroup
+    for name in ("BROOK_TRACE_GRID", "BROOK_FLOOD_GRID"):
+        assert all(v >= 0 for v in tune.SPACE[name])  # 0: two per compute unit
