@@ -260,3 +260,22 @@ def test_unsupported_options_raise(brook):
         brook.skeletonize(labels, voxel_graph=np.zeros(labels.shape, np.uint32), dust_threshold=0, progress=False)
     with pytest.raises(ValueError):
         brook.skeletonize(labels, dict(pdrf_exponent=3), dust_threshold=0, progress=False)
+
+
+@pytest.mark.parametrize("fix_branching", [True, False])
+def test_full_workload_equals_oracle_digests(brook, fix_branching):
+    """The 512^3 / 200-label tree workload, device-resident input, against the digests of the
+    oracle's result (tests/golden/workload_digests.py wrote them)."""
+    import torch
+    from brook_amd import synth
+    from tests.golden.workload_digests import PARAMS, digest
+    want = json.load(open(os.path.join(HERE, "golden", "workload_digests.json")))
+    want = want["fix_branching" if fix_branching else "parent_walks"]
+    volume = synth.trees_512()
+    flat = torch.from_numpy(np.ravel(volume, order="F")).to("cuda:0")
+    device = flat.view(volume.shape[2], volume.shape[1], volume.shape[0]).permute(2, 1, 0)
+    got = brook.skeletonize(device, PARAMS, (8, 8, 40), dust_threshold=1000, fix_branching=fix_branching,
+                            output="packed", progress=False).as_dict()
+    got = digest(got)
+    for name, value in got.items():
+        assert value == want[name], name
