@@ -227,12 +227,14 @@ def served(rates=(10.0,), count=24, seed=5005):
                 t0 = time.perf_counter()
                 at = t0
                 futures = []
-                for g, j in zip(gaps, picks):
+                for i, (g, j) in enumerate(zip(gaps, picks)):
                     at += g
                     wait = at - time.perf_counter()
                     if wait > 0:
                         time.sleep(wait)
                     futures.append(srv.submit(pool[j]))
+                    if (i + 1) % 50 == 0:
+                        _progress(f"served {lam:g} req/s, {mode}: {i + 1} of {count} submitted")
                 for f in futures:
                     f.result()
                 _progress(f"served {lam:g} req/s, {mode} done")
@@ -274,6 +276,8 @@ def main():
     ap.add_argument("--served-sweep", action="store_true",
                     help="serve at 1, 2, 5, 10, 20 and 50 requests/s (default: 10 only)")
     ap.add_argument("--served-requests", type=int, default=24, help="requests per arrival rate")
+    ap.add_argument("--served-rates", default="1,2,5,10,20,50",
+                    help="arrival rates (req/s) of --served-sweep, comma-separated")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy three times (the known-size reference "
@@ -382,7 +386,8 @@ def main():
         _progress("batched done")
         out["stress"] = stress()
         _progress("stress done")
-        out["served"] = served((1.0, 2.0, 5.0, 10.0, 20.0, 50.0) if args.served_sweep else (10.0,),
+        rates = tuple(float(r) for r in args.served_rates.split(","))
+        out["served"] = served(rates if args.served_sweep else (10.0,),
                                count=args.served_requests)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(vol, args.cpu_threads)
