@@ -170,3 +170,17 @@ def test_gpu_rejects_targets_off_their_component(brook):
     with pytest.raises(ValueError, match="flatten"):
         brook.skeletonize(np.asfortranarray(stubbed_bar()), PARAMS, dust_threshold=10, extra_targets_after=[(-16, 10, 22)],
                           progress=False)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fix_branching", [True, False])
+def test_gpu_after_targets_on_a_soma(brook, fix_branching):
+    """A soma-sized body with `after` targets (the process tip and a point inside the soma ball):
+    the ball dies first, the targets come once nothing is alive, and their paths are trimmed to
+    the soma radius like the others."""
+    body = targeted_samples()[0][4]
+    for before, after in (([], [(14, 14, 37)]), ([(14, 14, 37)], [(14, 14, 14), (13, 15, 30)])):
+        got = brook.skeletonize(np.asfortranarray(body), PARAMS, dust_threshold=10, fix_branching=fix_branching,
+                                extra_targets_before=before, extra_targets_after=after, output="packed",
+                                progress=False).as_dict()
+        assert_equal_packed(got, oracle_run(body, before, after, fix_branching))
