@@ -184,3 +184,28 @@ def test_gpu_after_targets_on_a_soma(brook, fix_branching):
                                 extra_targets_before=before, extra_targets_after=after, output="packed",
                                 progress=False).as_dict()
         assert_equal_packed(got, oracle_run(body, before, after, fix_branching))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", [11, 12, 13])
+def test_gpu_random_options_equal_oracle(brook, seed):
+    """Tube volumes under random anisotropy, TEASAR constants, budgets and manual targets, in both
+    branching modes, against the oracle."""
+    from brook_amd import synth
+    rng = np.random.default_rng(seed)
+    labels = synth.volume("tubes", (96, 80, 64), int(rng.integers(4, 12)), seed)
+    aniso = tuple(float(a) for a in rng.choice([1.0, 2.0, 4.0, 8.0, 16.0, 40.0], 3))
+    params = dict(scale=float(rng.choice([1.0, 1.5, 3.0])), const=float(rng.choice([2.0, 50.0, 300.0])),
+                  pdrf_scale=100000, pdrf_exponent=int(rng.choice([2, 4, 8])))
+    if rng.random() < 0.5:
+        params["max_paths"] = int(rng.integers(4, 16))
+    fg = np.argwhere(labels > 0)
+    before = [tuple(int(c) for c in fg[i]) for i in rng.choice(len(fg), int(rng.integers(0, 4)), replace=False)]
+    after = [tuple(int(c) for c in fg[i]) for i in rng.choice(len(fg), int(rng.integers(0, 6)), replace=False)]
+    for fix_branching in (True, False):
+        got = brook.skeletonize(labels, params, aniso, dust_threshold=100, fix_branching=fix_branching,
+                                extra_targets_before=before, extra_targets_after=after, output="packed",
+                                progress=False).as_dict()
+        want = oracle.skeletonize(labels, params, aniso, 100, fix_branching=fix_branching,
+                                  extra_targets_before=before, extra_targets_after=after, threads=8)
+        assert_equal_packed(got, want)
