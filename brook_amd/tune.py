@@ -53,9 +53,8 @@ def current():
 
 
 def _flags():
-    sys.path.insert(0, ROOT)
-    from __graft_entry__ import HIP_FLAGS
-    return HIP_FLAGS
+    from brook_amd._flags import HIP_FLAGS
+    return list(HIP_FLAGS)
 
 
 def build(values, out_dir, common_dir, jobs=16):
