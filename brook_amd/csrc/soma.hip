@@ -1,6 +1,6 @@
-// Soma preparation kernels: the enclosed-void check (a background flood of a label's bounding box
-// through face neighbours from the box's faces, one grid-wide round per launch) and the DBF maxima
-// of soma labels (a flagged selection in flat order).
+// Soma preparation kernels: the enclosed-void check (6-connected union-find over the background of
+// a label's bounding box, with the components that touch the box's faces marked) and the DBF
+// maxima of soma labels (a flagged selection in flat order).
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -23,50 +23,93 @@ template <class CC> __device__ bool is_label(const CC *cc, Dims d, const Crop &c
   return cc[(c.lo[0] + x) + (long long)d.sx * ((c.lo[1] + y) + (long long)d.sy * (c.lo[2] + z))] == id;
 }
 
-// Box voxels on a face that are not the label: reached, first front.
+// The box voxels that are not the label, labelled by 6-connected union-find over box-local
+// indices i = x + ex * (y + ey * z): every x-run hooks to its first voxel by a wave ballot, runs
+// touching in the y-1 and z-1 rows are united (larger root -> smaller), the roots of face voxels
+// are marked, and a voxel is reached when its root is marked. No host round trip.
+constexpr int kOut = -1;  // parent of a label voxel (not part of the background mask)
+
 template <class CC>
-__global__ void void_seed(const CC *cc, Dims d, Crop c, CC id, unsigned *reached, int *front, int *count) {
-  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
-  if (i >= c.size()) return;
+__device__ __forceinline__ bool open_at(const CC *cc, Dims d, const Crop &c, long long i, CC id) {
   const int x = int(i % c.ext[0]), y = int((i / c.ext[0]) % c.ext[1]), z = int(i / ((long long)c.ext[0] * c.ext[1]));
-  const bool face = x == 0 || y == 0 || z == 0 || x == c.ext[0] - 1 || y == c.ext[1] - 1 || z == c.ext[2] - 1;
-  if (!face || is_label(cc, d, c, x, y, z, id)) return;
-  g_st(reached + i, 1u);
-  g_st(front + wave_slot(count), int(i));
+  return !is_label(cc, d, c, x, y, z, id);
 }
 
-// One launch of the flood: every front entry explores depth-first from itself (a small stack per
-// thread, a budget of pops), claiming box voxels that are not the label with an exchange; what a
-// full stack or a spent budget cannot hold goes to the next front. Each voxel is claimed once, so
-// a front never exceeds the box.
-constexpr int kDfsStack = 16, kDfsBudget = 512;
 template <class CC>
-__global__ void void_round(const CC *cc, Dims d, Crop c, CC id, unsigned *reached, const int *front, int n, int *next,
-                           int *count) {
-  const int j = blockIdx.x * kThreads + threadIdx.x;
-  if (j >= n) return;
-  const long long sx = c.ext[0], sxy = (long long)c.ext[0] * c.ext[1];
-  int stack[kDfsStack];
-  int sp = 0, budget = kDfsBudget;
-  stack[sp++] = front[j];
-  while (sp > 0) {
-    const long long i = stack[--sp];
-    if (--budget < 0) {  // hand the rest back
-      g_st(next + atomicAdd(count, 1), int(i));
-      continue;
-    }
-    const int x = int(i % c.ext[0]), y = int((i / c.ext[0]) % c.ext[1]), z = int(i / sxy);
-    for (int k = 0; k < 6; ++k) {
-      const int nx = x + (k == 0 ? -1 : k == 1 ? 1 : 0), ny = y + (k == 2 ? -1 : k == 3 ? 1 : 0),
-                nz = z + (k == 4 ? -1 : k == 5 ? 1 : 0);
-      if (nx < 0 || ny < 0 || nz < 0 || nx >= c.ext[0] || ny >= c.ext[1] || nz >= c.ext[2]) continue;
-      if (is_label(cc, d, c, nx, ny, nz, id)) continue;
-      const long long v = nx + sx * ny + sxy * nz;
-      if (g_xchg(reached + v, 1u) != 0u) continue;
-      if (sp < kDfsStack) stack[sp++] = int(v);
-      else g_st(next + atomicAdd(count, 1), int(v));
+__global__ void box_runs(const CC *cc, Dims d, Crop c, CC id, int *parent) {
+  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool valid = i < c.size();
+  const bool open = valid && open_at(cc, d, c, i, id);
+  const int x = valid ? int(i % c.ext[0]) : 0;
+  int prev = shfl_up_any(int(open), 1);
+  if (lane == 0 && valid && x > 0) prev = int(open_at(cc, d, c, i - 1, id));
+  const bool start = open && (x == 0 || !prev);
+  const unsigned long long mask = __ballot(start);
+  if (!valid) return;
+  if (!open) { parent[i] = kOut; return; }
+  const unsigned long long below = mask & lanes_upto(lane);
+  const long long base = i - lane;
+  parent[i] = int(below ? base + (63 - __clzll((long long)below)) : base - 1);  // base - 1: same run
+}
+
+__device__ __forceinline__ int box_root(int *parent, int v) {
+  int cur = ld_i(parent + v);
+  if (cur != v) {
+    int prev = v, next;
+    while (cur > (next = ld_i(parent + cur))) {
+      st_i(parent + prev, next);  // path halving: only ever points closer to the root
+      prev = cur;
+      cur = next;
     }
   }
+  return cur;
+}
+
+__device__ __forceinline__ void box_unite(int *parent, int a, int b) {
+  int va = box_root(parent, a), vb = box_root(parent, b);
+  while (va != vb) {
+    if (va < vb) {
+      const int ret = atomicCAS(parent + vb, vb, va);
+      if (ret == vb) return;
+      vb = box_root(parent, ret);
+    } else {
+      const int ret = atomicCAS(parent + va, va, vb);
+      if (ret == va) return;
+      va = box_root(parent, ret);
+    }
+  }
+}
+
+// Face contacts between runs: voxel x of a run unites with the run at x in the y-1 row and in the
+// z-1 plane where that run begins at x or x is this run's first voxel.
+__global__ void box_union(Crop c, int *parent) {
+  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= c.size() || parent[i] == kOut) return;
+  const int ex = c.ext[0], ey = c.ext[1];
+  const int x = int(i % ex), y = int((i / ex) % ey), z = int(i / ((long long)ex * ey));
+  const bool first = x == 0 || parent[i - 1] == kOut;
+  const long long rows[2] = {y > 0 ? i - ex : -1, z > 0 ? i - (long long)ex * ey : -1};
+  for (int r = 0; r < 2; ++r) {
+    const long long j = rows[r];
+    if (j < 0 || parent[j] == kOut) continue;
+    if (first || parent[j - 1] == kOut) box_unite(parent, int(i), int(j));  // x > 0 when !first
+  }
+}
+
+__global__ void box_mark_faces(Crop c, int *parent, unsigned *reached) {
+  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= c.size() || parent[i] == kOut) return;
+  const int x = int(i % c.ext[0]), y = int((i / c.ext[0]) % c.ext[1]), z = int(i / ((long long)c.ext[0] * c.ext[1]));
+  const bool face = x == 0 || y == 0 || z == 0 || x == c.ext[0] - 1 || y == c.ext[1] - 1 || z == c.ext[2] - 1;
+  if (face) g_st(reached + box_root(parent, int(i)), 1u);
+}
+
+__global__ void box_resolve(Crop c, int *parent, unsigned *reached) {
+  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= c.size() || parent[i] == kOut) return;
+  const int r = box_root(parent, int(i));
+  if (r != int(i) && g_ld(reached + r)) g_st(reached + i, 1u);
 }
 
 // reached[i] = 1 for the box voxels that are not the label and reach a face of the box through
@@ -76,26 +119,14 @@ template <class CC> void reach_faces(Context &ctx, const CC *cc, Dims d, CC id, 
   const long long n = c.size();
   if (n > INT32_MAX) throw std::invalid_argument("label bounding box exceeds int32 capacity");
   reached = Buffer<unsigned>(ctx, size_t(n));
-  Buffer<int> a(ctx, size_t(n)), b(ctx, size_t(n)), counts(ctx, 2);
+  Buffer<int> parent(ctx, size_t(n));
   reached.clear(ctx);
-  counts.clear(ctx);
-  void_seed<CC><<<blocks_for(n), kThreads, 0, s>>>(cc, d, c, id, reached.data(), a.data(), counts.data());
+  const unsigned grid = blocks_for(n);
+  box_runs<CC><<<grid, kThreads, 0, s>>>(cc, d, c, id, parent.data());
+  box_union<<<grid, kThreads, 0, s>>>(c, parent.data());
+  box_mark_faces<<<grid, kThreads, 0, s>>>(c, parent.data(), reached.data());
+  box_resolve<<<grid, kThreads, 0, s>>>(c, parent.data(), reached.data());
   BROOK_LAUNCHED();
-  int front = 0;
-  BROOK_HIP(hipMemcpyAsync(&front, counts.data(), 4, hipMemcpyDeviceToHost, s));
-  ctx.synchronize();
-  int slot = 0;  // counts[slot] holds the current front's length
-  while (front > 0) {
-    const int other = 1 - slot;
-    BROOK_HIP(hipMemsetAsync(counts.data() + other, 0, 4, s));
-    void_round<CC><<<blocks_for(front), kThreads, 0, s>>>(cc, d, c, id, reached.data(), a.data(), front, b.data(),
-                                                          counts.data() + other);
-    BROOK_LAUNCHED();
-    BROOK_HIP(hipMemcpyAsync(&front, counts.data() + other, 4, hipMemcpyDeviceToHost, s));
-    ctx.synchronize();
-    std::swap(a, b);
-    slot = other;
-  }
 }
 
 template <class CC>
