@@ -48,6 +48,15 @@ class SkeletonsView(_c.Structure):
                 ("radii", _P), ("anisotropy", _c.c_float * 3)]
 
 
+class SkeletonsDeviceView(_c.Structure):
+    _fields_ = [("skeleton_count", _c.c_size_t), ("vertex_count", _c.c_size_t), ("edge_count", _c.c_size_t),
+                ("labels", _P), ("vertex_offsets", _P), ("edge_offsets", _P), ("vertices", Volume), ("edges", Volume),
+                ("radii", Volume), ("anisotropy", _c.c_float * 3)]
+
+
+VERTICES, EDGES, RADII = range(3)
+
+
 class DeviceError(RuntimeError):
     """An error reported by the device engine (a HIP error or an engine failure)."""
 
@@ -91,6 +100,8 @@ def lib():
                                                 _c.POINTER(SkeletonizeOptions), _c.POINTER(_P)]),
                 "brook_skeletons_get_view": (S, [_P, _c.POINTER(SkeletonsView)]),
                 "brook_skeletons_destroy": (None, [_P]),
+                "brook_skeletons_get_device_view": (S, [_P, _c.POINTER(SkeletonsDeviceView)]),
+                "brook_skeletons_get_array": (S, [_P, _c.c_int, _c.POINTER(_P)]),
                 "brook_merge_skeletons": (S, [_P, _c.POINTER(_P), _P, _c.c_size_t, _c.POINTER(_P)]),
                 "brook_merge_views": (S, [_P, _c.POINTER(SkeletonsView), _P, _c.c_size_t, _c.POINTER(_P)]),
                 "brook_prepare_fields": (S, [_P, _c.POINTER(Volume), _c.POINTER(SkeletonizeOptions), _P, _P, _P,

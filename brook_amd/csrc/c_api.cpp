@@ -137,6 +137,8 @@ struct brook_components {
 };
 struct brook_skeletons {
   brook::Packed packed;
+  // BROOK_DEVICE_OUTPUT: device copies of vertices (V x 3 f32), edges (E x 2 u32) and radii (V f32)
+  std::shared_ptr<DeviceMemory> vertices, edges, radii;
 };
 
 namespace {
@@ -191,7 +193,6 @@ brook::SkeletonizeOptions convert(const brook_skeletonize_options *p, std::vecto
       for (size_t i = 0; i < p->object_ids_count; ++i) ids->push_back(uint64_t(p->object_ids[i]));
     }
   }
-  if (p->flags & BROOK_DEVICE_OUTPUT) throw brook::Unsupported("BROOK_DEVICE_OUTPUT is not supported by this build");
   return o;
 }
 
@@ -378,6 +379,38 @@ void brook_skeletonize_options_init(brook_skeletonize_options *p) {
   p->flags = BROOK_FIX_BRANCHING | BROOK_FIX_BORDERS;
 }
 
+namespace {
+// BROOK_DEVICE_OUTPUT (brook.h:66, c_api.cpp:314-353): the packed arrays also live in device memory
+// the handle owns; the host arrays stay as the handle's stable snapshot.
+void device_output(brook::Context &ctx, brook_skeletons &r) {
+  auto copy = [&](const void *src, size_t bytes) {
+    auto m = std::make_shared<DeviceMemory>(ctx.device(), bytes);
+    if (bytes) BROOK_HIP(hipMemcpyAsync(m->ptr, src, bytes, hipMemcpyHostToDevice, ctx.stream()));
+    return m;
+  };
+  r.vertices = copy(r.packed.vertices.data(), r.packed.vertices.size() * sizeof(float));
+  r.edges = copy(r.packed.edges.data(), r.packed.edges.size() * sizeof(uint32_t));
+  r.radii = copy(r.packed.radii.data(), r.packed.radii.size() * sizeof(float));
+  ctx.synchronize();
+}
+
+brook_volume device_rows(const DeviceMemory &m, brook_dtype dtype, int64_t columns, int64_t rows) {
+  brook_volume v{};
+  v.struct_size = sizeof(brook_volume);
+  v.abi_version = BROOK_ABI_VERSION;
+  v.data = m.ptr;
+  v.dtype = dtype;
+  v.memory = BROOK_DEVICE;
+  v.shape[0] = columns;  // row-major rows x columns, i.e. columns fastest
+  v.shape[1] = rows;
+  v.shape[2] = 1;
+  v.strides[0] = 4;
+  v.strides[1] = 4 * columns;
+  v.strides[2] = 4 * columns * rows;
+  return v;
+}
+}  // namespace
+
 brook_status brook_skeletonize(brook_context *ctx, const brook_volume *labels, const brook_skeletonize_options *options,
                                brook_skeletons **out) {
   return protect([&] {
@@ -386,6 +419,7 @@ brook_status brook_skeletonize(brook_context *ctx, const brook_volume *labels, c
     brook::ContextCall call(ctx->value);
     auto r = std::make_unique<brook_skeletons>();
     r->packed = run_one(ctx->value, labels, options);
+    if (options->flags & BROOK_DEVICE_OUTPUT) device_output(ctx->value, *r);
     *out = r.release();
   });
 }
@@ -407,6 +441,7 @@ brook_status brook_skeletonize_batch(brook_context *ctx, const brook_volume *con
     for (size_t i = 0; i < count; ++i) {
       auto r = std::make_unique<brook_skeletons>();
       r->packed = std::move(packed[i]);
+      if (options->flags & BROOK_DEVICE_OUTPUT) device_output(ctx->value, *r);
       results.push_back(std::move(r));
     }
     for (size_t i = 0; i < count; ++i) out[i] = results[i].release();
@@ -430,6 +465,43 @@ brook_status brook_skeletons_get_view(const brook_skeletons *r, brook_skeletons_
     std::copy(p.anisotropy.begin(), p.anisotropy.end(), out->anisotropy);
   });
 }
+brook_status brook_skeletons_get_device_view(const brook_skeletons *r, brook_skeletons_device_view *out) {
+  return protect([&] {
+    require(r && out);
+    *out = brook_skeletons_device_view{};
+    if (!r->vertices) throw std::invalid_argument("skeleton result was not requested with BROOK_DEVICE_OUTPUT");
+    const brook::Packed &p = r->packed;
+    out->skeleton_count = p.labels.size();
+    out->vertex_count = p.radii.size();
+    out->edge_count = p.edges.size() / 2;
+    out->labels = p.labels.data();
+    out->vertex_offsets = reinterpret_cast<const int64_t *>(p.v_off.data());
+    out->edge_offsets = reinterpret_cast<const int64_t *>(p.e_off.data());
+    out->vertices = device_rows(*r->vertices, BROOK_F32, 3, int64_t(out->vertex_count));
+    out->edges = device_rows(*r->edges, BROOK_U32, 2, int64_t(out->edge_count));
+    out->radii = device_rows(*r->radii, BROOK_F32, 1, int64_t(out->vertex_count));
+    std::copy(p.anisotropy.begin(), p.anisotropy.end(), out->anisotropy);
+  });
+}
+
+brook_status brook_skeletons_get_array(const brook_skeletons *r, brook_skeleton_buffer buffer, brook_array **out) {
+  return protect([&] {
+    require(r && out);
+    *out = nullptr;
+    if (!r->vertices) throw std::invalid_argument("skeleton result was not requested with BROOK_DEVICE_OUTPUT");
+    const size_t nv = r->packed.radii.size(), ne = r->packed.edges.size() / 2;
+    auto a = std::make_unique<brook_array>();
+    switch (buffer) {  // the handle's storage, shared: the array stays valid after the handle is destroyed
+      case BROOK_VERTICES: a->memory = r->vertices; a->dtype = BROOK_F32; a->shape[0] = 3; a->shape[1] = int64_t(nv); break;
+      case BROOK_EDGES: a->memory = r->edges; a->dtype = BROOK_U32; a->shape[0] = 2; a->shape[1] = int64_t(ne); break;
+      case BROOK_RADII: a->memory = r->radii; a->dtype = BROOK_F32; a->shape[0] = int64_t(nv); a->shape[1] = 1; break;
+      default: throw std::invalid_argument("invalid skeleton buffer");
+    }
+    a->shape[2] = 1;
+    *out = a.release();
+  });
+}
+
 void brook_skeletons_destroy(brook_skeletons *r) { delete r; }
 
 namespace {

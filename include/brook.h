@@ -132,6 +132,20 @@ brook_status brook_skeletonize(brook_context *context, const brook_volume *label
 brook_status brook_skeletons_get_view(const brook_skeletons *result, brook_skeletons_view *out);
 void brook_skeletons_destroy(brook_skeletons *result);
 
+/* brook.h:96-102, 232-233: results requested with BROOK_DEVICE_OUTPUT also hold their vertices
+ * (vertex_count x 3 f32), edges (edge_count x 2 u32) and radii (f32) in device memory, row major;
+ * labels and offsets stay host metadata, and get_view reads the host snapshot. get_array shares
+ * the handle's storage (valid after the handle is destroyed, until the array is). */
+typedef struct brook_skeletons_device_view {
+  size_t skeleton_count, vertex_count, edge_count;
+  const int64_t *labels, *vertex_offsets, *edge_offsets;
+  brook_volume vertices, edges, radii;
+  float anisotropy[3];
+} brook_skeletons_device_view;
+typedef enum brook_skeleton_buffer { BROOK_VERTICES = 0, BROOK_EDGES = 1, BROOK_RADII = 2 } brook_skeleton_buffer;
+brook_status brook_skeletons_get_device_view(const brook_skeletons *result, brook_skeletons_device_view *out);
+brook_status brook_skeletons_get_array(const brook_skeletons *result, brook_skeleton_buffer buffer, brook_array **out);
+
 /* brook.h:234-237: one result from `count` fragments, fragment i offset by origins[3i..3i+2]
  * (count x 3 float32, NULL for zero offsets): one vertex per (label, position), the first
  * occurrence's radius, edges remapped without self-loops or duplicates, vertices on no edge

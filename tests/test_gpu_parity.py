@@ -279,3 +279,50 @@ def test_full_workload_equals_oracle_digests(brook, fix_branching):
     got = digest(got)
     for name, value in got.items():
         assert value == want[name], name
+
+
+def test_device_output_views(brook):
+    """BROOK_DEVICE_OUTPUT through the C ABI: the device view's arrays and get_array hold the host
+    view's bytes; a result requested without the flag refuses both."""
+    import ctypes
+    from brook_amd import _intake, _native as native, synth
+    labels = synth.tubes_128(1001)
+    L = native.lib()
+    ctx = native.context()
+    for flags in (native.DEVICE_OUTPUT, 0):
+        keep = []
+        vol, _ = _intake.volume_struct(labels, keep)
+        opts = _intake.options_struct(CFG1_PARAMS, (16, 16, 40), 1000, True, True, False, False, None, (), (), keep)
+        opts.flags |= flags
+        h = ctypes.c_void_p()
+        native.check(L.brook_skeletonize(ctx.handle, ctypes.byref(vol), ctypes.byref(opts), ctypes.byref(h)))
+        try:
+            host = native.SkeletonsView()
+            native.check(L.brook_skeletons_get_view(h, ctypes.byref(host)))
+            dev = native.SkeletonsDeviceView()
+            status = L.brook_skeletons_get_device_view(h, ctypes.byref(dev))
+            if not flags:
+                assert status == native.INVALID_ARGUMENT
+                arr = ctypes.c_void_p()
+                assert L.brook_skeletons_get_array(h, native.VERTICES, ctypes.byref(arr)) == native.INVALID_ARGUMENT
+                continue
+            native.check(status)
+            nv, ne = host.vertex_count, host.edge_count
+            assert (dev.skeleton_count, dev.vertex_count, dev.edge_count) == (host.skeleton_count, nv, ne) and nv > 0
+            assert dev.vertices.memory == native.DEVICE and tuple(dev.vertices.shape) == (3, nv, 1)
+            assert tuple(dev.edges.shape) == (2, ne, 1) and tuple(dev.radii.shape) == (1, nv, 1)
+            for which, ptr, dtype, count in ((native.VERTICES, host.vertices, np.float32, 3 * nv),
+                                             (native.EDGES, host.edges, np.uint32, 2 * ne),
+                                             (native.RADII, host.radii, np.float32, nv)):
+                want = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(np.ctypeslib.as_ctypes_type(dtype))),
+                                             (count,)).copy()
+                arr = ctypes.c_void_p()
+                native.check(L.brook_skeletons_get_array(h, which, ctypes.byref(arr)))
+                got = np.zeros(count, dtype)
+                try:
+                    native.check(L.brook_array_copy_to_host(arr, got.ctypes.data, got.nbytes))
+                finally:
+                    L.brook_array_destroy(arr)
+                assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), which
+        finally:
+            L.brook_skeletons_destroy(h)
