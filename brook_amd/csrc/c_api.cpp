@@ -528,6 +528,7 @@ brook_skeletons *merged(brook::Context &ctx, const std::vector<brook::PackedView
     for (size_t i = 0; i < parts.size(); ++i) at[i] = {origins[3 * i], origins[3 * i + 1], origins[3 * i + 2]};
   auto r = std::make_unique<brook_skeletons>();
   r->packed = brook::merge_packed(ctx, parts, at);
+  device_output(ctx, *r);  // merges return device-output handles (brook.h:234)
   return r.release();
 }
 }  // namespace

@@ -149,7 +149,8 @@ brook_status brook_skeletons_get_array(const brook_skeletons *result, brook_skel
 /* brook.h:234-237: one result from `count` fragments, fragment i offset by origins[3i..3i+2]
  * (count x 3 float32, NULL for zero offsets): one vertex per (label, position), the first
  * occurrence's radius, edges remapped without self-loops or duplicates, vertices on no edge
- * dropped, labels ascending (packed_merge.cu:88-166). The result is a host-output handle. */
+ * dropped, labels ascending (packed_merge.cu:88-166). The result is a device-output handle
+ * (brook_skeletons_get_device_view), with its host snapshot for get_view. */
 brook_status brook_merge_skeletons(brook_context *context, const brook_skeletons *const *fragments,
                                    const float *origins, size_t count, brook_skeletons **out);
 /* EXTENSION: brook_merge_skeletons over borrowed views (packed results from any source): labels and

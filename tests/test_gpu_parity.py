@@ -324,5 +324,13 @@ def test_device_output_views(brook):
                 finally:
                     L.brook_array_destroy(arr)
                 assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), which
+            frags, m = (ctypes.c_void_p * 2)(h, h), ctypes.c_void_p()  # merges return device-output handles
+            native.check(L.brook_merge_skeletons(ctx.handle, frags, None, 2, ctypes.byref(m)))
+            try:
+                merged = native.SkeletonsDeviceView()
+                native.check(L.brook_skeletons_get_device_view(m, ctypes.byref(merged)))
+                assert merged.vertex_count == nv and merged.skeleton_count == host.skeleton_count
+            finally:
+                L.brook_skeletons_destroy(m)
         finally:
             L.brook_skeletons_destroy(h)
