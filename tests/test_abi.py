@@ -64,3 +64,29 @@ def test_synthetic_volumes_are_deterministic():
     a = synth.volume("tubes", (48, 40, 32), 5, 7)
     b = synth.volume("tubes", (48, 40, 32), 5, 7)
     assert np.array_equal(a, b) and a.max() == 5 and a.flags.f_contiguous
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """The ctypes mirrors have the sizes and field offsets a C compiler gives the header's structs."""
+    import subprocess
+    from brook_amd import _native
+    structs = {"brook_volume": _native.Volume, "brook_skeletonize_options": _native.SkeletonizeOptions,
+               "brook_skeletons_view": _native.SkeletonsView,
+               "brook_skeletons_device_view": _native.SkeletonsDeviceView}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "brook.h"', 'int main(void) {']
+    for name, mirror in structs.items():
+        lines.append(f'  printf("{name} size %zu\\n", sizeof({name}));')
+        for field, _ in mirror._fields_:
+            lines.append(f'  printf("{name} {field} %zu\\n", offsetof({name}, {field}));')
+    lines += ['  return 0;', '}']
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    include = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.run(["gcc", "-std=c11", "-I", include, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(line.split()[:2]): int(line.split()[2]) for line in out if line}
+    for name, mirror in structs.items():
+        assert got[(name, "size")] == ctypes.sizeof(mirror), name
+        for field, _ in mirror._fields_:
+            assert got[(name, field)] == getattr(mirror, field).offset, (name, field)
