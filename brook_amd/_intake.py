@@ -145,16 +145,40 @@ def _torch_dtype_code(t):
     return _TORCH_DTYPES[t.dtype]
 
 
-def volume_struct(labels, keep):
-    """brook_volume describing host or device labels (the object itself is kept alive in `keep`)."""
+def stream_handle(stream, tensor=None):
+    """The producer stream device input waits for (the reference's stream argument,
+    bindings.cpp:105-120): a torch.cuda.Stream, an integer handle (0 or 1: the null stream, 2: the
+    per-thread default stream, else a hipStream_t), or None: torch's current stream on the
+    tensor's device."""
+    if stream is None:
+        import torch
+        device = tensor.device if tensor is not None else None
+        return int(torch.cuda.current_stream(device).cuda_stream)
+    handle = getattr(stream, "cuda_stream", stream)
+    handle = int(handle)
+    return 0 if handle == 1 else handle
+
+
+def fence(ctx, handle):
+    """The context's later work waits for the work queued so far on the producer stream."""
+    native.check(native.lib().brook_context_wait_stream(ctx.handle, ctypes.c_void_p(handle or None)))
+
+
+def volume_struct(labels, keep, streams=None):
+    """brook_volume describing host or device labels (the object itself is kept alive in `keep`).
+    For device labels the producer stream to wait for is appended to `streams` (see
+    stream_handle); the caller fences the engine's context on it before the call."""
     v = native.Volume()
     v.struct_size = ctypes.sizeof(native.Volume)
     v.abi_version = native.ABI_VERSION
     if _is_device(labels):
-        import torch
         t = _device_tensor(labels)
         code = _torch_dtype_code(t)
-        torch.cuda.current_stream(t.device).synchronize()  # device input must be ready
+        if streams is None:
+            import torch
+            torch.cuda.current_stream(t.device).synchronize()  # no fence requested: input made ready here
+        else:
+            streams.append(t)
         keep.append(t)
         v.data = t.data_ptr() if t.numel() else None
         item = t.element_size()
@@ -201,7 +225,9 @@ def skeletonize(all_labels, teasar_params=DEFAULT_TEASAR_PARAMS, anisotropy=(1, 
 
     Returns `{label: skeleton}` (osteoid.Skeleton when osteoid is installed) by default,
     `PackedSkeletons` for output="packed" (host arrays) or "packed_device" (torch tensors on the GPU).
-    `parallel`, `parallel_chunk_size`, `in_place`, `stream`, `borrow` and `progress` are accepted for
+    Device input produced on another stream is fenced, not host-synchronized: the engine waits for
+    `stream` (default: torch's current stream on the tensor's device; see stream_handle).
+    `parallel`, `parallel_chunk_size`, `in_place`, `borrow` and `progress` are accepted for
     compatibility. Keys left out of `teasar_params` take kimimaro.trace's defaults."""
     if output not in ("skeletons", "packed", "packed_device"):
         raise ValueError("brook: output must be 'skeletons', 'packed' or 'packed_device'")
@@ -210,13 +236,15 @@ def skeletonize(all_labels, teasar_params=DEFAULT_TEASAR_PARAMS, anisotropy=(1, 
     if streaming:
         raise NotImplementedError("streaming is not supported by this build (in-core on 288 GB of HBM)")
     anisotropy = tuple(float(np.float32(a)) for a in anisotropy)
-    keep = []
-    vol, size = volume_struct(all_labels, keep)
+    keep, devices = [], []
+    vol, size = volume_struct(all_labels, keep, devices)
     if size <= dust_threshold:
         return _empty(output, anisotropy)
     opts = options_struct(teasar_params, anisotropy, dust_threshold, fix_branching, fix_borders, fill_holes,
                           fix_avocados, object_ids, extra_targets_before, extra_targets_after, keep)
-    ctx = native.context()
+    ctx = native.context(devices[0].device.index if devices else None)
+    if devices:
+        fence(ctx, stream_handle(stream, devices[0]))
     h = ctypes.c_void_p()
     native.check(native.lib().brook_skeletonize(ctx.handle, ctypes.byref(vol), ctypes.byref(opts), ctypes.byref(h)))
     packed = _packed_from_handle(h)

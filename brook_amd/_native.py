@@ -107,6 +107,9 @@ def lib():
                 "brook_prepare_fields": (S, [_P, _c.POINTER(Volume), _c.POINTER(SkeletonizeOptions), _P, _P, _P,
                                              _c.c_size_t, _P, _P, _P, _P, _P, _P]),
                 "brook_context_stat": (S, [_P, _c.c_char_p, _c.POINTER(_c.c_double)]),
+                "brook_context_wait_stream": (S, [_P, _P]),
+                "brook_array_to_dlpack": (S, [_P, _c.POINTER(_P)]),
+                "brook_array_to_dlpack_versioned": (S, [_P, _c.c_uint32, _c.POINTER(_P)]),
             }
             for name, (res, args) in sig.items():
                 fn = getattr(L, name)

@@ -5,7 +5,7 @@ from __future__ import annotations
 import ctypes
 
 from . import _native as native
-from ._intake import DEFAULT_TEASAR_PARAMS, _packed_from_handle, options_struct, volume_struct
+from ._intake import DEFAULT_TEASAR_PARAMS, _packed_from_handle, fence, options_struct, stream_handle, volume_struct
 from ._packed import BatchedSkeletons, PackedSkeletons
 
 
@@ -31,17 +31,20 @@ def _per_sample(entries, count, name):
 
 
 def batch_parts(items, teasar_params, anisotropy, dust_threshold, fix_branching=True, fix_borders=True,
-                fill_holes=False, fix_avocados=False, before=None, after=None):
+                fill_holes=False, fix_avocados=False, before=None, after=None, stream=None):
     """One PackedSkeletons per sample (host arrays): samples without per-sample targets share one
-    engine batch (one flood and one trace launch per group); otherwise one engine call each."""
+    engine batch (one flood and one trace launch per group); otherwise one engine call each.
+    Device samples are fenced on `stream` (see _intake.stream_handle)."""
     before = before or [[] for _ in items]
     after = after or [[] for _ in items]
-    keep = []
-    ctx = native.context()
+    keep, devices = [], []
+    vols = [volume_struct(s, keep, devices)[0] for s in items]
+    ctx = native.context(devices[0].device.index if devices else None)
+    for handle in sorted({stream_handle(stream, t) for t in devices}):
+        fence(ctx, handle)
     if all(not b for b in before) and all(not a for a in after):
         if not items:
             return []
-        vols = [volume_struct(s, keep)[0] for s in items]
         opts = options_struct(teasar_params, anisotropy, dust_threshold, fix_branching, fix_borders, fill_holes,
                               fix_avocados, None, (), (), keep)
         arr = (ctypes.POINTER(native.Volume) * len(vols))(*[ctypes.pointer(v) for v in vols])
@@ -49,8 +52,7 @@ def batch_parts(items, teasar_params, anisotropy, dust_threshold, fix_branching=
         native.check(native.lib().brook_skeletonize_batch(ctx.handle, arr, len(vols), ctypes.byref(opts), handles))
         return [_packed_from_handle(ctypes.c_void_p(h)) for h in handles]
     parts = []
-    for s, b, a in zip(items, before, after):
-        vol, _ = volume_struct(s, keep)
+    for vol, b, a in zip(vols, before, after):
         opts = options_struct(teasar_params, anisotropy, dust_threshold, fix_branching, fix_borders, fill_holes,
                               fix_avocados, None, b, a, keep)
         h = ctypes.c_void_p()
@@ -75,7 +77,7 @@ def skeletonize_batch(samples, teasar_params=DEFAULT_TEASAR_PARAMS, anisotropy=(
     after = _per_sample(extra_targets_after, len(items), "extra_targets_after")
     anisotropy = tuple(float(a) for a in anisotropy)
     parts = batch_parts(items, teasar_params, anisotropy, dust_threshold, fix_branching, fix_borders, fill_holes,
-                        fix_avocados, before, after)
+                        fix_avocados, before, after, stream)
     parts = [p if len(p) else PackedSkeletons.from_skeletons({}, anisotropy) for p in parts]
     if output == "skeletons":
         return [p.to_skeletons() for p in parts]

@@ -2,6 +2,7 @@
 // engine (the conventions of the reference's c_api.cpp:17-41, 218-277).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -76,6 +77,20 @@ void check_volume(const brook_volume *v) {
   require(v->memory == BROOK_HOST || v->memory == BROOK_DEVICE, "invalid memory kind");
 }
 
+// Whether the strides lay the volume out densely from `data` upwards in some axis order (every
+// byte of the n * b span belongs to one voxel, positive strides).
+bool dense(const brook_volume *v, int b) {
+  int axes[3] = {0, 1, 2};
+  std::sort(axes, axes + 3, [&](int p, int q) { return v->strides[p] < v->strides[q]; });
+  int64_t pitch = b;
+  for (int a : axes) {
+    if (v->shape[a] <= 1) continue;
+    if (v->strides[a] != pitch) return false;
+    pitch *= v->shape[a];
+  }
+  return true;
+}
+
 // The volume as Fortran-ordered device memory: borrowed when it already is, else copied.
 struct Input {
   brook::LabelVolume view;
@@ -98,11 +113,18 @@ Input normalize(brook::Context &ctx, const brook_volume *v) {
     return in;
   }
   in.holder = std::make_unique<brook::Buffer<uint8_t>>(ctx, size_t(n) * b);
+  const long long strides[3] = {v->strides[0], v->strides[1], v->strides[2]};
   if (v->memory == BROOK_DEVICE) {
-    const long long strides[3] = {v->strides[0], v->strides[1], v->strides[2]};
     brook::gather_fortran(ctx, v->data, strides, b, in.view.dims, in.holder->data());
   } else if (fortran) {
     BROOK_HIP(hipMemcpyAsync(in.holder->data(), v->data, size_t(n) * b, hipMemcpyHostToDevice, ctx.stream()));
+  } else if (dense(v, b)) {
+    // a dense host layout in another axis order (C order included): its bytes go up as they are
+    // and are gathered into Fortran order on the GPU (layout.cu:33-61)
+    brook::Buffer<uint8_t> staged(ctx, size_t(n) * b);
+    BROOK_HIP(hipMemcpyAsync(staged.data(), v->data, size_t(n) * b, hipMemcpyHostToDevice, ctx.stream()));
+    brook::gather_fortran(ctx, staged.data(), strides, b, in.view.dims, in.holder->data());
+    ctx.synchronize();
   } else {
     std::vector<uint8_t> packed(size_t(n) * b);
     const char *base = static_cast<const char *>(v->data);
@@ -126,10 +148,13 @@ struct brook_context {
   brook::Context value;
   explicit brook_context(int device) : value(device) {}
 };
+// A device array: a volume (shape, Fortran order), or the rows of a skeleton buffer (rows x
+// width, row major; width 1: a vector), as the reference's tensor views (packed.cpp:7-14).
 struct brook_array {
   std::shared_ptr<DeviceMemory> memory;
   brook_dtype dtype = BROOK_U8;
   int64_t shape[3] = {0, 0, 0};
+  int width = 0;  // > 0: rows of `width` elements (shape[0] rows)
 };
 struct brook_components {
   brook_array labels;
@@ -142,6 +167,82 @@ struct brook_skeletons {
 };
 
 namespace {
+
+// The logical dimensions of an array: element counts and byte strides (ndim entries; the view's
+// remaining axes have extent 1 and the array's byte size as stride, as the reference's
+// Array::view, runtime.cpp:81-95).
+struct Dims3 {
+  int ndim = 3;
+  int64_t shape[3] = {0, 0, 0}, strides[3] = {0, 0, 0};
+};
+Dims3 logical(const brook_array &a) {
+  const int64_t b = dtype_bytes(a.dtype);
+  Dims3 d;
+  if (a.width > 0) {
+    d.ndim = a.width == 1 ? 1 : 2;
+    d.shape[0] = a.shape[0];
+    d.strides[0] = b * a.width;
+    d.shape[1] = a.width;
+    d.strides[1] = b;
+  } else {
+    int64_t pitch = b;
+    for (int i = 0; i < 3; ++i) { d.shape[i] = a.shape[i]; d.strides[i] = pitch; pitch *= a.shape[i]; }
+  }
+  return d;
+}
+brook_volume view_of_array(const brook_array &a) {
+  brook_volume out{};
+  out.struct_size = sizeof(brook_volume);
+  out.abi_version = BROOK_ABI_VERSION;
+  out.data = a.memory->ptr;
+  out.dtype = a.dtype;
+  out.memory = BROOK_DEVICE;
+  const Dims3 d = logical(a);
+  for (int i = 0; i < 3; ++i) {
+    out.shape[i] = i < d.ndim ? d.shape[i] : 1;
+    out.strides[i] = i < d.ndim ? d.strides[i] : int64_t(a.memory->bytes);
+  }
+  return out;
+}
+// Rows of `width` elements in device memory a skeleton result owns, as a shared array.
+brook_array rows_array(const std::shared_ptr<DeviceMemory> &m, brook_dtype dtype, int width, int64_t rows) {
+  brook_array a;
+  a.memory = m;
+  a.dtype = dtype;
+  a.width = width;
+  a.shape[0] = rows;
+  a.shape[1] = width;
+  a.shape[2] = 1;
+  return a;
+}
+
+// DLPack export of an array: the tensor owns a reference to the storage (brook.h:163-167).
+template <class Tensor> struct Export {
+  Tensor tensor{};
+  std::shared_ptr<DeviceMemory> storage;
+  int64_t shape[3] = {0, 0, 0}, strides[3] = {0, 0, 0};
+};
+template <class Tensor> void release_export(Tensor *t) noexcept { delete static_cast<Export<Tensor> *>(t->manager_ctx); }
+template <class Tensor> Tensor *export_array(const brook_array &a) {
+  auto e = std::make_unique<Export<Tensor>>();
+  e->storage = a.memory;
+  const Dims3 d = logical(a);
+  const int64_t b = dtype_bytes(a.dtype);
+  for (int i = 0; i < d.ndim; ++i) { e->shape[i] = d.shape[i]; e->strides[i] = d.strides[i] / b; }
+  e->tensor.manager_ctx = e.get();
+  e->tensor.deleter = release_export<Tensor>;
+  DLTensor &t = e->tensor.dl_tensor;
+  t.data = a.memory->ptr;
+  t.device = DLDevice{kDLROCM, int32_t(a.memory->device)};
+  t.ndim = d.ndim;
+  const bool is_float = a.dtype == BROOK_F32 || a.dtype == BROOK_F64;
+  const bool is_signed = a.dtype >= BROOK_I8 && a.dtype <= BROOK_I64;
+  t.dtype = DLDataType{uint8_t(is_float ? kDLFloat : is_signed ? kDLInt : kDLUInt), uint8_t(b * 8), 1};
+  t.shape = e->shape;
+  t.strides = e->strides;
+  t.byte_offset = 0;
+  return &e.release()->tensor;
+}
 
 brook_array *owned_array(brook::Context &ctx, const void *src, size_t bytes, brook_dtype dtype, const int64_t shape[3]) {
   auto a = std::make_unique<brook_array>();
@@ -258,25 +359,37 @@ void brook_array_destroy(brook_array *array) { delete array; }
 brook_status brook_array_view(const brook_array *a, brook_volume *out) {
   return protect([&] {
     require(a && out);
-    *out = brook_volume{};
-    out->struct_size = sizeof(brook_volume);
-    out->abi_version = BROOK_ABI_VERSION;
-    out->data = a->memory->ptr;
-    out->dtype = a->dtype;
-    out->memory = BROOK_DEVICE;
-    int b = dtype_bytes(a->dtype);
-    std::copy(a->shape, a->shape + 3, out->shape);
-    out->strides[0] = b;
-    out->strides[1] = b * a->shape[0];
-    out->strides[2] = b * a->shape[0] * a->shape[1];
+    *out = view_of_array(*a);
   });
 }
 brook_status brook_array_copy_to_host(const brook_array *a, void *out, size_t bytes) {
   return protect([&] {
-    require(a && (out || !bytes));
-    require(bytes == a->memory->bytes, "host buffer size does not match the array");
-    (void)hipSetDevice(a->memory->device);
-    if (bytes) BROOK_HIP(hipMemcpy(out, a->memory->ptr, bytes, hipMemcpyDeviceToHost));
+    require(a != nullptr);
+    const size_t size = a->memory->bytes;
+    require(bytes >= size && (out || !size), "output buffer is too small");
+    int previous = 0;
+    BROOK_HIP(hipGetDevice(&previous));
+    BROOK_HIP(hipSetDevice(a->memory->device));
+    const hipError_t e = size ? hipMemcpy(out, a->memory->ptr, size, hipMemcpyDeviceToHost) : hipSuccess;
+    (void)hipSetDevice(previous);
+    BROOK_HIP(e);
+  });
+}
+brook_status brook_array_to_dlpack(const brook_array *a, DLManagedTensor **out) {
+  return protect([&] {
+    require(a && out);
+    *out = nullptr;
+    *out = export_array<DLManagedTensor>(*a);
+  });
+}
+brook_status brook_array_to_dlpack_versioned(const brook_array *a, uint32_t max_minor, DLManagedTensorVersioned **out) {
+  return protect([&] {
+    require(a && out);
+    *out = nullptr;
+    DLManagedTensorVersioned *t = export_array<DLManagedTensorVersioned>(*a);
+    t->version = DLPackVersion{1, std::min<uint32_t>(max_minor, DLPACK_MINOR_VERSION)};
+    t->flags = 0;
+    *out = t;
   });
 }
 
@@ -320,7 +433,7 @@ brook_status brook_components_view(const brook_components *c, brook_volume *labe
   });
 }
 brook_status brook_components_copy_to_host(const brook_components *c, void *out, size_t bytes) {
-  require(c != nullptr);
+  if (!c) return protect([] { require(false); });
   return brook_array_copy_to_host(&c->labels, out, bytes);
 }
 brook_status brook_components_labels(const brook_components *c, brook_array **out) {
@@ -394,20 +507,10 @@ void device_output(brook::Context &ctx, brook_skeletons &r) {
   ctx.synchronize();
 }
 
-brook_volume device_rows(const DeviceMemory &m, brook_dtype dtype, int64_t columns, int64_t rows) {
-  brook_volume v{};
-  v.struct_size = sizeof(brook_volume);
-  v.abi_version = BROOK_ABI_VERSION;
-  v.data = m.ptr;
-  v.dtype = dtype;
-  v.memory = BROOK_DEVICE;
-  v.shape[0] = columns;  // row-major rows x columns, i.e. columns fastest
-  v.shape[1] = rows;
-  v.shape[2] = 1;
-  v.strides[0] = 4;
-  v.strides[1] = 4 * columns;
-  v.strides[2] = 4 * columns * rows;
-  return v;
+// Rows x width elements, row major, as (rows, width, 1) with byte strides (4 * width, 4, bytes)
+// (the reference's row_view, packed.cpp:7-14); width 1: (rows, 1, 1), strides (4, bytes, bytes).
+brook_volume device_rows(const std::shared_ptr<DeviceMemory> &m, brook_dtype dtype, int width, int64_t rows) {
+  return view_of_array(rows_array(m, dtype, width, rows));
 }
 }  // namespace
 
@@ -477,9 +580,9 @@ brook_status brook_skeletons_get_device_view(const brook_skeletons *r, brook_ske
     out->labels = p.labels.data();
     out->vertex_offsets = reinterpret_cast<const int64_t *>(p.v_off.data());
     out->edge_offsets = reinterpret_cast<const int64_t *>(p.e_off.data());
-    out->vertices = device_rows(*r->vertices, BROOK_F32, 3, int64_t(out->vertex_count));
-    out->edges = device_rows(*r->edges, BROOK_U32, 2, int64_t(out->edge_count));
-    out->radii = device_rows(*r->radii, BROOK_F32, 1, int64_t(out->vertex_count));
+    out->vertices = device_rows(r->vertices, BROOK_F32, 3, int64_t(out->vertex_count));
+    out->edges = device_rows(r->edges, BROOK_U32, 2, int64_t(out->edge_count));
+    out->radii = device_rows(r->radii, BROOK_F32, 1, int64_t(out->vertex_count));
     std::copy(p.anisotropy.begin(), p.anisotropy.end(), out->anisotropy);
   });
 }
@@ -489,16 +592,14 @@ brook_status brook_skeletons_get_array(const brook_skeletons *r, brook_skeleton_
     require(r && out);
     *out = nullptr;
     if (!r->vertices) throw std::invalid_argument("skeleton result was not requested with BROOK_DEVICE_OUTPUT");
-    const size_t nv = r->packed.radii.size(), ne = r->packed.edges.size() / 2;
-    auto a = std::make_unique<brook_array>();
-    switch (buffer) {  // the handle's storage, shared: the array stays valid after the handle is destroyed
-      case BROOK_VERTICES: a->memory = r->vertices; a->dtype = BROOK_F32; a->shape[0] = 3; a->shape[1] = int64_t(nv); break;
-      case BROOK_EDGES: a->memory = r->edges; a->dtype = BROOK_U32; a->shape[0] = 2; a->shape[1] = int64_t(ne); break;
-      case BROOK_RADII: a->memory = r->radii; a->dtype = BROOK_F32; a->shape[0] = int64_t(nv); a->shape[1] = 1; break;
+    const int64_t nv = int64_t(r->packed.radii.size()), ne = int64_t(r->packed.edges.size() / 2);
+    // the handle's storage, shared: the array stays valid after the handle is destroyed
+    switch (buffer) {
+      case BROOK_VERTICES: *out = new brook_array(rows_array(r->vertices, BROOK_F32, 3, nv)); break;
+      case BROOK_EDGES: *out = new brook_array(rows_array(r->edges, BROOK_U32, 2, ne)); break;
+      case BROOK_RADII: *out = new brook_array(rows_array(r->radii, BROOK_F32, 1, nv)); break;
       default: throw std::invalid_argument("invalid skeleton buffer");
     }
-    a->shape[2] = 1;
-    *out = a.release();
   });
 }
 
@@ -602,6 +703,19 @@ brook_status brook_context_stat(const brook_context *ctx, const char *name, doub
     require(ctx && name && value);
     auto it = ctx->value.stats.find(name);
     *value = it == ctx->value.stats.end() ? 0.0 : it->second;
+  });
+}
+
+brook_status brook_context_wait_stream(brook_context *ctx, void *stream) {
+  return protect([&] {
+    require(ctx != nullptr);
+    brook::ContextCall call(ctx->value);
+    hipEvent_t event = nullptr;
+    BROOK_HIP(hipEventCreateWithFlags(&event, hipEventDisableTiming));
+    hipError_t e = hipEventRecord(event, static_cast<hipStream_t>(stream));
+    if (e == hipSuccess) e = hipStreamWaitEvent(ctx->value.stream(), event, 0);
+    (void)hipEventDestroy(event);
+    BROOK_HIP(e);
   });
 }
 

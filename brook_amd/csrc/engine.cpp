@@ -247,7 +247,8 @@ class Sample {
     fill_value(ctx_, daf_.data(), n_, kInf);
     rec_ = Buffer<VoxelRec>(ctx_, size_t(n_));  // 16 bytes per voxel: PDRF, neighbour mask + flags, distance word
     voxel_records(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, rec_.data());
-    for (int i = 0; i < 6; ++i) lists_.emplace_back(ctx_, size_t(std::max<int64_t>(F_, 1)));
+    // lists 2 and 3 are the railroad's far lists: two entries per voxel (trace.hip far_compact)
+    for (int i = 0; i < 6; ++i) lists_.emplace_back(ctx_, size_t(std::max<int64_t>(F_, 1)) * (i == 2 || i == 3 ? 2 : 1));
     if (!o_.fix_branching) parent_ = Buffer<int>(ctx_, size_t(n_));
   }
 
@@ -432,7 +433,7 @@ class Sample {
     rec_label_ = Buffer<int>(ctx_, rec_cap_);
     rec_len_ = Buffer<int>(ctx_, rec_cap_);
     rec_start_ = Buffer<int>(ctx_, rec_cap_);
-    counters_ = Buffer<int>(ctx_, 8);
+    counters_ = Buffer<int>(ctx_, 16);
     ejected_ = Buffer<int>(ctx_, 64);
     counters_.clear(ctx_);
     label_stats_ = Buffer<unsigned long long>(ctx_, size_t(kTraceProfileSlots) * k_);
@@ -491,6 +492,8 @@ class Sample {
   void finish(Lap &lap, bool profile) {
     if (done_) return;
     std::vector<int> cnt = counters_.download(ctx_);
+    if (cnt[8] & 1) throw std::runtime_error("trace did not converge: a railroad or an invalidation exceeded its round budget");
+    if (cnt[8] & 2) throw std::runtime_error("trace far list overflowed its capacity");
     if (cnt[3]) throw std::runtime_error("path store capacity exceeded");
     if (cnt[2]) throw Unsupported("a label has no reachable rail (the per-component fallback tracer is not part of this build)");
     ctx_.stats["paths"] += cnt[1];

@@ -154,11 +154,14 @@ __device__ __forceinline__ unsigned flood_span(VoxelRec *rec, int u, unsigned m,
 // Label-correcting rounds: a voxel is dequeued by one atomic that also returns its distance, and
 // an improvement that finds the voxel not queued queues it, so no update is ever lost.
 // The front's words hold their start distances (queued) on entry; counts[0] is its length.
-__device__ __forceinline__ void flood_rounds(VoxelRec *rec, Dims d, const float *W, const int *off, int *cur, int *nxt,
-                                             int *counts /*LDS [2]*/) {
-  for (;;) {
+// Returns false when the rounds exceed `budget` (far above what the label-correcting rounds need:
+// a logic error then ends as a host-visible error instead of a hang).
+__device__ __forceinline__ bool flood_rounds(VoxelRec *rec, Dims d, const float *W, const int *off, int *cur, int *nxt,
+                                             int *counts /*LDS [2]*/, long long budget) {
+  for (long long round = 0;; ++round) {
     const int nc = counts[0];
     if (nc == 0) break;
+    if (round >= budget) return false;  // uniform: counts[0] and round are the same in every thread
     if (nc <= kFloodBlock / 4) {  // a group of lanes per front voxel; its first lane dequeues
       const Group q = group_of(nc, kFloodBlock);
       int u = 0;
@@ -195,10 +198,14 @@ __device__ __forceinline__ void flood_rounds(VoxelRec *rec, Dims d, const float 
     int *t = cur; cur = nxt; nxt = t;
     __syncthreads();
   }
+  return true;
 }
 
-__device__ __forceinline__ void flood_label(VoxelRec *rec, Dims d, int src, const float *W, const int *off, int *cur, int *nxt,
-                                            int *counts /*LDS [2]*/) {
+// Rounds one flood of a label of `count` voxels may take (see flood_rounds).
+__device__ __forceinline__ long long flood_budget(int count) { return 4LL * count + 4096; }
+
+__device__ __forceinline__ bool flood_label(VoxelRec *rec, Dims d, int src, const float *W, const int *off, int *cur,
+                                            int *nxt, int *counts /*LDS [2]*/, int count) {
   if (threadIdx.x == 0) {
     sp_u64(&rec[src].word, 0ULL);  // distance 0, queued
     g_st(cur, src);
@@ -206,7 +213,7 @@ __device__ __forceinline__ void flood_label(VoxelRec *rec, Dims d, int src, cons
     counts[1] = 0;
   }
   __syncthreads();
-  flood_rounds(rec, d, W, off, cur, nxt, counts);
+  return flood_rounds(rec, d, W, off, cur, nxt, counts, flood_budget(count));
 }
 
 // The free-space length of a soma voxel at offset (dxi, dyi, dzi) from the root, when its physical
@@ -226,7 +233,7 @@ __device__ __forceinline__ bool free_space(int dxi, int dyi, int dzi, float wx, 
 
 // A soma's DAF flood (batched.cu:222-231): every voxel of the label inside the free-space ball
 // starts at its free-space length (the root at 0), then the label-correcting rounds.
-__device__ __forceinline__ void flood_free_space(VoxelRec *rec, Dims d, const int *list, int cnt, int root, float radius,
+__device__ __forceinline__ bool flood_free_space(VoxelRec *rec, Dims d, const int *list, int cnt, int root, float radius,
                                                  const float *W, const int *off, int *cur, int *nxt, int *counts) {
   if (threadIdx.x == 0) { counts[0] = 0; counts[1] = 0; }
   __syncthreads();
@@ -242,7 +249,7 @@ __device__ __forceinline__ void flood_free_space(VoxelRec *rec, Dims d, const in
     g_st(cur + wave_slot(&counts[0]), v);
   }
   __syncthreads();
-  flood_rounds(rec, d, W, off, cur, nxt, counts);
+  return flood_rounds(rec, d, W, off, cur, nxt, counts, flood_budget(cnt));
 }
 
 // Argmax of the flooded distance over the label's voxel list: largest finite value, ties -> lowest
@@ -269,8 +276,11 @@ __device__ __forceinline__ unsigned long long label_argmax(const VoxelRec *rec, 
   return r;
 }
 
+// stalled: set when a flood exceeded its round budget (the label's fields are then left as they
+// are and the host raises).
 __global__ void __launch_bounds__(kFloodBlock) prepare_labels(const FloodJob *__restrict__ jobs,
-                                                              const int2 *__restrict__ queue, int nqueue, int *next) {
+                                                              const int2 *__restrict__ queue, int nqueue, int *next,
+                                                              int *stalled) {
   __shared__ float W[26];
   __shared__ int off[26];
   __shared__ int counts[2];
@@ -296,7 +306,10 @@ __global__ void __launch_bounds__(kFloodBlock) prepare_labels(const FloodJob *__
     int *a = job.lists_a + s0, *b = job.lists_b + s0;
     long long root = g_ld(t.root + L);
     if (root < 0) {  // no border target: flood from the component's first voxel, root = argmax
-      flood_label(t.rec, d, int(g_ld(t.first + L)), W, off, a, b, counts);
+      if (!flood_label(t.rec, d, int(g_ld(t.first + L)), W, off, a, b, counts, cnt)) {
+        if (threadIdx.x == 0) atomicOr(stalled, 1);
+        continue;
+      }
       unsigned long long key = label_argmax(t.rec, list, cnt, &best);
       root = key ? (long long)(0xFFFFFFFFu - (unsigned)key) : -1;
       for (int j = threadIdx.x; j < cnt; j += blockDim.x) sp_u64(&t.rec[g_ld(list + j)].word, kMaxKey);
@@ -304,8 +317,12 @@ __global__ void __launch_bounds__(kFloodBlock) prepare_labels(const FloodJob *__
       if (threadIdx.x == 0) g_st(t.root + L, root);
     }
     const float radius = t.free_radius ? g_ld(t.free_radius + L) : 0.0f;
-    if (radius > 0.0f) flood_free_space(t.rec, d, list, cnt, int(root), radius, W, off, a, b, counts);
-    else flood_label(t.rec, d, int(root), W, off, a, b, counts);
+    const bool converged = radius > 0.0f ? flood_free_space(t.rec, d, list, cnt, int(root), radius, W, off, a, b, counts)
+                                         : flood_label(t.rec, d, int(root), W, off, a, b, counts, cnt);
+    if (!converged) {
+      if (threadIdx.x == 0) atomicOr(stalled, 1);
+      continue;
+    }
     unsigned long long key = label_argmax(t.rec, list, cnt, &best);
     if (threadIdx.x == 0) {
       g_st(t.target + L, key ? (long long)(0xFFFFFFFFu - (unsigned)key) : -1LL);
@@ -401,15 +418,17 @@ void prepare_fields(Context &ctx, const std::vector<FloodJob> &jobs, const std::
   for (size_t i = 0; i < items.size(); ++i) q[i] = make_int2(items[i].sample, items[i].label);
   Buffer<int2> queue(ctx, q.size());
   queue.upload(ctx, q.data(), q.size());
-  Buffer<int> next(ctx, 1);
+  Buffer<int> next(ctx, 2);  // [0] queue cursor, [1] stalled
   next.clear(ctx);
   const int nq = int(q.size());
   // every workgroup takes labels from the queue until it is spent, so any grid drains it
   const int grid = std::min(nq, BROOK_FLOOD_GRID > 0 ? BROOK_FLOOD_GRID : ctx.compute_units() * 2);
-  KernelTimer kt(ctx, "prepare_labels");
-  prepare_labels<<<grid, kFloodBlock, 0, ctx.stream()>>>(jobs_dev.data(), queue.data(), nq, next.data());
-  BROOK_LAUNCHED();
-  ctx.synchronize();
+  {
+    KernelTimer kt(ctx, "prepare_labels");
+    prepare_labels<<<grid, kFloodBlock, 0, ctx.stream()>>>(jobs_dev.data(), queue.data(), nq, next.data(), next.data() + 1);
+    BROOK_LAUNCHED();
+  }
+  if (next.download(ctx)[1]) throw std::runtime_error("flood did not converge: a label exceeded its round budget");
 }
 
 void pdrf_and_alive(Context &ctx, const void *cc, int cc_bytes, long long n, const float *dbf, const float *daf,

@@ -17,6 +17,24 @@ __global__ void __launch_bounds__(256) gather(const char *base, long long s0, lo
   out[i] = *reinterpret_cast<const T *>(base + x * s0 + y * s1 + z * s2);
 }
 
+// C order (z fastest) to Fortran order (x fastest): a transpose of x and z per y plane through a
+// 64 x 64 LDS tile, so both the reads (along z) and the writes (along x) are coalesced.
+template <class T>
+__global__ void __launch_bounds__(256) transpose_xz(const T *in, Dims d, T *out) {
+  __shared__ T tile[64][65];
+  const int z0 = blockIdx.x * 64, x0 = blockIdx.y * 64, y = blockIdx.z;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int r = ty; r < 64; r += 4) {
+    const int x = x0 + r, z = z0 + tx;
+    if (x < d.sx && z < d.sz) tile[r][tx] = in[((long long)x * d.sy + y) * d.sz + z];
+  }
+  __syncthreads();
+  for (int r = ty; r < 64; r += 4) {
+    const int z = z0 + r, x = x0 + tx;
+    if (x < d.sx && z < d.sz) out[x + (long long)d.sx * (y + (long long)d.sy * z)] = tile[tx][r];
+  }
+}
+
 template <class T>
 __global__ void __launch_bounds__(256) mask_ids(const T *in, long long n, const T *ids, int count, T *out) {
   long long i = (long long)blockIdx.x * 256 + threadIdx.x;
@@ -34,6 +52,15 @@ __global__ void __launch_bounds__(256) mask_ids(const T *in, long long n, const 
 template <class T>
 void run_gather(Context &ctx, const void *base, const long long strides[3], Dims d, void *out) {
   long long n = (long long)d.sx * d.sy * d.sz;
+  const long long b = sizeof(T);
+  const bool c_order = (d.sz <= 1 || strides[2] == b) && (d.sy <= 1 || strides[1] == b * d.sz) &&
+                       (d.sx <= 1 || strides[0] == b * d.sz * d.sy);
+  if (c_order && (d.sx + 63) / 64 <= 65535 && d.sy <= 65535) {
+    const dim3 grid(unsigned((d.sz + 63) / 64), unsigned((d.sx + 63) / 64), unsigned(d.sy));
+    transpose_xz<T><<<grid, 256, 0, ctx.stream()>>>(static_cast<const T *>(base), d, static_cast<T *>(out));
+    BROOK_LAUNCHED();
+    return;
+  }
   gather<T><<<unsigned((n + 255) / 256), 256, 0, ctx.stream()>>>(static_cast<const char *>(base), strides[0], strides[1],
                                                                   strides[2], d, static_cast<T *>(out));
   BROOK_LAUNCHED();

@@ -58,6 +58,9 @@ struct Shared {
   float T;
   unsigned long long rail_best, minfar;
   int store_pos;
+  // The railroad's far lists hold 2 * count entries; a list longer than count after a round is
+  // compacted (see far_compact). `overflow`: an append found the list full (never expected).
+  int far_cap, overflow;
   int off[26];  // flat offsets of the 26 neighbours
   // per-label profile: [0] total [1] target [2] railroad [3] backtrace [4] invalidation (wall-clock
   // ticks), [5] railroad rounds, [6] invalidation rounds, [7] paths, [8] railroad phases,
@@ -88,6 +91,36 @@ __device__ __forceinline__ void emit_front(const Front &f, int *counter, unsigne
     mask &= mask - 1;
     front_put(f, pos++, u + off[k]);
   }
+}
+
+// emit_neighbours into a list of `cap` entries: a wave whose appends would pass the end writes
+// nothing and raises `overflow` instead.
+__device__ __forceinline__ void emit_capped(int *list, int *counter, unsigned mask, int u, const int *off, int cap,
+                                            int *overflow) {
+  const int n = __popc(mask);
+  int pos = wave_reserve(counter, n);
+  if (pos + n > cap) {
+    if (n) *overflow = 1;
+    return;
+  }
+  while (mask) {
+    const int k = __ffs(mask) - 1;
+    mask &= mask - 1;
+    g_st(list + pos++, u + off[k]);
+  }
+}
+
+// Rounds (and phases) one railroad or invalidation may take before the label is given up as not
+// converging: far above what the label-correcting rounds need (a few per voxel of the label at
+// most), and finite, so a logic error ends as a host-visible error instead of a hang.
+__device__ __forceinline__ long long round_budget(int count) { return 4LL * count + 4096; }
+
+// Statuses of one railroad: 1 nearest rail settled, 2 no rail reachable, 3 did not converge
+// within the round budget, 4 a far list overflowed.
+constexpr int kRailFound = 1, kNoRail = 2, kStalled = 3, kOverflowed = 4;
+// counters[8] bits: 1 = a railroad or invalidation did not converge, 2 = a far list overflowed.
+__device__ __forceinline__ void report_failure(const TraceArgs &a, int status) {
+  g_or(a.counters + 8, status == kStalled ? 1 : 2);
 }
 
 __device__ __forceinline__ void lap(Shared &sh, int slot) {
@@ -258,7 +291,7 @@ __device__ __forceinline__ void relax_groups(const TraceArgs &a, Shared &sh, Roa
   if (q.i < nc && du <= bound) relax_span(a, mins, u, m, q.kb, q.cnt, du, bound, sh.off, p);
   emit_neighbours(touched, &sh.n_touched, p.touched, u, sh.off);
   emit_front(nxt, n_nxt, p.near, u, sh.off);
-  emit_neighbours(far, &sh.n_far, p.far, u, sh.off);
+  emit_capped(far, &sh.n_far, p.far, u, sh.off, sh.far_cap, &sh.overflow);
 }
 
 // One railroad round over a larger front: a thread per entry, its neighbours in parts.
@@ -276,8 +309,29 @@ __device__ __forceinline__ void relax_front(const TraceArgs &a, Shared &sh, Road
     }
     emit_neighbours(touched, &sh.n_touched, p.touched, u, sh.off);
     emit_front(nxt, n_nxt, p.near, u, sh.off);
-    emit_neighbours(far, &sh.n_far, p.far, u, sh.off);
+    emit_capped(far, &sh.n_far, p.far, u, sh.off, sh.far_cap, &sh.overflow);
   }
+}
+
+// Far-list compaction between rounds. A voxel can hold several entries: its entry goes stale when
+// a near improvement clears its far bit, and it is far-listed anew when a later, smaller bound
+// (a closer rail) leaves its next improvement beyond the band. Within one round the bound is fixed,
+// so a round appends at most one entry per voxel (at most `count`); a list that holds more than
+// `count` entries after a round keeps one entry per far-listed voxel (the one whose atomic clears
+// the far bit, restored after a barrier), so 2 * count entries always suffice.
+__device__ __forceinline__ void far_compact(const TraceArgs &a, Shared &sh, int *&fcur, int *&fnxt) {
+  const int nf = sh.n_far;
+  for (int i = threadIdx.x; i < nf; i += blockDim.x) {
+    const int w = lp_i(fcur + i);
+    if (far_listed(g_and(&a.rec[w].word, ~2ULL))) g_st(fnxt + wave_slot(&sh.n_farnxt), w);
+  }
+  __syncthreads();
+  const int kept = sh.n_farnxt;
+  for (int i = threadIdx.x; i < kept; i += blockDim.x) g_or(&a.rec[lp_i(fnxt + i)].word, 2ULL);
+  __syncthreads();
+  if (threadIdx.x == 0) { sh.n_far = kept; sh.n_farnxt = 0; }
+  int *tmp = fcur; fcur = fnxt; fnxt = tmp;
+  __syncthreads();
 }
 
 __device__ __forceinline__ float rail_bound(const Shared &sh) {
@@ -287,21 +341,25 @@ __device__ __forceinline__ float rail_bound(const Shared &sh) {
   return b;
 }
 
-// Returns 1 when the nearest rail is settled (sh.rail_best), 2 when no rail is reachable. A
-// `source` run never takes t itself for a rail: with no rails in the label it settles every
-// voxel reachable from t (the vertex-weighted distances of the fix_branching=False field).
-__device__ __forceinline__ int railroad(const TraceArgs &a, Shared &sh, int t, const Front &near_a, const Front &near_b,
-                                        int *far_a, int *far_b, int *touched, bool source = false) {
+// Returns kRailFound when the nearest rail is settled (sh.rail_best), kNoRail when no rail is
+// reachable, kStalled / kOverflowed when the run is given up (see round_budget, far_compact). A
+// `source` run never takes t itself for a rail: with no rails in the label it settles every voxel
+// reachable from t (the vertex-weighted distances of the fix_branching=False field). The far lists
+// hold 2 * count entries.
+__device__ __forceinline__ int railroad(const TraceArgs &a, Shared &sh, int t, int count, const Front &near_a,
+                                        const Front &near_b, int *far_a, int *far_b, int *touched, bool source = false) {
   if (threadIdx.x == 0) {
     g_st(touched, t);
     sh.n_touched = 1;
     sh.cnt[0] = sh.cnt[1] = sh.cnt[2] = sh.n_far = sh.n_farnxt = 0;
+    sh.far_cap = 2 * count;
+    sh.overflow = 0;
     sh.minfar = kMinfarNone;
     float ft = rec_pdrf(a.rec, t);
     if (ft == 0.0f && !source) {
       sp_u64(&a.rec[t].word, 1ULL);  // distance 0, not queued
       sh.rail_best = (unsigned)t;
-      sh.done = 1;
+      sh.done = kRailFound;
     } else {
       sp_u64(&a.rec[t].word, 0ULL);  // distance 0, queued
       sh.rail_best = kMaxKey;
@@ -316,7 +374,10 @@ __device__ __forceinline__ int railroad(const TraceArgs &a, Shared &sh, int t, c
   Front cur = near_a, nxt = near_b;
   int *fcur = far_a, *fnxt = far_b;
   int r = 0;  // the near list's length is sh.cnt[r]
+  long long rounds = 0;  // rounds and phases so far (the same in every thread)
+  const long long budget = round_budget(count);
   while (sh.done == 0) {
+    bool stop = false;
     for (;;) {  // settle the near band
       const int nc = sh.cnt[r];
       if (nc == 0) break;
@@ -329,11 +390,20 @@ __device__ __forceinline__ int railroad(const TraceArgs &a, Shared &sh, int t, c
       __syncthreads();
       r = rn;
       const Front tmp = cur; cur = nxt; nxt = tmp;
+      if (sh.overflow || ++rounds > budget) { stop = true; break; }
+      if (sh.n_far > count) far_compact(a, sh, fcur, fnxt);
+    }
+    if (stop) {  // uniform: every thread read the same flags after the round's barrier
+      __syncthreads();
+      if (threadIdx.x == 0) sh.done = sh.overflow ? kOverflowed : kStalled;
+      __syncthreads();
+      break;
     }
     if (threadIdx.x == 0) {  // phase end: nearest rail final? else advance the band
       unsigned long long rb = sh.rail_best, mf = sh.minfar;
-      if (rb != kMaxKey && (__uint_as_float((unsigned)(rb >> 32)) <= sh.T || mf == kMinfarNone)) sh.done = 1;
-      else if (mf == kMinfarNone) sh.done = 2;
+      if (rb != kMaxKey && (__uint_as_float((unsigned)(rb >> 32)) <= sh.T || mf == kMinfarNone)) sh.done = kRailFound;
+      else if (mf == kMinfarNone) sh.done = kNoRail;
+      else if (rounds + 1 > budget) sh.done = kStalled;
       else {
         float dmin = __uint_as_float((unsigned)(mf >> 32)), f = __uint_as_float((unsigned)mf);
         float w = 2.0f * f, lo = 5e-4f * dmin;
@@ -344,6 +414,7 @@ __device__ __forceinline__ int railroad(const TraceArgs &a, Shared &sh, int t, c
       sh.prof[8] += 1;
       sh.prof[9] += sh.n_far;
     }
+    ++rounds;
     __syncthreads();
     if (sh.done) break;
     const int nf = sh.n_far;
@@ -351,7 +422,7 @@ __device__ __forceinline__ int railroad(const TraceArgs &a, Shared &sh, int t, c
     // Promote far voxels inside the new band. Entries whose far bit is clear are stale (the voxel
     // went near); of several entries of one voxel, the one whose atomic clears the far bit acts.
     // A far-listed word is never queued, so the claim also queues it; a voxel still beyond the
-    // band gets both bits back.
+    // band gets both bits back. Each voxel lands in at most one list (at most count entries).
     for (int i = threadIdx.x; i < nf; i += blockDim.x) {
       const int w = lp_i(fcur + i);
       if (!far_listed(lp_u64(&a.rec[w].word))) continue;
@@ -425,13 +496,14 @@ __device__ __forceinline__ void backtrace(const TraceArgs &a, Shared &sh, int t,
 // vertex-weighted distance over the PDRF; parents_lab (batched_regular.inc:113-134) then takes
 // the same-label neighbour of smallest distance, first k on ties (-1 at the root and at voxels the
 // run does not reach, as a label that filled holes split has), and the words go back to ~0.
-// Returns false when the run met a voxel of PDRF 0 besides the root (it would stop
-// there as at a rail; PDRF underflows to 0 only for extreme exponents): the label is ejected.
-__device__ __forceinline__ bool parent_field(const TraceArgs &a, Shared &sh, int root, const int *voxels, int count,
-                                             const Front &na, const Front &nb, int *l2, int *l3, int *touched) {
+// Returns the railroad's status: kNoRail is the complete field; kRailFound means the run met a
+// voxel of PDRF 0 besides the root (it would stop there as at a rail; PDRF underflows to 0 only for
+// extreme exponents), and the label is ejected; kStalled / kOverflowed end the label as failed.
+__device__ __forceinline__ int parent_field(const TraceArgs &a, Shared &sh, int root, const int *voxels, int count,
+                                            const Front &na, const Front &nb, int *l2, int *l3, int *touched) {
   for (int j = threadIdx.x; j < count; j += blockDim.x) g_st(a.parent + g_ld(voxels + j), -1);  // unreached: no parent
   __syncthreads();
-  const int status = railroad(a, sh, root, na, nb, l2, l3, touched, true);
+  const int status = railroad(a, sh, root, count, na, nb, l2, l3, touched, true);
   const int nt = sh.n_touched;
   for (int j = threadIdx.x; j < nt; j += blockDim.x) {
     const int v = g_ld(touched + j);
@@ -452,7 +524,7 @@ __device__ __forceinline__ bool parent_field(const TraceArgs &a, Shared &sh, int
   __syncthreads();
   for (int j = threadIdx.x; j < nt; j += blockDim.x) sp_u64(&a.rec[g_ld(touched + j)].word, kMaxKey);
   __syncthreads();
-  return status == 2;
+  return status;
 }
 
 // The walk (one thread):
@@ -729,10 +801,11 @@ __device__ __forceinline__ int band_of(unsigned mbits, float delta, int kmin) {
   return k < kmin ? kmin : k;
 }
 
-__device__ __forceinline__ void invalidate(const TraceArgs &a, const Ball &ball, Shared &sh, const int *path, int len,
-                                           const Front &la,
-                                           const Front &lb, int *lfar, int *lkeep, int *touched, unsigned long long *snap,
-                                           unsigned long long *pull) {
+// Returns false when the rounds did not converge within the budget (the label is then given up;
+// the claimed set still dies).
+__device__ __forceinline__ bool invalidate(const TraceArgs &a, const Ball &ball, Shared &sh, const int *path, int len,
+                                           int count, const Front &la, const Front &lb, int *lfar, int *lkeep,
+                                           int *touched, unsigned long long *snap, unsigned long long *pull) {
   if (threadIdx.x == 0) {
     sh.cnt[0] = sh.cnt[1] = sh.cnt[2] = sh.n_far = sh.n_keep = sh.n_touched = 0;
     sh.band = 0;
@@ -750,11 +823,15 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, const Ball &ball,
     }
   }
   __syncthreads();
+  long long rounds = 0;  // rounds and bands so far (the same in every thread)
+  const long long budget = round_budget(count);
+  bool converged = true;
   for (;;) {
     unsigned hi = __float_as_uint((float)(sh.band + 1) * a.delta);
     for (;;) {  // synchronous rounds inside the band
       const int nc = sh.cnt[r];
       if (nc == 0) break;
+      if (++rounds > budget) { converged = false; break; }
       const int rn = rot_next(r);
       if (threadIdx.x == 0) { sh.cnt[rot_prev(r)] = 0; sh.prof[6] += 1; sh.prof[11] += nc; }
       if (nc <= kGroupFront) {
@@ -784,7 +861,7 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, const Ball &ball,
       const Front tmp = cur; cur = nxt; nxt = tmp;
     }
     const int nf = sh.n_far;
-    if (nf == 0) break;
+    if (nf == 0 || !converged || ++rounds > budget) { converged = converged && nf == 0; break; }
     if (threadIdx.x == 0) { sh.lowest = 0x7f800000u; sh.prof[12] += nf; sh.prof[13] += 1; }
     __syncthreads();
     if (nf <= kGroupFront) {  // band boundary: exact pulls
@@ -825,14 +902,19 @@ __device__ __forceinline__ void invalidate(const TraceArgs &a, const Ball &ball,
     int *tmp = far; far = keep; keep = tmp;
     __syncthreads();
   }
+  __syncthreads();
   const int nt = sh.n_touched;
-  if (threadIdx.x == 0) g_add(reinterpret_cast<unsigned long long *>(a.counters + 6), (unsigned long long)nt);
+  if (threadIdx.x == 0) {
+    g_add(reinterpret_cast<unsigned long long *>(a.counters + 6), (unsigned long long)nt);
+    if (!converged) report_failure(a, kStalled);
+  }
   for (int j = threadIdx.x; j < nt; j += blockDim.x) {  // the claimed set dies
     int v = g_ld(touched + j);
     sp_u64(&a.rec[v].word, kMaxKey);
     g_and(&a.rec[v].meta, ~(kAlive | kWaiting));
   }
   __syncthreads();
+  return converged;
 }
 
 // ---- target selection -----------------------------------------------------------------------
@@ -932,7 +1014,8 @@ __global__ void __launch_bounds__(kBlock) soma_balls(const TraceArgs *__restrict
   const int L = __builtin_amdgcn_readfirstlane(q.y);
   if (threadIdx.x < 26) sh.off[threadIdx.x] = nb_offset(int(threadIdx.x), a.d);
   const int seg = g_ld(a.seg_start + L), count = g_ld(a.seg_count + L);
-  int *l2 = a.lists[2] + seg, *l3 = a.lists[3] + seg, *l4 = a.lists[4] + seg, *walk = a.lists[5] + seg;
+  // lists 2 and 3 (the far lists) hold two entries per voxel: segment 2 * seg
+  int *l2 = a.lists[2] + 2LL * seg, *l3 = a.lists[3] + 2LL * seg, *l4 = a.lists[4] + seg, *walk = a.lists[5] + seg;
   const Front na{a.lists[0] + seg, sh.front[0]}, nb{a.lists[1] + seg, sh.front[1]};
   unsigned long long *s0 = a.snaps[0] + seg, *s1 = a.snaps[1] + seg;
   const SomaLabel *soma = a.soma + L;
@@ -941,7 +1024,7 @@ __global__ void __launch_bounds__(kBlock) soma_balls(const TraceArgs *__restrict
     g_st(walk, int(g_ld(a.root + L)));
   }
   __syncthreads();
-  invalidate(a, soma->ball, sh, walk, 1, na, nb, l2, l3, l4, s0, s1);
+  invalidate(a, soma->ball, sh, walk, 1, count, na, nb, l2, l3, l4, s0, s1);
   if (threadIdx.x == 0) {
     const long long limit = g_ld(&soma->max_paths);
     const long long budget = limit >= 0 ? limit : (long long)count - sh.n_touched;
@@ -965,7 +1048,8 @@ __global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restri
     const int L = __builtin_amdgcn_readfirstlane(q.y);
     if (threadIdx.x < 26) sh.off[threadIdx.x] = nb_offset(int(threadIdx.x), a.d);
     const int seg = g_ld(a.seg_start + L), count = g_ld(a.seg_count + L);
-    int *l2 = a.lists[2] + seg, *l3 = a.lists[3] + seg, *l4 = a.lists[4] + seg, *walk = a.lists[5] + seg;
+    // lists 2 and 3 (the far lists) hold two entries per voxel: segment 2 * seg
+    int *l2 = a.lists[2] + 2LL * seg, *l3 = a.lists[3] + 2LL * seg, *l4 = a.lists[4] + seg, *walk = a.lists[5] + seg;
     const Front na{a.lists[0] + seg, sh.front[0]}, nb{a.lists[1] + seg, sh.front[1]};
     unsigned long long *s0 = a.snaps[0] + seg, *s1 = a.snaps[1] + seg;
     if (threadIdx.x == 0) {
@@ -980,9 +1064,11 @@ __global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restri
     __syncthreads();
     bool traceable = true;
     if (a.parent) {
-      traceable = parent_field(a, sh, int(g_ld(a.root + L)), a.order + seg, count, na, nb, l2, l3, l4);
+      const int field = parent_field(a, sh, int(g_ld(a.root + L)), a.order + seg, count, na, nb, l2, l3, l4);
       lap(sh, 2);
-      if (!traceable && threadIdx.x == 0) g_st(a.ejected + g_add(a.counters + 2, 1) % a.ejected_cap, L);
+      traceable = field == kNoRail;
+      if (field == kRailFound && threadIdx.x == 0) g_st(a.ejected + g_add(a.counters + 2, 1) % a.ejected_cap, L);
+      if ((field == kStalled || field == kOverflowed) && threadIdx.x == 0) report_failure(a, field);
     }
     // A soma's paths keep only the vertices beyond its radius (its ball died in soma_balls).
     const bool soma = a.soma && g_ld(&a.soma[L].radius) >= 0.0;
@@ -990,14 +1076,14 @@ __global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restri
       const int t = next_target(a, sh, L);
       lap(sh, 1);
       if (t < 0) break;
-      int status = 1;
+      int status = kRailFound;
       if (a.parent) {
         parent_walk(a, sh, t, count, walk);
         if (threadIdx.x == 0) sh.n_touched = 0;
       } else {
-        status = railroad(a, sh, t, na, nb, l2, l3, l4);
+        status = railroad(a, sh, t, count, na, nb, l2, l3, l4);
         lap(sh, 2);
-        if (status == 1) backtrace(a, sh, t, count, walk);
+        if (status == kRailFound) backtrace(a, sh, t, count, walk);
       }
       __syncthreads();
       lap(sh, 3);
@@ -1005,7 +1091,12 @@ __global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restri
       for (int j = threadIdx.x; j < nt; j += blockDim.x) {  // restore the railroad state
         sp_u64(&a.rec[g_ld(l4 + j)].word, kMaxKey);
       }
-      if (status == 2) {
+      if (status == kStalled || status == kOverflowed) {  // given up: the host raises
+        if (threadIdx.x == 0) report_failure(a, status);
+        __syncthreads();
+        break;
+      }
+      if (status == kNoRail) {
         // No rail reachable from t (a component that filled holes split): as the reference's
         // per-component tracer does for such a label (trace.cu:198-215), an ordered target ends
         // the label with the paths it has, a manual one counts as an empty path.
@@ -1045,8 +1136,9 @@ __global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restri
         else sp_f(&a.rec[v].pdrf, 0.0f);       // the path becomes rail
       }
       __syncthreads();
-      invalidate(a, a.ball, sh, walk, len, na, nb, l2, l3, l4, s0, s1);
+      const bool converged = invalidate(a, a.ball, sh, walk, len, count, na, nb, l2, l3, l4, s0, s1);
       lap(sh, 4);
+      if (!converged) break;  // reported by invalidate; the host raises
       if (threadIdx.x == 0) { sh.pcount += 1; sh.prof[7] += 1; }
       __syncthreads();
     }

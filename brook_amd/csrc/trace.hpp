@@ -23,14 +23,17 @@ struct TraceArgs {
   // acount[L] more (after it; acount null: none)
   const int *mstart = nullptr, *mcount = nullptr, *mflat = nullptr, *acount = nullptr;
   int *maxp = nullptr;  // path budgets (a soma's is set on the device, after its ball)
-  int *lists[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // int scratch, segment layout
+  // int scratch in segment layout: lists 0, 1, 4, 5 hold one entry per voxel (segment seg_start[L]),
+  // lists 2 and 3 (the railroad's far lists) two (segment 2 * seg_start[L])
+  int *lists[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   unsigned long long *snaps[2] = {nullptr, nullptr};                       // u64 scratch, segment layout
   int *store = nullptr;
   long long store_cap = 0;
   int *rec_label = nullptr, *rec_len = nullptr, *rec_start = nullptr;
   int rec_cap = 0;
-  // [0] store used [1] records [2] ejected labels [3] overflow; [4..5] u64 railroad-reached voxels,
-  // [6..7] u64 invalidated voxels
+  // [0] store used [1] records [2] ejected labels [3] path store overflow; [4..5] u64
+  // railroad-reached voxels, [6..7] u64 invalidated voxels; [8] failure bits (1: a railroad or an
+  // invalidation did not converge within its round budget, 2: a far list overflowed)
   int *counters = nullptr;
   // per label (kTraceProfileSlots u64): wall-clock ticks total / target / railroad / backtrace /
   // invalidation, railroad rounds, invalidation rounds, paths, railroad phases, far entries

@@ -11,6 +11,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "dlpack.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -22,10 +24,11 @@ typedef struct brook_array brook_array;
 typedef struct brook_components brook_components;
 typedef struct brook_skeletons brook_skeletons;
 
-/* brook.h:26-29 (BROOK_CUDA_ERROR there: a device (HIP) error here) */
+/* brook.h:26-29. BROOK_CUDA_ERROR keeps the reference's name of value 3 (a device error: HIP
+ * here), so consumer source that names it compiles unchanged. */
 typedef enum brook_status {
   BROOK_SUCCESS = 0, BROOK_INVALID_ARGUMENT = 1, BROOK_OUT_OF_MEMORY = 2,
-  BROOK_DEVICE_ERROR = 3, BROOK_INTERNAL_ERROR = 4
+  BROOK_DEVICE_ERROR = 3, BROOK_CUDA_ERROR = 3, BROOK_INTERNAL_ERROR = 4
 } brook_status;
 
 /* brook.h:31-35 */
@@ -103,11 +106,20 @@ brook_status brook_context_create(int device, brook_context **out);
 void brook_context_destroy(brook_context *context);
 brook_status brook_context_synchronize(brook_context *context);
 
-/* brook.h:159-162: device arrays owned by their handle. */
+/* brook.h:159-162: device arrays owned by their handle. copy_to_host copies the array's bytes into
+ * a host buffer of at least that many bytes. */
 brook_status brook_array_upload(brook_context *context, const brook_volume *input, brook_array **out);
 void brook_array_destroy(brook_array *array);
 brook_status brook_array_view(const brook_array *array, brook_volume *out);
 brook_status brook_array_copy_to_host(const brook_array *array, void *out, size_t bytes);
+/* brook.h:163-167: zero-copy DLPack exports (device type kDLROCM, the array's device; shape and
+ * element strides of the array's view: 3-D volumes, or the 2-D / 1-D rows of a skeleton buffer).
+ * An export keeps the storage alive independently of the array and context handles; the consumer
+ * calls its deleter exactly once. The versioned form reports version 1.min(max_minor,
+ * DLPACK_MINOR_VERSION) and no flags. */
+brook_status brook_array_to_dlpack(const brook_array *array, DLManagedTensor **out);
+brook_status brook_array_to_dlpack_versioned(const brook_array *array, uint32_t max_minor,
+                                             DLManagedTensorVersioned **out);
 
 /* brook.h:169-182: 26-connected components, ids 1..N by each component's first voxel in
  * Fortran order, the smallest unsigned dtype that holds N; mapping has N+1 entries
@@ -134,8 +146,10 @@ void brook_skeletons_destroy(brook_skeletons *result);
 
 /* brook.h:96-102, 232-233: results requested with BROOK_DEVICE_OUTPUT also hold their vertices
  * (vertex_count x 3 f32), edges (edge_count x 2 u32) and radii (f32) in device memory, row major;
- * labels and offsets stay host metadata, and get_view reads the host snapshot. get_array shares
- * the handle's storage (valid after the handle is destroyed, until the array is). */
+ * labels and offsets stay host metadata, and get_view reads the host snapshot. The device views
+ * describe rows: vertices shape (V, 3, 1), byte strides (12, 4, 12V); edges (E, 2, 1), (8, 4, 8E);
+ * radii (V, 1, 1), (4, 4V, 4V). get_array shares the handle's storage (valid after the handle is
+ * destroyed, until the array is) and views it the same way. */
 typedef struct brook_skeletons_device_view {
   size_t skeleton_count, vertex_count, edge_count;
   const int64_t *labels, *vertex_offsets, *edge_offsets;
@@ -176,6 +190,12 @@ brook_status brook_prepare_fields(brook_context *context, const brook_volume *la
 /* EXTENSION: a timing/counter of the context's last call (e.g. "trace_ms", "paths"; with kernel
  * timing on, "k_<kernel>_ms" and "k_<kernel>_n" from events on the context's stream). */
 brook_status brook_context_stat(const brook_context *context, const char *name, double *value);
+
+/* EXTENSION (the reference's Python binding fences a producer stream the same way,
+ * bindings.cpp:105-120): the context's later work waits for the work queued so far on `stream`
+ * (a hipStream_t; 0 the null stream), with an event and no host wait. Device input produced on
+ * another stream is then read only after it is ready. */
+brook_status brook_context_wait_stream(brook_context *context, void *stream);
 
 /* EXTENSION: time the engine's main kernels with HIP events on the context's stream (adds one
  * host wait per timed kernel). */

@@ -309,8 +309,11 @@ def test_device_output_views(brook):
             native.check(status)
             nv, ne = host.vertex_count, host.edge_count
             assert (dev.skeleton_count, dev.vertex_count, dev.edge_count) == (host.skeleton_count, nv, ne) and nv > 0
-            assert dev.vertices.memory == native.DEVICE and tuple(dev.vertices.shape) == (3, nv, 1)
-            assert tuple(dev.edges.shape) == (2, ne, 1) and tuple(dev.radii.shape) == (1, nv, 1)
+            # rows, as the reference's row_view (packed.cpp:7-14): (rows, width, 1), byte strides (row, item, bytes)
+            assert dev.vertices.memory == native.DEVICE and tuple(dev.vertices.shape) == (nv, 3, 1)
+            assert tuple(dev.vertices.strides) == (12, 4, 12 * nv)
+            assert tuple(dev.edges.shape) == (ne, 2, 1) and tuple(dev.edges.strides) == (8, 4, 8 * ne)
+            assert tuple(dev.radii.shape) == (nv, 1, 1) and tuple(dev.radii.strides) == (4, 4 * nv, 4 * nv)
             for which, ptr, dtype, count in ((native.VERTICES, host.vertices, np.float32, 3 * nv),
                                              (native.EDGES, host.edges, np.uint32, 2 * ne),
                                              (native.RADII, host.radii, np.float32, nv)):
@@ -321,6 +324,10 @@ def test_device_output_views(brook):
                 got = np.zeros(count, dtype)
                 try:
                     native.check(L.brook_array_copy_to_host(arr, got.ctypes.data, got.nbytes))
+                    view = native.Volume()  # the array views its rows as the device view does
+                    native.check(L.brook_array_view(arr, ctypes.byref(view)))
+                    expected = {native.VERTICES: dev.vertices, native.EDGES: dev.edges, native.RADII: dev.radii}[which]
+                    assert tuple(view.shape) == tuple(expected.shape) and tuple(view.strides) == tuple(expected.strides)
                 finally:
                     L.brook_array_destroy(arr)
                 assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), which
