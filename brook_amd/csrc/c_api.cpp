@@ -334,7 +334,9 @@ brook_status brook_context_create(int device, brook_context **out) {
     require(out != nullptr);
     *out = nullptr;
     require(device >= 0, "negative device index");
-    *out = new brook_context(device);
+    auto ctx = std::make_unique<brook_context>(device);
+    brook::noalloc_scratch_init(device);  // the device is current: its no-allocation scratch address
+    *out = ctx.release();
   });
 }
 void brook_context_destroy(brook_context *context) { delete context; }
@@ -440,6 +442,35 @@ brook_status brook_components_labels(const brook_components *c, brook_array **ou
   return protect([&] {
     require(c && out);
     *out = new brook_array(c->labels);
+  });
+}
+
+brook_status brook_connected_components_noalloc(brook_context *ctx, const brook_volume *labels, void *ids_out,
+                                                size_t ids_bytes, int64_t *mapping, int64_t *roots, size_t capacity,
+                                                size_t *count) {
+  return protect([&] {
+    require(ctx && labels && count && mapping && (roots || !capacity));
+    *count = 0;
+    check_volume(labels);
+    require(integer_dtype(labels->dtype), "connected components requires integer labels");
+    require(labels->memory == BROOK_DEVICE, "the no-allocation variant takes device labels");
+    const int b = dtype_bytes(labels->dtype);
+    const int64_t sx = labels->shape[0], sy = labels->shape[1], sz = labels->shape[2], n = sx * sy * sz;
+    const bool fortran = (sx <= 1 || labels->strides[0] == b) && (sy <= 1 || labels->strides[1] == b * sx) &&
+                         (sz <= 1 || labels->strides[2] == b * sx * sy);
+    require(fortran, "the no-allocation variant takes Fortran-contiguous labels");
+    require(ids_bytes >= size_t(n) * 4 && (ids_out || !n), "ids buffer smaller than 4 bytes per voxel");
+    require(capacity < (size_t(1) << 40), "table capacity out of range");
+    brook::ContextCall call(ctx->value);
+    brook::LabelVolume view;
+    view.data = labels->data;
+    view.bytes = b;
+    view.is_signed = labels->dtype >= BROOK_I8 && labels->dtype <= BROOK_I64;
+    view.dims = brook::Dims{int(sx), int(sy), int(sz)};
+    const int64_t got = brook::connected_components_noalloc(ctx->value, view, static_cast<uint32_t *>(ids_out), mapping,
+                                                            roots, int64_t(capacity));
+    *count = size_t(got);
+    require(got <= int64_t(capacity), "component count exceeds the table capacity");
   });
 }
 
@@ -695,6 +726,22 @@ brook_status brook_prepare_fields(brook_context *ctx, const brook_volume *labels
       if (max_daf) max_daf[i] = f.max_daf[i];
     }
     if (count) *count = f.count;
+  });
+}
+
+brook_status brook_target_order(brook_context *ctx, const brook_volume *ids, const float *daf, int64_t *order,
+                                size_t capacity, size_t *count) {
+  return protect([&] {
+    require(ctx && ids && count && (daf || !ids->shape[0]));
+    *count = 0;
+    require(ids->dtype == BROOK_U8 || ids->dtype == BROOK_U16 || ids->dtype == BROOK_U32,
+            "target orders take unsigned component ids of at most 32 bits");
+    brook::ContextCall call(ctx->value);
+    Input in = normalize(ctx->value, ids);
+    const std::vector<int> got = brook::target_orders(ctx->value, in.view, daf);
+    *count = got.size();
+    require(got.size() <= capacity && (order || got.empty()), "order capacity below the foreground voxel count");
+    std::copy(got.begin(), got.end(), order);
   });
 }
 

@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
+#include <string>
 
 #include "stages.hpp"
 
@@ -327,6 +329,107 @@ template <class CC> void run_analyze(Context &ctx, const Components &cc, Dims d,
 }
 
 }  // namespace
+
+// ---- connected components without device allocation ------------------------------------------
+// The working memory of connected_components_noalloc: one static array per device (the loader
+// places it in each device's memory with the code object). Host code never passes `scratch` itself
+// to a kernel (on the host the name is a stand-in, not a device address): each device's address is
+// looked up once, with that device current, by noalloc_scratch_init, and kept in g_scratch.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_lock[kMaxDevices];  // one call at a time per device: calls share the array
+void *g_scratch[kMaxDevices] = {};
+std::mutex g_scratch_init;
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+template <class T>
+int64_t run_ccl_noalloc(Context &ctx, const LabelVolume &in, uint32_t *ids, int64_t *mapping, int64_t *roots,
+                        int64_t capacity, unsigned char *base) {
+  const int n = int(in.size());
+  const Dims d = in.dims;
+  const T *lab = static_cast<const T *>(in.data);
+  const int tiles = int(blocks_for(n, kTile));
+  int *parent = reinterpret_cast<int *>(base);
+  size_t at = align256(size_t(n) * 4);
+  int *tile = reinterpret_cast<int *>(base + at);
+  at += align256(size_t(tiles) * 4);
+  long long *total = reinterpret_cast<long long *>(base + at);
+  at += 256;
+  long long *roots_dev = reinterpret_cast<long long *>(base + at);
+  at += align256(size_t(capacity) * 8);
+  long long *mapping_dev = reinterpret_cast<long long *>(base + at);
+  auto s = ctx.stream();
+  ccl_init<T><<<blocks_for(n, 256), 256, 0, s>>>(lab, parent, d, n);
+  BROOK_LAUNCHED();
+  ccl_union<T><<<blocks_for(n, 256), 256, 0, s>>>(lab, parent, d, n);
+  BROOK_LAUNCHED();
+  ccl_count<<<tiles, 256, 0, s>>>(parent, n, tile);
+  BROOK_LAUNCHED();
+  scan_tiles<<<1, 1024, 0, s>>>(tile, tiles, total);
+  BROOK_LAUNCHED();
+  long long count = 0;
+  BROOK_HIP(hipMemcpyAsync(&count, total, 8, hipMemcpyDeviceToHost, s));
+  ctx.synchronize();
+  if (count > capacity) return count;  // the caller reports it; no table is written
+  ccl_mark<T><<<tiles, 256, 0, s>>>(lab, parent, n, tile, roots_dev, mapping_dev);
+  BROOK_LAUNCHED();
+  ccl_relabel<T, uint32_t><<<blocks_for(n, 256), 256, 0, s>>>(parent, n, ids);
+  BROOK_LAUNCHED();
+  if (count) {
+    BROOK_HIP(hipMemcpyAsync(roots, roots_dev, size_t(count) * 8, hipMemcpyDeviceToHost, s));
+    BROOK_HIP(hipMemcpyAsync(mapping + 1, mapping_dev + 1, size_t(count) * 8, hipMemcpyDeviceToHost, s));
+  }
+  ctx.synchronize();
+  mapping[0] = 0;
+  if (in.is_signed && sizeof(T) < 8)
+    for (long long i = 1; i <= count; ++i) {
+      const int shift = 64 - 8 * int(sizeof(T));
+      mapping[i] = int64_t(uint64_t(mapping[i]) << shift) >> shift;  // sign-extend the label's width
+    }
+  return count;
+}
+}  // namespace
+
+void noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) throw std::invalid_argument("device index beyond the no-allocation table");
+  std::lock_guard<std::mutex> lock(g_scratch_init);
+  if (g_scratch[device]) return;
+  void *p = nullptr;
+  BROOK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));  // this device's copy (it is current)
+  if (!p) throw DeviceError(hipErrorInvalidSymbol, "no device address for the no-allocation scratch");
+  g_scratch[device] = p;
+}
+
+size_t noalloc_scratch_need(int64_t voxels, int64_t capacity) {
+  return align256(size_t(voxels) * 4) + align256(size_t(blocks_for(voxels, kTile)) * 4) + 256 +
+         align256(size_t(capacity) * 8) + size_t(capacity + 1) * 8;
+}
+
+int64_t connected_components_noalloc(Context &ctx, const LabelVolume &input, uint32_t *ids, int64_t *mapping,
+                                     int64_t *roots, int64_t capacity) {
+  const int device = ctx.device();
+  if (device < 0 || device >= kMaxDevices) throw std::invalid_argument("device index beyond the no-allocation table");
+  if (!g_scratch[device]) noalloc_scratch_init(device);
+  if (input.size() == 0) {
+    mapping[0] = 0;
+    return 0;
+  }
+  if (noalloc_scratch_need(input.size(), capacity) > kNoAllocScratchBytes)
+    throw std::invalid_argument("volume and table capacity exceed the no-allocation scratch (" +
+                                std::to_string(kNoAllocScratchBytes >> 20) +
+                                " MiB: 4 bytes per voxel plus 16 per table entry)");
+  std::lock_guard<std::mutex> lock(g_scratch_lock[device]);
+  unsigned char *base = static_cast<unsigned char *>(g_scratch[device]);
+  switch (input.bytes) {
+    case 1: return run_ccl_noalloc<uint8_t>(ctx, input, ids, mapping, roots, capacity, base);
+    case 2: return run_ccl_noalloc<uint16_t>(ctx, input, ids, mapping, roots, capacity, base);
+    case 4: return run_ccl_noalloc<uint32_t>(ctx, input, ids, mapping, roots, capacity, base);
+    case 8: return run_ccl_noalloc<unsigned long long>(ctx, input, ids, mapping, roots, capacity, base);
+    default: throw std::invalid_argument("connected components requires integer labels");
+  }
+}
 
 Components connected_components(Context &ctx, const LabelVolume &input) {
   if (input.size() >= (int64_t(1) << 31) - kTile) throw Unsupported("volume exceeds the int32 voxel index space");

@@ -311,6 +311,7 @@ class Sample {
       fill.paint(ctx_);
       for (size_t o = 1; o < k_; ++o)
         if (fill.taken()[o] && o != id) skip[o] = 1;
+      ctx_.stats["hole_filled_voxels"] += double(fill.voids());
       painted = true;
     }
     if (!painted) return;
@@ -672,6 +673,30 @@ Packed skeletonize(Context &ctx, const LabelVolume &in, const SkeletonizeOptions
   group.push_back(std::make_unique<Sample>(ctx, in, o, fields));
   run_group(ctx, group, lap);
   return group[0]->take_result();
+}
+
+std::vector<int> target_orders(Context &ctx, const LabelVolume &ids, const float *daf) {
+  if (ids.bytes > 4) throw std::invalid_argument("target orders take component ids of at most 32 bits");
+  const int64_t n = ids.size();
+  std::vector<int> out;
+  if (n == 0) return out;
+  const size_t b = size_t(ids.bytes);
+  std::vector<uint8_t> raw(size_t(n) * b);
+  BROOK_HIP(hipMemcpyAsync(raw.data(), ids.data, raw.size(), hipMemcpyDeviceToHost, ctx.stream()));
+  ctx.synchronize();
+  int64_t F = 0, top = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    uint32_t v = 0;
+    std::memcpy(&v, raw.data() + size_t(i) * b, b);
+    F += v != 0;
+    top = std::max<int64_t>(top, v);
+  }
+  if (F == 0) return out;
+  Buffer<float> daf_dev(ctx, size_t(n));
+  daf_dev.upload(ctx, daf, size_t(n));
+  Buffer<int> order(ctx, size_t(F));
+  target_order(ctx, ids.data, ids.bytes, n, daf_dev.data(), F, top, order.data());
+  return order.download(ctx);
 }
 
 std::vector<Packed> skeletonize_batch(Context &ctx, const std::vector<LabelVolume> &volumes, const SkeletonizeOptions &o) {

@@ -39,5 +39,8 @@ Packed skeletonize(Context &ctx, const LabelVolume &labels, const SkeletonizeOpt
 // and one trace launch.
 std::vector<Packed> skeletonize_batch(Context &ctx, const std::vector<LabelVolume> &volumes,
                                       const SkeletonizeOptions &options);
+// The tracer's target orders of every nonzero id of a component volume (the engine's sort on the
+// device), concatenated in ascending id: flat indices, one per nonzero voxel. daf: host, n floats.
+std::vector<int> target_orders(Context &ctx, const LabelVolume &ids, const float *daf);
 
 }  // namespace brook

@@ -42,6 +42,17 @@ struct Box {
 inline int id_bytes(int64_t count) { return count <= 255 ? 1 : (count <= 65535 ? 2 : 4); }
 
 Components connected_components(Context &ctx, const LabelVolume &input);
+// connected_components without device allocation: its working memory is a static device array of
+// kNoAllocScratchBytes per device (ccl.hip). Device labels in Fortran order in, u32 ids to the
+// caller's device buffer `ids` (n entries), mapping (capacity + 1) and roots (capacity) to host
+// arrays. Returns the component count; when it exceeds `capacity` no table or id is written.
+// Volumes whose working set (noalloc_scratch_need) exceeds the array are rejected. Calls on one
+// device are serialized (each returns after its kernels finished).
+constexpr size_t kNoAllocScratchBytes = size_t(128) << 20;
+void noalloc_scratch_init(int device);  // with `device` current: look up its copy of the array
+size_t noalloc_scratch_need(int64_t voxels, int64_t capacity);
+int64_t connected_components_noalloc(Context &ctx, const LabelVolume &input, uint32_t *ids, int64_t *mapping,
+                                     int64_t *roots, int64_t capacity);
 // Per-id bounding boxes and voxel counts (index 0: background count).
 std::vector<Box> analyze(Context &ctx, const Components &cc, Dims dims);
 // Multi-label anisotropic EDT of cc ids with the reference's rounding rule; background 0.

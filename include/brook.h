@@ -131,6 +131,22 @@ brook_status brook_components_view(const brook_components *components, brook_vol
 brook_status brook_components_copy_to_host(const brook_components *components, void *out, size_t bytes);
 brook_status brook_components_labels(const brook_components *components, brook_array **out);
 
+/* EXTENSION with no counterpart in the reference: brook_connected_components for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc); the call makes no device allocation. labels: device memory on the context's
+ * device, Fortran-contiguous. ids_out: device memory of at least 4 bytes per voxel; receives the
+ * ids as uint32 in Fortran order. mapping (capacity + 1 entries) and roots (capacity entries):
+ * host arrays, as brook_components_view gives them. *count receives the component count; a count
+ * above capacity returns BROOK_INVALID_ARGUMENT with only *count written. The working memory is a
+ * static device array of 128 MiB per device, held from the first context on that device until the
+ * process ends; a volume needing more (4 bytes per voxel plus 16 per table entry, rounded) is
+ * rejected with BROOK_INVALID_ARGUMENT. Every call on one device shares that array, which keeps
+ * the last call's contents: two calls on one GPU would race on it, so the library runs them one at
+ * a time (a per-device lock; each call returns after its kernels finished). */
+brook_status brook_connected_components_noalloc(brook_context *context, const brook_volume *labels, void *ids_out,
+                                                size_t ids_bytes, int64_t *mapping, int64_t *roots, size_t capacity,
+                                                size_t *count);
+
 /* brook.h:183-184: multi-label anisotropic EDT (float32, Fortran order, background 0). */
 brook_status brook_edt(brook_context *context, const brook_volume *labels, const float anisotropy[3], int black_border,
                        brook_array **out);
@@ -186,6 +202,14 @@ brook_status brook_prepare_fields(brook_context *context, const brook_volume *la
                                   const brook_skeletonize_options *options, float *dbf, float *daf, float *pdrf,
                                   size_t table_capacity, uint8_t *traced, int64_t *root, int64_t *target,
                                   float *dbf_max, float *max_daf, int64_t *count);
+
+/* EXTENSION: the tracer's target orders (lockstep_order.cu:47-151) of a component volume, for
+ * stage-level checks: for every nonzero id in ascending order, the id's voxels (flat Fortran
+ * indices) by descending DAF (+-inf and +-0 as 0), ties by descending index, concatenated into
+ * order (capacity entries; *count receives the foreground voxel count). ids: unsigned, at most 32
+ * bits, host or device; daf: n float32 host values in Fortran order. */
+brook_status brook_target_order(brook_context *context, const brook_volume *ids, const float *daf, int64_t *order,
+                                size_t capacity, size_t *count);
 
 /* EXTENSION: a timing/counter of the context's last call (e.g. "trace_ms", "paths"; with kernel
  * timing on, "k_<kernel>_ms" and "k_<kernel>_n" from events on the context's stream). */

@@ -53,6 +53,12 @@ def lib():
         L.oracle_border_targets.argtypes = [_P, _I64, _I64, _I64, _I64, _P, _P, _I64, _P]
         L.oracle_geometric_flood.restype = _I
         L.oracle_geometric_flood.argtypes = [_P, _I64, _I64, _I64, _P, _P, _I64, _P]
+        L.oracle_target_order.restype = _I64
+        L.oracle_target_order.argtypes = [_P, _I64, _P, _P]
+        L.oracle_invalidate.restype = _I64
+        L.oracle_invalidate.argtypes = [_P, _I64, _I64, _I64, _P, _P, _P, _I64, _P, ctypes.c_float, ctypes.c_float, _P]
+        L.oracle_analyze.restype = _I
+        L.oracle_analyze.argtypes = [_P, _I64, _I64, _I64, _I64, _P, _P, _P]
         L.oracle_railroad.restype = _I64
         L.oracle_railroad.argtypes = [_P, _P, _I64, _I64, _I64, _I64, _P, _I64]
         L.oracle_prepare.restype = _I64
@@ -154,6 +160,46 @@ def geometric_flood(cc, anisotropy, sources):
     if lib().oracle_geometric_flood(_ptr(cc), *cc.shape, _ptr(an), _ptr(src), len(src), _ptr(out)):
         _fail("geometric_flood")
     return out
+
+
+def target_order(cc, daf):
+    """Per-id target orders of a component volume (ids ascending, each the id's voxels by DAF
+    descending with +-inf and +-0 as 0, ties by flat index descending), concatenated: flat indices."""
+    cc = np.asfortranarray(np.asarray(cc).astype(np.uint32, copy=False))
+    daf = np.asfortranarray(np.asarray(daf, np.float32))
+    out = np.zeros(max(int(np.count_nonzero(cc)), 1), np.int64)
+    n = lib().oracle_target_order(_ptr(cc), cc.size, _ptr(daf), _ptr(out))
+    if n < 0:
+        _fail("target_order")
+    return out[:n]
+
+
+def invalidate(cc, dbf, alive, path, anisotropy, scale, constant):
+    """One rolling-ball invalidation of `path` (flat indices) in the label of path[0]:
+    (claimed count, alive after, claim keys: uint64 (bottleneck bits << 32) | source, all ones where
+    unclaimed)."""
+    cc = np.asfortranarray(np.asarray(cc).astype(np.uint32, copy=False))
+    dbf = np.asfortranarray(np.asarray(dbf, np.float32))
+    alive = np.asfortranarray(np.asarray(alive).astype(np.uint8))
+    p = np.ascontiguousarray(np.asarray(path, np.int64))
+    an = np.asarray(anisotropy, np.float32)
+    claims = np.zeros(cc.shape, np.uint64, order="F")
+    n = lib().oracle_invalidate(_ptr(cc), *cc.shape, _ptr(dbf), _ptr(alive), _ptr(p), len(p), _ptr(an),
+                                float(scale), float(constant), _ptr(claims))
+    if n < 0:
+        _fail("invalidate")
+    return n, alive, claims
+
+
+def analyze(cc, count):
+    """Per-id boxes and counts: (lo (count+1, 3), hi (count+1, 3), counts (count+1))."""
+    cc = np.asfortranarray(np.asarray(cc).astype(np.uint32, copy=False))
+    lo = np.zeros((count + 1, 3), np.int64)
+    hi = np.zeros((count + 1, 3), np.int64)
+    counts = np.zeros(count + 1, np.int64)
+    if lib().oracle_analyze(_ptr(cc), *cc.shape, count, _ptr(lo), _ptr(hi), _ptr(counts)):
+        _fail("analyze")
+    return lo, hi, counts
 
 
 def railroad(field, target, mask=None):

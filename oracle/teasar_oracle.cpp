@@ -911,6 +911,9 @@ struct Tracer {
   std::vector<uint8_t> &alive;
   std::vector<uint8_t> &inq;     // 0 idle / 1 frontier / 2 waiting (invalidation)
   float wx, wy, wz, scale, constant, delta;
+  // When set, each invalidation leaves here the final claim key of every voxel it claimed
+  // ((bottleneck bits << 32) | source; MAXKEY elsewhere), for the stage-level checks.
+  std::vector<u64> *claims = nullptr;
 
   // Railroad (lockstep_coop.inc:71-305 with brook_vw_step) + backtrace and join
   // (lockstep_regular.inc:61-154). Returns the path target-first; empty: no rail reachable.
@@ -1099,12 +1102,27 @@ struct Tracer {
       }
       far.swap(keep);
     }
+    if (claims)
+      for (i64 v : touched) (*claims)[v] = key[v];
     for (i64 v : touched) { alive[v] = 0; key[v] = MAXKEY; inq[v] = 0; }
     return i64(touched.size());
   }
 };
 
 struct Record { uint32_t cc; std::vector<i64> path; };
+
+// Target order of one label's voxels (lockstep_order.cu:21-43, the rule of
+// component_order_smoke.cpp:12-16): DAF descending with +-inf and +-0 as 0, ties by flat index
+// descending.
+void sort_target_order(std::vector<i64> &voxels, const float *daf) {
+  std::sort(voxels.begin(), voxels.end(), [&](i64 a, i64 b) {
+    float da = daf[a], db = daf[b];
+    if (std::isinf(da) || da == 0) da = 0;
+    if (std::isinf(db) || db == 0) db = 0;
+    if (da != db) return da > db;
+    return a > b;
+  });
+}
 
 struct Skeleton {
   i64 label = 0;
@@ -1222,13 +1240,7 @@ Result skeletonize(const i64 *labels, const Shape &s, const Params &p, const std
   parallel_for(labels_list.size(), threads, [&](size_t li) {
     uint32_t L = uint32_t(labels_list[li]);
     auto &ord = order[L];
-    std::sort(ord.begin(), ord.end(), [&](i64 a, i64 b) {
-      float da = pr.daf[a], db = pr.daf[b];
-      if (std::isinf(da) || da == 0) da = 0;
-      if (std::isinf(db) || db == 0) db = 0;
-      if (da != db) return da > db;
-      return a > b;
-    });
+    sort_target_order(ord, pr.daf.data());
     Tracer tr{s, pr.comp.cc, pr.dbf, pr.pdrf, dist, key, alive, inq, p.aniso[0], p.aniso[1], p.aniso[2],
               float(p.scale), float(p.constant), aniso_min};
     size_t cursor = 0;
@@ -1438,6 +1450,67 @@ int oracle_geometric_flood(const uint32_t *cc, int64_t sx, int64_t sy, int64_t s
     geometric_weights(aniso, W);
     geometric_flood(v, s, W, std::vector<i64>(sources, sources + nsources), d);
     std::copy(d.begin(), d.end(), dist_out);
+  });
+}
+
+// Target orders of every nonzero id (ascending id, each the id's voxels in the order of
+// sort_target_order), concatenated into order_out (one entry per nonzero voxel). Returns the count.
+int64_t oracle_target_order(const uint32_t *cc, int64_t n, const float *daf, int64_t *order_out) {
+  int64_t total = -1;
+  guarded([&] {
+    uint32_t top = 0;
+    for (int64_t v = 0; v < n; ++v) top = std::max(top, cc[v]);
+    std::vector<std::vector<i64>> order(size_t(top) + 1);
+    for (int64_t v = 0; v < n; ++v)
+      if (cc[v]) order[cc[v]].push_back(v);
+    int64_t pos = 0;
+    for (auto &ord : order) {
+      sort_target_order(ord, daf);
+      for (i64 v : ord) order_out[pos++] = v;
+    }
+    total = pos;
+  });
+  return total;
+}
+
+// One rolling-ball invalidation of `path` (voxel list) in label cc[path[0]]: alive (n bytes) is
+// updated in place (claimed voxels die) and claims_out (n u64, optional) receives each claimed
+// voxel's final key ((bottleneck bits << 32) | source), MAXKEY elsewhere. Returns the claim count.
+int64_t oracle_invalidate(const uint32_t *cc, int64_t sx, int64_t sy, int64_t sz, const float *dbf, uint8_t *alive,
+                          const int64_t *path, int64_t len, const float *aniso, float ball_scale, float ball_constant,
+                          uint64_t *claims_out) {
+  int64_t claimed = -1;
+  guarded([&] {
+    Shape s{sx, sy, sz};
+    if (len <= 0) throw std::invalid_argument("empty path");
+    std::vector<uint32_t> c(cc, cc + s.n());
+    std::vector<float> d(dbf, dbf + s.n()), f(s.n(), 1.0f), dist(s.n(), FINF);
+    std::vector<u64> key(s.n(), MAXKEY), claims(s.n(), MAXKEY);
+    std::vector<uint8_t> al(alive, alive + s.n()), inq(s.n(), 0);
+    const float delta = std::min({aniso[0], aniso[1], aniso[2]});
+    Tracer tr{s, c, d, f, dist, key, al, inq, aniso[0], aniso[1], aniso[2], ball_scale, ball_constant, delta};
+    tr.claims = &claims;
+    claimed = tr.invalidate(c[path[0]], std::vector<i64>(path, path + len));
+    std::copy(al.begin(), al.end(), alive);
+    if (claims_out) std::copy(claims.begin(), claims.end(), claims_out);
+  });
+  return claimed;
+}
+
+// Per-id boxes and voxel counts of a component volume with ids 1..count: lo/hi 3*(count+1) int64
+// (lo INT32_MAX and hi -1 for an absent id), counts count+1 (counts[0]: background).
+int oracle_analyze(const uint32_t *cc, int64_t sx, int64_t sy, int64_t sz, int64_t count, int64_t *lo, int64_t *hi,
+                   int64_t *counts) {
+  return guarded([&] {
+    Shape s{sx, sy, sz};
+    std::vector<uint32_t> c(cc, cc + s.n());
+    for (uint32_t id : c)
+      if (int64_t(id) > count) throw std::invalid_argument("id above count");
+    auto boxes = analyze(c, s, size_t(count));
+    for (size_t i = 0; i < boxes.size(); ++i) {
+      for (int a = 0; a < 3; ++a) { lo[3 * i + a] = boxes[i].lo[a]; hi[3 * i + a] = boxes[i].hi[a]; }
+      counts[i] = boxes[i].count;
+    }
   });
 }
 

@@ -167,3 +167,34 @@ def soma_cavity(size=48, nucleus=False, staircase=False):
         for i in range(12):
             labels[c + 5 + i, c + i, c] = 3
     return np.asfortranarray(labels)
+
+
+# ---- fill_holes (the reference's test_package.py:1422-1435 input, 1457-1458 options) ----
+
+HOLE_PARAMS = dict(scale=1.5, const=2, pdrf_scale=100000, pdrf_exponent=4, soma_detection_threshold=100,
+                   soma_acceptance_threshold=120, soma_invalidation_scale=1.0, soma_invalidation_const=0)
+
+
+def painted_root():
+    """Label 1's lowest voxels lie in label 2's cavity and it leaves through a diagonal step in the
+    wall, so filling label 2's holes paints label 1's first voxel; a dust speck (label 3) lies
+    between label 1's first voxel before and after the fill."""
+    labels = np.zeros((22, 22, 26), np.uint32)
+    labels[3:17, 3:17, 3:17] = 2
+    labels[5:15, 5:15, 5:15] = 0
+    labels[6:13, 6:13, 6:12] = 1
+    labels[9, 9, 12:16] = 1
+    labels[10, 10, 16:18] = 1
+    labels[6:16, 6:16, 18:24] = 1
+    labels[19, 19, 13] = 3
+    return np.asfortranarray(labels)
+
+
+# Kimimaro 5.8.1 summaries of painted_root() with HOLE_PARAMS, dust 10, fix_borders=False,
+# fill_holes=True (test_package.py:1810-1811): key order [2, 1]; with branching correction the
+# reference compares label 2 by its first three fields only. 1001 voxels are filled.
+PAINTED_ROOT_EXPECTED = {
+    False: {2: [38, 37, 7.0, "8850fde826c023e3"], 1: [24, 23, 3.1622776985168457, "bd25239c8e51a899"]},
+    True: {2: [35, 34, 7.0], 1: [20, 19, 3.1622776985168457, "13904653524930cd"]},
+}
+PAINTED_ROOT_FILLED = 1001

@@ -144,3 +144,31 @@ def test_gpu_fill_holes_with_somata(brook):
         assert got["labels"].tolist() == want["labels"].tolist()
         for name in ("v_off", "e_off", "vertices", "edges", "radii"):
             assert np.array_equal(np.asarray(got[name]), np.asarray(want[name])), name
+
+
+def _matches_painted_root(summary, fix_branching):
+    want = shapes.PAINTED_ROOT_EXPECTED[fix_branching]
+    return list(summary) == [2, 1] and all(summary[label][:len(v)] == v for label, v in want.items())
+
+
+@pytest.mark.parametrize("fix_branching", [False, True])
+def test_oracle_painted_root_equals_reference_digests(fix_branching):
+    """The reference's Kimimaro 5.8.1 summaries of its painted-root sample (test_package.py:1786-1811)."""
+    labels = shapes.painted_root()
+    for fix_avocados in (False, True):  # no avocado candidate exceeds the sample's thresholds
+        got = oracle.skeletonize(labels, shapes.HOLE_PARAMS, (1, 1, 1), 10, fix_branching=fix_branching,
+                                 fix_borders=False, fill_holes=True, fix_avocados=fix_avocados)
+        assert _matches_painted_root(shapes.summary_from_packed(got), fix_branching), fix_avocados
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fix_branching", [False, True])
+def test_gpu_painted_root_equals_reference_digests(brook, fix_branching):
+    from brook_amd import _native as native
+    labels = shapes.painted_root()
+    for fix_avocados in (False, True):
+        got = brook.skeletonize(labels, shapes.HOLE_PARAMS, dust_threshold=10, fix_branching=fix_branching,
+                                fix_borders=False, fill_holes=True, fix_avocados=fix_avocados, output="packed",
+                                progress=False)
+        assert native.context().stat("hole_filled_voxels") == shapes.PAINTED_ROOT_FILLED
+        assert _matches_painted_root(shapes.summary_from_packed(got.as_dict()), fix_branching), fix_avocados
