@@ -7,7 +7,7 @@ dust 1000, fix_borders) already resident in HBM as a Fortran-ordered tensor.
 
 Prints one JSON line (rank 0): throughput, p50/p99 step latency, the roofline of the dominant
 kernel (algorithmic bytes / event-timed kernel duration on the engine's own stream) and the
-CPU oracle timed on a bounded crop of the same volume on this host's cores.
+CPU oracle timed on the same volume on this host's cores.
 """
 from __future__ import annotations
 
@@ -67,6 +67,20 @@ def _traffic(kernel):
         return json.load(open(path)).get(kernel, {}).get("bytes_per_launch")
     except (OSError, ValueError):
         return None
+
+
+def edt_error(dev):
+    """The GPU EDT (DBF) of the workload against the digest of the oracle's float32 DBF; the
+    oracle's relative error against the float64 EDT (computed on the CPU when the digest was
+    written, tests/golden/workload_digests.py) is then the GPU's."""
+    import hashlib
+
+    from brook_amd import device
+    want = json.load(open(os.path.join(ROOT, "tests", "golden", "workload_digests.json")))["edt"]["trees_512"]
+    dbf = device.edt(dev, ANISO)
+    got = hashlib.sha256(dbf.permute(2, 1, 0).contiguous().cpu().numpy().tobytes()).hexdigest()
+    return dict(equals_oracle_dbf=got == want["sha256"], max_rel_err_f64=want["max_rel_err_f64"],
+                mean_rel_err_f64=want["mean_rel_err_f64"], foreground_voxels=want["foreground_voxels"])
 
 
 def host_io(vol, calls=3):
@@ -137,8 +151,9 @@ def merge(result, fragments=8, calls=5):
 
 
 def batched():
-    """64 mixed-shape volumes (64..256 per axis, 1-50 tube labels each, ~254 M voxels in all),
-    device-resident: one skeletonize_batch call against 64 plain calls."""
+    """64 mixed-shape volumes (64..256 per axis, 1-50 tube labels each, ~181 M voxels in all),
+    device-resident: one skeletonize_batch call against 64 plain calls; every sample's batch result
+    must equal its plain call byte for byte."""
     import torch
 
     import brook_amd
@@ -150,17 +165,24 @@ def batched():
     brook_amd.skeletonize_batch(dev[:4], params, aniso, dust_threshold=1000, output="packed")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    brook_amd.skeletonize_batch(dev, params, aniso, dust_threshold=1000, output="packed")
+    batch = brook_amd.skeletonize_batch(dev, params, aniso, dust_threshold=1000, output="packed")
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
+    plain = []
     t0 = time.perf_counter()
     for d in dev:
-        brook_amd.skeletonize(d, params, aniso, dust_threshold=1000, output="packed", progress=False)
+        plain.append(brook_amd.skeletonize(d, params, aniso, dust_threshold=1000, output="packed", progress=False))
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
+    names = ("labels", "v_off", "e_off", "vertices", "edges", "radii")
+    differ = [i for i, p in enumerate(plain)
+              if not all(np.array_equal(np.asarray(batch[i].as_dict()[k]), np.asarray(p.as_dict()[k])) for k in names)]
+    if differ:
+        raise RuntimeError(f"batch results differ from plain calls for samples {differ}")
     return dict(voxels=int(total), samples=len(vols), batch_mvox_s=round(total / t_batch / 1e6, 3),
                 plain_mvox_s=round(total / t_plain / 1e6, 3), batch_ms=round(t_batch * 1e3, 3),
-                plain_ms=round(t_plain * 1e3, 3), speedup=round(t_plain / t_batch, 3))
+                plain_ms=round(t_plain * 1e3, 3), speedup=round(t_plain / t_batch, 3),
+                skeletons=int(sum(len(p) for p in plain)), batch_equals_plain=True)
 
 
 def stress(calls=2):
@@ -202,37 +224,41 @@ def stress(calls=2):
     return out
 
 
-def served(rates=(10.0,), count=24, seed=5005):
-    """Serving form: Poisson request streams, each request a host uint32 volume with every edge in
-    [128, 512] and 5-50 tube labels (8 distinct volumes drawn at random per request). The same
-    arrival sequence is served by the continuous-batching server and by a serial server (plain
-    calls, max_batch=1). Latency = result ready - arrival."""
+def served(rates=(20.0, 50.0), count=40, seed=5005):
+    """Serving form: Poisson request streams; request i is synth.serving_request(i) (a distinct
+    host uint32 tube volume, every edge in [128, 512], 5-50 labels). The same arrival sequence is
+    served by the continuous-batching server (two slots) and by a serial server (one slot, plain
+    calls). Latency = result ready - arrival."""
+    from concurrent.futures import ThreadPoolExecutor
+
     import brook_amd
     from brook_amd import synth
-    rng = np.random.default_rng(seed)
-    pool = []
-    for i in range(8):
-        shape = tuple(int(x) for x in rng.integers(128, 513, 3))
-        pool.append(synth.volume("tubes", shape, int(rng.integers(5, 51)), seed * 10000 + i))
+    with ThreadPoolExecutor(16) as pool:
+        vols = list(pool.map(synth.serving_request, range(count)))
     params, aniso = PARAMS, (16, 16, 40)
-    brook_amd.skeletonize(pool[0], params, aniso, dust_threshold=1000, output="packed", progress=False)
+    brook_amd.skeletonize(vols[0], params, aniso, dust_threshold=1000, output="packed", progress=False)
+    rng = np.random.default_rng(seed)
+    voxels = sum(v.size for v in vols)
     out = {}
     for lam in rates:
         gaps = rng.exponential(1.0 / lam, count)
-        picks = rng.integers(0, len(pool), count)
-        voxels = sum(pool[j].size for j in picks)
         row = {}
-        for mode, max_batch in (("batched", 64), ("serial", 1)):
-            with brook_amd.Server(params, aniso, 1000, max_batch=max_batch) as srv:
+        for mode, slots, max_batch in (("batched", 2, 64), ("serial", 1, 1)):
+            with brook_amd.Server(params, aniso, 1000, max_batch=max_batch, slots=slots) as srv:
+                srv.submit(vols[0]).result()  # each slot's context warm
+                if slots > 1:
+                    a, b = srv.submit(vols[1]), srv.submit(vols[2])
+                    a.result(), b.result()
+                srv.batches.clear()
                 t0 = time.perf_counter()
                 at = t0
                 futures = []
-                for i, (g, j) in enumerate(zip(gaps, picks)):
+                for i, g in enumerate(gaps):
                     at += g
                     wait = at - time.perf_counter()
                     if wait > 0:
                         time.sleep(wait)
-                    futures.append(srv.submit(pool[j]))
+                    futures.append(srv.submit(vols[i]))
                     if (i + 1) % 50 == 0:
                         _progress(f"served {lam:g} req/s, {mode}: {i + 1} of {count} submitted")
                 for f in futures:
@@ -243,21 +269,44 @@ def served(rates=(10.0,), count=24, seed=5005):
                 row[mode] = dict(p50_ms=round(float(np.percentile(lat, 50)), 2),
                                  p99_ms=round(float(np.percentile(lat, 99)), 2),
                                  req_per_s=round(count / span, 3), mvox_s=round(voxels / span / 1e6, 3),
-                                 mean_batch=round(float(np.mean(srv.batches)), 2))
+                                 mean_batch=round(float(np.mean(srv.batches)), 2),
+                                 overlapped=round(float(np.mean([f.request.overlapped for f in futures])), 3))
         out[f"{lam:g}"] = dict(requests=count, **row)
     return out
 
 
+def _host_cpu():
+    """nproc and the CPU model name of this host."""
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return len(os.sched_getaffinity(0)), model
+
+
 def cpu_baseline(vol, threads):
-    """The oracle (a C++ port of the reference's algorithm) on a 256^3 crop of the same volume."""
+    """The oracle (a C++ port of the reference's algorithm, threaded over labels) on the same
+    512^3 volume as the GPU steps, and on the 128^3 / 20-label tube volume (anisotropy (16,16,40))."""
     import oracle
-    crop = np.asfortranarray(vol[:256, :256, :256])
+    from brook_amd import synth
     t0 = time.perf_counter()
-    oracle.skeletonize(crop, PARAMS, ANISO, 1000, threads=threads)
+    oracle.skeletonize(vol, PARAMS, ANISO, 1000, threads=threads)
     dt = time.perf_counter() - t0
-    return dict(value=crop.size / dt / 1e6, unit="Mvoxels/s", cores=threads, kind="port",
-                sample=f"oracle on the 256^3 corner crop of the 512^3 volume ({crop.size} voxels), "
-                       f"{threads} threads, {dt:.2f} s")
+    small = synth.tubes_128()
+    t1 = time.perf_counter()
+    oracle.skeletonize(small, PARAMS, (16, 16, 40), 1000, threads=threads)
+    dt1 = time.perf_counter() - t1
+    nproc, model = _host_cpu()
+    return dict(value=round(vol.size / dt / 1e6, 3), unit="Mvoxels/s", cores=threads, kind="port",
+                sample=f"oracle on the whole 512^3 / 200-label volume ({vol.size} voxels), {threads} threads, "
+                       f"{dt:.2f} s",
+                nproc=nproc, cpu_model=model,
+                tubes_128=dict(value=round(small.size / dt1 / 1e6, 3), unit="Mvoxels/s", seconds=round(dt1, 3),
+                               sample="128^3 uint32, 20 tube labels, anisotropy (16,16,40)"))
 
 
 def _progress(what):
@@ -274,8 +323,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the host-input, fix_branching=False, merge, batched, stress and served measurements")
     ap.add_argument("--served-sweep", action="store_true",
-                    help="serve at 1, 2, 5, 10, 20 and 50 requests/s (default: 10 only)")
-    ap.add_argument("--served-requests", type=int, default=24, help="requests per arrival rate")
+                    help="serve at 1, 2, 5, 10, 20 and 50 requests/s (default: 20 and 50)")
+    ap.add_argument("--served-requests", type=int, default=40, help="requests per arrival rate")
     ap.add_argument("--served-rates", default="1,2,5,10,20,50",
                     help="arrival rates (req/s) of --served-sweep, comma-separated")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
@@ -375,6 +424,7 @@ def main():
         generator_s=round(gen_s, 2),
     )
     _progress(f"plain steps done ({out['value']} Mvoxels/s)")
+    out["edt"] = edt_error(dev)
     if not args.no_extras:
         out["host_io"] = host_io(vol)
         _progress("host_io done")
@@ -387,7 +437,7 @@ def main():
         out["stress"] = stress()
         _progress("stress done")
         rates = tuple(float(r) for r in args.served_rates.split(","))
-        out["served"] = served(rates if args.served_sweep else (10.0,),
+        out["served"] = served(rates if args.served_sweep else (20.0, 50.0),
                                count=args.served_requests)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(vol, args.cpu_threads)

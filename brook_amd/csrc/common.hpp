@@ -249,6 +249,13 @@ constexpr unsigned kAlive = 1u << 26, kWaiting = 1u << 29;
 
 __device__ __forceinline__ float rec_pdrf(const VoxelRec *r, int v) { return lp_f(&r[v].pdrf); }
 __device__ __forceinline__ unsigned rec_meta(const VoxelRec *r, int v) { return lp_u(&r[v].meta); }
+// The whole record in one 16-byte load (x = pdrf bits, y = meta, z = word low, w = word high), for
+// walks that need two of its fields: one scattered access instead of two. During the flood and
+// trace kernels only the workgroup that owns the voxel's label touches its record, as for the
+// workgroup-scope loads above.
+__device__ __forceinline__ uint4 rec_load(const VoxelRec *r, int v) {
+  return *gp(reinterpret_cast<const uint4 *>(r + v));
+}
 // The high half of the word (distance bits or claim bottleneck) without the 64-bit load.
 __device__ __forceinline__ unsigned rec_word_hi(const VoxelRec *r, int v) {
   return lp_u(reinterpret_cast<const unsigned *>(&r[v].word) + 1);
@@ -277,18 +284,23 @@ constexpr int kSpan = 7;
 struct Group {
   int G, i, g, lead, kb, cnt;  // lanes per entry, entry, lane in group, group's first lane, span
 };
-__device__ __forceinline__ Group group_of(int n, int block) {
-  Group q;
+// The group of thread `tid` for a front of n entries in a workgroup of `block` threads (host-callable,
+// so the CPU tests check the layout for every n, block and thread).
+__host__ __device__ constexpr Group group_at(int n, int block, int tid) {
+  Group q{};
   q.G = 32;
   while (q.G > 4 && n * q.G > block) q.G >>= 1;
-  const int shift = __ffs(q.G) - 1, span = (26 + q.G - 1) / q.G;
-  q.i = threadIdx.x >> shift;
-  q.g = threadIdx.x & (q.G - 1);
-  q.lead = __lane_id() & ~(q.G - 1);
+  int shift = 0;
+  while ((1 << shift) < q.G) ++shift;
+  const int span = (26 + q.G - 1) / q.G;
+  q.i = tid >> shift;
+  q.g = tid & (q.G - 1);
+  q.lead = (tid & 63) & ~(q.G - 1);  // lane of the group's first thread in its wave
   q.kb = q.g * span;
-  q.cnt = min(span, 26 - q.kb);  // may be <= 0: nothing to do
+  q.cnt = span < 26 - q.kb ? span : 26 - q.kb;  // may be <= 0: nothing to do
   return q;
 }
+__device__ __forceinline__ Group group_of(int n, int block) { return group_at(n, block, int(threadIdx.x)); }
 
 // Mask of lanes 0..lane.
 __device__ __forceinline__ unsigned long long lanes_upto(int lane) {

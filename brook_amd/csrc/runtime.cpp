@@ -17,6 +17,7 @@ Context::Context(int device) : device_(device) {
   hipDeviceProp_t prop;
   BROOK_HIP(hipGetDeviceProperties(&prop, device));
   cus_ = prop.multiProcessorCount;
+  cache_cap_ = prop.totalGlobalMem / 4;  // several contexts (serving slots) share the device
   BROOK_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
 }
 
@@ -27,22 +28,27 @@ Context::~Context() {
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
-// Sizes round up to 2 MiB multiples (1 KiB below 1 MiB), so repeated calls of one shape reuse
-// their blocks without fresh hipMalloc/hipFree (each of which synchronizes the device).
-static size_t rounded(size_t bytes) {
+// Size classes: 1 KiB steps below 1 MiB, then four classes per octave (2^k times 1, 1.25, 1.5,
+// 1.75), so buffers of volumes of other shapes are reused (at most 25 % above the request)
+// without fresh hipMalloc/hipFree, each of which synchronizes the device.
+size_t size_class(size_t bytes) {
   if (bytes == 0) bytes = 1;
   if (bytes < (size_t(1) << 20)) return (bytes + 1023) & ~size_t(1023);
-  return (bytes + (size_t(2) << 20) - 1) & ~((size_t(2) << 20) - 1);
+  int k = 63 - __builtin_clzll((unsigned long long)bytes);  // 2^k <= bytes < 2^(k+1)
+  const size_t step = size_t(1) << (k - 2);                  // a quarter octave
+  return (bytes + step - 1) & ~(step - 1);
 }
 
 Block Context::allocate(size_t bytes) {
-  size_t size = rounded(bytes);
-  auto it = cache_.find(size);
-  if (it != cache_.end()) {
+  const size_t size = size_class(bytes);
+  // the smallest cached block of at least this class, if it is at most ~1.5 times the class
+  auto it = cache_.lower_bound(size);
+  if (it != cache_.end() && it->first <= size_class(size + size / 2)) {
     void *p = it->second;
+    const size_t got = it->first;
     cache_.erase(it);
-    cached_bytes_ -= size;
-    return Block(this, p, size);
+    cached_bytes_ -= got;
+    return Block(this, p, got);
   }
   void *p = nullptr;
   hipError_t e = hipMalloc(&p, size);
@@ -57,6 +63,15 @@ Block Context::allocate(size_t bytes) {
 void Context::give_back(void *ptr, size_t bytes) {
   cache_.emplace(bytes, ptr);
   cached_bytes_ += bytes;
+  if (cached_bytes_ > cache_cap_) {  // free the largest blocks down to the cap
+    (void)hipStreamSynchronize(stream_);
+    while (cached_bytes_ > cache_cap_ && !cache_.empty()) {
+      auto last = std::prev(cache_.end());
+      (void)hipFree(last->second);
+      cached_bytes_ -= last->first;
+      cache_.erase(last);
+    }
+  }
 }
 
 void Context::trim() {

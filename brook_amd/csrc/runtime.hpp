@@ -32,6 +32,9 @@ struct Unsupported : std::invalid_argument {
 
 class Context;
 
+// The allocation size class of a request of `bytes` (the pool reuses blocks by class).
+size_t size_class(size_t bytes);
+
 // A block of device memory; returned to its context's cache when released.
 class Block {
  public:
@@ -93,7 +96,7 @@ class Context {
   int device_ = 0, cus_ = 0;
   hipStream_t stream_ = nullptr;
   std::multimap<size_t, void *> cache_;
-  size_t cached_bytes_ = 0;
+  size_t cached_bytes_ = 0, cache_cap_ = ~size_t(0);
   std::mutex mutex_;
 };
 

@@ -1,16 +1,22 @@
 """Serving: a request loop with continuous batching on one GPU.
 
-`Server.submit(labels)` returns a future at once. One worker thread owns the GPU: whenever it is
-free it admits every waiting request that fits the device-memory budget (first come, first
-served; a request larger than the budget runs alone) and runs them as one engine batch, so the
-requests' labels share the flood and trace launches and their work overlaps on the device.
-Requests that arrive while a batch runs are admitted with the next one. Each result equals the
-result of a plain `skeletonize` call on the same volume (the engine's per-sample stages are the
-plain call's stages), which the parity tests check.
+`Server.submit(labels)` returns a future at once. The server runs `slots` engine workers, each a
+thread with its own device context (its own HIP stream and device-buffer pool). Whenever a slot is
+free it admits every waiting request that fits what is left of the device-memory budget beside
+the requests already on the GPU (first come, first served; a request larger than the whole budget
+runs alone once nothing else is in flight) and runs them as one engine batch, so those requests'
+labels share the flood and trace launches. A request that arrives while another batch is on the
+GPU starts at once on the next free slot: its CCL, EDT and floods run on its slot's stream beside
+the other slot's trace kernel, whose persistent workgroups take only one compute unit per label.
+Each result equals the result of a plain `skeletonize` call on the same volume (the engine's
+per-sample stages are the plain call's stages), which the parity tests check alone and in the mix.
 
-The reference has no serving loop (SURVEY of brook: `GpuPool.imap` in-flight bounding,
-`_pool.py:198-266`, is the nearest); this is the request loop of the serving form this build is
-measured on.
+Device buffers stay in each slot's pool between requests (size classes of a quarter octave, so
+volumes of other shapes reuse them), so a slot's label, EDT and frontier arrays are allocated once
+and refilled per request.
+
+The reference has no serving loop (`GpuPool.imap`'s in-flight bounding, `_pool.py:198-266`, is the
+nearest); this is the request loop of the serving form this build is measured on.
 """
 from __future__ import annotations
 
@@ -37,7 +43,7 @@ def request_bytes(labels) -> int:
 
 
 class Request:
-    __slots__ = ("labels", "future", "arrival", "admitted", "done", "batch_size")
+    __slots__ = ("labels", "future", "arrival", "admitted", "done", "batch_size", "slot", "overlapped")
 
     def __init__(self, labels):
         self.labels = labels
@@ -45,20 +51,26 @@ class Request:
         self.arrival = time.perf_counter()
         self.admitted = self.done = None
         self.batch_size = 0
+        self.slot = -1
+        self.overlapped = False  # admitted while another slot's batch was on the GPU
 
 
 class Server:
-    """Continuous-batching skeletonization server (one worker thread, one GPU).
+    """Continuous-batching skeletonization server on one GPU.
 
-    hbm_budget: device bytes the requests of one batch may take (default: half the free device
-    memory when the server starts); max_batch: requests per batch (1 = serial plain calls).
-    runner(list_of_labels) -> list of results replaces the engine (used by the host tests).
+    slots: engine workers (device contexts) that run batches side by side; hbm_budget: device bytes
+    the requests in flight may take together (default: half the free device memory when the server
+    starts); max_batch: requests per batch. slots=1, max_batch=1 is serial plain calls.
+    runner(list_of_labels) -> list of results replaces the engine (used by the host tests); it is
+    called from the slot's worker thread.
     """
 
     def __init__(self, teasar_params=DEFAULT_TEASAR_PARAMS, anisotropy=(1, 1, 1), dust_threshold=1000,
-                 fix_branching=True, fix_borders=True, hbm_budget=None, max_batch=64, runner=None):
+                 fix_branching=True, fix_borders=True, hbm_budget=None, max_batch=64, runner=None, slots=2):
         if max_batch < 1:
             raise ValueError("brook: max_batch must be at least 1")
+        if slots < 1:
+            raise ValueError("brook: slots must be at least 1")
         self._params = dict(teasar_params)
         self._aniso = tuple(float(a) for a in anisotropy)
         self._dust = dust_threshold
@@ -75,9 +87,13 @@ class Server:
         self._pending = collections.deque()
         self._cv = threading.Condition()
         self._stop = False
-        self.batches = []  # sizes of the batches run, in order
-        self._thread = threading.Thread(target=self._loop, name="brook-serve", daemon=True)
-        self._thread.start()
+        self._in_flight = 0  # bytes admitted and not yet done, over all slots
+        self._busy = 0  # slots running a batch
+        self.batches = []  # sizes of the batches run, in order of admission
+        self._threads = [threading.Thread(target=self._loop, args=(s,), name=f"brook-serve-{s}", daemon=True)
+                         for s in range(int(slots))]
+        for t in self._threads:
+            t.start()
 
     def submit(self, labels) -> Future:
         """Queues a label volume (3-D numpy array or device tensor); the future's result is a
@@ -94,8 +110,9 @@ class Server:
     def close(self):
         with self._cv:
             self._stop = True
-            self._cv.notify()
-        self._thread.join()
+            self._cv.notify_all()
+        for t in self._threads:
+            t.join()
 
     def __enter__(self):
         return self
@@ -104,41 +121,54 @@ class Server:
         self.close()
 
     def _admit(self):
-        """FIFO requests that fit the budget (at least one)."""
+        """FIFO requests that fit the budget beside what is in flight; an empty list when the head
+        request must wait for memory (it runs alone, at the latest once nothing is in flight)."""
         batch, used = [], 0
         while self._pending and len(batch) < self._max_batch:
             need = request_bytes(self._pending[0].labels)
-            if batch and used + need > self._budget:
+            if self._in_flight + used + need > self._budget and (batch or self._in_flight):
                 break
             batch.append(self._pending.popleft())
             used += need
-        return batch
+        return batch, used
 
     def _engine(self, volumes):
         from .batch import batch_parts
         parts = batch_parts(volumes, self._params, self._aniso, self._dust, self._fix_branching, self._fix_borders)
         return [p if len(p) else PackedSkeletons.from_skeletons({}, self._aniso) for p in parts]
 
-    def _loop(self):
+    def _loop(self, slot):
         while True:
             with self._cv:
-                while not self._pending and not self._stop:
+                while True:
+                    if self._pending:
+                        batch, used = self._admit()
+                        if batch:
+                            break
+                    elif self._stop:
+                        return
                     self._cv.wait()
-                if not self._pending and self._stop:
-                    return
-                batch = self._admit()
-            now = time.perf_counter()
-            for r in batch:
-                r.admitted, r.batch_size = now, len(batch)
-            self.batches.append(len(batch))
-            try:
-                results = self._runner([r.labels for r in batch])
-            except BaseException as e:  # every admitted request learns why
+                overlapped = self._busy > 0
+                self._busy += 1
+                self._in_flight += used
+                now = time.perf_counter()
                 for r in batch:
-                    r.done = time.perf_counter()
-                    r.future.set_exception(e)
-                continue
+                    r.admitted, r.batch_size, r.slot, r.overlapped = now, len(batch), slot, overlapped
+                self.batches.append(len(batch))
+                if self._pending:
+                    self._cv.notify()  # another free slot may take what this batch left
+            try:
+                results, error = self._runner([r.labels for r in batch]), None
+            except BaseException as e:  # every admitted request learns why
+                results, error = None, e
             done = time.perf_counter()
-            for r, res in zip(batch, results):
+            with self._cv:
+                self._busy -= 1
+                self._in_flight -= used
+                self._cv.notify_all()  # memory freed: waiting requests may fit now
+            for i, r in enumerate(batch):
                 r.done = done
-                r.future.set_result(res)
+                if error is not None:
+                    r.future.set_exception(error)
+                else:
+                    r.future.set_result(results[i])

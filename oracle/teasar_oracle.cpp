@@ -17,9 +17,12 @@
 //   PointSet       CPython's own set iteration order (python3 in the test process)
 //   border targets golden skeleton digests (fix_borders=True) from the reference's test suite
 //   geometric flood  brute-force Bellman-Ford to fixpoint on small volumes
-//   target order   numpy lexsort
+//   target order   numpy lexsort of the reference rule (component_order_smoke.cpp:9-21) on its grid
+//                  of lengths, densities and tie rates, with +inf and -0 keys
 //   railroad/join  test_package.py:751-811 known answers (26 singles, 325 pairs, all 26, 9^3 road)
-//   invalidation   golden skeleton digests; plain (M, source) Dijkstra cross-check on random balls
+//   invalidation   one source: the claimed set is the ball's component and every key the exact
+//                  minimax bottleneck (a Dijkstra over max); many sources: every claim valid and the
+//                  claimed set closed under the final keys; golden skeleton digests
 //   whole path     golden digests test_package.py:2079-2106 (Kimimaro 5.8.1 results)
 //   vertex-weighted flood and parents (fix_branching=False)  brute-force float32 Bellman-Ford of
 //                  the vw_step fixpoint; every parent chain satisfies d[v] = vw_step(d[parent], pdrf[v])

@@ -341,3 +341,44 @@ def test_device_output_views(brook):
                 L.brook_skeletons_destroy(m)
         finally:
             L.brook_skeletons_destroy(h)
+
+
+@pytest.mark.parametrize("workload", ["tubes_128", "trees_512"])
+def test_workload_edt_equals_oracle_digest(brook, workload):
+    """The GPU EDT of the 128^3 tube and 512^3 tree workloads equals the oracle's float32 DBF bit
+    for bit (tests/golden/workload_digests.py wrote its digest), so the oracle's stated error
+    against the float64 EDT (max ~6e-8 relative) is the GPU's; the bound asserted here is 1e-5."""
+    import hashlib
+
+    import torch
+    from brook_amd import device, synth
+    want = json.load(open(os.path.join(HERE, "golden", "workload_digests.json")))["edt"][workload]
+    volume = getattr(synth, workload)()
+    flat = torch.from_numpy(np.ravel(volume, order="F")).to("cuda:0")
+    dev = flat.view(volume.shape[2], volume.shape[1], volume.shape[0]).permute(2, 1, 0)
+    dbf = device.edt(dev, tuple(want["anisotropy"]))
+    host = dbf.permute(2, 1, 0).contiguous().cpu().numpy()  # Fortran order of (x, y, z)
+    assert hashlib.sha256(host.tobytes()).hexdigest() == want["sha256"]
+    assert want["max_rel_err_f64"] <= 1e-5
+
+
+def test_batch_workload_equals_oracle_digests(brook):
+    """The 64-volume batch workload (synth.batch_64) in one skeletonize_batch call, device-resident
+    samples: every sample's packed result against the oracle's digest (tests/golden/batch_digests.py
+    wrote them)."""
+    import torch
+    from brook_amd import synth
+    from tests.golden.batch_digests import ANISOTROPY, sample_digest
+    from tests.golden.workload_digests import PARAMS
+    want = json.load(open(os.path.join(HERE, "golden", "batch_digests.json")))
+    vols = synth.batch_64()
+    dev = []
+    for v in vols:
+        flat = torch.from_numpy(np.ravel(v, order="F")).to("cuda:0")
+        dev.append(flat.view(v.shape[2], v.shape[1], v.shape[0]).permute(2, 1, 0))
+    batch = brook.skeletonize_batch(dev, PARAMS, ANISOTROPY, dust_threshold=1000, output="packed")
+    assert len(batch) == len(want) == 64
+    for i in range(64):
+        got = batch[i].as_dict()
+        assert len(got["labels"]) == want[i]["skeletons"], i
+        assert sample_digest(got) == want[i]["sha256"], i
