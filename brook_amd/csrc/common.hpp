@@ -253,8 +253,9 @@ __device__ __forceinline__ unsigned rec_meta(const VoxelRec *r, int v) { return 
 // walks that need two of its fields: one scattered access instead of two. During the flood and
 // trace kernels only the workgroup that owns the voxel's label touches its record, as for the
 // workgroup-scope loads above.
-__device__ __forceinline__ uint4 rec_load(const VoxelRec *r, int v) {
-  return *gp(reinterpret_cast<const uint4 *>(r + v));
+typedef unsigned RecWords __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ RecWords rec_load(const VoxelRec *r, int v) {
+  return *gp(reinterpret_cast<const RecWords *>(r + v));
 }
 // The high half of the word (distance bits or claim bottleneck) without the 64-bit load.
 __device__ __forceinline__ unsigned rec_word_hi(const VoxelRec *r, int v) {
