@@ -65,7 +65,11 @@ def lib():
         L.oracle_prepare.argtypes = [_P, _I64, _I64, _I64, _P, _P, _D, _I, _I, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P]
         L.oracle_vw_parents.argtypes = [_P, _I64, _I64, _I64, _P, _P, _I64, _P, _P]
         L.oracle_skeletonize.restype = _P
-        L.oracle_skeletonize.argtypes = [_P, _I64, _I64, _I64, _P, _I64, _P, _D, _I, _I, _P, _I64, _P, _I64, _I]
+        L.oracle_skeletonize.argtypes = [_P, _I64, _I64, _I64, _P, _I64, _P, _D, _I, _I, _P, _I64, _P, _I64, _I, _P, _I]
+        L.oracle_graph_ccl.restype = _I64
+        L.oracle_graph_ccl.argtypes = [_P, _I64, _I64, _I64, _P, _I, _P, _P, _P, _P, _P]
+        L.oracle_graph_edt.restype = _I
+        L.oracle_graph_edt.argtypes = [_P, _I64, _I64, _I64, _P, _P, _I, _P]
         L.oracle_result_sizes.argtypes = [_P, _P]
         L.oracle_result_copy.argtypes = [_P, _P, _P, _P, _P, _P, _P]
         L.oracle_result_free.argtypes = [_P]
@@ -107,6 +111,51 @@ def ccl(labels):
     if count < 0:
         _fail("ccl")
     return cc, mapping[:count + 1].copy(), roots[:count].copy()
+
+
+def _graph(graph, shape):
+    """(uint32 Fortran graph, compact_2d): a uint8 graph (one XY plane) is widened value for value."""
+    g = np.asarray(graph)
+    while g.ndim < 3:
+        g = g[..., None]
+    if g.shape != tuple(shape):
+        raise ValueError("voxel_graph must match the label shape")
+    compact = g.dtype == np.uint8
+    if compact and g.shape[2] != 1:
+        raise ValueError("a uint8 voxel_graph must be one XY plane")
+    if not compact and g.dtype != np.uint32:
+        raise ValueError("voxel_graph must be uint32, or uint8 for an XY plane")
+    return np.asfortranarray(g.astype(np.uint32)), compact
+
+
+def graph_ccl(labels, graph):
+    """Voxel-graph components: (cc uint32 Fortran array, mapping int64[N+1], roots int64[N],
+    keys int64[N+1], input_uniform)."""
+    a, shape = _fortran_int64(labels)
+    g, compact = _graph(graph, shape)
+    n = a.size
+    cc = np.zeros(shape, np.uint32, order="F")
+    mapping, keys = np.zeros(n + 1, np.int64), np.zeros(n + 1, np.int64)
+    roots = np.zeros(max(n, 1), np.int64)
+    uniform = np.zeros(1, np.int32)
+    count = lib().oracle_graph_ccl(_ptr(a), *shape, _ptr(g), int(compact), _ptr(cc), _ptr(mapping), _ptr(roots),
+                                   _ptr(keys), _ptr(uniform))
+    if count < 0:
+        _fail("graph_ccl")
+    return cc, mapping[:count + 1].copy(), roots[:count].copy(), keys[:count + 1].copy(), bool(uniform[0])
+
+
+def graph_edt(cc, graph, anisotropy=(1, 1, 1), black_border=False):
+    """The graph-aware EDT of a label volume (float32, Fortran order)."""
+    cc = np.asfortranarray(np.asarray(cc).astype(np.uint32, copy=False))
+    while cc.ndim < 3:
+        cc = cc[..., None]
+    g, _ = _graph(graph, cc.shape)
+    out = np.zeros(cc.shape, np.float32, order="F")
+    an = np.asarray(anisotropy, np.float32)
+    if lib().oracle_graph_edt(_ptr(cc), *cc.shape, _ptr(g), _ptr(an), int(black_border), _ptr(out)):
+        _fail("graph_edt")
+    return out
 
 
 def edt(cc, anisotropy=(1, 1, 1), black_border=False, dims=3):
@@ -252,11 +301,12 @@ def vw_parents(labels, vw, sources):
 
 def skeletonize(labels, teasar_params=None, anisotropy=(1, 1, 1), dust_threshold=1000, fix_branching=True,
                 fix_borders=True, extra_targets_before=(), threads=1, stop_emitted=True, fill_holes=False,
-                fix_avocados=False, extra_targets_after=()):
+                fix_avocados=False, extra_targets_after=(), voxel_graph=None):
     """Packed result dict: labels, v_off, e_off, vertices (V,3) f32, edges (E,2) u32, radii f32, stats.
     fix_branching=False walks the parent field; stop_emitted=False makes every walk run to the root
     (Kimimaro's paths) instead of ending on the first emitted vertex."""
     a, shape = _fortran_int64(labels)
+    g, compact = _graph(voxel_graph, shape) if voxel_graph is not None else (None, False)
     pv = params_vector(teasar_params)
     mp = (teasar_params or {}).get("max_paths")
     an = np.asarray(anisotropy, np.float32)
@@ -266,7 +316,7 @@ def skeletonize(labels, teasar_params=None, anisotropy=(1, 1, 1), dust_threshold
     h = L.oracle_skeletonize(_ptr(a), *shape, _ptr(pv), -1 if mp is None else int(mp), _ptr(an), float(dust_threshold),
                              1 if fix_branching else (0 if stop_emitted else -1),
                              int(bool(fix_borders)) | (2 if fill_holes else 0) | (4 if fix_avocados else 0), _ptr(tb),
-                             len(tb), _ptr(ta), len(ta), threads)
+                             len(tb), _ptr(ta), len(ta), threads, None if g is None else _ptr(g), int(compact))
     if not h:
         _fail("skeletonize")
     try:

@@ -158,9 +158,17 @@ inline void fill_mask(std::vector<uint8_t> &m, i64 sx, i64 sy, i64 sz, bool thre
   for (auto &v : m) v = (v == 2) ? 0 : 1;
 }
 
-// The correction; returns the passes. refresh(labels, distance) recomputes the field.
+// The correction; returns the passes. refresh(labels, distance) recomputes the field. keys: with a
+// voxel graph, Kimimaro's id of each compact id (the rank of its graph component among all of them,
+// background ones included; ascending), which its candidate and change sets hold (avocado.cpp:85-91);
+// empty: the ids themselves.
 inline int correct(std::vector<uint32_t> &lab, std::vector<float> &dist, const Vol &V, size_t table, double threshold,
-                   const std::function<void(const std::vector<uint32_t> &, std::vector<float> &)> &refresh) {
+                   const std::function<void(const std::vector<uint32_t> &, std::vector<float> &)> &refresh,
+                   const std::vector<i64> &keys = {}) {
+  auto key = [&](uint32_t id) { return keys.empty() ? i64(id) : keys[id]; };
+  auto id_of = [&](i64 k) {
+    return keys.empty() ? uint32_t(k) : uint32_t(std::lower_bound(keys.begin(), keys.end(), k) - keys.begin());
+  };
   CSet unchanged;
   int passes = 0;
   while (passes < 20) {
@@ -168,7 +176,7 @@ inline int correct(std::vector<uint32_t> &lab, std::vector<float> &dist, const V
     std::vector<char> seen(table, 0);
     for (size_t i = 0; i < lab.size(); ++i) seen[dist[i] > cut ? lab[i] : 0] = 1;
     CSet cand;
-    for (size_t id = 0; id < table; ++id) if (seen[id]) cand.add(i64(id));
+    for (size_t id = 0; id < table; ++id) if (seen[id]) cand.add(key(uint32_t(id)));
     cand.difference_update(unchanged);
     cand.discard(0);
     if (cand.used == 0) break;
@@ -183,7 +191,8 @@ inline int correct(std::vector<uint32_t> &lab, std::vector<float> &dist, const V
           b[3] = std::max(b[3], x); b[4] = std::max(b[4], y); b[5] = std::max(b[5], z);
         }
     CSet keep_now, changed;
-    for (i64 L : cand.iterate()) {
+    for (i64 K : cand.iterate()) {
+      const uint32_t L = id_of(K);
       const auto &b = bb[size_t(L)];
       const i64 sx = b[3] - b[0] + 1, sy = b[4] - b[1] + 1, sz = b[5] - b[2] + 1;
       auto g = [&](i64 x, i64 y, i64 z) { return size_t(V.idx(x + b[0], y + b[1], z + b[2])); };
@@ -247,13 +256,13 @@ inline int correct(std::vector<uint32_t> &lab, std::vector<float> &dist, const V
         for (auto &h : hit) if (h.second > most) { most = h.second; top = h.first; }
         if (!(rays - most > (rays > 3 ? 1 : 0))) owner = top;
       }
-      if (pit == owner && !changed.contains(pit)) {
-        keep_now.add(pit);
+      if (pit == owner && !changed.contains(key(pit))) {
+        keep_now.add(key(pit));
       } else {
-        keep_now.discard(pit);
-        keep_now.discard(owner);
-        changed.add(pit);
-        changed.add(owner);
+        keep_now.discard(key(pit));
+        keep_now.discard(key(owner));
+        changed.add(key(pit));
+        changed.add(key(owner));
         for (i64 z = 0; z < sz; ++z)
           for (i64 y = 0; y < sy; ++y)
             for (i64 x = 0; x < sx; ++x)

@@ -59,6 +59,19 @@ static const int D3RANK[26] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14,
 static const int D3INV[26] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 22, 20, 19, 24, 23, 21, 25};
 static const int OPP[26] = {1, 0, 3, 2, 5, 4, 9, 8, 7, 6, 13, 12, 11, 10, 17, 16, 15, 14, 25, 24, 23, 22, 21, 20, 19, 18};
 
+// cc3d's voxel-graph bit of neighbour k (common.cuh:49-56): the step v -> v + N[k] is permitted iff
+// bit BIT[k] of graph[v] is set.
+static const int BIT[26] = {1, 0, 3, 2, 5, 4, 9, 7, 8, 6, 17, 13, 16, 12, 15, 11, 14, 10, 25, 21, 23, 19, 24, 20, 22, 18};
+
+// The gate a voxel graph puts on each step (common.cuh:84-100): a push from u to u + N[k] needs
+// bit BIT[k] of graph[u]; a pull by c from its neighbour u = c + N[k] needs the step u -> c, bit
+// BIT[OPP[k]] of graph[u]. Without a graph every step is open.
+struct Gate {
+  const uint32_t *vg = nullptr;
+  bool push(int64_t u, int k) const { return !vg || ((vg[u] >> BIT[k]) & 1u); }
+  bool pull(int64_t u, int k) const { return !vg || ((vg[u] >> BIT[OPP[k]]) & 1u); }
+};
+
 static const float FINF = std::numeric_limits<float>::infinity();
 static const u64 MAXKEY = ~u64(0);
 
@@ -133,6 +146,104 @@ Components ccl(const i64 *labels, const Shape &s) {
     }
   }
   return out;
+}
+
+// Components of a voxel graph (graph_components.cu:18-166, cc3d.color_connectivity_graph): the
+// graph's permitted steps alone join voxels, background included, whatever their labels. Ids go to
+// the components that hold a foreground voxel, 1..N in ascending order of each component's lowest
+// voxel (background included); roots[id - 1] is the lowest foreground voxel; mapping[id] the label
+// where the id's last run of voxels in flat order starts; keys[id] the component's rank among all
+// of them (background-only ones included), 1-based. A uint8 graph (one XY plane) holds the bits
+// of the steps within the plane, the corner bits (6..9) moved down by 2.
+struct GraphComponents : Components {
+  std::vector<i64> keys;  // [0] = 0
+  bool input_uniform = true;  // every voxel has the label of the first
+};
+
+GraphComponents graph_ccl(const i64 *labels, const uint32_t *vg, bool compact_2d, const Shape &s) {
+  const i64 n = s.n();
+  std::vector<i64> parent(n);
+  for (i64 i = 0; i < n; ++i) parent[i] = i;
+  for (i64 v = 0; v < n; ++v) {
+    i64 x, y, z;
+    s.unravel(v, x, y, z);
+    for (int k = 0; k < 26; ++k) {
+      int bit = BIT[k];
+      if (compact_2d && bit >= 6) bit -= 2;
+      if (!((vg[v] >> bit) & 1u)) continue;
+      i64 nx = x + NX[k], ny = y + NY[k], nz = z + NZ[k];
+      if (!s.inside(nx, ny, nz)) continue;
+      i64 u = s.at(nx, ny, nz);
+      if (u >= v) continue;
+      i64 a = uf_find(parent, u), b = uf_find(parent, v);
+      if (a == b) continue;
+      if (a < b) parent[b] = a; else parent[a] = b;
+    }
+  }
+  std::vector<i64> rank(n, 0), compact(n, 0);  // by root: 1-based rank among all roots; compact id
+  i64 roots_seen = 0;
+  for (i64 v = 0; v < n; ++v)
+    if (uf_find(parent, v) == v) rank[v] = ++roots_seen;
+  std::vector<uint8_t> has_fg(n, 0);
+  for (i64 v = 0; v < n; ++v)
+    if (labels[v] > 0) has_fg[uf_find(parent, v)] = 1;
+  GraphComponents out;
+  out.keys.push_back(0);
+  for (i64 r = 0; r < n; ++r)
+    if (has_fg[r]) { compact[r] = i64(out.keys.size()); out.keys.push_back(rank[r]); }
+  const size_t count = out.keys.size() - 1;
+  out.cc.assign(n, 0);
+  out.roots.assign(count, -1);
+  out.mapping.assign(count + 1, 0);
+  std::vector<i64> last_run(count + 1, -1);
+  for (i64 v = 0; v < n; ++v) {
+    const uint32_t id = labels[v] > 0 ? uint32_t(compact[uf_find(parent, v)]) : 0;
+    out.cc[v] = id;
+    if (labels[v] != labels[0]) out.input_uniform = false;
+    if (!id) continue;
+    if (out.roots[id - 1] < 0) out.roots[id - 1] = v;
+    if (v == 0 || out.cc[v - 1] != id) last_run[id] = v;
+  }
+  for (size_t id = 1; id <= count; ++id) out.mapping[id] = labels[last_run[id]];
+  return out;
+}
+
+// The graph-aware EDT (graph_edt.cu:14-132, after the edt package): a binary volume on the doubled
+// grid, the voxel centres at even positions; a voxel's +x, +y, +z half-steps (odd in one axis only)
+// are foreground when the graph permits that step, every other position of a foreground voxel is
+// foreground, and with a black border the last doubled position on each axis is background. Its
+// EDT with half the anisotropy (the reference's rounding rule, edt_rule), sampled at the voxel
+// centres; 0 on background. `vg` is the tracing graph (a uint8 plane graph's value widened as is,
+// graph_components.cu:167-176).
+template <class L>
+void edt_rule(const L *lab, const Shape &s, const float aniso[3], bool black_border, int dims, float *D);
+
+template <class L>
+void graph_edt(const L *lab, const uint32_t *vg, const Shape &s, const float aniso[3], bool black_border, float *D) {
+  const Shape d2{2 * s.sx, 2 * s.sy, 2 * s.sz};
+  std::vector<uint8_t> mask(size_t(d2.n()), 0);
+  for (i64 z = 0; z < d2.sz; ++z)
+    for (i64 y = 0; y < d2.sy; ++y)
+      for (i64 x = 0; x < d2.sx; ++x) {
+        if (black_border && (x == d2.sx - 1 || y == d2.sy - 1 || z == d2.sz - 1)) continue;
+        const i64 src = s.at(x >> 1, y >> 1, z >> 1);
+        if (!(lab[src] > 0)) continue;
+        const int sub = int((x & 1) | ((y & 1) << 1) | ((z & 1) << 2));
+        uint8_t m = 1;
+        if (sub == 1) m = (vg[src] & 1u) != 0;
+        else if (sub == 2) m = (vg[src] & 4u) != 0;
+        else if (sub == 4) m = (vg[src] & 16u) != 0;
+        mask[size_t(d2.at(x, y, z))] = m;
+      }
+  const float half[3] = {aniso[0] * 0.5f, aniso[1] * 0.5f, aniso[2] * 0.5f};
+  std::vector<float> D2(size_t(d2.n()));
+  edt_rule<uint8_t>(mask.data(), d2, half, black_border, 3, D2.data());
+  for (i64 z = 0; z < s.sz; ++z)
+    for (i64 y = 0; y < s.sy; ++y)
+      for (i64 x = 0; x < s.sx; ++x) {
+        const i64 v = s.at(x, y, z);
+        D[v] = lab[v] > 0 ? D2[size_t(d2.at(2 * x, 2 * y, 2 * z))] : 0.0f;
+      }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -444,7 +555,7 @@ struct HeapItem {
 using MinHeap = std::priority_queue<HeapItem, std::vector<HeapItem>, std::greater<HeapItem>>;
 
 void geometric_flood(const std::vector<uint32_t> &cc, const Shape &s, const float W[26], const std::vector<i64> &sources,
-                     std::vector<float> &dist) {
+                     std::vector<float> &dist, const Gate &gate = {}) {
   MinHeap heap;
   for (i64 src : sources) { dist[src] = 0.0f; heap.push({0.0f, src}); }
   while (!heap.empty()) {
@@ -456,7 +567,7 @@ void geometric_flood(const std::vector<uint32_t> &cc, const Shape &s, const floa
       i64 nx = x + NX[k], ny = y + NY[k], nz = z + NZ[k];
       if (!s.inside(nx, ny, nz)) continue;
       i64 u = s.at(nx, ny, nz);
-      if (cc[u] != l) continue;
+      if (cc[u] != l || !gate.push(it.v, k)) continue;
       float c = it.d + W[k];
       if (c < dist[u]) { dist[u] = c; heap.push({c, u}); }
     }
@@ -466,7 +577,8 @@ void geometric_flood(const std::vector<uint32_t> &cc, const Shape &s, const floa
 // The same fixpoint from sources with initial values (BatchedFlooder::flood seeded,
 // batched.cu:122-148): every finite seed starts the flood with its value.
 void geometric_flood_seeded(const std::vector<uint32_t> &cc, const Shape &s, const float W[26],
-                            const std::vector<std::pair<i64, float>> &seeds, std::vector<float> &dist) {
+                            const std::vector<std::pair<i64, float>> &seeds, std::vector<float> &dist,
+                            const Gate &gate = {}) {
   MinHeap heap;
   for (auto &sd : seeds) { dist[sd.first] = sd.second; heap.push({sd.second, sd.first}); }
   while (!heap.empty()) {
@@ -478,7 +590,7 @@ void geometric_flood_seeded(const std::vector<uint32_t> &cc, const Shape &s, con
       i64 nx = x + NX[k], ny = y + NY[k], nz = z + NZ[k];
       if (!s.inside(nx, ny, nz)) continue;
       i64 u = s.at(nx, ny, nz);
-      if (cc[u] != l) continue;
+      if (cc[u] != l || !gate.push(it.v, k)) continue;
       float c = it.d + W[k];
       if (c < dist[u]) { dist[u] = c; heap.push({c, u}); }
     }
@@ -598,7 +710,7 @@ i64 soma_root(const std::vector<uint32_t> &cc, const std::vector<float> &dbf, co
 // d[v] = min_u vw_step(d[u], vw[v]) over same-label 26-neighbours. vw_step is monotone
 // non-decreasing in d and adds at least one ulp for a positive weight, so Dijkstra reproduces it.
 void vw_flood(const std::vector<uint32_t> &cc, const Shape &s, const std::vector<float> &vw,
-              const std::vector<i64> &sources, std::vector<float> &dist) {
+              const std::vector<i64> &sources, std::vector<float> &dist, const Gate &gate = {}) {
   MinHeap heap;
   for (i64 src : sources) { dist[src] = 0.0f; heap.push({0.0f, src}); }
   while (!heap.empty()) {
@@ -610,7 +722,7 @@ void vw_flood(const std::vector<uint32_t> &cc, const Shape &s, const std::vector
       i64 nx = x + NX[k], ny = y + NY[k], nz = z + NZ[k];
       if (!s.inside(nx, ny, nz)) continue;
       i64 u = s.at(nx, ny, nz);
-      if (cc[u] != l) continue;
+      if (cc[u] != l || !gate.push(it.v, k)) continue;
       float c = vw_step(it.d, vw[u]);
       if (c < dist[u]) { dist[u] = c; heap.push({c, u}); }
     }
@@ -620,7 +732,7 @@ void vw_flood(const std::vector<uint32_t> &cc, const Shape &s, const std::vector
 // parents_lab, batched_regular.inc:113-134: the same-label neighbour with the smallest distance,
 // first k on ties; -1 for background, inactive labels, unreached voxels and the sources (d 0).
 void parent_field(const std::vector<uint32_t> &cc, const Shape &s, const std::vector<uint8_t> &active,
-                  const std::vector<float> &dist, std::vector<int64_t> &parent) {
+                  const std::vector<float> &dist, std::vector<int64_t> &parent, const Gate &gate = {}) {
   parent.assign(size_t(s.n()), -1);
   for (i64 v = 0; v < s.n(); ++v) {
     uint32_t l = cc[v];
@@ -632,7 +744,7 @@ void parent_field(const std::vector<uint32_t> &cc, const Shape &s, const std::ve
       i64 nx = x + NX[k], ny = y + NY[k], nz = z + NZ[k];
       if (!s.inside(nx, ny, nz)) continue;
       i64 u = s.at(nx, ny, nz);
-      if (cc[u] != l) continue;
+      if (cc[u] != l || !gate.pull(u, k)) continue;
       if (dist[u] < best) { best = dist[u]; arg = u; }
     }
     parent[v] = arg;
@@ -658,6 +770,10 @@ struct Params {
   double dust = 1000;
   bool fix_branching = true, fix_borders = true, fill_holes = false, fix_avocados = false;
   bool stop_emitted = true;  // fix_branching=False: walks end on the first emitted vertex (lockstep_regular.inc:160-165)
+  // voxel_graph (skeletonize.cu:76-91): the tracing graph (a uint8 plane graph's values widened
+  // as they are) and whether the input was such a plane graph (its CCL reads the corner bits moved)
+  const uint32_t *graph = nullptr;
+  bool graph_compact_2d = false;
 };
 
 struct Prepared {
@@ -713,11 +829,26 @@ static void parallel_for(size_t count, int threads, const std::function<void(siz
 Prepared prepare(const i64 *labels, const Shape &s, const Params &p, int threads) {
   Prepared out;
   out.s = s;
-  out.comp = ccl(labels, s);
+  const Gate gate{p.graph};
+  std::vector<i64> keys;  // Kimimaro's ids of the graph components (avocado order)
+  bool uniform = false;
+  if (p.graph) {
+    GraphComponents g = graph_ccl(labels, p.graph, p.graph_compact_2d, s);
+    keys = std::move(g.keys);
+    uniform = g.input_uniform;
+    out.comp = std::move(g);
+  } else {
+    out.comp = ccl(labels, s);
+  }
   size_t count = out.comp.roots.size();
   size_t k = count + 1;
   out.boxes = analyze(out.comp.cc, s, count);
-  bool black_border = count == 1 && out.boxes[1].count == s.n();  // of the volume before any fill
+  // of the volume before any fill; with a graph, every voxel of the input has one label
+  bool black_border = p.graph ? uniform : count == 1 && out.boxes[1].count == s.n();
+  auto field = [&](const std::vector<uint32_t> &lab, std::vector<float> &dist) {
+    if (p.graph) graph_edt<uint32_t>(lab.data(), p.graph, s, p.aniso, black_border, dist.data());
+    else edt_rule<uint32_t>(lab.data(), s, p.aniso, black_border, 3, dist.data());
+  };
   if (p.fill_holes) {
     // fill_all_holes (holes.cu:37-62, skeletonize.cu:83-90): in ascending id and with the original
     // boxes, a component's box voids become the component; a component painted over, even in
@@ -742,16 +873,13 @@ Prepared prepare(const i64 *labels, const Shape &s, const Params &p, int threads
     }
   }
   out.dbf.resize(s.n());
-  edt_rule<uint32_t>(out.comp.cc.data(), s, p.aniso, black_border, 3, out.dbf.data());
+  field(out.comp.cc, out.dbf);
   if (p.fix_avocados) {
     // skeletonize.cu:91-95: the correction (avocado_oracle.hpp), the field refreshed after each
     // changing pass, then fastremap.renumber with get_mapping's rule (the input label under each
     // run start of a new id, the last one kept)
     const std::vector<uint32_t> before = out.comp.cc;
-    oracle_avocado::correct(out.comp.cc, out.dbf, {s.sx, s.sy, s.sz}, k, p.soma_detection,
-                            [&](const std::vector<uint32_t> &lab, std::vector<float> &dist) {
-                              edt_rule<uint32_t>(lab.data(), s, p.aniso, black_border, 3, dist.data());
-                            });
+    oracle_avocado::correct(out.comp.cc, out.dbf, {s.sx, s.sy, s.sz}, k, p.soma_detection, field, keys);
     std::vector<uint32_t> renum(k, 0);
     std::vector<i64> mapping{0}, roots;
     uint32_t prev = UINT32_MAX;
@@ -814,7 +942,15 @@ Prepared prepare(const i64 *labels, const Shape &s, const Params &p, int threads
           for (i64 x = 0; x < c.sx; ++x)
             mask[c.at(x, y, z)] = out.comp.cc[s.at(x + b.lo[0], y + b.lo[1], z + b.lo[2])] == id;
       std::vector<float> D(size_t(c.n()));
-      edt_rule<uint8_t>(mask.data(), c, p.aniso, b.count == c.n(), 3, D.data());
+      if (p.graph) {  // the graph EDT of the crop with the crop of the graph (trace.cu:107)
+        std::vector<uint32_t> crop(size_t(c.n()));
+        for (i64 z = 0; z < c.sz; ++z)
+          for (i64 y = 0; y < c.sy; ++y)
+            for (i64 x = 0; x < c.sx; ++x) crop[c.at(x, y, z)] = p.graph[s.at(x + b.lo[0], y + b.lo[1], z + b.lo[2])];
+        graph_edt<uint8_t>(mask.data(), crop.data(), c, p.aniso, b.count == c.n(), D.data());
+      } else {
+        edt_rule<uint8_t>(mask.data(), c, p.aniso, b.count == c.n(), 3, D.data());
+      }
       float maximum = 0.0f;
       for (i64 z = 0; z < c.sz; ++z)
         for (i64 y = 0; y < c.sy; ++y)
@@ -845,7 +981,7 @@ Prepared prepare(const i64 *labels, const Shape &s, const Params &p, int threads
     // root search flood from each component's first voxel; labels are independent
     parallel_for(search.size(), threads, [&](size_t i) {
       size_t id = search[i];
-      geometric_flood(out.comp.cc, s, W, {out.comp.roots[id - 1]}, out.daf);
+      geometric_flood(out.comp.cc, s, W, {out.comp.roots[id - 1]}, out.daf, gate);
     });
     std::vector<float> v; std::vector<i64> idx;
     seg_argmax(out.daf, out.comp.cc, k, v, idx);
@@ -860,7 +996,7 @@ Prepared prepare(const i64 *labels, const Shape &s, const Params &p, int threads
     if (out.soma[out.comp.cc[v]]) members[out.comp.cc[v]].push_back(v);
   parallel_for(act.size(), threads, [&](size_t i) {
     size_t id = act[i];
-    if (!out.soma[id]) { geometric_flood(out.comp.cc, s, W, {out.root[id]}, out.daf); return; }
+    if (!out.soma[id]) { geometric_flood(out.comp.cc, s, W, {out.root[id]}, out.daf, gate); return; }
     i64 rx, ry, rz; s.unravel(out.root[id], rx, ry, rz);
     std::vector<std::pair<i64, float>> seeds = {{out.root[id], 0.0f}};
     for (i64 v : members[id]) {
@@ -868,7 +1004,7 @@ Prepared prepare(const i64 *labels, const Shape &s, const Params &p, int threads
       float val;
       if (freespace_seed(x - rx, y - ry, z - rz, p.aniso[0], p.aniso[1], p.aniso[2], out.dbf_max[id], val)) seeds.push_back({v, val});
     }
-    geometric_flood_seeded(out.comp.cc, s, W, seeds, out.daf);
+    geometric_flood_seeded(out.comp.cc, s, W, seeds, out.daf, gate);
   });
   seg_argmax(out.daf, out.comp.cc, k, out.max_daf, out.target);
   double exponent = p.pdrf_exponent;
@@ -917,6 +1053,7 @@ struct Tracer {
   // When set, each invalidation leaves here the final claim key of every voxel it claimed
   // ((bottleneck bits << 32) | source; MAXKEY elsewhere), for the stage-level checks.
   std::vector<u64> *claims = nullptr;
+  Gate gate;  // a voxel graph's steps (pushes: railroad, join, offers; pulls: backtrace, band pulls)
 
   // Railroad (lockstep_coop.inc:71-305 with brook_vw_step) + backtrace and join
   // (lockstep_regular.inc:61-154). Returns the path target-first; empty: no rail reachable.
@@ -938,7 +1075,7 @@ struct Tracer {
         i64 nx = x + NX[k], ny = y + NY[k], nz = z + NZ[k];
         if (!s.inside(nx, ny, nz)) continue;
         i64 v = s.at(nx, ny, nz);
-        if (cc[v] != L) continue;
+        if (cc[v] != L || !gate.push(it.v, k)) continue;
         float fv = pdrf[v];
         float c = vw_step(it.d, fv);
         if (c < dist[v]) {
@@ -963,18 +1100,19 @@ struct Tracer {
             i64 nx = x + NX[k], ny = y + NY[k], nz = z + NZ[k];
             if (!s.inside(nx, ny, nz)) continue;
             i64 u = s.at(nx, ny, nz);
-            if (cc[u] != L || pdrf[u] == 0.0f) continue;
+            if (cc[u] != L || pdrf[u] == 0.0f || !gate.pull(u, k)) continue;
             if (dist[u] < best) { best = dist[u]; kb = k; }
           }
         }
         if (cnt == 0 && kb >= 0) {  // the join: u's first rail in dijkstra3d order
           i64 ux = x + NX[kb], uy = y + NY[kb], uz = z + NZ[kb];
+          const i64 uidx = s.at(ux, uy, uz);
           int r = 64;
           for (int k = 0; k < 26; ++k) {
             i64 nx = ux + NX[k], ny = uy + NY[k], nz = uz + NZ[k];
             if (!s.inside(nx, ny, nz)) continue;
             i64 v = s.at(nx, ny, nz);
-            if (cc[v] == L && pdrf[v] == 0.0f) r = std::min(r, D3RANK[k]);
+            if (cc[v] == L && pdrf[v] == 0.0f && gate.push(uidx, k)) r = std::min(r, D3RANK[k]);
           }
           if (r < 26) {
             int ks = D3INV[r];
@@ -1034,7 +1172,7 @@ struct Tracer {
             i64 nx = x + NX[k], ny = y + NY[k], nz = z + NZ[k];
             if (!s.inside(nx, ny, nz)) continue;
             i64 v = s.at(nx, ny, nz);
-            if (cc[v] != L || !alive[v]) continue;
+            if (cc[v] != L || !alive[v] || !gate.push(u, k)) continue;
             float d = offer_d(nx, ny, nz, su);
             if (d >= maxd) continue;
             uint32_t mb = std::max(fbits(d), mu);
@@ -1066,7 +1204,7 @@ struct Tracer {
           i64 nx = x + NX[k], ny = y + NY[k], nz = z + NZ[k];
           if (!s.inside(nx, ny, nz)) continue;
           i64 u = s.at(nx, ny, nz);
-          if (cc[u] != L) continue;
+          if (cc[u] != L || !gate.pull(u, k)) continue;
           u64 ku = key[u];
           if (ku == MAXKEY) continue;
           i64 su = i64(ku & 0xffffffffULL);
@@ -1232,8 +1370,8 @@ Result skeletonize(const i64 *labels, const Shape &s, const Params &p, const std
     std::vector<float> vdist(s.n(), FINF);
     std::vector<i64> roots;
     for (size_t id : labels_list) roots.push_back(pr.root[id]);
-    vw_flood(pr.comp.cc, s, pr.pdrf, roots, vdist);
-    parent_field(pr.comp.cc, s, pr.active, vdist, parent);
+    vw_flood(pr.comp.cc, s, pr.pdrf, roots, vdist, Gate{p.graph});
+    parent_field(pr.comp.cc, s, pr.active, vdist, parent, Gate{p.graph});
   }
   // the root becomes a rail (lockstep.cu:183)
   for (size_t id : labels_list) pr.pdrf[pr.root[id]] = 0.0f;
@@ -1246,6 +1384,7 @@ Result skeletonize(const i64 *labels, const Shape &s, const Params &p, const std
     sort_target_order(ord, pr.daf.data());
     Tracer tr{s, pr.comp.cc, pr.dbf, pr.pdrf, dist, key, alive, inq, p.aniso[0], p.aniso[1], p.aniso[2],
               float(p.scale), float(p.constant), aniso_min};
+    tr.gate = Gate{p.graph};
     size_t cursor = 0;
     i64 pcount = 0;
     // a soma: its ball dies before the first path and the budget shrinks by what it took
@@ -1392,6 +1531,28 @@ int64_t oracle_ccl(const int64_t *labels, int64_t sx, int64_t sy, int64_t sz, ui
     count = int64_t(c.roots.size());
   });
   return count;
+}
+
+// Voxel-graph components: returns the count; keys_out holds count + 1 entries, uniform_out one.
+int64_t oracle_graph_ccl(const int64_t *labels, int64_t sx, int64_t sy, int64_t sz, const uint32_t *graph, int compact_2d,
+                         uint32_t *cc_out, int64_t *mapping_out, int64_t *roots_out, int64_t *keys_out,
+                         int32_t *uniform_out) {
+  int64_t count = -1;
+  guarded([&] {
+    GraphComponents c = graph_ccl(labels, graph, compact_2d != 0, Shape{sx, sy, sz});
+    std::copy(c.cc.begin(), c.cc.end(), cc_out);
+    std::copy(c.mapping.begin(), c.mapping.end(), mapping_out);
+    std::copy(c.roots.begin(), c.roots.end(), roots_out);
+    std::copy(c.keys.begin(), c.keys.end(), keys_out);
+    *uniform_out = c.input_uniform ? 1 : 0;
+    count = int64_t(c.roots.size());
+  });
+  return count;
+}
+
+int oracle_graph_edt(const uint32_t *cc, int64_t sx, int64_t sy, int64_t sz, const uint32_t *graph, const float *aniso,
+                     int black_border, float *out) {
+  return guarded([&] { graph_edt<uint32_t>(cc, graph, Shape{sx, sy, sz}, aniso, black_border != 0, out); });
 }
 
 int oracle_edt(const uint32_t *cc, int64_t sx, int64_t sy, int64_t sz, const float *aniso, int black_border, int dims,
@@ -1594,10 +1755,12 @@ int oracle_vw_parents(const int64_t *labels, int64_t sx, int64_t sy, int64_t sz,
 void *oracle_skeletonize(const int64_t *labels, int64_t sx, int64_t sy, int64_t sz, const double *params,
                          int64_t max_paths, const float *aniso, double dust, int fix_branching, int fix_borders,
                          const int64_t *targets_before, int64_t ntargets, const int64_t *targets_after,
-                         int64_t nafter, int threads) {
+                         int64_t nafter, int threads, const uint32_t *graph, int graph_compact_2d) {
   PackedOut *out = nullptr;
   guarded([&] {
     Params p;
+    p.graph = graph;
+    p.graph_compact_2d = graph_compact_2d != 0;
     p.scale = params[0]; p.constant = params[1]; p.pdrf_scale = params[2]; p.pdrf_exponent = params[3];
     p.soma_detection = params[4]; p.soma_acceptance = params[5]; p.soma_scale = params[6]; p.soma_constant = params[7];
     p.max_paths = max_paths;

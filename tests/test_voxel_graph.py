@@ -1,0 +1,172 @@
+"""Voxel graphs (the `voxel_graph` option; graph_components.cu, graph_edt.cu, the graph gates of the
+floods, railroads, backtraces and invalidations): the oracle pinned by the reference's known answers
+(c_smoke.c:37-50), its Kimimaro 5.8.1 answers (tests/golden/graphs.py), brute force and the plain
+path; then the GPU against the oracle through the C ABI."""
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle
+from tests.golden import graphs, shapes
+
+
+def _summary(packed):
+    return {k: tuple(v) for k, v in shapes.summary_from_packed(packed).items()}
+
+
+def _full_graph(labels):
+    """Every step between equal nonzero labels (no cut)."""
+    return graphs.voxel_graph(labels, np.zeros(labels.shape, bool), np.zeros(labels.shape, bool))
+
+
+# ---- oracle pins ---------------------------------------------------------------------------------
+
+def test_graph_ccl_known_answer():
+    # c_smoke.c:37-50: an all-zero graph isolates every voxel; the foreground voxels are components
+    # 1..4 in root order, numbered without gaps for the background
+    labels = np.array([1, 1, 0, 0, 0, 0, 2, 2], np.uint8)[:, None, None]
+    cc, mapping, roots, keys, uniform = oracle.graph_ccl(labels, np.zeros(labels.shape, np.uint32))
+    assert cc[:, 0, 0].tolist() == [1, 2, 0, 0, 0, 0, 3, 4]
+    assert mapping.tolist() == [0, 1, 1, 2, 2] and roots.tolist() == [0, 1, 6, 7]
+    assert keys.tolist() == [0, 1, 2, 7, 8] and not uniform
+    edt = oracle.graph_edt(cc, np.zeros(labels.shape, np.uint32))
+    assert edt[:, 0, 0].tolist() == [0.5, 0.5, 0, 0, 0, 0, 0.5, 0.5]
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_graph_ccl_of_the_labels_own_graph_is_ccl(seed):
+    rng = np.random.default_rng(seed)
+    shape = tuple(int(x) for x in rng.integers(4, 14, 3))
+    labels = (rng.integers(0, 4, shape) * (rng.random(shape) < 0.5)).astype(np.uint32)
+    cc, mapping, roots = oracle.ccl(labels)
+    gcc, gmapping, groots, keys, _ = oracle.graph_ccl(labels, _full_graph(labels))
+    assert np.array_equal(cc, gcc) and np.array_equal(mapping, gmapping) and np.array_equal(roots, groots)
+    # keys: each component's rank among all of them by lowest voxel; without steps from the
+    # background every background voxel is a component of its own
+    bg_before = np.concatenate([[0], np.cumsum(labels.ravel(order="F") == 0)])
+    assert keys.tolist() == [0] + [int(bg_before[r] + k + 1) for k, r in enumerate(roots)]
+
+
+def test_graph_ccl_joins_labels_the_graph_joins():
+    labels, graph = graphs.spanning_labels()
+    cc, mapping, roots, keys, _ = oracle.graph_ccl(labels, graph)
+    assert len(roots) == 1 and mapping.tolist() == [0, 5]  # the label where the last run starts
+    assert roots[0] == np.flatnonzero(labels.ravel(order="F"))[0]
+    assert keys.tolist() == [0, 1]  # the open graph joins the background too: one component in all
+
+
+def _graph_edt_brute(labels, graph, aniso, black_border):
+    """Distance from each voxel centre to the nearest background position of the doubled grid."""
+    s = labels.shape
+    d = tuple(2 * n for n in s)
+    mask = np.zeros(d, bool)
+    for X, Y, Z in itertools.product(*(range(n) for n in d)):
+        if black_border and (X == d[0] - 1 or Y == d[1] - 1 or Z == d[2] - 1):
+            continue
+        src = (X // 2, Y // 2, Z // 2)
+        if not labels[src]:
+            continue
+        sub = (X & 1) | ((Y & 1) << 1) | ((Z & 1) << 2)
+        bit = {1: 0, 2: 2, 4: 4}.get(sub)
+        mask[X, Y, Z] = True if bit is None else bool((int(graph[src]) >> bit) & 1)
+    zeros = np.argwhere(~mask).astype(np.float64)
+    out = np.zeros(s)
+    half = np.array(aniso, np.float64) / 2
+    for p in itertools.product(*(range(n) for n in s)):
+        if not labels[p]:
+            continue
+        c = 2 * np.array(p, np.float64)
+        best = np.inf
+        if len(zeros):
+            best = np.min(np.sum(((zeros - c) * half) ** 2, axis=1))
+        if black_border:  # outside the volume along any axis, one doubled position beyond each face
+            for a in range(3):
+                best = min(best, ((c[a] + 1) * half[a]) ** 2, ((d[a] - c[a]) * half[a]) ** 2)
+        out[p] = np.sqrt(best)
+    return out
+
+
+@pytest.mark.parametrize("seed,black_border", [(0, False), (1, True), (2, False), (3, True)])
+def test_graph_edt_is_the_doubled_grid_distance(seed, black_border):
+    rng = np.random.default_rng(40 + seed)
+    shape = tuple(int(x) for x in rng.integers(3, 7, 3))
+    labels = (rng.random(shape) < 0.7).astype(np.uint32) * rng.integers(1, 3, shape).astype(np.uint32)
+    graph = rng.integers(0, 1 << 26, shape).astype(np.uint32)
+    aniso = [(1., 1., 1.), (4., 4., 40.), (1., 1.13, 1.37)][seed % 3]
+    got = oracle.graph_edt(labels, graph, aniso, black_border)
+    want = _graph_edt_brute(labels, graph, aniso, black_border)
+    fg = labels > 0
+    assert np.allclose(got[fg], want[fg], rtol=1e-6, atol=0) and not got[~fg].any()
+
+
+@pytest.mark.parametrize("fix_branching", [True, False])
+def test_oracle_spanning_labels_equal_kimimaro(fix_branching):
+    labels, graph = graphs.spanning_labels()
+    for extra in (dict(), dict(fill_holes=True, fix_avocados=True)):
+        got = oracle.skeletonize(labels, graphs.HOLE_PARAMS, dust_threshold=10, fix_branching=fix_branching,
+                                 voxel_graph=graph, **extra)
+        assert _summary(got) == graphs.SPANNING
+
+
+@pytest.mark.parametrize("fix_branching", [False, True])
+def test_oracle_cut_tube_fill_holes_equals_kimimaro(fix_branching):
+    labels, gs = graphs.cut_tube_graphs()
+    for name, graph in gs.items():
+        for i, aniso in enumerate([(1., 1., 1.), (1., 1.13, 1.37)]):
+            got = _summary(oracle.skeletonize(labels, graphs.HOLE_PARAMS, aniso, 10, fix_branching=fix_branching,
+                                              fill_holes=True, voxel_graph=graph))
+            assert list(got) == [5] and got[5][0] - got[5][1] == 2, (name, aniso)
+            assert got[5][2] == graphs.CUT_TUBE_RADIUS[name][i], (name, aniso)
+            if not fix_branching and (name, aniso) in graphs.CUT_TUBE_DIGESTS:
+                assert got[5] == graphs.CUT_TUBE_DIGESTS[name, aniso], (name, aniso)
+
+
+def test_oracle_avocados_with_graph_equal_kimimaro():
+    labels = graphs.avocado(graphs.AVOCADO_JITTER)
+    graph = np.asfortranarray(graphs.label_graph(labels, np.zeros(labels.shape, bool)))
+    params = dict(graphs.HOLE_PARAMS, soma_detection_threshold=10, soma_acceptance_threshold=11)
+    plain = oracle.skeletonize(labels, params, dust_threshold=10, fix_branching=False, voxel_graph=graph)
+    assert _summary(plain) == graphs.AVOCADO_PLAIN
+    for fill_holes in (False, True):
+        got = oracle.skeletonize(labels, params, dust_threshold=10, fix_branching=False, voxel_graph=graph,
+                                 fill_holes=fill_holes, fix_avocados=True)
+        assert _summary(got) == {3: graphs.AVOCADO_PLAIN[3]}
+        branched = _summary(oracle.skeletonize(labels, params, dust_threshold=10, voxel_graph=graph,
+                                               fill_holes=fill_holes, fix_avocados=True))
+        assert list(branched) == [3] and branched[3][0] - branched[3][1] == 1 and branched[3][2] == 9.5
+
+
+def test_oracle_avocado_order_with_graph_equals_kimimaro():
+    labels, graph = graphs.avocado_order()
+    aniso = (1., 1.13, 1.37)
+    params = dict(graphs.HOLE_PARAMS, soma_detection_threshold=10, soma_acceptance_threshold=1000)
+    plain = _summary(oracle.skeletonize(labels, params, aniso, 10, fix_branching=False, voxel_graph=graph))
+    assert plain[8] == graphs.AVOCADO_ORDER_NUCLEUS and plain[3][:3] == graphs.AVOCADO_ORDER_SOMA
+    corrected = oracle.skeletonize(labels, params, aniso, 10, fix_branching=False, voxel_graph=graph, fix_avocados=True)
+    assert _summary(corrected) == graphs.AVOCADO_ORDER_CORRECTED
+    branched = _summary(oracle.skeletonize(labels, params, aniso, 10, voxel_graph=graph, fix_avocados=True))
+    assert list(branched) == [3] and branched[3][0] - branched[3][1] == 1
+    assert branched[3][2] == graphs.AVOCADO_ORDER_BRANCHED_RADIUS
+    assert oracle.graph_ccl(labels, graph)[3].tolist() == graphs.AVOCADO_ORDER_KEYS
+
+
+@pytest.mark.parametrize("fix_branching", [False, True])
+def test_oracle_soma_voids_with_graph_equal_kimimaro(fix_branching):
+    labels, graph = graphs.soma_voids()
+    params = dict(graphs.HOLE_PARAMS, soma_detection_threshold=3, soma_acceptance_threshold=4)
+    for aniso, radius in graphs.SOMA_VOID_RADIUS.items():
+        got = _summary(oracle.skeletonize(labels, params, aniso, 10, fix_branching=fix_branching, voxel_graph=graph))
+        assert list(got) == [1] and got[1][0] - got[1][1] == 1 and got[1][2] == radius
+
+
+@pytest.mark.parametrize("fix_branching", [False, True])
+def test_oracle_painted_root_with_graph_equals_kimimaro(fix_branching):
+    labels = shapes.painted_root()
+    graph = np.asfortranarray(graphs.label_graph(labels, np.zeros(labels.shape, bool)))
+    for fix_avocados in (False, True):
+        got = _summary(oracle.skeletonize(labels, graphs.HOLE_PARAMS, dust_threshold=10, fix_borders=False,
+                                          fix_branching=fix_branching, fill_holes=True, fix_avocados=fix_avocados,
+                                          voxel_graph=graph))
+        assert list(got) == [2, 1] and all(v[0] - v[1] == 1 for v in got.values())
+        assert {label: v[2] for label, v in got.items()} == graphs.PAINTED_ROOT_GRAPH_RADIUS
