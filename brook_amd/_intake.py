@@ -203,6 +203,18 @@ def volume_struct(labels, keep, streams=None):
     return v, a.size
 
 
+def graph_struct(voxel_graph, labels, keep, streams=None):
+    """brook_volume of a voxel graph (uint32 words, or uint8 for one XY plane; shape and layout
+    checked by the engine). A device graph needs device labels, as the reference's intake requires
+    (_intake.py:233-239)."""
+    if _is_device(voxel_graph) and not _is_device(labels):
+        raise TypeError("brook: a voxel_graph on the device needs device labels; pass the labels on the device too, "
+                        "or the graph as a host array")
+    v, _ = volume_struct(voxel_graph, keep, streams)
+    keep.append(v)
+    return v
+
+
 def _packed_from_handle(handle):
     view = native.SkeletonsView()
     try:
@@ -228,11 +240,12 @@ def skeletonize(all_labels, teasar_params=DEFAULT_TEASAR_PARAMS, anisotropy=(1, 
     Device input produced on another stream is fenced, not host-synchronized: the engine waits for
     `stream` (default: torch's current stream on the tensor's device; see stream_handle).
     `parallel`, `parallel_chunk_size`, `in_place`, `borrow` and `progress` are accepted for
-    compatibility. Keys left out of `teasar_params` take kimimaro.trace's defaults."""
+    compatibility. Keys left out of `teasar_params` take kimimaro.trace's defaults. `voxel_graph`
+    (uint32 words in cc3d's convention, or uint8 for one XY plane; host or device, device only with
+    device labels) replaces the labels' own connectivity: components are the graph's, and every
+    flood, path and invalidation steps only where the graph permits."""
     if output not in ("skeletons", "packed", "packed_device"):
         raise ValueError("brook: output must be 'skeletons', 'packed' or 'packed_device'")
-    if voxel_graph is not None:
-        raise NotImplementedError("voxel_graph is not supported by this build")
     if streaming:
         raise NotImplementedError("streaming is not supported by this build (in-core on 288 GB of HBM)")
     anisotropy = tuple(float(np.float32(a)) for a in anisotropy)
@@ -242,9 +255,11 @@ def skeletonize(all_labels, teasar_params=DEFAULT_TEASAR_PARAMS, anisotropy=(1, 
         return _empty(output, anisotropy)
     opts = options_struct(teasar_params, anisotropy, dust_threshold, fix_branching, fix_borders, fill_holes,
                           fix_avocados, object_ids, extra_targets_before, extra_targets_after, keep)
+    if voxel_graph is not None:
+        opts.voxel_graph = ctypes.addressof(graph_struct(voxel_graph, all_labels, keep, devices))
     ctx = native.context(devices[0].device.index if devices else None)
-    if devices:
-        fence(ctx, stream_handle(stream, devices[0]))
+    for handle in sorted({stream_handle(stream, t) for t in devices}):
+        fence(ctx, handle)
     h = ctypes.c_void_p()
     native.check(native.lib().brook_skeletonize(ctx.handle, ctypes.byref(vol), ctypes.byref(opts), ctypes.byref(h)))
     packed = _packed_from_handle(h)

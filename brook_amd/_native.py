@@ -94,10 +94,15 @@ def lib():
                 "brook_components_copy_to_host": (S, [_P, _P, _c.c_size_t]),
                 "brook_components_labels": (S, [_P, _c.POINTER(_P)]),
                 "brook_edt": (S, [_P, _c.POINTER(Volume), _P, _c.c_int, _c.POINTER(_P)]),
+                "brook_graph_components": (S, [_P, _c.POINTER(Volume), _c.POINTER(Volume), _c.POINTER(_P)]),
+                "brook_graph_edt": (S, [_P, _c.POINTER(Volume), _c.POINTER(Volume), _P, _c.c_int, _c.POINTER(_P)]),
+                "brook_components_keys": (S, [_P, _c.POINTER(_P), _c.POINTER(_c.c_size_t)]),
                 "brook_skeletonize_options_init": (None, [_c.POINTER(SkeletonizeOptions)]),
                 "brook_skeletonize": (S, [_P, _c.POINTER(Volume), _c.POINTER(SkeletonizeOptions), _c.POINTER(_P)]),
                 "brook_skeletonize_batch": (S, [_P, _c.POINTER(_c.POINTER(Volume)), _c.c_size_t,
                                                 _c.POINTER(SkeletonizeOptions), _c.POINTER(_P)]),
+                "brook_skeletonize_batch_graphs": (S, [_P, _c.POINTER(_c.POINTER(Volume)), _c.POINTER(_c.POINTER(Volume)),
+                                                       _c.c_size_t, _c.POINTER(SkeletonizeOptions), _c.POINTER(_P)]),
                 "brook_skeletons_get_view": (S, [_P, _c.POINTER(SkeletonsView)]),
                 "brook_skeletons_destroy": (None, [_P]),
                 "brook_skeletons_get_device_view": (S, [_P, _c.POINTER(SkeletonsDeviceView)]),

@@ -242,8 +242,14 @@ std::pair<uint32_t, uint32_t> fruit_of(const std::vector<uint32_t> &labels, cons
 }  // namespace
 
 int avocado_correct(std::vector<uint32_t> &labels, std::vector<float> &distance, Dims d, size_t table, double detection,
-                    const std::function<void(const std::vector<uint32_t> &, std::vector<float> &)> &refresh) {
+                    const std::function<void(const std::vector<uint32_t> &, std::vector<float> &)> &refresh,
+                    const std::vector<int64_t> &keys) {
   const Grid g{{d.sx, d.sy, d.sz}};
+  // the sets hold keys; labels and boxes are indexed by compact id
+  auto key = [&](size_t id) { return keys.empty() ? id : size_t(keys[id]); };
+  auto id_of = [&](size_t k) {
+    return keys.empty() ? k : size_t(std::lower_bound(keys.begin(), keys.end(), int64_t(k)) - keys.begin());
+  };
   PySet unchanged;
   int passes = 0;
   for (; passes < 20; ++passes) {
@@ -252,7 +258,7 @@ int avocado_correct(std::vector<uint32_t> &labels, std::vector<float> &distance,
     for (size_t i = 0; i < labels.size(); ++i) present[distance[i] > cutoff ? labels[i] : 0] = 1;
     PySet candidates;
     for (size_t id = 0; id < table; ++id)
-      if (present[id]) candidates.add(id);
+      if (present[id]) candidates.add(key(id));
     candidates.difference_update(unchanged);
     candidates.discard(0);
     if (!candidates.size()) break;
@@ -270,7 +276,8 @@ int avocado_correct(std::vector<uint32_t> &labels, std::vector<float> &distance,
       }
     }
     PySet unchanged_now, changed;
-    for (size_t label : candidates.items()) {
+    for (size_t candidate : candidates.items()) {
+      const size_t label = id_of(candidate);
       const Span &b = boxes[label];
       const Grid box{{b.hi[0] - b.lo[0] + 1, b.hi[1] - b.lo[1] + 1, b.hi[2] - b.lo[2] + 1}};
       auto global = [&](size_t i) {
@@ -295,13 +302,13 @@ int avocado_correct(std::vector<uint32_t> &labels, std::vector<float> &distance,
       for (int a = 0; a < 3; ++a) centre[a] += b.lo[a];
       const auto po = fruit_of(labels, g, centre);
       const uint32_t pit = po.first, owner = po.second;
-      if (pit == owner && !changed.contains(pit)) {
-        unchanged_now.add(pit);
+      if (pit == owner && !changed.contains(key(pit))) {
+        unchanged_now.add(key(pit));
       } else {
-        unchanged_now.discard(pit);
-        unchanged_now.discard(owner);
-        changed.add(pit);
-        changed.add(owner);
+        unchanged_now.discard(key(pit));
+        unchanged_now.discard(key(owner));
+        changed.add(key(pit));
+        changed.add(key(owner));
         for (size_t i = 0; i < mask.size(); ++i) mask[i] |= labels[global(i)] == owner;
       }
       fill_background(mask, box, 3);

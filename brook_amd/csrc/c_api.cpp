@@ -9,6 +9,7 @@
 
 #include "../../include/brook.h"
 #include "engine.hpp"
+#include "graph.hpp"
 #include "layout.hpp"
 #include "merge.hpp"
 
@@ -142,6 +143,35 @@ Input normalize(brook::Context &ctx, const brook_volume *v) {
 
 brook_dtype unsigned_dtype(int bytes) { return bytes == 1 ? BROOK_U8 : bytes == 2 ? BROOK_U16 : bytes == 4 ? BROOK_U32 : BROOK_U64; }
 
+// A voxel graph as one uint32 word per voxel in Fortran device memory (graph_components.cu:178-182:
+// uint32, or uint8 for one XY plane, whose values are widened as they are).
+struct GraphInput {
+  Input in;
+  std::unique_ptr<brook::Buffer<unsigned>> wide;
+  const unsigned *words = nullptr;
+  bool compact_2d = false;
+};
+
+GraphInput normalize_graph(brook::Context &ctx, const brook_volume *g, const brook_volume *labels) {
+  check_volume(g);
+  bool same = true;
+  for (int a = 0; a < 3; ++a) same = same && g->shape[a] == labels->shape[a];
+  require(same && (g->dtype == BROOK_U32 || (g->dtype == BROOK_U8 && g->shape[2] == 1)),
+          "voxel_graph must match the label shape and use uint32, or uint8 for an XY plane");
+  GraphInput gi;
+  gi.in = normalize(ctx, g);
+  gi.compact_2d = g->dtype == BROOK_U8;
+  const int64_t n = gi.in.view.size();
+  if (gi.compact_2d && n) {
+    gi.wide = std::make_unique<brook::Buffer<unsigned>>(ctx, size_t(n));
+    brook::widen_words(ctx, static_cast<const uint8_t *>(gi.in.view.data), n, gi.wide->data());
+    gi.words = gi.wide->data();
+  } else {
+    gi.words = static_cast<const unsigned *>(gi.in.view.data);
+  }
+  return gi;
+}
+
 }  // namespace
 
 struct brook_context {
@@ -159,6 +189,7 @@ struct brook_array {
 struct brook_components {
   brook_array labels;
   std::vector<int64_t> mapping, roots;
+  std::vector<int64_t> keys;  // voxel-graph components: Kimimaro's id of each id (empty otherwise)
 };
 struct brook_skeletons {
   brook::Packed packed;
@@ -317,7 +348,13 @@ brook::Packed run_one(brook::Context &ctx, const brook_volume *v, const brook_sk
   brook::SkeletonizeOptions o = convert(p, &ids);
   Input in;
   brook::Buffer<uint8_t> masked;
-  const brook::LabelVolume view = engine_input(ctx, v, p, ids, in, masked);
+  brook::LabelVolume view = engine_input(ctx, v, p, ids, in, masked);
+  GraphInput graph;
+  if (p->voxel_graph) {
+    graph = normalize_graph(ctx, p->voxel_graph, v);
+    view.graph = graph.words;
+    view.graph_compact_2d = graph.compact_2d;
+  }
   return brook::skeletonize(ctx, view, o, fields);
 }
 
@@ -474,6 +511,65 @@ brook_status brook_connected_components_noalloc(brook_context *ctx, const brook_
   });
 }
 
+brook_status brook_graph_components(brook_context *ctx, const brook_volume *labels, const brook_volume *voxel_graph,
+                                   brook_components **out) {
+  return protect([&] {
+    require(ctx && labels && voxel_graph && out);
+    *out = nullptr;
+    require(integer_dtype(labels->dtype), "voxel graph labels must be integers");
+    brook::ContextCall call(ctx->value);
+    Input in = normalize(ctx->value, labels);
+    GraphInput graph = normalize_graph(ctx->value, voxel_graph, labels);
+    brook::GraphComponents g = brook::graph_components(ctx->value, in.view, graph.words, graph.compact_2d);
+    auto r = std::make_unique<brook_components>();
+    const size_t n = size_t(in.view.size());
+    if (n) {
+      auto *a = owned_array(ctx->value, g.cc.labels.data(), n * g.cc.cc_bytes, unsigned_dtype(g.cc.cc_bytes), labels->shape);
+      r->labels = std::move(*a);
+      delete a;
+    } else {
+      r->labels.memory = std::make_shared<DeviceMemory>(ctx->value.device(), 0);
+      r->labels.dtype = BROOK_U8;
+      std::copy(labels->shape, labels->shape + 3, r->labels.shape);
+    }
+    r->mapping = g.cc.mapping;
+    r->roots = g.cc.roots;
+    r->keys = g.keys;
+    *out = r.release();
+  });
+}
+
+brook_status brook_components_keys(const brook_components *c, const int64_t **keys, size_t *count) {
+  return protect([&] {
+    require(c && keys && count);
+    *keys = c->keys.empty() ? nullptr : c->keys.data();
+    *count = c->keys.size();
+  });
+}
+
+brook_status brook_graph_edt(brook_context *ctx, const brook_volume *labels, const brook_volume *voxel_graph,
+                             const float anisotropy[3], int black_border, brook_array **out) {
+  return protect([&] {
+    require(ctx && labels && voxel_graph && anisotropy && out);
+    *out = nullptr;
+    require(labels->dtype >= BROOK_U8 && labels->dtype <= BROOK_U64, "graph EDT expects unsigned component labels");
+    brook::ContextCall call(ctx->value);
+    Input in = normalize(ctx->value, labels);
+    GraphInput graph = normalize_graph(ctx->value, voxel_graph, labels);
+    const size_t n = size_t(in.view.size());
+    auto a = std::make_unique<brook_array>();
+    a->memory = std::make_shared<DeviceMemory>(ctx->value.device(), n * 4);
+    a->dtype = BROOK_F32;
+    std::copy(labels->shape, labels->shape + 3, a->shape);
+    if (n) {
+      brook::graph_edt(ctx->value, in.view.data, in.view.bytes, in.view.dims, graph.words, anisotropy, black_border != 0,
+                       static_cast<float *>(a->memory->ptr));
+      ctx->value.synchronize();
+    }
+    *out = a.release();
+  });
+}
+
 brook_status brook_edt(brook_context *ctx, const brook_volume *labels, const float anisotropy[3], int black_border,
                        brook_array **out) {
   return protect([&] {
@@ -558,18 +654,28 @@ brook_status brook_skeletonize(brook_context *ctx, const brook_volume *labels, c
   });
 }
 
-brook_status brook_skeletonize_batch(brook_context *ctx, const brook_volume *const *volumes, size_t count,
-                                     const brook_skeletonize_options *options, brook_skeletons **out) {
+namespace {
+brook_status batch(brook_context *ctx, const brook_volume *const *volumes, const brook_volume *const *graphs,
+                   size_t count, const brook_skeletonize_options *options, brook_skeletons **out) {
   return protect([&] {
     require(ctx && (volumes || !count) && options && (out || !count));
     for (size_t i = 0; i < count; ++i) out[i] = nullptr;
+    require(!options->voxel_graph, "a batch takes its voxel graphs per sample (brook_skeletonize_batch_graphs)");
     brook::ContextCall call(ctx->value);
     std::vector<uint64_t> ids;
     brook::SkeletonizeOptions o = convert(options, &ids);
     std::vector<Input> inputs(count);
+    std::vector<GraphInput> graph_inputs(count);
     std::vector<brook::Buffer<uint8_t>> masked(count);
     std::vector<brook::LabelVolume> views;
-    for (size_t i = 0; i < count; ++i) views.push_back(engine_input(ctx->value, volumes[i], options, ids, inputs[i], masked[i]));
+    for (size_t i = 0; i < count; ++i) {
+      views.push_back(engine_input(ctx->value, volumes[i], options, ids, inputs[i], masked[i]));
+      if (graphs && graphs[i]) {
+        graph_inputs[i] = normalize_graph(ctx->value, graphs[i], volumes[i]);
+        views.back().graph = graph_inputs[i].words;
+        views.back().graph_compact_2d = graph_inputs[i].compact_2d;
+      }
+    }
     std::vector<brook::Packed> packed = brook::skeletonize_batch(ctx->value, views, o);
     std::vector<std::unique_ptr<brook_skeletons>> results;
     for (size_t i = 0; i < count; ++i) {
@@ -580,6 +686,18 @@ brook_status brook_skeletonize_batch(brook_context *ctx, const brook_volume *con
     }
     for (size_t i = 0; i < count; ++i) out[i] = results[i].release();
   });
+}
+}  // namespace
+
+brook_status brook_skeletonize_batch(brook_context *ctx, const brook_volume *const *volumes, size_t count,
+                                     const brook_skeletonize_options *options, brook_skeletons **out) {
+  return batch(ctx, volumes, nullptr, count, options, out);
+}
+
+brook_status brook_skeletonize_batch_graphs(brook_context *ctx, const brook_volume *const *volumes,
+                                            const brook_volume *const *graphs, size_t count,
+                                            const brook_skeletonize_options *options, brook_skeletons **out) {
+  return batch(ctx, volumes, graphs, count, options, out);
 }
 
 brook_status brook_skeletons_get_view(const brook_skeletons *r, brook_skeletons_view *out) {

@@ -11,6 +11,7 @@
 
 #include "avocado.hpp"
 #include "fields.hpp"
+#include "graph.hpp"
 #include "soma.hpp"
 #include "trace.hpp"
 
@@ -57,7 +58,6 @@ Packed empty_result(const SkeletonizeOptions &o) {
 void check_options(const SkeletonizeOptions &o) {
   for (float a : o.aniso)
     if (!(a > 0) || !std::isfinite(a)) throw std::invalid_argument("anisotropy must be finite and positive");
-  if (o.voxel_graph) throw Unsupported("voxel_graph is not supported by this build");
   const double exponent = o.teasar.pdrf_exponent;
   if (std::isfinite(exponent) && exponent == std::floor(exponent) && std::fmod(exponent, 2.0) == 1.0)
     throw std::invalid_argument("odd pdrf_exponent makes the background penalty negative");
@@ -91,14 +91,23 @@ class Sample {
 
   void preamble(Lap &lap) {
     if (n_ == 0 || double(n_) <= o_.dust) return finish_empty();
-    cc_ = connected_components(ctx_, in_);
+    bool uniform = false;
+    if (in_.graph) {  // components of the graph's steps (skeletonize.cu:76-78)
+      GraphComponents g = graph_components(ctx_, in_, in_.graph, in_.graph_compact_2d);
+      cc_ = std::move(g.cc);
+      keys_ = std::move(g.keys);
+      uniform = g.input_uniform;
+    } else {
+      cc_ = connected_components(ctx_, in_);
+    }
     lap("prep_ccl_ms");
     count_ = cc_.count;
     if (count_ == 0) return finish_empty();
     k_ = size_t(count_) + 1;
     boxes_ = analyze(ctx_, cc_, d_);
     lap("prep_analyze_ms");
-    const bool black_border = count_ == 1 && boxes_[1].count == n_;  // of the volume before any fill
+    // of the volume before any fill; with a graph, every input voxel has one label (skeletonize.cu:80)
+    const bool black_border = in_.graph ? uniform : count_ == 1 && boxes_[1].count == n_;
     if (o_.fill_holes) {
       fill_holes();
       lap("prep_fill_holes_ms");
@@ -106,7 +115,10 @@ class Sample {
     dbf_ = Buffer<float>(ctx_, size_t(n_));
     Buffer<unsigned> label_max(ctx_, k_);
     label_max.clear(ctx_);
-    edt(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, o_.aniso, black_border, 3, dbf_.data(), label_max.data());
+    if (in_.graph)
+      graph_edt(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, in_.graph, o_.aniso, black_border, dbf_.data(), label_max.data());
+    else
+      edt(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, o_.aniso, black_border, 3, dbf_.data(), label_max.data());
     std::vector<unsigned> max_bits = label_max.download(ctx_);
     dbf_max_.resize(k_);
     for (size_t i = 0; i < k_; ++i) std::memcpy(&dbf_max_[i], &max_bits[i], 4);
@@ -171,7 +183,7 @@ class Sample {
         fill.paint(ctx_);
         for (size_t o = 1; o < k_; ++o) boxes_[o].count -= taken[o];
         boxes_[id].count += fill.voids();
-        dbf_max_[id] = fill.dbf_refresh(ctx_, o_.aniso, dbf_.data(), boxes_[id].count);
+        dbf_max_[id] = fill.dbf_refresh(ctx_, o_.aniso, dbf_.data(), boxes_[id].count, in_.graph);
         any_fill = true;
       }
       soma_[id] = double(dbf_max_[id]) > o_.teasar.soma_acceptance;
@@ -247,6 +259,10 @@ class Sample {
     fill_value(ctx_, daf_.data(), n_, kInf);
     rec_ = Buffer<VoxelRec>(ctx_, size_t(n_));  // 16 bytes per voxel: PDRF, neighbour mask + flags, distance word
     voxel_records(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, rec_.data());
+    if (in_.graph) {  // push masks in the records, pull masks beside them
+      inmask_ = Buffer<unsigned>(ctx_, size_t(n_));
+      graph_gates(ctx_, rec_.data(), in_.graph, d_, inmask_.data());
+    }
     // lists 2 and 3 are the railroad's far lists: two entries per voxel (trace.hip far_compact)
     for (int i = 0; i < 6; ++i) lists_.emplace_back(ctx_, size_t(std::max<int64_t>(F_, 1)) * (i == 2 || i == 3 ? 2 : 1));
     if (!o_.fix_branching) parent_ = Buffer<int>(ctx_, size_t(n_));
@@ -272,10 +288,12 @@ class Sample {
     auto refresh = [&](const std::vector<uint32_t> &current, std::vector<float> &field) {
       if (!scratch.size()) scratch = Buffer<uint32_t>(ctx_, n);
       scratch.upload(ctx_, current.data(), n);
-      edt(ctx_, scratch.data(), 4, d_, o_.aniso, black_border, 3, dbf_.data());
+      if (in_.graph) graph_edt(ctx_, scratch.data(), 4, d_, in_.graph, o_.aniso, black_border, dbf_.data());
+      else edt(ctx_, scratch.data(), 4, d_, o_.aniso, black_border, 3, dbf_.data());
       field = dbf_.download(ctx_, n);
     };
-    ctx_.stats["avocado_passes"] = avocado_correct(labels, distance, d_, k_, o_.teasar.soma_detection, refresh);
+    ctx_.stats["avocado_passes"] = avocado_correct(labels, distance, d_, k_, o_.teasar.soma_detection, refresh, keys_);
+    keys_.clear();  // the renumbered ids are their own keys
     std::vector<int64_t> mapping, roots;
     const size_t count = avocado_renumber(labels, original, cc_.mapping, k_, mapping, roots);
     const int bytes = id_bytes(int64_t(count));
@@ -426,10 +444,19 @@ class Sample {
     maxp_dev_.upload(ctx_, maxp.data(), k_);
     mflat_dev_.upload(ctx_, mflat.data(), mflat.size());
 
-    // trace buffers
+    // trace buffers, sized by what the trace can emit (the reference grows its store mid-trace,
+    // lockstep.cu:389-391): a label's paths number at most its ordered targets (alive voxels of
+    // the label, each path kills its own) plus its manual ones; a path's vertices are new voxels
+    // (not yet rail or emitted, and they become so) plus at most two it ends on (the rail reached
+    // and the join's rail), or a soma's root; so records <= F + manual targets and the store
+    // <= F + 2 * records. (A soma's parent walks do not stop at emitted vertices, so they may
+    // repeat voxels: the store's overflow check stays and reports such a case.)
     snap_b_ = Buffer<unsigned long long>(ctx_, fl);
-    store_cap_ = std::max<long long>(int64_t(1) << 22, 4 * F_);
-    rec_cap_ = int(std::max<int64_t>(int64_t(1) << 16, F_ / 8 + 4 * int64_t(labels_.size())));
+    const int64_t manual_targets = int64_t(mflat.size());
+    const int64_t paths_bound = F_ + manual_targets + int64_t(labels_.size());
+    if (paths_bound > INT32_MAX / 2) throw Unsupported("path records exceed the int32 store index");
+    rec_cap_ = int(std::max<int64_t>(1, paths_bound));
+    store_cap_ = std::max<long long>(1, F_ + 2 * paths_bound);
     store_ = Buffer<int>(ctx_, size_t(store_cap_));
     rec_label_ = Buffer<int>(ctx_, rec_cap_);
     rec_len_ = Buffer<int>(ctx_, rec_cap_);
@@ -486,6 +513,7 @@ class Sample {
     ta.parent = o_.fix_branching ? nullptr : parent_.data();
     ta.root = root_dev_.data();
     if (soma_dev_.size()) ta.soma = soma_dev_.data();
+    if (inmask_.size()) ta.inmask = inmask_.data();
     return ta;
   }
 
@@ -609,6 +637,7 @@ class Sample {
   Packed result_;
 
   Components cc_;
+  std::vector<int64_t> keys_;  // voxel graph: Kimimaro's ids of the components (until fix_avocados)
   int64_t count_ = 0, F_ = 0, flood_voxels_ = 0;
   size_t k_ = 0;
   std::vector<Box> boxes_;
@@ -621,6 +650,7 @@ class Sample {
   Buffer<int> seg_start_dev_, vox_, seg_count_dev_;
   Buffer<long long> first_dev_, root_dev_, target_dev_;
   Buffer<VoxelRec> rec_;
+  Buffer<unsigned> inmask_;  // voxel graph: pull masks (empty: no graph)
   std::vector<Buffer<int>> lists_;
   Buffer<int> order_;
   Buffer<int> parent_;  // fix_branching=False: parent field

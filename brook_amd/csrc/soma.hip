@@ -6,6 +6,7 @@
 
 #include <cstring>
 
+#include "graph.hpp"
 #include "soma.hpp"
 
 namespace brook {
@@ -172,6 +173,14 @@ template <class CC> __global__ void paint_voids(CC *cc, Dims d, Crop c, CC id, c
   cc[(c.lo[0] + x) + (long long)d.sx * ((c.lo[1] + y) + (long long)d.sy * (c.lo[2] + z))] = id;
 }
 
+// The graph words of the box (Fortran order over the box).
+__global__ void crop_words(Dims d, Crop c, const unsigned *graph, unsigned *out) {
+  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= c.size()) return;
+  const int x = int(i % c.ext[0]), y = int((i / c.ext[0]) % c.ext[1]), z = int(i / ((long long)c.ext[0] * c.ext[1]));
+  out[i] = graph[(c.lo[0] + x) + (long long)d.sx * ((c.lo[1] + y) + (long long)d.sy * (c.lo[2] + z))];
+}
+
 __global__ void overlay_dbf(Dims d, Crop c, const uint8_t *mask, const float *box_dbf, float *dbf) {
   const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
   if (i >= c.size() || !mask[i]) return;
@@ -280,14 +289,21 @@ void VoidFill::paint(Context &ctx) {
   BROOK_LAUNCHED();
 }
 
-float VoidFill::dbf_refresh(Context &ctx, const float aniso[3], float *dbf, int64_t filled_count) {
+float VoidFill::dbf_refresh(Context &ctx, const float aniso[3], float *dbf, int64_t filled_count, const unsigned *graph) {
   Crop c;
   for (int a = 0; a < 3; ++a) { c.lo[a] = lo_[a]; c.ext[a] = ext_[a]; }
   const Dims box{ext_[0], ext_[1], ext_[2]};
   Buffer<float> box_dbf(ctx, size_t(c.size()));
   Buffer<unsigned> maximum(ctx, 2);
   maximum.clear(ctx);
-  edt(ctx, mask_.data(), 1, box, aniso, filled_count == c.size(), 3, box_dbf.data(), maximum.data());
+  if (graph) {  // the graph EDT of the box with the box's graph words (trace.cu:107)
+    Buffer<unsigned> words(ctx, size_t(c.size()));
+    crop_words<<<blocks_for(c.size()), kThreads, 0, ctx.stream()>>>(d_, c, graph, words.data());
+    BROOK_LAUNCHED();
+    graph_edt(ctx, mask_.data(), 1, box, words.data(), aniso, filled_count == c.size(), box_dbf.data(), maximum.data());
+  } else {
+    edt(ctx, mask_.data(), 1, box, aniso, filled_count == c.size(), 3, box_dbf.data(), maximum.data());
+  }
   overlay_dbf<<<blocks_for(c.size()), kThreads, 0, ctx.stream()>>>(d_, c, mask_.data(), box_dbf.data(), dbf);
   BROOK_LAUNCHED();
   const unsigned bits = maximum.download(ctx)[1];

@@ -16,7 +16,8 @@ int64_t enclosed_voids(Context &ctx, const void *cc, int cc_bytes, Dims d, int64
 // The voids of id's box, found once and then applied (soma.cu:141-237 for a soma without private
 // arenas): taken()[o] = voxels of id o inside the voids; paint() writes id into every void voxel;
 // dbf_refresh() replaces the DBF of id's filled box by that box's own EDT (black border when the
-// filled mask fills the box) and returns its maximum.
+// filled mask fills the box; with a voxel graph, the graph EDT of the box's words) and returns its
+// maximum.
 class VoidFill {
  public:
   VoidFill(Context &ctx, void *cc, int cc_bytes, Dims d, int64_t id, const Box &box, size_t ids);
@@ -24,7 +25,7 @@ class VoidFill {
   const std::vector<int64_t> &taken() const { return taken_; }
   void paint(Context &ctx);
   // `filled_count`: the label's voxels after paint() (the box's black border when it is the box)
-  float dbf_refresh(Context &ctx, const float aniso[3], float *dbf, int64_t filled_count);
+  float dbf_refresh(Context &ctx, const float aniso[3], float *dbf, int64_t filled_count, const unsigned *graph = nullptr);
 
  private:
   void *cc_;

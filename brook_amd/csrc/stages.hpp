@@ -16,6 +16,10 @@ struct LabelVolume {
   int bytes = 4;
   bool is_signed = false;  // only changes how mapping values are widened to int64
   Dims dims{0, 0, 0};
+  // voxel_graph: one uint32 word per voxel on the device, Fortran order (null: none); a uint8
+  // plane graph arrives widened value for value, with graph_compact_2d set
+  const unsigned *graph = nullptr;
+  bool graph_compact_2d = false;
   int64_t size() const { return int64_t(dims.sx) * dims.sy * dims.sz; }
 };
 

@@ -131,6 +131,11 @@ __device__ __forceinline__ void lap(Shared &sh, int slot) {
   }
 }
 
+// The neighbours v may pull from (see TraceArgs::inmask).
+__device__ __forceinline__ unsigned in_mask(const TraceArgs &a, int v) {
+  return a.inmask ? g_ld(a.inmask + v) : rec_meta(a.rec, v) & kNbrBits;
+}
+
 // ---- railroad -------------------------------------------------------------------------------
 // Distances live in the records' packed words: high half the distance bits, bit 0 "not
 // queued", bit 1 "far listed". A front voxel is dequeued by one atomicOr that also returns its
@@ -173,8 +178,9 @@ __device__ __forceinline__ void relax_part(const TraceArgs &a, RoadMins &mins, i
   for_nb<B, E>([&](auto K) {
     constexpr int k = decltype(K)::value, i = k - B;
     const bool on = (m >> k) & 1u;
-    c[i] = on ? rec_pdrf(a.rec, u + nb_offset<k>(a.d)) : kInf;
-    seen[i] = on ? rec_word_hi(a.rec, (u + nb_offset<k>(a.d))) : 0u;
+    const RecWords r = on ? rec_load(a.rec, u + nb_offset<k>(a.d)) : RecWords{0x7f800000u, 0u, 0u, 0u};
+    c[i] = __uint_as_float(r.x);
+    seen[i] = r.w;
   });
   unsigned long long old[E - B];
   for_nb<B, E>([&](auto K) {
@@ -220,7 +226,7 @@ __device__ __forceinline__ float dequeue(const TraceArgs &a, int u) {
 }
 
 // ---- fronts of up to a quarter workgroup: a group of lanes per entry (group_of) ----------------
-constexpr int kGroupFront = kBlock / 4;
+constexpr int kGroupFront = kBlock / 2;
 __device__ __forceinline__ Group group_of(int n) { return brook::group_of(n, kBlock); }
 
 // Neighbours kb..kb+cnt-1 (those set in m; cnt <= kSpan) of u: relax_part with a run-time span.
@@ -235,8 +241,9 @@ __device__ __forceinline__ void relax_span(const TraceArgs &a, RoadMins &mins, i
     const int k = kb + j;
     const bool on = j < cnt && ((m >> k) & 1u);
     v[j] = u + off[on ? k : 0];
-    c[j] = on ? rec_pdrf(a.rec, v[j]) : kInf;
-    seen[j] = on ? rec_word_hi(a.rec, v[j]) : 0u;
+    const RecWords r = on ? rec_load(a.rec, v[j]) : RecWords{0x7f800000u, 0u, 0u, 0u};
+    c[j] = __uint_as_float(r.x);
+    seen[j] = r.w;
   }
   unsigned long long old[S];
 #pragma unroll
@@ -288,7 +295,8 @@ __device__ __forceinline__ void relax_groups(const TraceArgs &a, Shared &sh, Roa
   m = __shfl(m, q.lead, 64);
   du = __shfl(du, q.lead, 64);
   Pushes p;
-  if (q.i < nc && du <= bound) relax_span(a, mins, u, m, q.kb, q.cnt, du, bound, sh.off, p);
+  if (q.i < nc && du <= bound)
+    for (int o = 0; o < q.cnt; o += kSpan) relax_span(a, mins, u, m, q.kb + o, min(kSpan, q.cnt - o), du, bound, sh.off, p);
   emit_neighbours(touched, &sh.n_touched, p.touched, u, sh.off);
   emit_front(nxt, n_nxt, p.near, u, sh.off);
   emit_capped(far, &sh.n_far, p.far, u, sh.off, sh.far_cap, &sh.overflow);
@@ -457,7 +465,7 @@ __device__ __forceinline__ void backtrace(const TraceArgs &a, Shared &sh, int t,
   bool first = true;
   for (;;) {
     int kb = -1;
-    const unsigned m = (rec_meta(a.rec, cur) & kNbrBits);
+    const unsigned m = in_mask(a, cur);  // the predecessor's step to cur must be permitted
     if (cur != t) {
       float du = kInf;
       if (lane < 26 && ((m >> lane) & 1u)) {
@@ -510,7 +518,7 @@ __device__ __forceinline__ int parent_field(const TraceArgs &a, Shared &sh, int 
     const unsigned dv = word_dist_bits(lp_u64(&a.rec[v].word));
     int p = -1;
     if (v != root && dv != 0u && dv < 0x7f800000u) {
-      unsigned m = rec_meta(a.rec, v) & kNbrBits, best = 0x7f800000u;
+      unsigned m = in_mask(a, v), best = 0x7f800000u;
       while (m) {  // ascending k: the first of equal distances stays
         const int k = __ffs(m) - 1;
         m &= m - 1;
@@ -593,8 +601,9 @@ __device__ __forceinline__ void offer_part(const TraceArgs &a, const int *off, i
   for_nb<B, E>([&](auto K) {
     constexpr int k = decltype(K)::value, i = k - B;
     const bool on = (m >> k) & 1u;
-    al[i] = on && (rec_meta(a.rec, u + nb_offset<k>(a.d)) & kAlive);
-    kh[i] = on ? rec_word_hi(a.rec, (u + nb_offset<k>(a.d))) : 0u;
+    const RecWords r = on ? rec_load(a.rec, u + nb_offset<k>(a.d)) : RecWords{0u, 0u, 0u, 0u};
+    al[i] = r.y & kAlive;
+    kh[i] = r.w;
   });
   unsigned mbs[E - B];
   unsigned wait = 0;
@@ -633,8 +642,9 @@ __device__ __forceinline__ void offer_span(const TraceArgs &a, const int *off, i
     const int k = kb + j;
     const bool on = j < cnt && ((m >> k) & 1u);
     v[j] = u + off[on ? k : 0];
-    al[j] = on && (rec_meta(a.rec, v[j]) & kAlive);
-    kh[j] = on ? rec_word_hi(a.rec, v[j]) : 0u;
+    const RecWords r = on ? rec_load(a.rec, v[j]) : RecWords{0u, 0u, 0u, 0u};
+    al[j] = r.y & kAlive;
+    kh[j] = r.w;
   }
   unsigned mbs[S];
   unsigned wait = 0;
@@ -689,7 +699,8 @@ __device__ __forceinline__ void offer_groups(const TraceArgs &a, const Ball &bal
     int x, y, z, qx, qy, qz;
     unravel(u, a.d, x, y, z);
     unravel(int(su), a.d, qx, qy, qz);
-    offer_span(a, sh.off, u, m, q.kb, q.cnt, x, y, z, qx, qy, qz, su, mu, maxd, hi, p);
+    for (int o = 0; o < q.cnt; o += kSpan)
+      offer_span(a, sh.off, u, m, q.kb + o, min(kSpan, q.cnt - o), x, y, z, qx, qy, qz, su, mu, maxd, hi, p);
   }
   emit_neighbours(touched, &sh.n_touched, p.touched, u, sh.off);
   emit_front(nxt, n_nxt, p.near, u, sh.off);
@@ -775,12 +786,12 @@ __device__ __forceinline__ void pull_groups(const TraceArgs &a, const Ball &ball
   if (q.i < nf) {
     const int c = lp_i(far + q.i);
     const unsigned long long w = lp_u64(&a.rec[c].word);
-    const unsigned mc = rec_meta(a.rec, c) & kNbrBits;
+    const unsigned mc = in_mask(a, c);
     claimed = w != kMaxKey;
     if (!claimed) {
       int x, y, z;
       unravel(c, a.d, x, y, z);
-      pull_span(a, ball, sh.off, c, mc, q.kb, q.cnt, x, y, z, best);
+      for (int o = 0; o < q.cnt; o += kSpan) pull_span(a, ball, sh.off, c, mc, q.kb + o, min(kSpan, q.cnt - o), x, y, z, best);
     }
   }
   for (int o = 1; o < q.G; o <<= 1) {
@@ -870,7 +881,7 @@ __device__ __forceinline__ bool invalidate(const TraceArgs &a, const Ball &ball,
       for (int i = threadIdx.x; i < nf; i += blockDim.x) {
         int c = lp_i(far + i);
         if (lp_u64(&a.rec[c].word) != kMaxKey) { g_st(pull + i, kClaimed); continue; }
-        const unsigned mc = (rec_meta(a.rec, c) & kNbrBits);
+        const unsigned mc = in_mask(a, c);
         int x, y, z;
         unravel(c, a.d, x, y, z);
         unsigned long long best = kMaxKey;

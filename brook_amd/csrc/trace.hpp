@@ -51,6 +51,10 @@ struct TraceArgs {
   const long long *root = nullptr;  // per label (both modes)
   // Somata, per label (null: none in the sample).
   const struct SomaLabel *soma = nullptr;
+  // With a voxel graph, the pull masks (bit k of v: neighbour v + N[k] is of v's id and its word
+  // permits the step to v) for backtraces, band-boundary pulls and parents; the records' masks are
+  // then the push masks. Null: no graph (pulls read the records' masks).
+  const unsigned *inmask = nullptr;
 };
 
 // A label's soma parameters: the trimming radius (< 0: not a soma), the soma's own ball and the

@@ -151,6 +151,20 @@ brook_status brook_connected_components_noalloc(brook_context *context, const br
 brook_status brook_edt(brook_context *context, const brook_volume *labels, const float anisotropy[3], int black_border,
                        brook_array **out);
 
+/* brook.h:192-196: components of a voxel graph (cc3d convention: bit b of a voxel's word permits
+ * one of its 26 steps; uint32 words, or uint8 for one XY plane) whatever the labels, numbered in
+ * order of each component's lowest voxel, background included; ids only on foreground voxels;
+ * mapping[id] = the label where the id's last run starts; roots[id - 1] = its lowest foreground
+ * voxel. brook_graph_edt: the graph-aware EDT (a blocked step is an interface half a voxel away)
+ * of unsigned labels. */
+brook_status brook_graph_components(brook_context *context, const brook_volume *labels, const brook_volume *voxel_graph,
+                                    brook_components **out);
+brook_status brook_graph_edt(brook_context *context, const brook_volume *labels, const brook_volume *voxel_graph,
+                             const float anisotropy[3], int black_border, brook_array **out);
+/* EXTENSION: the ids Kimimaro gives voxel-graph components (keys[id] = rank of the id's component
+ * among all of them, 1-based; keys[0] = 0), count + 1 entries; NULL and 0 for plain components. */
+brook_status brook_components_keys(const brook_components *components, const int64_t **keys, size_t *count);
+
 /* brook.h:198-201 */
 void brook_skeletonize_options_init(brook_skeletonize_options *options);
 brook_status brook_skeletonize(brook_context *context, const brook_volume *labels,
@@ -193,6 +207,11 @@ brook_status brook_merge_views(brook_context *context, const brook_skeletons_vie
  * brook_skeletonize of that volume. */
 brook_status brook_skeletonize_batch(brook_context *context, const brook_volume *const *volumes, size_t count,
                                      const brook_skeletonize_options *options, brook_skeletons **out);
+/* EXTENSION: brook_skeletonize_batch with one voxel graph per sample (graphs[i] NULL: none; graphs
+ * NULL: none at all); options->voxel_graph must be NULL. */
+brook_status brook_skeletonize_batch_graphs(brook_context *context, const brook_volume *const *volumes,
+                                            const brook_volume *const *graphs, size_t count,
+                                            const brook_skeletonize_options *options, brook_skeletons **out);
 
 /* EXTENSION: the preamble's fields for stage-level checks: DBF, DAF and PDRF (before the roots
  * become rails) as n-voxel float32 host arrays (any may be NULL), and per-component tables of

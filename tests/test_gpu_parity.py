@@ -257,7 +257,7 @@ def test_unsupported_options_raise(brook):
     labels = np.zeros((20, 20, 20), np.uint32)
     labels[5:15, 5:15, 5:15] = 1
     with pytest.raises(NotImplementedError):
-        brook.skeletonize(labels, voxel_graph=np.zeros(labels.shape, np.uint32), dust_threshold=0, progress=False)
+        brook.skeletonize(labels, streaming=True, dust_threshold=0, progress=False)
     with pytest.raises(ValueError):
         brook.skeletonize(labels, dict(pdrf_exponent=3), dust_threshold=0, progress=False)
 

@@ -170,3 +170,173 @@ def test_oracle_painted_root_with_graph_equals_kimimaro(fix_branching):
                                           voxel_graph=graph))
         assert list(got) == [2, 1] and all(v[0] - v[1] == 1 for v in got.values())
         assert {label: v[2] for label, v in got.items()} == graphs.PAINTED_ROOT_GRAPH_RADIUS
+
+
+# ---- GPU against the oracle -------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def brook():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import brook_amd
+    return brook_amd
+
+
+def _dev(a):
+    import torch
+    a = np.asarray(a)
+    t = torch.from_numpy(np.ravel(a, order="F").copy()).cuda()
+    return t.view(a.shape[2], a.shape[1], a.shape[0]).permute(2, 1, 0)
+
+
+def _host(t):
+    return t.permute(2, 1, 0).contiguous().cpu().numpy().transpose(2, 1, 0)
+
+
+def _assert_packed_equal(got, want, what=""):
+    g = got.as_dict() if hasattr(got, "as_dict") else got
+    for name in ("labels", "v_off", "e_off", "vertices", "edges", "radii"):
+        assert np.array_equal(np.asarray(g[name]), np.asarray(want[name])), (what, name)
+
+
+def _random_graph_case(seed):
+    rng = np.random.default_rng(seed)
+    shape = tuple(int(x) for x in rng.integers(8, 30, 3))
+    labels = (rng.integers(0, 5, shape) * (rng.random(shape) < 0.55)).astype(np.uint32)
+    graph = (rng.integers(0, 1 << 26, shape) & rng.integers(0, 1 << 26, shape)).astype(np.uint32)
+    return np.asfortranarray(labels), np.asfortranarray(graph)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", range(4))
+def test_gpu_graph_components_equal_oracle(brook, seed):
+    from brook_amd import device
+    labels, graph = _random_graph_case(seed)
+    for lab, g in ((labels, graph), (labels, _full_graph(labels)), (labels.astype(np.uint16), graph)):
+        ids, mapping, roots, keys = device.graph_components(lab, g)
+        cc, m, r, k, _ = oracle.graph_ccl(lab, g)
+        assert np.array_equal(_host(ids).astype(np.uint32), cc)
+        assert np.array_equal(mapping, m) and np.array_equal(roots, r) and np.array_equal(keys, k)
+    plane = (np.random.default_rng(seed).integers(0, 3, (33, 27, 1))).astype(np.uint32)
+    words = np.random.default_rng(seed + 9).integers(0, 256, (33, 27, 1)).astype(np.uint8)
+    ids, mapping, roots, keys = device.graph_components(plane, words)  # a uint8 plane graph
+    cc, m, r, k, _ = oracle.graph_ccl(plane, words)
+    assert np.array_equal(_host(ids).astype(np.uint32), cc) and np.array_equal(keys, k)
+
+
+@pytest.mark.gpu
+def test_gpu_graph_known_answer(brook):
+    from brook_amd import device
+    labels = np.array([1, 1, 0, 0, 0, 0, 2, 2], np.uint8)[:, None, None]
+    ids, mapping, roots, keys = device.graph_components(labels, np.zeros(labels.shape, np.uint32))
+    assert _host(ids)[:, 0, 0].tolist() == [1, 2, 0, 0, 0, 0, 3, 4]
+    assert mapping.tolist() == [0, 1, 1, 2, 2] and roots.tolist() == [0, 1, 6, 7]
+    d = device.graph_edt(labels, np.zeros(labels.shape, np.uint32))
+    assert _host(d)[:, 0, 0].tolist() == [0.5, 0.5, 0, 0, 0, 0, 0.5, 0.5]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed,black_border", [(0, False), (1, True), (2, False), (3, True)])
+def test_gpu_graph_edt_equals_oracle(brook, seed, black_border):
+    from brook_amd import device
+    labels, graph = _random_graph_case(10 + seed)
+    aniso = [(1., 1., 1.), (4., 4., 40.), (1., 1.13, 1.37), (16., 16., 40.)][seed]
+    got = _host(device.graph_edt(labels, graph, aniso, black_border))
+    want = oracle.graph_edt(labels, graph, aniso, black_border)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fix_branching", [True, False])
+def test_gpu_graph_cases_equal_oracle(brook, fix_branching):
+    for i, (labels, side, region) in enumerate(graphs.graph_cases()):
+        graph = np.asfortranarray(graphs.voxel_graph(labels, side, region))
+        labels = np.asfortranarray(labels)
+        opts = dict(graphs.GRAPH_OPTIONS, fix_branching=fix_branching)
+        want = oracle.skeletonize(labels, opts["teasar_params"], dust_threshold=opts["dust_threshold"],
+                                  fix_branching=fix_branching, voxel_graph=graph)
+        got = brook.skeletonize(labels, voxel_graph=graph, output="packed", progress=False, **opts)
+        _assert_packed_equal(got, want, f"case {i}")
+        device_got = brook.skeletonize(_dev(labels), voxel_graph=_dev(graph), output="packed", progress=False, **opts)
+        _assert_packed_equal(device_got, want, f"case {i} device")
+        plain = oracle.skeletonize(labels, opts["teasar_params"], dust_threshold=opts["dust_threshold"],
+                                   fix_branching=fix_branching)
+        if i == 0:  # the cut changes the skeleton
+            assert not np.array_equal(plain["vertices"], want["vertices"]) or len(plain["labels"]) != len(want["labels"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fix_branching", [True, False])
+def test_gpu_kimimaro_graph_answers(brook, fix_branching):
+    labels, graph = graphs.spanning_labels()
+    for extra in (dict(), dict(fill_holes=True, fix_avocados=True)):
+        for lab, g in ((labels, graph), (np.ascontiguousarray(labels), np.ascontiguousarray(graph)),
+                       (_dev(labels), _dev(graph)), (_dev(labels), graph)):
+            got = brook.skeletonize(lab, graphs.HOLE_PARAMS, dust_threshold=10, fix_branching=fix_branching,
+                                    voxel_graph=g, output="packed", progress=False, **extra)
+            assert _summary(got.as_dict()) == graphs.SPANNING
+    tube, gs = graphs.cut_tube_graphs()
+    for name, g in gs.items():
+        for i, aniso in enumerate([(1., 1., 1.), (1., 1.13, 1.37)]):
+            got = _summary(brook.skeletonize(tube, graphs.HOLE_PARAMS, aniso, dust_threshold=10, fix_branching=fix_branching,
+                                             fill_holes=True, voxel_graph=g, output="packed", progress=False).as_dict())
+            assert list(got) == [5] and got[5][0] - got[5][1] == 2 and got[5][2] == graphs.CUT_TUBE_RADIUS[name][i]
+            if not fix_branching and (name, aniso) in graphs.CUT_TUBE_DIGESTS:
+                assert got[5] == graphs.CUT_TUBE_DIGESTS[name, aniso]
+    av = graphs.avocado(graphs.AVOCADO_JITTER)
+    ag = np.asfortranarray(graphs.label_graph(av, np.zeros(av.shape, bool)))
+    params = dict(graphs.HOLE_PARAMS, soma_detection_threshold=10, soma_acceptance_threshold=11)
+    if not fix_branching:
+        got = brook.skeletonize(av, params, dust_threshold=10, fix_branching=False, voxel_graph=ag, output="packed",
+                                progress=False)
+        assert _summary(got.as_dict()) == graphs.AVOCADO_PLAIN
+    got = _summary(brook.skeletonize(av, params, dust_threshold=10, fix_branching=fix_branching, voxel_graph=ag,
+                                     fix_avocados=True, output="packed", progress=False).as_dict())
+    assert list(got) == [3] and got[3][0] - got[3][1] == 1 and got[3][2] == 9.5
+    if not fix_branching:
+        assert got == {3: graphs.AVOCADO_PLAIN[3]}
+    lab, og = graphs.avocado_order()
+    params = dict(graphs.HOLE_PARAMS, soma_detection_threshold=10, soma_acceptance_threshold=1000)
+    got = _summary(brook.skeletonize(lab, params, (1., 1.13, 1.37), dust_threshold=10, fix_branching=fix_branching,
+                                     voxel_graph=og, fix_avocados=True, output="packed", progress=False).as_dict())
+    if fix_branching:
+        assert list(got) == [3] and got[3][2] == graphs.AVOCADO_ORDER_BRANCHED_RADIUS
+    else:
+        assert got == graphs.AVOCADO_ORDER_CORRECTED
+    sv, sg = graphs.soma_voids()
+    params = dict(graphs.HOLE_PARAMS, soma_detection_threshold=3, soma_acceptance_threshold=4)
+    for aniso, radius in graphs.SOMA_VOID_RADIUS.items():
+        got = _summary(brook.skeletonize(sv, params, aniso, dust_threshold=10, fix_branching=fix_branching,
+                                         voxel_graph=sg, output="packed", progress=False).as_dict())
+        assert list(got) == [1] and got[1][0] - got[1][1] == 1 and got[1][2] == radius
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fix_branching", [True, False])
+def test_gpu_batch_with_graphs_equals_separate_calls(brook, fix_branching):
+    cases = graphs.graph_cases()
+    samples = [np.asfortranarray(c[0]) for c in cases]
+    gs = [np.asfortranarray(graphs.voxel_graph(*c)) for c in cases]
+    opts = dict(graphs.GRAPH_OPTIONS, fix_branching=fix_branching)
+    single = [brook.skeletonize(s, voxel_graph=g, output="packed", progress=False, **opts) for s, g in zip(samples, gs)]
+    batch = brook.skeletonize_batch(samples, voxel_graph=gs, output="packed", **opts)
+    for i in range(len(samples)):
+        _assert_packed_equal(batch[i], single[i].as_dict(), f"sample {i}")
+    mixed = brook.skeletonize_batch(samples, voxel_graph=[gs[0], None, gs[2]], output="packed", **opts)
+    plain = brook.skeletonize(samples[1], output="packed", progress=False, **opts)
+    _assert_packed_equal(mixed[1], plain.as_dict(), "no graph")
+    _assert_packed_equal(mixed[2], single[2].as_dict(), "graph after a sample without one")
+    with pytest.raises(ValueError):
+        brook.skeletonize_batch(samples, voxel_graph=gs[:2], **opts)
+
+
+@pytest.mark.gpu
+def test_gpu_graph_argument_errors(brook):
+    labels, graph = graphs.spanning_labels()
+    with pytest.raises(TypeError):  # a device graph needs device labels
+        brook.skeletonize(labels, voxel_graph=_dev(graph), dust_threshold=10, progress=False)
+    with pytest.raises(ValueError):  # shape
+        brook.skeletonize(labels, voxel_graph=graph[:-1], dust_threshold=10, progress=False)
+    with pytest.raises(ValueError):  # a uint8 graph of a volume, not a plane
+        brook.skeletonize(labels, voxel_graph=graph.astype(np.uint8), dust_threshold=10, progress=False)
