@@ -275,14 +275,13 @@ template <class T> __device__ __forceinline__ T shfl_up_any(T v, unsigned delta)
   else return T(__shfl_up((unsigned)v, delta, 64));
 }
 
-// A group of lanes per front entry, for fronts of at most half the workgroup: each entry gets G
-// lanes (the largest power of two up to 32 that fits every entry into the workgroup, at least 2);
-// lane g of a group handles neighbours [g*S, g*S + S), S = ceil(26 / G). From G = 4 on, S <= kSpan
-// and all of an entry's loads, then all of its atomics, are in flight together instead of part
-// after part; at G = 2 a lane takes its 13 neighbours in two spans. One code path serves every G
-// (a span is kSpan slots, the tail predicated off), which keeps a kernel's register pressure at
-// the level of its one-thread-per-entry path. Callers with fronts of at most a quarter workgroup
-// always get G >= 4 (one span per lane).
+// A group of lanes per front entry, for fronts of at most a quarter of the workgroup: each entry
+// gets G lanes (the largest power of two up to 32 that fits every entry into the workgroup, at
+// least 4); lane g of a group handles neighbours [g*S, g*S + S), S = ceil(26 / G) <= kSpan, so all
+// of an entry's loads, then all of its atomics, are in flight together instead of part after part.
+// One code path serves every G (a span is kSpan slots, the tail predicated off), which keeps a
+// kernel's register pressure at the level of its one-thread-per-entry path. (Pairs of lanes for
+// fronts up to half a workgroup, two spans each, measured no faster on the 512^3 workload.)
 constexpr int kSpan = 7;
 struct Group {
   int G, i, g, lead, kb, cnt;  // lanes per entry, entry, lane in group, group's first lane, span
@@ -292,7 +291,7 @@ struct Group {
 __host__ __device__ constexpr Group group_at(int n, int block, int tid) {
   Group q{};
   q.G = 32;
-  while (q.G > 2 && n * q.G > block) q.G >>= 1;
+  while (q.G > 4 && n * q.G > block) q.G >>= 1;
   int shift = 0;
   while ((1 << shift) < q.G) ++shift;
   const int span = (26 + q.G - 1) / q.G;

@@ -226,7 +226,7 @@ __device__ __forceinline__ float dequeue(const TraceArgs &a, int u) {
 }
 
 // ---- fronts of up to a quarter workgroup: a group of lanes per entry (group_of) ----------------
-constexpr int kGroupFront = kBlock / 2;
+constexpr int kGroupFront = kBlock / 4;
 __device__ __forceinline__ Group group_of(int n) { return brook::group_of(n, kBlock); }
 
 // Neighbours kb..kb+cnt-1 (those set in m; cnt <= kSpan) of u: relax_part with a run-time span.
@@ -295,8 +295,7 @@ __device__ __forceinline__ void relax_groups(const TraceArgs &a, Shared &sh, Roa
   m = __shfl(m, q.lead, 64);
   du = __shfl(du, q.lead, 64);
   Pushes p;
-  if (q.i < nc && du <= bound)
-    for (int o = 0; o < q.cnt; o += kSpan) relax_span(a, mins, u, m, q.kb + o, min(kSpan, q.cnt - o), du, bound, sh.off, p);
+  if (q.i < nc && du <= bound) relax_span(a, mins, u, m, q.kb, q.cnt, du, bound, sh.off, p);
   emit_neighbours(touched, &sh.n_touched, p.touched, u, sh.off);
   emit_front(nxt, n_nxt, p.near, u, sh.off);
   emit_capped(far, &sh.n_far, p.far, u, sh.off, sh.far_cap, &sh.overflow);
@@ -699,8 +698,7 @@ __device__ __forceinline__ void offer_groups(const TraceArgs &a, const Ball &bal
     int x, y, z, qx, qy, qz;
     unravel(u, a.d, x, y, z);
     unravel(int(su), a.d, qx, qy, qz);
-    for (int o = 0; o < q.cnt; o += kSpan)
-      offer_span(a, sh.off, u, m, q.kb + o, min(kSpan, q.cnt - o), x, y, z, qx, qy, qz, su, mu, maxd, hi, p);
+    offer_span(a, sh.off, u, m, q.kb, q.cnt, x, y, z, qx, qy, qz, su, mu, maxd, hi, p);
   }
   emit_neighbours(touched, &sh.n_touched, p.touched, u, sh.off);
   emit_front(nxt, n_nxt, p.near, u, sh.off);
@@ -791,7 +789,7 @@ __device__ __forceinline__ void pull_groups(const TraceArgs &a, const Ball &ball
     if (!claimed) {
       int x, y, z;
       unravel(c, a.d, x, y, z);
-      for (int o = 0; o < q.cnt; o += kSpan) pull_span(a, ball, sh.off, c, mc, q.kb + o, min(kSpan, q.cnt - o), x, y, z, best);
+      pull_span(a, ball, sh.off, c, mc, q.kb, q.cnt, x, y, z, best);
     }
   }
   for (int o = 1; o < q.G; o <<= 1) {

@@ -1,10 +1,9 @@
 // CPU check of the tracer's lane groups (common.hpp group_at, the layout behind group_of): for
-// every workgroup size the kernels use and every front length a group round takes (1 .. block/2),
+// every workgroup size the kernels use and every front length a group round takes (1 .. block/4),
 // each entry is served by exactly one group whose lanes lie in one wave, whose first lane (the one
 // that dequeues the entry and broadcasts it) has g == 0, and whose spans cover neighbours 0..25
-// exactly once with at most kSpan each up to a quarter workgroup (two spans per lane beyond); no
-// thread of an entry index >= n touches the front, and every index a thread reads is below n.
-// Prints "ok" or the first violation.
+// exactly once with at most kSpan each; no thread of an entry index >= n touches the front, and
+// every index a thread reads is below n. Prints "ok" or the first violation.
 #include <cstdio>
 #include <vector>
 
@@ -16,11 +15,11 @@ using brook::kSpan;
 
 int main() {
   for (int block : {256, 512, 1024}) {
-    for (int n = 1; n <= block / 2; ++n) {
+    for (int n = 1; n <= block / 4; ++n) {
       std::vector<int> cover(size_t(n) * 26, 0);
       for (int tid = 0; tid < block; ++tid) {
         const Group q = group_at(n, block, tid);
-        if (q.G < (n <= block / 4 ? 4 : 2) || q.G > 32 || (q.G & (q.G - 1)) || n * q.G > block) {
+        if (q.G < 4 || q.G > 32 || (q.G & (q.G - 1)) || n * q.G > block) {
           std::printf("block %d n %d tid %d: lanes per entry %d\n", block, n, tid, q.G);
           return 1;
         }
@@ -31,7 +30,7 @@ int main() {
           std::printf("block %d n %d tid %d: group lead lane %d outside the entry's wave\n", block, n, tid, q.lead);
           return 1;
         }
-        if (q.cnt > (n <= block / 4 ? kSpan : 2 * kSpan) || q.kb < 0) {
+        if (q.cnt > kSpan || q.kb < 0) {
           std::printf("block %d n %d tid %d: span %d from %d\n", block, n, tid, q.cnt, q.kb);
           return 1;
         }
