@@ -774,6 +774,20 @@ struct Params {
   // as they are) and whether the input was such a plane graph (its CCL reads the corner bits moved)
   const uint32_t *graph = nullptr;
   bool graph_compact_2d = false;
+  // A soma's private arena (soma.cu:54-72, 141-237): the run on its filled box takes the label's
+  // border targets as given (the last one the root) and places vertices at origin + voxel.
+  const std::vector<std::array<i64, 3>> *given_borders = nullptr;
+  i64 origin[3] = {0, 0, 0};
+  bool arena = false;  // the input is one filled component: no graph components, plain black-border rule
+};
+
+// A soma whose filled box takes part of another traced label is traced in a private arena: its
+// filled box alone, while the main volume keeps every other label's voxels.
+struct Arena {
+  uint32_t id = 0;
+  i64 lo[3] = {0, 0, 0};
+  Shape c{0, 0, 0};
+  std::vector<i64> labels;  // 1 on the filled soma, Fortran order over c
 };
 
 struct Prepared {
@@ -785,6 +799,7 @@ struct Prepared {
   std::vector<float> dbf_max, max_daf;
   std::vector<i64> root, target;
   std::vector<std::vector<std::array<i64, 3>>> borders;
+  std::vector<Arena> arenas;
 };
 
 // Per-label argmax, batched_regular.inc:9-22: largest finite value, ties -> lowest flat index.
@@ -832,7 +847,7 @@ Prepared prepare(const i64 *labels, const Shape &s, const Params &p, int threads
   const Gate gate{p.graph};
   std::vector<i64> keys;  // Kimimaro's ids of the graph components (avocado order)
   bool uniform = false;
-  if (p.graph) {
+  if (p.graph && !p.arena) {
     GraphComponents g = graph_ccl(labels, p.graph, p.graph_compact_2d, s);
     keys = std::move(g.keys);
     uniform = g.input_uniform;
@@ -844,7 +859,7 @@ Prepared prepare(const i64 *labels, const Shape &s, const Params &p, int threads
   size_t k = count + 1;
   out.boxes = analyze(out.comp.cc, s, count);
   // of the volume before any fill; with a graph, every voxel of the input has one label
-  bool black_border = p.graph ? uniform : count == 1 && out.boxes[1].count == s.n();
+  bool black_border = p.graph && !p.arena ? uniform : count == 1 && out.boxes[1].count == s.n();
   auto field = [&](const std::vector<uint32_t> &lab, std::vector<float> &dist) {
     if (p.graph) graph_edt<uint32_t>(lab.data(), p.graph, s, p.aniso, black_border, dist.data());
     else edt_rule<uint32_t>(lab.data(), s, p.aniso, black_border, 3, dist.data());
@@ -901,6 +916,7 @@ Prepared prepare(const i64 *labels, const Shape &s, const Params &p, int threads
   }
   out.borders.assign(k, {});
   if (p.fix_borders) out.borders = border_targets(out.comp.cc, s, count, p.aniso);
+  if (p.given_borders && count == 1) out.borders[1] = *p.given_borders;
   std::vector<i64> unused;
   seg_argmax(out.dbf, out.comp.cc, k, out.dbf_max, unused);
   out.active.assign(k, 0);
@@ -925,10 +941,29 @@ Prepared prepare(const i64 *labels, const Shape &s, const Params &p, int threads
     if (!voids.empty()) {
       std::vector<i64> taken(k, 0);
       for (i64 v : voids) if (out.comp.cc[v]) taken[out.comp.cc[v]]++;
+      auto owner = [&](size_t o) { return taken[o] && double(out.boxes[o].count) > p.dust && out.active[o] && !evicted[o]; };
+      bool partial = false;
+      for (size_t o = 1; o < k; ++o) partial |= owner(o) && taken[o] != out.boxes[o].count;
+      if (partial) {  // soma.cu:207-215: the label goes to a private arena; nothing is evicted or painted
+        Arena a;
+        a.id = uint32_t(id);
+        for (int ax = 0; ax < 3; ++ax) a.lo[ax] = b.lo[ax];
+        a.c = Shape{b.hi[0] - b.lo[0] + 1, b.hi[1] - b.lo[1] + 1, b.hi[2] - b.lo[2] + 1};
+        a.labels.assign(size_t(a.c.n()), 0);
+        for (i64 z = 0; z < a.c.sz; ++z)
+          for (i64 y = 0; y < a.c.sy; ++y)
+            for (i64 x = 0; x < a.c.sx; ++x)
+              a.labels[a.c.at(x, y, z)] = out.comp.cc[s.at(x + b.lo[0], y + b.lo[1], z + b.lo[2])] == id;
+        for (i64 v : voids) {
+          i64 x, y, z; s.unravel(v, x, y, z);
+          a.labels[a.c.at(x - b.lo[0], y - b.lo[1], z - b.lo[2])] = 1;
+        }
+        out.arenas.push_back(std::move(a));
+        out.active[id] = 0;
+        continue;
+      }
       for (size_t o = 1; o < k; ++o) {
-        if (!taken[o] || !(double(out.boxes[o].count) > p.dust) || !out.active[o] || evicted[o]) continue;
-        if (taken[o] != out.boxes[o].count)
-          throw std::runtime_error("a soma's voids hold part of another traced label (soma.cu private arenas): not restated");
+        if (!owner(o)) continue;
         evicted[o] = 1;
         out.active[o] = 0;
       }
@@ -1467,7 +1502,7 @@ Result skeletonize(const i64 *labels, const Shape &s, const Params &p, const std
       size_t base = sk.vertices.size();
       for (size_t j = 0; j < rec.path.size(); ++j) {
         i64 x, y, z; s.unravel(rec.path[j], x, y, z);
-        sk.vertices.push_back({float(x), float(y), float(z)});
+        sk.vertices.push_back({float(x + p.origin[0]), float(y + p.origin[1]), float(z + p.origin[2])});
         sk.radii.push_back(pr.dbf[rec.path[j]]);
         if (j) sk.edges.push_back({uint32_t(base + j - 1), uint32_t(base + j)});
       }
@@ -1475,6 +1510,35 @@ Result skeletonize(const i64 *labels, const Shape &s, const Params &p, const std
     consolidate(sk);
     for (auto &v : sk.vertices) for (int a = 0; a < 3; ++a) v[a] *= p.aniso[a];
     res.invalidated += inval_counts[id];
+  }
+  // private arenas (soma.cu:54-139): each soma runs alone on its filled box, with its border and
+  // manual targets moved into the box, its vertices placed back at the box's origin
+  for (const Arena &a : pr.arenas) {
+    Params q = p;
+    q.fix_borders = false;
+    q.fill_holes = false;
+    q.fix_avocados = false;
+    auto shift = [&](std::vector<std::array<i64, 3>> pts) {
+      for (auto &pt : pts) for (int ax = 0; ax < 3; ++ax) pt[ax] -= a.lo[ax];
+      return pts;
+    };
+    const std::vector<std::array<i64, 3>> borders = shift(pr.borders[a.id]);
+    q.given_borders = &borders;
+    q.arena = true;
+    for (int ax = 0; ax < 3; ++ax) q.origin[ax] = p.origin[ax] + a.lo[ax];
+    std::vector<uint32_t> graph_crop;
+    if (p.graph) {
+      graph_crop.resize(size_t(a.c.n()));
+      for (i64 z = 0; z < a.c.sz; ++z)
+        for (i64 y = 0; y < a.c.sy; ++y)
+          for (i64 x = 0; x < a.c.sx; ++x) graph_crop[a.c.at(x, y, z)] = p.graph[s.at(x + a.lo[0], y + a.lo[1], z + a.lo[2])];
+      q.graph = graph_crop.data();
+      q.graph_compact_2d = false;
+    }
+    Result r = skeletonize(a.labels.data(), a.c, q, shift(extra[a.id]), shift(extra_after[a.id]), threads);
+    res.paths += r.paths;
+    res.invalidated += r.invalidated;
+    if (!r.skeletons.empty()) per_cc[a.id] = std::move(r.skeletons[0]);
   }
   std::unordered_map<i64, size_t> index;
   for (size_t id = 1; id < k; ++id) {

@@ -202,9 +202,41 @@ def test_oracle_soma_voids_equal_a_prefilled_volume(fix_branching, nucleus):
     assert p["soma"][1] == 1 and (not nucleus or p["active"][2] == 0)
 
 
-def test_oracle_soma_voids_errors():
-    with pytest.raises(RuntimeError, match="private arenas"):
-        oracle.skeletonize(shapes.soma_cavity(staircase=True), P, (1, 1, 1), 0, fix_borders=False)
+def _label(packed, label):
+    k = packed["labels"].tolist().index(label)
+    v0, v1, e0, e1 = packed["v_off"][k], packed["v_off"][k + 1], packed["e_off"][k], packed["e_off"][k + 1]
+    return packed["vertices"][v0:v1], packed["edges"][e0:e1], packed["radii"][v0:v1]
+
+
+def _same_label(got, want, label):
+    for a, b, name in zip(_label(got, label), _label(want, label), ("vertices", "edges", "radii")):
+        assert np.array_equal(a, b), (label, name)
+
+
+@pytest.mark.parametrize("fix_branching", [True, False])
+def test_oracle_soma_private_arena(fix_branching):
+    """A staircase (label 3) runs from the soma's cavity through its wall to the outside, so filling
+    the soma's voids would take part of it: the soma is traced in a private arena (soma.cu:207-215),
+    its filled box alone, and the staircase keeps every voxel. The box is the whole volume here, so
+    the soma's skeleton is that of the filled soma alone, and the staircase's is the one it has when
+    no soma is detected at all (labels never interact)."""
+    labels = shapes.soma_cavity(staircase=True)
+    opts = dict(fix_branching=fix_branching, fix_borders=False)
+    got = oracle.skeletonize(labels, P, (1, 1, 1), 0, **opts)
+    assert got["labels"].tolist() == [1, 3]
+    alone = _prefilled(np.where(labels == 1, 1, 0).astype(np.uint8))
+    _same_label(got, oracle.skeletonize(alone, P, (1, 1, 1), 0, **opts), 1)
+    no_soma = dict(P, soma_detection_threshold=1e9, soma_acceptance_threshold=1e9)
+    _same_label(got, oracle.skeletonize(labels, no_soma, (1, 1, 1), 0, **opts), 3)
+    # the arena depends on its box alone: padded with 5 background voxels and a far label, the
+    # soma's skeleton moves by 5 voxels, bit for bit
+    padded = np.pad(labels, ((5, 9), (5, 3), (5, 6)))
+    padded[-3:, -2:, -4:] = 7
+    moved = oracle.skeletonize(np.asfortranarray(padded), P, (1, 1, 1), 0, **opts)
+    assert moved["labels"].tolist() == [1, 3, 7]
+    v, e, r = _label(moved, 1)
+    v0, e0, r0 = _label(got, 1)
+    assert np.array_equal(v, v0 + np.float32(5)) and np.array_equal(e, e0) and np.array_equal(r, r0)
     r = oracle.skeletonize(shapes.soma_with_void(), P, (1, 1, 1), 0, fix_borders=False)
     assert r["labels"].tolist() == [1] and len(r["edges"]) > 0
 
