@@ -94,15 +94,15 @@ __global__ void group_offsets(const unsigned long long *verts, int nverts, const
 }
 
 __global__ void emit_vertices(const unsigned long long *verts, int nverts, Dims d, const float *dbf, float ax, float ay,
-                              float az, float *vout, float *rout) {
+                              float az, int ox, int oy, int oz, float *vout, float *rout) {
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= nverts) return;
   long long c = (long long)(verts[i] & ((1ULL << kGroupShift) - 1));
   long long syz = (long long)d.sy * d.sz;
   int x = int(c / syz), y = int((c / d.sz) % d.sy), z = int(c % d.sz);
-  vout[3LL * i] = float(x) * ax;
-  vout[3LL * i + 1] = float(y) * ay;
-  vout[3LL * i + 2] = float(z) * az;
+  vout[3LL * i] = float(x + ox) * ax;
+  vout[3LL * i + 1] = float(y + oy) * ay;
+  vout[3LL * i + 2] = float(z + oz) * az;
   rout[i] = dbf[flat(x, y, z, d)];
 }
 
@@ -174,7 +174,7 @@ Packed run_assembly(Context &ctx, const AssemblyInput &in) {
   Buffer<float> verts(ctx, 3 * size_t(std::max(nv, 1))), radii(ctx, std::max(nv, 1));
   Buffer<unsigned> edges(ctx, 2 * size_t(std::max(ne, 1)));
   emit_vertices<<<g256(nv), 256, 0, ctx.stream()>>>(vkeys.data(), nv, in.d, in.dbf, in.aniso[0], in.aniso[1], in.aniso[2],
-                                                    verts.data(), radii.data());
+                                                    in.origin[0], in.origin[1], in.origin[2], verts.data(), radii.data());
   BROOK_LAUNCHED();
   emit_edges<<<g256(ne), 256, 0, ctx.stream()>>>(ekeys.data(), ne, vkeys.data(), voff.data(), edges.data());
   BROOK_LAUNCHED();

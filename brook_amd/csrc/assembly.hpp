@@ -30,6 +30,7 @@ struct AssemblyInput {
   std::vector<int> group_of;            // per component id: output group, -1 none
   std::vector<int64_t> group_labels;    // per group: input label
   float aniso[3] = {1, 1, 1};
+  int origin[3] = {0, 0, 0};  // added to the voxel coordinates before the anisotropy (a private arena's box)
 };
 
 Packed assemble(Context &ctx, int cc_bytes, const AssemblyInput &in);

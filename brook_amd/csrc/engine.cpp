@@ -1,6 +1,7 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -80,11 +81,33 @@ std::vector<float> step_weights(const float aniso[3]) {
 // PDRF and target order, trace, assembly. Device buffers live as long as the object.
 class Sample {
  public:
-  Sample(Context &ctx, const LabelVolume &in, const SkeletonizeOptions &o, FieldsOut *fields)
-      : ctx_(ctx), in_(in), o_(o), fields_(fields), d_(in.dims), n_(in.size()) {}
+  // A soma whose voids hold part of another traced component is traced in a private arena
+  // (soma.cu:54-72, 141-237): its filled box as a volume of its own, whose one component is the
+  // filled soma, with the label's border targets given (the last one the root) and its vertices
+  // placed at the box's corner. Nothing of the main volume is painted or evicted for it.
+  struct Arena {
+    int64_t id = 0;               // the soma's component id in the main volume
+    int origin[3] = {0, 0, 0};    // the box's corner in the input volume
+    Buffer<uint8_t> mask;         // 1 on the filled soma, Fortran order over the box
+    Buffer<unsigned> graph;       // the voxel graph's words of the box (empty: no graph)
+    LabelVolume volume;           // the mask (and graph) as the arena's input
+    SkeletonizeOptions o;         // the main options without border targets, hole filling or avocados
+    std::vector<Point> borders;   // the label's border targets in box coordinates
+    std::unique_ptr<Sample> sample;
+  };
+
+  Sample(Context &ctx, const LabelVolume &in, const SkeletonizeOptions &o, FieldsOut *fields, const Arena *arena = nullptr)
+      : ctx_(ctx), in_(in), o_(o), fields_(fields), d_(in.dims), n_(in.size()), arena_(arena) {
+    if (arena) std::copy(arena->origin, arena->origin + 3, origin_);
+  }
 
   bool done() const { return done_; }
   Packed take_result() { return std::move(result_); }
+  // This sample and its arenas' samples, which share its flood and trace launches.
+  void collect(std::vector<Sample *> &all) {
+    all.push_back(this);
+    for (auto &a : arenas_) a->sample->collect(all);
+  }
   // Rough device bytes a sample holds from its preamble to its assembly (40 B per voxel, 56 B per
   // foreground voxel at the usual 40% occupancy, plus slack).
   static int64_t footprint(int64_t n) { return n * 40 + (n * 2 / 5) * 56 + (int64_t(64) << 20); }
@@ -92,7 +115,7 @@ class Sample {
   void preamble(Lap &lap) {
     if (n_ == 0 || double(n_) <= o_.dust) return finish_empty();
     bool uniform = false;
-    if (in_.graph) {  // components of the graph's steps (skeletonize.cu:76-78)
+    if (in_.graph && !arena_) {  // components of the graph's steps (skeletonize.cu:76-78); an arena is one component
       GraphComponents g = graph_components(ctx_, in_, in_.graph, in_.graph_compact_2d);
       cc_ = std::move(g.cc);
       keys_ = std::move(g.keys);
@@ -107,7 +130,7 @@ class Sample {
     boxes_ = analyze(ctx_, cc_, d_);
     lap("prep_analyze_ms");
     // of the volume before any fill; with a graph, every input voxel has one label (skeletonize.cu:80)
-    const bool black_border = in_.graph ? uniform : count_ == 1 && boxes_[1].count == n_;
+    const bool black_border = in_.graph && !arena_ ? uniform : count_ == 1 && boxes_[1].count == n_;
     if (o_.fill_holes) {
       fill_holes();
       lap("prep_fill_holes_ms");
@@ -130,6 +153,7 @@ class Sample {
     }
     borders_.assign(k_, {});
     if (o_.fix_borders) borders_ = border_targets(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, count_, o_.aniso);
+    if (arena_ && count_ == 1) borders_[1] = arena_->borders;
     lap("prep_borders_ms");
     // Each point is filed under the component its wrapped coordinates fall on and traced at the
     // coordinates as written (skeletonize.cu:59-65). A point whose written coordinates flatten off
@@ -161,9 +185,9 @@ class Sample {
     }
     // Somata (batched.cu:194-205, soma.cu:141-237), in ascending id: a label above the detection
     // threshold whose box encloses voids takes them, evicting every traced label wholly inside
-    // them (one partly inside needs the reference's private arenas: not supported), and its DBF
-    // becomes the EDT of its filled box. A label is a soma when its (refreshed) DBF maximum
-    // exceeds the acceptance threshold.
+    // them, and its DBF becomes the EDT of its filled box; when a traced label lies partly inside
+    // them, the soma goes to a private arena instead and leaves the main volume as it is. A label
+    // is a soma when its (refreshed) DBF maximum exceeds the acceptance threshold.
     soma_.assign(k_, 0);
     bool any_soma = false, any_fill = false;
     std::vector<uint8_t> evicted(k_, 0);
@@ -172,11 +196,18 @@ class Sample {
       if (enclosed_voids(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, id, boxes_[id])) {
         VoidFill fill(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, id, boxes_[id], k_);
         const std::vector<int64_t> &taken = fill.taken();
+        auto owner = [&](size_t o) { return taken[o] && active_[o] && !evicted[o]; };
+        bool partial = false;
+        for (size_t o = 1; o < k_; ++o) partial |= owner(o) && taken[o] != boxes_[o].count;
+        if (partial) {
+          arenas_.push_back(make_arena(fill, id));
+          evicted[id] = 1;
+          active_[id] = 0;
+          any_fill = true;
+          continue;
+        }
         for (size_t o = 1; o < k_; ++o) {
-          if (!taken[o] || !active_[o] || evicted[o]) continue;
-          if (taken[o] != boxes_[o].count)
-            throw Unsupported("a soma's voids hold part of another traced component (the reference's private "
-                              "arenas); not supported by this build");
+          if (!owner(o)) continue;
           evicted[o] = 1;
           active_[o] = 0;
         }
@@ -266,6 +297,32 @@ class Sample {
     // lists 2 and 3 are the railroad's far lists: two entries per voxel (trace.hip far_compact)
     for (int i = 0; i < 6; ++i) lists_.emplace_back(ctx_, size_t(std::max<int64_t>(F_, 1)) * (i == 2 || i == 3 ? 2 : 1));
     if (!o_.fix_branching) parent_ = Buffer<int>(ctx_, size_t(n_));
+    for (auto &a : arenas_) a->sample->preamble(lap);
+  }
+
+  std::unique_ptr<Arena> make_arena(VoidFill &fill, int id) {
+    auto a = std::make_unique<Arena>();
+    a->id = id;
+    const int *lo = fill.lo(), *ext = fill.ext();
+    for (int ax = 0; ax < 3; ++ax) a->origin[ax] = origin_[ax] + lo[ax];
+    a->mask = fill.take_mask();
+    if (in_.graph) a->graph = fill.crop_graph(ctx_, in_.graph);
+    a->volume.data = a->mask.data();
+    a->volume.bytes = 1;
+    a->volume.dims = Dims{ext[0], ext[1], ext[2]};
+    a->volume.graph = in_.graph ? a->graph.data() : nullptr;
+    a->o = o_;
+    a->o.fix_borders = a->o.fill_holes = a->o.fix_avocados = false;
+    auto shift = [&](const std::vector<Point> &points) {
+      std::vector<Point> out = points;
+      for (Point &p : out) for (int ax = 0; ax < 3; ++ax) p[ax] -= lo[ax];
+      return out;
+    };
+    a->o.before = shift(before_[id]);
+    a->o.after = shift(after_[id]);
+    a->borders = shift(borders_[id]);
+    a->sample = std::make_unique<Sample>(ctx_, a->volume, a->o, nullptr, a.get());
+    return a;
   }
 
   // fix_avocados (skeletonize.cu:91-95): the correction runs on host copies of the ids and the DBF
@@ -520,6 +577,7 @@ class Sample {
   // Trace outcome, statistics (per-label profiles only when `profile`) and assembly.
   void finish(Lap &lap, bool profile) {
     if (done_) return;
+    for (auto &a : arenas_) a->sample->finish(lap, false);
     std::vector<int> cnt = counters_.download(ctx_);
     if (cnt[8] & 1) throw std::runtime_error("trace did not converge: a railroad or an invalidation exceeded its round budget");
     if (cnt[8] & 2) throw std::runtime_error("trace far list overflowed its capacity");
@@ -541,6 +599,9 @@ class Sample {
     std::vector<uint8_t> has_edges(k_, 0);
     for (int r = 0; r < records; ++r)
       if (rn[r] >= 2) has_edges[rl[r]] = 1;
+    const std::vector<uint8_t> main_edges = has_edges;
+    for (auto &a : arenas_)
+      if (!a->sample->result_.labels.empty()) has_edges[size_t(a->id)] = 1;
     AssemblyInput ai;
     ai.cc = cc_.labels.data();
     ai.d = d_;
@@ -563,7 +624,9 @@ class Sample {
       ai.group_of[id] = it->second;
     }
     std::copy(o_.aniso, o_.aniso + 3, ai.aniso);
+    std::copy(origin_, origin_ + 3, ai.origin);
     result_ = assemble(ctx_, cc_.cc_bytes, ai);
+    if (!arenas_.empty()) join_arenas(ai.group_of, main_edges, rl, rn, cnt[0]);
     lap("assembly_ms");
     done_ = true;
   }
@@ -580,6 +643,114 @@ class Sample {
   void finish_empty() {
     result_ = empty_result(o_);
     done_ = true;
+    for (auto &a : arenas_) a->sample->finish_empty();
+  }
+
+  // The arenas' skeletons joined to their labels' groups as the final merge joins per-component
+  // skeletons (skeletonize.cu:184-198): concatenated in component id order, vertices made unique by
+  // position with the first one's radius kept, edges sorted and unique. A group's main-volume part
+  // comes merged already; where an arena vertex meets a main vertex, the lower component id (the
+  // main one's looked up in the volume) gives the radius. A main path of one vertex has no edge, yet
+  // its vertex is in that concatenation when its component has edges, so it takes part as well.
+  void join_arenas(const std::vector<int> &group_of, const std::vector<uint8_t> &main_edges, const std::vector<int> &rl,
+                   const std::vector<int> &rn, long long store_used) {
+    struct Vert {
+      std::array<float, 3> p;
+      float r;
+      int64_t owner;  // component id; -1: a main vertex not looked up yet
+    };
+    using Edge = std::array<uint32_t, 2>;
+    Packed &m = result_;
+    const size_t groups = m.labels.size();
+    std::vector<std::vector<Vert>> gv(groups);
+    std::vector<std::vector<Edge>> ge(groups);
+    for (size_t g = 0; g < groups; ++g) {
+      for (long long v = m.v_off[g]; v < m.v_off[g + 1]; ++v)
+        gv[g].push_back(Vert{{m.vertices[3 * v], m.vertices[3 * v + 1], m.vertices[3 * v + 2]}, m.radii[size_t(v)], -1});
+      for (long long e = m.e_off[g]; e < m.e_off[g + 1]; ++e) ge[g].push_back(Edge{m.edges[2 * e], m.edges[2 * e + 1]});
+    }
+    std::vector<uint8_t> arena_group(groups, 0);
+    for (auto &a : arenas_)
+      if (!a->sample->result_.labels.empty()) arena_group[size_t(group_of[size_t(a->id)])] = 1;
+    std::map<int64_t, int64_t> single;  // voxel of a one-vertex main path in an arena's group -> its component
+    const int records = int(rl.size());
+    std::vector<int> starts;
+    std::vector<int> store;
+    for (int r = 0; r < records; ++r) {
+      if (rn[r] != 1 || !main_edges[size_t(rl[r])] || !arena_group[size_t(group_of[size_t(rl[r])])]) continue;
+      if (starts.empty()) {
+        starts = rec_start_.download(ctx_, size_t(records));
+        store = store_.download(ctx_, size_t(store_used));
+      }
+      single[store[size_t(starts[size_t(r)])]] = rl[r];
+    }
+    auto voxel_of = [&](const std::array<float, 3> &p) {
+      Point q;
+      for (int ax = 0; ax < 3; ++ax) q[ax] = std::llround(double(p[ax]) / double(o_.aniso[ax])) - origin_[ax];
+      return q;
+    };
+    auto main_radius = [&](int64_t flat) {
+      float r = 0;
+      BROOK_HIP(hipMemcpyAsync(&r, dbf_.data() + flat, 4, hipMemcpyDeviceToHost, ctx_.stream()));
+      ctx_.synchronize();
+      return r;
+    };
+    for (auto &a : arenas_) {  // ascending id
+      const Packed &ar = a->sample->result_;
+      if (ar.labels.empty()) continue;
+      const size_t g = size_t(group_of[size_t(a->id)]);
+      std::vector<Vert> &cur = gv[g];
+      std::vector<Vert> in;
+      for (long long v = ar.v_off[0]; v < ar.v_off[1]; ++v) {
+        Vert x{{ar.vertices[3 * v], ar.vertices[3 * v + 1], ar.vertices[3 * v + 2]}, ar.radii[size_t(v)], a->id};
+        const int64_t flat = flat_of(voxel_of(x.p), d_);
+        auto it = single.find(flat);
+        if (it != single.end() && it->second < a->id) { x.owner = it->second; x.r = main_radius(flat); }
+        in.push_back(x);
+      }
+      std::vector<Vert> out;
+      std::vector<uint32_t> map_cur(cur.size()), map_in(in.size());
+      size_t i = 0, j = 0;
+      while (i < cur.size() || j < in.size()) {
+        if (j == in.size() || (i < cur.size() && cur[i].p < in[j].p)) {
+          map_cur[i] = uint32_t(out.size());
+          out.push_back(cur[i++]);
+        } else if (i == cur.size() || in[j].p < cur[i].p) {
+          map_in[j] = uint32_t(out.size());
+          out.push_back(in[j++]);
+        } else {
+          Vert &c = cur[i];
+          if (c.owner < 0) c.owner = component_at(ctx_, cc_, d_, voxel_of(c.p));
+          map_cur[i] = map_in[j] = uint32_t(out.size());
+          out.push_back(c.owner < in[j].owner ? c : in[j]);
+          ++i, ++j;
+        }
+      }
+      std::vector<Edge> edges;
+      for (const Edge &e : ge[g]) edges.push_back(Edge{map_cur[e[0]], map_cur[e[1]]});
+      for (long long e = ar.e_off[0]; e < ar.e_off[1]; ++e)
+        edges.push_back(Edge{map_in[ar.edges[2 * e]], map_in[ar.edges[2 * e + 1]]});
+      for (Edge &e : edges)
+        if (e[0] > e[1]) std::swap(e[0], e[1]);
+      std::sort(edges.begin(), edges.end());
+      edges.erase(std::unique(edges.begin(), edges.end()), edges.end());
+      cur.swap(out);
+      ge[g].swap(edges);
+    }
+    m.vertices.clear();
+    m.radii.clear();
+    m.edges.clear();
+    m.v_off.assign(1, 0);
+    m.e_off.assign(1, 0);
+    for (size_t g = 0; g < groups; ++g) {
+      for (const Vert &v : gv[g]) {
+        m.vertices.insert(m.vertices.end(), v.p.begin(), v.p.end());
+        m.radii.push_back(v.r);
+      }
+      for (const Edge &e : ge[g]) m.edges.insert(m.edges.end(), e.begin(), e.end());
+      m.v_off.push_back(m.v_off.back() + (long long)gv[g].size());
+      m.e_off.push_back(m.e_off.back() + (long long)ge[g].size());
+    }
   }
 
   void record_profile() {  // per-label trace profile (device wall clock)
@@ -633,8 +804,11 @@ class Sample {
   FieldsOut *fields_;
   const Dims d_;
   const int64_t n_;
+  const Arena *arena_;        // this sample is that arena (null: an input volume)
+  int origin_[3] = {0, 0, 0};  // the sample's corner in the input volume
   bool done_ = false;
   Packed result_;
+  std::vector<std::unique_ptr<Arena>> arenas_;  // somata traced in private arenas, ascending id
 
   Components cc_;
   std::vector<int64_t> keys_;  // voxel graph: Kimimaro's ids of the components (until fix_avocados)
@@ -671,22 +845,24 @@ bool larger_first(const LabelItem &a, const LabelItem &b) { return a.size > b.si
 // (largest first), every PDRF/order pass, one trace launch, every assembly.
 void run_group(Context &ctx, std::vector<std::unique_ptr<Sample>> &group, Lap &lap) {
   for (auto &s : group) s->preamble(lap);
-  for (auto &s : group) s->record_counts();
+  std::vector<Sample *> all;  // the samples and their private arenas
+  for (auto &s : group) s->collect(all);
+  for (Sample *s : all) s->record_counts();
   std::vector<LabelItem> items;
-  for (size_t i = 0; i < group.size(); ++i) group[i]->items(int(i), items);
+  for (size_t i = 0; i < all.size(); ++i) all[i]->items(int(i), items);
   std::stable_sort(items.begin(), items.end(), larger_first);
   std::vector<FloodJob> jobs;
-  for (auto &s : group) jobs.push_back(s->done() ? FloodJob{} : s->flood_job());
+  for (Sample *s : all) jobs.push_back(s->done() ? FloodJob{} : s->flood_job());
   prepare_fields(ctx, jobs, items);
   lap("prep_floods_ms");
-  for (auto &s : group) s->after_floods(lap);
+  for (Sample *s : all) s->after_floods(lap);
   items.clear();
-  for (size_t i = 0; i < group.size(); ++i) group[i]->items(int(i), items);
+  for (size_t i = 0; i < all.size(); ++i) all[i]->items(int(i), items);
   std::stable_sort(items.begin(), items.end(), larger_first);
   std::vector<TraceArgs> args;
-  for (auto &s : group) args.push_back(s->done() ? TraceArgs{} : s->trace_args());
+  for (Sample *s : all) args.push_back(s->done() ? TraceArgs{} : s->trace_args());
   std::vector<LabelItem> somata;
-  for (size_t i = 0; i < group.size(); ++i) group[i]->soma_items(int(i), somata);
+  for (size_t i = 0; i < all.size(); ++i) all[i]->soma_items(int(i), somata);
   trace_all(ctx, args, items, somata);
   lap("trace_ms");
   const bool profile = group.size() == 1;

@@ -312,6 +312,15 @@ float VoidFill::dbf_refresh(Context &ctx, const float aniso[3], float *dbf, int6
   return m;
 }
 
+Buffer<unsigned> VoidFill::crop_graph(Context &ctx, const unsigned *graph) const {
+  Crop c;
+  for (int a = 0; a < 3; ++a) { c.lo[a] = lo_[a]; c.ext[a] = ext_[a]; }
+  Buffer<unsigned> words(ctx, size_t(c.size()));
+  crop_words<<<blocks_for(c.size()), kThreads, 0, ctx.stream()>>>(d_, c, graph, words.data());
+  BROOK_LAUNCHED();
+  return words;
+}
+
 std::vector<int64_t> first_voxels(Context &ctx, const void *cc, int cc_bytes, long long n, int64_t count) {
   Buffer<unsigned long long> first(ctx, size_t(count + 1));
   first.fill_bytes(ctx, 0xFF);

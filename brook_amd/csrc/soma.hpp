@@ -3,6 +3,7 @@
 // the soma roots.
 #pragma once
 #include <cstdint>
+#include <utility>
 #include <vector>
 
 #include "stages.hpp"
@@ -26,6 +27,12 @@ class VoidFill {
   void paint(Context &ctx);
   // `filled_count`: the label's voxels after paint() (the box's black border when it is the box)
   float dbf_refresh(Context &ctx, const float aniso[3], float *dbf, int64_t filled_count, const unsigned *graph = nullptr);
+  // The box as a volume of its own (a private arena, soma.cu:54-72): the mask (1 = id or a void,
+  // Fortran order over the box, moved out: paint() and dbf_refresh() need it) and the graph's words.
+  Buffer<uint8_t> take_mask() { return std::move(mask_); }
+  Buffer<unsigned> crop_graph(Context &ctx, const unsigned *graph) const;
+  const int *lo() const { return lo_; }
+  const int *ext() const { return ext_; }
 
  private:
   void *cc_;

@@ -7,8 +7,10 @@ branching modes, the root rule on a plateau of maxima with a known centroid and 
 DAF against a brute-force float32 Bellman-Ford from the free-space seeds and the seed formula
 against its closed forms, trimming (no vertex within the soma radius but the root), and filled
 voids: a soma whose box is the whole volume skeletonizes like the volume with its cavity filled
-(a label inside the cavity disappears), a label partly inside raises. GPU tests compare brook_amd
-with the oracle bit for bit.
+(a label inside the cavity disappears); a soma whose voids hold part of another label is traced in
+a private arena (soma.cu:54-72, 207-215): its skeleton is the filled soma's alone, the other label's
+is the one it has without any soma, and the arena moves with its box bit for bit. GPU tests compare
+brook_amd with the oracle bit for bit.
 """
 import numpy as np
 import pytest
@@ -323,8 +325,63 @@ def test_gpu_soma_voids_equal_oracle(brook, fix_branching, case):
     assert np.array_equal(fields["active"], ref["active"])
 
 
+def _arena_case(name):
+    """Volumes whose soma goes to a private arena: the staircase alone; with a label 2 wholly inside
+    the cavity as well (it stays, nothing is evicted); with a 2-voxel component of the soma's own
+    label inside the cavity (its group joins the arena's skeleton, shared vertices keep the radius of
+    the lower component id); padded with background and a far label (the arena's box is not the
+    volume)."""
+    labels = shapes.soma_cavity(staircase=True, nucleus=name == "nucleus")
+    c = labels.shape[0] // 2
+    if name == "blob":
+        labels[c + 4, c - 1, c - 1:c + 1] = 1
+    if name == "padded":
+        labels = np.pad(labels, ((5, 9), (5, 3), (5, 6)))
+        labels[-3:, -2:, -4:] = 7
+    return np.asfortranarray(labels)
+
+
 @pytest.mark.gpu
-def test_gpu_soma_partial_enclosure_is_unsupported(brook):
-    with pytest.raises(NotImplementedError, match="private"):
-        brook.skeletonize(shapes.soma_cavity(staircase=True), P, (1, 1, 1), dust_threshold=0, fix_borders=False,
-                          progress=False)
+@pytest.mark.parametrize("fix_branching", [True, False])
+@pytest.mark.parametrize("fix_borders", [True, False])
+@pytest.mark.parametrize("case", ["staircase", "nucleus", "blob", "padded"])
+def test_gpu_soma_private_arena_equals_oracle(brook, fix_branching, fix_borders, case):
+    labels = _arena_case(case)
+    aniso = (2, 3, 5) if case == "padded" else (1, 1, 1)
+    got = brook.skeletonize(labels, P, aniso, dust_threshold=0, fix_branching=fix_branching, fix_borders=fix_borders,
+                            output="packed", progress=False)
+    want = oracle.skeletonize(labels, P, aniso, 0, fix_branching=fix_branching, fix_borders=fix_borders)
+    assert 3 in want["labels"].tolist()
+    _same(got, want, (case, fix_branching, fix_borders))
+    fields = brook.prepare_fields(labels, teasar_params=P, anisotropy=aniso, dust_threshold=0, fix_borders=fix_borders)
+    ref = oracle.prepare(labels, P, aniso, 0, fix_borders=fix_borders)
+    assert np.array_equal(fields["active"], ref["active"]) and np.array_equal(fields["dbf"], ref["dbf"])
+
+
+@pytest.mark.gpu
+def test_gpu_soma_private_arena_batch_targets_and_graph(brook):
+    """Arenas inside a batch share its launches; manual targets on the soma move into the arena; a
+    voxel graph is cropped to the arena's box (the labels' graph, and one cutting a process off)."""
+    vols = [_arena_case("staircase"), shapes.soma_cells(), _arena_case("padded")]
+    batch = brook.skeletonize_batch(vols, teasar_params=P, dust_threshold=0, fix_borders=True, output="packed",
+                                    progress=False)
+    for i, v in enumerate(vols):
+        _same(batch[i], oracle.skeletonize(v, P, (1, 1, 1), 0, fix_borders=True, threads=4), i)
+    labels = _arena_case("padded")
+    before, after = [[29, 29, 6]], [[29, 6, 29], [29, 29, 50]]
+    assert all(labels[tuple(p)] == 1 for p in before + after)
+    got = brook.skeletonize(labels, P, (1, 1, 1), dust_threshold=0, fix_borders=False, extra_targets_before=before,
+                            extra_targets_after=after, output="packed", progress=False)
+    want = oracle.skeletonize(labels, P, (1, 1, 1), 0, fix_borders=False, extra_targets_before=before,
+                              extra_targets_after=after)
+    _same(got, want, "targets")
+    from tests.golden import graphs
+    c = labels.shape[1] // 2
+    side = np.zeros(labels.shape, bool)
+    side[:, c + 14:, :] = True
+    for cut in (False, True):
+        graph = np.asfortranarray(graphs.label_graph(labels, side if cut else np.zeros(labels.shape, bool)))
+        got = brook.skeletonize(labels, P, (1, 1, 1), dust_threshold=0, fix_borders=True, voxel_graph=graph,
+                                output="packed", progress=False)
+        want = oracle.skeletonize(labels, P, (1, 1, 1), 0, fix_borders=True, voxel_graph=graph)
+        _same(got, want, ("graph", cut))
