@@ -12,7 +12,8 @@ from ._native import DeviceError  # noqa: E402
 from ._packed import BatchedSkeletons, PackedSkeletons, Skeleton  # noqa: E402
 from .batch import skeletonize_batch  # noqa: E402
 from .serve import Server  # noqa: E402
+from . import _device_post as device_post  # noqa: E402
 
 __all__ = ["skeletonize", "skeletonize_batch", "BatchedSkeletons", "PackedSkeletons", "Skeleton",
-           "DEFAULT_TEASAR_PARAMS", "DimensionError", "DeviceError", "format_labels", "prepare_fields", "Server",
+           "DEFAULT_TEASAR_PARAMS", "DimensionError", "DeviceError", "format_labels", "prepare_fields", "Server", "device_post",
            "__version__"]

@@ -109,6 +109,10 @@ def lib():
                 "brook_skeletons_get_array": (S, [_P, _c.c_int, _c.POINTER(_P)]),
                 "brook_merge_skeletons": (S, [_P, _c.POINTER(_P), _P, _c.c_size_t, _c.POINTER(_P)]),
                 "brook_merge_views": (S, [_P, _c.POINTER(SkeletonsView), _P, _c.c_size_t, _c.POINTER(_P)]),
+                "brook_postprocess_canonical": (S, [_P, _P, _c.c_double, _c.c_double, _c.POINTER(_P)]),
+                "brook_postprocess_views": (S, [_P, _c.POINTER(SkeletonsView), _c.c_int, _c.c_double, _c.c_double,
+                                                _c.POINTER(_P)]),
+                "brook_skeleton_components": (S, [_P, _c.POINTER(SkeletonsView), _P, _c.c_size_t]),
                 "brook_prepare_fields": (S, [_P, _c.POINTER(Volume), _c.POINTER(SkeletonizeOptions), _P, _P, _P,
                                              _c.c_size_t, _P, _P, _P, _P, _P, _P]),
                 "brook_context_stat": (S, [_P, _c.c_char_p, _c.POINTER(_c.c_double)]),

@@ -12,6 +12,7 @@
 #include "graph.hpp"
 #include "layout.hpp"
 #include "merge.hpp"
+#include "post.hpp"
 
 namespace {
 
@@ -622,7 +623,7 @@ void brook_skeletonize_options_init(brook_skeletonize_options *p) {
 namespace {
 // BROOK_DEVICE_OUTPUT (brook.h:66, c_api.cpp:314-353): the packed arrays also live in device memory
 // the handle owns; the host arrays stay as the handle's stable snapshot.
-void device_output(brook::Context &ctx, brook_skeletons &r) {
+static void device_output(brook::Context &ctx, brook_skeletons &r) {
   auto copy = [&](const void *src, size_t bytes) {
     auto m = std::make_shared<DeviceMemory>(ctx.device(), bytes);
     if (bytes) BROOK_HIP(hipMemcpyAsync(m->ptr, src, bytes, hipMemcpyHostToDevice, ctx.stream()));
@@ -636,7 +637,7 @@ void device_output(brook::Context &ctx, brook_skeletons &r) {
 
 // Rows x width elements, row major, as (rows, width, 1) with byte strides (4 * width, 4, bytes)
 // (the reference's row_view, packed.cpp:7-14); width 1: (rows, 1, 1), strides (4, bytes, bytes).
-brook_volume device_rows(const std::shared_ptr<DeviceMemory> &m, brook_dtype dtype, int width, int64_t rows) {
+static brook_volume device_rows(const std::shared_ptr<DeviceMemory> &m, brook_dtype dtype, int width, int64_t rows) {
   return view_of_array(rows_array(m, dtype, width, rows));
 }
 }  // namespace
@@ -655,7 +656,7 @@ brook_status brook_skeletonize(brook_context *ctx, const brook_volume *labels, c
 }
 
 namespace {
-brook_status batch(brook_context *ctx, const brook_volume *const *volumes, const brook_volume *const *graphs,
+static brook_status batch(brook_context *ctx, const brook_volume *const *volumes, const brook_volume *const *graphs,
                    size_t count, const brook_skeletonize_options *options, brook_skeletons **out) {
   return protect([&] {
     require(ctx && (volumes || !count) && options && (out || !count));
@@ -755,7 +756,7 @@ brook_status brook_skeletons_get_array(const brook_skeletons *r, brook_skeleton_
 void brook_skeletons_destroy(brook_skeletons *r) { delete r; }
 
 namespace {
-brook::PackedView view_of(const brook_skeletons_view &v) {
+static brook::PackedView view_of(const brook_skeletons_view &v) {
   require(v.vertex_offsets && v.edge_offsets && (v.labels || !v.skeleton_count));
   require((v.vertices && v.radii) || !v.vertex_count);
   require(v.edges || !v.edge_count);
@@ -772,7 +773,7 @@ brook::PackedView view_of(const brook_skeletons_view &v) {
   for (int a = 0; a < 3; ++a) p.anisotropy[a] = v.anisotropy[a];
   return p;
 }
-brook_skeletons *merged(brook::Context &ctx, const std::vector<brook::PackedView> &parts, const float *origins) {
+static brook_skeletons *merged(brook::Context &ctx, const std::vector<brook::PackedView> &parts, const float *origins) {
   std::vector<std::array<float, 3>> at(parts.size(), std::array<float, 3>{0, 0, 0});
   if (origins)
     for (size_t i = 0; i < parts.size(); ++i) at[i] = {origins[3 * i], origins[3 * i + 1], origins[3 * i + 2]};
@@ -808,6 +809,64 @@ brook_status brook_merge_skeletons(brook_context *ctx, const brook_skeletons *co
       parts.push_back(view_of(v));
     }
     *out = merged(ctx->value, parts, origins);
+  });
+}
+
+namespace {
+static brook_skeletons *postprocessed(brook::Context &ctx, const brook::PackedView &view, int step, double a, double b) {
+  require(step >= BROOK_POST_ALL && step <= BROOK_POST_PRUNE, "unknown postprocess step");
+  brook::DevicePacked in = brook::upload_packed(ctx, view);
+  brook::DevicePacked done = brook::postprocess(ctx, std::move(in), brook::PostStep(step), a, b);
+  auto r = std::make_unique<brook_skeletons>();
+  r->packed = brook::download_packed(ctx, done);
+  device_output(ctx, *r);  // as merges: a device-output handle (brook.h:234-237)
+  return r.release();
+}
+static brook_skeletons_view view_of_result(const brook::Packed &p) {
+  brook_skeletons_view v{};
+  v.skeleton_count = p.labels.size();
+  v.vertex_count = p.radii.size();
+  v.edge_count = p.edges.size() / 2;
+  v.labels = p.labels.data();
+  v.vertex_offsets = reinterpret_cast<const int64_t *>(p.v_off.data());
+  v.edge_offsets = reinterpret_cast<const int64_t *>(p.e_off.data());
+  v.vertices = p.vertices.data();
+  v.edges = p.edges.data();
+  v.radii = p.radii.data();
+  std::copy(p.anisotropy.begin(), p.anisotropy.end(), v.anisotropy);
+  return v;
+}
+}  // namespace
+
+brook_status brook_postprocess_canonical(brook_context *ctx, const brook_skeletons *input, double dust, double ticks,
+                                         brook_skeletons **out) {
+  return protect([&] {
+    require(ctx && input && out);
+    *out = nullptr;
+    brook::ContextCall call(ctx->value);
+    const brook_skeletons_view v = view_of_result(input->packed);
+    *out = postprocessed(ctx->value, view_of(v), BROOK_POST_ALL, dust, ticks);
+  });
+}
+
+brook_status brook_postprocess_views(brook_context *ctx, const brook_skeletons_view *input, int step, double a, double b,
+                                     brook_skeletons **out) {
+  return protect([&] {
+    require(ctx && input && out);
+    *out = nullptr;
+    brook::ContextCall call(ctx->value);
+    *out = postprocessed(ctx->value, view_of(*input), step, a, b);
+  });
+}
+
+brook_status brook_skeleton_components(brook_context *ctx, const brook_skeletons_view *input, int64_t *out, size_t capacity) {
+  return protect([&] {
+    require(ctx && input && (out || !input->vertex_count));
+    require(capacity >= input->vertex_count, "output buffer is too small");
+    brook::ContextCall call(ctx->value);
+    brook::DevicePacked p = brook::upload_packed(ctx->value, view_of(*input));
+    const std::vector<int64_t> roots = brook::skeleton_components(ctx->value, p);
+    std::copy(roots.begin(), roots.end(), out);
   });
 }
 

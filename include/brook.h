@@ -202,6 +202,32 @@ brook_status brook_merge_skeletons(brook_context *context, const brook_skeletons
 brook_status brook_merge_views(brook_context *context, const brook_skeletons_view *fragments, const float *origins,
                                size_t count, brook_skeletons **out);
 
+/* brook.h:234-239: the canonical postprocess of a result on the device with the reference's tie
+ * rules (post.cu:649-651): remove_dust(dust_threshold), remove_loops, join_components,
+ * remove_ticks(tick_threshold), then vertices on no edge dropped. Labels keep their order. The
+ * result is a device-output handle. */
+brook_status brook_postprocess_canonical(brook_context *context, const brook_skeletons *input,
+                                         double dust_threshold, double tick_threshold, brook_skeletons **out);
+/* EXTENSION: the postprocess over a borrowed view (as brook_merge_views), whole or one step, as
+ * the reference's Python binding offers them (Context.remove_dust, remove_loops, join_components,
+ * remove_ticks, postprocess; post.cu:426-651): BROOK_POST_ALL (a = dust, b = ticks),
+ * BROOK_POST_DUST (a), BROOK_POST_LOOPS, BROOK_POST_JOIN, BROOK_POST_TICKS (a), BROOK_POST_PRUNE.
+ * An invalid packed graph is BROOK_INVALID_ARGUMENT. */
+typedef enum {
+  BROOK_POST_ALL = 0,
+  BROOK_POST_DUST = 1,
+  BROOK_POST_LOOPS = 2,
+  BROOK_POST_JOIN = 3,
+  BROOK_POST_TICKS = 4,
+  BROOK_POST_PRUNE = 5
+} brook_post_step;
+brook_status brook_postprocess_views(brook_context *context, const brook_skeletons_view *input, int step, double a,
+                                     double b, brook_skeletons **out);
+/* EXTENSION (the reference's Context.skeleton_components, post.cu:421-425): per vertex of the
+ * view, the least vertex index of its connected component; capacity >= vertex_count. */
+brook_status brook_skeleton_components(brook_context *context, const brook_skeletons_view *input, int64_t *out,
+                                       size_t capacity);
+
 /* EXTENSION (the reference batches only in Python/C++, batch.hpp:20-22): skeletonize `count`
  * volumes in one call; out receives `count` result handles, each equal to a separate
  * brook_skeletonize of that volume. */

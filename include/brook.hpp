@@ -183,6 +183,13 @@ class Context {
     check(brook_merge_skeletons(handle_.get(), handles.data(), flat.empty() ? nullptr : flat.data(), handles.size(), &out));
     return Skeletons(out);
   }
+  // The canonical postprocess on the device (brook.hpp:177-178 of the reference): dust, loops,
+  // joins, ticks; a device-output handle.
+  Skeletons postprocess_canonical(const Skeletons &input, double dust_threshold = 1500, double tick_threshold = 3500) {
+    brook_skeletons *out = nullptr;
+    check(brook_postprocess_canonical(handle_.get(), input.get(), dust_threshold, tick_threshold, &out));
+    return Skeletons(out);
+  }
 
  private:
   detail::Owned<brook_context, brook_context_destroy> handle_;

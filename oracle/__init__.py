@@ -1,4 +1,5 @@
-"""TEST INFRASTRUCTURE ONLY: ctypes access to the CPU oracle (oracle/teasar_oracle.cpp).
+"""TEST INFRASTRUCTURE ONLY: ctypes access to the CPU oracle (oracle/teasar_oracle.cpp,
+oracle/post_oracle.cpp).
 
 The oracle is the CPU restatement of the reference's TEASAR path. Only tests/,
 __graft_entry__.smoke() and bench.py's cpu_baseline leg may import this module; the product
@@ -73,6 +74,16 @@ def lib():
         L.oracle_result_sizes.argtypes = [_P, _P]
         L.oracle_result_copy.argtypes = [_P, _P, _P, _P, _P, _P, _P]
         L.oracle_result_free.argtypes = [_P]
+        L.oracle_post.restype = _P
+        L.oracle_post.argtypes = [_I, ctypes.c_size_t, _P, _P, _P, _P, _P, _P, _D, _D]
+        L.oracle_post_error.restype = ctypes.c_char_p
+        L.oracle_post_sizes.argtypes = [_P, _P]
+        L.oracle_post_copy.argtypes = [_P, _P, _P, _P, _P, _P, _P]
+        L.oracle_post_free.argtypes = [_P]
+        L.oracle_post_components.restype = _I
+        L.oracle_post_components.argtypes = [ctypes.c_size_t, _P, _P, _P, _P, _P, _P, _P]
+        L.oracle_pairwise_sum.restype = ctypes.c_float
+        L.oracle_pairwise_sum.argtypes = [_P, ctypes.c_size_t]
         _lib = L
     return _lib
 
@@ -334,3 +345,55 @@ def skeletonize(labels, teasar_params=None, anisotropy=(1, 1, 1), dust_threshold
         L.oracle_result_free(h)
 
 from .merge import merge_fragments  # noqa: E402,F401
+
+_POST_STEPS = {"postprocess": 0, "remove_dust": 1, "remove_loops": 2, "join_components": 3, "remove_ticks": 4,
+               "prune_unused": 5}
+
+
+def _packed_arrays(p):
+    labels = np.ascontiguousarray(np.asarray(p["labels"], np.int64))
+    voff = np.ascontiguousarray(np.asarray(p["v_off"], np.int64))
+    eoff = np.ascontiguousarray(np.asarray(p["e_off"], np.int64))
+    verts = np.ascontiguousarray(np.asarray(p["vertices"], np.float32).reshape(-1, 3))
+    edges = np.ascontiguousarray(np.asarray(p["edges"], np.uint32).reshape(-1, 2))
+    radii = np.ascontiguousarray(np.asarray(p["radii"], np.float32))
+    return labels, voff, eoff, verts, edges, radii
+
+
+def post(step, packed, a=0.0, b=0.0):
+    """The canonical postprocess of a packed result (dict of labels, v_off, e_off, vertices,
+    edges, radii), one step or all of them: postprocess(a = dust, b = ticks), remove_dust(a),
+    remove_loops, join_components, remove_ticks(a), prune_unused. Returns a packed dict; raises
+    ValueError on an invalid packed graph."""
+    arrays = _packed_arrays(packed)
+    L = lib()
+    h = L.oracle_post(_POST_STEPS[step], len(arrays[0]), *(_ptr(x) for x in arrays), float(a), float(b))
+    if not h:
+        raise ValueError(L.oracle_post_error().decode())
+    try:
+        sizes = np.zeros(3, np.int64)
+        L.oracle_post_sizes(h, _ptr(sizes))
+        k, nv, ne = (int(x) for x in sizes)
+        out = dict(labels=np.zeros(k, np.int64), v_off=np.zeros(k + 1, np.int64), e_off=np.zeros(k + 1, np.int64),
+                   vertices=np.zeros((nv, 3), np.float32), edges=np.zeros((ne, 2), np.uint32),
+                   radii=np.zeros(nv, np.float32))
+        L.oracle_post_copy(h, _ptr(out["labels"]), _ptr(out["v_off"]), _ptr(out["e_off"]), _ptr(out["vertices"]),
+                           _ptr(out["edges"]), _ptr(out["radii"]))
+        return out
+    finally:
+        L.oracle_post_free(h)
+
+
+def post_components(packed):
+    """Per vertex of a packed result: the least vertex index of its connected component."""
+    arrays = _packed_arrays(packed)
+    out = np.zeros(len(arrays[5]), np.int64)
+    if lib().oracle_post_components(len(arrays[0]), *(_ptr(x) for x in arrays), _ptr(out)) != 0:
+        raise ValueError(lib().oracle_post_error().decode())
+    return out
+
+
+def pairwise_sum(values):
+    """NumPy's pairwise float32 sum, as the dust rule uses it."""
+    v = np.ascontiguousarray(np.asarray(values, np.float32))
+    return np.float32(lib().oracle_pairwise_sum(_ptr(v), len(v)))

@@ -1,7 +1,7 @@
 // A C++ consumer of include/brook.hpp, built against libbrook_amd.so by __graft_entry__.build() and
 // run by tests/test_consumers.py on a GPU, modelled on the reference's C++ test
 // (cpp/tests/cpp_smoke.cpp): a 28-voxel bar of label 42 through the RAII handles, device output,
-// DLPack views of skeleton buffers, merge and batch. Exit status 0 = pass; else the failed check.
+// DLPack views of skeleton buffers, merge, postprocess and batch. Exit status 0 = pass; else the failed check.
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -90,6 +90,12 @@ int main() try {
   REQUIRE(after.skeleton_count == 1 && after.vertex_count == data.vertex_count && after.edge_count == data.edge_count, 13);
   for (size_t i = 0; i < 3 * data.vertex_count; ++i) REQUIRE(after.vertices[i] == data.vertices[i], 14);
   for (size_t i = 0; i < 2 * data.edge_count; ++i) REQUIRE(after.edges[i] == data.edges[i], 15);
+  // the canonical postprocess without dust or tick thresholds leaves it as it is (cpp_smoke.cpp:44-48)
+  brook::api::Skeletons processed = context.postprocess_canonical(merged, 0, 0);
+  const brook_skeletons_view post = processed.view();
+  REQUIRE(post.skeleton_count == 1 && post.vertex_count == data.vertex_count && post.edge_count == data.edge_count, 19);
+  for (size_t i = 0; i < 3 * data.vertex_count; ++i) REQUIRE(post.vertices[i] == data.vertices[i], 20);
+  for (size_t i = 0; i < 2 * data.edge_count; ++i) REQUIRE(post.edges[i] == data.edges[i], 21);
 
   // a batch of the volume and its mirror image equals separate calls
   std::vector<uint32_t> mirror(image.size());
