@@ -44,7 +44,7 @@ def _kernel_table(stats, n, b_in, b_cc, F):
         "edt": (["edt_x", "edt_y", "edt_z"], (3 * b_cc + 4 + 8 + 8) * n),
         "prepare_labels": (["prepare_labels"], 12 * flood),
         "init_records": (["init_records"], (b_cc + 16) * n),
-        "pdrf_alive": (["pdrf_alive"], (4 + 4 + b_cc + 4 + 1) * n),
+        "pdrf_alive": (["pdrf_alive"], b_cc * n + (4 + 4 + 4 + 8) * F),
         "trace_labels": (["trace_labels"], 12 * reached + 13 * claimed),
     }
     rows = {}
@@ -148,6 +148,31 @@ def merge(result, fragments=8, calls=5):
     return dict(value=round(nv_in / wall / 1e6, 3), unit="M input vertices/s", fragments=fragments, calls=calls,
                 p50_ms=round(wall * 1e3, 3), input_vertices=nv_in, input_edges=int(h.edges.shape[0]),
                 output_vertices=int(merged.vertices.shape[0]), split_equals_whole=bool(same))
+
+
+def postprocess(result, dust=1500.0, ticks=3500.0, calls=5):
+    """The canonical GPU postprocess (dust, loops, joins, ticks; Kimimaro's default thresholds in
+    physical units) of the workload's skeletons, device-resident input: input vertices per second
+    over `calls` timed calls, with the result's size and whether repeated calls agree."""
+    import torch
+
+    from brook_amd import device_post
+    part = result.to_host().to_device("cuda")
+    first = device_post.postprocess(part, dust, ticks).to_host()
+    times = []
+    for _ in range(calls):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        device_post.postprocess(part, dust, ticks)
+        times.append(time.perf_counter() - t0)
+    again = device_post.postprocess(part, dust, ticks).to_host()
+    same = all(np.array_equal(getattr(first, k), getattr(again, k)) for k in ("vertices", "edges", "radii", "v_off", "e_off"))
+    nv = int(part.vertices.shape[0])
+    wall = float(np.median(times))
+    return dict(value=round(nv / wall / 1e6, 3), unit="M input vertices/s", calls=calls, p50_ms=round(wall * 1e3, 3),
+                dust_threshold=dust, tick_threshold=ticks, input_vertices=nv, input_edges=int(part.edges.shape[0]),
+                output_vertices=int(first.vertices.shape[0]), output_edges=int(first.edges.shape[0]),
+                repeat_equal=bool(same))
 
 
 def batched():
@@ -432,6 +457,8 @@ def main():
         _progress("fix_branching_false done")
         out["merge"] = merge(result)
         _progress("merge done")
+        out["postprocess"] = postprocess(result)
+        _progress("postprocess done")
         out["batched"] = batched()
         _progress("batched done")
         out["stress"] = stress()

@@ -446,9 +446,10 @@ class Sample {
     M_dev.upload(ctx_, M.data(), k_);
     inv_dev.upload(ctx_, inv.data(), k_);
     use_dev.upload(ctx_, use.data(), k_);
-    Buffer<float> pdrf(ctx_, size_t(n_));
+    Buffer<float> pdrf;  // the PDRF as a volume of its own only when the fields are asked for
+    if (fields_ && fields_->pdrf) pdrf = Buffer<float>(ctx_, size_t(n_));
     pdrf_and_alive(ctx_, cc_.labels.data(), cc_.cc_bytes, n_, dbf_.data(), daf_.data(), M_dev.data(), inv_dev.data(),
-                   use_dev.data(), squarings, float(o_.teasar.pdrf_scale), pdrf.data(), rec_.data());
+                   use_dev.data(), squarings, float(o_.teasar.pdrf_scale), pdrf.size() ? pdrf.data() : nullptr, rec_.data());
     Buffer<int> labels_dev(ctx_, std::max<size_t>(labels_.size(), 1));
     labels_dev.upload(ctx_, labels_.data(), labels_.size());
     if (fields_) {

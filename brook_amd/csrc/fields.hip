@@ -65,26 +65,42 @@ __global__ void __launch_bounds__(256) fill_segments(const CC *cc, int n, Dims d
 
 // Voxel records: the neighbour mask (bit k = neighbour k of v is inside the volume and has v's
 // nonzero id; every later neighbour walk reads this instead of 26 ids and bounds checks),
-// PDRF +inf, word ~0.
+// PDRF +inf, word ~0. A workgroup covers a kRecX x kRecY x kRecZ box: its ids and a one-voxel
+// halo (0 outside the volume, which never equals an id) are staged in LDS by coalesced row loads,
+// the 26 comparisons of a voxel read LDS, and each wave stores one row of 64 records contiguously.
+constexpr int kRecX = 64, kRecY = 4, kRecZ = 4;
 template <class CC>
-__global__ void __launch_bounds__(256) init_records(const CC *cc, Dims d, long long n, VoxelRec *rec) {
-  long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  CC l = cc[i];
-  unsigned m = 0;
-  if (l != CC(0)) {
-    int x, y, z;
-    unravel(int(i), d, x, y, z);
-    for_each_nb([&](auto K) {
-      constexpr int k = decltype(K)::value;
-      if (inside(x + kDX[k], y + kDY[k], z + kDZ[k], d) && cc[i + nb_offset<k>(d)] == l) m |= 1u << k;
-    });
+__global__ void __launch_bounds__(256) init_records(const CC *cc, Dims d, VoxelRec *rec) {
+  constexpr int TX = kRecX + 2, TY = kRecY + 2, TZ = kRecZ + 2;
+  __shared__ CC tile[TZ][TY][TX];
+  const int bx = int(blockIdx.x) * kRecX, by = int(blockIdx.y) * kRecY, bz = int(blockIdx.z) * kRecZ;
+  for (int t = threadIdx.x; t < TX * TY * TZ; t += 256) {
+    const int tx = t % TX, ty = (t / TX) % TY, tz = t / (TX * TY);
+    const int x = bx + tx - 1, y = by + ty - 1, z = bz + tz - 1;
+    CC v = CC(0);
+    if (x >= 0 && x < d.sx && y >= 0 && y < d.sy && z >= 0 && z < d.sz)
+      v = cc[x + (long long)d.sx * (y + (long long)d.sy * z)];
+    tile[tz][ty][tx] = v;
   }
-  VoxelRec r;
-  r.pdrf = kInf;
-  r.meta = m;
-  r.word = kMaxKey;
-  rec[i] = r;
+  __syncthreads();
+  const int tx = int(threadIdx.x) % kRecX, ty = int(threadIdx.x) / kRecX;
+  const int x = bx + tx, y = by + ty;
+  if (x >= d.sx || y >= d.sy) return;
+  for (int tz = 0; tz < kRecZ && bz + tz < d.sz; ++tz) {
+    const CC l = tile[tz + 1][ty + 1][tx + 1];
+    unsigned m = 0;
+    if (l != CC(0)) {
+      for_each_nb([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        if (tile[tz + 1 + kDZ[k]][ty + 1 + kDY[k]][tx + 1 + kDX[k]] == l) m |= 1u << k;
+      });
+    }
+    VoxelRec r;
+    r.pdrf = kInf;
+    r.meta = m;
+    r.word = kMaxKey;
+    rec[x + (long long)d.sx * (y + (long long)d.sy * (bz + tz))] = r;
+  }
 }
 
 __global__ void fill_float(float *p, long long n, float v) {
@@ -345,7 +361,10 @@ __global__ void __launch_bounds__(256) pdrf_alive(const CC *cc, const float *dbf
   if (i >= n) return;
   CC l = cc[i];
   unsigned char u = use[l];
-  if (l == CC(0) || u == 0) { pdrf[i] = kInf; return; }
+  if (l == CC(0) || u == 0) {
+    if (pdrf) pdrf[i] = kInf;
+    return;
+  }
   float dv = dbf[i];
   if (dv == 0.0f) dv = kInf;
   float p = 1.0f - dv * M[l];
@@ -356,9 +375,11 @@ __global__ void __launch_bounds__(256) pdrf_alive(const CC *cc, const float *dbf
     if (isinf(a)) a = 0.0f;
     p = p + a * inv[l];
   }
-  pdrf[i] = p;
-  rec[i].pdrf = p;
-  rec[i].meta |= kAlive;
+  if (pdrf) pdrf[i] = p;
+  // the record's first half (PDRF, meta) in one 8-byte load and store
+  unsigned long long *head = reinterpret_cast<unsigned long long *>(rec + i);
+  const unsigned meta = unsigned(*head >> 32) | kAlive;
+  *head = ((unsigned long long)meta << 32) | __float_as_uint(p);
 }
 
 __global__ void set_rails(VoxelRec *rec, const long long *root, const int *labels, int count) {
@@ -395,12 +416,15 @@ void voxel_records(Context &ctx, const void *cc, int cc_bytes, Dims d, VoxelRec 
   const long long n = (long long)d.sx * d.sy * d.sz;
   if (n == 0) return;
   KernelTimer kt(ctx, "init_records");
+  if ((d.sy + kRecY - 1) / kRecY > 65535 || (d.sz + kRecZ - 1) / kRecZ > 65535)
+    throw std::invalid_argument("volume too tall for the record tiles");
+  const dim3 g(unsigned((d.sx + kRecX - 1) / kRecX), unsigned((d.sy + kRecY - 1) / kRecY), unsigned((d.sz + kRecZ - 1) / kRecZ));
   if (cc_bytes == 1)
-    init_records<uint8_t><<<grid_for(n), 256, 0, ctx.stream()>>>(static_cast<const uint8_t *>(cc), d, n, rec);
+    init_records<uint8_t><<<g, 256, 0, ctx.stream()>>>(static_cast<const uint8_t *>(cc), d, rec);
   else if (cc_bytes == 2)
-    init_records<uint16_t><<<grid_for(n), 256, 0, ctx.stream()>>>(static_cast<const uint16_t *>(cc), d, n, rec);
+    init_records<uint16_t><<<g, 256, 0, ctx.stream()>>>(static_cast<const uint16_t *>(cc), d, rec);
   else
-    init_records<uint32_t><<<grid_for(n), 256, 0, ctx.stream()>>>(static_cast<const uint32_t *>(cc), d, n, rec);
+    init_records<uint32_t><<<g, 256, 0, ctx.stream()>>>(static_cast<const uint32_t *>(cc), d, rec);
   BROOK_LAUNCHED();
 }
 
