@@ -411,6 +411,8 @@ class Sample {
     j.daf = daf_.data();
     j.lists_a = lists_[0].data();
     j.lists_b = lists_[1].data();
+    j.far_a = lists_[2].data();
+    j.far_b = lists_[3].data();
     return j;
   }
 

@@ -35,7 +35,8 @@ struct FloodJob {
   Dims d{0, 0, 0};
   LabelTables t;
   float *daf = nullptr;
-  int *lists_a = nullptr, *lists_b = nullptr;
+  int *lists_a = nullptr, *lists_b = nullptr;  // one entry per voxel (segment seg_start[l])
+  int *far_a = nullptr, *far_b = nullptr;      // two entries per voxel (segment 2 * seg_start[l])
 };
 // Root search and DAF floods of every item (one workgroup per item, in item order).
 void prepare_fields(Context &ctx, const std::vector<FloodJob> &jobs, const std::vector<LabelItem> &items);
