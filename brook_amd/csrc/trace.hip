@@ -45,6 +45,7 @@ __device__ __forceinline__ int wave_min(int v) {
 // j < kLdsFront and in the label's global list beyond, so a round's appends and the next round's
 // reads of a small front never wait for device memory.
 constexpr int kLdsFront = 4096;
+constexpr int kRadiusSlots = 4096;
 
 struct Shared {
   // Front-list lengths rotate over three counters: round r reads cnt[r], appends to cnt[r+1] and
@@ -69,6 +70,8 @@ struct Shared {
   unsigned long long prof[kProf];
   long long tick;
   int front[2][kLdsFront];
+  // invalidation: ball radii of the path's voxels by source (see source_radius)
+  unsigned long long radius[kRadiusSlots];
 };
 
 __device__ __forceinline__ int rot_next(int r) { return r == 2 ? 0 : r + 1; }
@@ -561,6 +564,20 @@ __device__ __forceinline__ unsigned long long claim_word(unsigned mb, unsigned s
   return ((unsigned long long)mb << 32) | ((unsigned long long)su << 1) | 1ULL;
 }
 __device__ __forceinline__ unsigned claim_source(unsigned long long w) { return (unsigned)w >> 1; }
+
+// The invalidation ball around source s: scale * DBF(s) + constant. Every source of an
+// invalidation is a voxel of its path, so the radii are staged once in an LDS table of (source,
+// radius) slots, direct-mapped by a multiplicative hash (of colliding path voxels the lowest index
+// keeps the slot); a source found in its slot costs no device-memory load, any other is read as
+// before. Table and fallback give the same bits (one expression).
+__device__ __forceinline__ float ball_radius(const Ball &ball, float dbf) { return ball.scale * dbf + ball.constant; }
+__device__ __forceinline__ int radius_slot(unsigned s) { return int((s * 0x9E3779B1u) >> 20); }
+static_assert(kRadiusSlots == 1 << 12, "radius_slot yields 12 bits");
+__device__ __forceinline__ float source_radius(const TraceArgs &a, const Ball &ball, const unsigned long long *tab,
+                                               unsigned s) {
+  const unsigned long long e = tab[radius_slot(s)];
+  return (unsigned)(e >> 32) == s ? __uint_as_float((unsigned)e) : ball_radius(ball, g_ld(a.dbf + s));
+}
 // An improving claim whose old word says the voxel still has to be queued.
 __device__ __forceinline__ bool needs_queue(unsigned long long old) { return old == kMaxKey || !(old & 1ULL); }
 
@@ -569,40 +586,28 @@ __device__ __forceinline__ float offer_distance(const TraceArgs &a, int x, int y
   return sqrtf(dx * dx + dy * dy + dz * dz);
 }
 
-// Neighbours in `wait` are beyond the band: each unclaimed one not yet waiting is marked waiting
-// (one CAS winner) and reported in p.far, to be pulled at the band boundary.
-__device__ __forceinline__ void wait_for_band(const TraceArgs &a, const int *off, int u, unsigned wait, Pushes &p) {
-  while (wait) {
-    const int k = __ffs(wait) - 1;
-    wait &= wait - 1;
-    const int v = u + off[k];
-    if (lp_u64(&a.rec[v].word) != kMaxKey) continue;
-    unsigned seen = rec_meta(a.rec, v);
-    while (!(seen & kWaiting)) {
-      const unsigned got = g_cas(&a.rec[v].meta, seen, seen | kWaiting);
-      if (got == seen) { p.far |= 1u << k; break; }
-      seen = got;
-    }
-  }
-}
+// A neighbour whose offer is beyond the band waits for it: if its record showed it unclaimed and
+// not yet waiting, one atomic sets the waiting flag and its winner reports it in p.far, to be
+// pulled at the band boundary. (A voxel claimed meanwhile may be listed too: the boundary skips
+// claimed entries; a claimed voxel keeps the flag until the claimed set dies.)
+__device__ __forceinline__ bool may_wait(const RecWords &r) { return r.w == 0xFFFFFFFFu && !(r.y & kWaiting); }
 
-// Masked neighbours B..E-1 of u at once: alive and key loads, then claim atomics, in flight
-// together. Only the key's high word (the bottleneck) is pre-loaded: a claim whose bottleneck is
-// above it cannot win and issues no atomic. The outcome is a set of bit masks; the rarer waits
-// beyond the band run as a rolled loop. (A waiting voxel that is claimed keeps its waiting flag:
-// the band boundary skips claimed entries, and the flag is cleared when the claimed set dies.)
+// Masked neighbours B..E-1 of u at once: alive and key loads, then the claim and waiting atomics,
+// in flight together. Only the key's high word (the bottleneck) is pre-loaded: a claim whose
+// bottleneck is above it cannot win and issues no atomic. The outcome is a set of bit masks.
 template <int B, int E>
 __device__ __forceinline__ void offer_part(const TraceArgs &a, const int *off, int u, unsigned m, int x, int y, int z,
                                            int qx, int qy, int qz, unsigned su, unsigned mu, float maxd, unsigned hi,
                                            Pushes &p) {
   bool al[E - B];
-  unsigned kh[E - B];
+  unsigned kh[E - B], fresh = 0;
   for_nb<B, E>([&](auto K) {
     constexpr int k = decltype(K)::value, i = k - B;
     const bool on = (m >> k) & 1u;
     const RecWords r = on ? rec_load(a.rec, u + nb_offset<k>(a.d)) : RecWords{0u, 0u, 0u, 0u};
     al[i] = r.y & kAlive;
     kh[i] = r.w;
+    if (may_wait(r)) fresh |= 1u << k;
   });
   unsigned mbs[E - B];
   unsigned wait = 0;
@@ -614,18 +619,25 @@ __device__ __forceinline__ void offer_part(const TraceArgs &a, const int *off, i
     mbs[i] = mb;
     old[i] = 0;  // "no improvement" unless the atomic says otherwise
     if (al[i] && d < maxd) {
-      if (mb >= hi) wait |= 1u << k;
-      else if (kh[i] >= mb) old[i] = g_min(&a.rec[u + nb_offset<k>(a.d)].word, claim_word(mb, su));
+      if (mb >= hi) {
+        if ((fresh >> k) & 1u) {
+          wait |= 1u << k;
+          old[i] = g_or(&a.rec[u + nb_offset<k>(a.d)].meta, kWaiting);
+        }
+      } else if (kh[i] >= mb) {
+        old[i] = g_min(&a.rec[u + nb_offset<k>(a.d)].word, claim_word(mb, su));
+      }
     }
   });
   for_nb<B, E>([&](auto K) {
     constexpr int k = decltype(K)::value, i = k - B;
-    if (old[i] > claim_word(mbs[i], su)) {  // won (old is 0 where no atomic was issued)
+    if ((wait >> k) & 1u) {  // old: the meta before the waiting flag
+      if (!(old[i] & kWaiting)) p.far |= 1u << k;
+    } else if (old[i] > claim_word(mbs[i], su)) {  // won (old is 0 where no atomic was issued)
       if (old[i] == kMaxKey) p.touched |= 1u << k;
       if (needs_queue(old[i])) p.near |= 1u << k;
     }
   });
-  wait_for_band(a, off, u, wait, p);
 }
 
 // Neighbours kb..kb+cnt-1 (those set in m; cnt <= kSpan) of u: offer_part with a run-time span.
@@ -634,7 +646,7 @@ __device__ __forceinline__ void offer_span(const TraceArgs &a, const int *off, i
                                            unsigned hi, Pushes &p) {
   constexpr int S = kSpan;
   bool al[S];
-  unsigned kh[S];
+  unsigned kh[S], fresh = 0;
   int v[S];
 #pragma unroll
   for (int j = 0; j < S; ++j) {
@@ -644,9 +656,10 @@ __device__ __forceinline__ void offer_span(const TraceArgs &a, const int *off, i
     const RecWords r = on ? rec_load(a.rec, v[j]) : RecWords{0u, 0u, 0u, 0u};
     al[j] = r.y & kAlive;
     kh[j] = r.w;
+    if (may_wait(r)) fresh |= 1u << j;
   }
   unsigned mbs[S];
-  unsigned wait = 0;
+  unsigned wait = 0;  // bit j: slot j's atomic set the waiting flag
   unsigned long long old[S];
 #pragma unroll
   for (int j = 0; j < S; ++j) {
@@ -656,19 +669,26 @@ __device__ __forceinline__ void offer_span(const TraceArgs &a, const int *off, i
     mbs[j] = mb;
     old[j] = 0;  // "no improvement" unless the atomic says otherwise
     if (al[j] && d < maxd) {
-      if (mb >= hi) wait |= 1u << k;
-      else if (kh[j] >= mb) old[j] = g_min(&a.rec[v[j]].word, claim_word(mb, su));
+      if (mb >= hi) {
+        if ((fresh >> j) & 1u) {
+          wait |= 1u << j;
+          old[j] = g_or(&a.rec[v[j]].meta, kWaiting);
+        }
+      } else if (kh[j] >= mb) {
+        old[j] = g_min(&a.rec[v[j]].word, claim_word(mb, su));
+      }
     }
   }
 #pragma unroll
   for (int j = 0; j < S; ++j) {
     const int k = kb + j;
-    if (old[j] > claim_word(mbs[j], su)) {  // won
+    if ((wait >> j) & 1u) {  // old: the meta before the waiting flag
+      if (!(old[j] & kWaiting)) p.far |= 1u << k;
+    } else if (old[j] > claim_word(mbs[j], su)) {  // won
       if (old[j] == kMaxKey) p.touched |= 1u << k;
       if (needs_queue(old[j])) p.near |= 1u << k;
     }
   }
-  wait_for_band(a, off, u, wait, p);
 }
 
 // One invalidation round over a front of at most kGroupFront entries, a group of lanes per entry.
@@ -686,7 +706,7 @@ __device__ __forceinline__ void offer_groups(const TraceArgs &a, const Ball &bal
     m = rec_meta(a.rec, u) & kNbrBits;
     if (q.g == 0) {
       ku = g_and(&a.rec[u].word, ~1ULL);
-      maxd = ball.scale * g_ld(a.dbf + claim_source(ku)) + ball.constant;
+      maxd = source_radius(a, ball, sh.radius, claim_source(ku));
     }
   }
   __syncthreads();
@@ -719,8 +739,8 @@ __device__ __forceinline__ void offer_all(const TraceArgs &a, const int *off, in
 // smallest (max(bottleneck, distance to the neighbour's source), source) among those whose source's
 // ball holds c. Claim loads, then the sources' DBF loads, each issued together.
 template <int B, int E>
-__device__ __forceinline__ void pull_part(const TraceArgs &a, const Ball &ball, int c, unsigned mc, int x, int y, int z,
-                                          unsigned long long &best) {
+__device__ __forceinline__ void pull_part(const TraceArgs &a, const Ball &ball, const unsigned long long *tab, int c,
+                                          unsigned mc, int x, int y, int z, unsigned long long &best) {
   unsigned long long ku[E - B];
   for_nb<B, E>([&](auto K) {
     constexpr int k = decltype(K)::value, i = k - B;
@@ -729,7 +749,7 @@ __device__ __forceinline__ void pull_part(const TraceArgs &a, const Ball &ball, 
   float r[E - B];
   for_nb<B, E>([&](auto K) {
     constexpr int k = decltype(K)::value, i = k - B;
-    r[i] = ku[i] != kMaxKey ? g_ld(a.dbf + claim_source(ku[i])) : 0.0f;
+    r[i] = ku[i] != kMaxKey ? source_radius(a, ball, tab, claim_source(ku[i])) : 0.0f;
   });
   for_nb<B, E>([&](auto K) {
     constexpr int k = decltype(K)::value, i = k - B;
@@ -738,7 +758,7 @@ __device__ __forceinline__ void pull_part(const TraceArgs &a, const Ball &ball, 
     int qx, qy, qz;
     unravel(int(su), a.d, qx, qy, qz);
     const float d = offer_distance(a, x, y, z, qx, qy, qz);
-    if (d >= ball.scale * r[i] + ball.constant) return;
+    if (d >= r[i]) return;
     const unsigned mb = max(__float_as_uint(d), (unsigned)(ku[i] >> 32));
     const unsigned long long cand = ((unsigned long long)mb << 32) | su;
     if (cand < best) best = cand;
@@ -747,8 +767,8 @@ __device__ __forceinline__ void pull_part(const TraceArgs &a, const Ball &ball, 
 
 // Neighbours kb..kb+cnt-1 (those set in mc; cnt <= kSpan) of waiting voxel c: pull_part with a
 // run-time span.
-__device__ __forceinline__ void pull_span(const TraceArgs &a, const Ball &ball, const int *off, int c, unsigned mc, int kb, int cnt, int x,
-                                          int y, int z, unsigned long long &best) {
+__device__ __forceinline__ void pull_span(const TraceArgs &a, const Ball &ball, const unsigned long long *tab, const int *off, int c,
+                                          unsigned mc, int kb, int cnt, int x, int y, int z, unsigned long long &best) {
   constexpr int S = kSpan;
   unsigned long long ku[S];
 #pragma unroll
@@ -758,7 +778,7 @@ __device__ __forceinline__ void pull_span(const TraceArgs &a, const Ball &ball, 
   }
   float r[S];
 #pragma unroll
-  for (int j = 0; j < S; ++j) r[j] = ku[j] != kMaxKey ? g_ld(a.dbf + claim_source(ku[j])) : 0.0f;
+  for (int j = 0; j < S; ++j) r[j] = ku[j] != kMaxKey ? source_radius(a, ball, tab, claim_source(ku[j])) : 0.0f;
 #pragma unroll
   for (int j = 0; j < S; ++j) {
     if (ku[j] == kMaxKey) continue;
@@ -766,7 +786,7 @@ __device__ __forceinline__ void pull_span(const TraceArgs &a, const Ball &ball, 
     int qx, qy, qz;
     unravel(int(su), a.d, qx, qy, qz);
     const float d = offer_distance(a, x, y, z, qx, qy, qz);
-    if (d >= ball.scale * r[j] + ball.constant) continue;
+    if (d >= r[j]) continue;
     const unsigned mb = max(__float_as_uint(d), (unsigned)(ku[j] >> 32));
     const unsigned long long cand = ((unsigned long long)mb << 32) | su;
     if (cand < best) best = cand;
@@ -789,7 +809,7 @@ __device__ __forceinline__ void pull_groups(const TraceArgs &a, const Ball &ball
     if (!claimed) {
       int x, y, z;
       unravel(c, a.d, x, y, z);
-      pull_span(a, ball, sh.off, c, mc, q.kb, q.cnt, x, y, z, best);
+      pull_span(a, ball, sh.radius, sh.off, c, mc, q.kb, q.cnt, x, y, z, best);
     }
   }
   for (int o = 1; o < q.G; o <<= 1) {
@@ -819,12 +839,15 @@ __device__ __forceinline__ bool invalidate(const TraceArgs &a, const Ball &ball,
     sh.cnt[0] = sh.cnt[1] = sh.cnt[2] = sh.n_far = sh.n_keep = sh.n_touched = 0;
     sh.band = 0;
   }
+  for (int j = threadIdx.x; j < kRadiusSlots; j += blockDim.x) sh.radius[j] = kMaxKey;
   __syncthreads();
   Front cur = la, nxt = lb;
   int *far = lfar, *keep = lkeep;
   int r = 0;  // the current list's length is sh.cnt[r]
   for (int j = threadIdx.x; j < len; j += blockDim.x) {  // sources: path voxels still alive
     int p = g_ld(path + j);
+    atomicMin(&sh.radius[radius_slot(unsigned(p))],
+              ((unsigned long long)unsigned(p) << 32) | __float_as_uint(ball_radius(ball, g_ld(a.dbf + p))));
     if (!(rec_meta(a.rec, p) & kAlive)) continue;
     if (g_cas(&a.rec[p].word, kMaxKey, claim_word(0u, unsigned(p))) == kMaxKey) {
       front_put(cur, wave_slot(&sh.cnt[0]), p);
@@ -858,7 +881,7 @@ __device__ __forceinline__ bool invalidate(const TraceArgs &a, const Ball &ball,
             const unsigned m = (rec_meta(a.rec, u) & kNbrBits);
             const unsigned long long ku = g_ld(snap + i);
             const unsigned su = claim_source(ku), mu = (unsigned)(ku >> 32);
-            offer_all(a, sh.off, u, m, su, mu, ball.scale * a.dbf[su] + ball.constant, hi, p);
+            offer_all(a, sh.off, u, m, su, mu, source_radius(a, ball, sh.radius, su), hi, p);
           }
           emit_neighbours(touched, &sh.n_touched, p.touched, u, sh.off);
           emit_front(nxt, &sh.cnt[rn], p.near, u, sh.off);
@@ -883,7 +906,7 @@ __device__ __forceinline__ bool invalidate(const TraceArgs &a, const Ball &ball,
         int x, y, z;
         unravel(c, a.d, x, y, z);
         unsigned long long best = kMaxKey;
-        for_parts<kNbBatch>(mc, [&](auto B, auto E) { pull_part<decltype(B)::value, decltype(E)::value>(a, ball, c, mc, x, y, z, best); });
+        for_parts<kNbBatch>(mc, [&](auto B, auto E) { pull_part<decltype(B)::value, decltype(E)::value>(a, ball, sh.radius, c, mc, x, y, z, best); });
         g_st(pull + i, best);
         if (best != kMaxKey) atomicMin(&sh.lowest, (unsigned)(best >> 32));
       }
