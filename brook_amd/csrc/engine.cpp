@@ -5,6 +5,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <future>
 #include <limits>
 #include <map>
 #include <memory>
@@ -135,6 +136,25 @@ class Sample {
       fill_holes();
       lap("prep_fill_holes_ms");
     }
+    // Border targets read only the components, so unless fix_avocados relabels them first they
+    // are found on the auxiliary context's stream, from a host thread of their own, while the EDT
+    // runs on this one.
+    const bool early_borders = o_.fix_borders && !o_.fix_avocados;
+    struct EventGuard {
+      hipEvent_t e = nullptr;
+      ~EventGuard() { if (e) (void)hipEventDestroy(e); }
+    } cc_ready;
+    std::future<std::vector<std::vector<Point>>> border_job;  // joined before cc_ready is destroyed
+    if (early_borders) {
+      Context &aux = ctx_.aux();
+      BROOK_HIP(hipEventCreateWithFlags(&cc_ready.e, hipEventDisableTiming));
+      BROOK_HIP(hipEventRecord(cc_ready.e, ctx_.stream()));
+      BROOK_HIP(hipStreamWaitEvent(aux.stream(), cc_ready.e, 0));
+      border_job = std::async(std::launch::async, [this, &aux] {
+        aux.activate();
+        return border_targets(aux, cc_.labels.data(), cc_.cc_bytes, d_, count_, o_.aniso);
+      });
+    }
     dbf_ = Buffer<float>(ctx_, size_t(n_));
     Buffer<unsigned> label_max(ctx_, k_);
     label_max.clear(ctx_);
@@ -152,7 +172,16 @@ class Sample {
       if (count_ == 0) return finish_empty();
     }
     borders_.assign(k_, {});
-    if (o_.fix_borders) borders_ = border_targets(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, count_, o_.aniso);
+    if (early_borders) {
+      borders_ = border_job.get();
+      Context &aux = ctx_.aux();
+      for (const char *key : {"borders_device_ms", "borders_host_ms"}) {
+        ctx_.stats[key] += aux.stats[key];
+        aux.stats.erase(key);
+      }
+    } else if (o_.fix_borders) {
+      borders_ = border_targets(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, count_, o_.aniso);
+    }
     if (arena_ && count_ == 1) borders_[1] = arena_->borders;
     lap("prep_borders_ms");
     // Each point is filed under the component its wrapped coordinates fall on and traced at the

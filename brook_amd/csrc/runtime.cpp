@@ -74,6 +74,11 @@ void Context::give_back(void *ptr, size_t bytes) {
   }
 }
 
+Context &Context::aux() {
+  if (!aux_) aux_ = std::make_unique<Context>(device_);
+  return *aux_;
+}
+
 void Context::trim() {
   if (!cache_.empty()) (void)hipStreamSynchronize(stream_);
   for (auto &kv : cache_) (void)hipFree(kv.second);
