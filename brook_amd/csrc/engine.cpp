@@ -114,6 +114,14 @@ class Sample {
   static int64_t footprint(int64_t n) { return n * 40 + (n * 2 / 5) * 56 + (int64_t(64) << 20); }
 
   void preamble(Lap &lap) {
+    preamble_components(lap, 0);
+    preamble_labels(lap);
+  }
+
+  // The preamble in two halves, so that a group runs every sample's first half before any second
+  // half: components, analysis, filled holes and the EDT (the border targets found meanwhile on
+  // auxiliary context `aux`, see below), then everything from the border targets on.
+  void preamble_components(Lap &lap, int aux) {
     if (n_ == 0 || double(n_) <= o_.dust) return finish_empty();
     bool uniform = false;
     if (in_.graph && !arena_) {  // components of the graph's steps (skeletonize.cu:76-78); an arena is one component
@@ -137,22 +145,24 @@ class Sample {
       lap("prep_fill_holes_ms");
     }
     // Border targets read only the components, so unless fix_avocados relabels them first they
-    // are found on the auxiliary context's stream, from a host thread of their own, while the EDT
-    // runs on this one.
-    const bool early_borders = o_.fix_borders && !o_.fix_avocados;
-    struct EventGuard {
-      hipEvent_t e = nullptr;
-      ~EventGuard() { if (e) (void)hipEventDestroy(e); }
-    } cc_ready;
-    std::future<std::vector<std::vector<Point>>> border_job;  // joined before cc_ready is destroyed
-    if (early_borders) {
-      Context &aux = ctx_.aux();
-      BROOK_HIP(hipEventCreateWithFlags(&cc_ready.e, hipEventDisableTiming));
-      BROOK_HIP(hipEventRecord(cc_ready.e, ctx_.stream()));
-      BROOK_HIP(hipStreamWaitEvent(aux.stream(), cc_ready.e, 0));
-      border_job = std::async(std::launch::async, [this, &aux] {
-        aux.activate();
-        return border_targets(aux, cc_.labels.data(), cc_.cc_bytes, d_, count_, o_.aniso);
+    // are found on an auxiliary context's stream, from a host thread of their own, while the EDT
+    // (and, in a group, the other samples' first halves) run on this one. The jobs of one
+    // auxiliary context take turns (its stream and buffer pool serve one job at a time).
+    early_borders_ = o_.fix_borders && !o_.fix_avocados;
+    if (early_borders_) {
+      Context &ax = ctx_.aux(aux);
+      BROOK_HIP(hipEventCreateWithFlags(&cc_ready_.e, hipEventDisableTiming));
+      BROOK_HIP(hipEventRecord(cc_ready_.e, ctx_.stream()));
+      border_job_ = std::async(std::launch::async, [this, &ax, ready = cc_ready_.e] {
+        std::lock_guard<std::mutex> turn(ax.mutex());
+        ax.activate();
+        BROOK_HIP(hipStreamWaitEvent(ax.stream(), ready, 0));
+        auto targets = border_targets(ax, cc_.labels.data(), cc_.cc_bytes, d_, count_, o_.aniso);
+        border_ms_[0] = ax.stats["borders_device_ms"];
+        border_ms_[1] = ax.stats["borders_host_ms"];
+        ax.stats.erase("borders_device_ms");
+        ax.stats.erase("borders_host_ms");
+        return targets;
       });
     }
     dbf_ = Buffer<float>(ctx_, size_t(n_));
@@ -171,14 +181,15 @@ class Sample {
       lap("prep_avocados_ms");
       if (count_ == 0) return finish_empty();
     }
+  }
+
+  void preamble_labels(Lap &lap) {
+    if (done_) return;
     borders_.assign(k_, {});
-    if (early_borders) {
-      borders_ = border_job.get();
-      Context &aux = ctx_.aux();
-      for (const char *key : {"borders_device_ms", "borders_host_ms"}) {
-        ctx_.stats[key] += aux.stats[key];
-        aux.stats.erase(key);
-      }
+    if (early_borders_) {
+      borders_ = border_job_.get();
+      ctx_.stats["borders_device_ms"] += border_ms_[0];
+      ctx_.stats["borders_host_ms"] += border_ms_[1];
     } else if (o_.fix_borders) {
       borders_ = border_targets(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, count_, o_.aniso);
     }
@@ -831,6 +842,15 @@ class Sample {
   }
 
   Context &ctx_;
+  // The border job of the first preamble half (joined in the second; the future is declared after
+  // the event it waits on, so it is destroyed, i.e. joined, first).
+  struct EventGuard {
+    hipEvent_t e = nullptr;
+    ~EventGuard() { if (e) (void)hipEventDestroy(e); }
+  } cc_ready_;
+  bool early_borders_ = false;
+  double border_ms_[2] = {0, 0};  // device and host milliseconds of the border job
+  std::future<std::vector<std::vector<Point>>> border_job_;
   LabelVolume in_;
   const SkeletonizeOptions &o_;
   FieldsOut *fields_;
@@ -873,10 +893,14 @@ class Sample {
 
 bool larger_first(const LabelItem &a, const LabelItem &b) { return a.size > b.size; }
 
+// Auxiliary contexts a group's border jobs rotate over (each runs one job at a time).
+constexpr int kAuxContexts = 4;
+
 // Runs a group of samples together: every preamble, one flood launch over all their labels
 // (largest first), every PDRF/order pass, one trace launch, every assembly.
 void run_group(Context &ctx, std::vector<std::unique_ptr<Sample>> &group, Lap &lap) {
-  for (auto &s : group) s->preamble(lap);
+  for (size_t i = 0; i < group.size(); ++i) group[i]->preamble_components(lap, int(i % kAuxContexts));
+  for (auto &s : group) s->preamble_labels(lap);
   std::vector<Sample *> all;  // the samples and their private arenas
   for (auto &s : group) s->collect(all);
   for (Sample *s : all) s->record_counts();

@@ -74,9 +74,11 @@ void Context::give_back(void *ptr, size_t bytes) {
   }
 }
 
-Context &Context::aux() {
-  if (!aux_) aux_ = std::make_unique<Context>(device_);
-  return *aux_;
+Context &Context::aux(int i) {
+  if (i < 0) throw std::invalid_argument("auxiliary context index");
+  if (size_t(i) >= aux_.size()) aux_.resize(size_t(i) + 1);
+  if (!aux_[size_t(i)]) aux_[size_t(i)] = std::make_unique<Context>(device_);
+  return *aux_[size_t(i)];
 }
 
 void Context::trim() {

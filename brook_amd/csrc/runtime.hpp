@@ -88,15 +88,16 @@ class Context {
   std::mutex &mutex() { return mutex_; }
   std::map<std::string, double> stats;
   bool kernel_timing = false;  // time instrumented kernels with events on this stream
-  // A second context on the same device (a stream and an allocation cache of its own), made on
-  // first use, for work that one host thread runs beside this context's stream.
-  Context &aux();
+  // Auxiliary context i on the same device (a stream and an allocation cache of its own), made on
+  // first use, for work that other host threads run beside this context's stream. Only the
+  // thread that calls into this context makes them; each serves one thread at a time.
+  Context &aux(int i = 0);
 
  private:
   friend class KernelTimer;
 
  private:
-  std::unique_ptr<Context> aux_;
+  std::vector<std::unique_ptr<Context>> aux_;
   int device_ = 0, cus_ = 0;
   hipStream_t stream_ = nullptr;
   std::multimap<size_t, void *> cache_;
