@@ -80,6 +80,10 @@ std::vector<float> step_weights(const float aniso[3]) {
 // One volume through the pipeline, stage by stage, so that several volumes can share the flood
 // and trace launches: preamble (CCL, analysis, EDT, border targets, voxel records), floods,
 // PDRF and target order, trace, assembly. Device buffers live as long as the object.
+// Auxiliary contexts a group's border jobs rotate over (each runs one job at a time); the early
+// root searches use the one after them.
+constexpr int kAuxContexts = 4;
+
 class Sample {
  public:
   // A soma whose voids hold part of another traced component is traced in a private arena
@@ -101,6 +105,14 @@ class Sample {
       : ctx_(ctx), in_(in), o_(o), fields_(fields), d_(in.dims), n_(in.size()), arena_(arena) {
     if (arena) std::copy(arena->origin, arena->origin + 3, origin_);
   }
+  // The preamble's jobs read this sample's components and write its buffers: they end before any
+  // member goes (when an error skipped the second half that joins them).
+  ~Sample() {
+    if (border_job_.valid()) border_job_.wait();
+    if (root_job_.valid()) root_job_.wait();
+  }
+  Sample(const Sample &) = delete;
+  Sample &operator=(const Sample &) = delete;
 
   bool done() const { return done_; }
   Packed take_result() { return std::move(result_); }
@@ -114,14 +126,15 @@ class Sample {
   static int64_t footprint(int64_t n) { return n * 40 + (n * 2 / 5) * 56 + (int64_t(64) << 20); }
 
   void preamble(Lap &lap) {
-    preamble_components(lap, 0);
+    preamble_components(lap, 0, false);
     preamble_labels(lap);
   }
 
   // The preamble in two halves, so that a group runs every sample's first half before any second
   // half: components, analysis, filled holes and the EDT (the border targets found meanwhile on
-  // auxiliary context `aux`, see below), then everything from the border targets on.
-  void preamble_components(Lap &lap, int aux) {
+  // auxiliary context `aux`, and with early_roots the root searches, see below), then everything
+  // from the border targets on.
+  void preamble_components(Lap &lap, int aux, bool early_roots) {
     if (n_ == 0 || double(n_) <= o_.dust) return finish_empty();
     bool uniform = false;
     if (in_.graph && !arena_) {  // components of the graph's steps (skeletonize.cu:76-78); an arena is one component
@@ -149,10 +162,13 @@ class Sample {
     // (and, in a group, the other samples' first halves) run on this one. The jobs of one
     // auxiliary context take turns (its stream and buffer pool serve one job at a time).
     early_borders_ = o_.fix_borders && !o_.fix_avocados;
-    if (early_borders_) {
-      Context &ax = ctx_.aux(aux);
+    early_roots = early_roots && !in_.graph && !arena_ && !o_.fix_avocados;
+    if (early_borders_ || early_roots) {  // the components are final: the jobs below start after them
       BROOK_HIP(hipEventCreateWithFlags(&cc_ready_.e, hipEventDisableTiming));
       BROOK_HIP(hipEventRecord(cc_ready_.e, ctx_.stream()));
+    }
+    if (early_borders_) {
+      Context &ax = ctx_.aux(aux);
       border_job_ = std::async(std::launch::async, [this, &ax, ready = cc_ready_.e] {
         std::lock_guard<std::mutex> turn(ax.mutex());
         ax.activate();
@@ -165,6 +181,7 @@ class Sample {
         return targets;
       });
     }
+    if (early_roots) search_roots_early();
     dbf_ = Buffer<float>(ctx_, size_t(n_));
     Buffer<unsigned> label_max(ctx_, k_);
     label_max.clear(ctx_);
@@ -181,6 +198,92 @@ class Sample {
       lap("prep_avocados_ms");
       if (count_ == 0) return finish_empty();
     }
+  }
+
+  // A label with neither a border target nor a soma root takes a root search flood before its DAF
+  // flood, and the flood launch waits for the largest of those two-flood labels. The search needs
+  // only the components, so the labels that can have no border target (box off every face of the
+  // volume, or no fix_borders) and are large enough to end the flood launch that way (at least half
+  // the largest label) are searched here, one workgroup each, on auxiliary context kAuxContexts with
+  // records, voxel lists and scratch of its own, from a host thread of their own, while the EDT and
+  // the border targets run; the second half gives the roots found to the labels that end up
+  // without one (the same flood, so the same root), and the flood launch floods them once.
+  void search_roots_early() {
+    std::vector<LabelItem> items;
+    int64_t largest = 0;
+    auto traced = [&](const Box &b) {
+      const int64_t ext = int64_t(b.hi[0] - b.lo[0] + 1) * (b.hi[1] - b.lo[1] + 1) * (b.hi[2] - b.lo[2] + 1);
+      return double(b.count) > o_.dust && ext > 1;
+    };
+    for (size_t id = 1; id < k_; ++id)
+      if (traced(boxes_[id])) largest = std::max(largest, boxes_[id].count);
+    for (size_t id = 1; id < k_; ++id) {
+      const Box &b = boxes_[id];
+      if (!traced(b) || 2 * b.count < largest) continue;
+      const int dims[3] = {d_.sx, d_.sy, d_.sz};
+      bool face = false;
+      for (int a = 0; a < 3; ++a) face |= b.lo[a] == 0 || b.hi[a] == dims[a] - 1;
+      if (o_.fix_borders && face) continue;
+      items.push_back(LabelItem{0, int(id), b.count});
+    }
+    if (items.empty()) return;
+    std::stable_sort(items.begin(), items.end(), [](const LabelItem &a, const LabelItem &b) { return a.size > b.size; });
+    Context &ax = ctx_.aux(kAuxContexts);
+    std::vector<int64_t> counts(k_, 0);
+    for (size_t id = 1; id < k_; ++id) counts[id] = boxes_[id].count;
+    std::vector<long long> first(k_, -1);
+    for (size_t id = 1; id < k_; ++id) first[id] = cc_.roots[id - 1];
+    root_job_ = std::async(std::launch::async, [this, &ax, ready = cc_ready_.e, items = std::move(items),
+                                                counts = std::move(counts), first = std::move(first)]() -> std::vector<long long> {
+      try {  // an early search that fails (memory, a stalled flood) leaves the search to the flood launch
+        std::lock_guard<std::mutex> turn(ax.mutex());
+        ax.activate();
+        BROOK_HIP(hipStreamWaitEvent(ax.stream(), ready, 0));
+        const size_t k = counts.size();
+        std::vector<int> seg_start(k + 1, 0), seg_count(k, 0);
+        for (size_t id = 1; id < k; ++id) seg_count[id] = int(counts[id]);
+        for (size_t id = 0; id < k; ++id) seg_start[id + 1] = seg_start[id] + seg_count[id];
+        const size_t F = size_t(std::max(seg_start[k], 1));
+        Buffer<int> seg_start_dev, vox, seg_count_dev(ax, k);
+        build_segments(ax, cc_.labels.data(), cc_.cc_bytes, d_, seg_start, seg_start_dev, vox);
+        seg_count_dev.upload(ax, seg_count.data(), k);
+        std::vector<long long> none(k, -1);
+        Buffer<long long> first_dev(ax, k), root_dev(ax, k), target_dev(ax, k);
+        Buffer<float> max_daf_dev(ax, k), weights(ax, 26);
+        first_dev.upload(ax, first.data(), k);
+        root_dev.upload(ax, none.data(), k);
+        const std::vector<float> W = step_weights(o_.aniso);
+        weights.upload(ax, W.data(), 26);
+        Buffer<VoxelRec> rec(ax, size_t(n_));
+        voxel_records(ax, cc_.labels.data(), cc_.cc_bytes, d_, rec.data());
+        Buffer<int> la(ax, F), lb(ax, F), fa(ax, 2 * F), fb(ax, 2 * F);
+        FloodJob j;
+        j.d = d_;
+        j.t.seg_start = seg_start_dev.data();
+        j.t.seg_count = seg_count_dev.data();
+        j.t.vox = vox.data();
+        j.t.first = first_dev.data();
+        j.t.root = root_dev.data();
+        j.t.target = target_dev.data();
+        j.t.max_daf = max_daf_dev.data();
+        j.t.weights = weights.data();
+        j.t.rec = rec.data();
+        j.lists_a = la.data();
+        j.lists_b = lb.data();
+        j.far_a = fa.data();
+        j.far_b = fb.data();
+        j.root_only = 1;
+        // one workgroup per label, at most a quarter of the chip: the EDT runs beside it
+        prepare_fields(ax, {j}, items, std::max(1, ax.compute_units() / 4));
+        std::vector<long long> roots = root_dev.download(ax);  // (synchronizes the auxiliary stream)
+        early_rec_ = std::move(rec);
+        early_seg_start_ = std::move(seg_start_dev);
+        early_vox_ = std::move(vox);
+        return roots;
+      } catch (...) {
+        return {};
+      }
+    });
   }
 
   void preamble_labels(Lap &lap) {
@@ -266,11 +369,21 @@ class Sample {
         if (!evicted[id]) kept.push_back(id);
       labels_.swap(kept);
     }
+    // The early root searches built the records and voxel lists of the components as they were
+    // before any soma painting: they serve the floods too unless a soma's voids were painted.
+    std::vector<long long> early_roots;
+    if (root_job_.valid()) early_roots = root_job_.get();
+    const bool reuse = !any_fill && early_roots.size() == k_ && early_rec_.size() == size_t(n_);
     std::vector<int> seg_start(k_ + 1, 0), seg_count(k_, 0);
     for (size_t id = 1; id < k_; ++id) seg_count[id] = int(boxes_[id].count);
     for (size_t id = 0; id < k_; ++id) seg_start[id + 1] = seg_start[id] + seg_count[id];
     F_ = seg_start[k_];
-    build_segments(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, seg_start, seg_start_dev_, vox_);
+    if (reuse) {
+      seg_start_dev_ = std::move(early_seg_start_);
+      vox_ = std::move(early_vox_);
+    } else {
+      build_segments(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, seg_start, seg_start_dev_, vox_);
+    }
     seg_count_dev_ = Buffer<int>(ctx_, k_);
     seg_count_dev_.upload(ctx_, seg_count.data(), k_);
 
@@ -315,6 +428,9 @@ class Sample {
       free_radius_dev_ = Buffer<float>(ctx_, k_);
       free_radius_dev_.upload(ctx_, radius.data(), k_);
     }
+    if (early_roots.size() == k_)  // roots searched beside the EDT (search_roots_early)
+      for (int id : labels_)
+        if (root_[id] < 0 && early_roots[size_t(id)] >= 0) root_[id] = early_roots[size_t(id)];
     flood_voxels_ = 0;
     for (int id : labels_) flood_voxels_ += boxes_[id].count * (root_[id] < 0 ? 2 : 1);
     first_dev_ = Buffer<long long>(ctx_, k_);
@@ -328,8 +444,15 @@ class Sample {
     weights_.upload(ctx_, W.data(), 26);
     daf_ = Buffer<float>(ctx_, size_t(n_));
     fill_value(ctx_, daf_.data(), n_, kInf);
-    rec_ = Buffer<VoxelRec>(ctx_, size_t(n_));  // 16 bytes per voxel: PDRF, neighbour mask + flags, distance word
-    voxel_records(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, rec_.data());
+    if (reuse) {
+      rec_ = std::move(early_rec_);  // the root searches left every word ~0
+    } else {
+      rec_ = Buffer<VoxelRec>(ctx_, size_t(n_));  // 16 bytes per voxel: PDRF, neighbour mask + flags, distance word
+      voxel_records(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, rec_.data());
+    }
+    early_rec_ = Buffer<VoxelRec>();
+    early_seg_start_ = Buffer<int>();
+    early_vox_ = Buffer<int>();
     if (in_.graph) {  // push masks in the records, pull masks beside them
       inmask_ = Buffer<unsigned>(ctx_, size_t(n_));
       graph_gates(ctx_, rec_.data(), in_.graph, d_, inmask_.data());
@@ -842,8 +965,7 @@ class Sample {
   }
 
   Context &ctx_;
-  // The border job of the first preamble half (joined in the second; the future is declared after
-  // the event it waits on, so it is destroyed, i.e. joined, first).
+  // The border job of the first preamble half (joined in the second, or by the destructor).
   struct EventGuard {
     hipEvent_t e = nullptr;
     ~EventGuard() { if (e) (void)hipEventDestroy(e); }
@@ -851,6 +973,11 @@ class Sample {
   bool early_borders_ = false;
   double border_ms_[2] = {0, 0};  // device and host milliseconds of the border job
   std::future<std::vector<std::vector<Point>>> border_job_;
+  // The early root searches: the records and voxel lists they built, and per id the root found, or
+  // -1 (written by the job, read after joining it).
+  Buffer<VoxelRec> early_rec_;
+  Buffer<int> early_seg_start_, early_vox_;
+  std::future<std::vector<long long>> root_job_;
   LabelVolume in_;
   const SkeletonizeOptions &o_;
   FieldsOut *fields_;
@@ -893,13 +1020,10 @@ class Sample {
 
 bool larger_first(const LabelItem &a, const LabelItem &b) { return a.size > b.size; }
 
-// Auxiliary contexts a group's border jobs rotate over (each runs one job at a time).
-constexpr int kAuxContexts = 4;
-
 // Runs a group of samples together: every preamble, one flood launch over all their labels
 // (largest first), every PDRF/order pass, one trace launch, every assembly.
 void run_group(Context &ctx, std::vector<std::unique_ptr<Sample>> &group, Lap &lap) {
-  for (size_t i = 0; i < group.size(); ++i) group[i]->preamble_components(lap, int(i % kAuxContexts));
+  for (size_t i = 0; i < group.size(); ++i) group[i]->preamble_components(lap, int(i % kAuxContexts), group.size() == 1);
   for (auto &s : group) s->preamble_labels(lap);
   std::vector<Sample *> all;  // the samples and their private arenas
   for (auto &s : group) s->collect(all);

@@ -569,6 +569,7 @@ __global__ void __launch_bounds__(kFloodBlock) prepare_labels(const FloodJob *__
       for (int j = threadIdx.x; j < cnt; j += blockDim.x) sp_u64(&t.rec[g_ld(list + j)].word, kMaxKey);
       __syncthreads();
       if (threadIdx.x == 0) g_st(t.root + L, root);
+      if (job.root_only) continue;
     }
     const float radius = t.free_radius ? g_ld(t.free_radius + L) : 0.0f;
     const bool converged = radius > 0.0f ? flood_free_space(t.rec, d, list, cnt, int(root), radius, W, off, a, b, fa, fb, fs)
@@ -672,7 +673,7 @@ void fill_value(Context &ctx, float *p, long long n, float v) {
   BROOK_LAUNCHED();
 }
 
-void prepare_fields(Context &ctx, const std::vector<FloodJob> &jobs, const std::vector<LabelItem> &items) {
+void prepare_fields(Context &ctx, const std::vector<FloodJob> &jobs, const std::vector<LabelItem> &items, int max_grid) {
   if (items.empty()) return;
   Buffer<FloodJob> jobs_dev(ctx, jobs.size());
   jobs_dev.upload(ctx, jobs.data(), jobs.size());
@@ -684,7 +685,8 @@ void prepare_fields(Context &ctx, const std::vector<FloodJob> &jobs, const std::
   next.clear(ctx);
   const int nq = int(q.size());
   // every workgroup takes labels from the queue until it is spent, so any grid drains it
-  const int grid = std::min(nq, BROOK_FLOOD_GRID > 0 ? BROOK_FLOOD_GRID : ctx.compute_units() * 2);
+  int grid = std::min(nq, BROOK_FLOOD_GRID > 0 ? BROOK_FLOOD_GRID : ctx.compute_units() * 2);
+  if (max_grid > 0) grid = std::min(grid, max_grid);
   {
     KernelTimer kt(ctx, "prepare_labels");
     prepare_labels<<<grid, kFloodBlock, 0, ctx.stream()>>>(jobs_dev.data(), queue.data(), nq, next.data(), next.data() + 1);

@@ -37,9 +37,12 @@ struct FloodJob {
   float *daf = nullptr;
   int *lists_a = nullptr, *lists_b = nullptr;  // one entry per voxel (segment seg_start[l])
   int *far_a = nullptr, *far_b = nullptr;      // two entries per voxel (segment 2 * seg_start[l])
+  int root_only = 0;  // 1: the root search alone (t.root out; no DAF flood, daf untouched)
 };
-// Root search and DAF floods of every item (one workgroup per item, in item order).
-void prepare_fields(Context &ctx, const std::vector<FloodJob> &jobs, const std::vector<LabelItem> &items);
+// Root search and DAF floods of every item (one workgroup per item, in item order), at most
+// max_grid workgroups at once (0: two per compute unit).
+void prepare_fields(Context &ctx, const std::vector<FloodJob> &jobs, const std::vector<LabelItem> &items,
+                    int max_grid = 0);
 void pdrf_and_alive(Context &ctx, const void *cc, int cc_bytes, long long n, const float *dbf, const float *daf,
                     const float *M, const float *inv, const unsigned char *use, int squarings, float scale, float *pdrf,
                     VoxelRec *rec);
