@@ -187,7 +187,7 @@ def batched():
     dev = [_device_volume(v) for v in vols]
     total = sum(v.size for v in vols)
     params, aniso = PARAMS, (16, 16, 40)
-    brook_amd.skeletonize_batch(dev[:4], params, aniso, dust_threshold=1000, output="packed")
+    brook_amd.skeletonize_batch(dev, params, aniso, dust_threshold=1000, output="packed")  # pools warm
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     batch = brook_amd.skeletonize_batch(dev, params, aniso, dust_threshold=1000, output="packed")

@@ -81,8 +81,10 @@ std::vector<float> step_weights(const float aniso[3]) {
 // and trace launches: preamble (CCL, analysis, EDT, border targets, voxel records), floods,
 // PDRF and target order, trace, assembly. Device buffers live as long as the object.
 // Auxiliary contexts a group's border jobs rotate over (each runs one job at a time); the early
-// root searches use the one after them.
+// root searches use the one after them, for labels of at least kEarlyRootVoxels voxels in volumes
+// of at least kEarlyRootVolume voxels.
 constexpr int kAuxContexts = 4;
+constexpr int64_t kEarlyRootVoxels = int64_t(1) << 17, kEarlyRootVolume = int64_t(1) << 26;
 
 class Sample {
  public:
@@ -162,7 +164,7 @@ class Sample {
     // (and, in a group, the other samples' first halves) run on this one. The jobs of one
     // auxiliary context take turns (its stream and buffer pool serve one job at a time).
     early_borders_ = o_.fix_borders && !o_.fix_avocados;
-    early_roots = early_roots && !in_.graph && !arena_ && !o_.fix_avocados;
+    early_roots = early_roots && !in_.graph && !arena_ && !o_.fix_avocados && n_ >= kEarlyRootVolume;
     if (early_borders_ || early_roots) {  // the components are final: the jobs below start after them
       BROOK_HIP(hipEventCreateWithFlags(&cc_ready_.e, hipEventDisableTiming));
       BROOK_HIP(hipEventRecord(cc_ready_.e, ctx_.stream()));
@@ -204,7 +206,8 @@ class Sample {
   // flood, and the flood launch waits for the largest of those two-flood labels. The search needs
   // only the components, so the labels that can have no border target (box off every face of the
   // volume, or no fix_borders) and are large enough to end the flood launch that way (at least half
-  // the largest label) are searched here, one workgroup each, on auxiliary context kAuxContexts with
+  // the largest label, and at least kEarlyRootVoxels voxels: below that the flood is short and the
+  // auxiliary buffers cost more) are searched here, one workgroup each, on auxiliary context kAuxContexts with
   // records, voxel lists and scratch of its own, from a host thread of their own, while the EDT and
   // the border targets run; the second half gives the roots found to the labels that end up
   // without one (the same flood, so the same root), and the flood launch floods them once.
@@ -219,7 +222,7 @@ class Sample {
       if (traced(boxes_[id])) largest = std::max(largest, boxes_[id].count);
     for (size_t id = 1; id < k_; ++id) {
       const Box &b = boxes_[id];
-      if (!traced(b) || 2 * b.count < largest) continue;
+      if (!traced(b) || 2 * b.count < largest || b.count < kEarlyRootVoxels) continue;
       const int dims[3] = {d_.sx, d_.sy, d_.sz};
       bool face = false;
       for (int a = 0; a < 3; ++a) face |= b.lo[a] == 0 || b.hi[a] == dims[a] - 1;
@@ -1023,7 +1026,9 @@ bool larger_first(const LabelItem &a, const LabelItem &b) { return a.size > b.si
 // Runs a group of samples together: every preamble, one flood launch over all their labels
 // (largest first), every PDRF/order pass, one trace launch, every assembly.
 void run_group(Context &ctx, std::vector<std::unique_ptr<Sample>> &group, Lap &lap) {
-  for (size_t i = 0; i < group.size(); ++i) group[i]->preamble_components(lap, int(i % kAuxContexts), group.size() == 1);
+  // small groups (a serving batch) keep to one auxiliary context: fewer buffer pools to fill
+  const int rotate = group.size() >= 8 ? kAuxContexts : 1;
+  for (size_t i = 0; i < group.size(); ++i) group[i]->preamble_components(lap, int(i % rotate), group.size() == 1);
   for (auto &s : group) s->preamble_labels(lap);
   std::vector<Sample *> all;  // the samples and their private arenas
   for (auto &s : group) s->collect(all);
