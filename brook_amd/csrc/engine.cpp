@@ -80,10 +80,12 @@ std::vector<float> step_weights(const float aniso[3]) {
 // One volume through the pipeline, stage by stage, so that several volumes can share the flood
 // and trace launches: preamble (CCL, analysis, EDT, border targets, voxel records), floods,
 // PDRF and target order, trace, assembly. Device buffers live as long as the object.
-// Auxiliary contexts a group's border jobs rotate over (each runs one job at a time); the early
-// root searches use the one after them, for labels of at least kEarlyRootVoxels voxels in volumes
-// of at least kEarlyRootVolume voxels.
-constexpr int kAuxContexts = 4;
+// Auxiliary contexts (each serves one thread at a time): kBorderAux the border jobs of groups under
+// kWorkerMin samples, kRootAux the early root searches (labels of at least kEarlyRootVoxels voxels
+// in volumes of at least kEarlyRootVolume voxels); a larger group runs its first preamble halves
+// on kWorkers host threads, each with auxiliary context kWorkerBase + w of its own.
+constexpr int kBorderAux = 0, kRootAux = 1, kWorkerBase = 2;
+constexpr int kWorkers = 8, kWorkerMin = 8;
 constexpr int64_t kEarlyRootVoxels = int64_t(1) << 17, kEarlyRootVolume = int64_t(1) << 26;
 
 class Sample {
@@ -104,7 +106,7 @@ class Sample {
   };
 
   Sample(Context &ctx, const LabelVolume &in, const SkeletonizeOptions &o, FieldsOut *fields, const Arena *arena = nullptr)
-      : ctx_(ctx), in_(in), o_(o), fields_(fields), d_(in.dims), n_(in.size()), arena_(arena) {
+      : cx_(&ctx), main_(&ctx), in_(in), o_(o), fields_(fields), d_(in.dims), n_(in.size()), arena_(arena) {
     if (arena) std::copy(arena->origin, arena->origin + 3, origin_);
   }
   // The preamble's jobs read this sample's components and write its buffers: they end before any
@@ -118,6 +120,8 @@ class Sample {
 
   bool done() const { return done_; }
   Packed take_result() { return std::move(result_); }
+  // The context the sample's next work runs on (its buffers stay in the pool they came from).
+  void use_context(Context &c) { cx_ = &c; }
   // This sample and its arenas' samples, which share its flood and trace launches.
   void collect(std::vector<Sample *> &all) {
     all.push_back(this);
@@ -134,24 +138,24 @@ class Sample {
 
   // The preamble in two halves, so that a group runs every sample's first half before any second
   // half: components, analysis, filled holes and the EDT (the border targets found meanwhile on
-  // auxiliary context `aux`, and with early_roots the root searches, see below), then everything
-  // from the border targets on.
+  // auxiliary context `aux`, or at the end of the half on this context when aux < 0, and with
+  // early_roots the root searches, see below), then everything from the border targets on.
   void preamble_components(Lap &lap, int aux, bool early_roots) {
     if (n_ == 0 || double(n_) <= o_.dust) return finish_empty();
     bool uniform = false;
     if (in_.graph && !arena_) {  // components of the graph's steps (skeletonize.cu:76-78); an arena is one component
-      GraphComponents g = graph_components(ctx_, in_, in_.graph, in_.graph_compact_2d);
+      GraphComponents g = graph_components(*cx_, in_, in_.graph, in_.graph_compact_2d);
       cc_ = std::move(g.cc);
       keys_ = std::move(g.keys);
       uniform = g.input_uniform;
     } else {
-      cc_ = connected_components(ctx_, in_);
+      cc_ = connected_components(*cx_, in_);
     }
     lap("prep_ccl_ms");
     count_ = cc_.count;
     if (count_ == 0) return finish_empty();
     k_ = size_t(count_) + 1;
-    boxes_ = analyze(ctx_, cc_, d_);
+    boxes_ = analyze(*cx_, cc_, d_);
     lap("prep_analyze_ms");
     // of the volume before any fill; with a graph, every input voxel has one label (skeletonize.cu:80)
     const bool black_border = in_.graph && !arena_ ? uniform : count_ == 1 && boxes_[1].count == n_;
@@ -163,14 +167,14 @@ class Sample {
     // are found on an auxiliary context's stream, from a host thread of their own, while the EDT
     // (and, in a group, the other samples' first halves) run on this one. The jobs of one
     // auxiliary context take turns (its stream and buffer pool serve one job at a time).
-    early_borders_ = o_.fix_borders && !o_.fix_avocados;
+    early_borders_ = o_.fix_borders && !o_.fix_avocados && aux >= 0;
     early_roots = early_roots && !in_.graph && !arena_ && !o_.fix_avocados && n_ >= kEarlyRootVolume;
     if (early_borders_ || early_roots) {  // the components are final: the jobs below start after them
       BROOK_HIP(hipEventCreateWithFlags(&cc_ready_.e, hipEventDisableTiming));
-      BROOK_HIP(hipEventRecord(cc_ready_.e, ctx_.stream()));
+      BROOK_HIP(hipEventRecord(cc_ready_.e, cx_->stream()));
     }
     if (early_borders_) {
-      Context &ax = ctx_.aux(aux);
+      Context &ax = main_->aux(aux);
       border_job_ = std::async(std::launch::async, [this, &ax, ready = cc_ready_.e] {
         std::lock_guard<std::mutex> turn(ax.mutex());
         ax.activate();
@@ -184,14 +188,14 @@ class Sample {
       });
     }
     if (early_roots) search_roots_early();
-    dbf_ = Buffer<float>(ctx_, size_t(n_));
-    Buffer<unsigned> label_max(ctx_, k_);
-    label_max.clear(ctx_);
+    dbf_ = Buffer<float>(*cx_, size_t(n_));
+    Buffer<unsigned> label_max(*cx_, k_);
+    label_max.clear(*cx_);
     if (in_.graph)
-      graph_edt(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, in_.graph, o_.aniso, black_border, dbf_.data(), label_max.data());
+      graph_edt(*cx_, cc_.labels.data(), cc_.cc_bytes, d_, in_.graph, o_.aniso, black_border, dbf_.data(), label_max.data());
     else
-      edt(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, o_.aniso, black_border, 3, dbf_.data(), label_max.data());
-    std::vector<unsigned> max_bits = label_max.download(ctx_);
+      edt(*cx_, cc_.labels.data(), cc_.cc_bytes, d_, o_.aniso, black_border, 3, dbf_.data(), label_max.data());
+    std::vector<unsigned> max_bits = label_max.download(*cx_);
     dbf_max_.resize(k_);
     for (size_t i = 0; i < k_; ++i) std::memcpy(&dbf_max_[i], &max_bits[i], 4);
     lap("prep_edt_ms");
@@ -200,6 +204,11 @@ class Sample {
       lap("prep_avocados_ms");
       if (count_ == 0) return finish_empty();
     }
+    if (aux < 0 && o_.fix_borders) {  // in place (a worker's context)
+      borders_ = border_targets(*cx_, cc_.labels.data(), cc_.cc_bytes, d_, count_, o_.aniso);
+      borders_ready_ = true;
+      lap("prep_borders_ms");
+    }
   }
 
   // A label with neither a border target nor a soma root takes a root search flood before its DAF
@@ -207,7 +216,7 @@ class Sample {
   // only the components, so the labels that can have no border target (box off every face of the
   // volume, or no fix_borders) and are large enough to end the flood launch that way (at least half
   // the largest label, and at least kEarlyRootVoxels voxels: below that the flood is short and the
-  // auxiliary buffers cost more) are searched here, one workgroup each, on auxiliary context kAuxContexts with
+  // auxiliary buffers cost more) are searched here, one workgroup each, on auxiliary context kRootAux with
   // records, voxel lists and scratch of its own, from a host thread of their own, while the EDT and
   // the border targets run; the second half gives the roots found to the labels that end up
   // without one (the same flood, so the same root), and the flood launch floods them once.
@@ -231,7 +240,7 @@ class Sample {
     }
     if (items.empty()) return;
     std::stable_sort(items.begin(), items.end(), [](const LabelItem &a, const LabelItem &b) { return a.size > b.size; });
-    Context &ax = ctx_.aux(kAuxContexts);
+    Context &ax = main_->aux(kRootAux);
     std::vector<int64_t> counts(k_, 0);
     for (size_t id = 1; id < k_; ++id) counts[id] = boxes_[id].count;
     std::vector<long long> first(k_, -1);
@@ -291,13 +300,13 @@ class Sample {
 
   void preamble_labels(Lap &lap) {
     if (done_) return;
-    borders_.assign(k_, {});
+    if (!borders_ready_) borders_.assign(k_, {});
     if (early_borders_) {
       borders_ = border_job_.get();
-      ctx_.stats["borders_device_ms"] += border_ms_[0];
-      ctx_.stats["borders_host_ms"] += border_ms_[1];
-    } else if (o_.fix_borders) {
-      borders_ = border_targets(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, count_, o_.aniso);
+      cx_->stats["borders_device_ms"] += border_ms_[0];
+      cx_->stats["borders_host_ms"] += border_ms_[1];
+    } else if (o_.fix_borders && !borders_ready_) {
+      borders_ = border_targets(*cx_, cc_.labels.data(), cc_.cc_bytes, d_, count_, o_.aniso);
     }
     if (arena_ && count_ == 1) borders_[1] = arena_->borders;
     lap("prep_borders_ms");
@@ -307,11 +316,11 @@ class Sample {
     auto file_points = [&](const std::vector<Point> &points, std::vector<std::vector<Point>> &out) {
       out.assign(k_, {});
       for (const Point &p : points) {
-        const int64_t id = component_at(ctx_, cc_, d_, p);
+        const int64_t id = component_at(*cx_, cc_, d_, p);
         if (id) {
           const int64_t flat = flat_of(p, d_), plane = int64_t(d_.sx) * d_.sy;
           const Point at = {flat % d_.sx, (flat / d_.sx) % d_.sy, flat / plane};
-          if (flat < 0 || flat >= n_ || component_at(ctx_, cc_, d_, at) != id)
+          if (flat < 0 || flat >= n_ || component_at(*cx_, cc_, d_, at) != id)
             throw std::invalid_argument("extra target's coordinates do not flatten onto its component");
         }
         out.at(size_t(id)).push_back(p);
@@ -339,8 +348,8 @@ class Sample {
     std::vector<uint8_t> evicted(k_, 0);
     for (int id : labels_) {
       if (evicted[id] || !(double(dbf_max_[id]) > o_.teasar.soma_detection)) continue;
-      if (enclosed_voids(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, id, boxes_[id])) {
-        VoidFill fill(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, id, boxes_[id], k_);
+      if (enclosed_voids(*cx_, cc_.labels.data(), cc_.cc_bytes, d_, id, boxes_[id])) {
+        VoidFill fill(*cx_, cc_.labels.data(), cc_.cc_bytes, d_, id, boxes_[id], k_);
         const std::vector<int64_t> &taken = fill.taken();
         auto owner = [&](size_t o) { return taken[o] && active_[o] && !evicted[o]; };
         bool partial = false;
@@ -357,10 +366,10 @@ class Sample {
           evicted[o] = 1;
           active_[o] = 0;
         }
-        fill.paint(ctx_);
+        fill.paint(*cx_);
         for (size_t o = 1; o < k_; ++o) boxes_[o].count -= taken[o];
         boxes_[id].count += fill.voids();
-        dbf_max_[id] = fill.dbf_refresh(ctx_, o_.aniso, dbf_.data(), boxes_[id].count, in_.graph);
+        dbf_max_[id] = fill.dbf_refresh(*cx_, o_.aniso, dbf_.data(), boxes_[id].count, in_.graph);
         any_fill = true;
       }
       soma_[id] = double(dbf_max_[id]) > o_.teasar.soma_acceptance;
@@ -385,10 +394,10 @@ class Sample {
       seg_start_dev_ = std::move(early_seg_start_);
       vox_ = std::move(early_vox_);
     } else {
-      build_segments(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, seg_start, seg_start_dev_, vox_);
+      build_segments(*cx_, cc_.labels.data(), cc_.cc_bytes, d_, seg_start, seg_start_dev_, vox_);
     }
-    seg_count_dev_ = Buffer<int>(ctx_, k_);
-    seg_count_dev_.upload(ctx_, seg_count.data(), k_);
+    seg_count_dev_ = Buffer<int>(*cx_, k_);
+    seg_count_dev_.upload(*cx_, seg_count.data(), k_);
 
     // roots, DAF, targets: inputs of the floods
     std::vector<long long> first(k_, -1);
@@ -399,11 +408,11 @@ class Sample {
     if (any_soma) {
       // the soma root (batched.cu:58-108): among the label's DBF maxima, in box-relative
       // coordinates sorted by (x, y, z), the first of least squared distance to their centroid
-      Buffer<uint8_t> soma_dev(ctx_, k_);
-      soma_dev.upload(ctx_, soma_.data(), k_);
-      Buffer<float> dbf_max_dev(ctx_, k_);
-      dbf_max_dev.upload(ctx_, dbf_max_.data(), k_);
-      std::vector<Maximum> maxima = soma_maxima(ctx_, cc_.labels.data(), cc_.cc_bytes, n_, dbf_.data(), dbf_max_dev.data(),
+      Buffer<uint8_t> soma_dev(*cx_, k_);
+      soma_dev.upload(*cx_, soma_.data(), k_);
+      Buffer<float> dbf_max_dev(*cx_, k_);
+      dbf_max_dev.upload(*cx_, dbf_max_.data(), k_);
+      std::vector<Maximum> maxima = soma_maxima(*cx_, cc_.labels.data(), cc_.cc_bytes, n_, dbf_.data(), dbf_max_dev.data(),
                                                 soma_dev.data());
       std::map<int64_t, std::vector<std::pair<Point, int64_t>>> by_label;
       for (const Maximum &m : maxima) {
@@ -428,41 +437,41 @@ class Sample {
       std::vector<float> radius(k_, 0.0f);
       for (int id : labels_)
         if (soma_[id]) radius[id] = dbf_max_[id];
-      free_radius_dev_ = Buffer<float>(ctx_, k_);
-      free_radius_dev_.upload(ctx_, radius.data(), k_);
+      free_radius_dev_ = Buffer<float>(*cx_, k_);
+      free_radius_dev_.upload(*cx_, radius.data(), k_);
     }
     if (early_roots.size() == k_)  // roots searched beside the EDT (search_roots_early)
       for (int id : labels_)
         if (root_[id] < 0 && early_roots[size_t(id)] >= 0) root_[id] = early_roots[size_t(id)];
     flood_voxels_ = 0;
     for (int id : labels_) flood_voxels_ += boxes_[id].count * (root_[id] < 0 ? 2 : 1);
-    first_dev_ = Buffer<long long>(ctx_, k_);
-    root_dev_ = Buffer<long long>(ctx_, k_);
-    target_dev_ = Buffer<long long>(ctx_, k_);
-    max_daf_dev_ = Buffer<float>(ctx_, k_);
-    first_dev_.upload(ctx_, first.data(), k_);
-    root_dev_.upload(ctx_, root_.data(), k_);
+    first_dev_ = Buffer<long long>(*cx_, k_);
+    root_dev_ = Buffer<long long>(*cx_, k_);
+    target_dev_ = Buffer<long long>(*cx_, k_);
+    max_daf_dev_ = Buffer<float>(*cx_, k_);
+    first_dev_.upload(*cx_, first.data(), k_);
+    root_dev_.upload(*cx_, root_.data(), k_);
     std::vector<float> W = step_weights(o_.aniso);
-    weights_ = Buffer<float>(ctx_, 26);
-    weights_.upload(ctx_, W.data(), 26);
-    daf_ = Buffer<float>(ctx_, size_t(n_));
-    fill_value(ctx_, daf_.data(), n_, kInf);
+    weights_ = Buffer<float>(*cx_, 26);
+    weights_.upload(*cx_, W.data(), 26);
+    daf_ = Buffer<float>(*cx_, size_t(n_));
+    fill_value(*cx_, daf_.data(), n_, kInf);
     if (reuse) {
       rec_ = std::move(early_rec_);  // the root searches left every word ~0
     } else {
-      rec_ = Buffer<VoxelRec>(ctx_, size_t(n_));  // 16 bytes per voxel: PDRF, neighbour mask + flags, distance word
-      voxel_records(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, rec_.data());
+      rec_ = Buffer<VoxelRec>(*cx_, size_t(n_));  // 16 bytes per voxel: PDRF, neighbour mask + flags, distance word
+      voxel_records(*cx_, cc_.labels.data(), cc_.cc_bytes, d_, rec_.data());
     }
     early_rec_ = Buffer<VoxelRec>();
     early_seg_start_ = Buffer<int>();
     early_vox_ = Buffer<int>();
     if (in_.graph) {  // push masks in the records, pull masks beside them
-      inmask_ = Buffer<unsigned>(ctx_, size_t(n_));
-      graph_gates(ctx_, rec_.data(), in_.graph, d_, inmask_.data());
+      inmask_ = Buffer<unsigned>(*cx_, size_t(n_));
+      graph_gates(*cx_, rec_.data(), in_.graph, d_, inmask_.data());
     }
     // lists 2 and 3 are the railroad's far lists: two entries per voxel (trace.hip far_compact)
-    for (int i = 0; i < 6; ++i) lists_.emplace_back(ctx_, size_t(std::max<int64_t>(F_, 1)) * (i == 2 || i == 3 ? 2 : 1));
-    if (!o_.fix_branching) parent_ = Buffer<int>(ctx_, size_t(n_));
+    for (int i = 0; i < 6; ++i) lists_.emplace_back(*cx_, size_t(std::max<int64_t>(F_, 1)) * (i == 2 || i == 3 ? 2 : 1));
+    if (!o_.fix_branching) parent_ = Buffer<int>(*cx_, size_t(n_));
     for (auto &a : arenas_) a->sample->preamble(lap);
   }
 
@@ -472,7 +481,7 @@ class Sample {
     const int *lo = fill.lo(), *ext = fill.ext();
     for (int ax = 0; ax < 3; ++ax) a->origin[ax] = origin_[ax] + lo[ax];
     a->mask = fill.take_mask();
-    if (in_.graph) a->graph = fill.crop_graph(ctx_, in_.graph);
+    if (in_.graph) a->graph = fill.crop_graph(*cx_, in_.graph);
     a->volume.data = a->mask.data();
     a->volume.bytes = 1;
     a->volume.dims = Dims{ext[0], ext[1], ext[2]};
@@ -487,7 +496,7 @@ class Sample {
     a->o.before = shift(before_[id]);
     a->o.after = shift(after_[id]);
     a->borders = shift(borders_[id]);
-    a->sample = std::make_unique<Sample>(ctx_, a->volume, a->o, nullptr, a.get());
+    a->sample = std::make_unique<Sample>(*cx_, a->volume, a->o, nullptr, a.get());
     return a;
   }
 
@@ -498,7 +507,7 @@ class Sample {
     const size_t n = size_t(n_);
     std::vector<uint32_t> labels(n);
     {
-      const std::vector<uint8_t> raw = cc_.labels.download(ctx_, n * size_t(cc_.cc_bytes));
+      const std::vector<uint8_t> raw = cc_.labels.download(*cx_, n * size_t(cc_.cc_bytes));
       for (size_t i = 0; i < n; ++i) {
         uint32_t v = 0;
         std::memcpy(&v, raw.data() + i * cc_.cc_bytes, size_t(cc_.cc_bytes));
@@ -506,24 +515,24 @@ class Sample {
       }
     }
     const std::vector<uint32_t> original = labels;
-    std::vector<float> distance = dbf_.download(ctx_, n);
+    std::vector<float> distance = dbf_.download(*cx_, n);
     Buffer<uint32_t> scratch;
     auto refresh = [&](const std::vector<uint32_t> &current, std::vector<float> &field) {
-      if (!scratch.size()) scratch = Buffer<uint32_t>(ctx_, n);
-      scratch.upload(ctx_, current.data(), n);
-      if (in_.graph) graph_edt(ctx_, scratch.data(), 4, d_, in_.graph, o_.aniso, black_border, dbf_.data());
-      else edt(ctx_, scratch.data(), 4, d_, o_.aniso, black_border, 3, dbf_.data());
-      field = dbf_.download(ctx_, n);
+      if (!scratch.size()) scratch = Buffer<uint32_t>(*cx_, n);
+      scratch.upload(*cx_, current.data(), n);
+      if (in_.graph) graph_edt(*cx_, scratch.data(), 4, d_, in_.graph, o_.aniso, black_border, dbf_.data());
+      else edt(*cx_, scratch.data(), 4, d_, o_.aniso, black_border, 3, dbf_.data());
+      field = dbf_.download(*cx_, n);
     };
-    ctx_.stats["avocado_passes"] = avocado_correct(labels, distance, d_, k_, o_.teasar.soma_detection, refresh, keys_);
+    cx_->stats["avocado_passes"] = avocado_correct(labels, distance, d_, k_, o_.teasar.soma_detection, refresh, keys_);
     keys_.clear();  // the renumbered ids are their own keys
     std::vector<int64_t> mapping, roots;
     const size_t count = avocado_renumber(labels, original, cc_.mapping, k_, mapping, roots);
     const int bytes = id_bytes(int64_t(count));
     std::vector<uint8_t> raw(n * size_t(bytes));
     for (size_t i = 0; i < n; ++i) std::memcpy(raw.data() + i * bytes, &labels[i], size_t(bytes));  // little endian
-    cc_.labels = Buffer<uint8_t>(ctx_, std::max<size_t>(raw.size(), 1));
-    cc_.labels.upload(ctx_, raw.data(), raw.size());
+    cc_.labels = Buffer<uint8_t>(*cx_, std::max<size_t>(raw.size(), 1));
+    cc_.labels.upload(*cx_, raw.data(), raw.size());
     cc_.cc_bytes = bytes;
     cc_.count = int64_t(count);
     cc_.mapping = mapping;
@@ -533,7 +542,7 @@ class Sample {
     dbf_max_.assign(k_, 0.0f);
     for (size_t i = 0; i < n; ++i)
       if (labels[i] && distance[i] < kInfHost) dbf_max_[labels[i]] = std::max(dbf_max_[labels[i]], distance[i]);
-    if (count) boxes_ = analyze(ctx_, cc_, d_);
+    if (count) boxes_ = analyze(*cx_, cc_, d_);
   }
 
   // fill_all_holes (holes.cu:37-62, skeletonize.cu:83-90): in ascending id and with the original
@@ -547,17 +556,17 @@ class Sample {
       if (!b.count || skip[id]) continue;
       const int64_t ex = b.hi[0] - b.lo[0] + 1, ey = b.hi[1] - b.lo[1] + 1, ez = b.hi[2] - b.lo[2] + 1;
       if (ex <= 2 || ey <= 2 || ez <= 2 || ex * ey * ez == b.count) continue;  // no voxel can be enclosed
-      if (!enclosed_voids(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, int64_t(id), b)) continue;
-      VoidFill fill(ctx_, cc_.labels.data(), cc_.cc_bytes, d_, int64_t(id), b, k_);
-      fill.paint(ctx_);
+      if (!enclosed_voids(*cx_, cc_.labels.data(), cc_.cc_bytes, d_, int64_t(id), b)) continue;
+      VoidFill fill(*cx_, cc_.labels.data(), cc_.cc_bytes, d_, int64_t(id), b, k_);
+      fill.paint(*cx_);
       for (size_t o = 1; o < k_; ++o)
         if (fill.taken()[o] && o != id) skip[o] = 1;
-      ctx_.stats["hole_filled_voxels"] += double(fill.voids());
+      cx_->stats["hole_filled_voxels"] += double(fill.voids());
       painted = true;
     }
     if (!painted) return;
-    boxes_ = analyze(ctx_, cc_, d_);
-    const std::vector<int64_t> first = first_voxels(ctx_, cc_.labels.data(), cc_.cc_bytes, n_, count_);
+    boxes_ = analyze(*cx_, cc_, d_);
+    const std::vector<int64_t> first = first_voxels(*cx_, cc_.labels.data(), cc_.cc_bytes, n_, count_);
     for (size_t id = 1; id < k_; ++id) cc_.roots[id - 1] = first[id];
   }
 
@@ -595,9 +604,9 @@ class Sample {
 
   void after_floods(Lap &lap) {
     if (done_) return;
-    root_ = root_dev_.download(ctx_);
-    target_ = target_dev_.download(ctx_);
-    max_daf_ = max_daf_dev_.download(ctx_);
+    root_ = root_dev_.download(*cx_);
+    target_ = target_dev_.download(*cx_);
+    max_daf_ = max_daf_dev_.download(*cx_);
 
     // PDRF (exponent a power of two: squarings), alive bits, rails, target order
     std::vector<float> M(k_, 0.0f), inv(k_, 0.0f);
@@ -609,17 +618,17 @@ class Sample {
     }
     int squarings = 0;
     for (int e = int(o_.teasar.pdrf_exponent); e > 1; e >>= 1) ++squarings;
-    Buffer<float> M_dev(ctx_, k_), inv_dev(ctx_, k_);
-    Buffer<uint8_t> use_dev(ctx_, k_);
-    M_dev.upload(ctx_, M.data(), k_);
-    inv_dev.upload(ctx_, inv.data(), k_);
-    use_dev.upload(ctx_, use.data(), k_);
+    Buffer<float> M_dev(*cx_, k_), inv_dev(*cx_, k_);
+    Buffer<uint8_t> use_dev(*cx_, k_);
+    M_dev.upload(*cx_, M.data(), k_);
+    inv_dev.upload(*cx_, inv.data(), k_);
+    use_dev.upload(*cx_, use.data(), k_);
     Buffer<float> pdrf;  // the PDRF as a volume of its own only when the fields are asked for
-    if (fields_ && fields_->pdrf) pdrf = Buffer<float>(ctx_, size_t(n_));
-    pdrf_and_alive(ctx_, cc_.labels.data(), cc_.cc_bytes, n_, dbf_.data(), daf_.data(), M_dev.data(), inv_dev.data(),
+    if (fields_ && fields_->pdrf) pdrf = Buffer<float>(*cx_, size_t(n_));
+    pdrf_and_alive(*cx_, cc_.labels.data(), cc_.cc_bytes, n_, dbf_.data(), daf_.data(), M_dev.data(), inv_dev.data(),
                    use_dev.data(), squarings, float(o_.teasar.pdrf_scale), pdrf.size() ? pdrf.data() : nullptr, rec_.data());
-    Buffer<int> labels_dev(ctx_, std::max<size_t>(labels_.size(), 1));
-    labels_dev.upload(ctx_, labels_.data(), labels_.size());
+    Buffer<int> labels_dev(*cx_, std::max<size_t>(labels_.size(), 1));
+    labels_dev.upload(*cx_, labels_.data(), labels_.size());
     if (fields_) {
       fields_->count = count_;
       fields_->root.assign(root_.begin(), root_.end());
@@ -627,18 +636,18 @@ class Sample {
       fields_->dbf_max = dbf_max_;
       fields_->max_daf = max_daf_;
       fields_->active = active_;
-      ctx_.synchronize();
+      cx_->synchronize();
       if (fields_->dbf) BROOK_HIP(hipMemcpy(fields_->dbf, dbf_.data(), size_t(n_) * 4, hipMemcpyDeviceToHost));
       if (fields_->daf) BROOK_HIP(hipMemcpy(fields_->daf, daf_.data(), size_t(n_) * 4, hipMemcpyDeviceToHost));
       if (fields_->pdrf) BROOK_HIP(hipMemcpy(fields_->pdrf, pdrf.data(), size_t(n_) * 4, hipMemcpyDeviceToHost));
       if (fields_->stop) return finish_empty();
     }
     // the root becomes a rail (lockstep.cu:183); the fix_branching=False field floods from it
-    if (o_.fix_branching) root_rails(ctx_, rec_.data(), root_dev_.data(), labels_dev.data(), int(labels_.size()));
+    if (o_.fix_branching) root_rails(*cx_, rec_.data(), root_dev_.data(), labels_dev.data(), int(labels_.size()));
     const size_t fl = size_t(std::max<int64_t>(F_, 1));
-    order_ = Buffer<int>(ctx_, fl);
-    snap_a_ = Buffer<unsigned long long>(ctx_, fl);
-    target_order(ctx_, cc_.labels.data(), cc_.cc_bytes, n_, daf_.data(), F_, count_, order_.data());
+    order_ = Buffer<int>(*cx_, fl);
+    snap_a_ = Buffer<unsigned long long>(*cx_, fl);
+    target_order(*cx_, cc_.labels.data(), cc_.cc_bytes, n_, daf_.data(), F_, count_, order_.data());
     lap("prep_pdrf_order_ms");
 
     // manual targets and path budgets (lockstep.cu:116-143)
@@ -659,16 +668,16 @@ class Sample {
       maxp[id] = mcount[id] + acount[id] >= budget ? 0 : int(std::min<int64_t>(INT32_MAX, budget));
     }
     mflat.push_back(0);
-    mstart_dev_ = Buffer<int>(ctx_, k_);
-    mcount_dev_ = Buffer<int>(ctx_, k_);
-    maxp_dev_ = Buffer<int>(ctx_, k_);
-    mflat_dev_ = Buffer<int>(ctx_, mflat.size());
-    mstart_dev_.upload(ctx_, mstart.data(), k_);
-    mcount_dev_.upload(ctx_, mcount.data(), k_);
-    acount_dev_ = Buffer<int>(ctx_, k_);
-    acount_dev_.upload(ctx_, acount.data(), k_);
-    maxp_dev_.upload(ctx_, maxp.data(), k_);
-    mflat_dev_.upload(ctx_, mflat.data(), mflat.size());
+    mstart_dev_ = Buffer<int>(*cx_, k_);
+    mcount_dev_ = Buffer<int>(*cx_, k_);
+    maxp_dev_ = Buffer<int>(*cx_, k_);
+    mflat_dev_ = Buffer<int>(*cx_, mflat.size());
+    mstart_dev_.upload(*cx_, mstart.data(), k_);
+    mcount_dev_.upload(*cx_, mcount.data(), k_);
+    acount_dev_ = Buffer<int>(*cx_, k_);
+    acount_dev_.upload(*cx_, acount.data(), k_);
+    maxp_dev_.upload(*cx_, maxp.data(), k_);
+    mflat_dev_.upload(*cx_, mflat.data(), mflat.size());
 
     // trace buffers, sized by what the trace can emit (the reference grows its store mid-trace,
     // lockstep.cu:389-391): a label's paths number at most its ordered targets (alive voxels of
@@ -677,21 +686,21 @@ class Sample {
     // and the join's rail), or a soma's root; so records <= F + manual targets and the store
     // <= F + 2 * records. (A soma's parent walks do not stop at emitted vertices, so they may
     // repeat voxels: the store's overflow check stays and reports such a case.)
-    snap_b_ = Buffer<unsigned long long>(ctx_, fl);
+    snap_b_ = Buffer<unsigned long long>(*cx_, fl);
     const int64_t manual_targets = int64_t(mflat.size());
     const int64_t paths_bound = F_ + manual_targets + int64_t(labels_.size());
     if (paths_bound > INT32_MAX / 2) throw Unsupported("path records exceed the int32 store index");
     rec_cap_ = int(std::max<int64_t>(1, paths_bound));
     store_cap_ = std::max<long long>(1, F_ + 2 * paths_bound);
-    store_ = Buffer<int>(ctx_, size_t(store_cap_));
-    rec_label_ = Buffer<int>(ctx_, rec_cap_);
-    rec_len_ = Buffer<int>(ctx_, rec_cap_);
-    rec_start_ = Buffer<int>(ctx_, rec_cap_);
-    counters_ = Buffer<int>(ctx_, 16);
-    ejected_ = Buffer<int>(ctx_, 64);
-    counters_.clear(ctx_);
-    label_stats_ = Buffer<unsigned long long>(ctx_, size_t(kTraceProfileSlots) * k_);
-    label_stats_.clear(ctx_);
+    store_ = Buffer<int>(*cx_, size_t(store_cap_));
+    rec_label_ = Buffer<int>(*cx_, rec_cap_);
+    rec_len_ = Buffer<int>(*cx_, rec_cap_);
+    rec_start_ = Buffer<int>(*cx_, rec_cap_);
+    counters_ = Buffer<int>(*cx_, 16);
+    ejected_ = Buffer<int>(*cx_, 64);
+    counters_.clear(*cx_);
+    label_stats_ = Buffer<unsigned long long>(*cx_, size_t(kTraceProfileSlots) * k_);
+    label_stats_.clear(*cx_);
     if (free_radius_dev_.size()) {  // the somata's trimming radii and balls (lockstep.cu:137-141)
       std::vector<SomaLabel> table(k_);
       for (int id : labels_)
@@ -700,8 +709,8 @@ class Sample {
           table[id].ball = Ball{float(o_.teasar.soma_scale), float(o_.teasar.soma_constant)};
           table[id].max_paths = o_.teasar.max_paths;
         }
-      soma_dev_ = Buffer<SomaLabel>(ctx_, k_);
-      soma_dev_.upload(ctx_, table.data(), k_);
+      soma_dev_ = Buffer<SomaLabel>(*cx_, k_);
+      soma_dev_.upload(*cx_, table.data(), k_);
     }
   }
 
@@ -747,24 +756,24 @@ class Sample {
   void finish(Lap &lap, bool profile) {
     if (done_) return;
     for (auto &a : arenas_) a->sample->finish(lap, false);
-    std::vector<int> cnt = counters_.download(ctx_);
+    std::vector<int> cnt = counters_.download(*cx_);
     if (cnt[8] & 1) throw std::runtime_error("trace did not converge: a railroad or an invalidation exceeded its round budget");
     if (cnt[8] & 2) throw std::runtime_error("trace far list overflowed its capacity");
     if (cnt[3]) throw std::runtime_error("path store capacity exceeded");
     if (cnt[2]) throw Unsupported("a label has no reachable rail (the per-component fallback tracer is not part of this build)");
-    ctx_.stats["paths"] += cnt[1];
-    ctx_.stats["path_vertices"] += cnt[0];
+    cx_->stats["paths"] += cnt[1];
+    cx_->stats["path_vertices"] += cnt[0];
     uint64_t reached = 0, claimed = 0;
     std::memcpy(&reached, &cnt[4], 8);
     std::memcpy(&claimed, &cnt[6], 8);
-    ctx_.stats["railroad_reached_voxels"] += double(reached);
-    ctx_.stats["invalidated_voxels"] += double(claimed);
-    ctx_.stats["labels_traced"] += double(labels_.size());
+    cx_->stats["railroad_reached_voxels"] += double(reached);
+    cx_->stats["invalidated_voxels"] += double(claimed);
+    cx_->stats["labels_traced"] += double(labels_.size());
     if (profile) record_profile();
 
     // assembly: groups = input labels in first-skeleton-component order
     const int records = cnt[1];
-    std::vector<int> rl = rec_label_.download(ctx_, records), rn = rec_len_.download(ctx_, records);
+    std::vector<int> rl = rec_label_.download(*cx_, records), rn = rec_len_.download(*cx_, records);
     std::vector<uint8_t> has_edges(k_, 0);
     for (int r = 0; r < records; ++r)
       if (rn[r] >= 2) has_edges[rl[r]] = 1;
@@ -794,7 +803,7 @@ class Sample {
     }
     std::copy(o_.aniso, o_.aniso + 3, ai.aniso);
     std::copy(origin_, origin_ + 3, ai.origin);
-    result_ = assemble(ctx_, cc_.cc_bytes, ai);
+    result_ = assemble(*cx_, cc_.cc_bytes, ai);
     if (!arenas_.empty()) join_arenas(ai.group_of, main_edges, rl, rn, cnt[0]);
     lap("assembly_ms");
     done_ = true;
@@ -802,10 +811,10 @@ class Sample {
 
   void record_counts() const {
     if (!count_) return;
-    ctx_.stats["flood_voxels"] += double(flood_voxels_);
-    ctx_.stats["foreground_voxels"] += double(F_);
-    ctx_.stats["cc_bytes"] = cc_.cc_bytes;
-    ctx_.stats["components"] += double(count_);
+    cx_->stats["flood_voxels"] += double(flood_voxels_);
+    cx_->stats["foreground_voxels"] += double(F_);
+    cx_->stats["cc_bytes"] = cc_.cc_bytes;
+    cx_->stats["components"] += double(count_);
   }
 
  private:
@@ -848,8 +857,8 @@ class Sample {
     for (int r = 0; r < records; ++r) {
       if (rn[r] != 1 || !main_edges[size_t(rl[r])] || !arena_group[size_t(group_of[size_t(rl[r])])]) continue;
       if (starts.empty()) {
-        starts = rec_start_.download(ctx_, size_t(records));
-        store = store_.download(ctx_, size_t(store_used));
+        starts = rec_start_.download(*cx_, size_t(records));
+        store = store_.download(*cx_, size_t(store_used));
       }
       single[store[size_t(starts[size_t(r)])]] = rl[r];
     }
@@ -860,8 +869,8 @@ class Sample {
     };
     auto main_radius = [&](int64_t flat) {
       float r = 0;
-      BROOK_HIP(hipMemcpyAsync(&r, dbf_.data() + flat, 4, hipMemcpyDeviceToHost, ctx_.stream()));
-      ctx_.synchronize();
+      BROOK_HIP(hipMemcpyAsync(&r, dbf_.data() + flat, 4, hipMemcpyDeviceToHost, cx_->stream()));
+      cx_->synchronize();
       return r;
     };
     for (auto &a : arenas_) {  // ascending id
@@ -889,7 +898,7 @@ class Sample {
           out.push_back(in[j++]);
         } else {
           Vert &c = cur[i];
-          if (c.owner < 0) c.owner = component_at(ctx_, cc_, d_, voxel_of(c.p));
+          if (c.owner < 0) c.owner = component_at(*cx_, cc_, d_, voxel_of(c.p));
           map_cur[i] = map_in[j] = uint32_t(out.size());
           out.push_back(c.owner < in[j].owner ? c : in[j]);
           ++i, ++j;
@@ -924,9 +933,9 @@ class Sample {
 
   void record_profile() {  // per-label trace profile (device wall clock)
     int khz = 0;
-    BROOK_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx_.device()));
+    BROOK_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, cx_->device()));
     const double tick_ms = khz > 0 ? 1.0 / khz : 0.0;
-    std::vector<unsigned long long> ls = label_stats_.download(ctx_);
+    std::vector<unsigned long long> ls = label_stats_.download(*cx_);
     const size_t P = kTraceProfileSlots;
     double sums[kTraceProfileSlots] = {0}, worst = 0;
     size_t worst_id = 0;
@@ -934,7 +943,7 @@ class Sample {
       for (size_t j = 0; j < P; ++j) sums[j] += double(ls[P * size_t(id) + j]);
       if (double(ls[P * size_t(id)]) > worst) { worst = double(ls[P * size_t(id)]); worst_id = size_t(id); }
     }
-    auto &st = ctx_.stats;
+    auto &st = cx_->stats;
     const char *work[6] = {"railroad_phases", "far_scanned", "near_expanded", "invalidate_expanded",
                            "waiting_pulled", "bands"};
     for (int j = 0; j < 6; ++j) {
@@ -954,7 +963,7 @@ class Sample {
     st["slowest_railroad_rounds"] = double(ls[P * worst_id + 5]);
     st["slowest_invalidate_rounds"] = double(ls[P * worst_id + 6]);
     st["slowest_id"] = double(worst_id);
-    if (ctx_.kernel_timing)  // per-label detail for diagnostics
+    if (cx_->kernel_timing)  // per-label detail for diagnostics
       for (int id : labels_) {
         const std::string s = std::to_string(id);
         st["label_ms:" + s] = double(ls[P * id]) * tick_ms;
@@ -967,13 +976,14 @@ class Sample {
       }
   }
 
-  Context &ctx_;
+  Context *cx_;    // the context the sample's work runs on (a worker's during a group's first halves)
+  Context *main_;  // the caller's context (its auxiliary contexts serve the border and root jobs)
   // The border job of the first preamble half (joined in the second, or by the destructor).
   struct EventGuard {
     hipEvent_t e = nullptr;
     ~EventGuard() { if (e) (void)hipEventDestroy(e); }
   } cc_ready_;
-  bool early_borders_ = false;
+  bool early_borders_ = false, borders_ready_ = false;
   double border_ms_[2] = {0, 0};  // device and host milliseconds of the border job
   std::future<std::vector<std::vector<Point>>> border_job_;
   // The early root searches: the records and voxel lists they built, and per id the root found, or
@@ -1026,9 +1036,44 @@ bool larger_first(const LabelItem &a, const LabelItem &b) { return a.size > b.si
 // Runs a group of samples together: every preamble, one flood launch over all their labels
 // (largest first), every PDRF/order pass, one trace launch, every assembly.
 void run_group(Context &ctx, std::vector<std::unique_ptr<Sample>> &group, Lap &lap) {
-  // small groups (a serving batch) keep to one auxiliary context: fewer buffer pools to fill
-  const int rotate = group.size() >= 8 ? kAuxContexts : 1;
-  for (size_t i = 0; i < group.size(); ++i) group[i]->preamble_components(lap, int(i % rotate), group.size() == 1);
+  if (group.size() >= size_t(kWorkerMin)) {
+    // The first halves (components, EDT, border targets; latency-bound for small volumes) of
+    // different samples overlap on kWorkers worker contexts, each a host thread with its own
+    // stream and buffer pool; the second halves then run here, on this context, in sample order.
+    for (int w = 0; w < kWorkers; ++w) ctx.aux(kWorkerBase + w);  // made here: the threads only look them up
+    ctx.synchronize();  // the inputs are complete before other streams read them
+    std::vector<std::future<void>> workers;
+    for (int w = 0; w < kWorkers; ++w)
+      workers.push_back(std::async(std::launch::async, [&group, &ctx, w] {
+        Context &wc = ctx.aux(kWorkerBase + w);
+        std::lock_guard<std::mutex> turn(wc.mutex());
+        wc.activate();
+        Lap wl{wc};
+        for (size_t i = size_t(w); i < group.size(); i += kWorkers) {
+          group[i]->use_context(wc);
+          group[i]->preamble_components(wl, -1, false);
+        }
+        wc.synchronize();
+      }));
+    std::exception_ptr err;
+    for (auto &f : workers) {
+      try {
+        f.get();
+      } catch (...) {
+        if (!err) err = std::current_exception();
+      }
+    }
+    for (auto &s : group) s->use_context(ctx);
+    for (int w = 0; w < kWorkers; ++w) {  // the workers' stage times and counts, summed over workers
+      Context &wc = ctx.aux(kWorkerBase + w);
+      for (const auto &kv : wc.stats) ctx.stats[kv.first] += kv.second;
+      wc.stats.clear();
+    }
+    if (err) std::rethrow_exception(err);
+    lap("prep_first_halves_ms");
+  } else {
+    for (auto &s : group) s->preamble_components(lap, kBorderAux, group.size() == 1);
+  }
   for (auto &s : group) s->preamble_labels(lap);
   std::vector<Sample *> all;  // the samples and their private arenas
   for (auto &s : group) s->collect(all);

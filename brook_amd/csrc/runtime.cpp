@@ -82,6 +82,11 @@ Context &Context::aux(int i) {
 }
 
 void Context::trim() {
+  for (auto &a : aux_)  // the auxiliary pools too, those no thread is using
+    if (a && a->mutex().try_lock()) {
+      a->trim();
+      a->mutex().unlock();
+    }
   if (!cache_.empty()) (void)hipStreamSynchronize(stream_);
   for (auto &kv : cache_) (void)hipFree(kv.second);
   cache_.clear();
