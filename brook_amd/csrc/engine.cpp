@@ -120,8 +120,13 @@ class Sample {
 
   bool done() const { return done_; }
   Packed take_result() { return std::move(result_); }
-  // The context the sample's next work runs on (its buffers stay in the pool they came from).
+  // The context the sample's next work runs on (its buffers stay in the pool they came from);
+  // use_context_all also moves its private arenas' samples.
   void use_context(Context &c) { cx_ = &c; }
+  void use_context_all(Context &c) {
+    cx_ = &c;
+    for (auto &a : arenas_) a->sample->use_context_all(c);
+  }
   // This sample and its arenas' samples, which share its flood and trace launches.
   void collect(std::vector<Sample *> &all) {
     all.push_back(this);
@@ -1033,43 +1038,62 @@ class Sample {
 
 bool larger_first(const LabelItem &a, const LabelItem &b) { return a.size > b.size; }
 
-// Runs a group of samples together: every preamble, one flood launch over all their labels
-// (largest first), every PDRF/order pass, one trace launch, every assembly.
-void run_group(Context &ctx, std::vector<std::unique_ptr<Sample>> &group, Lap &lap) {
-  if (group.size() >= size_t(kWorkerMin)) {
-    // The first halves (components, EDT, border targets; latency-bound for small volumes) of
-    // different samples overlap on kWorkers worker contexts, each a host thread with its own
-    // stream and buffer pool; the second halves then run here, on this context, in sample order.
-    for (int w = 0; w < kWorkers; ++w) ctx.aux(kWorkerBase + w);  // made here: the threads only look them up
-    ctx.synchronize();  // the inputs are complete before other streams read them
-    std::vector<std::future<void>> workers;
-    for (int w = 0; w < kWorkers; ++w)
-      workers.push_back(std::async(std::launch::async, [&group, &ctx, w] {
-        Context &wc = ctx.aux(kWorkerBase + w);
-        std::lock_guard<std::mutex> turn(wc.mutex());
-        wc.activate();
-        Lap wl{wc};
-        for (size_t i = size_t(w); i < group.size(); i += kWorkers) {
-          group[i]->use_context(wc);
-          group[i]->preamble_components(wl, -1, false);
-        }
-        wc.synchronize();
-      }));
-    std::exception_ptr err;
-    for (auto &f : workers) {
-      try {
-        f.get();
-      } catch (...) {
-        if (!err) err = std::current_exception();
-      }
-    }
-    for (auto &s : group) s->use_context(ctx);
-    for (int w = 0; w < kWorkers; ++w) {  // the workers' stage times and counts, summed over workers
+// Runs fn(i, context, lap) for every i < n on kWorkers host threads, worker w taking i = w,
+// w + kWorkers, ... on auxiliary context kWorkerBase + w (each a stream and buffer pool of its
+// own); adds the workers' stage times to ctx.stats and rethrows the first error once all ended.
+// Blocks a worker frees from other pools than its own (this context's, another worker's) go back
+// once every worker's stream is idle.
+template <class F> void on_workers(Context &ctx, size_t n, F fn) {
+  std::vector<std::unique_ptr<FreeHold>> holds;
+  for (int w = 0; w < kWorkers; ++w) {  // made here: the threads only look them up
+    Context &wc = ctx.aux(kWorkerBase + w);
+    wc.kernel_timing = ctx.kernel_timing;
+    holds.push_back(std::make_unique<FreeHold>(wc));
+  }
+  ctx.synchronize();  // what this context's stream did is complete before the workers read it
+  std::vector<std::future<void>> workers;
+  for (int w = 0; w < kWorkers; ++w)
+    workers.push_back(std::async(std::launch::async, [&ctx, &fn, &holds, n, w] {
       Context &wc = ctx.aux(kWorkerBase + w);
-      for (const auto &kv : wc.stats) ctx.stats[kv.first] += kv.second;
-      wc.stats.clear();
+      std::lock_guard<std::mutex> turn(wc.mutex());
+      wc.activate();
+      FreeHoldScope held(*holds[size_t(w)]);
+      Lap wl{wc};
+      for (size_t i = size_t(w); i < n; i += kWorkers) fn(i, wc, wl);
+      wc.synchronize();
+    }));
+  std::exception_ptr err;
+  for (auto &f : workers) {
+    try {
+      f.get();
+    } catch (...) {
+      if (!err) err = std::current_exception();
     }
-    if (err) std::rethrow_exception(err);
+  }
+  for (int w = 0; w < kWorkers; ++w) {
+    Context &wc = ctx.aux(kWorkerBase + w);
+    if (err) (void)hipStreamSynchronize(wc.stream());  // a worker that threw left its stream running
+    for (const auto &kv : wc.stats) ctx.stats[kv.first] += kv.second;  // stage times and counts, summed
+    wc.stats.clear();
+  }
+  for (auto &h : holds) h->release();
+  if (err) std::rethrow_exception(err);
+}
+
+// Runs a group of samples together: every preamble, one flood launch over all their labels
+// (largest first), every PDRF/order pass, one trace launch, every assembly. In a group of at
+// least kWorkerMin samples the per-sample stages (first preamble halves, PDRF/order, assembly;
+// latency-bound for small volumes: kernel launches and host reads) of different samples overlap
+// on the worker contexts; the launches shared by the group and the second preamble halves run on
+// this context.
+void run_group(Context &ctx, std::vector<std::unique_ptr<Sample>> &group, Lap &lap) {
+  const bool workers = group.size() >= size_t(kWorkerMin);
+  if (workers) {
+    on_workers(ctx, group.size(), [&group](size_t i, Context &wc, Lap &wl) {
+      group[i]->use_context(wc);
+      group[i]->preamble_components(wl, -1, false);  // border targets in place
+    });
+    for (auto &s : group) s->use_context(ctx);
     lap("prep_first_halves_ms");
   } else {
     for (auto &s : group) s->preamble_components(lap, kBorderAux, group.size() == 1);
@@ -1085,7 +1109,16 @@ void run_group(Context &ctx, std::vector<std::unique_ptr<Sample>> &group, Lap &l
   for (Sample *s : all) jobs.push_back(s->done() ? FloodJob{} : s->flood_job());
   prepare_fields(ctx, jobs, items);
   lap("prep_floods_ms");
-  for (Sample *s : all) s->after_floods(lap);
+  if (workers) {
+    on_workers(ctx, all.size(), [&all](size_t i, Context &wc, Lap &wl) {
+      all[i]->use_context(wc);
+      all[i]->after_floods(wl);
+    });
+    for (Sample *s : all) s->use_context(ctx);
+    lap("prep_after_floods_ms");
+  } else {
+    for (Sample *s : all) s->after_floods(lap);
+  }
   items.clear();
   for (size_t i = 0; i < all.size(); ++i) all[i]->items(int(i), items);
   std::stable_sort(items.begin(), items.end(), larger_first);
@@ -1095,8 +1128,17 @@ void run_group(Context &ctx, std::vector<std::unique_ptr<Sample>> &group, Lap &l
   for (size_t i = 0; i < all.size(); ++i) all[i]->soma_items(int(i), somata);
   trace_all(ctx, args, items, somata);
   lap("trace_ms");
-  const bool profile = group.size() == 1;
-  for (auto &s : group) s->finish(lap, profile);
+  if (workers) {
+    on_workers(ctx, group.size(), [&group](size_t i, Context &wc, Lap &wl) {
+      group[i]->use_context_all(wc);  // its arenas assemble in its finish
+      group[i]->finish(wl, false);
+    });
+    for (auto &s : group) s->use_context_all(ctx);
+    lap("assembly_wall_ms");
+  } else {
+    const bool profile = group.size() == 1;
+    for (auto &s : group) s->finish(lap, profile);
+  }
 }
 
 }  // namespace

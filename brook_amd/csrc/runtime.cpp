@@ -2,12 +2,31 @@
 
 namespace brook {
 
+namespace {
+thread_local FreeHold *t_hold = nullptr;  // the calling thread's installed hold, if any
+}  // namespace
+
 void Block::release() {
-  if (ptr_ && ctx_) ctx_->give_back(ptr_, bytes_);
+  if (ptr_ && ctx_) {
+    if (t_hold && ctx_ != t_hold->own_) t_hold->kept_.emplace_back(ctx_, ptr_, bytes_);
+    else ctx_->give_back(ptr_, bytes_);
+  }
   ptr_ = nullptr;
   ctx_ = nullptr;
   bytes_ = 0;
 }
+
+void FreeHold::release() {
+  std::vector<Block> kept;
+  kept.swap(kept_);
+  FreeHold *installed = t_hold;
+  t_hold = nullptr;  // the kept blocks go to their own caches now
+  kept.clear();
+  t_hold = installed;
+}
+
+FreeHoldScope::FreeHoldScope(FreeHold &hold) : previous_(t_hold) { t_hold = &hold; }
+FreeHoldScope::~FreeHoldScope() { t_hold = previous_; }
 
 Context::Context(int device) : device_(device) {
   int count = 0;

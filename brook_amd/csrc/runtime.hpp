@@ -56,6 +56,36 @@ class Block {
   size_t bytes_ = 0;
 };
 
+// While a host thread has a FreeHold installed (FreeHoldScope), the blocks it releases that belong
+// to other contexts' pools than the hold's own are kept back instead of going to their caches:
+// those pools may serve other threads meanwhile, and their streams could reuse a block that this
+// thread's stream still reads. release() gives them back; call it once no stream uses them and no
+// other thread uses their pools.
+class FreeHold {
+ public:
+  explicit FreeHold(Context &own) : own_(&own) {}
+  FreeHold(const FreeHold &) = delete;
+  FreeHold &operator=(const FreeHold &) = delete;
+  ~FreeHold() { release(); }
+  void release();
+
+ private:
+  friend class Block;
+  Context *own_;
+  std::vector<Block> kept_;
+};
+
+class FreeHoldScope {
+ public:
+  explicit FreeHoldScope(FreeHold &hold);
+  FreeHoldScope(const FreeHoldScope &) = delete;
+  FreeHoldScope &operator=(const FreeHoldScope &) = delete;
+  ~FreeHoldScope();
+
+ private:
+  FreeHold *previous_;
+};
+
 template <class T> class Buffer {
  public:
   Buffer() = default;

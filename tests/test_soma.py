@@ -365,8 +365,14 @@ def test_gpu_soma_private_arena_batch_targets_and_graph(brook):
     vols = [_arena_case("staircase"), shapes.soma_cells(), _arena_case("padded")]
     batch = brook.skeletonize_batch(vols, teasar_params=P, dust_threshold=0, fix_borders=True, output="packed",
                                     progress=False)
-    for i, v in enumerate(vols):
-        _same(batch[i], oracle.skeletonize(v, P, (1, 1, 1), 0, fix_borders=True, threads=4), i)
+    wants = [oracle.skeletonize(v, P, (1, 1, 1), 0, fix_borders=True, threads=4) for v in vols]
+    for i in range(len(vols)):
+        _same(batch[i], wants[i], i)
+    # nine samples (three with arenas): the per-sample stages run on the engine's worker threads
+    big = brook.skeletonize_batch(vols * 3, teasar_params=P, dust_threshold=0, fix_borders=True, output="packed",
+                                  progress=False)
+    for i in range(len(big)):
+        _same(big[i], wants[i % len(vols)], ("nine", i))
     labels = _arena_case("padded")
     before, after = [[29, 29, 6]], [[29, 6, 29], [29, 29, 50]]
     assert all(labels[tuple(p)] == 1 for p in before + after)

@@ -142,8 +142,14 @@ def test_gpu_targets_equal_oracle(brook, fix_branching):
         assert_equal_packed(got, oracle_run(sample, b, a, fix_branching))
     batch = brook.skeletonize_batch(samples, teasar_params=PARAMS, extra_targets_before=before,
                                     extra_targets_after=after, output="packed", **options)
-    for i, (sample, b, a) in enumerate(zip(samples, before, after)):
-        assert_equal_packed(batch[i].as_dict(), oracle_run(sample, b, a, fix_branching))
+    wants = [oracle_run(sample, b, a, fix_branching) for sample, b, a in zip(samples, before, after)]
+    for i in range(len(samples)):
+        assert_equal_packed(batch[i].as_dict(), wants[i])
+    reps = -(-8 // len(samples))  # at least eight samples: the per-sample stages run on worker threads
+    big = brook.skeletonize_batch(samples * reps, teasar_params=PARAMS, extra_targets_before=before * reps,
+                                  extra_targets_after=after * reps, output="packed", **options)
+    for i in range(len(big)):
+        assert_equal_packed(big[i].as_dict(), wants[i % len(samples)])
 
 
 @pytest.mark.gpu

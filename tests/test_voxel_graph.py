@@ -323,6 +323,10 @@ def test_gpu_batch_with_graphs_equals_separate_calls(brook, fix_branching):
     batch = brook.skeletonize_batch(samples, voxel_graph=gs, output="packed", **opts)
     for i in range(len(samples)):
         _assert_packed_equal(batch[i], single[i].as_dict(), f"sample {i}")
+    reps = -(-8 // len(samples))  # at least eight samples: the per-sample stages run on worker threads
+    big = brook.skeletonize_batch(samples * reps, voxel_graph=gs * reps, output="packed", **opts)
+    for i in range(len(big)):
+        _assert_packed_equal(big[i], single[i % len(samples)].as_dict(), f"repeated sample {i}")
     mixed = brook.skeletonize_batch(samples, voxel_graph=[gs[0], None, gs[2]], output="packed", **opts)
     plain = brook.skeletonize(samples[1], output="packed", progress=False, **opts)
     _assert_packed_equal(mixed[1], plain.as_dict(), "no graph")
