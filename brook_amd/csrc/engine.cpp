@@ -1082,23 +1082,23 @@ template <class F> void on_workers(Context &ctx, size_t n, F fn) {
 
 // Runs a group of samples together: every preamble, one flood launch over all their labels
 // (largest first), every PDRF/order pass, one trace launch, every assembly. In a group of at
-// least kWorkerMin samples the per-sample stages (first preamble halves, PDRF/order, assembly;
-// latency-bound for small volumes: kernel launches and host reads) of different samples overlap
-// on the worker contexts; the launches shared by the group and the second preamble halves run on
-// this context.
+// least kWorkerMin samples the per-sample stages (preambles, PDRF/order, assembly; latency-bound
+// for small volumes: kernel launches and host reads) of different samples overlap on the worker
+// contexts; the launches shared by the group run on this context.
 void run_group(Context &ctx, std::vector<std::unique_ptr<Sample>> &group, Lap &lap) {
   const bool workers = group.size() >= size_t(kWorkerMin);
   if (workers) {
     on_workers(ctx, group.size(), [&group](size_t i, Context &wc, Lap &wl) {
       group[i]->use_context(wc);
       group[i]->preamble_components(wl, -1, false);  // border targets in place
+      group[i]->preamble_labels(wl);                 // arenas made here run their preambles here too
     });
-    for (auto &s : group) s->use_context(ctx);
-    lap("prep_first_halves_ms");
+    for (auto &s : group) s->use_context_all(ctx);
+    lap("prep_preambles_ms");
   } else {
     for (auto &s : group) s->preamble_components(lap, kBorderAux, group.size() == 1);
+    for (auto &s : group) s->preamble_labels(lap);
   }
-  for (auto &s : group) s->preamble_labels(lap);
   std::vector<Sample *> all;  // the samples and their private arenas
   for (auto &s : group) s->collect(all);
   for (Sample *s : all) s->record_counts();
