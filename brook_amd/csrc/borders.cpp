@@ -1,7 +1,8 @@
 // Border targets (fix_borders): for every volume face, 2-D components of the face's component
-// ids and their 2-D EDT are computed on the GPU with the volume kernels (a face is a (fy, fx, 1)
-// volume); the farthest-pixel pick with its tie keys and the insertion-ordered point sets are
-// host work, as in the reference's serial plan (borders.cu:53-116, point_set.hpp:11-55).
+// ids and their 2-D EDT are computed on the GPU with the volume kernels (the six faces on one
+// canvas: one CCL, one EDT per face pair); the farthest-pixel pick with its tie keys and the
+// insertion-ordered point sets are host work, as in the reference's serial plan (borders.cu:53-116,
+// point_set.hpp:11-55).
 #include <algorithm>
 #include <array>
 #include <chrono>
@@ -163,18 +164,16 @@ std::vector<FacePick> face_picks(const std::vector<uint8_t> &raw, const std::vec
 std::vector<std::vector<Point>> border_targets(Context &ctx, const void *cc, int cc_bytes, Dims d, int64_t count,
                                                const float aniso[3]) {
   std::vector<TupleSet> sets(size_t(count) + 1);
+  auto items = [&sets]() {
+    std::vector<std::vector<Point>> out;
+    out.reserve(sets.size());
+    for (auto &s : sets) out.push_back(s.items());
+    return out;
+  };
   const int axes[3][3] = {{0, 1, 2}, {0, 2, 1}, {1, 2, 0}};
   const int dims[3] = {d.sx, d.sy, d.sz};
   using Clock = std::chrono::steady_clock;
   auto ms = [](Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
-  // The device work of the faces runs in order on the context's stream; each face's host pass runs
-  // on a thread of its own meanwhile. The point sets are filled in face order afterwards.
-  struct Face {
-    std::future<std::vector<FacePick>> picks;
-    std::vector<int64_t> mapping;  // 2-D component -> volume component id
-    int a, b, c, fixed;
-  };
-  std::vector<Face> faces;
   // The faces' CCL and EDT kernels share the volume kernels' timer names; their time belongs to
   // this stage, so per-kernel timing pauses here (the stage's own laps still count it).
   struct TimingPause {
@@ -184,54 +183,102 @@ std::vector<std::vector<Point>> border_targets(Context &ctx, const void *cc, int
     ~TimingPause() { ctx.kernel_timing = was; }
   } pause(ctx);
   const auto t0 = Clock::now();
-  for (int face = 0; face < 6; ++face) {
-    const int a = axes[face / 2][0], b = axes[face / 2][1], c = axes[face / 2][2];
-    const int fixed = face % 2 ? dims[c] - 1 : 0, fx = dims[a], fy = dims[b];
-    const int64_t np = int64_t(fx) * fy;
-    Buffer<uint8_t> plane(ctx, size_t(np) * cc_bytes);
-    extract_face(ctx, cc, cc_bytes, d, a, b, c, fixed, plane.data());
-    LabelVolume pv;
-    pv.data = plane.data();
-    pv.bytes = cc_bytes;
-    pv.dims = Dims{fy, fx, 1};
-    Components c2 = connected_components(ctx, pv);
-    if (c2.count == 0) continue;
-    Buffer<float> dist(ctx, size_t(np));
+  // All six faces on one canvas, zero rows and columns between them: one 2-D CCL labels every
+  // face's components at once. Each face pair (same axes, same anisotropy) is a contiguous block
+  // of columns; its EDT sees the same lines as a face's own EDT would (x runs only by their
+  // offsets, y lines from the face's first column), so the distances are the per-face ones.
+  FaceCanvas fc{};
+  int col = 0;
+  for (int o = 0; o < 3; ++o) {
+    fc.a[o] = axes[o][0];
+    fc.b[o] = axes[o][1];
+    fc.c[o] = axes[o][2];
+    fc.col0[o] = col;
+    col += dims[fc.a[o]] + 1;
+    fc.rows = std::max(fc.rows, 2 * dims[fc.b[o]] + 1);
+  }
+  fc.cols = col - 1;
+  const size_t np = size_t(fc.rows) * size_t(fc.cols);
+  Buffer<uint8_t> canvas(ctx, np * size_t(cc_bytes));
+  extract_faces(ctx, cc, cc_bytes, d, fc, canvas.data());
+  LabelVolume pv;
+  pv.data = canvas.data();
+  pv.bytes = cc_bytes;
+  pv.dims = Dims{fc.rows, fc.cols, 1};
+  Components c2 = connected_components(ctx, pv);
+  if (c2.count == 0) return items();
+  const int bytes = c2.cc_bytes;
+  Buffer<float> dist(ctx, np);
+  for (int o = 0; o < 3; ++o) {
+    const int a = fc.a[o], b = fc.b[o];
+    const size_t at = size_t(fc.col0[o]) * size_t(fc.rows);
     const float pa[3] = {aniso[b], aniso[a], 1.0f};
-    edt(ctx, c2.labels.data(), c2.cc_bytes, pv.dims, pa, true, 2, dist.data());
-    std::vector<uint8_t> raw = c2.labels.download(ctx, size_t(np) * c2.cc_bytes);
-    std::vector<float> dv = dist.download(ctx, size_t(np));
-    const size_t k = size_t(c2.count) + 1;
-    const float wx = aniso[a], wy = aniso[b];
-    const int bytes = c2.cc_bytes;
-    Face f;
-    f.picks = std::async(std::launch::async, [raw = std::move(raw), dv = std::move(dv), k, fx, fy, wx, wy, bytes]() {
-      if (bytes == 1) return face_picks<uint8_t>(raw, dv, k, fx, fy, wx, wy);
-      if (bytes == 2) return face_picks<uint16_t>(raw, dv, k, fx, fy, wx, wy);
-      return face_picks<uint32_t>(raw, dv, k, fx, fy, wx, wy);
-    });
-    f.mapping.assign(c2.mapping.begin(), c2.mapping.end());
-    f.a = a;
-    f.b = b;
-    f.c = c;
-    f.fixed = fixed;
-    faces.push_back(std::move(f));
+    edt(ctx, c2.labels.data() + at * size_t(bytes), bytes, Dims{fc.rows, dims[a], 1}, pa, true, 2, dist.data() + at);
+  }
+  // each face's pixels, y + fy * x, gathered from its rectangle of the canvas
+  struct Face {
+    std::vector<uint8_t> raw;
+    std::vector<float> dv;
+    int a, b, c, fixed, fx, fy;
+  };
+  std::vector<Face> faces(6);
+  size_t total = 0;
+  for (int f = 0; f < 6; ++f) {
+    Face &face = faces[size_t(f)];
+    const int o = f / 2;
+    face.a = fc.a[o];
+    face.b = fc.b[o];
+    face.c = fc.c[o];
+    face.fixed = f % 2 ? dims[face.c] - 1 : 0;
+    face.fx = dims[face.a];
+    face.fy = dims[face.b];
+    total += size_t(face.fx) * size_t(face.fy);
+  }
+  Buffer<uint8_t> face_ids(ctx, total * size_t(bytes));
+  Buffer<float> face_dist(ctx, total);
+  gather_faces(ctx, c2.labels.data(), bytes, dist.data(), d, fc, face_ids.data(), face_dist.data());
+  const std::vector<uint8_t> all_ids = face_ids.download(ctx, total * size_t(bytes));
+  const std::vector<float> all_dist = face_dist.download(ctx, total);
+  size_t at = 0;
+  for (Face &face : faces) {
+    const size_t pixels = size_t(face.fx) * size_t(face.fy);
+    face.raw.assign(all_ids.begin() + ptrdiff_t(at * size_t(bytes)), all_ids.begin() + ptrdiff_t((at + pixels) * size_t(bytes)));
+    face.dv.assign(all_dist.begin() + ptrdiff_t(at), all_dist.begin() + ptrdiff_t(at + pixels));
+    at += pixels;
   }
   const auto t1 = Clock::now();
-  ctx.stats["borders_device_ms"] += ms(t0, t1);  // face extraction, 2-D CCL and EDT, downloads
-  for (Face &f : faces)
-    for (const FacePick &q : f.picks.get()) {
+  ctx.stats["borders_device_ms"] += ms(t0, t1);  // canvas, 2-D CCL and EDTs, downloads
+  // The host passes, the larger faces on threads of their own; the point sets are filled in face
+  // order afterwards.
+  const size_t k = size_t(c2.count) + 1;
+  auto pass = [&faces, &aniso, k, bytes](int f) {
+    const Face &face = faces[size_t(f)];
+    const float wx = aniso[face.a], wy = aniso[face.b];
+    if (bytes == 1) return face_picks<uint8_t>(face.raw, face.dv, k, face.fx, face.fy, wx, wy);
+    if (bytes == 2) return face_picks<uint16_t>(face.raw, face.dv, k, face.fx, face.fy, wx, wy);
+    return face_picks<uint32_t>(face.raw, face.dv, k, face.fx, face.fy, wx, wy);
+  };
+  constexpr size_t kThreadPixels = size_t(1) << 16;
+  std::vector<std::future<std::vector<FacePick>>> jobs(6);
+  std::vector<std::vector<FacePick>> picks(6);
+  for (int f = 0; f < 6; ++f)
+    if (faces[size_t(f)].dv.size() >= kThreadPixels) jobs[size_t(f)] = std::async(std::launch::async, pass, f);
+  for (int f = 0; f < 6; ++f)
+    if (!jobs[size_t(f)].valid()) picks[size_t(f)] = pass(f);
+  for (int f = 0; f < 6; ++f)
+    if (jobs[size_t(f)].valid()) picks[size_t(f)] = jobs[size_t(f)].get();
+  for (int f = 0; f < 6; ++f) {
+    const Face &face = faces[size_t(f)];
+    for (const FacePick &q : picks[size_t(f)]) {
       Point p{};
-      p[f.a] = q.x;
-      p[f.b] = q.y;
-      p[f.c] = f.fixed;
-      sets.at(size_t(f.mapping[size_t(q.id)])).add(p);
+      p[face.a] = q.x;
+      p[face.b] = q.y;
+      p[face.c] = face.fixed;
+      sets.at(size_t(c2.mapping[size_t(q.id)])).add(p);
     }
-  ctx.stats["borders_host_ms"] += ms(t1, Clock::now());  // what the host passes add after the device work
-  std::vector<std::vector<Point>> out;
-  out.reserve(sets.size());
-  for (auto &s : sets) out.push_back(s.items());
-  return out;
+  }
+  ctx.stats["borders_host_ms"] += ms(t1, Clock::now());  // the host passes
+  return items();
 }
 
 }  // namespace brook

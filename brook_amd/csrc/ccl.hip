@@ -229,15 +229,47 @@ __global__ void __launch_bounds__(256) analyze_runs(const CC *cc, Dims d, int n,
 }
 
 template <class CC>
-__global__ void __launch_bounds__(256) face_copy(const CC *cc, Dims d, int a, int b, int c, int fixed, int fx, int fy,
-                                                 CC *plane) {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= fx * fy) return;
-  int p[3];
-  p[a] = i / fy;
-  p[b] = i % fy;
-  p[c] = fixed;
-  plane[i] = cc[flat(p[0], p[1], p[2], d)];
+__global__ void __launch_bounds__(256) faces_copy(const CC *cc, Dims d, FaceCanvas fc, CC *canvas) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (long long)fc.rows * fc.cols) return;
+  const int r = int(i % fc.rows), col = int(i / fc.rows);
+  const int dims[3] = {d.sx, d.sy, d.sz};
+  CC v = CC(0);
+  for (int o = 0; o < 3; ++o) {
+    const int x = col - fc.col0[o], fy = dims[fc.b[o]];
+    if (x < 0 || x >= dims[fc.a[o]]) continue;
+    const bool far = r > fy;
+    const int y = far ? r - fy - 1 : r;
+    if (r != fy && y < fy) {
+      int p[3];
+      p[fc.a[o]] = x;
+      p[fc.b[o]] = y;
+      p[fc.c[o]] = far ? dims[fc.c[o]] - 1 : 0;
+      v = cc[flat(p[0], p[1], p[2], d)];
+    }
+    break;
+  }
+  canvas[i] = v;
+}
+
+template <class CC>
+__global__ void __launch_bounds__(256) faces_gather(const CC *ids, const float *dist, Dims d, FaceCanvas fc,
+                                                   long long total, CC *face_ids, float *face_dist) {
+  const long long j = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (j >= total) return;
+  const int dims[3] = {d.sx, d.sy, d.sz};
+  long long k = j;
+  int f = 0;
+  for (; f < 5; ++f) {
+    const long long pixels = (long long)dims[fc.a[f / 2]] * dims[fc.b[f / 2]];
+    if (k < pixels) break;
+    k -= pixels;
+  }
+  const int o = f / 2, fy = dims[fc.b[o]];
+  const long long x = k / fy, y = k % fy;
+  const long long at = ((long long)fc.col0[o] + x) * fc.rows + (f % 2 ? fy + 1 : 0) + y;
+  face_ids[j] = ids[at];
+  face_dist[j] = dist[at];
 }
 
 unsigned blocks_for(int64_t n, int per) { return unsigned((n + per - 1) / per); }
@@ -458,19 +490,34 @@ std::vector<Box> analyze(Context &ctx, const Components &cc, Dims d) {
   return boxes;
 }
 
-void extract_face(Context &ctx, const void *cc, int cc_bytes, Dims d, int a, int b, int c, int fixed, void *plane) {
-  int dims[3] = {d.sx, d.sy, d.sz};
-  int fx = dims[a], fy = dims[b];
-  unsigned g = blocks_for(int64_t(fx) * fy, 256);
-  if (cc_bytes == 1)
-    face_copy<uint8_t><<<g, 256, 0, ctx.stream()>>>(static_cast<const uint8_t *>(cc), d, a, b, c, fixed, fx, fy,
-                                                    static_cast<uint8_t *>(plane));
-  else if (cc_bytes == 2)
-    face_copy<uint16_t><<<g, 256, 0, ctx.stream()>>>(static_cast<const uint16_t *>(cc), d, a, b, c, fixed, fx, fy,
-                                                     static_cast<uint16_t *>(plane));
+void gather_faces(Context &ctx, const void *ids, int id_bytes, const float *dist, Dims d, const FaceCanvas &fc,
+                  void *face_ids, float *face_dist) {
+  const int dims[3] = {d.sx, d.sy, d.sz};
+  long long total = 0;
+  for (int o = 0; o < 3; ++o) total += 2LL * dims[fc.a[o]] * dims[fc.b[o]];
+  const unsigned g = blocks_for(total, 256);
+  if (id_bytes == 1)
+    faces_gather<uint8_t><<<g, 256, 0, ctx.stream()>>>(static_cast<const uint8_t *>(ids), dist, d, fc, total,
+                                                       static_cast<uint8_t *>(face_ids), face_dist);
+  else if (id_bytes == 2)
+    faces_gather<uint16_t><<<g, 256, 0, ctx.stream()>>>(static_cast<const uint16_t *>(ids), dist, d, fc, total,
+                                                        static_cast<uint16_t *>(face_ids), face_dist);
   else
-    face_copy<uint32_t><<<g, 256, 0, ctx.stream()>>>(static_cast<const uint32_t *>(cc), d, a, b, c, fixed, fx, fy,
-                                                     static_cast<uint32_t *>(plane));
+    faces_gather<uint32_t><<<g, 256, 0, ctx.stream()>>>(static_cast<const uint32_t *>(ids), dist, d, fc, total,
+                                                        static_cast<uint32_t *>(face_ids), face_dist);
+  BROOK_LAUNCHED();
+}
+
+void extract_faces(Context &ctx, const void *cc, int cc_bytes, Dims d, const FaceCanvas &fc, void *canvas) {
+  const unsigned g = blocks_for(int64_t(fc.rows) * fc.cols, 256);
+  if (cc_bytes == 1)
+    faces_copy<uint8_t><<<g, 256, 0, ctx.stream()>>>(static_cast<const uint8_t *>(cc), d, fc, static_cast<uint8_t *>(canvas));
+  else if (cc_bytes == 2)
+    faces_copy<uint16_t><<<g, 256, 0, ctx.stream()>>>(static_cast<const uint16_t *>(cc), d, fc,
+                                                      static_cast<uint16_t *>(canvas));
+  else
+    faces_copy<uint32_t><<<g, 256, 0, ctx.stream()>>>(static_cast<const uint32_t *>(cc), d, fc,
+                                                      static_cast<uint32_t *>(canvas));
   BROOK_LAUNCHED();
 }
 

@@ -67,7 +67,20 @@ void edt(Context &ctx, const void *cc, int cc_bytes, Dims dims, const float anis
 // Exclusive prefix sums on the context's stream (int and 64-bit element forms).
 void exclusive_sum(Context &ctx, const int *in, int *out, long long n);
 void exclusive_sum(Context &ctx, const long long *in, long long *out, long long n);
-// Label-dependent copy of one volume face into a plane of shape (fy, fx, 1) (borders.cu:14-20).
-void extract_face(Context &ctx, const void *cc, int cc_bytes, Dims dims, int a, int b, int c, int fixed, void *plane);
+// The six faces of a volume on one canvas of `rows` x `cols` pixels (rows along the fast axis;
+// the copy of each face is borders.cu:14-20's). Face pair o spans the volume's axes a[o] and b[o]
+// at c[o] = 0 and at c[o] = dims[c[o]] - 1, in columns col0[o] .. col0[o] + dims[a[o]] - 1: the
+// near face in rows 0 .. dims[b[o]] - 1, the far face in rows dims[b[o]] + 1 .. 2 * dims[b[o]].
+// Face pixel (x along a, y along b) sits at row y (near) or dims[b] + 1 + y (far) of column
+// col0 + x; every other pixel is 0, so no two faces touch.
+struct FaceCanvas {
+  int a[3], b[3], c[3], col0[3];
+  int rows, cols;
+};
+void extract_faces(Context &ctx, const void *cc, int cc_bytes, Dims dims, const FaceCanvas &fc, void *canvas);
+// The six faces' ids (`id_bytes` wide) and distances from the canvas, face after face in the
+// order near/far of pair 0, 1, 2, each face's pixel (x, y) at y + dims[b] * x.
+void gather_faces(Context &ctx, const void *ids, int id_bytes, const float *dist, Dims dims, const FaceCanvas &fc,
+                  void *face_ids, float *face_dist);
 
 }  // namespace brook
