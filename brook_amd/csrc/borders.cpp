@@ -209,20 +209,16 @@ std::vector<std::vector<Point>> border_targets(Context &ctx, const void *cc, int
   if (c2.count == 0) return items();
   const int bytes = c2.cc_bytes;
   Buffer<float> dist(ctx, np);
-  for (int o = 0; o < 3; ++o) {
-    const int a = fc.a[o], b = fc.b[o];
-    const size_t at = size_t(fc.col0[o]) * size_t(fc.rows);
-    const float pa[3] = {aniso[b], aniso[a], 1.0f};
-    edt(ctx, c2.labels.data() + at * size_t(bytes), bytes, Dims{fc.rows, dims[a], 1}, pa, true, 2, dist.data() + at);
-  }
-  // each face's pixels, y + fy * x, gathered from its rectangle of the canvas
+  // Pair by pair: the EDT of the pair's block, its two faces gathered (pixel y + fy * x) and read,
+  // and their host passes started (the larger faces on threads of their own) while the next
+  // pair's device work runs.
   struct Face {
     std::vector<uint8_t> raw;
     std::vector<float> dv;
     int a, b, c, fixed, fx, fy;
   };
   std::vector<Face> faces(6);
-  size_t total = 0;
+  size_t most = 0;
   for (int f = 0; f < 6; ++f) {
     Face &face = faces[size_t(f)];
     const int o = f / 2;
@@ -232,24 +228,8 @@ std::vector<std::vector<Point>> border_targets(Context &ctx, const void *cc, int
     face.fixed = f % 2 ? dims[face.c] - 1 : 0;
     face.fx = dims[face.a];
     face.fy = dims[face.b];
-    total += size_t(face.fx) * size_t(face.fy);
+    most = std::max(most, 2 * size_t(face.fx) * size_t(face.fy));
   }
-  Buffer<uint8_t> face_ids(ctx, total * size_t(bytes));
-  Buffer<float> face_dist(ctx, total);
-  gather_faces(ctx, c2.labels.data(), bytes, dist.data(), d, fc, face_ids.data(), face_dist.data());
-  const std::vector<uint8_t> all_ids = face_ids.download(ctx, total * size_t(bytes));
-  const std::vector<float> all_dist = face_dist.download(ctx, total);
-  size_t at = 0;
-  for (Face &face : faces) {
-    const size_t pixels = size_t(face.fx) * size_t(face.fy);
-    face.raw.assign(all_ids.begin() + ptrdiff_t(at * size_t(bytes)), all_ids.begin() + ptrdiff_t((at + pixels) * size_t(bytes)));
-    face.dv.assign(all_dist.begin() + ptrdiff_t(at), all_dist.begin() + ptrdiff_t(at + pixels));
-    at += pixels;
-  }
-  const auto t1 = Clock::now();
-  ctx.stats["borders_device_ms"] += ms(t0, t1);  // canvas, 2-D CCL and EDTs, downloads
-  // The host passes, the larger faces on threads of their own; the point sets are filled in face
-  // order afterwards.
   const size_t k = size_t(c2.count) + 1;
   auto pass = [&faces, &aniso, k, bytes](int f) {
     const Face &face = faces[size_t(f)];
@@ -261,8 +241,27 @@ std::vector<std::vector<Point>> border_targets(Context &ctx, const void *cc, int
   constexpr size_t kThreadPixels = size_t(1) << 16;
   std::vector<std::future<std::vector<FacePick>>> jobs(6);
   std::vector<std::vector<FacePick>> picks(6);
-  for (int f = 0; f < 6; ++f)
-    if (faces[size_t(f)].dv.size() >= kThreadPixels) jobs[size_t(f)] = std::async(std::launch::async, pass, f);
+  Buffer<uint8_t> pair_ids(ctx, most * size_t(bytes));
+  Buffer<float> pair_dist(ctx, most);
+  for (int o = 0; o < 3; ++o) {
+    const int a = fc.a[o], b = fc.b[o];
+    const size_t at = size_t(fc.col0[o]) * size_t(fc.rows);
+    const float pa[3] = {aniso[b], aniso[a], 1.0f};
+    edt(ctx, c2.labels.data() + at * size_t(bytes), bytes, Dims{fc.rows, dims[a], 1}, pa, true, 2, dist.data() + at);
+    gather_faces(ctx, c2.labels.data(), bytes, dist.data(), d, fc, 2 * o, 2 * o + 2, pair_ids.data(), pair_dist.data());
+    const size_t pixels = size_t(dims[a]) * size_t(dims[b]);
+    const std::vector<uint8_t> ids = pair_ids.download(ctx, 2 * pixels * size_t(bytes));
+    const std::vector<float> dv = pair_dist.download(ctx, 2 * pixels);
+    for (int h = 0; h < 2; ++h) {
+      const int f = 2 * o + h;
+      Face &face = faces[size_t(f)];
+      face.raw.assign(ids.begin() + ptrdiff_t(h * pixels * size_t(bytes)), ids.begin() + ptrdiff_t((h + 1) * pixels * size_t(bytes)));
+      face.dv.assign(dv.begin() + ptrdiff_t(h * pixels), dv.begin() + ptrdiff_t((h + 1) * pixels));
+      if (pixels >= kThreadPixels) jobs[size_t(f)] = std::async(std::launch::async, pass, f);
+    }
+  }
+  const auto t1 = Clock::now();
+  ctx.stats["borders_device_ms"] += ms(t0, t1);  // canvas, 2-D CCL and EDTs, downloads
   for (int f = 0; f < 6; ++f)
     if (!jobs[size_t(f)].valid()) picks[size_t(f)] = pass(f);
   for (int f = 0; f < 6; ++f)

@@ -253,14 +253,14 @@ __global__ void __launch_bounds__(256) faces_copy(const CC *cc, Dims d, FaceCanv
 }
 
 template <class CC>
-__global__ void __launch_bounds__(256) faces_gather(const CC *ids, const float *dist, Dims d, FaceCanvas fc,
-                                                   long long total, CC *face_ids, float *face_dist) {
+__global__ void __launch_bounds__(256) faces_gather(const CC *ids, const float *dist, Dims d, FaceCanvas fc, int f0,
+                                                   int f1, long long total, CC *face_ids, float *face_dist) {
   const long long j = (long long)blockIdx.x * 256 + threadIdx.x;
   if (j >= total) return;
   const int dims[3] = {d.sx, d.sy, d.sz};
   long long k = j;
-  int f = 0;
-  for (; f < 5; ++f) {
+  int f = f0;
+  for (; f < f1 - 1; ++f) {
     const long long pixels = (long long)dims[fc.a[f / 2]] * dims[fc.b[f / 2]];
     if (k < pixels) break;
     k -= pixels;
@@ -491,19 +491,20 @@ std::vector<Box> analyze(Context &ctx, const Components &cc, Dims d) {
 }
 
 void gather_faces(Context &ctx, const void *ids, int id_bytes, const float *dist, Dims d, const FaceCanvas &fc,
-                  void *face_ids, float *face_dist) {
+                  int f0, int f1, void *face_ids, float *face_dist) {
+  if (f0 < 0 || f1 > 6 || f0 >= f1) throw std::invalid_argument("gather_faces: face range");
   const int dims[3] = {d.sx, d.sy, d.sz};
   long long total = 0;
-  for (int o = 0; o < 3; ++o) total += 2LL * dims[fc.a[o]] * dims[fc.b[o]];
+  for (int f = f0; f < f1; ++f) total += (long long)dims[fc.a[f / 2]] * dims[fc.b[f / 2]];
   const unsigned g = blocks_for(total, 256);
   if (id_bytes == 1)
-    faces_gather<uint8_t><<<g, 256, 0, ctx.stream()>>>(static_cast<const uint8_t *>(ids), dist, d, fc, total,
+    faces_gather<uint8_t><<<g, 256, 0, ctx.stream()>>>(static_cast<const uint8_t *>(ids), dist, d, fc, f0, f1, total,
                                                        static_cast<uint8_t *>(face_ids), face_dist);
   else if (id_bytes == 2)
-    faces_gather<uint16_t><<<g, 256, 0, ctx.stream()>>>(static_cast<const uint16_t *>(ids), dist, d, fc, total,
+    faces_gather<uint16_t><<<g, 256, 0, ctx.stream()>>>(static_cast<const uint16_t *>(ids), dist, d, fc, f0, f1, total,
                                                         static_cast<uint16_t *>(face_ids), face_dist);
   else
-    faces_gather<uint32_t><<<g, 256, 0, ctx.stream()>>>(static_cast<const uint32_t *>(ids), dist, d, fc, total,
+    faces_gather<uint32_t><<<g, 256, 0, ctx.stream()>>>(static_cast<const uint32_t *>(ids), dist, d, fc, f0, f1, total,
                                                         static_cast<uint32_t *>(face_ids), face_dist);
   BROOK_LAUNCHED();
 }

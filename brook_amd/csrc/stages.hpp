@@ -78,9 +78,9 @@ struct FaceCanvas {
   int rows, cols;
 };
 void extract_faces(Context &ctx, const void *cc, int cc_bytes, Dims dims, const FaceCanvas &fc, void *canvas);
-// The six faces' ids (`id_bytes` wide) and distances from the canvas, face after face in the
-// order near/far of pair 0, 1, 2, each face's pixel (x, y) at y + dims[b] * x.
+// Faces f0 .. f1-1 (face 2o near, 2o+1 far of pair o): their ids (`id_bytes` wide) and distances
+// from the canvas, face after face, each face's pixel (x, y) at y + dims[b] * x.
 void gather_faces(Context &ctx, const void *ids, int id_bytes, const float *dist, Dims dims, const FaceCanvas &fc,
-                  void *face_ids, float *face_dist);
+                  int f0, int f1, void *face_ids, float *face_dist);
 
 }  // namespace brook

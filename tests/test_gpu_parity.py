@@ -253,6 +253,23 @@ def test_edge_cases(brook):
     _assert_packed_equal(got, want, "full volume")
 
 
+@pytest.mark.parametrize("shape", [(1, 40, 30), (40, 1, 30), (40, 30, 1), (2, 50, 20), (64, 24, 8), (30, 30, 30)])
+def test_border_targets_thin_and_skewed(brook, shape):
+    """fix_borders on volumes whose faces differ in size or coincide (a dimension of 1: the near and
+    far faces are one plane), with an anisotropy that differs per axis: blocky labels touching every
+    face, each face's border targets and the skeletons equal to the oracle's."""
+    rng = np.random.default_rng(sum(shape))
+    coarse = rng.integers(0, 6, size=tuple(max(1, s // 4 + 1) for s in shape)).astype(np.uint32)
+    labels = np.kron(coarse, np.ones((4, 4, 4), np.uint32))[: shape[0], : shape[1], : shape[2]]
+    labels = np.ascontiguousarray(labels)
+    params = dict(scale=1.5, const=4)
+    aniso = (4, 6, 10)
+    got = brook.skeletonize(labels, params, aniso, dust_threshold=0, fix_borders=True, output="packed", progress=False)
+    want = oracle.skeletonize(labels, params, aniso, 0, fix_borders=True)
+    assert len(want["labels"]) > 0
+    _assert_packed_equal(got, want, f"shape {shape}")
+
+
 def test_unsupported_options_raise(brook):
     labels = np.zeros((20, 20, 20), np.uint32)
     labels[5:15, 5:15, 5:15] = 1
