@@ -1,6 +1,6 @@
 // Border targets (fix_borders): for every volume face, 2-D components of the face's component
 // ids and their 2-D EDT are computed on the GPU with the volume kernels (the six faces on one
-// canvas: one CCL, one EDT per face pair); the farthest-pixel pick with its tie keys and the
+// canvas: one CCL, one EDT per block of a face or a face pair); the farthest-pixel pick with its tie keys and the
 // insertion-ordered point sets are host work, as in the reference's serial plan (borders.cu:53-116,
 // point_set.hpp:11-55).
 #include <algorithm>
@@ -184,18 +184,44 @@ std::vector<std::vector<Point>> border_targets(Context &ctx, const void *cc, int
   } pause(ctx);
   const auto t0 = Clock::now();
   // All six faces on one canvas, zero rows and columns between them: one 2-D CCL labels every
-  // face's components at once. Each face pair (same axes, same anisotropy) is a contiguous block
-  // of columns; its EDT sees the same lines as a face's own EDT would (x runs only by their
-  // offsets, y lines from the face's first column), so the distances are the per-face ones.
+  // face's components at once. The canvas is cut into blocks of whole columns, each holding one
+  // face or a pair of opposite faces stacked along the rows (near face, a zero row, far face);
+  // every block gets one EDT, which sees the same lines as a face's own EDT would (x runs only by
+  // their offsets, y lines from the face's first column), so the distances are the per-face ones.
+  // The row count is the tallest block: a pair is stacked when it fits in rows the EDT's x pass
+  // takes a wave per row for (kWaveRows); otherwise its faces get a block each.
+  constexpr int kWaveRows = 2048;
   FaceCanvas fc{};
+  for (int f = 0; f < 6; ++f) {
+    fc.a[f] = axes[f / 2][0];
+    fc.b[f] = axes[f / 2][1];
+    fc.c[f] = axes[f / 2][2];
+  }
+  for (int o = 0; o < 3; ++o) {
+    const int fy = dims[fc.b[2 * o]];
+    fc.rows = std::max(fc.rows, 2 * fy + 1 <= kWaveRows ? 2 * fy + 1 : fy);
+  }
+  struct Block {
+    int col0, cols, f0, f1;  // columns, faces f0 .. f1-1
+  };
+  std::vector<Block> blocks;
   int col = 0;
   for (int o = 0; o < 3; ++o) {
-    fc.a[o] = axes[o][0];
-    fc.b[o] = axes[o][1];
-    fc.c[o] = axes[o][2];
-    fc.col0[o] = col;
-    col += dims[fc.a[o]] + 1;
-    fc.rows = std::max(fc.rows, 2 * dims[fc.b[o]] + 1);
+    const int f = 2 * o, fx = dims[fc.a[f]], fy = dims[fc.b[f]];
+    if (2 * fy + 1 <= fc.rows) {
+      fc.col0[f] = fc.col0[f + 1] = col;
+      fc.row0[f] = 0;
+      fc.row0[f + 1] = fy + 1;
+      blocks.push_back(Block{col, fx, f, f + 2});
+      col += fx + 1;
+    } else {
+      for (int h = 0; h < 2; ++h) {
+        fc.col0[f + h] = col;
+        fc.row0[f + h] = 0;
+        blocks.push_back(Block{col, fx, f + h, f + h + 1});
+        col += fx + 1;
+      }
+    }
   }
   fc.cols = col - 1;
   const size_t np = size_t(fc.rows) * size_t(fc.cols);
@@ -209,9 +235,8 @@ std::vector<std::vector<Point>> border_targets(Context &ctx, const void *cc, int
   if (c2.count == 0) return items();
   const int bytes = c2.cc_bytes;
   Buffer<float> dist(ctx, np);
-  // Pair by pair: the EDT of the pair's block, its two faces gathered (pixel y + fy * x) and read,
-  // and their host passes started (the larger faces on threads of their own) while the next
-  // pair's device work runs.
+  // Block by block: its EDT, its faces gathered (pixel y + fy * x) and read, and their host passes
+  // started (the larger faces on threads of their own) while the next block's device work runs.
   struct Face {
     std::vector<uint8_t> raw;
     std::vector<float> dv;
@@ -221,10 +246,9 @@ std::vector<std::vector<Point>> border_targets(Context &ctx, const void *cc, int
   size_t most = 0;
   for (int f = 0; f < 6; ++f) {
     Face &face = faces[size_t(f)];
-    const int o = f / 2;
-    face.a = fc.a[o];
-    face.b = fc.b[o];
-    face.c = fc.c[o];
+    face.a = fc.a[f];
+    face.b = fc.b[f];
+    face.c = fc.c[f];
     face.fixed = f % 2 ? dims[face.c] - 1 : 0;
     face.fx = dims[face.a];
     face.fy = dims[face.b];
@@ -241,19 +265,19 @@ std::vector<std::vector<Point>> border_targets(Context &ctx, const void *cc, int
   constexpr size_t kThreadPixels = size_t(1) << 16;
   std::vector<std::future<std::vector<FacePick>>> jobs(6);
   std::vector<std::vector<FacePick>> picks(6);
-  Buffer<uint8_t> pair_ids(ctx, most * size_t(bytes));
-  Buffer<float> pair_dist(ctx, most);
-  for (int o = 0; o < 3; ++o) {
-    const int a = fc.a[o], b = fc.b[o];
-    const size_t at = size_t(fc.col0[o]) * size_t(fc.rows);
+  Buffer<uint8_t> block_ids(ctx, most * size_t(bytes));
+  Buffer<float> block_dist(ctx, most);
+  for (const Block &blk : blocks) {
+    const int a = fc.a[blk.f0], b = fc.b[blk.f0];
+    const size_t at = size_t(blk.col0) * size_t(fc.rows);
     const float pa[3] = {aniso[b], aniso[a], 1.0f};
-    edt(ctx, c2.labels.data() + at * size_t(bytes), bytes, Dims{fc.rows, dims[a], 1}, pa, true, 2, dist.data() + at);
-    gather_faces(ctx, c2.labels.data(), bytes, dist.data(), d, fc, 2 * o, 2 * o + 2, pair_ids.data(), pair_dist.data());
-    const size_t pixels = size_t(dims[a]) * size_t(dims[b]);
-    const std::vector<uint8_t> ids = pair_ids.download(ctx, 2 * pixels * size_t(bytes));
-    const std::vector<float> dv = pair_dist.download(ctx, 2 * pixels);
-    for (int h = 0; h < 2; ++h) {
-      const int f = 2 * o + h;
+    edt(ctx, c2.labels.data() + at * size_t(bytes), bytes, Dims{fc.rows, blk.cols, 1}, pa, true, 2, dist.data() + at);
+    gather_faces(ctx, c2.labels.data(), bytes, dist.data(), d, fc, blk.f0, blk.f1, block_ids.data(), block_dist.data());
+    const size_t pixels = size_t(dims[a]) * size_t(dims[b]), nf = size_t(blk.f1 - blk.f0);
+    const std::vector<uint8_t> ids = block_ids.download(ctx, nf * pixels * size_t(bytes));
+    const std::vector<float> dv = block_dist.download(ctx, nf * pixels);
+    for (size_t h = 0; h < nf; ++h) {
+      const int f = blk.f0 + int(h);
       Face &face = faces[size_t(f)];
       face.raw.assign(ids.begin() + ptrdiff_t(h * pixels * size_t(bytes)), ids.begin() + ptrdiff_t((h + 1) * pixels * size_t(bytes)));
       face.dv.assign(dv.begin() + ptrdiff_t(h * pixels), dv.begin() + ptrdiff_t((h + 1) * pixels));

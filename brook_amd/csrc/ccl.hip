@@ -235,18 +235,14 @@ __global__ void __launch_bounds__(256) faces_copy(const CC *cc, Dims d, FaceCanv
   const int r = int(i % fc.rows), col = int(i / fc.rows);
   const int dims[3] = {d.sx, d.sy, d.sz};
   CC v = CC(0);
-  for (int o = 0; o < 3; ++o) {
-    const int x = col - fc.col0[o], fy = dims[fc.b[o]];
-    if (x < 0 || x >= dims[fc.a[o]]) continue;
-    const bool far = r > fy;
-    const int y = far ? r - fy - 1 : r;
-    if (r != fy && y < fy) {
-      int p[3];
-      p[fc.a[o]] = x;
-      p[fc.b[o]] = y;
-      p[fc.c[o]] = far ? dims[fc.c[o]] - 1 : 0;
-      v = cc[flat(p[0], p[1], p[2], d)];
-    }
+  for (int f = 0; f < 6; ++f) {
+    const int x = col - fc.col0[f], y = r - fc.row0[f];
+    if (x < 0 || x >= dims[fc.a[f]] || y < 0 || y >= dims[fc.b[f]]) continue;
+    int p[3];
+    p[fc.a[f]] = x;
+    p[fc.b[f]] = y;
+    p[fc.c[f]] = f % 2 ? dims[fc.c[f]] - 1 : 0;
+    v = cc[flat(p[0], p[1], p[2], d)];
     break;
   }
   canvas[i] = v;
@@ -261,13 +257,13 @@ __global__ void __launch_bounds__(256) faces_gather(const CC *ids, const float *
   long long k = j;
   int f = f0;
   for (; f < f1 - 1; ++f) {
-    const long long pixels = (long long)dims[fc.a[f / 2]] * dims[fc.b[f / 2]];
+    const long long pixels = (long long)dims[fc.a[f]] * dims[fc.b[f]];
     if (k < pixels) break;
     k -= pixels;
   }
-  const int o = f / 2, fy = dims[fc.b[o]];
+  const int fy = dims[fc.b[f]];
   const long long x = k / fy, y = k % fy;
-  const long long at = ((long long)fc.col0[o] + x) * fc.rows + (f % 2 ? fy + 1 : 0) + y;
+  const long long at = ((long long)fc.col0[f] + x) * fc.rows + fc.row0[f] + y;
   face_ids[j] = ids[at];
   face_dist[j] = dist[at];
 }
@@ -495,7 +491,7 @@ void gather_faces(Context &ctx, const void *ids, int id_bytes, const float *dist
   if (f0 < 0 || f1 > 6 || f0 >= f1) throw std::invalid_argument("gather_faces: face range");
   const int dims[3] = {d.sx, d.sy, d.sz};
   long long total = 0;
-  for (int f = f0; f < f1; ++f) total += (long long)dims[fc.a[f / 2]] * dims[fc.b[f / 2]];
+  for (int f = f0; f < f1; ++f) total += (long long)dims[fc.a[f]] * dims[fc.b[f]];
   const unsigned g = blocks_for(total, 256);
   if (id_bytes == 1)
     faces_gather<uint8_t><<<g, 256, 0, ctx.stream()>>>(static_cast<const uint8_t *>(ids), dist, d, fc, f0, f1, total,

@@ -68,18 +68,17 @@ void edt(Context &ctx, const void *cc, int cc_bytes, Dims dims, const float anis
 void exclusive_sum(Context &ctx, const int *in, int *out, long long n);
 void exclusive_sum(Context &ctx, const long long *in, long long *out, long long n);
 // The six faces of a volume on one canvas of `rows` x `cols` pixels (rows along the fast axis;
-// the copy of each face is borders.cu:14-20's). Face pair o spans the volume's axes a[o] and b[o]
-// at c[o] = 0 and at c[o] = dims[c[o]] - 1, in columns col0[o] .. col0[o] + dims[a[o]] - 1: the
-// near face in rows 0 .. dims[b[o]] - 1, the far face in rows dims[b[o]] + 1 .. 2 * dims[b[o]].
-// Face pixel (x along a, y along b) sits at row y (near) or dims[b] + 1 + y (far) of column
-// col0 + x; every other pixel is 0, so no two faces touch.
+// the copy of each face is borders.cu:14-20's). Face f spans the volume's axes a[f] and b[f] at
+// c[f] = 0 (f even) or c[f] = dims[c[f]] - 1 (f odd); its pixel (x along a, y along b) sits at
+// row row0[f] + y of column col0[f] + x. Every other pixel is 0, and the layout leaves a zero row
+// or column between any two faces, so no two faces touch.
 struct FaceCanvas {
-  int a[3], b[3], c[3], col0[3];
+  int a[6], b[6], c[6], col0[6], row0[6];
   int rows, cols;
 };
 void extract_faces(Context &ctx, const void *cc, int cc_bytes, Dims dims, const FaceCanvas &fc, void *canvas);
-// Faces f0 .. f1-1 (face 2o near, 2o+1 far of pair o): their ids (`id_bytes` wide) and distances
-// from the canvas, face after face, each face's pixel (x, y) at y + dims[b] * x.
+// Faces f0 .. f1-1: their ids (`id_bytes` wide) and distances from the canvas, face after face,
+// each face's pixel (x, y) at y + dims[b] * x.
 void gather_faces(Context &ctx, const void *ids, int id_bytes, const float *dist, Dims dims, const FaceCanvas &fc,
                   int f0, int f1, void *face_ids, float *face_dist);
 

@@ -153,6 +153,26 @@ def test_gpu_targets_equal_oracle(brook, fix_branching):
 
 
 @pytest.mark.gpu
+def test_gpu_batch_error_on_a_worker_thread(brook):
+    """In a batch of nine (per-sample stages on the engine's worker threads), a manual target off
+    the volume in one sample raises ValueError from that worker; the context then runs the same
+    batch with valid targets, every sample equal to the oracle."""
+    samples, before, after = targeted_samples()
+    reps = -(-9 // len(samples))
+    s9, b9, a9 = (samples * reps)[:9], (before * reps)[:9], (after * reps)[:9]
+    bad = [list(b) for b in b9]
+    bad[5] = [(1000, 0, 0)]
+    options = dict(dust_threshold=10, progress=False)
+    with pytest.raises(ValueError):
+        brook.skeletonize_batch(s9, teasar_params=PARAMS, extra_targets_before=bad, extra_targets_after=a9,
+                                output="packed", **options)
+    ok = brook.skeletonize_batch(s9, teasar_params=PARAMS, extra_targets_before=b9, extra_targets_after=a9,
+                                 output="packed", **options)
+    for i in range(9):
+        assert_equal_packed(ok[i].as_dict(), oracle_run(s9[i], b9[i], a9[i]))
+
+
+@pytest.mark.gpu
 def test_gpu_after_targets_on_tubes(brook):
     """Several `after` targets per label, on labels many paths cover, and a budget they fill."""
     from brook_amd import synth
