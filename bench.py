@@ -359,7 +359,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the path would shard by volume and by label, see DESIGN.md)", file=sys.stderr)
+              "(the path would shard by volume: batch samples and serving requests per GPU)", file=sys.stderr)
         return 2
 
     import torch
