@@ -82,10 +82,11 @@ std::vector<float> step_weights(const float aniso[3]) {
 // PDRF and target order, trace, assembly. Device buffers live as long as the object.
 // Auxiliary contexts (each serves one thread at a time): kBorderAux the border jobs of groups under
 // kWorkerMin samples, kRootAux the early root searches (labels of at least kEarlyRootVoxels voxels
-// in volumes of at least kEarlyRootVolume voxels); a larger group runs its first preamble halves
-// on kWorkers host threads, each with auxiliary context kWorkerBase + w of its own.
+// in volumes of at least kEarlyRootVolume voxels); a group of at least kWorkerMin samples runs
+// its per-sample stages on kWorkers host threads, each with auxiliary context kWorkerBase + w of
+// its own.
 constexpr int kBorderAux = 0, kRootAux = 1, kWorkerBase = 2;
-constexpr int kWorkers = 8, kWorkerMin = 8;
+constexpr int kWorkers = 16, kWorkerMin = 8;
 constexpr int64_t kEarlyRootVoxels = int64_t(1) << 17, kEarlyRootVolume = int64_t(1) << 26;
 
 class Sample {
