@@ -271,9 +271,10 @@ def served(rates=(20.0, 50.0), count=40, seed=5005):
         for mode, slots, max_batch in (("batched", 2, 64), ("serial", 1, 1)):
             with brook_amd.Server(params, aniso, 1000, max_batch=max_batch, slots=slots) as srv:
                 srv.submit(vols[0]).result()  # each slot's context warm
-                if slots > 1:
-                    a, b = srv.submit(vols[1]), srv.submit(vols[2])
-                    a.result(), b.result()
+                if slots > 1:  # the other slots' contexts warm
+                    warm = [srv.submit(vols[1 + i]) for i in range(slots)]
+                    for f in warm:
+                        f.result()
                 srv.batches.clear()
                 t0 = time.perf_counter()
                 at = t0
