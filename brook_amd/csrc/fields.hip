@@ -531,18 +531,21 @@ __device__ __forceinline__ unsigned long long label_argmax(const VoxelRec *rec, 
 }
 
 // stalled: set when a flood exceeded its round budget (the label's fields are then left as they
-// are and the host raises).
+// are and the host raises). The queue is sorted largest first: workgroup w < nfixed (resident from
+// the start, at most the queue's length) takes item w first (workgroups go to the XCDs in turn, so
+// the largest labels start on different XCDs); every other item comes from the shared cursor,
+// from item nfixed on.
 __global__ void __launch_bounds__(kFloodBlock) prepare_labels(const FloodJob *__restrict__ jobs,
-                                                              const int2 *__restrict__ queue, int nqueue, int *next,
-                                                              int *stalled) {
+                                                              const int2 *__restrict__ queue, int nqueue, int nfixed,
+                                                              int *next, int *stalled) {
   __shared__ float W[26];
   __shared__ int off[26];
   __shared__ FloodShared fs;
   __shared__ unsigned long long best;
   __shared__ int item;
-  for (;;) {
+  for (bool first_item = true;; first_item = false) {
     __syncthreads();
-    if (threadIdx.x == 0) item = atomicAdd(next, 1);
+    if (threadIdx.x == 0) item = first_item && int(blockIdx.x) < nfixed ? int(blockIdx.x) : nfixed + atomicAdd(next, 1);
     __syncthreads();
     if (item >= nqueue) return;
     const int2 q = queue[item];
@@ -689,7 +692,9 @@ void prepare_fields(Context &ctx, const std::vector<FloodJob> &jobs, const std::
   if (max_grid > 0) grid = std::min(grid, max_grid);
   {
     KernelTimer kt(ctx, "prepare_labels");
-    prepare_labels<<<grid, kFloodBlock, 0, ctx.stream()>>>(jobs_dev.data(), queue.data(), nq, next.data(), next.data() + 1);
+    prepare_labels<<<grid, kFloodBlock, 0, ctx.stream()>>>(jobs_dev.data(), queue.data(), nq,
+                                                           std::min(grid, ctx.compute_units()), next.data(),
+                                                           next.data() + 1);
     BROOK_LAUNCHED();
   }
   if (next.download(ctx)[1]) throw std::runtime_error("flood did not converge: a label exceeded its round budget");

@@ -1065,13 +1065,18 @@ __global__ void __launch_bounds__(kBlock) soma_balls(const TraceArgs *__restrict
   }
 }
 
+// The queue is sorted largest first. Workgroup w < nfixed (the workgroups resident from the start,
+// at most the queue's length) takes item w first; every other item comes from the shared cursor,
+// from item nfixed on. Workgroups go to the XCDs in turn (w mod 8), so the largest labels start on
+// different XCDs instead of wherever the first atomics land, and no XCD's L2 carries more than
+// its share of them.
 __global__ void __launch_bounds__(kBlock) trace_labels(const TraceArgs *__restrict__ all, const int2 *__restrict__ queue,
-                                                      int nqueue, int *next) {
+                                                      int nqueue, int nfixed, int *next) {
   __shared__ Shared sh;
   __shared__ int item;
-  for (;;) {
+  for (bool first_item = true;; first_item = false) {
     __syncthreads();
-    if (threadIdx.x == 0) item = g_add(next, 1);
+    if (threadIdx.x == 0) item = first_item && int(blockIdx.x) < nfixed ? int(blockIdx.x) : nfixed + g_add(next, 1);
     __syncthreads();
     if (item >= nqueue) return;
     const int2 q = queue[item];
@@ -1208,7 +1213,8 @@ void trace_all(Context &ctx, const std::vector<TraceArgs> &args, const std::vect
   }
   {
     KernelTimer kt(ctx, "trace_labels");
-    trace_labels<<<grid, kBlock, 0, ctx.stream()>>>(args_dev.data(), queue.data(), nq, next.data());
+    trace_labels<<<grid, kBlock, 0, ctx.stream()>>>(args_dev.data(), queue.data(), nq,
+                                                    std::min(grid, ctx.compute_units()), next.data());
     BROOK_LAUNCHED();
   }
 }
